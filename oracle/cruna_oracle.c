@@ -1,0 +1,914 @@
+/*
+ * CPU oracle for the CRUNA time-stepping hot path -- TEST INFRASTRUCTURE.
+ *
+ * A plain-C restatement of the reference Fortran, used only as the checker
+ * for the HIP library (tests/, __graft_entry__.smoke(), bench.py's
+ * cpu_baseline leg).  Expression order follows the Fortran array
+ * expressions term by term so that FP64 results agree to rounding.
+ * Each function cites the reference file:line it restates (paths are
+ * relative to the reference root).
+ */
+#include "cruna_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NV 5
+
+static size_t npts(const orc_cfg *c) { return (size_t)c->n[0] * c->n[1] * c->n[2]; }
+
+static double *xalloc(size_t n) {
+  double *p = (double *)malloc(n * sizeof(double));
+  if (!p) {
+    fprintf(stderr, "oracle: out of memory (%zu doubles)\n", n);
+    abort();
+  }
+  return p;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LAPACK dgttrf / dgttrs (dgtts2, itrans = 0).  Third-party routines called by
+ * the reference at src/diff/diff_pen_core_o6_i_c_p.f90:213,219,135 and
+ * src/diff/diff_pen_core_o6_i_c_o.f90:194,129 (and the filter cores).  This
+ * restates LAPACK 3.x's published algorithm: tridiagonal LU with partial
+ * pivoting, second superdiagonal du2, row interchanges in ipiv (1-based). */
+int orc_dgttrf(int n, double *dl, double *d, double *du, double *du2, int *ipiv) {
+  int info = 0;
+  for (int i = 0; i < n; ++i) ipiv[i] = i + 1;
+  for (int i = 0; i < n - 2; ++i) du2[i] = 0.0;
+  for (int i = 0; i < n - 2; ++i) {
+    if (fabs(d[i]) >= fabs(dl[i])) {
+      if (d[i] != 0.0) {
+        double fact = dl[i] / d[i];
+        dl[i] = fact;
+        d[i + 1] = d[i + 1] - fact * du[i];
+      }
+    } else {
+      double fact = d[i] / dl[i];
+      d[i] = dl[i];
+      dl[i] = fact;
+      double temp = du[i];
+      du[i] = d[i + 1];
+      d[i + 1] = temp - fact * d[i + 1];
+      du2[i] = du[i + 1];
+      du[i + 1] = -fact * du[i + 1];
+      ipiv[i] = i + 2;
+    }
+  }
+  if (n > 1) {
+    int i = n - 2;
+    if (fabs(d[i]) >= fabs(dl[i])) {
+      if (d[i] != 0.0) {
+        double fact = dl[i] / d[i];
+        dl[i] = fact;
+        d[i + 1] = d[i + 1] - fact * du[i];
+      }
+    } else {
+      double fact = d[i] / dl[i];
+      d[i] = dl[i];
+      dl[i] = fact;
+      double temp = du[i];
+      du[i] = d[i + 1];
+      d[i + 1] = temp - fact * d[i + 1];
+      ipiv[i] = i + 2;
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    if (d[i] == 0.0) {
+      info = i + 1;
+      break;
+    }
+  return info;
+}
+
+void orc_dgttrs(int n, int nrhs, const double *dl, const double *d, const double *du,
+                const double *du2, const int *ipiv, double *b, int ldb) {
+  for (int j = 0; j < nrhs; ++j) {
+    double *x = b + (size_t)j * ldb;
+    /* solve L x = b */
+    for (int i = 0; i < n - 1; ++i) {
+      if (ipiv[i] == i + 1) {
+        x[i + 1] = x[i + 1] - dl[i] * x[i];
+      } else {
+        double temp = x[i];
+        x[i] = x[i + 1];
+        x[i + 1] = temp - dl[i] * x[i];
+      }
+    }
+    /* solve U x = b */
+    x[n - 1] = x[n - 1] / d[n - 1];
+    if (n > 1) x[n - 2] = (x[n - 2] - du[n - 2] * x[n - 1]) / d[n - 2];
+    for (int i = n - 3; i >= 0; --i)
+      x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / d[i];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Factorisations of the left matrices B.                                   */
+typedef struct {
+  int n;
+  double *dl, *dc, *du, *du2, *fbnorm;
+  int *ipiv;
+  double fbrat;
+  int cyclic;
+} orc_lu;
+
+static void lu_free(orc_lu *f) {
+  free(f->dl);
+  free(f->dc);
+  free(f->du);
+  free(f->du2);
+  free(f->fbnorm);
+  free(f->ipiv);
+}
+
+static void lu_alloc(orc_lu *f, int n) {
+  f->n = n;
+  f->dl = xalloc(n);
+  f->dc = xalloc(n);
+  f->du = xalloc(n);
+  f->du2 = xalloc(n);
+  f->fbnorm = xalloc(n);
+  f->ipiv = (int *)malloc(n * sizeof(int));
+  f->fbrat = 0.0;
+  f->cyclic = 0;
+}
+
+/* Cyclic tridiag(alpha,1,alpha) with the Sherman-Morrison split used by
+ * src/diff/diff_pen_core_o6_i_c_p.f90:182-228 and
+ * src/filter/filter_pen_core_o6_i_c_p.f90:217-246. */
+static void lu_cyclic(orc_lu *f, int n, double alpha) {
+  lu_alloc(f, n);
+  f->cyclic = 1;
+  double *fa = xalloc(n), *fb = xalloc(n);
+  for (int i = 0; i < n - 1; ++i) {
+    f->dl[i] = alpha;
+    f->du[i] = alpha;
+  }
+  for (int i = 0; i < n; ++i) {
+    f->dc[i] = 1.0;
+    fa[i] = 0.0;
+    fb[i] = 0.0;
+  }
+  fa[0] = -f->dc[0];
+  fa[n - 1] = +alpha;
+  fb[0] = +1.0;
+  fb[n - 1] = -alpha / f->dc[0];
+  f->dc[n - 1] = f->dc[n - 1] + (alpha * alpha / f->dc[0]);
+  f->dc[0] = f->dc[0] + f->dc[0];
+  int info = orc_dgttrf(n, f->dl, f->dc, f->du, f->du2, f->ipiv);
+  if (info) fprintf(stderr, "oracle: dgttrf info %d\n", info);
+  orc_dgttrs(n, 1, f->dl, f->dc, f->du, f->du2, f->ipiv, fa, n);
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += fb[i] * fa[i];
+  for (int i = 0; i < n; ++i) f->fbnorm[i] = fa[i] * fb[0] / (1.0 + s);
+  f->fbrat = fb[n - 1] / fb[0];
+  free(fa);
+  free(fb);
+}
+
+/* Open-boundary B of the Lele o6 derivative, src/diff/diff_pen_core_o6_i_c_o.f90:159-199. */
+static void lu_diff_open(orc_lu *f, int n) {
+  lu_alloc(f, n);
+  const double alpha = 1.0 / 3.0;
+  for (int i = 0; i < n - 1; ++i) {
+    f->dl[i] = alpha;
+    f->du[i] = alpha;
+  }
+  for (int i = 0; i < n; ++i) f->dc[i] = 1.0;
+  f->du[0] = 3.00;
+  f->du[1] = 0.25;
+  f->du[n - 2] = 0.25;
+  f->dl[0] = 0.25;
+  f->dl[n - 2] = 3.00;
+  f->dl[n - 3] = 0.25;
+  int info = orc_dgttrf(n, f->dl, f->dc, f->du, f->du2, f->ipiv);
+  if (info) fprintf(stderr, "oracle: dgttrf info %d\n", info);
+}
+
+/* Open-boundary B of the GV2000 o6 filter, src/filter/filter_pen_core_o6_i_c_o.f90:203-240. */
+static void lu_filter_open(orc_lu *f, int n, double alpha) {
+  lu_alloc(f, n);
+  for (int i = 0; i < n - 1; ++i) {
+    f->dl[i] = alpha;
+    f->du[i] = alpha;
+  }
+  for (int i = 0; i < n; ++i) f->dc[i] = 1.0;
+  f->du[0] = 0.0;
+  f->dl[n - 2] = 0.0;
+  int info = orc_dgttrf(n, f->dl, f->dc, f->du, f->du2, f->ipiv);
+  if (info) fprintf(stderr, "oracle: dgttrf info %d\n", info);
+}
+
+static void lu_solve(const orc_lu *f, double *q, int nlines) {
+  const int n = f->n;
+  orc_dgttrs(n, nlines, f->dl, f->dc, f->du, f->du2, f->ipiv, q, n);
+  if (f->cyclic) {
+    for (int l = 0; l < nlines; ++l) {
+      double *x = q + (size_t)l * n;
+      const double s = x[0] + f->fbrat * x[n - 1];
+      for (int i = 0; i < n; ++i) x[i] = x[i] - f->fbnorm[i] * s;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Derivative cores.                                                        */
+
+/* src/diff/diff_pen_core_o6_i_c_p.f90:87-145 (stencil 157-165: a = 14/9/2, b = 1/9/4). */
+static void diff_periodic(int n, int nlines, double *q, double dx) {
+  const double a = 14.00 / 9.0 / 2.0;
+  const double b = 1.00 / 9.0 / 4.0;
+  orc_lu f;
+  lu_cyclic(&f, n, 1.0 / 3.0);
+  double *t = xalloc(n);
+  for (int l = 0; l < nlines; ++l) {
+    double *x = q + (size_t)l * n;
+    for (int i = 0; i < n; ++i) {
+      const int ip1 = (i + 1) % n, ip2 = (i + 2) % n;
+      const int im1 = (i - 1 + n) % n, im2 = (i - 2 + 2 * n) % n;
+      t[i] = a * (x[ip1] - x[im1]) + b * (x[ip2] - x[im2]);
+    }
+    for (int i = 0; i < n; ++i) x[i] = t[i] / dx;
+  }
+  lu_solve(&f, q, nlines);
+  free(t);
+  lu_free(&f);
+}
+
+/* src/diff/diff_pen_core_o6_i_c_o.f90:85-134 (stencil 140-155). */
+static void diff_open(int n, int nlines, double *q, double dx) {
+  const double a = 14.00 / 9.0 / 2.0, b = 1.00 / 9.0 / 4.0;
+  const double c = 17.00 / 6.0, d = 1.50, e = 1.00 / 6.0, f = 0.75;
+  orc_lu lu;
+  lu_diff_open(&lu, n);
+  double *t = xalloc(n);
+  for (int l = 0; l < nlines; ++l) {
+    double *x = q + (size_t)l * n;
+    t[0] = -c * x[0] + d * (x[1] + x[2]) - e * x[3];
+    t[1] = -f * x[0] + f * x[2];
+    for (int i = 2; i <= n - 3; ++i) t[i] = a * (x[i + 1] - x[i - 1]) + b * (x[i + 2] - x[i - 2]);
+    t[n - 2] = f * x[n - 1] - f * x[n - 3];
+    t[n - 1] = c * x[n - 1] - d * (x[n - 2] + x[n - 3]) + e * x[n - 4];
+    for (int i = 0; i < n; ++i) x[i] = t[i] / dx;
+  }
+  lu_solve(&lu, q, nlines);
+  free(t);
+  lu_free(&lu);
+}
+
+void orc_diff_core(int scheme, int n, int nlines, double *q, double dx) {
+  if (scheme == ORC_PERIODIC)
+    diff_periodic(n, nlines, q, dx);
+  else
+    diff_open(n, nlines, q, dx);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Filter cores.                                                            */
+
+/* Stencil a0..a3, src/filter/filter_pen_core_o6_i_c_p.f90:151-182. */
+static void filter_stencil(double alpha, double s[4]) {
+  double a0 = 11.0 / 16.0 + 5.0 * alpha / 8.0;
+  double a1 = 15.0 / 32.0 + 17.0 * alpha / 16.0;
+  a1 = a1 / 2.0;
+  double a2 = -3.0 / 16.0 + 3.0 * alpha / 8.0;
+  a2 = a2 / 2.0;
+  double a3 = 1.0 / 32.0 - 1.0 * alpha / 16.0;
+  a3 = a3 / 2.0;
+  s[0] = a0;
+  s[1] = a1;
+  s[2] = a2;
+  s[3] = a3;
+}
+
+/* src/filter/filter_pen_core_o6_i_c_p.f90:84-145. */
+static void filter_periodic(int n, int nlines, double *q, double alpha) {
+  double s[4];
+  filter_stencil(alpha, s);
+  orc_lu f;
+  lu_cyclic(&f, n, alpha);
+  double *t = xalloc(n);
+  for (int l = 0; l < nlines; ++l) {
+    double *x = q + (size_t)l * n;
+    for (int i = 0; i < n; ++i) {
+      const int p1 = (i + 1) % n, p2 = (i + 2) % n, p3 = (i + 3) % n;
+      const int m1 = (i - 1 + n) % n, m2 = (i - 2 + 2 * n) % n, m3 = (i - 3 + 3 * n) % n;
+      t[i] = s[0] * x[i] + s[1] * (x[p1] + x[m1]) + s[2] * (x[p2] + x[m2]) + s[3] * (x[p3] + x[m3]);
+    }
+    memcpy(x, t, n * sizeof(double));
+  }
+  lu_solve(&f, q, nlines);
+  free(t);
+  lu_free(&f);
+}
+
+/* Boundary rows of the open filter, src/filter/filter_pen_core_o6_i_c_o.f90:179-195. */
+static void filter_open_boundary(double alpha, double cb[2][7]) {
+  cb[0][0] = 1.0 / 64.0 + 31.0 * alpha / 32.0;
+  cb[0][1] = 29.0 / 32.0 + 3.0 * alpha / 16.0;
+  cb[0][2] = 15.0 / 64.0 + 17.0 * alpha / 32.0;
+  cb[0][3] = -5.0 / 16.0 + 5.0 * alpha / 8.0;
+  cb[0][4] = 15.0 / 64.0 - 15.0 * alpha / 32.0;
+  cb[0][5] = -3.0 / 32.0 + 3.0 * alpha / 16.0;
+  cb[0][6] = 1.0 / 64.0 - 1.0 * alpha / 32.0;
+  cb[1][0] = -1.0 / 64.0 + 1.0 * alpha / 32.0;
+  cb[1][1] = 3.0 / 32.0 + 13.0 * alpha / 16.0;
+  cb[1][2] = 49.0 / 64.0 + 15.0 * alpha / 32.0;
+  cb[1][3] = 5.0 / 16.0 + 3.0 * alpha / 8.0;
+  cb[1][4] = -15.0 / 64.0 + 15.0 * alpha / 32.0;
+  cb[1][5] = 3.0 / 32.0 - 3.0 * alpha / 16.0;
+  cb[1][6] = -1.0 / 64.0 + 1.0 * alpha / 32.0;
+}
+
+/* src/filter/filter_pen_core_o6_i_c_o.f90:79-143. */
+static void filter_open(int n, int nlines, double *q, double alpha) {
+  double s[4], cb[2][7];
+  filter_stencil(alpha, s);
+  filter_open_boundary(alpha, cb);
+  orc_lu f;
+  lu_filter_open(&f, n, alpha);
+  double *t = xalloc(n);
+  for (int l = 0; l < nlines; ++l) {
+    double *x = q + (size_t)l * n;
+    for (int i = 3; i <= n - 4; ++i)
+      t[i] = s[0] * x[i] + s[1] * (x[i + 1] + x[i - 1]) + s[2] * (x[i + 2] + x[i - 2]) +
+             s[3] * (x[i + 3] + x[i - 3]);
+    t[0] = x[0];
+    t[n - 1] = x[n - 1];
+    t[1] = 0.0;
+    t[n - 2] = 0.0;
+    for (int i = 0; i < 7; ++i) {
+      t[1] = t[1] + cb[0][i] * x[i];
+      t[n - 2] = t[n - 2] + cb[0][i] * x[n - 1 - i];
+    }
+    t[2] = 0.0;
+    t[n - 3] = 0.0;
+    for (int i = 0; i < 7; ++i) {
+      t[2] = t[2] + cb[1][i] * x[i];
+      t[n - 3] = t[n - 3] + cb[1][i] * x[n - 1 - i];
+    }
+    memcpy(x, t, n * sizeof(double));
+  }
+  lu_solve(&f, q, nlines);
+  free(t);
+  lu_free(&f);
+}
+
+void orc_filter_core(int scheme, int n, int nlines, double *q, double alpha) {
+  if (scheme == ORC_PERIODIC)
+    filter_periodic(n, nlines, q, alpha);
+  else
+    filter_open(n, nlines, q, alpha);
+}
+
+/* ------------------------------------------------------------------------ */
+/* 3-D wrappers: gather each line along dir into a pencil (the single-rank
+ * ftranspose/btranspose of src/parallelism/transpose_mpi_cartesian.f90:110-127,
+ * 326-350, 566-594 only reorder lines, which the per-line algorithm ignores). */
+
+static void gather_pencils(const orc_cfg *c, int dir, const double *in, double *pen) {
+  const int n1 = c->n[0], n2 = c->n[1], n3 = c->n[2];
+  const int nd = c->n[dir];
+  size_t line = 0;
+  if (dir == 0) {
+    memcpy(pen, in, (size_t)n1 * n2 * n3 * sizeof(double));
+    return;
+  }
+  for (int k = 0; k < (dir == 2 ? n2 : n3); ++k)
+    for (int i = 0; i < n1; ++i, ++line)
+      for (int p = 0; p < nd; ++p) {
+        size_t idx = (dir == 1) ? (size_t)i + (size_t)n1 * (p + (size_t)n2 * k)
+                                : (size_t)i + (size_t)n1 * (k + (size_t)n2 * p);
+        pen[line * nd + p] = in[idx];
+      }
+}
+
+static void scatter_pencils(const orc_cfg *c, int dir, const double *pen, double *out) {
+  const int n1 = c->n[0], n2 = c->n[1], n3 = c->n[2];
+  const int nd = c->n[dir];
+  size_t line = 0;
+  if (dir == 0) {
+    memcpy(out, pen, (size_t)n1 * n2 * n3 * sizeof(double));
+    return;
+  }
+  for (int k = 0; k < (dir == 2 ? n2 : n3); ++k)
+    for (int i = 0; i < n1; ++i, ++line)
+      for (int p = 0; p < nd; ++p) {
+        size_t idx = (dir == 1) ? (size_t)i + (size_t)n1 * (p + (size_t)n2 * k)
+                                : (size_t)i + (size_t)n1 * (k + (size_t)n2 * p);
+        out[idx] = pen[line * nd + p];
+      }
+}
+
+/* src/diff/diff_i1_mpi_cartesian.f90:83-103 (and i2/i3). */
+void orc_diff3d(const orc_cfg *c, int dir, double *out, const double *in, double h) {
+  const size_t N = npts(c);
+  double *pen = xalloc(N);
+  gather_pencils(c, dir, in, pen);
+  orc_diff_core(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, h);
+  scatter_pencils(c, dir, pen, out);
+  free(pen);
+}
+
+/* src/filter/filter_i1_mpi_cartesian.f90:83-97 (and i2/i3). */
+void orc_filter4d(const orc_cfg *c, int dir, double *q, int nv) {
+  const size_t N = npts(c);
+  double *pen = xalloc(N);
+  for (int v = 0; v < nv; ++v) {
+    gather_pencils(c, dir, q + v * N, pen);
+    orc_filter_core(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, c->filter_alpha);
+    scatter_pencils(c, dir, pen, q + v * N);
+  }
+  free(pen);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Thermodynamic closures, src/equations/ns_rho_rhou_p_trafos.f90.          */
+
+/* give_T :36-60  (T = (W/R) * p / rho) */
+double orc_T(const orc_cfg *c, double rho, double p) {
+  const double w_inv_r = c->W / c->R;
+  return w_inv_r * p / rho;
+}
+
+/* give_mu :119-142 (Sutherland, S1 = 110.4).  The power law
+ * mu0*(T/T0)**exp is the case-level override of the same function. */
+double orc_mu(const orc_cfg *c, double T) {
+  if (c->visc_law == ORC_VISC_POWER) return c->mu0 * pow(T / c->T0, c->visc_exp);
+  const double S1 = 110.4;
+  return c->mu0 * pow(T / c->T0, 1.5) * ((c->T0 + S1) / (T + S1));
+}
+
+/* give_lambda :146-171 (lambda = mu*cp/Pr, cp = gamma*(R/W)/(gamma-1)) */
+double orc_lambda(const orc_cfg *c, double mu) {
+  const double cp = c->gamma * (c->R / c->W) / (c->gamma - 1.0);
+  return mu * cp / c->Pr;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Boundary conditions.                                                     */
+
+static int bc_type(const orc_cfg *c, int r) { return (int)lround(c->bc[r][0]); }
+
+/* Visit the points of the box of BC row r (1-based inclusive indices). */
+#define FOR_BOX(c, r, idx)                                                                      \
+  for (int kk = (int)lround((c)->bc[r][5]) - 1; kk <= (int)lround((c)->bc[r][6]) - 1; ++kk)    \
+    for (int jj = (int)lround((c)->bc[r][3]) - 1; jj <= (int)lround((c)->bc[r][4]) - 1; ++jj)  \
+      for (int ii = (int)lround((c)->bc[r][1]) - 1; ii <= (int)lround((c)->bc[r][2]) - 1; ++ii) \
+        for (size_t idx = (size_t)ii + (size_t)(c)->n[0] * (jj + (size_t)(c)->n[1] * kk), _o = 1; _o; _o = 0)
+
+/* src/boundaries/nonreflecting.f90:46-137 on one point (stride_v = distance
+ * between variables).  npts points starting at q, contiguous. */
+void orc_characteristic_q_k1d(int np, int sv, double *q, const double *aim, const double k[3],
+                              double gamma) {
+  for (int m = 0; m < np; ++m) {
+    double *Q = q + m;
+    const double *A = aim + m;
+    double r = Q[0];
+    double u1 = Q[sv] / Q[0], u2 = Q[2 * sv] / Q[0], u3 = Q[3 * sv] / Q[0];
+    double p = Q[4 * sv];
+    double c = sqrt(gamma * Q[4 * sv] / Q[0]);
+    double dq1 = A[0] - r;
+    double dq2 = A[sv] / A[0] - u1;
+    double dq3 = A[2 * sv] / A[0] - u2;
+    double dq4 = A[3 * sv] / A[0] - u3;
+    double dq5 = A[4 * sv] - p;
+    const double kx = -k[0], ky = -k[1], kz = -k[2];
+    double R1 = kx * dq1 + kz * dq3 - ky * dq4 - kx * dq5 / (c * c);
+    double R2 = ky * dq1 - kz * dq2 + kx * dq4 - ky * dq5 / (c * c);
+    double R3 = kz * dq1 + ky * dq2 - kx * dq3 - kz * dq5 / (c * c);
+    double R4 = kx * dq2 + ky * dq3 + kz * dq4 + dq5 / (r * c);
+    double R5 = -kx * dq2 - ky * dq3 - kz * dq4 + dq5 / (r * c);
+    double lamS = kx * u1 + ky * u2 + kz * u3;
+    double lamP = lamS + c, lamM = lamS - c;
+    if (lamS < 0.0) R1 = R2 = R3 = 0.0;
+    if (lamP < 0.0) R4 = 0.0;
+    if (lamM < 0.0) R5 = 0.0;
+    dq1 = kx * R1 + ky * R2 + kz * R3 + r / (2.0 * c) * R4 + r / (2.0 * c) * R5;
+    dq2 = -kz * R2 + ky * R3 + 0.5 * kx * R4 - 0.5 * kx * R5;
+    dq3 = kz * R1 - kx * R3 + 0.5 * ky * R4 - 0.5 * ky * R5;
+    dq4 = -ky * R1 + kx * R2 + 0.5 * kz * R4 - 0.5 * kz * R5;
+    dq5 = +0.5 * r * c * R4 + 0.5 * r * c * R5;
+    r = r + dq1;
+    u1 = u1 + dq2;
+    u2 = u2 + dq3;
+    u3 = u3 + dq4;
+    p = p + dq5;
+    Q[0] = r;
+    Q[sv] = r * u1;
+    Q[2 * sv] = r * u2;
+    Q[3 * sv] = r * u3;
+    Q[4 * sv] = p;
+  }
+}
+
+/* src/boundaries/nonreflecting.f90:453-554 (adjoint, target zero). */
+void orc_characteristic_qs_k1d(int np, int sv, double *qs, const double *q, const double k[3],
+                               double gamma) {
+  for (int m = 0; m < np; ++m) {
+    double *S = qs + m;
+    const double *Q = q + m;
+    double ds1 = -S[0], ds2 = -S[sv], ds3 = -S[2 * sv], ds4 = -S[3 * sv], ds5 = -S[4 * sv];
+    const double kx = -k[0], ky = -k[1], kz = -k[2];
+    const double gm1 = gamma - 1.0;
+    double c = sqrt(gamma * (Q[4 * sv] / Q[0]));
+    double g_c = gm1 / c;
+    double u = Q[sv] / Q[0], v = Q[2 * sv] / Q[0], w = Q[3 * sv] / Q[0];
+    double ku = kx * u + ky * v + kz * w;
+    double kus = kx * ds2 + ky * ds3 + kz * ds4;
+    double ruus = (ds1 + u * ds2 + v * ds3 + w * ds4) / c;
+    double rups = ruus + ds5 / g_c;
+    int Sin = ku < 0.0, Pin = ku + c < 0.0, Min = ku - c < 0.0;
+    double drs = 0, dus = 0, dvs = 0, dws = 0, dps = 0, tmp;
+    if (Sin) {
+      drs = ds1 + ku * kus;
+      dus = ds2 - kx * kus;
+      dvs = ds3 - ky * kus;
+      dws = ds4 - kz * kus;
+      dps = -g_c * ruus;
+    }
+    if (Pin) {
+      tmp = 0.5 * (kus + rups);
+      drs = drs - ku * tmp;
+      dus = dus + kx * tmp;
+      dvs = dvs + ky * tmp;
+      dws = dws + kz * tmp;
+      dps = dps + g_c * tmp;
+    }
+    if (Min) {
+      tmp = 0.5 * (kus - rups);
+      drs = drs - ku * tmp;
+      dus = dus + kx * tmp;
+      dvs = dvs + ky * tmp;
+      dws = dws + kz * tmp;
+      dps = dps - g_c * tmp;
+    }
+    S[0] = S[0] + drs;
+    S[sv] = S[sv] + dus;
+    S[2 * sv] = S[2 * sv] + dvs;
+    S[3 * sv] = S[3 * sv] + dws;
+    S[4 * sv] = S[4 * sv] + dps;
+  }
+}
+
+static void bc_unsupported(int type, const char *where) {
+  fprintf(stderr, "oracle: boundary type %d not implemented in %s\n", type, where);
+  abort();
+}
+
+/* set_boundary_condition_q, src/boundaries/boundary_conditions.f90:52-136
+ * (types 0, 201, 400, 401). */
+void orc_bc_q(const orc_cfg *c, double *q, const double *qref, const double *shape) {
+  const size_t N = npts(c);
+  for (int r = 0; r < c->nbc; ++r) {
+    const int t = bc_type(c, r);
+    if (t == 0 || t == 400) continue;
+    if (t == 201) {
+      const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
+      FOR_BOX(c, r, idx) orc_characteristic_q_k1d(1, (int)N, q + idx, qref + idx, k, c->gamma);
+    } else if (t == 401) {
+      /* sponge_q with qref, src/boundaries/sponge_layer_core.f90:70-103 */
+      for (int v = 0; v < NV; ++v)
+        for (size_t i = 0; i < N; ++i)
+          q[v * N + i] = q[v * N + i] - shape[i] * (q[v * N + i] - qref[v * N + i]);
+    } else {
+      bc_unsupported(t, "set_boundary_condition_q");
+    }
+  }
+}
+
+/* set_boundary_condition_qs, boundary_conditions.f90:140-224 (types 0, 201, 400, 401). */
+void orc_bc_qs(const orc_cfg *c, double *qs, const double *q, const double *shape) {
+  const size_t N = npts(c);
+  for (int r = 0; r < c->nbc; ++r) {
+    const int t = bc_type(c, r);
+    if (t == 0 || t == 400) continue;
+    if (t == 201) {
+      const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
+      FOR_BOX(c, r, idx) orc_characteristic_qs_k1d(1, (int)N, qs + idx, q + idx, k, c->gamma);
+    } else if (t == 401) {
+      /* sponge_qs, sponge_layer_core.f90:155-167 */
+      for (int v = 0; v < NV; ++v)
+        for (size_t i = 0; i < N; ++i) qs[v * N + i] = qs[v * N + i] - shape[i] * qs[v * N + i];
+    } else {
+      bc_unsupported(t, "set_boundary_condition_qs");
+    }
+  }
+}
+
+/* set_boundary_condition_rhs, boundary_conditions.f90:228-298 (types 0, 201, 400, 401). */
+void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qref,
+                const double *shape) {
+  const size_t N = npts(c);
+  for (int r = 0; r < c->nbc; ++r) {
+    const int t = bc_type(c, r);
+    if (t == 0 || t == 201 || t == 401) continue;
+    if (t == 400) {
+      /* sponge_rhs with qref, sponge_layer_core.f90:34-66 */
+      for (int v = 0; v < NV; ++v)
+        for (size_t i = 0; i < N; ++i)
+          rhs[v * N + i] = rhs[v * N + i] - shape[i] * (q[v * N + i] - qref[v * N + i]);
+    } else {
+      bc_unsupported(t, "set_boundary_condition_rhs");
+    }
+  }
+}
+
+/* set_boundary_condition_rhss called without q (ns_rho_rhou_p.f90:518):
+ * type 400 resolves to the empty sponge_rhss (sponge_layer_core.f90:189-196). */
+static void bc_rhss(const orc_cfg *c) {
+  for (int r = 0; r < c->nbc; ++r) {
+    const int t = bc_type(c, r);
+    if (t == 0 || t == 201 || t == 400 || t == 401) continue;
+    bc_unsupported(t, "set_boundary_condition_rhss");
+  }
+}
+
+/* src/boundaries/shape_functions.f90:45-63 (shape fcts) and :69-150 (composition). */
+void orc_sponge_shape(const orc_cfg *c, const double x0[3], const double x1[3],
+                      const double len[3], double amplitude, int cos_shape,
+                      const double *X1, const double *X2, const double *X3, double *shape) {
+  const double pi = 4.0 * atan(1.0);
+  const double *X[3] = {X1, X2, X3};
+  const int n1 = c->n[0], n2 = c->n[1], n3 = c->n[2];
+  for (int k = 0; k < n3; ++k)
+    for (int j = 0; j < n2; ++j)
+      for (int i = 0; i < n1; ++i) {
+        const double xv[3] = {X[0][i], X[1][j], X[2][k]};
+        double phi = 1.0;
+        for (int d = 0; d < 3; ++d) {
+          for (int side = 0; side < 2; ++side) {
+            const double xo = side == 0 ? x0[d] + len[d] : x1[d] - len[d];
+            double s = cos_shape ? 0.5 + 0.5 * cos((xv[d] - xo) * (pi / len[d]))
+                                 : 1.0 - ((xv[d] - xo) / len[d]) * ((xv[d] - xo) / len[d]);
+            if (side == 0 && xv[d] > xo) s = 1.0;
+            if (side == 1 && xv[d] < xo) s = 1.0;
+            phi = phi * s;
+          }
+        }
+        phi = 1.0 - phi;
+        shape[(size_t)i + (size_t)n1 * (j + (size_t)n2 * k)] = amplitude * phi;
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Direct NS right-hand side, src/equations/ns_rho_rhou_p.f90:31-253.       */
+void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const double *qref,
+                    const double *shape, int heat_unit_spacing) {
+  const size_t N = npts(c);
+  const double dx1 = c->dx[0], dx2 = c->dx[1], dx3 = c->dx[2];
+  const double hx1 = heat_unit_spacing ? 1.0 : dx1;
+  const double hx2 = heat_unit_spacing ? 1.0 : dx2;
+  const double hx3 = heat_unit_spacing ? 1.0 : dx3;
+  double *q = xalloc(5 * N), *u = xalloc(3 * N);
+  double *ax = xalloc(N), *ay = xalloc(N), *az = xalloc(N);
+  double *px = xalloc(N), *py = xalloc(N), *pz = xalloc(N);
+  double *ux = xalloc(N), *uy = xalloc(N), *uz = xalloc(N);
+  double *vx = xalloc(N), *vy = xalloc(N), *vz = xalloc(N);
+  double *wx = xalloc(N), *wy = xalloc(N), *wz = xalloc(N);
+  double *t11 = xalloc(N), *t12 = xalloc(N), *t13 = xalloc(N);
+  double *t22 = xalloc(N), *t23 = xalloc(N), *t33 = xalloc(N);
+  double *temp = xalloc(N), *mu = xalloc(N), *mudiv = xalloc(N), *lam = xalloc(N);
+  double *tmp = xalloc(N);
+  double *r1 = rhs, *r2 = rhs + N, *r3 = rhs + 2 * N, *r4 = rhs + 3 * N, *r5 = rhs + 4 * N;
+
+  memcpy(q, q_in, 5 * N * sizeof(double)); /* :89 */
+  orc_bc_q(c, q, qref, shape);              /* :92 */
+  const double *q1 = q, *q2 = q + N, *q3 = q + 2 * N, *q4 = q + 3 * N, *q5 = q + 4 * N;
+  double *u1 = u, *u2 = u + N, *u3 = u + 2 * N;
+  for (size_t i = 0; i < N; ++i) { /* :95-99 */
+    u1[i] = q2[i] / q1[i];
+    u2[i] = q3[i] / q1[i];
+    u3[i] = q4[i] / q1[i];
+  }
+  /* :102-112 */
+  orc_diff3d(c, 0, ux, u1, dx1);
+  orc_diff3d(c, 1, uy, u1, dx2);
+  orc_diff3d(c, 2, uz, u1, dx3);
+  orc_diff3d(c, 0, vx, u2, dx1);
+  orc_diff3d(c, 1, vy, u2, dx2);
+  orc_diff3d(c, 2, vz, u2, dx3);
+  orc_diff3d(c, 0, wx, u3, dx1);
+  orc_diff3d(c, 1, wy, u3, dx2);
+  orc_diff3d(c, 2, wz, u3, dx3);
+  /* :115-128 */
+  for (size_t i = 0; i < N; ++i) {
+    temp[i] = orc_T(c, q1[i], q5[i]);
+    mu[i] = orc_mu(c, temp[i]);
+    mudiv[i] = -2.0 / 3.0 * mu[i] * (ux[i] + vy[i] + wz[i]);
+    t11[i] = mudiv[i] + 2 * mu[i] * ux[i];
+    t22[i] = mudiv[i] + 2 * mu[i] * vy[i];
+    t33[i] = mudiv[i] + 2 * mu[i] * wz[i];
+    t12[i] = mu[i] * (uy[i] + vx[i]);
+    t13[i] = mu[i] * (uz[i] + wx[i]);
+    t23[i] = mu[i] * (vz[i] + wy[i]);
+  }
+  /* mass :131-135 */
+  orc_diff3d(c, 0, ax, q2, dx1);
+  orc_diff3d(c, 1, ay, q3, dx2);
+  orc_diff3d(c, 2, az, q4, dx3);
+  for (size_t i = 0; i < N; ++i) r1[i] = -ax[i] - ay[i] - az[i];
+
+  /* momentum :138-177 */
+  const double *qd[3] = {q2, q3, q4};
+  double *pd[3] = {px, py, pz};
+  double *rm[3] = {r2, r3, r4};
+  const double *tau[3][3] = {{t11, t12, t13}, {t12, t22, t23}, {t13, t23, t33}};
+  const double dxs[3] = {dx1, dx2, dx3};
+  for (int m = 0; m < 3; ++m) {
+    const double *um = u + m * N;
+    double *a[3] = {ax, ay, az};
+    for (int d = 0; d < 3; ++d) {
+      for (size_t i = 0; i < N; ++i) tmp[i] = qd[d][i] * um[i];
+      orc_diff3d(c, d, a[d], tmp, dxs[d]);
+    }
+    orc_diff3d(c, m, pd[m], q5, dxs[m]);
+    for (size_t i = 0; i < N; ++i) rm[m][i] = -ax[i] - ay[i] - az[i] - pd[m][i];
+    for (int d = 0; d < 3; ++d) orc_diff3d(c, d, a[d], tau[m][d], dxs[d]);
+    for (size_t i = 0; i < N; ++i) rm[m][i] = rm[m][i] + ax[i] + ay[i] + az[i];
+  }
+
+  /* energy (pressure) :180-209 */
+  {
+    double *a[3] = {ax, ay, az};
+    for (int d = 0; d < 3; ++d) {
+      const double *ud = u + d * N;
+      for (size_t i = 0; i < N; ++i) tmp[i] = q5[i] * ud[i];
+      orc_diff3d(c, d, a[d], tmp, dxs[d]);
+    }
+    const double gt = c->gamma / (c->gamma - 1.0);
+    for (size_t i = 0; i < N; ++i) {
+      r5[i] = gt * (-ax[i] - ay[i] - az[i]);
+      r5[i] = r5[i] + u1[i] * px[i] + u2[i] * py[i] + u3[i] * pz[i];
+    }
+    /* heat: Dx without h (dx = 1) in the reference, :189-197 */
+    orc_diff3d(c, 0, ax, temp, hx1);
+    orc_diff3d(c, 1, ay, temp, hx2);
+    orc_diff3d(c, 2, az, temp, hx3);
+    for (size_t i = 0; i < N; ++i) lam[i] = orc_lambda(c, mu[i]);
+    const double hx[3] = {hx1, hx2, hx3};
+    for (int d = 0; d < 3; ++d) {
+      for (size_t i = 0; i < N; ++i) tmp[i] = lam[i] * a[d][i];
+      orc_diff3d(c, d, a[d], tmp, hx[d]);
+    }
+    for (size_t i = 0; i < N; ++i) {
+      r5[i] = r5[i] + ax[i] + ay[i] + az[i];
+      r5[i] = r5[i] + t11[i] * ux[i] + t12[i] * uy[i] + t13[i] * uz[i] + t12[i] * vx[i] +
+              t22[i] * vy[i] + t23[i] * vz[i] + t13[i] * wx[i] + t23[i] * wy[i] + t33[i] * wz[i];
+      r5[i] = (c->gamma - 1.0) * r5[i];
+    }
+  }
+  /* force (no-op stub src/opt/force.f90), in-rhs sponge (empty, sponge_layer.f90:15) */
+  orc_bc_rhs(c, rhs, q, qref, shape); /* :218 */
+
+  free(q); free(u); free(ax); free(ay); free(az); free(px); free(py); free(pz);
+  free(ux); free(uy); free(uz); free(vx); free(vy); free(vz); free(wx); free(wy); free(wz);
+  free(t11); free(t12); free(t13); free(t22); free(t23); free(t33);
+  free(temp); free(mu); free(mudiv); free(lam); free(tmp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint (continuous Euler adjoint) RHS, ns_rho_rhou_p.f90:257-536.       */
+void orc_rhs_adjoint(const orc_cfg *c, double *rhs, const double *qs_in, const double *q,
+                     const double *g, const double *shape) {
+  const size_t N = npts(c);
+  const double gam = c->gamma, gm1 = c->gamma - 1.0;
+  double *qs = xalloc(5 * N);
+  double *u = xalloc(3 * N), *gdp = xalloc(N), *irho = xalloc(N);
+  double *a = xalloc(N), *b = xalloc(N), *cc = xalloc(N), *d = xalloc(N), *e = xalloc(N);
+  double *tmp = xalloc(N);
+  memcpy(qs, qs_in, 5 * N * sizeof(double));
+  orc_bc_qs(c, qs, q, shape); /* :290 */
+  const double *q1 = q, *q5 = q + 4 * N;
+  double *r[5];
+  for (int v = 0; v < 5; ++v) r[v] = rhs + v * N;
+  for (size_t i = 0; i < N; ++i) {
+    gdp[i] = gam * q5[i] / (q1[i] * gm1);
+    irho[i] = 1.0 / q1[i];
+  }
+  memset(rhs, 0, 5 * N * sizeof(double));
+  for (int m = 0; m < 3; ++m)
+    for (size_t i = 0; i < N; ++i) u[m * N + i] = q[(m + 1) * N + i] / q1[i];
+  const double *u1 = u, *u2 = u + N, *u3 = u + 2 * N;
+  const double *qs5 = qs + 4 * N;
+
+  for (int dd = 0; dd < 3; ++dd) {
+    const double h = c->dx[dd];
+    orc_diff3d(c, dd, a, qs, h);
+    orc_diff3d(c, dd, b, qs + N, h);
+    orc_diff3d(c, dd, cc, qs + 2 * N, h);
+    orc_diff3d(c, dd, d, qs + 3 * N, h);
+    orc_diff3d(c, dd, e, qs + 4 * N, h);
+    const double *ud = u + dd * N;
+    for (size_t i = 0; i < N; ++i) {
+      if (dd == 0) { /* :317-339 */
+        r[0][i] = r[0][i] + (u1[i] * u1[i]) * b[i] + (u1[i] * u2[i]) * cc[i] +
+                  (u1[i] * u3[i]) * d[i] + gdp[i] * u1[i] * e[i];
+        r[1][i] = r[1][i] - a[i] - 2.0 * u1[i] * b[i] - u2[i] * cc[i] - u3[i] * d[i] - gdp[i] * e[i];
+        r[2][i] = r[2][i] - u1[i] * cc[i];
+        r[3][i] = r[3][i] - u1[i] * d[i];
+        r[4][i] = r[4][i] - gm1 * b[i] - gam * u1[i] * e[i];
+      } else if (dd == 1) { /* :407-428 */
+        r[0][i] = r[0][i] + (u2[i] * u1[i]) * b[i] + (u2[i] * u2[i]) * cc[i] +
+                  (u2[i] * u3[i]) * d[i] + gdp[i] * u2[i] * e[i];
+        r[1][i] = r[1][i] - u2[i] * b[i];
+        r[2][i] = r[2][i] - a[i] - u1[i] * b[i] - 2 * u2[i] * cc[i] - u3[i] * d[i] - gdp[i] * e[i];
+        r[3][i] = r[3][i] - u2[i] * d[i];
+        r[4][i] = r[4][i] - gm1 * cc[i] - gam * u2[i] * e[i];
+      } else { /* :475-497 */
+        r[0][i] = r[0][i] + (u3[i] * u1[i]) * b[i] + (u3[i] * u2[i]) * cc[i] +
+                  (u3[i] * u3[i]) * d[i] + gdp[i] * u3[i] * e[i];
+        r[1][i] = r[1][i] - u3[i] * b[i];
+        r[2][i] = r[2][i] - u3[i] * cc[i];
+        r[3][i] = r[3][i] - a[i] - u1[i] * b[i] - u2[i] * cc[i] - 2 * u3[i] * d[i] - gdp[i] * e[i];
+        r[4][i] = r[4][i] - gm1 * d[i] - gam * u3[i] * e[i];
+      }
+    }
+    /* -(A^T)^-1 (C^T qs)_x : :355-357, :443-445, :512-514 */
+    for (size_t i = 0; i < N; ++i) tmp[i] = -ud[i] * qs5[i];
+    orc_diff3d(c, dd, a, tmp, h);
+    for (size_t i = 0; i < N; ++i) r[4][i] = r[4][i] - gm1 * a[i];
+    /* +(A^T)^-1 C~ c_x : :366-372 etc. */
+    orc_diff3d(c, dd, a, q5, h);
+    for (size_t i = 0; i < N; ++i) {
+      r[0][i] = r[0][i] + qs5[i] * ud[i] * irho[i] * a[i];
+      r[dd + 1][i] = r[dd + 1][i] - qs5[i] * irho[i] * a[i];
+    }
+  }
+  if (g)
+    for (size_t i = 0; i < 5 * N; ++i) rhs[i] = rhs[i] + g[i]; /* :512 */
+  /* sponge_adjoint is the empty in-rhs sponge; rhss BCs without q */
+  bc_rhss(c);
+  free(qs); free(u); free(gdp); free(irho); free(a); free(b); free(cc); free(d); free(e);
+  free(tmp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* RK4, src/time/rk_o4_e.f90:17-58.                                          */
+void orc_time_stepper_direct(const orc_cfg *c, double *q1, const double *q0,
+                             const double *qref, const double *shape, int heat_unit_spacing) {
+  const size_t M = 5 * npts(c);
+  const double dt = c->dt, dth = c->dt * (0.5);
+  double *k1 = xalloc(M), *k2 = xalloc(M), *k3 = xalloc(M), *k4 = xalloc(M), *s = xalloc(M);
+  orc_rhs_direct(c, k1, q0, qref, shape, heat_unit_spacing);
+  for (size_t i = 0; i < M; ++i) s[i] = q0[i] + dth * k1[i];
+  orc_rhs_direct(c, k2, s, qref, shape, heat_unit_spacing);
+  for (size_t i = 0; i < M; ++i) s[i] = q0[i] + dth * k2[i];
+  orc_rhs_direct(c, k3, s, qref, shape, heat_unit_spacing);
+  for (size_t i = 0; i < M; ++i) s[i] = q0[i] + dt * k3[i];
+  orc_rhs_direct(c, k4, s, qref, shape, heat_unit_spacing);
+  for (size_t i = 0; i < M; ++i) q1[i] = q0[i] + dt / 6.0 * (k1[i] + 2 * k2[i] + 2 * k3[i] + k4[i]);
+  free(k1); free(k2); free(k3); free(k4); free(s);
+}
+
+/* src/time/rk_o4_e.f90:62-105 (dt -> -dt; q0 and g frozen over the stages). */
+void orc_time_stepper_adjoint(const orc_cfg *c, double *qs1, const double *qs0,
+                              const double *q0, const double *g, const double *shape) {
+  const size_t M = 5 * npts(c);
+  const double dt = -c->dt, dth = -c->dt * (0.5);
+  double *k1 = xalloc(M), *k2 = xalloc(M), *k3 = xalloc(M), *k4 = xalloc(M), *s = xalloc(M);
+  orc_rhs_adjoint(c, k1, qs0, q0, g, shape);
+  for (size_t i = 0; i < M; ++i) s[i] = qs0[i] + dth * k1[i];
+  orc_rhs_adjoint(c, k2, s, q0, g, shape);
+  for (size_t i = 0; i < M; ++i) s[i] = qs0[i] + dth * k2[i];
+  orc_rhs_adjoint(c, k3, s, q0, g, shape);
+  for (size_t i = 0; i < M; ++i) s[i] = qs0[i] + dt * k3[i];
+  orc_rhs_adjoint(c, k4, s, q0, g, shape);
+  for (size_t i = 0; i < M; ++i) qs1[i] = qs0[i] + dt / 6.0 * (k1[i] + 2 * k2[i] + 2 * k3[i] + k4[i]);
+  free(k1); free(k2); free(k3); free(k4); free(s);
+}
+
+/* Step-loop body, src/main/computation_direct.f90:67-98 / cruna_hpc.f90:171-229:
+ * stepper -> qn = q1 -> BC on q (outside rhs) -> filters i1, i2, i3 at their frequencies. */
+void orc_run_direct(const orc_cfg *c, double *q, int nt0, int nsteps, const double *qref,
+                    const double *shape, int heat_unit_spacing) {
+  const size_t M = 5 * npts(c);
+  double *q1 = xalloc(M);
+  for (int nt = nt0 + 1; nt <= nt0 + nsteps; ++nt) {
+    orc_time_stepper_direct(c, q1, q, qref, shape, heat_unit_spacing);
+    memcpy(q, q1, M * sizeof(double));
+    orc_bc_q(c, q, qref, shape);
+    for (int d = 0; d < 3; ++d)
+      if (c->filter_freq[d] > 0 && nt % c->filter_freq[d] == 0) orc_filter4d(c, d, q, 5);
+  }
+  free(q1);
+}
+
+/* src/main/computation_adjoint.f90:62-99 with g = 0 (calc_objective_g stub). */
+void orc_run_adjoint(const orc_cfg *c, double *qs, const double *Q, int nsteps,
+                     const double *shape) {
+  const size_t M = 5 * npts(c);
+  double *qs1 = xalloc(M);
+  for (int nt = nsteps; nt >= 1; --nt) {
+    const double *q0 = Q + (size_t)(nt - 1) * M;
+    orc_time_stepper_adjoint(c, qs1, qs, q0, NULL, shape);
+    memcpy(qs, qs1, M * sizeof(double));
+    orc_bc_qs(c, qs, q0, shape);
+    for (int d = 0; d < 3; ++d)
+      if (c->afilter_freq[d] > 0 && nt % c->afilter_freq[d] == 0) orc_filter4d(c, d, qs, 5);
+  }
+  free(qs1);
+}
