@@ -1,0 +1,124 @@
+/*
+ * cruna_hip -- MI355X (gfx950) implementation of the CRUNA time-stepping hot
+ * path, exported as a plain C ABI for the reference's Fortran host
+ * (iso_c_binding) and for any other FFI (ctypes, ...).
+ *
+ * Data layout is the reference's: every 3-D field is Fortran column-major
+ * f(n1,n2,n3) with i fastest, and a state is q(n1,n2,n3,5) holding
+ * (rho, rho*u, rho*v, rho*w, p) with the variable index last
+ * (src/data/data_rhs.f90:16-41, src/equations/ns_rho_rhou_p.f90:95-99).
+ * Host pointers are plain double*; every call returns 0 on success or a
+ * negative CRUNA_HIP_E* code, with a message in cruna_hip_last_error().
+ *
+ * A context is bound to one HIP device and one stream and is not
+ * thread-safe, like the reference's module state (save variables).
+ */
+#ifndef CRUNA_HIP_H
+#define CRUNA_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRUNA_HIP_OK 0
+#define CRUNA_HIP_EINVAL (-1)   /* bad argument / unsupported configuration */
+#define CRUNA_HIP_EDEVICE (-2)  /* HIP runtime error */
+#define CRUNA_HIP_ENOMEM (-3)   /* device allocation failed */
+
+#define CRUNA_HIP_PERIODIC 0 /* diff/filter_pen_core_o6_i_c_p */
+#define CRUNA_HIP_OPEN 1     /* diff/filter_pen_core_o6_i_c_o */
+
+#define CRUNA_HIP_VISC_SUTHERLAND 0 /* ns_rho_rhou_p_trafos.f90:119-142 */
+#define CRUNA_HIP_VISC_POWER 1      /* case-level power-law override */
+
+#define CRUNA_HIP_FP64 0       /* all arithmetic in FP64 */
+#define CRUNA_HIP_FP32_MIXED 1 /* stencil/filter/RHS in FP32, RK4 state in FP64 */
+
+#define CRUNA_HIP_MAX_BC 32
+
+typedef struct cruna_hip_cfg {
+  int n[3];              /* params%geom%n1..n3 (block = whole grid on one GPU) */
+  double dx[3];          /* params%geom%dx1..dx3 */
+  int scheme[3];         /* CRUNA_HIP_PERIODIC / CRUNA_HIP_OPEN per direction */
+  double filter_alpha;   /* filter.alpha (default 0.35) */
+  double gamma;          /* material.gamma */
+  double R;              /* material.R */
+  double W;              /* material.W */
+  double Pr;             /* material.Pr */
+  double mu0;            /* material.mu0 */
+  double T0;             /* material.T0 */
+  int visc_law;          /* CRUNA_HIP_VISC_* */
+  double visc_exp;       /* exponent of the power law */
+  double dt;             /* time.dt */
+  int filter_freq[3];    /* filter.di1freq..di3freq (0 = never) */
+  int heat_unit_spacing; /* 1: heat-flux Dx without h, as ns_rho_rhou_p.f90:189-197 */
+  int precision;         /* CRUNA_HIP_FP64 / CRUNA_HIP_FP32_MIXED */
+  /* boundary_conditions.dat rows: type; i1s; i1e; i2s; i2e; i3s; i3e; k1; k2; k3 */
+  int nbc;
+  double bc[CRUNA_HIP_MAX_BC][10];
+} cruna_hip_cfg;
+
+typedef struct cruna_hip_ctx cruna_hip_ctx;
+
+/* Fill cfg with the reference defaults (filter.alpha 0.35, Pr 0.71, ...). */
+void cruna_hip_default_cfg(cruna_hip_cfg *cfg);
+
+/* Create / destroy a context: builds the line-operator tables
+ * (init_left_implicit_diff_part / init_right_explicit_diff_part of
+ * src/diff/diff_pen_core_o6_i_c_{p,o}.f90 and the filter equivalents) and
+ * allocates the device-resident state. */
+int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out);
+int cruna_hip_destroy(cruna_hip_ctx *ctx);
+const char *cruna_hip_last_error(void);
+
+/* Device-resident state q^n (host <-> device copies, layout q(n1,n2,n3,5)). */
+int cruna_hip_set_state(cruna_hip_ctx *ctx, const double *q);
+int cruna_hip_get_state(cruna_hip_ctx *ctx, double *q);
+/* Reference state qref (src/data/data_reference.f90, qref = q0 at init) and
+ * the sponge shape sigma(n1,n2,n3) (shape_functions.f90:69-150). */
+int cruna_hip_set_qref(cruna_hip_ctx *ctx, const double *qref);
+int cruna_hip_set_sponge_shape(cruna_hip_ctx *ctx, const double *shape);
+
+/* Dx1/Dx2/Dx3 on a 3-D field: diff_iN_mpi_cartesian::diff_3d
+ * (src/diff/diff_i1_mpi_cartesian.f90:83-103).  dir = 0,1,2; h = spacing
+ * (pass 1.0 for the optional-h default). */
+int cruna_hip_diff(cruna_hip_ctx *ctx, int dir, double *q_x, const double *q, double h);
+
+/* filter_i1/i2/i3 on a 4-D field in place: filter_iN_mpi_cartesian::filter_4d
+ * (src/filter/filter_i1_mpi_cartesian.f90:83-97); nv variables. */
+int cruna_hip_filter(cruna_hip_ctx *ctx, int dir, double *q, int nv);
+
+/* right_hand_side_direct_3d(rhs,q_in,t) (src/equations/ns_rho_rhou_p.f90:31-253). */
+int cruna_hip_rhs_direct(cruna_hip_ctx *ctx, double *rhs, const double *q, double t);
+
+/* right_hand_side_adjoint_3d(rhs,qs_in,q,g,t) (ns_rho_rhou_p.f90:257-536); g may be NULL. */
+int cruna_hip_rhs_adjoint(cruna_hip_ctx *ctx, double *rhs, const double *qs, const double *q,
+                          const double *g, double t);
+
+/* time_stepper_direct(q1,q0,t) (src/time/rk_o4_e.f90:17-58), host in/out. */
+int cruna_hip_time_stepper_direct(cruna_hip_ctx *ctx, double *q1, const double *q0, double t);
+
+/* time_stepper_adjoint(qs1,qs0,q0,g,t) (src/time/rk_o4_e.f90:62-105), host in/out. */
+int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *ctx, double *qs1, const double *qs0,
+                                   const double *q0, const double *g, double t);
+
+/* Device-resident step loop body of cruna_hpc / calc_direct
+ * (src/main/cruna_hpc.f90:171-229): for nt = nt0+1..nt0+nsteps:
+ * RK4 -> BC on q (outside rhs) -> filter i1,i2,i3 where mod(nt,freq) == 0.
+ * Asynchronous on the context stream; cruna_hip_synchronize waits. */
+int cruna_hip_run_direct(cruna_hip_ctx *ctx, int nt0, int nsteps);
+int cruna_hip_synchronize(cruna_hip_ctx *ctx);
+
+/* Per-kernel timing with HIP events on the context stream: enable, run,
+ * then read the summed milliseconds and launch counts per kernel id. */
+int cruna_hip_profile_enable(cruna_hip_ctx *ctx, int on);
+int cruna_hip_profile_read(cruna_hip_ctx *ctx, int kernel_id, double *ms, long *launches,
+                           char *name, size_t name_len);
+int cruna_hip_profile_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,9 @@
+"""MI355X (gfx950) implementation of the CRUNA time-stepping hot path.
+
+The product is libcruna_hip.so (C ABI, include/cruna_hip.h); `cruna` is the
+Python host mirror of the reference's interface and `ic` builds the seeded
+synthetic workloads.
+"""
+from . import ic  # noqa: F401
+from .cruna import (FP32_MIXED, FP64, OPEN, PERIODIC, POWER, SUTHERLAND, Cruna,  # noqa: F401
+                    CrunaError, lib)

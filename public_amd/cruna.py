@@ -1,0 +1,240 @@
+"""Python host mirror of the CRUNA hot-path interface over libcruna_hip.so.
+
+Names follow the reference's Fortran surface (cruna-toolkit/public):
+  Dx1/Dx2/Dx3                 diff_iN_mpi_cartesian::diff (src/diff/diff_i1_mpi_cartesian.f90:7-12)
+  filter_i1/filter_i2/...     filter_iN_mpi_cartesian::filter (src/filter/filter_i1_mpi_cartesian.f90:8-13)
+  right_hand_side_direct_3d   src/equations/ns_rho_rhou_p.f90:31
+  time_stepper_direct         src/time/rk_o4_e.f90:17
+and the device-resident step loop of src/main/cruna_hpc.f90:171-229.
+Arrays are numpy float64, Fortran order, shape (n1, n2, n3[, 5]).  Errors
+from the library raise CrunaError, as the reference's "error in ..." + stop.
+The HIP library is required: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcruna_hip.so")
+
+PERIODIC, OPEN = 0, 1
+SUTHERLAND, POWER = 0, 1
+FP64, FP32_MIXED = 0, 1
+MAX_BC = 32
+
+
+class CrunaError(RuntimeError):
+    pass
+
+
+class Cfg(C.Structure):
+    """cruna_hip_cfg of include/cruna_hip.h."""
+    _fields_ = [
+        ("n", C.c_int * 3),
+        ("dx", C.c_double * 3),
+        ("scheme", C.c_int * 3),
+        ("filter_alpha", C.c_double),
+        ("gamma", C.c_double),
+        ("R", C.c_double),
+        ("W", C.c_double),
+        ("Pr", C.c_double),
+        ("mu0", C.c_double),
+        ("T0", C.c_double),
+        ("visc_law", C.c_int),
+        ("visc_exp", C.c_double),
+        ("dt", C.c_double),
+        ("filter_freq", C.c_int * 3),
+        ("heat_unit_spacing", C.c_int),
+        ("precision", C.c_int),
+        ("nbc", C.c_int),
+        ("bc", (C.c_double * 10) * MAX_BC),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libcruna_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CrunaError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+        L.cruna_hip_default_cfg.argtypes = [C.POINTER(Cfg)]
+        L.cruna_hip_default_cfg.restype = None
+        L.cruna_hip_create.argtypes = [C.POINTER(Cfg), C.POINTER(vp)]
+        L.cruna_hip_destroy.argtypes = [vp]
+        L.cruna_hip_last_error.restype = C.c_char_p
+        L.cruna_hip_set_state.argtypes = [vp, dp]
+        L.cruna_hip_get_state.argtypes = [vp, dp]
+        L.cruna_hip_set_qref.argtypes = [vp, dp]
+        L.cruna_hip_set_sponge_shape.argtypes = [vp, dp]
+        L.cruna_hip_diff.argtypes = [vp, ip, dp, dp, C.c_double]
+        L.cruna_hip_filter.argtypes = [vp, ip, dp, ip]
+        L.cruna_hip_rhs_direct.argtypes = [vp, dp, dp, C.c_double]
+        L.cruna_hip_rhs_adjoint.argtypes = [vp, dp, dp, dp, dp, C.c_double]
+        L.cruna_hip_time_stepper_direct.argtypes = [vp, dp, dp, C.c_double]
+        L.cruna_hip_time_stepper_adjoint.argtypes = [vp, dp, dp, dp, dp, C.c_double]
+        L.cruna_hip_run_direct.argtypes = [vp, ip, ip]
+        L.cruna_hip_synchronize.argtypes = [vp]
+        L.cruna_hip_profile_enable.argtypes = [vp, ip]
+        L.cruna_hip_profile_read.argtypes = [vp, ip, dp, C.POINTER(C.c_long), C.c_char_p, C.c_size_t]
+        L.cruna_hip_profile_count.argtypes = []
+        for f in ("create", "destroy", "set_state", "get_state", "set_qref", "set_sponge_shape", "diff",
+                  "filter", "rhs_direct", "rhs_adjoint", "time_stepper_direct", "time_stepper_adjoint",
+                  "run_direct", "synchronize", "profile_enable", "profile_read", "profile_count"):
+            getattr(L, "cruna_hip_" + f).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise CrunaError(lib().cruna_hip_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a, shape):
+    a = np.asfortranarray(a, dtype=np.float64)
+    if a.shape != shape:
+        raise CrunaError(f"array shape {a.shape} does not match the grid {shape}")
+    return a
+
+
+class Cruna:
+    """One device context: grid, schemes, material constants, resident state."""
+
+    def __init__(self, n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0, W=1.0,
+                 Pr=0.71, mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76, dt=1e-3,
+                 filter_freq=(1, 1, 1), heat_unit_spacing=1, precision=FP64, bc=()):
+        L = lib()
+        cfg = Cfg()
+        L.cruna_hip_default_cfg(C.byref(cfg))
+        for d in range(3):
+            cfg.n[d] = int(n[d])
+            cfg.dx[d] = float(dx[d])
+            cfg.scheme[d] = int(scheme[d])
+            cfg.filter_freq[d] = int(filter_freq[d])
+        cfg.filter_alpha = filter_alpha
+        cfg.gamma, cfg.R, cfg.W, cfg.Pr, cfg.mu0, cfg.T0 = gamma, R, W, Pr, mu0, T0
+        cfg.visc_law, cfg.visc_exp, cfg.dt = visc_law, visc_exp, dt
+        cfg.heat_unit_spacing, cfg.precision = heat_unit_spacing, precision
+        cfg.nbc = len(bc)
+        for r, row in enumerate(bc):
+            for j, v in enumerate(row):
+                cfg.bc[r][j] = float(v)
+        self.cfg = cfg
+        self.shape3 = tuple(int(v) for v in n)
+        self.shape4 = self.shape3 + (5,)
+        h = C.c_void_p()
+        _check(L.cruna_hip_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cruna_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ----- derivative and filter (parity-level entry points) -----
+    def Dx(self, d, f, h=None):
+        """diff_3d: compact derivative of a 3-D field along direction d; h defaults to 1."""
+        f = _f64(f, self.shape3)
+        out = np.empty(self.shape3, order="F")
+        _check(lib().cruna_hip_diff(self._h, d, _dp(out), _dp(f), 1.0 if h is None else float(h)))
+        return out
+
+    def Dx1(self, f, h=None):
+        return self.Dx(0, f, h)
+
+    def Dx2(self, f, h=None):
+        return self.Dx(1, f, h)
+
+    def Dx3(self, f, h=None):
+        return self.Dx(2, f, h)
+
+    def filter(self, d, q):
+        """filter_4d along direction d (in place on a copy, returned)."""
+        q = np.array(q, dtype=np.float64, order="F", copy=True)
+        nv = 1 if q.ndim == 3 else q.shape[3]
+        if q.shape[:3] != self.shape3:
+            raise CrunaError("field does not match the grid")
+        _check(lib().cruna_hip_filter(self._h, d, _dp(q), nv))
+        return q
+
+    def filter_i1(self, q):
+        return self.filter(0, q)
+
+    def filter_i2(self, q):
+        return self.filter(1, q)
+
+    def filter_i3(self, q):
+        return self.filter(2, q)
+
+    # ----- RHS and RK4 -----
+    def right_hand_side_direct_3d(self, q, t=0.0):
+        q = _f64(q, self.shape4)
+        rhs = np.empty(self.shape4, order="F")
+        _check(lib().cruna_hip_rhs_direct(self._h, _dp(rhs), _dp(q), float(t)))
+        return rhs
+
+    def time_stepper_direct(self, q0, t=0.0):
+        q0 = _f64(q0, self.shape4)
+        q1 = np.empty(self.shape4, order="F")
+        _check(lib().cruna_hip_time_stepper_direct(self._h, _dp(q1), _dp(q0), float(t)))
+        return q1
+
+    # ----- device-resident loop -----
+    def set_state(self, q):
+        q = _f64(q, self.shape4)
+        _check(lib().cruna_hip_set_state(self._h, _dp(q)))
+
+    def get_state(self):
+        q = np.empty(self.shape4, order="F")
+        _check(lib().cruna_hip_get_state(self._h, _dp(q)))
+        return q
+
+    def set_qref(self, qref):
+        qref = _f64(qref, self.shape4)
+        _check(lib().cruna_hip_set_qref(self._h, _dp(qref)))
+
+    def set_sponge_shape(self, shape):
+        shape = _f64(shape, self.shape3)
+        _check(lib().cruna_hip_set_sponge_shape(self._h, _dp(shape)))
+
+    def run_direct(self, nsteps, nt0=0):
+        _check(lib().cruna_hip_run_direct(self._h, int(nt0), int(nsteps)))
+
+    def synchronize(self):
+        _check(lib().cruna_hip_synchronize(self._h))
+
+    def profile(self, on=True):
+        _check(lib().cruna_hip_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self):
+        out = {}
+        for k in range(lib().cruna_hip_profile_count()):
+            ms = C.c_double()
+            n = C.c_long()
+            name = C.create_string_buffer(64)
+            _check(lib().cruna_hip_profile_read(self._h, k, C.byref(ms), C.byref(n), name, 64))
+            if n.value:
+                out[name.value.decode()] = (ms.value, n.value)
+        return out

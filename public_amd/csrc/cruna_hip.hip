@@ -1,0 +1,560 @@
+// C-ABI implementation of include/cruna_hip.h for MI355X (gfx950).
+#include "../../include/cruna_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "line_ops.hpp"
+#include "sweeps.hpp"
+
+using namespace cruna;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(CRUNA_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+enum KernelId {
+  KID_S1X = 0, KID_S2Y, KID_S3Z, KID_S4X, KID_S5Y, KID_FX, KID_FY, KID_FZ, KID_DIFF, KID_COUNT
+};
+const char *kKernelNames[KID_COUNT] = {"sweep1_x", "sweep1_y", "sweep1z_sweep2z", "sweep2_x",
+                                       "sweep2_y_rk", "filter_x", "filter_y", "filter_z",
+                                       "diff"};
+
+template <typename T>
+struct DevOp {
+  T *buf = nullptr;
+  LineOp<T> op{};
+  int upload(const HostLineOp &h) {
+    const size_t nM = h.M.size(), nL = h.L.size(), nR = h.R.size(), nW = h.W.size();
+    std::vector<T> all(nM + nL + nR + nW);
+    size_t o = 0;
+    for (double v : h.M) all[o++] = T(v);
+    for (double v : h.L) all[o++] = T(v);
+    for (double v : h.R) all[o++] = T(v);
+    for (double v : h.W) all[o++] = T(v);
+    HIP_TRY(hipMalloc(&buf, all.size() * sizeof(T)));
+    HIP_TRY(hipMemcpy(buf, all.data(), all.size() * sizeof(T), hipMemcpyHostToDevice));
+    op.M = buf;
+    op.L = buf + nM;
+    op.R = buf + nM + nL;
+    op.W = buf + nM + nL + nR;
+    op.n = h.n;
+    op.NB = h.NB;
+    op.K = h.K;
+    op.per = h.per;
+    return CRUNA_HIP_OK;
+  }
+  void release() {
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+  }
+};
+
+}  // namespace
+
+struct cruna_hip_ctx {
+  cruna_hip_cfg cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  long N = 0;
+  SweepGeom geom[3]{};
+  int wg[3] = {0, 0, 0};
+  DevOp<double> dop64[3], fop64[3];
+  DevOp<float> dop32[3], fop32[3];
+  Phys phys{};
+  // device state
+  double *qn = nullptr, *acc = nullptr, *qref = nullptr, *shape = nullptr, *tmp = nullptr;
+  void *qs = nullptr, *rhs = nullptr, *G = nullptr;  // compute precision
+  // profiling
+  bool prof = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> pool;
+  double prof_ms[KID_COUNT] = {};
+  long prof_n[KID_COUNT] = {};
+};
+
+namespace {
+
+int build_geom(cruna_hip_ctx *c, int d) {
+  const int n1 = c->cfg.n[0], n2 = c->cfg.n[1], n3 = c->cfg.n[2];
+  const int n = c->cfg.n[d];
+  SweepGeom &g = c->geom[d];
+  g.dir = d;
+  g.n = n;
+  g.NB = n / PPT;
+  g.n1 = n1;
+  g.n2 = n2;
+  g.n3 = n3;
+  const long lines_contig = d == 0 ? (long)n2 * n3 : n1;
+  int TL = 1;
+  while (g.NB * TL * 2 <= NT && lines_contig % (TL * 2) == 0) TL *= 2;
+  g.TL = TL;
+  if (d == 0) {
+    g.PS = 1;
+    g.ntiles = (int)(lines_contig / TL);
+    g.ntiles_in = 1;
+  } else {
+    g.PS = d == 1 ? n1 : (long)n1 * n2;
+    g.ntiles_in = n1 / TL;
+    g.ntiles = g.ntiles_in * (d == 1 ? n3 : n2);
+  }
+  c->wg[d] = g.NB * TL;
+  return CRUNA_HIP_OK;
+}
+
+hipEvent_t get_event(cruna_hip_ctx *c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+struct ProfScope {
+  cruna_hip_ctx *c;
+  int kid;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(cruna_hip_ctx *c_, int k) : c(c_), kid(k) {
+    if (c->prof) {
+      a = get_event(c);
+      b = get_event(c);
+      (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~ProfScope() {
+    if (c->prof) {
+      (void)hipEventRecord(b, c->stream);
+      c->pending.push_back({kid, {a, b}});
+    }
+  }
+};
+
+int collect_profile(cruna_hip_ctx *c) {
+  for (auto &p : c->pending) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    c->prof_ms[p.first] += ms;
+    c->prof_n[p.first] += 1;
+    c->pool.push_back(p.second.first);
+    c->pool.push_back(p.second.second);
+  }
+  c->pending.clear();
+  return CRUNA_HIP_OK;
+}
+
+int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(CRUNA_HIP_EDEVICE, std::string("launch: ") + hipGetErrorString(e));
+  return CRUNA_HIP_OK;
+}
+
+template <typename T>
+DevOp<T> *dops(cruna_hip_ctx *c);
+template <>
+DevOp<double> *dops<double>(cruna_hip_ctx *c) { return c->dop64; }
+template <>
+DevOp<float> *dops<float>(cruna_hip_ctx *c) { return c->dop32; }
+template <typename T>
+DevOp<T> *fops(cruna_hip_ctx *c);
+template <>
+DevOp<double> *fops<double>(cruna_hip_ctx *c) { return c->fop64; }
+template <>
+DevOp<float> *fops<float>(cruna_hip_ctx *c) { return c->fop32; }
+
+// One RHS evaluation (mode 4) or RK stage (mode 0..3) from stage input `in`.
+template <typename T, typename Q>
+int rhs_stage(cruna_hip_ctx *c, const Q *in, int mode) {
+  const long N = c->N;
+  T *rhs = (T *)c->rhs, *G = (T *)c->G, *qs = (T *)c->qs;
+  DevOp<T> *D = dops<T>(c);
+  {
+    ProfScope p(c, KID_S1X);
+    if (c->cfg.scheme[0] == CRUNA_HIP_PERIODIC)
+      hipLaunchKernelGGL((k_sweep1<T, Q, 0, true>), dim3(c->geom[0].ntiles), dim3(c->wg[0]), 0,
+                         c->stream, in, rhs, G, D[0].op, c->geom[0], c->phys, N);
+    else
+      hipLaunchKernelGGL((k_sweep1<T, Q, 0, false>), dim3(c->geom[0].ntiles), dim3(c->wg[0]), 0,
+                         c->stream, in, rhs, G, D[0].op, c->geom[0], c->phys, N);
+  }
+  if (int rc = check_launch()) return rc;
+  {
+    ProfScope p(c, KID_S2Y);
+    if (c->cfg.scheme[1] == CRUNA_HIP_PERIODIC)
+      hipLaunchKernelGGL((k_sweep1<T, Q, 1, true>), dim3(c->geom[1].ntiles), dim3(c->wg[1]), 0,
+                         c->stream, in, rhs, G, D[1].op, c->geom[1], c->phys, N);
+    else
+      hipLaunchKernelGGL((k_sweep1<T, Q, 1, false>), dim3(c->geom[1].ntiles), dim3(c->wg[1]), 0,
+                         c->stream, in, rhs, G, D[1].op, c->geom[1], c->phys, N);
+  }
+  if (int rc = check_launch()) return rc;
+  {
+    ProfScope p(c, KID_S3Z);
+    if (c->cfg.scheme[2] == CRUNA_HIP_PERIODIC)
+      hipLaunchKernelGGL((k_sweep1<T, Q, 2, true>), dim3(c->geom[2].ntiles), dim3(c->wg[2]), 0,
+                         c->stream, in, rhs, G, D[2].op, c->geom[2], c->phys, N);
+    else
+      hipLaunchKernelGGL((k_sweep1<T, Q, 2, false>), dim3(c->geom[2].ntiles), dim3(c->wg[2]), 0,
+                         c->stream, in, rhs, G, D[2].op, c->geom[2], c->phys, N);
+  }
+  if (int rc = check_launch()) return rc;
+  {
+    ProfScope p(c, KID_S4X);
+    if (c->cfg.scheme[0] == CRUNA_HIP_PERIODIC)
+      hipLaunchKernelGGL((k_sweep2<T, Q, 0, true>), dim3(c->geom[0].ntiles), dim3(c->wg[0]), 0,
+                         c->stream, in, rhs, (const T *)G, D[0].op, c->geom[0], c->phys, N, mode,
+                         c->qn, c->acc, qs, c->cfg.dt);
+    else
+      hipLaunchKernelGGL((k_sweep2<T, Q, 0, false>), dim3(c->geom[0].ntiles), dim3(c->wg[0]), 0,
+                         c->stream, in, rhs, (const T *)G, D[0].op, c->geom[0], c->phys, N, mode,
+                         c->qn, c->acc, qs, c->cfg.dt);
+  }
+  if (int rc = check_launch()) return rc;
+  {
+    ProfScope p(c, KID_S5Y);
+    if (c->cfg.scheme[1] == CRUNA_HIP_PERIODIC)
+      hipLaunchKernelGGL((k_sweep2<T, Q, 1, true>), dim3(c->geom[1].ntiles), dim3(c->wg[1]), 0,
+                         c->stream, in, rhs, (const T *)G, D[1].op, c->geom[1], c->phys, N, mode,
+                         c->qn, c->acc, qs, c->cfg.dt);
+    else
+      hipLaunchKernelGGL((k_sweep2<T, Q, 1, false>), dim3(c->geom[1].ntiles), dim3(c->wg[1]), 0,
+                         c->stream, in, rhs, (const T *)G, D[1].op, c->geom[1], c->phys, N, mode,
+                         c->qn, c->acc, qs, c->cfg.dt);
+  }
+  return check_launch();
+}
+
+template <typename T>
+int rk4_step(cruna_hip_ctx *c) {
+  if (int rc = rhs_stage<T, double>(c, c->qn, 0)) return rc;
+  for (int st = 1; st < 4; ++st)
+    if (int rc = rhs_stage<T, T>(c, (const T *)c->qs, st)) return rc;
+  return CRUNA_HIP_OK;
+}
+
+template <typename T>
+int filter_dir(cruna_hip_ctx *c, double *q, int d, int nv) {
+  ProfScope p(c, KID_FX + d);
+  if (c->cfg.scheme[d] == CRUNA_HIP_PERIODIC)
+    hipLaunchKernelGGL((k_filter<T, true>), dim3(c->geom[d].ntiles), dim3(c->wg[d]), 0, c->stream,
+                       q, fops<T>(c)[d].op, c->geom[d], c->N, nv);
+  else
+    hipLaunchKernelGGL((k_filter<T, false>), dim3(c->geom[d].ntiles), dim3(c->wg[d]), 0, c->stream,
+                       q, fops<T>(c)[d].op, c->geom[d], c->N, nv);
+  return check_launch();
+}
+
+int step_once(cruna_hip_ctx *c, int nt) {
+  const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
+  int rc = mixed ? rk4_step<float>(c) : rk4_step<double>(c);
+  if (rc) return rc;
+  for (int d = 0; d < 3; ++d) {
+    const int fq = c->cfg.filter_freq[d];
+    if (fq > 0 && nt % fq == 0) {
+      rc = mixed ? filter_dir<float>(c, c->qn, d, 5) : filter_dir<double>(c, c->qn, d, 5);
+      if (rc) return rc;
+    }
+  }
+  return CRUNA_HIP_OK;
+}
+
+int ensure_tmp(cruna_hip_ctx *c) {
+  if (!c->tmp) HIP_TRY(hipMalloc(&c->tmp, 5 * c->N * sizeof(double)));
+  return CRUNA_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void cruna_hip_default_cfg(cruna_hip_cfg *cfg) {
+  std::memset(cfg, 0, sizeof(*cfg));
+  for (int d = 0; d < 3; ++d) {
+    cfg->n[d] = 64;
+    cfg->dx[d] = 1.0;
+    cfg->scheme[d] = CRUNA_HIP_PERIODIC;
+    cfg->filter_freq[d] = 1;
+  }
+  cfg->filter_alpha = 0.35;
+  cfg->gamma = 1.4;
+  cfg->R = 8.3144598;
+  cfg->W = 0.0289647;
+  cfg->Pr = 0.71;
+  cfg->mu0 = 0.0;
+  cfg->T0 = 273.0;
+  cfg->visc_law = CRUNA_HIP_VISC_SUTHERLAND;
+  cfg->visc_exp = 0.76;
+  cfg->dt = 1e-3;
+  cfg->heat_unit_spacing = 1;
+  cfg->precision = CRUNA_HIP_FP64;
+}
+
+const char *cruna_hip_last_error(void) { return g_last_error.c_str(); }
+
+int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
+  if (!cfg || !out) return fail(CRUNA_HIP_EINVAL, "null argument");
+  *out = nullptr;
+  for (int d = 0; d < 3; ++d) {
+    if (cfg->n[d] < 16 || cfg->n[d] % PPT != 0)
+      return fail(CRUNA_HIP_EINVAL, "n" + std::to_string(d + 1) + " = " +
+                                        std::to_string(cfg->n[d]) +
+                                        ": each grid size must be a multiple of 8 and >= 16");
+    if (cfg->n[d] / PPT > NT)
+      return fail(CRUNA_HIP_EINVAL, "grid size above 2048 points per line is not supported");
+    if (cfg->scheme[d] != CRUNA_HIP_PERIODIC && cfg->scheme[d] != CRUNA_HIP_OPEN)
+      return fail(CRUNA_HIP_EINVAL, "unknown scheme");
+    if (!(cfg->dx[d] > 0)) return fail(CRUNA_HIP_EINVAL, "dx must be positive");
+  }
+  if (cfg->precision != CRUNA_HIP_FP64 && cfg->precision != CRUNA_HIP_FP32_MIXED)
+    return fail(CRUNA_HIP_EINVAL, "unknown precision");
+  if (cfg->nbc < 0 || cfg->nbc > CRUNA_HIP_MAX_BC) return fail(CRUNA_HIP_EINVAL, "bad nbc");
+  for (int r = 0; r < cfg->nbc; ++r) {
+    const int t = (int)lround(cfg->bc[r][0]);
+    if (t != 0)
+      return fail(CRUNA_HIP_EINVAL,
+                  "boundary type " + std::to_string(t) + " not implemented on the device");
+  }
+  std::unique_ptr<cruna_hip_ctx> c(new cruna_hip_ctx());
+  c->cfg = *cfg;
+  HIP_TRY(hipGetDevice(&c->device));
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->N = (long)cfg->n[0] * cfg->n[1] * cfg->n[2];
+  for (int d = 0; d < 3; ++d) {
+    build_geom(c.get(), d);
+    try {
+      HostLineOp dd = build_line_op(OP_DIFF, cfg->scheme[d] == CRUNA_HIP_PERIODIC, cfg->n[d], 0.0);
+      HostLineOp ff =
+          build_line_op(OP_FILTER, cfg->scheme[d] == CRUNA_HIP_PERIODIC, cfg->n[d], cfg->filter_alpha);
+      if (int rc = c->dop64[d].upload(dd)) return rc;
+      if (int rc = c->fop64[d].upload(ff)) return rc;
+      if (int rc = c->dop32[d].upload(dd)) return rc;
+      if (int rc = c->fop32[d].upload(ff)) return rc;
+    } catch (const std::exception &e) {
+      return fail(CRUNA_HIP_EINVAL, e.what());
+    }
+  }
+  Phys &ph = c->phys;
+  ph.gamma = cfg->gamma;
+  ph.gm1 = cfg->gamma - 1.0;
+  ph.gt = cfg->gamma / (cfg->gamma - 1.0);
+  ph.w_inv_r = cfg->W / cfg->R;
+  ph.mu0 = cfg->mu0;
+  ph.T0 = cfg->T0;
+  ph.visc_exp = cfg->visc_exp;
+  ph.visc_law = cfg->visc_law;
+  ph.cp = cfg->gamma * (cfg->R / cfg->W) / (cfg->gamma - 1.0);
+  ph.Pr = cfg->Pr;
+  for (int d = 0; d < 3; ++d) {
+    ph.inv_h[d] = 1.0 / cfg->dx[d];
+    ph.heat_inv_h[d] = cfg->heat_unit_spacing ? 1.0 : 1.0 / cfg->dx[d];
+  }
+  const size_t N = c->N;
+  const size_t ts = cfg->precision == CRUNA_HIP_FP32_MIXED ? sizeof(float) : sizeof(double);
+  if (hipMalloc(&c->qn, 5 * N * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->acc, 5 * N * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->qs, 5 * N * ts) != hipSuccess || hipMalloc(&c->rhs, 5 * N * ts) != hipSuccess ||
+      hipMalloc(&c->G, 9 * N * ts) != hipSuccess) {
+    (void)hipGetLastError();
+    cruna_hip_destroy(c.release());
+    return fail(CRUNA_HIP_ENOMEM, "device allocation failed");
+  }
+  *out = c.release();
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_destroy(cruna_hip_ctx *c) {
+  if (!c) return CRUNA_HIP_OK;
+  (void)hipStreamSynchronize(c->stream);
+  for (int d = 0; d < 3; ++d) {
+    c->dop64[d].release();
+    c->fop64[d].release();
+    c->dop32[d].release();
+    c->fop32[d].release();
+  }
+  for (void *p : {(void *)c->qn, (void *)c->acc, (void *)c->qref, (void *)c->shape, (void *)c->tmp,
+                  c->qs, c->rhs, c->G})
+    if (p) (void)hipFree(p);
+  for (auto &p : c->pending) {
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_set_state(cruna_hip_ctx *c, const double *q) {
+  if (!c || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  HIP_TRY(hipMemcpyAsync(c->qn, q, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_get_state(cruna_hip_ctx *c, double *q) {
+  if (!c || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  HIP_TRY(hipMemcpyAsync(q, c->qn, 5 * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_set_qref(cruna_hip_ctx *c, const double *qref) {
+  if (!c || !qref) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (!c->qref) HIP_TRY(hipMalloc(&c->qref, 5 * c->N * sizeof(double)));
+  HIP_TRY(hipMemcpy(c->qref, qref, 5 * c->N * sizeof(double), hipMemcpyHostToDevice));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_set_sponge_shape(cruna_hip_ctx *c, const double *shape) {
+  if (!c || !shape) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (!c->shape) HIP_TRY(hipMalloc(&c->shape, c->N * sizeof(double)));
+  HIP_TRY(hipMemcpy(c->shape, shape, c->N * sizeof(double), hipMemcpyHostToDevice));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_diff(cruna_hip_ctx *c, int dir, double *q_x, const double *q, double h) {
+  if (!c || !q_x || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (dir < 0 || dir > 2) return fail(CRUNA_HIP_EINVAL, "dir must be 0, 1 or 2");
+  if (!(h != 0.0)) return fail(CRUNA_HIP_EINVAL, "h must be nonzero");
+  if (int rc = ensure_tmp(c)) return rc;
+  double *in = c->tmp, *out = c->tmp + c->N;
+  HIP_TRY(hipMemcpyAsync(in, q, c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  {
+    ProfScope p(c, KID_DIFF);
+    if (c->cfg.scheme[dir] == CRUNA_HIP_PERIODIC)
+      hipLaunchKernelGGL((k_diff<double, true>), dim3(c->geom[dir].ntiles), dim3(c->wg[dir]), 0,
+                         c->stream, (const double *)in, out, c->dop64[dir].op, c->geom[dir], 1.0 / h);
+    else
+      hipLaunchKernelGGL((k_diff<double, false>), dim3(c->geom[dir].ntiles), dim3(c->wg[dir]), 0,
+                         c->stream, (const double *)in, out, c->dop64[dir].op, c->geom[dir], 1.0 / h);
+  }
+  if (int rc = check_launch()) return rc;
+  HIP_TRY(hipMemcpyAsync(q_x, out, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_filter(cruna_hip_ctx *c, int dir, double *q, int nv) {
+  if (!c || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (dir < 0 || dir > 2) return fail(CRUNA_HIP_EINVAL, "dir must be 0, 1 or 2");
+  if (nv < 1 || nv > 5) return fail(CRUNA_HIP_EINVAL, "nv must be 1..5");
+  if (int rc = ensure_tmp(c)) return rc;
+  HIP_TRY(hipMemcpyAsync(c->tmp, q, nv * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int rc = c->cfg.precision == CRUNA_HIP_FP32_MIXED ? filter_dir<float>(c, c->tmp, dir, nv)
+                                                    : filter_dir<double>(c, c->tmp, dir, nv);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(q, c->tmp, nv * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_rhs_direct(cruna_hip_ctx *c, double *rhs, const double *q, double t) {
+  (void)t;  // the direct RHS of the reference does not depend on t
+  if (!c || !rhs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (int rc = ensure_tmp(c)) return rc;
+  HIP_TRY(hipMemcpyAsync(c->tmp, q, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
+  int rc = mixed ? rhs_stage<float, double>(c, c->tmp, 4) : rhs_stage<double, double>(c, c->tmp, 4);
+  if (rc) return rc;
+  if (mixed) {
+    std::vector<float> h(5 * c->N);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->rhs, 5 * c->N * sizeof(float), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (long i = 0; i < 5 * c->N; ++i) rhs[i] = h[i];
+  } else {
+    HIP_TRY(hipMemcpyAsync(rhs, c->rhs, 5 * c->N * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const double *q,
+                          const double *g, double t) {
+  (void)rhs; (void)qs; (void)q; (void)g; (void)t;
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  return fail(CRUNA_HIP_EINVAL, "adjoint RHS is not implemented on the device yet");
+}
+
+int cruna_hip_time_stepper_direct(cruna_hip_ctx *c, double *q1, const double *q0, double t) {
+  (void)t;
+  if (!c || !q1 || !q0) return fail(CRUNA_HIP_EINVAL, "null argument");
+  HIP_TRY(hipMemcpyAsync(c->qn, q0, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
+  int rc = mixed ? rk4_step<float>(c) : rk4_step<double>(c);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(q1, c->qn, 5 * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *qs0,
+                                   const double *q0, const double *g, double t) {
+  (void)qs1; (void)qs0; (void)q0; (void)g; (void)t;
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  return fail(CRUNA_HIP_EINVAL, "adjoint stepper is not implemented on the device yet");
+}
+
+int cruna_hip_run_direct(cruna_hip_ctx *c, int nt0, int nsteps) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
+  for (int nt = nt0 + 1; nt <= nt0 + nsteps; ++nt)
+    if (int rc = step_once(c, nt)) return rc;
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_synchronize(cruna_hip_ctx *c) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->prof || !c->pending.empty()) return collect_profile(c);
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_profile_enable(cruna_hip_ctx *c, int on) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (int rc = collect_profile(c)) return rc;
+  c->prof = on != 0;
+  for (int k = 0; k < KID_COUNT; ++k) {
+    c->prof_ms[k] = 0.0;
+    c->prof_n[k] = 0;
+  }
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_profile_read(cruna_hip_ctx *c, int kid, double *ms, long *launches, char *name,
+                           size_t name_len) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (kid < 0 || kid >= KID_COUNT) return fail(CRUNA_HIP_EINVAL, "bad kernel id");
+  if (ms) *ms = c->prof_ms[kid];
+  if (launches) *launches = c->prof_n[kid];
+  if (name && name_len) {
+    std::strncpy(name, kKernelNames[kid], name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_profile_count(void) { return KID_COUNT; }
+
+}  // extern "C"

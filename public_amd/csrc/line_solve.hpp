@@ -1,0 +1,217 @@
+// Device side of the block-partitioned line solve (tables: line_ops.hpp).
+//
+// A work-group owns TL complete grid lines along one direction; each thread
+// owns one block of PPT = 8 consecutive points of one line (register
+// resident).  Cross-thread traffic goes through LDS only for the explicit
+// stencil halo (3 values per side) and the 2-unknown-per-block reduced
+// system, so a derivative costs ~4 LDS accesses per point instead of the
+// ~30 of a point-wise parallel cyclic reduction.  On periodic lines every
+// table entry is the same for all threads and is read with scalar loads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "line_ops.hpp"
+
+namespace cruna {
+
+constexpr int NT = 256;  // max threads per work-group
+
+template <typename T>
+struct LineOp {
+  const T *M;  // [3][PPT][FE]  block types: first, interior, last
+  const T *L;  // [3][PPT]
+  const T *R;  // [3][PPT]
+  const T *W;  // [K][nw][12]
+  int n, NB, K, per;
+};
+
+// Sweep geometry: which lines a work-group owns and where they live in HBM.
+struct SweepGeom {
+  int dir;        // 0: x (contiguous lines), 1: y, 2: z
+  int n;          // line length
+  int NB;         // blocks per line (n / PPT)
+  int TL;         // lines per work-group
+  int n1, n2, n3;
+  long PS;        // distance between consecutive line points in memory
+  int ntiles;     // work-groups in the grid
+  int ntiles_in;  // (y/z) tiles along i
+};
+
+// Thread placement inside a tile.  The LDS slot of a (line, block) pair is the
+// thread index; BS / LS are the slot strides between blocks / lines.
+struct ThreadPos {
+  int l, b;       // line in tile, block in line
+  int BS, LS;
+  long base;      // element offset of this thread's first point
+};
+
+// Work-groups are dealt round-robin to the 8 XCDs; map consecutive tiles to the
+// same XCD so that tiles sharing cache lines meet in the same L2.
+__device__ inline int xcd_tile(int bid, int ntiles) {
+  if (ntiles % 8 != 0) return bid;
+  return (bid % 8) * (ntiles / 8) + bid / 8;
+}
+
+__device__ inline ThreadPos thread_pos(const SweepGeom &g) {
+  ThreadPos t;
+  const int tid = threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, g.ntiles);
+  if (g.dir == 0) {
+    t.b = tid % g.NB;
+    t.l = tid / g.NB;
+    t.BS = 1;
+    t.LS = g.NB;
+    const long line = (long)tile * g.TL + t.l;
+    t.base = line * g.n1 + (long)t.b * PPT;
+  } else {
+    t.l = tid % g.TL;
+    t.b = tid / g.TL;
+    t.BS = g.TL;
+    t.LS = 1;
+    const int i = (tile % g.ntiles_in) * g.TL + t.l;
+    const long o = tile / g.ntiles_in;  // k for y-lines, j for z-lines
+    const long off = (g.dir == 1) ? (long)i + (long)g.n1 * g.n2 * o : (long)i + (long)g.n1 * o;
+    t.base = off + (long)t.b * PPT * g.PS;
+  }
+  return t;
+}
+
+// LDS scratch of one work-group for the line engine.
+template <typename T>
+struct LineLds {
+  T halo[NT * 2 * HALO];
+  T z[2][NT * 2];
+};
+
+template <typename T, typename P>
+__device__ __forceinline__ void matvec(P M, const T (&fe)[FE], T (&y)[PPT]) {
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    T acc = M[r * FE] * fe[0];
+#pragma unroll
+    for (int j = 1; j < FE; ++j) acc += M[r * FE + j] * fe[j];
+    y[r] = acc;
+  }
+}
+
+// x = B^-1 A f on the thread's block (unit spacing; the caller scales by 1/h).
+// All threads of the work-group must call it together (barriers inside).
+// Opaque copy of a uniform pointer: keeps the compiler from hoisting the table
+// reads of one solve out of the surrounding sweep (which would pin ~130 table
+// values in registers for the whole kernel); each solve re-reads them through
+// the scalar cache instead.
+// Tables are never written while a kernel runs, so uniform table reads may go
+// through the scalar data cache: a pointer into the constant address space
+// tells the compiler so.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+
+template <typename T>
+__device__ __forceinline__ cptr<T> launder_uniform(const T *p) {
+  asm volatile("" : "+s"(p));
+  return (cptr<T>)(p);
+}
+
+template <typename T, bool PER>
+__device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T> &s, const ThreadPos &tp,
+                                           const T (&f)[PPT], T (&x)[PPT]) {
+  const int tid = threadIdx.x;
+  const cptr<T> opM = launder_uniform(op.M);
+  const cptr<T> opL = launder_uniform(op.L);
+  const cptr<T> opR = launder_uniform(op.R);
+  const cptr<T> opW = launder_uniform(op.W);
+  const int NB = op.NB;
+  const int b = tp.b;
+  // 1. halo exchange
+#pragma unroll
+  for (int m = 0; m < HALO; ++m) {
+    s.halo[tid * 2 * HALO + m] = f[m];
+    s.halo[tid * 2 * HALO + HALO + m] = f[PPT - HALO + m];
+  }
+  __syncthreads();
+  const bool hasL = PER || b > 0;
+  const bool hasR = PER || b < NB - 1;
+  const int bl = b == 0 ? NB - 1 : b - 1;
+  const int br = b == NB - 1 ? 0 : b + 1;
+  const int sl = tp.l * tp.LS + bl * tp.BS;
+  const int sr = tp.l * tp.LS + br * tp.BS;
+  T fe[FE];
+#pragma unroll
+  for (int m = 0; m < HALO; ++m) {
+    fe[m] = hasL ? s.halo[sl * 2 * HALO + HALO + m] : T(0);
+    fe[HALO + PPT + m] = hasR ? s.halo[sr * 2 * HALO + m] : T(0);
+  }
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) fe[HALO + m] = f[m];
+  // 2. local operator y = M f (interior operator for every block; the two end
+  //    blocks of an open line redo it with their own operator)
+  T y[PPT];
+  matvec(opM + PPT * FE, fe, y);
+  int ty = 1;
+  if (!PER && (b == 0 || b == NB - 1)) {
+    ty = b == 0 ? 0 : 2;
+    const T *Mb = op.M + ty * PPT * FE;
+#pragma unroll 1
+    for (int r = 0; r < PPT; ++r) {
+      T acc = T(0);
+#pragma unroll
+      for (int j = 0; j < FE; ++j) acc += Mb[r * FE + j] * fe[j];
+#pragma unroll
+      for (int q = 0; q < PPT; ++q)
+        if (q == r) y[q] = acc;
+    }
+  }
+  // 3. reduced 2x2 block system by cyclic reduction over blocks
+  T z0 = y[0], z1 = y[PPT - 1];
+  int cur = 0;
+  s.z[0][tid * 2] = z0;
+  s.z[0][tid * 2 + 1] = z1;
+  for (int k = 0; k < op.K; ++k) {
+    __syncthreads();
+    const int st = 1 << k;
+    const bool hm = PER || b - st >= 0;
+    const bool hp = PER || b + st < NB;
+    int bm = b - st, bp = b + st;
+    if (PER) {
+      bm = ((bm % NB) + NB) % NB;
+      bp = bp % NB;
+    } else {
+      bm = hm ? bm : b;
+      bp = hp ? bp : b;
+    }
+    const int sm = tp.l * tp.LS + bm * tp.BS;
+    const int sp = tp.l * tp.LS + bp * tp.BS;
+    const T m0 = hm ? s.z[cur][sm * 2] : T(0), m1 = hm ? s.z[cur][sm * 2 + 1] : T(0);
+    const T p0 = hp ? s.z[cur][sp * 2] : T(0), p1 = hp ? s.z[cur][sp * 2 + 1] : T(0);
+    T w[12];
+    if (PER) {
+#pragma unroll
+      for (int i = 0; i < 12; ++i) w[i] = opW[k * 12 + i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 12; ++i) w[i] = op.W[(k * NB + b) * 12 + i];
+    }
+    const T n0 = w[0] * z0 + w[1] * z1 + w[4] * m0 + w[5] * m1 + w[8] * p0 + w[9] * p1;
+    const T n1 = w[2] * z0 + w[3] * z1 + w[6] * m0 + w[7] * m1 + w[10] * p0 + w[11] * p1;
+    z0 = n0;
+    z1 = n1;
+    cur ^= 1;
+    s.z[cur][tid * 2] = z0;
+    s.z[cur][tid * 2 + 1] = z1;
+  }
+  __syncthreads();
+  const T xl = hasL ? s.z[cur][sl * 2 + 1] : T(0);
+  const T xf = hasR ? s.z[cur][sr * 2] : T(0);
+  if (PER || ty == 1) {
+#pragma unroll
+    for (int r = 0; r < PPT; ++r) x[r] = y[r] + opL[PPT + r] * xl + opR[PPT + r] * xf;
+  } else {
+    const T *L = op.L + ty * PPT;
+    const T *R = op.R + ty * PPT;
+#pragma unroll
+    for (int r = 0; r < PPT; ++r) x[r] = y[r] + L[r] * xl + R[r] * xf;
+  }
+}
+
+}  // namespace cruna

@@ -1,0 +1,167 @@
+"""Parity of the HIP library (through the C ABI) against the CPU oracle.
+
+FP64 results must agree with the oracle to rounding: <= 1e-12 relative L2
+(the tolerance of the north-star parity target) for RHS, RK4 steps and
+multi-step runs; <= 1e-13 for single derivatives and filters.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = "tests/golden/cores.npz"
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def _gpu():
+    from public_amd import Cruna
+    return Cruna
+
+
+def _rand(shape, seed):
+    return np.asfortranarray(np.random.default_rng(seed).standard_normal(shape))
+
+
+SHAPES = [(32, 24, 16), (48, 16, 32), (24, 40, 16)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("scheme", [O.PERIODIC, O.OPEN])
+def test_dx_matches_oracle(shape, scheme):
+    Cruna = _gpu()
+    f = _rand(shape, 1)
+    dx = (0.11, 0.23, 0.37)
+    cfg = O.Cfg(shape, dx, scheme=(scheme,) * 3)
+    with Cruna(shape, dx, scheme=(scheme,) * 3) as g:
+        for d in range(3):
+            got = g.Dx(d, f, dx[d])
+            ref = O.diff3d(cfg, d, f, dx[d])
+            assert rel(got, ref) < 1e-13, (d, rel(got, ref))
+            # optional h defaults to 1 (diff_i1_mpi_cartesian.f90:93-97)
+            assert rel(g.Dx(d, f), O.diff3d(cfg, d, f, 1.0)) < 1e-13
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("scheme", [O.PERIODIC, O.OPEN])
+def test_filter_matches_oracle(shape, scheme):
+    Cruna = _gpu()
+    q = _rand(shape + (5,), 2)
+    cfg = O.Cfg(shape, (1, 1, 1), scheme=(scheme,) * 3, filter_alpha=0.35)
+    with Cruna(shape, (1, 1, 1), scheme=(scheme,) * 3, filter_alpha=0.35) as g:
+        for d in range(3):
+            got = g.filter(d, q)
+            ref = O.filter4d(cfg, d, q)
+            assert rel(got, ref) < 1e-13, (d, rel(got, ref))
+
+
+def test_dx_golden_vectors():
+    """Lines of the reference's own Fortran cores (tests/golden/cores.npz) placed along x."""
+    Cruna = _gpu()
+    g = np.load(GOLDEN)
+    for key in g.files:
+        if not key.endswith("_in") or not key.startswith("diff"):
+            continue
+        base = key[:-3]
+        scheme, n = int(base.split("_")[1]), int(base.split("_")[2])
+        if n < 16:
+            continue
+        lines = g[key]
+        h = float(g[base + "_arg"])
+        f = _rand((n, 16, 16), 3)
+        f[:, : lines.shape[1], 0] = lines
+        with Cruna((n, 16, 16), (h, 1, 1), scheme=(scheme, 0, 0)) as ctx:
+            got = ctx.Dx(0, f, h)[:, : lines.shape[1], 0]
+        ref = g[base + "_out"]
+        assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max(), base
+
+
+def _box(n, seed=0, mach=0.3):
+    from public_amd import ic
+    return ic.periodic_box_state(n, seed, mach)
+
+
+def _kw(mach, n, visc_law=O.SUTHERLAND):
+    from public_amd import ic
+    h = ic.periodic_dx(n)
+    q = _box(n, 0, mach)
+    return dict(mu0=1.0 / 1600.0, T0=1.0 / (1.4 * mach * mach), dt=ic.cfl_dt(q, h),
+                visc_law=visc_law, visc_exp=0.76), h
+
+
+@pytest.mark.parametrize("visc", [O.SUTHERLAND, O.POWER])
+def test_rhs_direct_matches_oracle(visc):
+    Cruna = _gpu()
+    n = 32
+    kw, h = _kw(0.3, n, visc)
+    q = _box(n, 5, 0.3)
+    ref = O.rhs_direct(O.Cfg((n,) * 3, (h,) * 3, **kw), q)
+    with Cruna((n,) * 3, (h,) * 3, **kw) as g:
+        got = g.right_hand_side_direct_3d(q)
+    for v in range(5):
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+
+
+def test_rhs_direct_anisotropic_grid():
+    Cruna = _gpu()
+    shape, dx = (48, 32, 24), (0.13, 0.2, 0.26)
+    rng = np.random.default_rng(9)
+    q = np.zeros(shape + (5,), order="F")
+    q[..., 0] = 1 + 0.05 * rng.random(shape)
+    q[..., 1:4] = 0.3 * rng.standard_normal(shape + (3,))
+    q[..., 4] = 5 + 0.1 * rng.random(shape)
+    kw = dict(mu0=0.01, T0=5.0, dt=1e-3)
+    ref = O.rhs_direct(O.Cfg(shape, dx, **kw), q)
+    with Cruna(shape, dx, **kw) as g:
+        got = g.right_hand_side_direct_3d(q)
+    assert rel(got, ref) < 1e-12
+
+
+def test_time_stepper_direct_matches_oracle():
+    Cruna = _gpu()
+    n = 32
+    kw, h = _kw(0.3, n)
+    q = _box(n, 1, 0.3)
+    ref = O.time_stepper_direct(O.Cfg((n,) * 3, (h,) * 3, **kw), q)
+    with Cruna((n,) * 3, (h,) * 3, **kw) as g:
+        got = g.time_stepper_direct(q)
+    assert rel(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("mach,nsteps", [(0.1, 10), (0.3, 10)])
+def test_run_direct_matches_oracle(mach, nsteps):
+    """Device-resident loop: RK4 -> filter i1, i2, i3 every step (periodic box)."""
+    Cruna = _gpu()
+    n = 32
+    kw, h = _kw(mach, n)
+    q = _box(n, 0, mach)
+    ref = O.run_direct(O.Cfg((n,) * 3, (h,) * 3, filter_freq=(1, 1, 1), **kw), q, nsteps)
+    with Cruna((n,) * 3, (h,) * 3, filter_freq=(1, 1, 1), **kw) as g:
+        g.set_state(q)
+        g.run_direct(nsteps)
+        g.synchronize()
+        got = g.get_state()
+    for v in range(5):
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+
+
+def test_fp32_mixed_close_to_fp64():
+    from public_amd import FP32_MIXED
+    Cruna = _gpu()
+    n = 32
+    kw, h = _kw(0.3, n)
+    q = _box(n, 0, 0.3)
+    out = []
+    for prec in (0, FP32_MIXED):
+        with Cruna((n,) * 3, (h,) * 3, precision=prec, **kw) as g:
+            g.set_state(q)
+            g.run_direct(10)
+            g.synchronize()
+            out.append(g.get_state())
+    err = rel(out[1], out[0])
+    print(f"fp32-mixed vs fp64 after 10 steps: {err:.3e}")
+    assert err < 1e-5
