@@ -1,0 +1,265 @@
+#!/usr/bin/env python3
+"""Headline benchmark: RK4 grid-point updates per second on one MI355X.
+
+Workload (the 256^3 periodic-box configuration): 3-D triply periodic
+compressible Navier-Stokes on [0, 2pi)^3, gamma 1.4, Pr 0.71, Re 1600
+(mu0 = 1/1600), power-law viscosity mu0 (T/T0)^0.76 with T0 the initial
+temperature, seeded solenoidal random velocity E(k) ~ k^4 exp(-2 (k/4)^2) at
+rms Mach 0.3, Lele 6th-order compact derivatives and the GV2000 compact
+filter (alpha 0.35) every step in x, y and z, RK4 at CFL 0.5.  The state
+lives in HBM for the whole timed region.
+
+One JSON line is printed.  `value` is the FP64 throughput in Mpts/s
+(N^3 * steps / s / 1e6); the FP32-mixed form (stencil/filter/RHS in FP32,
+FP64 RK4 accumulator) is reported beside it with its error against FP64.
+`roofline` describes the dominant kernel (largest share of the timed step):
+its algorithmic bytes per launch (compulsory reads and writes of the sweep)
+over its HIP-event-timed average duration, against 8 TB/s.  `cpu_baseline`
+times the C oracle on a bounded sample of the same workload (one RHS
+evaluation and the three filter passes of the 256^3 state, one host core).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic FP64-equivalent words per grid point and launch (compulsory
+# reads + writes of each sweep; FP32-mixed fields count as half words).
+SWEEP_WORDS = {
+    "sweep1_x": 13.0,
+    "sweep1_y": 19.0,
+    "sweep1z_sweep2z": 22.0,
+    "sweep2_x": 13.0,
+    "sweep2_y_rk": 30.5,  # mean over the four stages: 28, 33, 33, 28
+    "filter_x": 10.0,
+    "filter_y": 10.0,
+    "filter_z": 10.0,
+}
+# words of each sweep that are stage-state / RHS / gradient fields held in the
+# compute precision (the rest are FP64 state or accumulator words)
+SWEEP_COMPUTE_WORDS = {
+    "sweep1_x": 13.0, "sweep1_y": 19.0, "sweep1z_sweep2z": 22.0, "sweep2_x": 13.0,
+    "sweep2_y_rk": 15.0, "filter_x": 0.0, "filter_y": 0.0, "filter_z": 0.0,
+}
+# The step-level byte model of the reference's dependency graph (4624 B per
+# point per FP64 RK4 step incl. filtering; 2672 B in FP32-mixed).
+MODEL_BYTES = {"f64": 4624.0, "fp32-mixed": 2672.0}
+
+
+def kernel_bytes_per_point(name, mixed):
+    words = SWEEP_WORDS[name]
+    if not mixed:
+        return 8.0 * words
+    cw = SWEEP_COMPUTE_WORDS[name]
+    return 4.0 * cw + 8.0 * (words - cw)
+
+
+def run_gpu(q0, n, h, dt, precision, steps, warmup, kw):
+    from public_amd import Cruna
+    g = Cruna((n,) * 3, (h,) * 3, precision=precision, dt=dt, filter_freq=(1, 1, 1), **kw)
+    g.set_state(q0)
+    g.run_direct(warmup)
+    g.synchronize()
+    g.profile(True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run_direct(steps, nt0=warmup)
+    g.synchronize()
+    t1 = time.perf_counter()
+    prof = g.profile_read()
+    g.profile(False)
+    q = g.get_state()
+    g.close()
+    return (t1 - t0), prof, q
+
+
+def ke_history(q0, n, h, dt, precision, steps, every, kw):
+    from public_amd import Cruna, ic
+    g = Cruna((n,) * 3, (h,) * 3, precision=precision, dt=dt, filter_freq=(1, 1, 1), **kw)
+    g.set_state(q0)
+    hist = [ic.kinetic_energy(q0)]
+    done = 0
+    while done < steps:
+        k = min(every, steps - done)
+        g.run_direct(k, nt0=done)
+        done += k
+        hist.append(ic.kinetic_energy(g.get_state()))
+    g.close()
+    return np.array(hist)
+
+
+def rel_errors(q, ref, gamma):
+    from public_amd import ic
+    out = {}
+    names = ["rho", "rho_u", "rho_v", "rho_w", "p"]
+    for v, name in enumerate(names):
+        out[name] = float(np.linalg.norm(q[..., v] - ref[..., v]) / np.linalg.norm(ref[..., v]))
+    rE, rEr = ic.rho_E(q, gamma), ic.rho_E(ref, gamma)
+    out["rho_E"] = float(np.linalg.norm(rE - rEr) / np.linalg.norm(rEr))
+    return out
+
+
+def cpu_baseline(q0, n, h, dt, kw):
+    """Time the C oracle (one host core) on one RHS evaluation and the three
+    filter passes of the same state; an RK4 step is four RHS evaluations plus
+    the filters (the stage combinations are < 1% and are left out)."""
+    from oracle import oracle as O
+    cfg = O.Cfg((n,) * 3, (h,) * 3, dt=dt, filter_freq=(1, 1, 1), **kw)
+    t0 = time.perf_counter()
+    O.rhs_direct(cfg, q0)
+    t1 = time.perf_counter()
+    q = q0
+    for d in range(3):
+        q = O.filter4d(cfg, d, q)
+    t2 = time.perf_counter()
+    step_s = 4.0 * (t1 - t0) + (t2 - t1)
+    return {
+        "value": round(n ** 3 / step_s / 1e6, 4),
+        "unit": "Mpts/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"C oracle (oracle/cruna_oracle.c, gcc -O2) on the {n}^3 state: one "
+                   f"right_hand_side_direct_3d ({t1 - t0:.1f} s) and filter i1,i2,i3 "
+                   f"({t2 - t1:.1f} s); step time = 4 RHS + filters = {step_s:.1f} s"),
+    }
+
+
+def read_traffic(name, n, precision_tag):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{n}_{precision_tag}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == name:
+            try:
+                return float(parts[1])
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--mach", type=float, default=0.3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--ke-every", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run 3 device-to-device copies of 1 GiB (for PMC scaling)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the multi-GPU "
+              "(z-slab, partitioned tridiagonal) design is documented but not built", file=sys.stderr)
+        sys.exit(2)
+
+    from public_amd import FP32_MIXED, FP64, POWER, ic
+
+    n = args.n
+    gamma = 1.4
+    h = ic.periodic_dx(n)
+    q0 = ic.periodic_box_state(n, args.seed, args.mach, gamma)
+    dt = ic.cfl_dt(q0, h, gamma)
+    T0 = 1.0 / (gamma * args.mach ** 2)  # initial temperature with R = W = 1
+    kw = dict(gamma=gamma, R=1.0, W=1.0, Pr=0.71, mu0=1.0 / 1600.0, T0=T0, visc_law=POWER,
+              visc_exp=0.76, filter_alpha=0.35)
+
+    secs64, prof64, q64 = run_gpu(q0, n, h, dt, FP64, args.steps, args.warmup, kw)
+    secs32, prof32, q32 = run_gpu(q0, n, h, dt, FP32_MIXED, args.steps, args.warmup, kw)
+    pts = float(n) ** 3
+    mpts64 = pts * args.steps / secs64 / 1e6
+    mpts32 = pts * args.steps / secs32 / 1e6
+
+    # dominant kernel of the FP64 step
+    dom = max(prof64.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dcount) = dom
+    avg_s = dms / dcount / 1e3
+    bytes_launch = kernel_bytes_per_point(dname, False) * pts
+    achieved = bytes_launch / avg_s / 1e9
+    traffic = read_traffic(dname, n, "fp64")
+    roofline = {
+        "kernel": dname,
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": bytes_launch,
+        "avg_launch_ms": round(avg_s * 1e3, 4),
+    }
+    kernels = {k: {"avg_ms": round(v[0] / v[1], 4), "launches": v[1],
+                   "GBps": round(kernel_bytes_per_point(k, False) * pts / (v[0] / v[1] / 1e3) / 1e9, 1)}
+               for k, v in prof64.items()}
+    step_ms64 = secs64 / args.steps * 1e3
+    step_ms32 = secs32 / args.steps * 1e3
+    step_roofline = {
+        "model_bytes_per_point": MODEL_BYTES["f64"],
+        "frac": round(MODEL_BYTES["f64"] * pts / (step_ms64 / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "fp32_mixed_frac": round(MODEL_BYTES["fp32-mixed"] * pts / (step_ms32 / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+
+    errors = None
+    if not args.no_errors:
+        errors = rel_errors(q32, q64, gamma)
+        total = args.warmup + args.steps
+        ke64 = ke_history(q0, n, h, dt, FP64, total, args.ke_every, kw)
+        ke32 = ke_history(q0, n, h, dt, FP32_MIXED, total, args.ke_every, kw)
+        errors["kinetic_energy_history"] = float(np.linalg.norm(ke32 - ke64) / np.linalg.norm(ke64))
+        errors["after_steps"] = total
+
+    if args.copy_calibration:
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        nb = ctypes.c_size_t(1 << 30)
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(a), nb) == 0 and hip.hipMalloc(ctypes.byref(b), nb) == 0:
+            hip.hipMemset(a, 1, nb)
+            for _ in range(3):
+                hip.hipMemcpy(b, a, nb, 3)
+            hip.hipDeviceSynchronize()
+            hip.hipFree(a)
+            hip.hipFree(b)
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(q0, n, h, dt, kw)
+
+    out = {
+        "metric": "grid-pt RK4 updates/s (Mpts/s) FP64 vs FP32-mixed; % HBM roofline; err vs FP64",
+        "value": round(mpts64, 2),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms64, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded solenoidal random velocity field, no dataset)",
+        "config": {"workload": f"periodic box {n}^3 compressible NS, RK4 + compact filter, Ma {args.mach}, Re 1600",
+                   "grid": [n, n, n], "dt": dt, "seed": args.seed},
+        "fp32_mixed": {"value": round(mpts32, 2), "unit": "Mpts/s", "ms_per_step": round(step_ms32, 4),
+                       "speedup_vs_fp64": round(mpts32 / mpts64, 3), "err_vs_fp64": errors},
+        "roofline": roofline,
+        "step_roofline": step_roofline,
+        "kernels_fp64": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -168,6 +168,24 @@ int check_launch() {
   return CRUNA_HIP_OK;
 }
 
+// Launch KERNEL<..., DIR, PER> for a runtime direction d with the periodic/open
+// specialisation of that direction.
+#define DISPATCH_DIR(c, d, KER, TARGS, ...)                                                 \
+  do {                                                                                      \
+    const dim3 grid_((c)->geom[d].ntiles), block_((c)->wg[d]);                              \
+    const bool per_ = (c)->cfg.scheme[d] == CRUNA_HIP_PERIODIC;                            \
+    if (d == 0) {                                                                           \
+      if (per_) hipLaunchKernelGGL((KER<TARGS, 0, true>), grid_, block_, 0, (c)->stream, __VA_ARGS__); \
+      else hipLaunchKernelGGL((KER<TARGS, 0, false>), grid_, block_, 0, (c)->stream, __VA_ARGS__);     \
+    } else if (d == 1) {                                                                    \
+      if (per_) hipLaunchKernelGGL((KER<TARGS, 1, true>), grid_, block_, 0, (c)->stream, __VA_ARGS__); \
+      else hipLaunchKernelGGL((KER<TARGS, 1, false>), grid_, block_, 0, (c)->stream, __VA_ARGS__);     \
+    } else {                                                                                \
+      if (per_) hipLaunchKernelGGL((KER<TARGS, 2, true>), grid_, block_, 0, (c)->stream, __VA_ARGS__); \
+      else hipLaunchKernelGGL((KER<TARGS, 2, false>), grid_, block_, 0, (c)->stream, __VA_ARGS__);     \
+    }                                                                                       \
+  } while (0)
+
 template <typename T>
 DevOp<T> *dops(cruna_hip_ctx *c);
 template <>
@@ -254,12 +272,7 @@ int rk4_step(cruna_hip_ctx *c) {
 template <typename T>
 int filter_dir(cruna_hip_ctx *c, double *q, int d, int nv) {
   ProfScope p(c, KID_FX + d);
-  if (c->cfg.scheme[d] == CRUNA_HIP_PERIODIC)
-    hipLaunchKernelGGL((k_filter<T, true>), dim3(c->geom[d].ntiles), dim3(c->wg[d]), 0, c->stream,
-                       q, fops<T>(c)[d].op, c->geom[d], c->N, nv);
-  else
-    hipLaunchKernelGGL((k_filter<T, false>), dim3(c->geom[d].ntiles), dim3(c->wg[d]), 0, c->stream,
-                       q, fops<T>(c)[d].op, c->geom[d], c->N, nv);
+  DISPATCH_DIR(c, d, k_filter, T, q, fops<T>(c)[d].op, c->geom[d], c->N, nv);
   return check_launch();
 }
 
@@ -324,6 +337,8 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
       return fail(CRUNA_HIP_EINVAL, "unknown scheme");
     if (!(cfg->dx[d] > 0)) return fail(CRUNA_HIP_EINVAL, "dx must be positive");
   }
+  if ((double)cfg->n[0] * cfg->n[1] * cfg->n[2] * sizeof(double) >= 4294967296.0)
+    return fail(CRUNA_HIP_EINVAL, "one FP64 field must fit in 4 GiB (32-bit device offsets)");
   if (cfg->precision != CRUNA_HIP_FP64 && cfg->precision != CRUNA_HIP_FP32_MIXED)
     return fail(CRUNA_HIP_EINVAL, "unknown precision");
   if (cfg->nbc < 0 || cfg->nbc > CRUNA_HIP_MAX_BC) return fail(CRUNA_HIP_EINVAL, "bad nbc");
@@ -372,7 +387,7 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   if (hipMalloc(&c->qn, 5 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->acc, 5 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->qs, 5 * N * ts) != hipSuccess || hipMalloc(&c->rhs, 5 * N * ts) != hipSuccess ||
-      hipMalloc(&c->G, 9 * N * ts) != hipSuccess) {
+      hipMalloc(&c->G, 6 * N * ts) != hipSuccess) {
     (void)hipGetLastError();
     cruna_hip_destroy(c.release());
     return fail(CRUNA_HIP_ENOMEM, "device allocation failed");
@@ -440,12 +455,8 @@ int cruna_hip_diff(cruna_hip_ctx *c, int dir, double *q_x, const double *q, doub
   HIP_TRY(hipMemcpyAsync(in, q, c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
   {
     ProfScope p(c, KID_DIFF);
-    if (c->cfg.scheme[dir] == CRUNA_HIP_PERIODIC)
-      hipLaunchKernelGGL((k_diff<double, true>), dim3(c->geom[dir].ntiles), dim3(c->wg[dir]), 0,
-                         c->stream, (const double *)in, out, c->dop64[dir].op, c->geom[dir], 1.0 / h);
-    else
-      hipLaunchKernelGGL((k_diff<double, false>), dim3(c->geom[dir].ntiles), dim3(c->wg[dir]), 0,
-                         c->stream, (const double *)in, out, c->dop64[dir].op, c->geom[dir], 1.0 / h);
+    DISPATCH_DIR(c, dir, k_diff, double, (const double *)in, out, c->dop64[dir].op, c->geom[dir],
+                 1.0 / h);
   }
   if (int rc = check_launch()) return rc;
   HIP_TRY(hipMemcpyAsync(q_x, out, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -77,6 +77,15 @@ __device__ inline ThreadPos thread_pos(const SweepGeom &g) {
   return t;
 }
 
+// Work-group barrier that orders LDS traffic only.  __syncthreads() also waits
+// for every outstanding global load, which would drain the prefetches the
+// sweeps issue ahead of a line solve; this barrier leaves them in flight.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // LDS scratch of one work-group for the line engine.
 template <typename T>
 struct LineLds {
@@ -129,7 +138,7 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T> &s, c
     s.halo[tid * 2 * HALO + m] = f[m];
     s.halo[tid * 2 * HALO + HALO + m] = f[PPT - HALO + m];
   }
-  __syncthreads();
+  lds_barrier();
   const bool hasL = PER || b > 0;
   const bool hasR = PER || b < NB - 1;
   const int bl = b == 0 ? NB - 1 : b - 1;
@@ -168,7 +177,7 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T> &s, c
   s.z[0][tid * 2] = z0;
   s.z[0][tid * 2 + 1] = z1;
   for (int k = 0; k < op.K; ++k) {
-    __syncthreads();
+    lds_barrier();
     const int st = 1 << k;
     const bool hm = PER || b - st >= 0;
     const bool hp = PER || b + st < NB;
@@ -200,7 +209,7 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T> &s, c
     s.z[cur][tid * 2] = z0;
     s.z[cur][tid * 2 + 1] = z1;
   }
-  __syncthreads();
+  lds_barrier();
   const T xl = hasL ? s.z[cur][sl * 2 + 1] : T(0);
   const T xf = hasR ? s.z[cur][sr * 2] : T(0);
   if (PER || ty == 1) {

@@ -6,19 +6,21 @@
 // reference become 42 line solves on register-resident blocks:
 //   S1 x : u_x v_x w_x, d/dx of (rho u), (rho u u_i), p, (p u), T and lambda T_x
 //   S2 y : the same along y                                  (accumulates rhs)
-//   S3 z : the same along z, plus d/dz tau_{iz} and tau:grad(u)
+//   S3 z : the same along z, plus d/dz tau_{iz}
 //   S4 x : d/dx tau_{ix}
-//   S5 y : d/dy tau_{iy}, then the RK4 stage update (src/time/rk_o4_e.f90:45-50)
+//   S5 y : d/dy tau_{iy} and tau:grad(u), then the RK4 stage update
+//          (src/time/rk_o4_e.f90:45-50)
 // The pressure equation is accumulated as the bracket of ns_rho_rhou_p.f90:200-206
 // and multiplied by (gamma - 1) once, as the reference does.
 //
-// Each sweep is written as a sequence of phases that keep only the operands of
-// the next line solve live; inputs used by several phases are re-read (the
-// tile was just fetched, so those reads are served by L2).
 //
 // Field layout in HBM: every field is f(n1,n2,n3) Fortran order; a state holds
-// 5 fields at a stride of N = n1*n2*n3 elements; G holds the 9 velocity
-// gradients, G[3*i + d] = d u_i / d x_d.
+// 5 fields at a stride of N = n1*n2*n3 elements.  The viscous stress and the
+// dissipation use the velocity gradient only through its diagonal and the
+// symmetric sums, so G holds 6 fields:
+//   G0 = u_x, G1 = v_y, G2 = w_z, G3 = u_y + v_x, G4 = u_z + w_x, G5 = v_z + w_y
+// (each sum is completed by the second sweep that contributes to it; the
+// result equals the reference's u_y + v_x etc. bit for bit).
 #pragma once
 
 #include "line_solve.hpp"
@@ -59,23 +61,66 @@ __device__ __forceinline__ T lambda_of(const Phys &ph, T mu) {
   return mu * T(ph.cp) / T(ph.Pr);
 }
 
-template <typename T, typename Q>
-__device__ __forceinline__ void ld(const Q *__restrict__ p, long base, long PS, T (&v)[PPT]) {
-#pragma unroll
-  for (int m = 0; m < PPT; ++m) v[m] = T(p[base + m * PS]);
+// A thread's block is PPT consecutive points of its line.  Along x those are
+// contiguous and 64-byte aligned (FP64), so they move as 16-byte vectors;
+// along y/z they are PS elements apart.  Addresses are formed as a uniform
+// field base (SGPRs) plus a 32-bit byte offset, so every field of a sweep
+// shares the same PPT offset registers (the host checks that a field fits
+// in 4 GiB).
+template <typename Q>
+__device__ __forceinline__ const Q *at(const Q *p, unsigned byte_off) {
+  return reinterpret_cast<const Q *>(reinterpret_cast<const char *>(p) + byte_off);
+}
+template <typename Q>
+__device__ __forceinline__ Q *at(Q *p, unsigned byte_off) {
+  return reinterpret_cast<Q *>(reinterpret_cast<char *>(p) + byte_off);
 }
 
-template <typename T, typename Q>
-__device__ __forceinline__ void st(Q *__restrict__ p, long base, long PS, const T (&v)[PPT]) {
+template <int DIR, typename T, typename Q>
+__device__ __forceinline__ void ld(const Q *__restrict__ p, long base, long PS, T (&v)[PPT]) {
+  const unsigned b0 = unsigned(base), ps = unsigned(PS);
+  if (DIR == 0) {
+    typedef Q V __attribute__((ext_vector_type(16 / sizeof(Q))));
+    constexpr int W = 16 / sizeof(Q);
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) p[base + m * PS] = Q(v[m]);
+    for (int k = 0; k < PPT / W; ++k) {
+      const V t = *reinterpret_cast<const V *>(at(p, (b0 + k * W) * unsigned(sizeof(Q))));
+#pragma unroll
+      for (int j = 0; j < W; ++j) v[k * W + j] = T(t[j]);
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) v[m] = T(*at(p, (b0 + m * ps) * unsigned(sizeof(Q))));
+  }
+}
+
+template <int DIR, typename T, typename Q>
+__device__ __forceinline__ void st(Q *__restrict__ p, long base, long PS, const T (&v)[PPT]) {
+  const unsigned b0 = unsigned(base), ps = unsigned(PS);
+  if (DIR == 0) {
+    typedef Q V __attribute__((ext_vector_type(16 / sizeof(Q))));
+    constexpr int W = 16 / sizeof(Q);
+#pragma unroll
+    for (int k = 0; k < PPT / W; ++k) {
+      V t;
+#pragma unroll
+      for (int j = 0; j < W; ++j) t[j] = Q(v[k * W + j]);
+      *reinterpret_cast<V *>(at(p, (b0 + k * W) * unsigned(sizeof(Q)))) = t;
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) *at(p, (b0 + m * ps) * unsigned(sizeof(Q))) = Q(v[m]);
+  }
 }
 
 // ----------------------------------------------------------------------------
 // S1 / S2 / S3: first-derivative sweeps along DIR.
 //   qs  : stage input state (5 fields, type Q)
 //   rhs : 5 accumulator fields (S1 writes, S2/S3 read-modify-write)
-//   G   : velocity gradients; S3 additionally reads the x/y components
+//   G   : velocity-gradient fields (layout above)
+// The state is read once into registers; read-modify-write operands are
+// fetched just before the line solve whose result they meet, so their latency
+// hides behind the solve (the engine's barriers do not wait on global loads).
 template <typename T, typename Q, int DIR, bool PER>
 __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__restrict__ rhs,
                                                T *__restrict__ G, LineOp<T> op, SweepGeom g,
@@ -84,160 +129,152 @@ __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__re
   const ThreadPos tp = thread_pos(g);
   const long B = tp.base, PS = g.PS;
   const T ih = T(ph.inv_h[DIR]), hih = T(ph.heat_inv_h[DIR]);
-  const Q *rho_ = qs, *p_ = qs + 4 * N, *md_ = qs + (1 + DIR) * N;
-  T a[PPT], f[PPT], x[PPT], r[PPT];
+  // the state stays in registers as loaded; velocities are formed on use
+  T rho[PPT], m1[PPT], m2[PPT], m3[PPT], p[PPT];
+  ld<DIR>(qs, B, PS, rho);
+  ld<DIR>(qs + N, B, PS, m1);
+  ld<DIR>(qs + 2 * N, B, PS, m2);
+  ld<DIR>(qs + 3 * N, B, PS, m3);
+  ld<DIR>(qs + 4 * N, B, PS, p);
+  T(&mom)[PPT] = DIR == 0 ? m1 : (DIR == 1 ? m2 : m3);
+  auto vel = [&](int i, int m) -> T { return (i == 0 ? m1[m] : (i == 1 ? m2[m] : m3[m])) / rho[m]; };
+  T f[PPT], x[PPT], r[PPT];
 
-  // velocity gradients u_d, v_d, w_d (ns_rho_rhou_p.f90:102-112)
+  // velocity gradients u_d, v_d, w_d (ns_rho_rhou_p.f90:102-112) into the
+  // diagonal / symmetric-sum layout of G
+  {
+    T sa[PPT], sb[PPT];
+    if (DIR == 1) ld<DIR>(G + 3 * N, B, PS, sa);  // v_x
+    if (DIR == 2) {
+      ld<DIR>(G + 4 * N, B, PS, sa);  // w_x
+      ld<DIR>(G + 5 * N, B, PS, sb);  // w_y
+    }
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    ld(rho_, B, PS, a);
-    ld(qs + (1 + c) * N, B, PS, f);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = f[m] / a[m];
+    for (int m = 0; m < PPT; ++m) f[m] = vel(0, m);
     line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) x[m] = x[m] * ih;
-    st(G + (3 * c + DIR) * N, B, PS, x);
+    if (DIR == 0) st<DIR>(G + 0 * N, B, PS, x);  // u_x
+    if (DIR == 1) {
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) sa[m] = x[m] + sa[m];
+      st<DIR>(G + 3 * N, B, PS, sa);  // u_y + v_x
+    }
+    if (DIR == 2) {
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) sa[m] = x[m] + sa[m];
+      st<DIR>(G + 4 * N, B, PS, sa);  // u_z + w_x
+    }
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) f[m] = vel(1, m);
+    line_solve<T, PER>(op, lds, tp, f, x);
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) x[m] = x[m] * ih;
+    if (DIR == 0) st<DIR>(G + 3 * N, B, PS, x);  // v_x
+    if (DIR == 1) st<DIR>(G + 1 * N, B, PS, x);  // v_y
+    if (DIR == 2) {
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) sb[m] = x[m] + sb[m];
+      st<DIR>(G + 5 * N, B, PS, sb);  // v_z + w_y
+    }
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) f[m] = vel(2, m);
+    line_solve<T, PER>(op, lds, tp, f, x);
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) x[m] = x[m] * ih;
+    if (DIR == 0) st<DIR>(G + 4 * N, B, PS, x);  // w_x
+    if (DIR == 1) st<DIR>(G + 5 * N, B, PS, x);  // w_y
+    if (DIR == 2) st<DIR>(G + 2 * N, B, PS, x);  // w_z
+
+    if (DIR == 2) {
+      // viscous fluxes d/dz tau_{iz} (ns_rho_rhou_p.f90:158-177):
+      // tau_13 = mu (u_z + w_x), tau_23 = mu (v_z + w_y), tau_33 = -2/3 mu div u + 2 mu w_z
+      T mu[PPT], a[PPT], c[PPT];
+      ld<DIR>(G + 0 * N, B, PS, a);
+      ld<DIR>(G + 1 * N, B, PS, c);
+      ld<DIR>(rhs + 1 * N, B, PS, r);
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) {
+        mu[m] = viscosity(ph, temperature(ph, rho[m], p[m]));
+        f[m] = mu[m] * sa[m];
+        const T md = -2.0 / 3.0 * mu[m] * (a[m] + c[m] + x[m]);
+        a[m] = md + 2 * mu[m] * x[m];
+        c[m] = mu[m] * sb[m];
+      }
+      line_solve<T, PER>(op, lds, tp, f, x);
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) r[m] = r[m] + x[m] * ih;
+      st<DIR>(rhs + 1 * N, B, PS, r);
+      ld<DIR>(rhs + 2 * N, B, PS, r);
+      line_solve<T, PER>(op, lds, tp, c, x);
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) r[m] = r[m] + x[m] * ih;
+      st<DIR>(rhs + 2 * N, B, PS, r);
+      ld<DIR>(rhs + 3 * N, B, PS, r);
+      line_solve<T, PER>(op, lds, tp, a, x);
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) r[m] = r[m] + x[m] * ih;
+      st<DIR>(rhs + 3 * N, B, PS, r);
+    }
   }
 
   // mass: -d/dx_d (rho u_d) (:131-135)
-  ld(md_, B, PS, f);
-  line_solve<T, PER>(op, lds, tp, f, x);
-  if (DIR == 0) {
+  if (DIR != 0) ld<DIR>(rhs, B, PS, r);
+  line_solve<T, PER>(op, lds, tp, mom, x);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) r[m] = -(x[m] * ih);
-  } else {
-    ld(rhs, B, PS, r);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) r[m] = r[m] - x[m] * ih;
-  }
-  st(rhs, B, PS, r);
+  for (int m = 0; m < PPT; ++m) r[m] = DIR == 0 ? -(x[m] * ih) : r[m] - x[m] * ih;
+  st<DIR>(rhs, B, PS, r);
 
   // pressure gradient p_d
   T pd[PPT];
-  ld(p_, B, PS, f);
-  line_solve<T, PER>(op, lds, tp, f, pd);
+  line_solve<T, PER>(op, lds, tp, p, pd);
 #pragma unroll
   for (int m = 0; m < PPT; ++m) pd[m] = pd[m] * ih;
 
   // momentum: -d/dx_d (rho u_d u_i) - p_i (:138-172)
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    ld(rho_, B, PS, a);
-    ld(qs + (1 + i) * N, B, PS, f);
-    ld(md_, B, PS, x);
+    if (DIR != 0) ld<DIR>(rhs + (1 + i) * N, B, PS, r);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = x[m] * (f[m] / a[m]);
+    for (int m = 0; m < PPT; ++m) f[m] = mom[m] * vel(i, m);
     line_solve<T, PER>(op, lds, tp, f, x);
-    if (DIR == 0) {
 #pragma unroll
-      for (int m = 0; m < PPT; ++m) r[m] = -(x[m] * ih);
-    } else {
-      ld(rhs + (1 + i) * N, B, PS, r);
-#pragma unroll
-      for (int m = 0; m < PPT; ++m) r[m] = r[m] - x[m] * ih;
+    for (int m = 0; m < PPT; ++m) {
+      r[m] = DIR == 0 ? -(x[m] * ih) : r[m] - x[m] * ih;
+      if (i == DIR) r[m] = r[m] - pd[m];
     }
-    if (i == DIR) {
-#pragma unroll
-      for (int m = 0; m < PPT; ++m) r[m] = r[m] - pd[m];
-    }
-    st(rhs + (1 + i) * N, B, PS, r);
+    st<DIR>(rhs + (1 + i) * N, B, PS, r);
   }
 
   // pressure-equation bracket: gamma/(gamma-1) (-d(p u_d)) + u_d p_d (:180-187)
   T e[PPT];
-  ld(rho_, B, PS, a);
-  ld(md_, B, PS, x);
-  ld(p_, B, PS, f);
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) f[m] = f[m] * (x[m] / a[m]);
+  for (int m = 0; m < PPT; ++m) f[m] = p[m] * vel(DIR, m);
   line_solve<T, PER>(op, lds, tp, f, x);
-  ld(rho_, B, PS, a);
-  ld(md_, B, PS, f);
-#pragma unroll
-  for (int m = 0; m < PPT; ++m) e[m] = T(ph.gt) * (-(x[m] * ih)) + (f[m] / a[m]) * pd[m];
-
-  // heat flux d/dx_d (lambda d/dx_d T) (:189-199)
-  ld(rho_, B, PS, a);
-  ld(p_, B, PS, f);
-#pragma unroll
-  for (int m = 0; m < PPT; ++m) f[m] = temperature(ph, a[m], f[m]);
-  line_solve<T, PER>(op, lds, tp, f, x);
-  ld(rho_, B, PS, a);
-  ld(p_, B, PS, f);
 #pragma unroll
   for (int m = 0; m < PPT; ++m)
-    f[m] = lambda_of(ph, viscosity(ph, temperature(ph, a[m], f[m]))) * (x[m] * hih);
+    e[m] = T(ph.gt) * (-(x[m] * ih)) + vel(DIR, m) * pd[m];
+
+  // heat flux d/dx_d (lambda d/dx_d T) (:189-199)
+  if (DIR != 0) ld<DIR>(rhs + 4 * N, B, PS, r);
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) f[m] = temperature(ph, rho[m], p[m]);
   line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) e[m] = e[m] + x[m] * hih;
-
-  if (DIR == 2) {
-    // viscous fluxes d/dz tau_{iz} (:158-177) and tau:grad(u) (:202-205)
-    T mu[PPT];
-    ld(rho_, B, PS, a);
-    ld(p_, B, PS, f);
+  for (int m = 0; m < PPT; ++m)
+    f[m] = lambda_of(ph, viscosity(ph, temperature(ph, rho[m], p[m]))) * (x[m] * hih);
+  line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) mu[m] = viscosity(ph, temperature(ph, a[m], f[m]));
-    // tau_13 = mu (u_z + w_x)
-    ld(G + 2 * N, B, PS, a);
-    ld(G + 6 * N, B, PS, f);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = mu[m] * (a[m] + f[m]);
-    line_solve<T, PER>(op, lds, tp, f, x);
-    ld(rhs + 1 * N, B, PS, r);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) r[m] = r[m] + x[m] * ih;
-    st(rhs + 1 * N, B, PS, r);
-    // tau_23 = mu (v_z + w_y)
-    ld(G + 5 * N, B, PS, a);
-    ld(G + 7 * N, B, PS, f);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = mu[m] * (a[m] + f[m]);
-    line_solve<T, PER>(op, lds, tp, f, x);
-    ld(rhs + 2 * N, B, PS, r);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) r[m] = r[m] + x[m] * ih;
-    st(rhs + 2 * N, B, PS, r);
-    // tau_33 = -2/3 mu div(u) + 2 mu w_z
-    ld(G + 0 * N, B, PS, a);
-    ld(G + 4 * N, B, PS, f);
-    ld(G + 8 * N, B, PS, x);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-      const T md = -2.0 / 3.0 * mu[m] * (a[m] + f[m] + x[m]);
-      f[m] = md + 2 * mu[m] * x[m];
-    }
-    line_solve<T, PER>(op, lds, tp, f, x);
-    ld(rhs + 3 * N, B, PS, r);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) r[m] = r[m] + x[m] * ih;
-    st(rhs + 3 * N, B, PS, r);
-    // dissipation tau_ij du_i/dx_j, summed in the reference's order
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-      const long o = B + m * PS;
-      const T ux = G[o], uy = G[N + o], uz = G[2 * N + o];
-      const T vx = G[3 * N + o], vy = G[4 * N + o], vz = G[5 * N + o];
-      const T wx = G[6 * N + o], wy = G[7 * N + o], wz = G[8 * N + o];
-      const T md = -2.0 / 3.0 * mu[m] * (ux + vy + wz);
-      const T t11 = md + 2 * mu[m] * ux, t22 = md + 2 * mu[m] * vy, t33 = md + 2 * mu[m] * wz;
-      const T t12 = mu[m] * (uy + vx), t13 = mu[m] * (uz + wx), t23 = mu[m] * (vz + wy);
-      e[m] = e[m] + t11 * ux + t12 * uy + t13 * uz + t12 * vx + t22 * vy + t23 * vz + t13 * wx +
-             t23 * wy + t33 * wz;
-    }
+  for (int m = 0; m < PPT; ++m) {
+    e[m] = e[m] + x[m] * hih;
+    if (DIR != 0) e[m] = r[m] + e[m];
   }
-
-  if (DIR != 0) {
-    ld(rhs + 4 * N, B, PS, r);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) e[m] = r[m] + e[m];
-  }
-  st(rhs + 4 * N, B, PS, e);
+  st<DIR>(rhs + 4 * N, B, PS, e);
 }
 
 // ----------------------------------------------------------------------------
-// S4 (x) / S5 (y): viscous fluxes d/dx_d tau_{i d}; S5 then finishes the stage.
+// S4 (x) / S5 (y): viscous fluxes d/dx_d tau_{i d}; S5 also adds tau:grad(u)
+// (ns_rho_rhou_p.f90:202-205) and finishes the stage.
 // mode 0..3: RK4 stage update; mode 4: write the complete RHS into rhs.
 template <typename T, typename Q, int DIR, bool PER>
 __global__ __launch_bounds__(NT) void k_sweep2(const Q *__restrict__ qs, T *__restrict__ rhs,
@@ -249,57 +286,47 @@ __global__ __launch_bounds__(NT) void k_sweep2(const Q *__restrict__ qs, T *__re
   const ThreadPos tp = thread_pos(g);
   const long B = tp.base, PS = g.PS;
   const T ih = T(ph.inv_h[DIR]);
-  T a[PPT], b[PPT], c[PPT], f[PPT];
-
-  T mu[PPT];
-  ld(qs, B, PS, a);
-  ld(qs + 4 * N, B, PS, b);
+  T t1[PPT], t2[PPT], t3[PPT], diss[PPT];
+  {
+    T rho[PPT], p[PPT], ux[PPT], vy[PPT], wz[PPT], s12[PPT], s3[PPT];
+    ld<DIR>(qs, B, PS, rho);
+    ld<DIR>(qs + 4 * N, B, PS, p);
+    ld<DIR>(G + 0 * N, B, PS, ux);
+    ld<DIR>(G + 1 * N, B, PS, vy);
+    ld<DIR>(G + 2 * N, B, PS, wz);
+    ld<DIR>(G + 3 * N, B, PS, s12);
+    ld<DIR>(G + (DIR == 0 ? 4 : 5) * N, B, PS, s3);  // x: u_z + w_x, y: v_z + w_y
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) mu[m] = viscosity(ph, temperature(ph, a[m], b[m]));
-
+    for (int m = 0; m < PPT; ++m) {
+      const T mu = viscosity(ph, temperature(ph, rho[m], p[m]));
+      const T md = -2.0 / 3.0 * mu * (ux[m] + vy[m] + wz[m]);
+      const T t11 = md + 2 * mu * ux[m], t22 = md + 2 * mu * vy[m], t33 = md + 2 * mu * wz[m];
+      if (DIR == 0) {
+        t1[m] = t11;           // tau_11
+        t2[m] = mu * s12[m];   // tau_21
+        t3[m] = mu * s3[m];    // tau_31
+      } else {
+        t1[m] = mu * s12[m];   // tau_12
+        t2[m] = t22;           // tau_22
+        t3[m] = mu * s3[m];    // tau_32
+        // tau_ij du_i/dx_j = tau_11 u_x + tau_22 v_y + tau_33 w_z + mu (s12^2 + s13^2 + s23^2)
+        diss[m] = t11 * ux[m] + t22 * vy[m] + t33 * wz[m] + t1[m] * s12[m] + t3[m] * s3[m];
+      }
+    }
+    if (DIR == 1) {
+      T s13[PPT];
+      ld<DIR>(G + 4 * N, B, PS, s13);
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) {
+        const T mu = viscosity(ph, temperature(ph, rho[m], p[m]));
+        diss[m] = diss[m] + (mu * s13[m]) * s13[m];
+      }
+    }
+  }
   T x1[PPT], x2[PPT], x3[PPT];
-  // first component: tau_11 (x) or tau_12 (y)
-  if (DIR == 0) {
-    ld(G + 0 * N, B, PS, a);
-    ld(G + 4 * N, B, PS, b);
-    ld(G + 8 * N, B, PS, c);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-      const T md = -2.0 / 3.0 * mu[m] * (a[m] + b[m] + c[m]);
-      f[m] = md + 2 * mu[m] * a[m];
-    }
-  } else {
-    ld(G + 1 * N, B, PS, a);
-    ld(G + 3 * N, B, PS, b);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = mu[m] * (a[m] + b[m]);
-  }
-  line_solve<T, PER>(op, lds, tp, f, x1);
-  // second component: tau_21 (x) or tau_22 (y)
-  if (DIR == 0) {
-    ld(G + 1 * N, B, PS, a);
-    ld(G + 3 * N, B, PS, b);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = mu[m] * (a[m] + b[m]);
-  } else {
-    ld(G + 0 * N, B, PS, a);
-    ld(G + 4 * N, B, PS, b);
-    ld(G + 8 * N, B, PS, c);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-      const T md = -2.0 / 3.0 * mu[m] * (a[m] + b[m] + c[m]);
-      f[m] = md + 2 * mu[m] * b[m];
-    }
-  }
-  line_solve<T, PER>(op, lds, tp, f, x2);
-  // third component: tau_31 = mu (u_z + w_x) (x) or tau_32 = mu (v_z + w_y) (y)
-  ld(G + (DIR == 0 ? 2 : 5) * N, B, PS, a);
-  ld(G + (DIR == 0 ? 6 : 7) * N, B, PS, b);
-#pragma unroll
-  for (int m = 0; m < PPT; ++m) f[m] = mu[m] * (a[m] + b[m]);
-  line_solve<T, PER>(op, lds, tp, f, x3);
-
-  // momentum components with the viscous flux added
+  line_solve<T, PER>(op, lds, tp, t1, x1);
+  line_solve<T, PER>(op, lds, tp, t2, x2);
+  line_solve<T, PER>(op, lds, tp, t3, x3);
 #pragma unroll
   for (int m = 0; m < PPT; ++m) {
     x1[m] = x1[m] * ih;
@@ -307,18 +334,14 @@ __global__ __launch_bounds__(NT) void k_sweep2(const Q *__restrict__ qs, T *__re
     x3[m] = x3[m] * ih;
   }
   if (DIR == 0) {
-    ld(rhs + 1 * N, B, PS, a);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) a[m] = a[m] + x1[m];
-    st(rhs + 1 * N, B, PS, a);
-    ld(rhs + 2 * N, B, PS, a);
+    for (int i = 0; i < 3; ++i) {
+      T a[PPT];
+      ld<DIR>(rhs + (1 + i) * N, B, PS, a);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) a[m] = a[m] + x2[m];
-    st(rhs + 2 * N, B, PS, a);
-    ld(rhs + 3 * N, B, PS, a);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) a[m] = a[m] + x3[m];
-    st(rhs + 3 * N, B, PS, a);
+      for (int m = 0; m < PPT; ++m) a[m] = a[m] + (i == 0 ? x1[m] : (i == 1 ? x2[m] : x3[m]));
+      st<DIR>(rhs + (1 + i) * N, B, PS, a);
+    }
     return;
   }
   // complete RHS k_v and the RK4 update (rk_o4_e.f90:45-50): stage inputs
@@ -326,69 +349,72 @@ __global__ __launch_bounds__(NT) void k_sweep2(const Q *__restrict__ qs, T *__re
   const double c_stage = mode == 2 ? dt : dt * 0.5;
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
+    // one variable at a time keeps the tail's loads from being hoisted together
+    if (v > 0) asm volatile("" ::: "memory");
     T k[PPT];
-    ld(rhs + v * N, B, PS, k);
+    ld<DIR>(rhs + v * N, B, PS, k);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
       if (v == 1) k[m] = k[m] + x1[m];
       if (v == 2) k[m] = k[m] + x2[m];
       if (v == 3) k[m] = k[m] + x3[m];
-      if (v == 4) k[m] = T(ph.gm1) * k[m];
+      if (v == 4) k[m] = T(ph.gm1) * (k[m] + diss[m]);
     }
     if (mode == 4) {
-      if (v > 0) st(rhs + v * N, B, PS, k);
+      if (v > 0) st<DIR>(rhs + v * N, B, PS, k);
       continue;
     }
     double q0[PPT], s[PPT];
-    ld(qn + v * N, B, PS, q0);
+    ld<DIR>(qn + v * N, B, PS, q0);
     if (mode == 3) {
-      ld(acc + v * N, B, PS, s);
+      ld<DIR>(acc + v * N, B, PS, s);
 #pragma unroll
       for (int m = 0; m < PPT; ++m) q0[m] = q0[m] + dt / 6.0 * (s[m] + double(k[m]));
-      st(qn + v * N, B, PS, q0);
+      st<DIR>(qn + v * N, B, PS, q0);
     } else {
       if (mode == 0) {
 #pragma unroll
         for (int m = 0; m < PPT; ++m) s[m] = double(k[m]);
       } else {
-        ld(acc + v * N, B, PS, s);
+        ld<DIR>(acc + v * N, B, PS, s);
 #pragma unroll
         for (int m = 0; m < PPT; ++m) s[m] = s[m] + 2 * double(k[m]);
       }
-      st(acc + v * N, B, PS, s);
+      st<DIR>(acc + v * N, B, PS, s);
 #pragma unroll
       for (int m = 0; m < PPT; ++m) s[m] = q0[m] + c_stage * double(k[m]);
-      st(qs_out + v * N, B, PS, s);
+      st<DIR>(qs_out + v * N, B, PS, s);
     }
   }
 }
 
 // ----------------------------------------------------------------------------
 // Single derivative (Dx1/2/3) of one field: out = (B^-1 A in) / h.
-template <typename T, bool PER>
+template <typename T, int DIR, bool PER>
 __global__ __launch_bounds__(NT) void k_diff(const T *__restrict__ in, T *__restrict__ out,
                                              LineOp<T> op, SweepGeom g, double inv_h) {
   __shared__ LineLds<T> lds;
   const ThreadPos tp = thread_pos(g);
   T f[PPT], x[PPT];
-  ld(in, tp.base, g.PS, f);
+  ld<DIR>(in, tp.base, g.PS, f);
   line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
   for (int m = 0; m < PPT; ++m) x[m] = x[m] * T(inv_h);
-  st(out, tp.base, g.PS, x);
+  st<DIR>(out, tp.base, g.PS, x);
 }
 
 // Filter of nv fields in place (filter_4d): q = B_f^-1 A_f q per field.
-template <typename T, bool PER>
+template <typename T, int DIR, bool PER>
 __global__ __launch_bounds__(NT) void k_filter(double *__restrict__ q, LineOp<T> op, SweepGeom g,
                                                long N, int nv) {
   __shared__ LineLds<T> lds;
   const ThreadPos tp = thread_pos(g);
+  T f[PPT], x[PPT];
+  ld<DIR>(q, tp.base, g.PS, f);
   for (int v = 0; v < nv; ++v) {
-    T f[PPT], x[PPT];
-    ld(q + v * N, tp.base, g.PS, f);
     line_solve<T, PER>(op, lds, tp, f, x);
-    st(q + v * N, tp.base, g.PS, x);
+    if (v + 1 < nv) ld<DIR>(q + (v + 1) * N, tp.base, g.PS, f);  // next field in flight
+    st<DIR>(q + v * N, tp.base, g.PS, x);
   }
 }
 
