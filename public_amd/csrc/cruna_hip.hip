@@ -42,19 +42,22 @@ struct DevOp {
   T *buf = nullptr;
   LineOp<T> op{};
   int upload(const HostLineOp &h) {
-    const size_t nM = h.M.size(), nL = h.L.size(), nR = h.R.size(), nW = h.W.size();
-    std::vector<T> all(nM + nL + nR + nW);
+    const size_t nI = h.I.size(), nM = h.M.size(), nL = h.L.size(), nR = h.R.size(),
+                 nW = h.W.size();
+    std::vector<T> all(nI + nM + nL + nR + nW);
     size_t o = 0;
+    for (double v : h.I) all[o++] = T(v);
     for (double v : h.M) all[o++] = T(v);
     for (double v : h.L) all[o++] = T(v);
     for (double v : h.R) all[o++] = T(v);
     for (double v : h.W) all[o++] = T(v);
     HIP_TRY(hipMalloc(&buf, all.size() * sizeof(T)));
     HIP_TRY(hipMemcpy(buf, all.data(), all.size() * sizeof(T), hipMemcpyHostToDevice));
-    op.M = buf;
-    op.L = buf + nM;
-    op.R = buf + nM + nL;
-    op.W = buf + nM + nL + nR;
+    op.I = buf;
+    op.M = buf + nI;
+    op.L = buf + nI + nM;
+    op.R = buf + nI + nM + nL;
+    op.W = buf + nI + nM + nL + nR;
     op.n = h.n;
     op.NB = h.NB;
     op.K = h.K;

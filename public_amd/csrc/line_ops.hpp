@@ -42,8 +42,16 @@ enum OpKind { OP_DIFF = 0, OP_FILTER = 1 };
 // blocks, 2 = last block (all equal on a periodic line).  W is stored per
 // block for open lines ([K][NB][12]) and once per stage for periodic lines
 // ([K][1][12]): W0, Wm, Wp as row-major 2x2 matrices.
+// I holds the interior-block form of M (an 8x14 product) as the explicit
+// stencil plus a local Thomas solve, which needs 21 numbers instead of 112:
+//   I[0..3] = c0..c3, I[4] = s:  r_i = c0 f_i + sum_k c_k (f_{i+k} + s f_{i-k})
+//   I[5..11]  = l_1..l_7   forward elimination  y_i = r_i - l_i y_{i-1}
+//   I[12..19] = 1/d_0..1/d_7, I[20] = u:  x_i = (y_i - u x_{i+1}) / d_i
+constexpr int NI = 21;
+
 struct HostLineOp {
   int n = 0, NB = 0, K = 0, per = 1, nw = 1;
+  std::vector<double> I;  // [NI] interior blocks
   std::vector<double> M;  // [3][PPT][FE]
   std::vector<double> L;  // [3][PPT]
   std::vector<double> R;  // [3][PPT]
@@ -223,6 +231,7 @@ inline HostLineOp build_line_op(OpKind kind, bool per, int n, double alpha, doub
   std::vector<double> lo, di, up;
   implicit_rows(kind, per, n, alpha, lo, di, up);
 
+  op.I.assign(NI, 0.0);
   op.M.assign(3 * PPT * FE, 0.0);
   op.L.assign(3 * PPT, 0.0);
   op.R.assign(3 * PPT, 0.0);
@@ -288,6 +297,38 @@ inline HostLineOp build_line_op(OpKind kind, bool per, int n, double alpha, doub
     // reduced system  z_b + P_b z_{b-1} + Q_b z_{b+1} = Y_b,  z = (x_first, x_last)
     Pm[b].a[0] = 0.0; Pm[b].a[1] = -Lb[0]; Pm[b].a[2] = 0.0; Pm[b].a[3] = -Lb[PPT - 1];
     Qm[b].a[0] = -Rb[0]; Qm[b].a[1] = 0.0; Qm[b].a[2] = -Rb[PPT - 1]; Qm[b].a[3] = 0.0;
+  }
+  // interior-block stencil and Thomas factors (rows of a block away from the
+  // ends); an open line of two blocks has no interior block
+  if (per || NB > 2) {
+    const int ri = per ? 0 : PPT + HALO;  // a row of an interior block
+    double c[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (auto &jc : A[ri]) {
+      int off = jc.first - ri;
+      if (per) off = ((off + n / 2) % n + n) % n - n / 2;
+      if (off < -3 || off > 3) throw std::runtime_error("interior stencil wider than 3");
+      c[off + 3] += jc.second;
+    }
+    const double sgn = (c[4] == 0.0 && c[2] == 0.0) ? 1.0 : (c[2] == -c[4] ? -1.0 : 1.0);
+    for (int k = 1; k <= 3; ++k)
+      if (std::fabs(c[3 + k] - (sgn > 0 ? c[3 - k] : -c[3 - k])) > 1e-15 * (1 + std::fabs(c[3 + k])))
+        throw std::runtime_error("interior stencil is neither symmetric nor antisymmetric");
+    if (sgn < 0 && c[3] != 0.0) throw std::runtime_error("antisymmetric stencil with a centre term");
+    op.I[0] = c[3];
+    op.I[1] = c[4];
+    op.I[2] = c[5];
+    op.I[3] = c[6];
+    op.I[4] = sgn;
+    const double a = lo[ri], u = up[ri], dg = di[ri];
+    double d = dg;
+    op.I[12] = 1.0 / d;
+    for (int i = 1; i < PPT; ++i) {
+      const double l = a / d;
+      d = dg - l * u;
+      op.I[4 + i] = l;
+      op.I[12 + i] = 1.0 / d;
+    }
+    op.I[20] = u;
   }
   // parallel cyclic reduction over blocks
   int s = 1;

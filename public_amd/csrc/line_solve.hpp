@@ -19,7 +19,8 @@ constexpr int NT = 256;  // max threads per work-group
 
 template <typename T>
 struct LineOp {
-  const T *M;  // [3][PPT][FE]  block types: first, interior, last
+  const T *I;  // [NI] interior blocks: stencil + local Thomas factors
+  const T *M;  // [3][PPT][FE]  dense local operators: first, interior, last
   const T *L;  // [3][PPT]
   const T *R;  // [3][PPT]
   const T *W;  // [K][nw][12]
@@ -93,17 +94,6 @@ struct LineLds {
   T z[2][NT * 2];
 };
 
-template <typename T, typename P>
-__device__ __forceinline__ void matvec(P M, const T (&fe)[FE], T (&y)[PPT]) {
-#pragma unroll
-  for (int r = 0; r < PPT; ++r) {
-    T acc = M[r * FE] * fe[0];
-#pragma unroll
-    for (int j = 1; j < FE; ++j) acc += M[r * FE + j] * fe[j];
-    y[r] = acc;
-  }
-}
-
 // x = B^-1 A f on the thread's block (unit spacing; the caller scales by 1/h).
 // All threads of the work-group must call it together (barriers inside).
 // Opaque copy of a uniform pointer: keeps the compiler from hoisting the table
@@ -126,10 +116,6 @@ template <typename T, bool PER>
 __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T> &s, const ThreadPos &tp,
                                            const T (&f)[PPT], T (&x)[PPT]) {
   const int tid = threadIdx.x;
-  const cptr<T> opM = launder_uniform(op.M);
-  const cptr<T> opL = launder_uniform(op.L);
-  const cptr<T> opR = launder_uniform(op.R);
-  const cptr<T> opW = launder_uniform(op.W);
   const int NB = op.NB;
   const int b = tp.b;
   // 1. halo exchange
@@ -153,11 +139,28 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T> &s, c
   }
 #pragma unroll
   for (int m = 0; m < PPT; ++m) fe[HALO + m] = f[m];
-  // 2. local operator y = M f (interior operator for every block; the two end
-  //    blocks of an open line redo it with their own operator)
+  // 2. local operator y = T_b^-1 A_b f: explicit stencil and an 8-point
+  //    Thomas solve for interior blocks (every block of a periodic line); the
+  //    two end blocks of an open line use their dense operator instead.
   T y[PPT];
-  matvec(opM + PPT * FE, fe, y);
   int ty = 1;
+  {
+    const cptr<T> I = launder_uniform(op.I);
+    const T c0 = I[0], c1 = I[1], c2 = I[2], c3 = I[3], sg = I[4];
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+      const T d1 = fe[HALO + i + 1] + sg * fe[HALO + i - 1];
+      const T d2 = fe[HALO + i + 2] + sg * fe[HALO + i - 2];
+      const T d3 = fe[HALO + i + 3] + sg * fe[HALO + i - 3];
+      y[i] = c0 * fe[HALO + i] + c1 * d1 + c2 * d2 + c3 * d3;
+    }
+#pragma unroll
+    for (int i = 1; i < PPT; ++i) y[i] = y[i] - I[4 + i] * y[i - 1];
+    const T up = I[20];
+    y[PPT - 1] = y[PPT - 1] * I[12 + PPT - 1];
+#pragma unroll
+    for (int i = PPT - 2; i >= 0; --i) y[i] = (y[i] - up * y[i + 1]) * I[12 + i];
+  }
   if (!PER && (b == 0 || b == NB - 1)) {
     ty = b == 0 ? 0 : 2;
     const T *Mb = op.M + ty * PPT * FE;
@@ -183,8 +186,11 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T> &s, c
     const bool hp = PER || b + st < NB;
     int bm = b - st, bp = b + st;
     if (PER) {
-      bm = ((bm % NB) + NB) % NB;
-      bp = bp % NB;
+      const int sr = st % NB;  // wave-uniform
+      bm = b - sr;
+      bm = bm < 0 ? bm + NB : bm;
+      bp = b + sr;
+      bp = bp >= NB ? bp - NB : bp;
     } else {
       bm = hm ? bm : b;
       bp = hp ? bp : b;
@@ -195,6 +201,7 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T> &s, c
     const T p0 = hp ? s.z[cur][sp * 2] : T(0), p1 = hp ? s.z[cur][sp * 2 + 1] : T(0);
     T w[12];
     if (PER) {
+      const cptr<T> opW = launder_uniform(op.W);
 #pragma unroll
       for (int i = 0; i < 12; ++i) w[i] = opW[k * 12 + i];
     } else {
@@ -213,6 +220,8 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T> &s, c
   const T xl = hasL ? s.z[cur][sl * 2 + 1] : T(0);
   const T xf = hasR ? s.z[cur][sr * 2] : T(0);
   if (PER || ty == 1) {
+    const cptr<T> opL = launder_uniform(op.L);
+    const cptr<T> opR = launder_uniform(op.R);
 #pragma unroll
     for (int r = 0; r < PPT; ++r) x[r] = y[r] + opL[PPT + r] * xl + opR[PPT + r] * xf;
   } else {
