@@ -35,7 +35,8 @@ HBM_PEAK_GBS = 8000.0
 SWEEP_WORDS = {
     "sweep1_x": 13.0,
     "sweep1_y": 19.0,
-    "sweep1z_sweep2z": 22.0,
+    "sweep1_z": 20.0,
+    "sweep2_z": 13.0,
     "sweep2_x": 13.0,
     "sweep2_y_rk": 30.5,  # mean over the four stages: 28, 33, 33, 28
     "filter_x": 10.0,
@@ -45,7 +46,7 @@ SWEEP_WORDS = {
 # words of each sweep that are stage-state / RHS / gradient fields held in the
 # compute precision (the rest are FP64 state or accumulator words)
 SWEEP_COMPUTE_WORDS = {
-    "sweep1_x": 13.0, "sweep1_y": 19.0, "sweep1z_sweep2z": 22.0, "sweep2_x": 13.0,
+    "sweep1_x": 13.0, "sweep1_y": 19.0, "sweep1_z": 20.0, "sweep2_z": 13.0, "sweep2_x": 13.0,
     "sweep2_y_rk": 15.0, "filter_x": 0.0, "filter_y": 0.0, "filter_z": 0.0,
 }
 # The step-level byte model of the reference's dependency graph (4624 B per

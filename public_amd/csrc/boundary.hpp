@@ -1,0 +1,116 @@
+// Boundary conditions of src/boundaries/boundary_conditions.f90 used by the
+// hot path: type 201 (characteristic, nonreflecting toward qref), 400 (sponge
+// on the RHS) and 401 (sponge on q).  Point-wise kernels, one thread per
+// point; the characteristic decomposition always runs in FP64, also in the
+// FP32-mixed mode.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cruna {
+
+struct BoxRow {
+  int i0, i1, j0, j1, k0, k1;  // 0-based inclusive
+  double k[3];                 // outward normal (kvector), as given in the row
+};
+
+// characteristic_q_k1d (src/boundaries/nonreflecting.f90:46-137) on the
+// points of one box: q <- q + (incoming part of prim(aim) - prim(q)).
+template <typename Q>
+__global__ __launch_bounds__(256) void k_bc_characteristic_q(Q *__restrict__ q,
+                                                              const double *__restrict__ aim,
+                                                              BoxRow row, int n1, int n2, long N,
+                                                              double gamma) {
+  const long nx = row.i1 - row.i0 + 1, ny = row.j1 - row.j0 + 1, nz = row.k1 - row.k0 + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nx * ny * nz) return;
+  const long i = row.i0 + t % nx, j = row.j0 + (t / nx) % ny, k = row.k0 + t / (nx * ny);
+  const long o = i + (long)n1 * (j + (long)n2 * k);
+  double r = double(q[o]);
+  double u1 = double(q[N + o]) / double(q[o]);
+  double u2 = double(q[2 * N + o]) / double(q[o]);
+  double u3 = double(q[3 * N + o]) / double(q[o]);
+  double p = double(q[4 * N + o]);
+  const double c = sqrt(gamma * double(q[4 * N + o]) / double(q[o]));
+  double dq1 = aim[o] - r;
+  double dq2 = aim[N + o] / aim[o] - u1;
+  double dq3 = aim[2 * N + o] / aim[o] - u2;
+  double dq4 = aim[3 * N + o] / aim[o] - u3;
+  double dq5 = aim[4 * N + o] - p;
+  const double kx = -row.k[0], ky = -row.k[1], kz = -row.k[2];
+  // primitive -> characteristic waves (:97-101)
+  double R1 = kx * dq1 + kz * dq3 - ky * dq4 - kx * dq5 / (c * c);
+  double R2 = ky * dq1 - kz * dq2 + kx * dq4 - ky * dq5 / (c * c);
+  double R3 = kz * dq1 + ky * dq2 - kx * dq3 - kz * dq5 / (c * c);
+  double R4 = kx * dq2 + ky * dq3 + kz * dq4 + dq5 / (r * c);
+  double R5 = -kx * dq2 - ky * dq3 - kz * dq4 + dq5 / (r * c);
+  // drop outgoing waves (:104-114)
+  const double lamS = kx * u1 + ky * u2 + kz * u3;
+  if (lamS < 0.0) R1 = R2 = R3 = 0.0;
+  if (lamS + c < 0.0) R4 = 0.0;
+  if (lamS - c < 0.0) R5 = 0.0;
+  // back to primitive increments (:118-122) and conservative state (:125-135)
+  dq1 = kx * R1 + ky * R2 + kz * R3 + r / (2.0 * c) * R4 + r / (2.0 * c) * R5;
+  dq2 = -kz * R2 + ky * R3 + 0.5 * kx * R4 - 0.5 * kx * R5;
+  dq3 = kz * R1 - kx * R3 + 0.5 * ky * R4 - 0.5 * ky * R5;
+  dq4 = -ky * R1 + kx * R2 + 0.5 * kz * R4 - 0.5 * kz * R5;
+  dq5 = +0.5 * r * c * R4 + 0.5 * r * c * R5;
+  r = r + dq1;
+  u1 = u1 + dq2;
+  u2 = u2 + dq3;
+  u3 = u3 + dq4;
+  p = p + dq5;
+  q[o] = Q(r);
+  q[N + o] = Q(r * u1);
+  q[2 * N + o] = Q(r * u2);
+  q[3 * N + o] = Q(r * u3);
+  q[4 * N + o] = Q(p);
+}
+
+// sponge_q with qref (src/boundaries/sponge_layer_core.f90:70-103):
+// q <- q - sigma (q - qref) for all five variables where sigma != 0.
+template <typename Q>
+__global__ __launch_bounds__(256) void k_bc_sponge_q(Q *__restrict__ q,
+                                                      const double *__restrict__ qref,
+                                                      const double *__restrict__ sigma, long N) {
+  const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= N) return;
+  const double s = sigma[o];
+  if (s == 0.0) return;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double x = double(q[v * N + o]);
+    q[v * N + o] = Q(x - s * (x - qref[v * N + o]));
+  }
+}
+
+// sponge_rhs with qref (sponge_layer_core.f90:34-66): rhs -= sigma (q - qref).
+// The pressure slot of the accumulator holds the bracket that the last sweep
+// multiplies by (gamma - 1), so its share enters divided by (gamma - 1).
+template <typename T, typename Q>
+__global__ __launch_bounds__(256) void k_sponge_rhs(T *__restrict__ rhs, const Q *__restrict__ q,
+                                                     const double *__restrict__ qref,
+                                                     const double *__restrict__ sigma, long N,
+                                                     double gm1) {
+  const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= N) return;
+  const double s = sigma[o];
+  if (s == 0.0) return;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    double d = s * (double(q[v * N + o]) - qref[v * N + o]);
+    if (v == 4) d = d / gm1;
+    rhs[v * N + o] = T(double(rhs[v * N + o]) - d);
+  }
+}
+
+// Stage input with boundary conditions: a private copy of the state
+// (q = q_in, ns_rho_rhou_p.f90:89) in the compute precision.
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_state(T *__restrict__ dst, const double *__restrict__ src,
+                                                     long n) {
+  const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o < n) dst[o] = T(src[o]);
+}
+
+}  // namespace cruna

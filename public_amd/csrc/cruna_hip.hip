@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -10,6 +11,7 @@
 #include <vector>
 
 #include "line_ops.hpp"
+#include "boundary.hpp"
 #include "sweeps.hpp"
 
 using namespace cruna;
@@ -31,11 +33,12 @@ int fail(int code, const std::string &msg) {
   } while (0)
 
 enum KernelId {
-  KID_S1X = 0, KID_S2Y, KID_S3Z, KID_S4X, KID_S5Y, KID_FX, KID_FY, KID_FZ, KID_DIFF, KID_COUNT
+  KID_S1X = 0, KID_S2Y, KID_S3Z, KID_S4Z, KID_S5X, KID_S6Y, KID_FX, KID_FY, KID_FZ, KID_DIFF,
+  KID_COUNT
 };
-const char *kKernelNames[KID_COUNT] = {"sweep1_x", "sweep1_y", "sweep1z_sweep2z", "sweep2_x",
-                                       "sweep2_y_rk", "filter_x", "filter_y", "filter_z",
-                                       "diff"};
+const char *kKernelNames[KID_COUNT] = {"sweep1_x", "sweep1_y", "sweep1_z", "sweep2_z",
+                                       "sweep2_x", "sweep2_y_rk", "filter_x", "filter_y",
+                                       "filter_z", "diff"};
 
 template <typename T>
 struct DevOp {
@@ -84,6 +87,9 @@ struct cruna_hip_ctx {
   Phys phys{};
   // device state
   double *qn = nullptr, *acc = nullptr, *qref = nullptr, *shape = nullptr, *tmp = nullptr;
+  // boundary_conditions.dat rows in file order (types 201, 400, 401)
+  std::vector<std::pair<int, BoxRow>> bc;
+  bool has_bcq = false, has_bcrhs = false;
   void *qs = nullptr, *rhs = nullptr, *G = nullptr;  // compute precision
   // profiling
   bool prof = false;
@@ -202,6 +208,49 @@ DevOp<double> *fops<double>(cruna_hip_ctx *c) { return c->fop64; }
 template <>
 DevOp<float> *fops<float>(cruna_hip_ctx *c) { return c->fop32; }
 
+int need_bc_inputs(cruna_hip_ctx *c) {
+  for (auto &r : c->bc) {
+    if (!c->qref) return fail(CRUNA_HIP_EINVAL, "boundary rows need qref (cruna_hip_set_qref)");
+    if ((r.first == 400 || r.first == 401) && !c->shape)
+      return fail(CRUNA_HIP_EINVAL, "sponge rows need a shape (cruna_hip_set_sponge_shape)");
+  }
+  return CRUNA_HIP_OK;
+}
+
+// set_boundary_condition_q (boundary_conditions.f90:52-136) on q, rows in order.
+template <typename Q>
+int bc_q(cruna_hip_ctx *c, Q *q) {
+  if (int rc = need_bc_inputs(c)) return rc;
+  for (auto &r : c->bc) {
+    if (r.first == 201) {
+      const BoxRow &b = r.second;
+      const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
+      hipLaunchKernelGGL((k_bc_characteristic_q<Q>), dim3((np + 255) / 256), dim3(256), 0,
+                         c->stream, q, (const double *)c->qref, b, c->cfg.n[0], c->cfg.n[1], c->N,
+                         c->cfg.gamma);
+    } else if (r.first == 401) {
+      hipLaunchKernelGGL((k_bc_sponge_q<Q>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream, q,
+                         (const double *)c->qref, (const double *)c->shape, c->N);
+    }
+    if (int rc = check_launch()) return rc;
+  }
+  return CRUNA_HIP_OK;
+}
+
+// set_boundary_condition_rhs (boundary_conditions.f90:228-298): type 400.
+template <typename T, typename Q>
+int bc_rhs(cruna_hip_ctx *c, T *rhs, const Q *q) {
+  if (int rc = need_bc_inputs(c)) return rc;
+  for (auto &r : c->bc)
+    if (r.first == 400) {
+      hipLaunchKernelGGL((k_sponge_rhs<T, Q>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream,
+                         rhs, q, (const double *)c->qref, (const double *)c->shape, c->N,
+                         c->cfg.gamma - 1.0);
+      if (int rc = check_launch()) return rc;
+    }
+  return CRUNA_HIP_OK;
+}
+
 // One RHS evaluation (mode 4) or RK stage (mode 0..3) from stage input `in`.
 template <typename T, typename Q>
 int rhs_stage(cruna_hip_ctx *c, const Q *in, int mode) {
@@ -239,7 +288,19 @@ int rhs_stage(cruna_hip_ctx *c, const Q *in, int mode) {
   }
   if (int rc = check_launch()) return rc;
   {
-    ProfScope p(c, KID_S4X);
+    ProfScope p(c, KID_S4Z);
+    if (c->cfg.scheme[2] == CRUNA_HIP_PERIODIC)
+      hipLaunchKernelGGL((k_sweep2<T, Q, 2, true>), dim3(c->geom[2].ntiles), dim3(c->wg[2]), 0,
+                         c->stream, in, rhs, (const T *)G, D[2].op, c->geom[2], c->phys, N, mode,
+                         c->qn, c->acc, qs, c->cfg.dt);
+    else
+      hipLaunchKernelGGL((k_sweep2<T, Q, 2, false>), dim3(c->geom[2].ntiles), dim3(c->wg[2]), 0,
+                         c->stream, in, rhs, (const T *)G, D[2].op, c->geom[2], c->phys, N, mode,
+                         c->qn, c->acc, qs, c->cfg.dt);
+  }
+  if (int rc = check_launch()) return rc;
+  {
+    ProfScope p(c, KID_S5X);
     if (c->cfg.scheme[0] == CRUNA_HIP_PERIODIC)
       hipLaunchKernelGGL((k_sweep2<T, Q, 0, true>), dim3(c->geom[0].ntiles), dim3(c->wg[0]), 0,
                          c->stream, in, rhs, (const T *)G, D[0].op, c->geom[0], c->phys, N, mode,
@@ -250,8 +311,10 @@ int rhs_stage(cruna_hip_ctx *c, const Q *in, int mode) {
                          c->qn, c->acc, qs, c->cfg.dt);
   }
   if (int rc = check_launch()) return rc;
+  if (c->has_bcrhs)
+    if (int rc = bc_rhs<T, Q>(c, rhs, in)) return rc;
   {
-    ProfScope p(c, KID_S5Y);
+    ProfScope p(c, KID_S6Y);
     if (c->cfg.scheme[1] == CRUNA_HIP_PERIODIC)
       hipLaunchKernelGGL((k_sweep2<T, Q, 1, true>), dim3(c->geom[1].ntiles), dim3(c->wg[1]), 0,
                          c->stream, in, rhs, (const T *)G, D[1].op, c->geom[1], c->phys, N, mode,
@@ -264,11 +327,28 @@ int rhs_stage(cruna_hip_ctx *c, const Q *in, int mode) {
   return check_launch();
 }
 
+// A stage from the FP64 state `from` (stage 0, or a single RHS evaluation) or,
+// with from == nullptr, from the stage input already in qs.  With q-type
+// boundary rows the RHS works on a private copy (ns_rho_rhou_p.f90:89-92).
+template <typename T>
+int run_stage(cruna_hip_ctx *c, const double *from, int mode) {
+  T *qs = (T *)c->qs;
+  if (from && !c->has_bcq) return rhs_stage<T, double>(c, from, mode);
+  if (from) {
+    const long n = 5 * c->N;
+    hipLaunchKernelGGL((k_copy_state<T>), dim3((n + 255) / 256), dim3(256), 0, c->stream, qs, from, n);
+    if (int rc = check_launch()) return rc;
+  }
+  if (c->has_bcq)
+    if (int rc = bc_q<T>(c, qs)) return rc;
+  return rhs_stage<T, T>(c, (const T *)qs, mode);
+}
+
 template <typename T>
 int rk4_step(cruna_hip_ctx *c) {
-  if (int rc = rhs_stage<T, double>(c, c->qn, 0)) return rc;
+  if (int rc = run_stage<T>(c, c->qn, 0)) return rc;
   for (int st = 1; st < 4; ++st)
-    if (int rc = rhs_stage<T, T>(c, (const T *)c->qs, st)) return rc;
+    if (int rc = run_stage<T>(c, nullptr, st)) return rc;
   return CRUNA_HIP_OK;
 }
 
@@ -283,6 +363,9 @@ int step_once(cruna_hip_ctx *c, int nt) {
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
   int rc = mixed ? rk4_step<float>(c) : rk4_step<double>(c);
   if (rc) return rc;
+  // set_boundary_condition_q(qn, outside_rhs=.true.) (cruna_hpc.f90:216)
+  if (c->has_bcq)
+    if ((rc = bc_q<double>(c, c->qn))) return rc;
   for (int d = 0; d < 3; ++d) {
     const int fq = c->cfg.filter_freq[d];
     if (fq > 0 && nt % fq == 0) {
@@ -347,7 +430,7 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   if (cfg->nbc < 0 || cfg->nbc > CRUNA_HIP_MAX_BC) return fail(CRUNA_HIP_EINVAL, "bad nbc");
   for (int r = 0; r < cfg->nbc; ++r) {
     const int t = (int)lround(cfg->bc[r][0]);
-    if (t != 0)
+    if (t != 0 && t != 201 && t != 400 && t != 401)
       return fail(CRUNA_HIP_EINVAL,
                   "boundary type " + std::to_string(t) + " not implemented on the device");
   }
@@ -356,6 +439,23 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   HIP_TRY(hipGetDevice(&c->device));
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->N = (long)cfg->n[0] * cfg->n[1] * cfg->n[2];
+  for (int r = 0; r < cfg->nbc; ++r) {
+    const int t = (int)lround(cfg->bc[r][0]);
+    if (t == 0) continue;
+    BoxRow b;
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+      // 1-based inclusive indices, clamped to the grid (modify_boundary_conditions_array)
+      lo[d] = std::max(1, (int)lround(cfg->bc[r][1 + 2 * d])) - 1;
+      hi[d] = std::min(cfg->n[d], (int)lround(cfg->bc[r][2 + 2 * d])) - 1;
+      b.k[d] = cfg->bc[r][7 + d];
+    }
+    if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) continue;
+    b.i0 = lo[0]; b.i1 = hi[0]; b.j0 = lo[1]; b.j1 = hi[1]; b.k0 = lo[2]; b.k1 = hi[2];
+    c->bc.push_back({t, b});
+    if (t == 201 || t == 401) c->has_bcq = true;
+    if (t == 400) c->has_bcrhs = true;
+  }
   for (int d = 0; d < 3; ++d) {
     build_geom(c.get(), d);
     try {
@@ -487,7 +587,7 @@ int cruna_hip_rhs_direct(cruna_hip_ctx *c, double *rhs, const double *q, double 
   if (int rc = ensure_tmp(c)) return rc;
   HIP_TRY(hipMemcpyAsync(c->tmp, q, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
-  int rc = mixed ? rhs_stage<float, double>(c, c->tmp, 4) : rhs_stage<double, double>(c, c->tmp, 4);
+  int rc = mixed ? run_stage<float>(c, c->tmp, 4) : run_stage<double>(c, c->tmp, 4);
   if (rc) return rc;
   if (mixed) {
     std::vector<float> h(5 * c->N);

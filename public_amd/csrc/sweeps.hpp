@@ -1,14 +1,15 @@
 // Fused direct-RHS sweeps, filter sweep and single-derivative kernel.
 //
 // One RK stage of right_hand_side_direct_3d (src/equations/ns_rho_rhou_p.f90:31-253)
-// is evaluated in five line sweeps; each sweep applies every compact
+// is evaluated in six line sweeps; each sweep applies every compact
 // derivative that runs along its direction, so the 42 Dx calls of the
 // reference become 42 line solves on register-resident blocks:
 //   S1 x : u_x v_x w_x, d/dx of (rho u), (rho u u_i), p, (p u), T and lambda T_x
 //   S2 y : the same along y                                  (accumulates rhs)
-//   S3 z : the same along z, plus d/dz tau_{iz}
-//   S4 x : d/dx tau_{ix}
-//   S5 y : d/dy tau_{iy} and tau:grad(u), then the RK4 stage update
+//   S3 z : the same along z
+//   S4 z : d/dz tau_{iz}
+//   S5 x : d/dx tau_{ix}
+//   S6 y : d/dy tau_{iy} and tau:grad(u), then the RK4 stage update
 //          (src/time/rk_o4_e.f90:45-50)
 // The pressure equation is accumulated as the bracket of ns_rho_rhou_p.f90:200-206
 // and multiplied by (gamma - 1) once, as the reference does.
@@ -46,9 +47,23 @@ struct Phys {
 template <typename T>
 __device__ __forceinline__ T viscosity(const Phys &ph, T Tk) {
   const T r = Tk / T(ph.T0);
-  if (ph.visc_law == 1) return T(ph.mu0) * pow(r, T(ph.visc_exp));
+  // r > 0, so r**e = 2**(e log2 r) without pow's special-case branches
+  if (ph.visc_law == 1) return T(ph.mu0) * exp2(T(ph.visc_exp) * log2(r));
   const T S1 = T(110.4);
   return T(ph.mu0) * (r * sqrt(r)) * ((T(ph.T0) + S1) / (Tk + S1));
+}
+
+// Issue point of a group of global loads: the compiler may not sink them below
+// this line (it otherwise moves each point's loads down to its first use and
+// serialises their latencies).
+__device__ __forceinline__ void issue_loads() { asm volatile("" ::: "memory"); }
+
+// Materialise a value here: the compiler otherwise sinks a computation to its
+// (late) use and keeps all of its operands live across the line solves between.
+template <typename T>
+__device__ __forceinline__ void materialize(T (&a)[PPT]) {
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) asm volatile("" : "+v"(a[m]));
 }
 
 // give_T (:57) and give_lambda (:169): T = (W/R) p / rho, lambda = mu cp / Pr
@@ -67,12 +82,17 @@ __device__ __forceinline__ T lambda_of(const Phys &ph, T mu) {
 // field base (SGPRs) plus a 32-bit byte offset, so every field of a sweep
 // shares the same PPT offset registers (the host checks that a field fits
 // in 4 GiB).
+// The field base goes through an opaque SGPR copy so that the compiler keeps
+// the (scalar base + 32-bit vector offset) form instead of folding the bases of
+// several fields into per-point 64-bit vector addresses.
 template <typename Q>
 __device__ __forceinline__ const Q *at(const Q *p, unsigned byte_off) {
+  asm volatile("" : "+s"(p));
   return reinterpret_cast<const Q *>(reinterpret_cast<const char *>(p) + byte_off);
 }
 template <typename Q>
 __device__ __forceinline__ Q *at(Q *p, unsigned byte_off) {
+  asm volatile("" : "+s"(p));
   return reinterpret_cast<Q *>(reinterpret_cast<char *>(p) + byte_off);
 }
 
@@ -136,6 +156,7 @@ __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__re
   ld<DIR>(qs + 2 * N, B, PS, m2);
   ld<DIR>(qs + 3 * N, B, PS, m3);
   ld<DIR>(qs + 4 * N, B, PS, p);
+  issue_loads();
   T(&mom)[PPT] = DIR == 0 ? m1 : (DIR == 1 ? m2 : m3);
   auto vel = [&](int i, int m) -> T { return (i == 0 ? m1[m] : (i == 1 ? m2[m] : m3[m])) / rho[m]; };
   T f[PPT], x[PPT], r[PPT];
@@ -186,36 +207,6 @@ __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__re
     if (DIR == 1) st<DIR>(G + 5 * N, B, PS, x);  // w_y
     if (DIR == 2) st<DIR>(G + 2 * N, B, PS, x);  // w_z
 
-    if (DIR == 2) {
-      // viscous fluxes d/dz tau_{iz} (ns_rho_rhou_p.f90:158-177):
-      // tau_13 = mu (u_z + w_x), tau_23 = mu (v_z + w_y), tau_33 = -2/3 mu div u + 2 mu w_z
-      T mu[PPT], a[PPT], c[PPT];
-      ld<DIR>(G + 0 * N, B, PS, a);
-      ld<DIR>(G + 1 * N, B, PS, c);
-      ld<DIR>(rhs + 1 * N, B, PS, r);
-#pragma unroll
-      for (int m = 0; m < PPT; ++m) {
-        mu[m] = viscosity(ph, temperature(ph, rho[m], p[m]));
-        f[m] = mu[m] * sa[m];
-        const T md = -2.0 / 3.0 * mu[m] * (a[m] + c[m] + x[m]);
-        a[m] = md + 2 * mu[m] * x[m];
-        c[m] = mu[m] * sb[m];
-      }
-      line_solve<T, PER>(op, lds, tp, f, x);
-#pragma unroll
-      for (int m = 0; m < PPT; ++m) r[m] = r[m] + x[m] * ih;
-      st<DIR>(rhs + 1 * N, B, PS, r);
-      ld<DIR>(rhs + 2 * N, B, PS, r);
-      line_solve<T, PER>(op, lds, tp, c, x);
-#pragma unroll
-      for (int m = 0; m < PPT; ++m) r[m] = r[m] + x[m] * ih;
-      st<DIR>(rhs + 2 * N, B, PS, r);
-      ld<DIR>(rhs + 3 * N, B, PS, r);
-      line_solve<T, PER>(op, lds, tp, a, x);
-#pragma unroll
-      for (int m = 0; m < PPT; ++m) r[m] = r[m] + x[m] * ih;
-      st<DIR>(rhs + 3 * N, B, PS, r);
-    }
   }
 
   // mass: -d/dx_d (rho u_d) (:131-135)
@@ -273,67 +264,79 @@ __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__re
 }
 
 // ----------------------------------------------------------------------------
-// S4 (x) / S5 (y): viscous fluxes d/dx_d tau_{i d}; S5 also adds tau:grad(u)
-// (ns_rho_rhou_p.f90:202-205) and finishes the stage.
+// S4 (z) / S5 (x) / S6 (y): viscous fluxes d/dx_d tau_{i d}; the y sweep also
+// adds tau:grad(u) (ns_rho_rhou_p.f90:202-205) and finishes the stage.
 // mode 0..3: RK4 stage update; mode 4: write the complete RHS into rhs.
+// qs_out may be the stage input itself (stages 1-3 update it in place, each
+// thread only its own points), so neither carries __restrict__.
 template <typename T, typename Q, int DIR, bool PER>
-__global__ __launch_bounds__(NT) void k_sweep2(const Q *__restrict__ qs, T *__restrict__ rhs,
+__global__ __launch_bounds__(NT) void k_sweep2(const Q *qs, T *__restrict__ rhs,
                                                const T *__restrict__ G, LineOp<T> op, SweepGeom g,
                                                Phys ph, long N, int mode, double *__restrict__ qn,
-                                               double *__restrict__ acc, T *__restrict__ qs_out,
-                                               double dt) {
+                                               double *__restrict__ acc, T *qs_out, double dt) {
   __shared__ LineLds<T> lds;
   const ThreadPos tp = thread_pos(g);
   const long B = tp.base, PS = g.PS;
   const T ih = T(ph.inv_h[DIR]);
   T t1[PPT], t2[PPT], t3[PPT], diss[PPT];
   {
-    T rho[PPT], p[PPT], ux[PPT], vy[PPT], wz[PPT], s12[PPT], s3[PPT];
-    ld<DIR>(qs, B, PS, rho);
-    ld<DIR>(qs + 4 * N, B, PS, p);
-    ld<DIR>(G + 0 * N, B, PS, ux);
-    ld<DIR>(G + 1 * N, B, PS, vy);
-    ld<DIR>(G + 2 * N, B, PS, wz);
-    ld<DIR>(G + 3 * N, B, PS, s12);
-    ld<DIR>(G + (DIR == 0 ? 4 : 5) * N, B, PS, s3);  // x: u_z + w_x, y: v_z + w_y
+    T mu[PPT], ux[PPT], vy[PPT], wz[PPT], sa[PPT], sb[PPT];
+    {
+      T rho[PPT], p[PPT];
+      ld<DIR>(qs, B, PS, rho);
+      ld<DIR>(qs + 4 * N, B, PS, p);
+      ld<DIR>(G + 0 * N, B, PS, ux);
+      ld<DIR>(G + 1 * N, B, PS, vy);
+      ld<DIR>(G + 2 * N, B, PS, wz);
+      // x: s12 = u_y + v_x, s13 = u_z + w_x;  y: s12, s23 = v_z + w_y;  z: s13, s23
+      ld<DIR>(G + (DIR == 2 ? 4 : 3) * N, B, PS, sa);
+      ld<DIR>(G + (DIR == 0 ? 4 : 5) * N, B, PS, sb);
+      issue_loads();
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) mu[m] = viscosity(ph, temperature(ph, rho[m], p[m]));
+    }
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
-      const T mu = viscosity(ph, temperature(ph, rho[m], p[m]));
-      const T md = -2.0 / 3.0 * mu * (ux[m] + vy[m] + wz[m]);
-      const T t11 = md + 2 * mu * ux[m], t22 = md + 2 * mu * vy[m], t33 = md + 2 * mu * wz[m];
+      const T md = -2.0 / 3.0 * mu[m] * (ux[m] + vy[m] + wz[m]);
+      const T t11 = md + 2 * mu[m] * ux[m], t22 = md + 2 * mu[m] * vy[m], t33 = md + 2 * mu[m] * wz[m];
       if (DIR == 0) {
-        t1[m] = t11;           // tau_11
-        t2[m] = mu * s12[m];   // tau_21
-        t3[m] = mu * s3[m];    // tau_31
-      } else {
-        t1[m] = mu * s12[m];   // tau_12
-        t2[m] = t22;           // tau_22
-        t3[m] = mu * s3[m];    // tau_32
+        t1[m] = t11;            // tau_11
+        t2[m] = mu[m] * sa[m];  // tau_21
+        t3[m] = mu[m] * sb[m];  // tau_31
+      } else if (DIR == 1) {
+        t1[m] = mu[m] * sa[m];  // tau_12
+        t2[m] = t22;            // tau_22
+        t3[m] = mu[m] * sb[m];  // tau_32
         // tau_ij du_i/dx_j = tau_11 u_x + tau_22 v_y + tau_33 w_z + mu (s12^2 + s13^2 + s23^2)
-        diss[m] = t11 * ux[m] + t22 * vy[m] + t33 * wz[m] + t1[m] * s12[m] + t3[m] * s3[m];
+        diss[m] = t11 * ux[m] + t22 * vy[m] + t33 * wz[m] + t1[m] * sa[m] + t3[m] * sb[m];
+      } else {
+        t1[m] = mu[m] * sa[m];  // tau_13
+        t2[m] = mu[m] * sb[m];  // tau_23
+        t3[m] = t33;            // tau_33
       }
     }
     if (DIR == 1) {
-      T s13[PPT];
-      ld<DIR>(G + 4 * N, B, PS, s13);
+      ld<DIR>(G + 4 * N, B, PS, sa);
 #pragma unroll
-      for (int m = 0; m < PPT; ++m) {
-        const T mu = viscosity(ph, temperature(ph, rho[m], p[m]));
-        diss[m] = diss[m] + (mu * s13[m]) * s13[m];
-      }
+      for (int m = 0; m < PPT; ++m) diss[m] = diss[m] + (mu[m] * sa[m]) * sa[m];
+      materialize(diss);
     }
   }
   T x1[PPT], x2[PPT], x3[PPT];
   line_solve<T, PER>(op, lds, tp, t1, x1);
   line_solve<T, PER>(op, lds, tp, t2, x2);
   line_solve<T, PER>(op, lds, tp, t3, x3);
+  // keep the read-modify-write traffic below the solves: hoisting it above
+  // them (legal, the engine's barriers only order LDS) would hold every
+  // operand in registers across the solves
+  asm volatile("" ::: "memory");
 #pragma unroll
   for (int m = 0; m < PPT; ++m) {
     x1[m] = x1[m] * ih;
     x2[m] = x2[m] * ih;
     x3[m] = x3[m] * ih;
   }
-  if (DIR == 0) {
+  if (DIR != 1) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       T a[PPT];

@@ -165,3 +165,79 @@ def test_fp32_mixed_close_to_fp64():
     err = rel(out[1], out[0])
     print(f"fp32-mixed vs fp64 after 10 steps: {err:.3e}")
     assert err < 1e-5
+
+
+def _shear_case(shape, seed=4):
+    """A small non-periodic-in-y shear layer with characteristic BCs (type 201)
+    on both y faces and sponge layers (400 on the RHS, or 401 on q)."""
+    n1, n2, n3 = shape
+    Lx, Ly, Lz = 6.0, 4.0, 3.0
+    dx = (Lx / n1, Ly / (n2 - 1), Lz / n3)
+    x = np.arange(n1) * dx[0]
+    y = np.arange(n2) * dx[1] - Ly / 2
+    z = np.arange(n3) * dx[2]
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    rng = np.random.default_rng(seed)
+    gamma, mach = 1.4, 0.5
+    rho = np.ones(shape)
+    u = 0.5 * np.tanh(2 * Y) + 1e-2 * rng.standard_normal(shape) * np.exp(-Y ** 2)
+    v = 1e-2 * rng.standard_normal(shape) * np.exp(-Y ** 2)
+    w = 1e-2 * rng.standard_normal(shape) * np.exp(-Y ** 2)
+    p = np.full(shape, 1.0 / (gamma * mach ** 2))
+    q = np.zeros(shape + (5,), order="F")
+    q[..., 0], q[..., 1], q[..., 2], q[..., 3], q[..., 4] = rho, rho * u, rho * v, rho * w, p
+    return q, dx, (x, y, z), (Lx, Ly, Lz)
+
+
+@pytest.mark.parametrize("sponge_type", [400, 401])
+def test_run_direct_open_y_with_boundaries(sponge_type):
+    Cruna = _gpu()
+    shape = (32, 48, 16)
+    q, dx, (x, y, z), (Lx, Ly, Lz) = _shear_case(shape)
+    n1, n2, n3 = shape
+    bc = [(201, 1, n1, 1, 1, 1, n3, 0, -1, 0),
+          (201, 1, n1, n2, n2, 1, n3, 0, 1, 0),
+          (sponge_type, 1, n1, 1, n2, 1, n3)]
+    scheme = (O.PERIODIC, O.OPEN, O.PERIODIC)
+    kw = dict(mu0=2e-3, T0=1.0 / (1.4 * 0.25), dt=2e-3, scheme=scheme, bc=bc)
+    cfg = O.Cfg(shape, dx, filter_freq=(1, 1, 1), **kw)
+    amp = 2.0 if sponge_type == 400 else 0.05
+    shape_fn = O.sponge_shape(cfg, (0, -Ly / 2, 0), (Lx, Ly / 2, Lz), (-1.0, 0.2 * Ly, -1.0), amp,
+                              np.asfortranarray(x), np.asfortranarray(y), np.asfortranarray(z))
+    assert shape_fn.max() > 0 and (shape_fn == 0).any()
+    qref = q.copy(order="F")
+    nsteps = 6
+    ref = O.run_direct(cfg, q, nsteps, qref=qref, shape=shape_fn)
+    with Cruna(shape, dx, filter_freq=(1, 1, 1), **kw) as g:
+        g.set_qref(qref)
+        g.set_sponge_shape(shape_fn)
+        g.set_state(q)
+        g.run_direct(nsteps)
+        g.synchronize()
+        got = g.get_state()
+    for v in range(5):
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+    # the boundary rows act: the run differs from one without them
+    with Cruna(shape, dx, filter_freq=(1, 1, 1), **dict(kw, bc=())) as g:
+        g.set_state(q)
+        g.run_direct(nsteps)
+        g.synchronize()
+        free = g.get_state()
+    assert rel(free, ref) > 1e-8
+
+
+def test_rhs_direct_with_characteristic_rows():
+    Cruna = _gpu()
+    shape = (16, 24, 16)
+    q, dx, _, _ = _shear_case(shape, seed=8)
+    n1, n2, n3 = shape
+    bc = [(201, 1, n1, 1, 1, 1, n3, 0, -1, 0), (201, 1, n1, n2, n2, 1, n3, 0, 1, 0)]
+    scheme = (O.PERIODIC, O.OPEN, O.PERIODIC)
+    kw = dict(mu0=2e-3, T0=1.0 / (1.4 * 0.25), dt=2e-3, scheme=scheme, bc=bc)
+    qref = q.copy(order="F")
+    qref[..., 4] *= 1.01  # a target that differs from the state
+    ref = O.rhs_direct(O.Cfg(shape, dx, **kw), q, qref=qref)
+    with Cruna(shape, dx, **kw) as g:
+        g.set_qref(qref)
+        got = g.right_hand_side_direct_3d(q)
+    assert rel(got, ref) < 1e-12
