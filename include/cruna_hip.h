@@ -53,6 +53,7 @@ typedef struct cruna_hip_cfg {
   double visc_exp;       /* exponent of the power law */
   double dt;             /* time.dt */
   int filter_freq[3];    /* filter.di1freq..di3freq (0 = never) */
+  int afilter_freq[3];   /* filter.ai1freq..ai3freq of the adjoint loop (0 = never) */
   int heat_unit_spacing; /* 1: heat-flux Dx without h, as ns_rho_rhou_p.f90:189-197 */
   int precision;         /* CRUNA_HIP_FP64 / CRUNA_HIP_FP32_MIXED */
   /* boundary_conditions.dat rows: type; i1s; i1e; i2s; i2e; i3s; i3e; k1; k2; k3 */
@@ -100,7 +101,8 @@ int cruna_hip_rhs_adjoint(cruna_hip_ctx *ctx, double *rhs, const double *qs, con
 /* time_stepper_direct(q1,q0,t) (src/time/rk_o4_e.f90:17-58), host in/out. */
 int cruna_hip_time_stepper_direct(cruna_hip_ctx *ctx, double *q1, const double *q0, double t);
 
-/* time_stepper_adjoint(qs1,qs0,q0,g,t) (src/time/rk_o4_e.f90:62-105), host in/out. */
+/* time_stepper_adjoint(qs1,qs0,q0,g,t) (src/time/rk_o4_e.f90:62-105), host in/out;
+ * g may be NULL.  Leaves qs1 also in the device adjoint state. */
 int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *ctx, double *qs1, const double *qs0,
                                    const double *q0, const double *g, double t);
 
@@ -110,6 +112,28 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *ctx, double *qs1, const double
  * Asynchronous on the context stream; cruna_hip_synchronize waits. */
 int cruna_hip_run_direct(cruna_hip_ctx *ctx, int nt0, int nsteps);
 int cruna_hip_synchronize(cruna_hip_ctx *ctx);
+
+/* Device-resident adjoint state qs^n (layout as q). */
+int cruna_hip_set_adjoint_state(cruna_hip_ctx *ctx, const double *qs);
+int cruna_hip_get_adjoint_state(cruna_hip_ctx *ctx, double *qs);
+
+/* Forward trajectory Q(:,:,:,:,nt) of calc_direct / calc_adjoint
+ * (src/main/computation_direct.f90:100-103, computation_adjoint.f90:69-73),
+ * kept in device memory: slot nt (1-based) holds the state after forward
+ * step nt.  alloc reserves nslots states (0 frees them); set/get copy one
+ * slot from/to the host (the reference's load of data_direct); with
+ * record on, cruna_hip_run_direct stores each step's state in its slot. */
+int cruna_hip_trajectory_alloc(cruna_hip_ctx *ctx, int nslots);
+int cruna_hip_trajectory_set(cruna_hip_ctx *ctx, int nt, const double *q);
+int cruna_hip_trajectory_get(cruna_hip_ctx *ctx, int nt, double *q);
+int cruna_hip_trajectory_record(cruna_hip_ctx *ctx, int on);
+
+/* Adjoint step loop of calc_adjoint (src/main/computation_adjoint.f90:62-99)
+ * with g = 0: for nt = nt_end down to nt_end-nsteps+1, with q0 = Q(nt)
+ * (or the only slot when one is allocated: the linear adjoint):
+ * time_stepper_adjoint -> BC on qs (outside rhs) -> adjoint filters i1,i2,i3
+ * where mod(nt, afilter_freq) == 0.  Asynchronous on the context stream. */
+int cruna_hip_run_adjoint(cruna_hip_ctx *ctx, int nt_end, int nsteps);
 
 /* Per-kernel timing with HIP events on the context stream: enable, run,
  * then read the summed milliseconds and launch counts per kernel id. */

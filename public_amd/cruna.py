@@ -45,6 +45,7 @@ class Cfg(C.Structure):
         ("visc_exp", C.c_double),
         ("dt", C.c_double),
         ("filter_freq", C.c_int * 3),
+        ("afilter_freq", C.c_int * 3),
         ("heat_unit_spacing", C.c_int),
         ("precision", C.c_int),
         ("nbc", C.c_int),
@@ -80,12 +81,21 @@ def lib():
         L.cruna_hip_time_stepper_adjoint.argtypes = [vp, dp, dp, dp, dp, C.c_double]
         L.cruna_hip_run_direct.argtypes = [vp, ip, ip]
         L.cruna_hip_synchronize.argtypes = [vp]
+        L.cruna_hip_set_adjoint_state.argtypes = [vp, dp]
+        L.cruna_hip_get_adjoint_state.argtypes = [vp, dp]
+        L.cruna_hip_trajectory_alloc.argtypes = [vp, ip]
+        L.cruna_hip_trajectory_set.argtypes = [vp, ip, dp]
+        L.cruna_hip_trajectory_get.argtypes = [vp, ip, dp]
+        L.cruna_hip_trajectory_record.argtypes = [vp, ip]
+        L.cruna_hip_run_adjoint.argtypes = [vp, ip, ip]
         L.cruna_hip_profile_enable.argtypes = [vp, ip]
         L.cruna_hip_profile_read.argtypes = [vp, ip, dp, C.POINTER(C.c_long), C.c_char_p, C.c_size_t]
         L.cruna_hip_profile_count.argtypes = []
         for f in ("create", "destroy", "set_state", "get_state", "set_qref", "set_sponge_shape", "diff",
                   "filter", "rhs_direct", "rhs_adjoint", "time_stepper_direct", "time_stepper_adjoint",
-                  "run_direct", "synchronize", "profile_enable", "profile_read", "profile_count"):
+                  "run_direct", "synchronize", "profile_enable", "profile_read", "profile_count",
+                  "set_adjoint_state", "get_adjoint_state", "trajectory_alloc", "trajectory_set",
+                  "trajectory_get", "trajectory_record", "run_adjoint"):
             getattr(L, "cruna_hip_" + f).restype = C.c_int
         _lib = L
     return _lib
@@ -112,7 +122,8 @@ class Cruna:
 
     def __init__(self, n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0, W=1.0,
                  Pr=0.71, mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76, dt=1e-3,
-                 filter_freq=(1, 1, 1), heat_unit_spacing=1, precision=FP64, bc=()):
+                 filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), heat_unit_spacing=1, precision=FP64,
+                 bc=()):
         L = lib()
         cfg = Cfg()
         L.cruna_hip_default_cfg(C.byref(cfg))
@@ -121,6 +132,7 @@ class Cruna:
             cfg.dx[d] = float(dx[d])
             cfg.scheme[d] = int(scheme[d])
             cfg.filter_freq[d] = int(filter_freq[d])
+            cfg.afilter_freq[d] = int(afilter_freq[d])
         cfg.filter_alpha = filter_alpha
         cfg.gamma, cfg.R, cfg.W, cfg.Pr, cfg.mu0, cfg.T0 = gamma, R, W, Pr, mu0, T0
         cfg.visc_law, cfg.visc_exp, cfg.dt = visc_law, visc_exp, dt
@@ -201,6 +213,23 @@ class Cruna:
         _check(lib().cruna_hip_time_stepper_direct(self._h, _dp(q1), _dp(q0), float(t)))
         return q1
 
+    def right_hand_side_adjoint_3d(self, qs, q, g=None, t=0.0):
+        """Adjoint RHS at adjoint state qs about the forward state q; g may be None."""
+        qs, q = _f64(qs, self.shape4), _f64(q, self.shape4)
+        gp = None if g is None else _dp(_f64(g, self.shape4))
+        rhs = np.empty(self.shape4, order="F")
+        _check(lib().cruna_hip_rhs_adjoint(self._h, _dp(rhs), _dp(qs), _dp(q), gp, float(t)))
+        return rhs
+
+    def time_stepper_adjoint(self, qs0, q0, g=None, t=0.0):
+        """One RK4 step backward in time (dt -> -dt), q0 and g frozen over the stages."""
+        qs0, q0 = _f64(qs0, self.shape4), _f64(q0, self.shape4)
+        g = None if g is None else _f64(g, self.shape4)
+        qs1 = np.empty(self.shape4, order="F")
+        _check(lib().cruna_hip_time_stepper_adjoint(self._h, _dp(qs1), _dp(qs0), _dp(q0),
+                                                    None if g is None else _dp(g), float(t)))
+        return qs1
+
     # ----- device-resident loop -----
     def set_state(self, q):
         q = _f64(q, self.shape4)
@@ -224,6 +253,37 @@ class Cruna:
 
     def synchronize(self):
         _check(lib().cruna_hip_synchronize(self._h))
+
+    # ----- adjoint loop -----
+    def set_adjoint_state(self, qs):
+        qs = _f64(qs, self.shape4)
+        _check(lib().cruna_hip_set_adjoint_state(self._h, _dp(qs)))
+
+    def get_adjoint_state(self):
+        qs = np.empty(self.shape4, order="F")
+        _check(lib().cruna_hip_get_adjoint_state(self._h, _dp(qs)))
+        return qs
+
+    def trajectory_alloc(self, nslots):
+        """Device slots for Q(:,:,:,:,nt), nt = 1..nslots (state after forward step nt)."""
+        _check(lib().cruna_hip_trajectory_alloc(self._h, int(nslots)))
+
+    def trajectory_set(self, nt, q):
+        q = _f64(q, self.shape4)
+        _check(lib().cruna_hip_trajectory_set(self._h, int(nt), _dp(q)))
+
+    def trajectory_get(self, nt):
+        q = np.empty(self.shape4, order="F")
+        _check(lib().cruna_hip_trajectory_get(self._h, int(nt), _dp(q)))
+        return q
+
+    def trajectory_record(self, on=True):
+        _check(lib().cruna_hip_trajectory_record(self._h, 1 if on else 0))
+
+    def run_adjoint(self, nsteps, nt_end=None):
+        """calc_adjoint's loop for nt = nt_end .. nt_end-nsteps+1 (nt_end defaults to nsteps)."""
+        nt_end = nsteps if nt_end is None else nt_end
+        _check(lib().cruna_hip_run_adjoint(self._h, int(nt_end), int(nsteps)))
 
     def profile(self, on=True):
         _check(lib().cruna_hip_profile_enable(self._h, 1 if on else 0))

@@ -13,6 +13,7 @@
 #include "line_ops.hpp"
 #include "boundary.hpp"
 #include "sweeps.hpp"
+#include "adjoint.hpp"
 
 using namespace cruna;
 
@@ -34,11 +35,12 @@ int fail(int code, const std::string &msg) {
 
 enum KernelId {
   KID_S1X = 0, KID_S2Y, KID_S3Z, KID_S4Z, KID_S5X, KID_S6Y, KID_FX, KID_FY, KID_FZ, KID_DIFF,
-  KID_COUNT
+  KID_AX, KID_AY, KID_AZ, KID_COUNT
 };
-const char *kKernelNames[KID_COUNT] = {"sweep1_x", "sweep1_y", "sweep1_z", "sweep2_z",
+const char *kKernelNames[KID_COUNT] = {"sweep1_x", "sweep1_y",    "sweep1_z", "sweep2_z",
                                        "sweep2_x", "sweep2_y_rk", "filter_x", "filter_y",
-                                       "filter_z", "diff"};
+                                       "filter_z", "diff",        "asweep_x_rk", "asweep_y",
+                                       "asweep_z"};
 
 template <typename T>
 struct DevOp {
@@ -90,6 +92,10 @@ struct cruna_hip_ctx {
   // boundary_conditions.dat rows in file order (types 201, 400, 401)
   std::vector<std::pair<int, BoxRow>> bc;
   bool has_bcq = false, has_bcrhs = false;
+  // adjoint state qs^n and the forward trajectory Q(:,:,:,:,nt) (slot nt-1)
+  double *qa = nullptr, *traj = nullptr, *gobj = nullptr;
+  int traj_slots = 0;
+  bool record = false;
   void *qs = nullptr, *rhs = nullptr, *G = nullptr;  // compute precision
   // profiling
   bool prof = false;
@@ -112,8 +118,11 @@ int build_geom(cruna_hip_ctx *c, int d) {
   g.n2 = n2;
   g.n3 = n3;
   const long lines_contig = d == 0 ? (long)n2 * n3 : n1;
+  // x-lines are contiguous: 256 threads; y/z tiles take up to 16 neighbouring
+  // x-lines (128-byte rows in FP64) in up to 512 threads
+  const int wg_max = d == 0 ? 256 : NT;
   int TL = 1;
-  while (g.NB * TL * 2 <= NT && lines_contig % (TL * 2) == 0) TL *= 2;
+  while (g.NB * TL * 2 <= wg_max && lines_contig % (TL * 2) == 0 && (d == 0 || TL < 16)) TL *= 2;
   g.TL = TL;
   if (d == 0) {
     g.PS = 1;
@@ -376,10 +385,131 @@ int step_once(cruna_hip_ctx *c, int nt) {
   return CRUNA_HIP_OK;
 }
 
+// Staging area of the host-pointer entry points: two states.
 int ensure_tmp(cruna_hip_ctx *c) {
-  if (!c->tmp) HIP_TRY(hipMalloc(&c->tmp, 5 * c->N * sizeof(double)));
+  if (!c->tmp) HIP_TRY(hipMalloc(&c->tmp, 10 * c->N * sizeof(double)));
   return CRUNA_HIP_OK;
 }
+
+// The RHS (compute precision) back to a host FP64 array.
+int copy_rhs_out(cruna_hip_ctx *c, double *rhs) {
+  if (c->cfg.precision == CRUNA_HIP_FP32_MIXED) {
+    std::vector<float> h(5 * c->N);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->rhs, 5 * c->N * sizeof(float), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (long i = 0; i < 5 * c->N; ++i) rhs[i] = h[i];
+  } else {
+    HIP_TRY(hipMemcpyAsync(rhs, c->rhs, 5 * c->N * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return CRUNA_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint
+
+// set_boundary_condition_qs (boundary_conditions.f90:140-224) on qs, rows in
+// order: 201 characteristic_qs toward zero with the forward state q, 401
+// sponge_qs; 400 has no adjoint-state part.
+template <typename Q>
+int bc_qs(cruna_hip_ctx *c, Q *qs, const double *q) {
+  for (auto &r : c->bc) {
+    if (r.first == 201) {
+      const BoxRow &b = r.second;
+      const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
+      hipLaunchKernelGGL((k_bc_characteristic_qs<Q>), dim3((np + 255) / 256), dim3(256), 0,
+                         c->stream, qs, q, b, c->cfg.n[0], c->cfg.n[1], c->N, c->cfg.gamma);
+    } else if (r.first == 401) {
+      if (!c->shape)
+        return fail(CRUNA_HIP_EINVAL, "sponge rows need a shape (cruna_hip_set_sponge_shape)");
+      hipLaunchKernelGGL((k_bc_sponge_qs<Q>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream,
+                         qs, (const double *)c->shape, c->N);
+    } else {
+      continue;
+    }
+    if (int rc = check_launch()) return rc;
+  }
+  return CRUNA_HIP_OK;
+}
+
+template <typename T, typename Q, int DIR>
+int launch_asweep(cruna_hip_ctx *c, const Q *in, const double *q0, const double *g, int mode) {
+  ProfScope p(c, DIR == 0 ? KID_AX : (DIR == 1 ? KID_AY : KID_AZ));
+  const dim3 grid(c->geom[DIR].ntiles), block(c->wg[DIR]);
+  T *rhs = (T *)c->rhs, *qs = (T *)c->qs;
+  const double dt = -c->cfg.dt;  // backward in time (rk_o4_e.f90:62-105)
+  if (c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC)
+    hipLaunchKernelGGL((k_asweep<T, Q, DIR, true>), grid, block, 0, c->stream, in, q0, rhs, g,
+                       dops<T>(c)[DIR].op, c->geom[DIR], c->cfg.gamma, c->phys.inv_h[DIR], c->N,
+                       mode, c->qa, c->acc, qs, dt);
+  else
+    hipLaunchKernelGGL((k_asweep<T, Q, DIR, false>), grid, block, 0, c->stream, in, q0, rhs, g,
+                       dops<T>(c)[DIR].op, c->geom[DIR], c->cfg.gamma, c->phys.inv_h[DIR], c->N,
+                       mode, c->qa, c->acc, qs, dt);
+  return check_launch();
+}
+
+// right_hand_side_adjoint_3d from stage input `in` with forward state q0:
+// sweeps z, y, x; the x sweep adds g and finishes the stage (mode as rhs_stage).
+template <typename T, typename Q>
+int arhs_stage(cruna_hip_ctx *c, const Q *in, const double *q0, const double *g, int mode) {
+  if (int rc = launch_asweep<T, Q, 2>(c, in, q0, g, mode)) return rc;
+  if (int rc = launch_asweep<T, Q, 1>(c, in, q0, g, mode)) return rc;
+  return launch_asweep<T, Q, 0>(c, in, q0, g, mode);
+}
+
+// An adjoint stage from the FP64 state `from` or, with from == nullptr, from
+// the stage input in qs.  With 201/401 rows the RHS works on a private copy
+// to which set_boundary_condition_qs is applied (ns_rho_rhou_p.f90:288-290).
+template <typename T>
+int arun_stage(cruna_hip_ctx *c, const double *from, const double *q0, const double *g, int mode) {
+  T *qs = (T *)c->qs;
+  if (from && !c->has_bcq) return arhs_stage<T, double>(c, from, q0, g, mode);
+  if (from) {
+    const long n = 5 * c->N;
+    hipLaunchKernelGGL((k_copy_state<T>), dim3((n + 255) / 256), dim3(256), 0, c->stream, qs, from, n);
+    if (int rc = check_launch()) return rc;
+  }
+  if (c->has_bcq)
+    if (int rc = bc_qs<T>(c, qs, q0)) return rc;
+  return arhs_stage<T, T>(c, (const T *)qs, q0, g, mode);
+}
+
+// time_stepper_adjoint (rk_o4_e.f90:62-105) on the device adjoint state qa.
+template <typename T>
+int ark4_step(cruna_hip_ctx *c, const double *q0, const double *g) {
+  if (int rc = arun_stage<T>(c, c->qa, q0, g, 0)) return rc;
+  for (int st = 1; st < 4; ++st)
+    if (int rc = arun_stage<T>(c, nullptr, q0, g, st)) return rc;
+  return CRUNA_HIP_OK;
+}
+
+// One step of the calc_adjoint loop (computation_adjoint.f90:72-99) with g = 0:
+// stepper -> BC on qs (outside rhs) -> adjoint filters where mod(nt, freq) == 0.
+int astep_once(cruna_hip_ctx *c, int nt, const double *q0) {
+  const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
+  int rc = mixed ? ark4_step<float>(c, q0, nullptr) : ark4_step<double>(c, q0, nullptr);
+  if (rc) return rc;
+  if (c->has_bcq)
+    if ((rc = bc_qs<double>(c, c->qa, q0))) return rc;
+  for (int d = 0; d < 3; ++d) {
+    const int fq = c->cfg.afilter_freq[d];
+    if (fq > 0 && nt % fq == 0) {
+      rc = mixed ? filter_dir<float>(c, c->qa, d, 5) : filter_dir<double>(c, c->qa, d, 5);
+      if (rc) return rc;
+    }
+  }
+  return CRUNA_HIP_OK;
+}
+
+int ensure_adjoint(cruna_hip_ctx *c) {
+  if (!c->qa) HIP_TRY(hipMalloc(&c->qa, 5 * c->N * sizeof(double)));
+  return CRUNA_HIP_OK;
+}
+
+double *traj_slot(cruna_hip_ctx *c, int nt) { return c->traj + (size_t)(nt - 1) * 5 * c->N; }
 
 }  // namespace
 
@@ -392,6 +522,7 @@ void cruna_hip_default_cfg(cruna_hip_cfg *cfg) {
     cfg->dx[d] = 1.0;
     cfg->scheme[d] = CRUNA_HIP_PERIODIC;
     cfg->filter_freq[d] = 1;
+    cfg->afilter_freq[d] = 0;
   }
   cfg->filter_alpha = 0.35;
   cfg->gamma = 1.4;
@@ -417,7 +548,7 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
       return fail(CRUNA_HIP_EINVAL, "n" + std::to_string(d + 1) + " = " +
                                         std::to_string(cfg->n[d]) +
                                         ": each grid size must be a multiple of 8 and >= 16");
-    if (cfg->n[d] / PPT > NT)
+    if (cfg->n[d] / PPT > 256)
       return fail(CRUNA_HIP_EINVAL, "grid size above 2048 points per line is not supported");
     if (cfg->scheme[d] != CRUNA_HIP_PERIODIC && cfg->scheme[d] != CRUNA_HIP_OPEN)
       return fail(CRUNA_HIP_EINVAL, "unknown scheme");
@@ -509,7 +640,7 @@ int cruna_hip_destroy(cruna_hip_ctx *c) {
     c->fop32[d].release();
   }
   for (void *p : {(void *)c->qn, (void *)c->acc, (void *)c->qref, (void *)c->shape, (void *)c->tmp,
-                  c->qs, c->rhs, c->G})
+                  (void *)c->qa, (void *)c->traj, (void *)c->gobj, c->qs, c->rhs, c->G})
     if (p) (void)hipFree(p);
   for (auto &p : c->pending) {
     (void)hipEventDestroy(p.second.first);
@@ -589,25 +720,34 @@ int cruna_hip_rhs_direct(cruna_hip_ctx *c, double *rhs, const double *q, double 
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
   int rc = mixed ? run_stage<float>(c, c->tmp, 4) : run_stage<double>(c, c->tmp, 4);
   if (rc) return rc;
-  if (mixed) {
-    std::vector<float> h(5 * c->N);
-    HIP_TRY(hipMemcpyAsync(h.data(), c->rhs, 5 * c->N * sizeof(float), hipMemcpyDeviceToHost,
-                           c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    for (long i = 0; i < 5 * c->N; ++i) rhs[i] = h[i];
-  } else {
-    HIP_TRY(hipMemcpyAsync(rhs, c->rhs, 5 * c->N * sizeof(double), hipMemcpyDeviceToHost,
-                           c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+  return copy_rhs_out(c, rhs);
+}
+
+// Host inputs of the adjoint entry points: qs -> qa, q -> second half of
+// tmp, g -> gobj.
+int stage_adjoint_inputs(cruna_hip_ctx *c, const double *qs, const double *q, const double *g) {
+  if (int rc = ensure_tmp(c)) return rc;
+  if (int rc = ensure_adjoint(c)) return rc;
+  HIP_TRY(hipMemcpyAsync(c->qa, qs, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->tmp + 5 * c->N, q, 5 * c->N * sizeof(double), hipMemcpyHostToDevice,
+                         c->stream));
+  if (g) {
+    if (!c->gobj) HIP_TRY(hipMalloc(&c->gobj, 5 * c->N * sizeof(double)));
+    HIP_TRY(hipMemcpyAsync(c->gobj, g, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
   }
   return CRUNA_HIP_OK;
 }
 
 int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const double *q,
                           const double *g, double t) {
-  (void)rhs; (void)qs; (void)q; (void)g; (void)t;
-  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
-  return fail(CRUNA_HIP_EINVAL, "adjoint RHS is not implemented on the device yet");
+  (void)t;  // the adjoint RHS of the reference does not depend on t
+  if (!c || !rhs || !qs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (int rc = stage_adjoint_inputs(c, qs, q, g)) return rc;
+  const double *q0 = c->tmp + 5 * c->N, *gd = g ? c->gobj : nullptr;
+  const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
+  int rc = mixed ? arun_stage<float>(c, c->qa, q0, gd, 4) : arun_stage<double>(c, c->qa, q0, gd, 4);
+  if (rc) return rc;
+  return copy_rhs_out(c, rhs);
 }
 
 int cruna_hip_time_stepper_direct(cruna_hip_ctx *c, double *q1, const double *q0, double t) {
@@ -624,16 +764,100 @@ int cruna_hip_time_stepper_direct(cruna_hip_ctx *c, double *q1, const double *q0
 
 int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *qs0,
                                    const double *q0, const double *g, double t) {
-  (void)qs1; (void)qs0; (void)q0; (void)g; (void)t;
+  (void)t;
+  if (!c || !qs1 || !qs0 || !q0) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (int rc = stage_adjoint_inputs(c, qs0, q0, g)) return rc;
+  const double *qf = c->tmp + 5 * c->N, *gd = g ? c->gobj : nullptr;
+  const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
+  int rc = mixed ? ark4_step<float>(c, qf, gd) : ark4_step<double>(c, qf, gd);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(qs1, c->qa, 5 * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_set_adjoint_state(cruna_hip_ctx *c, const double *qs) {
+  if (!c || !qs) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (int rc = ensure_adjoint(c)) return rc;
+  HIP_TRY(hipMemcpyAsync(c->qa, qs, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_get_adjoint_state(cruna_hip_ctx *c, double *qs) {
+  if (!c || !qs) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (!c->qa) return fail(CRUNA_HIP_EINVAL, "no adjoint state (cruna_hip_set_adjoint_state)");
+  HIP_TRY(hipMemcpyAsync(qs, c->qa, 5 * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_trajectory_alloc(cruna_hip_ctx *c, int nslots) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
-  return fail(CRUNA_HIP_EINVAL, "adjoint stepper is not implemented on the device yet");
+  if (nslots < 0) return fail(CRUNA_HIP_EINVAL, "nslots must be >= 0");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->traj) HIP_TRY(hipFree(c->traj));
+  c->traj = nullptr;
+  c->traj_slots = 0;
+  if (nslots == 0) return CRUNA_HIP_OK;
+  if (hipMalloc(&c->traj, (size_t)nslots * 5 * c->N * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    c->traj = nullptr;
+    return fail(CRUNA_HIP_ENOMEM, "trajectory of " + std::to_string(nslots) +
+                                      " states does not fit in device memory");
+  }
+  c->traj_slots = nslots;
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_trajectory_set(cruna_hip_ctx *c, int nt, const double *q) {
+  if (!c || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (nt < 1 || nt > c->traj_slots) return fail(CRUNA_HIP_EINVAL, "nt outside the trajectory");
+  HIP_TRY(hipMemcpyAsync(traj_slot(c, nt), q, 5 * c->N * sizeof(double), hipMemcpyHostToDevice,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_trajectory_get(cruna_hip_ctx *c, int nt, double *q) {
+  if (!c || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (nt < 1 || nt > c->traj_slots) return fail(CRUNA_HIP_EINVAL, "nt outside the trajectory");
+  HIP_TRY(hipMemcpyAsync(q, traj_slot(c, nt), 5 * c->N * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_trajectory_record(cruna_hip_ctx *c, int on) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (on && c->traj_slots == 0) return fail(CRUNA_HIP_EINVAL, "no trajectory (cruna_hip_trajectory_alloc)");
+  c->record = on != 0;
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_run_adjoint(cruna_hip_ctx *c, int nt_end, int nsteps) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
+  if (!c->qa) return fail(CRUNA_HIP_EINVAL, "no adjoint state (cruna_hip_set_adjoint_state)");
+  if (c->traj_slots == 0) return fail(CRUNA_HIP_EINVAL, "no trajectory (cruna_hip_trajectory_alloc)");
+  // one slot: the linear adjoint about a fixed state (computation_adjoint.f90:69-73)
+  if (c->traj_slots > 1 && nsteps > 0 && (nt_end > c->traj_slots || nt_end - nsteps + 1 < 1))
+    return fail(CRUNA_HIP_EINVAL, "steps outside the trajectory");
+  for (int nt = nt_end; nt > nt_end - nsteps; --nt)
+    if (int rc = astep_once(c, nt, traj_slot(c, c->traj_slots > 1 ? nt : 1))) return rc;
+  return CRUNA_HIP_OK;
 }
 
 int cruna_hip_run_direct(cruna_hip_ctx *c, int nt0, int nsteps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
-  for (int nt = nt0 + 1; nt <= nt0 + nsteps; ++nt)
+  for (int nt = nt0 + 1; nt <= nt0 + nsteps; ++nt) {
     if (int rc = step_once(c, nt)) return rc;
+    // Q(:,:,:,:,nt) = qn (computation_direct.f90:100-103)
+    if (c->record && nt >= 1 && nt <= c->traj_slots)
+      HIP_TRY(hipMemcpyAsync(traj_slot(c, nt), c->qn, 5 * c->N * sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
+  }
   return CRUNA_HIP_OK;
 }
 

@@ -15,7 +15,7 @@
 
 namespace cruna {
 
-constexpr int NT = 256;  // max threads per work-group
+constexpr int NT = 512;  // max threads per work-group
 
 template <typename T>
 struct LineOp {

@@ -86,14 +86,18 @@ __device__ __forceinline__ T lambda_of(const Phys &ph, T mu) {
 // the (scalar base + 32-bit vector offset) form instead of folding the bases of
 // several fields into per-point 64-bit vector addresses.
 template <typename Q>
-__device__ __forceinline__ const Q *at(const Q *p, unsigned byte_off) {
+using gptr = __attribute__((address_space(1))) Q *;  // global memory
+template <typename Q>
+__device__ __forceinline__ gptr<const Q> at(const Q *p, unsigned byte_off) {
   asm volatile("" : "+s"(p));
-  return reinterpret_cast<const Q *>(reinterpret_cast<const char *>(p) + byte_off);
+  // the opaque copy is a generic pointer again; naming the global address
+  // space keeps global (not flat) loads, whose completion LDS waits ignore
+  return (gptr<const Q>)(reinterpret_cast<const char *>(p) + byte_off);
 }
 template <typename Q>
-__device__ __forceinline__ Q *at(Q *p, unsigned byte_off) {
+__device__ __forceinline__ gptr<Q> at(Q *p, unsigned byte_off) {
   asm volatile("" : "+s"(p));
-  return reinterpret_cast<Q *>(reinterpret_cast<char *>(p) + byte_off);
+  return (gptr<Q>)(reinterpret_cast<char *>(p) + byte_off);
 }
 
 template <int DIR, typename T, typename Q>
@@ -104,7 +108,7 @@ __device__ __forceinline__ void ld(const Q *__restrict__ p, long base, long PS, 
     constexpr int W = 16 / sizeof(Q);
 #pragma unroll
     for (int k = 0; k < PPT / W; ++k) {
-      const V t = *reinterpret_cast<const V *>(at(p, (b0 + k * W) * unsigned(sizeof(Q))));
+      const V t = *(gptr<const V>)(at(p, (b0 + k * W) * unsigned(sizeof(Q))));
 #pragma unroll
       for (int j = 0; j < W; ++j) v[k * W + j] = T(t[j]);
     }
@@ -125,7 +129,7 @@ __device__ __forceinline__ void st(Q *__restrict__ p, long base, long PS, const 
       V t;
 #pragma unroll
       for (int j = 0; j < W; ++j) t[j] = Q(v[k * W + j]);
-      *reinterpret_cast<V *>(at(p, (b0 + k * W) * unsigned(sizeof(Q)))) = t;
+      *(gptr<V>)(at(p, (b0 + k * W) * unsigned(sizeof(Q)))) = t;
     }
   } else {
 #pragma unroll
@@ -263,6 +267,42 @@ __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__re
   st<DIR>(rhs + 4 * N, B, PS, e);
 }
 
+// RK4 stage update of variable v from its complete RHS k (rk_o4_e.f90:45-50,
+// and :90-95 for the adjoint with dt -> -dt): stage inputs q0 + dt/2 k1,
+// q0 + dt/2 k2, q0 + dt k3, then q1 = q0 + dt/6 (k1 + 2 k2 + 2 k3 + k4).
+// mode 4 writes k into rhs instead (a single RHS evaluation).
+template <int DIR, typename T>
+__device__ __forceinline__ void rk_update(int mode, int v, const T (&k)[PPT], long B, long PS,
+                                          long N, T *rhs, double *__restrict__ qn,
+                                          double *__restrict__ acc, T *qs_out, double dt) {
+  if (mode == 4) {
+    st<DIR>(rhs + v * N, B, PS, k);
+    return;
+  }
+  const double c_stage = mode == 2 ? dt : dt * 0.5;
+  double q0[PPT], s[PPT];
+  ld<DIR>(qn + v * N, B, PS, q0);
+  if (mode == 3) {
+    ld<DIR>(acc + v * N, B, PS, s);
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) q0[m] = q0[m] + dt / 6.0 * (s[m] + double(k[m]));
+    st<DIR>(qn + v * N, B, PS, q0);
+    return;
+  }
+  if (mode == 0) {
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) s[m] = double(k[m]);
+  } else {
+    ld<DIR>(acc + v * N, B, PS, s);
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) s[m] = s[m] + 2 * double(k[m]);
+  }
+  st<DIR>(acc + v * N, B, PS, s);
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) s[m] = q0[m] + c_stage * double(k[m]);
+  st<DIR>(qs_out + v * N, B, PS, s);
+}
+
 // ----------------------------------------------------------------------------
 // S4 (z) / S5 (x) / S6 (y): viscous fluxes d/dx_d tau_{i d}; the y sweep also
 // adds tau:grad(u) (ns_rho_rhou_p.f90:202-205) and finishes the stage.
@@ -347,9 +387,7 @@ __global__ __launch_bounds__(NT) void k_sweep2(const Q *qs, T *__restrict__ rhs,
     }
     return;
   }
-  // complete RHS k_v and the RK4 update (rk_o4_e.f90:45-50): stage inputs
-  // q0 + dth k1, q0 + dth k2, q0 + dt k3; q1 = q0 + dt/6 (k1 + 2 k2 + 2 k3 + k4)
-  const double c_stage = mode == 2 ? dt : dt * 0.5;
+  // complete RHS k_v and the RK4 update
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     // one variable at a time keeps the tail's loads from being hoisted together
@@ -363,31 +401,8 @@ __global__ __launch_bounds__(NT) void k_sweep2(const Q *qs, T *__restrict__ rhs,
       if (v == 3) k[m] = k[m] + x3[m];
       if (v == 4) k[m] = T(ph.gm1) * (k[m] + diss[m]);
     }
-    if (mode == 4) {
-      if (v > 0) st<DIR>(rhs + v * N, B, PS, k);
-      continue;
-    }
-    double q0[PPT], s[PPT];
-    ld<DIR>(qn + v * N, B, PS, q0);
-    if (mode == 3) {
-      ld<DIR>(acc + v * N, B, PS, s);
-#pragma unroll
-      for (int m = 0; m < PPT; ++m) q0[m] = q0[m] + dt / 6.0 * (s[m] + double(k[m]));
-      st<DIR>(qn + v * N, B, PS, q0);
-    } else {
-      if (mode == 0) {
-#pragma unroll
-        for (int m = 0; m < PPT; ++m) s[m] = double(k[m]);
-      } else {
-        ld<DIR>(acc + v * N, B, PS, s);
-#pragma unroll
-        for (int m = 0; m < PPT; ++m) s[m] = s[m] + 2 * double(k[m]);
-      }
-      st<DIR>(acc + v * N, B, PS, s);
-#pragma unroll
-      for (int m = 0; m < PPT; ++m) s[m] = q0[m] + c_stage * double(k[m]);
-      st<DIR>(qs_out + v * N, B, PS, s);
-    }
+    if (mode == 4 && v == 0) continue;  // k_1 is already complete in rhs
+    rk_update<DIR, T>(mode, v, k, B, PS, N, rhs, qn, acc, qs_out, dt);
   }
 }
 

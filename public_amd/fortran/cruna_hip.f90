@@ -31,6 +31,7 @@ module cruna_hip
     real(c_double) :: visc_exp
     real(c_double) :: dt
     integer(c_int) :: filter_freq(3)
+    integer(c_int) :: afilter_freq(3)
     integer(c_int) :: heat_unit_spacing
     integer(c_int) :: precision
     integer(c_int) :: nbc
@@ -112,12 +113,69 @@ module cruna_hip
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function c_rhs_adjoint(ctx, rhs, qs, q, g, t) bind(C, name="cruna_hip_rhs_adjoint")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: rhs(*)
+      real(c_double), intent(in) :: qs(*), q(*)
+      type(c_ptr), value :: g
+      real(c_double), value :: t
+    end function
+    integer(c_int) function c_time_stepper_adjoint(ctx, qs1, qs0, q0, g, t) &
+        bind(C, name="cruna_hip_time_stepper_adjoint")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: qs1(*)
+      real(c_double), intent(in) :: qs0(*), q0(*)
+      type(c_ptr), value :: g
+      real(c_double), value :: t
+    end function
+    integer(c_int) function c_set_adjoint_state(ctx, qs) bind(C, name="cruna_hip_set_adjoint_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: qs(*)
+    end function
+    integer(c_int) function c_get_adjoint_state(ctx, qs) bind(C, name="cruna_hip_get_adjoint_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: qs(*)
+    end function
+    integer(c_int) function c_trajectory_alloc(ctx, nslots) bind(C, name="cruna_hip_trajectory_alloc")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nslots
+    end function
+    integer(c_int) function c_trajectory_set(ctx, nt, q) bind(C, name="cruna_hip_trajectory_set")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nt
+      real(c_double), intent(in) :: q(*)
+    end function
+    integer(c_int) function c_trajectory_get(ctx, nt, q) bind(C, name="cruna_hip_trajectory_get")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nt
+      real(c_double), intent(out) :: q(*)
+    end function
+    integer(c_int) function c_trajectory_record(ctx, on) bind(C, name="cruna_hip_trajectory_record")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function c_run_adjoint(ctx, nt_end, nsteps) bind(C, name="cruna_hip_run_adjoint")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nt_end, nsteps
+    end function
   end interface
 
   public :: cruna_hip_default_cfg
   public :: hip_create, hip_destroy, hip_set_state, hip_get_state, hip_set_qref
   public :: hip_set_sponge_shape, hip_diff, hip_filter, hip_rhs_direct
   public :: hip_time_stepper_direct, hip_run_direct, hip_synchronize
+  public :: hip_rhs_adjoint, hip_time_stepper_adjoint, hip_set_adjoint_state
+  public :: hip_get_adjoint_state, hip_trajectory_alloc, hip_trajectory_set
+  public :: hip_trajectory_get, hip_trajectory_record, hip_run_adjoint
 
 contains
 
@@ -220,6 +278,78 @@ contains
   subroutine hip_synchronize(ctx)
     type(c_ptr), intent(in) :: ctx
     call check(c_synchronize(ctx), "synchronize")
+  end subroutine
+
+  ! right_hand_side_adjoint_3d(rhs, qs_in, q, g, t); g optional
+  subroutine hip_rhs_adjoint(ctx, rhs, qs, q, g, t)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(out), contiguous :: rhs(:,:,:,:)
+    real(c_double), intent(in), contiguous :: qs(:,:,:,:), q(:,:,:,:)
+    real(c_double), intent(in), contiguous, target, optional :: g(:,:,:,:)
+    real(c_double), intent(in) :: t
+    type(c_ptr) :: gp
+    gp = c_null_ptr
+    if (present(g)) gp = c_loc(g)
+    call check(c_rhs_adjoint(ctx, rhs, qs, q, gp, t), "rhs_adjoint")
+  end subroutine
+
+  ! time_stepper_adjoint(qs1, qs0, q0, g, t); g optional
+  subroutine hip_time_stepper_adjoint(ctx, qs1, qs0, q0, g, t)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(out), contiguous :: qs1(:,:,:,:)
+    real(c_double), intent(in), contiguous :: qs0(:,:,:,:), q0(:,:,:,:)
+    real(c_double), intent(in), contiguous, target, optional :: g(:,:,:,:)
+    real(c_double), intent(in) :: t
+    type(c_ptr) :: gp
+    gp = c_null_ptr
+    if (present(g)) gp = c_loc(g)
+    call check(c_time_stepper_adjoint(ctx, qs1, qs0, q0, gp, t), "time_stepper_adjoint")
+  end subroutine
+
+  subroutine hip_set_adjoint_state(ctx, qs)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(in), contiguous :: qs(:,:,:,:)
+    call check(c_set_adjoint_state(ctx, qs), "set_adjoint_state")
+  end subroutine
+
+  subroutine hip_get_adjoint_state(ctx, qs)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(out), contiguous :: qs(:,:,:,:)
+    call check(c_get_adjoint_state(ctx, qs), "get_adjoint_state")
+  end subroutine
+
+  subroutine hip_trajectory_alloc(ctx, nslots)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nslots
+    call check(c_trajectory_alloc(ctx, int(nslots, c_int)), "trajectory_alloc")
+  end subroutine
+
+  ! Q(:,:,:,:,nt) <- q
+  subroutine hip_trajectory_set(ctx, nt, q)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nt
+    real(c_double), intent(in), contiguous :: q(:,:,:,:)
+    call check(c_trajectory_set(ctx, int(nt, c_int), q), "trajectory_set")
+  end subroutine
+
+  subroutine hip_trajectory_get(ctx, nt, q)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nt
+    real(c_double), intent(out), contiguous :: q(:,:,:,:)
+    call check(c_trajectory_get(ctx, int(nt, c_int), q), "trajectory_get")
+  end subroutine
+
+  subroutine hip_trajectory_record(ctx, on)
+    type(c_ptr), intent(in) :: ctx
+    logical, intent(in) :: on
+    call check(c_trajectory_record(ctx, merge(1_c_int, 0_c_int, on)), "trajectory_record")
+  end subroutine
+
+  ! calc_adjoint's loop body for nt = nt_end, nt_end-1, ..., nt_end-nsteps+1
+  subroutine hip_run_adjoint(ctx, nt_end, nsteps)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nt_end, nsteps
+    call check(c_run_adjoint(ctx, int(nt_end, c_int), int(nsteps, c_int)), "run_adjoint")
   end subroutine
 
 end module cruna_hip

@@ -241,3 +241,141 @@ def test_rhs_direct_with_characteristic_rows():
         g.set_qref(qref)
         got = g.right_hand_side_direct_3d(q)
     assert rel(got, ref) < 1e-12
+
+
+# ----------------------------------------------------------------------------
+# adjoint
+
+
+def _adjoint_pair(shape, seed):
+    """A forward state about a periodic box and a smooth random adjoint state."""
+    rng = np.random.default_rng(seed)
+    q = np.zeros(shape + (5,), order="F")
+    q[..., 0] = 1 + 0.05 * rng.random(shape)
+    q[..., 1:4] = 0.3 * rng.standard_normal(shape + (3,))
+    q[..., 4] = 1 / 1.4 + 0.02 * rng.random(shape)
+    qs = np.asfortranarray(rng.standard_normal(shape + (5,)))
+    g = np.asfortranarray(rng.standard_normal(shape + (5,)))
+    return q, qs, g
+
+
+@pytest.mark.parametrize("with_g", [False, True])
+def test_rhs_adjoint_matches_oracle(with_g):
+    Cruna = _gpu()
+    shape, dx = (32, 24, 16), (0.2, 0.26, 0.4)
+    q, qs, g = _adjoint_pair(shape, 11)
+    g = g if with_g else None
+    cfg = O.Cfg(shape, dx, dt=1e-3)
+    ref = O.rhs_adjoint(cfg, qs, q, g=g)
+    with Cruna(shape, dx, dt=1e-3) as ctx:
+        got = ctx.right_hand_side_adjoint_3d(qs, q, g)
+    for v in range(5):
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+
+
+def test_time_stepper_adjoint_matches_oracle():
+    Cruna = _gpu()
+    shape, dx = (32, 16, 24), (0.2, 0.3, 0.25)
+    q, qs, g = _adjoint_pair(shape, 12)
+    cfg = O.Cfg(shape, dx, dt=2e-3)
+    ref = O.time_stepper_adjoint(cfg, qs, q, g=g)
+    with Cruna(shape, dx, dt=2e-3) as ctx:
+        got = ctx.time_stepper_adjoint(qs, q, g)
+    assert rel(got, ref) < 1e-12
+
+
+def test_run_adjoint_matches_oracle():
+    """calc_adjoint over a forward trajectory uploaded slot by slot; adjoint
+    filters every step; and the same trajectory recorded by run_direct."""
+    Cruna = _gpu()
+    n, nsteps = 32, 6
+    kw, h = _kw(0.3, n)
+    q0 = _box(n, 2, 0.3)
+    cfg = O.Cfg((n,) * 3, (h,) * 3, filter_freq=(1, 1, 1), afilter_freq=(1, 1, 1), **kw)
+    Q = np.zeros((n,) * 3 + (5, nsteps), order="F")
+    q = q0
+    for nt in range(1, nsteps + 1):
+        q = O.run_direct(cfg, q, 1, nt0=nt - 1)
+        Q[..., nt - 1] = q
+    qs_T = np.asfortranarray(np.random.default_rng(3).standard_normal((n,) * 3 + (5,)))
+    ref = O.run_adjoint(cfg, qs_T, Q)
+    with Cruna((n,) * 3, (h,) * 3, filter_freq=(1, 1, 1), afilter_freq=(1, 1, 1), **kw) as g:
+        g.trajectory_alloc(nsteps)
+        for nt in range(1, nsteps + 1):
+            g.trajectory_set(nt, Q[..., nt - 1])
+        g.set_adjoint_state(qs_T)
+        g.run_adjoint(nsteps)
+        g.synchronize()
+        got = g.get_adjoint_state()
+        assert rel(got, ref) < 1e-12, rel(got, ref)
+        # forward run on the device recording Q(:,:,:,:,nt), then the adjoint
+        g.trajectory_alloc(nsteps)
+        g.trajectory_record(True)
+        g.set_state(q0)
+        g.run_direct(nsteps)
+        g.synchronize()
+        assert rel(g.trajectory_get(nsteps), Q[..., -1]) < 1e-12
+        g.set_adjoint_state(qs_T)
+        g.run_adjoint(nsteps)
+        g.synchronize()
+        got2 = g.get_adjoint_state()
+    assert rel(got2, ref) < 1e-12, rel(got2, ref)
+
+
+def test_run_adjoint_open_y_with_boundaries():
+    """Adjoint loop with characteristic_qs (201) on both y faces and sponge_qs (401)."""
+    Cruna = _gpu()
+    shape = (32, 48, 16)
+    q, dx, (x, y, z), (Lx, Ly, Lz) = _shear_case(shape, seed=6)
+    n1, n2, n3 = shape
+    bc = [(201, 1, n1, 1, 1, 1, n3, 0, -1, 0),
+          (201, 1, n1, n2, n2, 1, n3, 0, 1, 0),
+          (401, 1, n1, 1, n2, 1, n3)]
+    scheme = (O.PERIODIC, O.OPEN, O.PERIODIC)
+    kw = dict(mu0=2e-3, T0=1.0 / (1.4 * 0.25), dt=2e-3, scheme=scheme, bc=bc)
+    cfg = O.Cfg(shape, dx, filter_freq=(1, 1, 1), afilter_freq=(1, 1, 1), **kw)
+    shape_fn = O.sponge_shape(cfg, (0, -Ly / 2, 0), (Lx, Ly / 2, Lz), (-1.0, 0.2 * Ly, -1.0), 0.05,
+                              np.asfortranarray(x), np.asfortranarray(y), np.asfortranarray(z))
+    nsteps = 4
+    Q = np.zeros(shape + (5, nsteps), order="F")
+    qq = q
+    for nt in range(1, nsteps + 1):
+        qq = O.run_direct(cfg, qq, 1, nt0=nt - 1, qref=q, shape=shape_fn)
+        Q[..., nt - 1] = qq
+    qs_T = np.asfortranarray(np.random.default_rng(4).standard_normal(shape + (5,)))
+    ref = O.run_adjoint(cfg, qs_T, Q, shape=shape_fn)
+    with Cruna(shape, dx, filter_freq=(1, 1, 1), afilter_freq=(1, 1, 1), **kw) as g:
+        g.set_qref(q)
+        g.set_sponge_shape(shape_fn)
+        g.trajectory_alloc(nsteps)
+        for nt in range(1, nsteps + 1):
+            g.trajectory_set(nt, Q[..., nt - 1])
+        g.set_adjoint_state(qs_T)
+        g.run_adjoint(nsteps)
+        g.synchronize()
+        got = g.get_adjoint_state()
+    for v in range(5):
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+    # the rows act on the adjoint
+    ref_free = O.run_adjoint(O.Cfg(shape, dx, filter_freq=(1, 1, 1), afilter_freq=(1, 1, 1),
+                                   **dict(kw, bc=())), qs_T, Q)
+    assert rel(ref_free, ref) > 1e-8
+
+
+def test_adjoint_fp32_mixed_close_to_fp64():
+    from public_amd import FP32_MIXED
+    Cruna = _gpu()
+    shape, dx = (32, 32, 32), (0.2, 0.2, 0.2)
+    q, qs, _ = _adjoint_pair(shape, 13)
+    out = []
+    for prec in (0, FP32_MIXED):
+        with Cruna(shape, dx, dt=1e-3, precision=prec) as ctx:
+            ctx.trajectory_alloc(1)
+            ctx.trajectory_set(1, q)
+            ctx.set_adjoint_state(qs)
+            ctx.run_adjoint(5)
+            ctx.synchronize()
+            out.append(ctx.get_adjoint_state())
+    err = rel(out[1], out[0])
+    print(f"adjoint fp32-mixed vs fp64 after 5 steps: {err:.3e}")
+    assert err < 1e-5
