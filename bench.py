@@ -11,7 +11,9 @@ lives in HBM for the whole timed region.
 
 One JSON line is printed.  `value` is the FP64 throughput in Mpts/s
 (N^3 * steps / s / 1e6); the FP32-mixed form (stencil/filter/RHS in FP32,
-FP64 RK4 accumulator) is reported beside it with its error against FP64.
+FP64 RK4 accumulator) is reported beside it with its error against FP64, and
+the FP64 adjoint loop (over a forward trajectory recorded on the device) with
+its own rate.
 `roofline` describes the dominant kernel (largest share of the timed step):
 its algorithmic bytes per launch (compulsory reads and writes of the sweep)
 over its HIP-event-timed average duration, against 8 TB/s.  `cpu_baseline`
@@ -42,12 +44,18 @@ SWEEP_WORDS = {
     "filter_x": 10.0,
     "filter_y": 10.0,
     "filter_z": 10.0,
+    # adjoint sweeps: q and qs read, rhs written (z) or read-modify-written;
+    # the x sweep also carries the RK4 tail (30, 35, 35, 30 words)
+    "asweep_z": 15.0,
+    "asweep_y": 20.0,
+    "asweep_x_rk": 32.5,
 }
 # words of each sweep that are stage-state / RHS / gradient fields held in the
 # compute precision (the rest are FP64 state or accumulator words)
 SWEEP_COMPUTE_WORDS = {
     "sweep1_x": 13.0, "sweep1_y": 19.0, "sweep1_z": 20.0, "sweep2_z": 13.0, "sweep2_x": 13.0,
     "sweep2_y_rk": 15.0, "filter_x": 0.0, "filter_y": 0.0, "filter_z": 0.0,
+    "asweep_z": 5.0, "asweep_y": 10.0, "asweep_x_rk": 12.5,
 }
 # The step-level byte model of the reference's dependency graph (4624 B per
 # point per FP64 RK4 step incl. filtering; 2672 B in FP32-mixed).
@@ -79,6 +87,30 @@ def run_gpu(q0, n, h, dt, precision, steps, warmup, kw):
     q = g.get_state()
     g.close()
     return (t1 - t0), prof, q
+
+
+def run_adjoint_gpu(q0, n, h, dt, steps, kw):
+    """Forward run recording the trajectory on the device, then the timed
+    adjoint loop over it (adjoint filters every step)."""
+    from public_amd import Cruna
+    g = Cruna((n,) * 3, (h,) * 3, dt=dt, filter_freq=(1, 1, 1), afilter_freq=(1, 1, 1), **kw)
+    g.trajectory_alloc(steps)
+    g.trajectory_record(True)
+    g.set_state(q0)
+    g.run_direct(steps)
+    g.set_adjoint_state(np.asfortranarray(np.random.default_rng(1).standard_normal(q0.shape)))
+    g.synchronize()
+    g.profile(True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run_adjoint(steps)
+    g.synchronize()
+    t1 = time.perf_counter()
+    prof = g.profile_read()
+    g.profile(False)
+    qs = g.get_adjoint_state()
+    g.close()
+    return (t1 - t0), prof, bool(np.isfinite(qs).all())
 
 
 def ke_history(q0, n, h, dt, precision, steps, every, kw):
@@ -156,6 +188,8 @@ def main():
     ap.add_argument("--mach", type=float, default=0.3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--ke-every", type=int, default=20)
+    ap.add_argument("--adjoint-steps", type=int, default=20,
+                    help="adjoint steps timed after a recorded forward run (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -180,6 +214,16 @@ def main():
 
     secs64, prof64, q64 = run_gpu(q0, n, h, dt, FP64, args.steps, args.warmup, kw)
     secs32, prof32, q32 = run_gpu(q0, n, h, dt, FP32_MIXED, args.steps, args.warmup, kw)
+    adjoint = None
+    if args.adjoint_steps > 0:
+        k = args.adjoint_steps
+        secs_a, prof_a, finite = run_adjoint_gpu(q0, n, h, dt, k, kw)
+        adjoint = {"value": round(float(n) ** 3 * k / secs_a / 1e6, 2), "unit": "Mpts/s",
+                   "steps": k, "ms_per_step": round(secs_a / k * 1e3, 4), "finite": finite,
+                   "kernels": {name: {"avg_ms": round(v[0] / v[1], 4), "launches": v[1],
+                                      "GBps": round(kernel_bytes_per_point(name, False) * float(n) ** 3
+                                                    / (v[0] / v[1] / 1e3) / 1e9, 1)}
+                               for name, v in prof_a.items()}}
     pts = float(n) ** 3
     mpts64 = pts * args.steps / secs64 / 1e6
     mpts32 = pts * args.steps / secs32 / 1e6
@@ -254,6 +298,7 @@ def main():
                    "grid": [n, n, n], "dt": dt, "seed": args.seed},
         "fp32_mixed": {"value": round(mpts32, 2), "unit": "Mpts/s", "ms_per_step": round(step_ms32, 4),
                        "speedup_vs_fp64": round(mpts32 / mpts64, 3), "err_vs_fp64": errors},
+        "adjoint_fp64": adjoint,
         "roofline": roofline,
         "step_roofline": step_roofline,
         "kernels_fp64": kernels,

@@ -56,6 +56,9 @@ typedef struct cruna_hip_cfg {
   int afilter_freq[3];   /* filter.ai1freq..ai3freq of the adjoint loop (0 = never) */
   int heat_unit_spacing; /* 1: heat-flux Dx without h, as ns_rho_rhou_p.f90:189-197 */
   int precision;         /* CRUNA_HIP_FP64 / CRUNA_HIP_FP32_MIXED */
+  int tile_lines[3];     /* lines per work-group of the sweeps along each direction:
+                            0 = library choice, else a power of two dividing the
+                            line count with (n/8) * tile_lines <= 512 */
   /* boundary_conditions.dat rows: type; i1s; i1e; i2s; i2e; i3s; i3e; k1; k2; k3 */
   int nbc;
   double bc[CRUNA_HIP_MAX_BC][10];

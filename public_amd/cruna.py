@@ -48,6 +48,7 @@ class Cfg(C.Structure):
         ("afilter_freq", C.c_int * 3),
         ("heat_unit_spacing", C.c_int),
         ("precision", C.c_int),
+        ("tile_lines", C.c_int * 3),
         ("nbc", C.c_int),
         ("bc", (C.c_double * 10) * MAX_BC),
     ]
@@ -123,7 +124,7 @@ class Cruna:
     def __init__(self, n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0, W=1.0,
                  Pr=0.71, mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76, dt=1e-3,
                  filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), heat_unit_spacing=1, precision=FP64,
-                 bc=()):
+                 bc=(), tile_lines=(0, 0, 0)):
         L = lib()
         cfg = Cfg()
         L.cruna_hip_default_cfg(C.byref(cfg))
@@ -133,6 +134,7 @@ class Cruna:
             cfg.scheme[d] = int(scheme[d])
             cfg.filter_freq[d] = int(filter_freq[d])
             cfg.afilter_freq[d] = int(afilter_freq[d])
+            cfg.tile_lines[d] = int(tile_lines[d])
         cfg.filter_alpha = filter_alpha
         cfg.gamma, cfg.R, cfg.W, cfg.Pr, cfg.mu0, cfg.T0 = gamma, R, W, Pr, mu0, T0
         cfg.visc_law, cfg.visc_exp, cfg.dt = visc_law, visc_exp, dt

@@ -118,11 +118,19 @@ int build_geom(cruna_hip_ctx *c, int d) {
   g.n2 = n2;
   g.n3 = n3;
   const long lines_contig = d == 0 ? (long)n2 * n3 : n1;
-  // x-lines are contiguous: 256 threads; y/z tiles take up to 16 neighbouring
-  // x-lines (128-byte rows in FP64) in up to 512 threads
-  const int wg_max = d == 0 ? 256 : NT;
-  int TL = 1;
-  while (g.NB * TL * 2 <= wg_max && lines_contig % (TL * 2) == 0 && (d == 0 || TL < 16)) TL *= 2;
+  int TL = c->cfg.tile_lines[d];
+  if (TL == 0) {
+    // x-lines are contiguous: up to 256 threads; y/z tiles take up to 16
+    // neighbouring x-lines (128-byte rows in FP64) in up to 512 threads
+    const int wg_max = d == 0 ? 256 : NT;
+    TL = 1;
+    while (g.NB * TL * 2 <= wg_max && lines_contig % (TL * 2) == 0 && (d == 0 || TL < 16)) TL *= 2;
+  } else if (TL < 0 || (TL & (TL - 1)) != 0 || lines_contig % TL != 0 || g.NB * TL > NT) {
+    return fail(CRUNA_HIP_EINVAL, "tile_lines[" + std::to_string(d) + "] = " + std::to_string(TL) +
+                                      ": needs a power of two dividing " +
+                                      std::to_string(lines_contig) + " with (n/8)*tile_lines <= " +
+                                      std::to_string(NT));
+  }
   g.TL = TL;
   if (d == 0) {
     g.PS = 1;
@@ -565,7 +573,8 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
       return fail(CRUNA_HIP_EINVAL,
                   "boundary type " + std::to_string(t) + " not implemented on the device");
   }
-  std::unique_ptr<cruna_hip_ctx> c(new cruna_hip_ctx());
+  // partial contexts are released through cruna_hip_destroy on every error return
+  std::unique_ptr<cruna_hip_ctx, int (*)(cruna_hip_ctx *)> c(new cruna_hip_ctx(), cruna_hip_destroy);
   c->cfg = *cfg;
   HIP_TRY(hipGetDevice(&c->device));
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -588,7 +597,7 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
     if (t == 400) c->has_bcrhs = true;
   }
   for (int d = 0; d < 3; ++d) {
-    build_geom(c.get(), d);
+    if (int rc = build_geom(c.get(), d)) return rc;
     try {
       HostLineOp dd = build_line_op(OP_DIFF, cfg->scheme[d] == CRUNA_HIP_PERIODIC, cfg->n[d], 0.0);
       HostLineOp ff =

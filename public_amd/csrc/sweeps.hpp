@@ -427,11 +427,15 @@ __global__ __launch_bounds__(NT) void k_filter(double *__restrict__ q, LineOp<T>
                                                long N, int nv) {
   __shared__ LineLds<T> lds;
   const ThreadPos tp = thread_pos(g);
-  T f[PPT], x[PPT];
-  ld<DIR>(q, tp.base, g.PS, f);
+  double a[PPT];
+  ld<DIR>(q, tp.base, g.PS, a);
   for (int v = 0; v < nv; ++v) {
+    T f[PPT], x[PPT];
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) f[m] = T(a[m]);
+    // the next field's loads fly while this one is filtered
+    if (v + 1 < nv) ld<DIR>(q + (v + 1) * N, tp.base, g.PS, a);
     line_solve<T, PER>(op, lds, tp, f, x);
-    if (v + 1 < nv) ld<DIR>(q + (v + 1) * N, tp.base, g.PS, f);  // next field in flight
     st<DIR>(q + v * N, tp.base, g.PS, x);
   }
 }

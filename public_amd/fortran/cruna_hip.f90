@@ -34,6 +34,7 @@ module cruna_hip
     integer(c_int) :: afilter_freq(3)
     integer(c_int) :: heat_unit_spacing
     integer(c_int) :: precision
+    integer(c_int) :: tile_lines(3)
     integer(c_int) :: nbc
     real(c_double) :: bc(10, CRUNA_HIP_MAX_BC)
   end type cruna_hip_cfg
