@@ -35,12 +35,12 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic FP64-equivalent words per grid point and launch (compulsory
 # reads + writes of each sweep; FP32-mixed fields count as half words).
 SWEEP_WORDS = {
-    "sweep1_x": 13.0,
-    "sweep1_y": 19.0,
-    "sweep1_z": 20.0,
-    "sweep2_z": 13.0,
-    "sweep2_x": 13.0,
-    "sweep2_y_rk": 30.5,  # mean over the four stages: 28, 33, 33, 28
+    "sweep1_x": 14.0,
+    "sweep1_y": 20.0,
+    "sweep1_z": 21.0,
+    "sweep2_z": 12.0,
+    "sweep2_y": 12.0,
+    "sweep2_x_rk": 29.5,  # mean over the four stages: 27, 32, 32, 27
     "filter_x": 10.0,
     "filter_y": 10.0,
     "filter_z": 10.0,
@@ -53,8 +53,8 @@ SWEEP_WORDS = {
 # words of each sweep that are stage-state / RHS / gradient fields held in the
 # compute precision (the rest are FP64 state or accumulator words)
 SWEEP_COMPUTE_WORDS = {
-    "sweep1_x": 13.0, "sweep1_y": 19.0, "sweep1_z": 20.0, "sweep2_z": 13.0, "sweep2_x": 13.0,
-    "sweep2_y_rk": 15.0, "filter_x": 0.0, "filter_y": 0.0, "filter_z": 0.0,
+    "sweep1_x": 14.0, "sweep1_y": 20.0, "sweep1_z": 21.0, "sweep2_z": 12.0, "sweep2_y": 12.0,
+    "sweep2_x_rk": 15.75, "filter_x": 0.0, "filter_y": 0.0, "filter_z": 0.0,
     "asweep_z": 5.0, "asweep_y": 10.0, "asweep_x_rk": 12.5,
 }
 # The step-level byte model of the reference's dependency graph (4624 B per

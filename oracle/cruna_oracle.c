@@ -658,6 +658,10 @@ void orc_sponge_shape(const orc_cfg *c, const double x0[3], const double x1[3],
 /* Direct NS right-hand side, src/equations/ns_rho_rhou_p.f90:31-253.       */
 void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const double *qref,
                     const double *shape, int heat_unit_spacing) {
+  if (c->X_metric) { /* governing_equations => ns_rho_rhou_p_grid_transform */
+    orc_rhs_direct_gt(c, rhs, q_in, c->X_metric, c->X_jacobian, qref, shape);
+    return;
+  }
   const size_t N = npts(c);
   const double dx1 = c->dx[0], dx2 = c->dx[1], dx3 = c->dx[2];
   const double hx1 = heat_unit_spacing ? 1.0 : dx1;
@@ -766,6 +770,228 @@ void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const dou
 
   free(q); free(u); free(ax); free(ay); free(az); free(px); free(py); free(pz);
   free(ux); free(uy); free(uz); free(vx); free(vy); free(vz); free(wx); free(wy); free(wz);
+  free(t11); free(t12); free(t13); free(t22); free(t23); free(t33);
+  free(temp); free(mu); free(mudiv); free(lam); free(tmp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Deformed grids.  X_metric(:,:,:,k,j) is stored at X_metric + N*(k + 3*j)  */
+/* (0-based k, j; Fortran order, k fastest).                                  */
+#define MET(Mp, k, j, N) ((Mp) + (N) * ((size_t)(k) + 3 * (size_t)(j)))
+
+/* src/geometry/grid_metric.f90:16-188 (Visbal & Gaitonde's conservative form
+ * of the metric, with the periodic-direction corrections c1..c3). */
+void orc_calc_metric(const orc_cfg *c, const double *X, const double *X_calc, double *X_metric,
+                     double *X_jacobian) {
+  const size_t N = npts(c);
+  const double c1 = c->scheme[0] == ORC_PERIODIC ? 1.0 : 0.0; /* :41-55 */
+  const double c2 = c->scheme[1] == ORC_PERIODIC ? 1.0 : 0.0;
+  const double c3 = c->scheme[2] == ORC_PERIODIC ? 1.0 : 0.0;
+  const double dx1 = c->dx[0], dx2 = c->dx[1], dx3 = c->dx[2];
+  double *Xt = xalloc(3 * N); /* :68-70 */
+  const double cc[3] = {c1, c2, c3};
+  for (int d = 0; d < 3; ++d)
+    for (size_t i = 0; i < N; ++i) Xt[d * N + i] = X[d * N + i] - cc[d] * X_calc[d * N + i];
+  const double *X1 = Xt, *X2 = Xt + N, *X3 = Xt + 2 * N;
+  double *d11 = xalloc(N), *d12 = xalloc(N), *d13 = xalloc(N); /* dx1dxi1, dx1dxi2, dx1dxi3 */
+  double *d21 = xalloc(N), *d22 = xalloc(N), *d23 = xalloc(N);
+  double *d31 = xalloc(N), *d32 = xalloc(N), *d33 = xalloc(N);
+  double *t1 = xalloc(N), *t2 = xalloc(N), *pr = xalloc(N);
+  /* :86-96 */
+  orc_diff3d(c, 0, d11, X1, dx1);
+  orc_diff3d(c, 0, d21, X2, dx1);
+  orc_diff3d(c, 0, d31, X3, dx1);
+  orc_diff3d(c, 1, d12, X1, dx2);
+  orc_diff3d(c, 1, d22, X2, dx2);
+  orc_diff3d(c, 1, d32, X3, dx2);
+  orc_diff3d(c, 2, d13, X1, dx3);
+  orc_diff3d(c, 2, d23, X2, dx3);
+  orc_diff3d(c, 2, d33, X3, dx3);
+#define PROD(a, b) for (size_t i = 0; i < N; ++i) pr[i] = (a)[i] * (b)[i]
+  double *M;
+  /* m11 :116-118 */
+  PROD(d22, X3); orc_diff3d(c, 2, t1, pr, dx3);
+  PROD(d23, X3); orc_diff3d(c, 1, t2, pr, dx2);
+  M = MET(X_metric, 0, 0, N);
+  for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] + c2 * d33[i] + c3 * d22[i] + c2 * c3;
+  /* m12 :124-126 */
+  PROD(d13, X3); orc_diff3d(c, 1, t1, pr, dx2);
+  PROD(d12, X3); orc_diff3d(c, 2, t2, pr, dx3);
+  M = MET(X_metric, 0, 1, N);
+  for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] - c3 * d12[i];
+  /* m13 :132-134 */
+  PROD(d12, X2); orc_diff3d(c, 2, t1, pr, dx3);
+  PROD(d13, X2); orc_diff3d(c, 1, t2, pr, dx2);
+  M = MET(X_metric, 0, 2, N);
+  for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] - c2 * d13[i];
+  /* m21 :139-141 */
+  PROD(d23, X3); orc_diff3d(c, 0, t1, pr, dx1);
+  PROD(d21, X3); orc_diff3d(c, 2, t2, pr, dx3);
+  M = MET(X_metric, 1, 0, N);
+  for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] - c3 * d21[i];
+  /* m22 :147-149 */
+  PROD(d11, X3); orc_diff3d(c, 2, t1, pr, dx3);
+  PROD(d13, X3); orc_diff3d(c, 0, t2, pr, dx1);
+  M = MET(X_metric, 1, 1, N);
+  for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] + c1 * d33[i] + c3 * d11[i] + c1 * c3;
+  /* m23 :155-157 */
+  PROD(d13, X2); orc_diff3d(c, 0, t1, pr, dx1);
+  PROD(d11, X2); orc_diff3d(c, 2, t2, pr, dx3);
+  M = MET(X_metric, 1, 2, N);
+  for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] - c1 * d23[i];
+  /* m31 :162-164 */
+  PROD(X3, d21); orc_diff3d(c, 1, t1, pr, dx2);
+  PROD(X3, d22); orc_diff3d(c, 0, t2, pr, dx1);
+  M = MET(X_metric, 2, 0, N);
+  for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] - c2 * d31[i];
+  /* m32 :169-171 */
+  PROD(d12, X3); orc_diff3d(c, 0, t1, pr, dx1);
+  PROD(d11, X3); orc_diff3d(c, 1, t2, pr, dx2);
+  M = MET(X_metric, 2, 1, N);
+  for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] - c1 * d32[i];
+  /* m33 :176-178 */
+  PROD(d11, X2); orc_diff3d(c, 1, t1, pr, dx2);
+  PROD(d12, X2); orc_diff3d(c, 0, t2, pr, dx1);
+  M = MET(X_metric, 2, 2, N);
+  for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] + c1 * d22[i] + d11[i] * c2 + c1 * c2;
+#undef PROD
+  /* Jacobian :182-185 */
+  const double *M31 = MET(X_metric, 2, 0, N), *M32 = MET(X_metric, 2, 1, N),
+               *M33 = MET(X_metric, 2, 2, N);
+  for (size_t i = 0; i < N; ++i) {
+    X_jacobian[i] = M31[i] * d13[i] + M32[i] * d23[i] + M33[i] * (d33[i] + c3);
+    X_jacobian[i] = 1.0 / X_jacobian[i];
+  }
+  free(Xt); free(d11); free(d12); free(d13); free(d21); free(d22); free(d23);
+  free(d31); free(d32); free(d33); free(t1); free(t2); free(pr);
+}
+
+/* Direct RHS on a deformed grid, src/equations/ns_rho_rhou_p_grid_transform.f90:31-330.
+ * Gradients are formed in physical coordinates without the Jacobian,
+ * fluxes with the contravariant velocity u_hat_k = sum_j M(k,j) u_j, and
+ * every equation is multiplied by X_jacobian at the end; the heat flux's
+ * outer derivatives take no h (:252-268). */
+void orc_rhs_direct_gt(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
+                       const double *Xj, const double *qref, const double *shape) {
+  const size_t N = npts(c);
+  const double dxs[3] = {c->dx[0], c->dx[1], c->dx[2]};
+  double *q = xalloc(5 * N), *u = xalloc(3 * N), *uh = xalloc(3 * N);
+  double *a[3] = {xalloc(N), xalloc(N), xalloc(N)};
+  double *g[3][3]; /* g[i][j] = d(u_i)/dx_j without Jacobian */
+  double *pg[3] = {xalloc(N), xalloc(N), xalloc(N)};
+  double *tg[3] = {xalloc(N), xalloc(N), xalloc(N)};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) g[i][j] = xalloc(N);
+  double *tau[3][3];
+  double *t11 = xalloc(N), *t12 = xalloc(N), *t13 = xalloc(N);
+  double *t22 = xalloc(N), *t23 = xalloc(N), *t33 = xalloc(N);
+  tau[0][0] = t11; tau[0][1] = t12; tau[0][2] = t13;
+  tau[1][0] = t12; tau[1][1] = t22; tau[1][2] = t23;
+  tau[2][0] = t13; tau[2][1] = t23; tau[2][2] = t33;
+  double *temp = xalloc(N), *mu = xalloc(N), *mudiv = xalloc(N), *lam = xalloc(N);
+  double *tmp = xalloc(N);
+  double *r[5];
+  for (int v = 0; v < 5; ++v) r[v] = rhs + v * N;
+
+  memcpy(q, q_in, 5 * N * sizeof(double)); /* :102 */
+  orc_bc_q(c, q, qref, shape);              /* :105 */
+  for (int m = 0; m < 3; ++m) /* :108-112 */
+    for (size_t i = 0; i < N; ++i) u[m * N + i] = q[(m + 1) * N + i] / q[i];
+  /* gradient of f in physical coordinates without Jacobian: out_j = sum_k M(k,j) D_k f */
+#define PHYS_GRAD(f, out)                                                                     \
+  do {                                                                                        \
+    for (int k_ = 0; k_ < 3; ++k_) orc_diff3d(c, k_, a[k_], (f), dxs[k_]);                    \
+    for (int j_ = 0; j_ < 3; ++j_) {                                                          \
+      const double *m0 = MET(Xm, 0, j_, N), *m1 = MET(Xm, 1, j_, N), *m2 = MET(Xm, 2, j_, N); \
+      for (size_t i = 0; i < N; ++i)                                                          \
+        (out)[j_][i] = m0[i] * a[0][i] + m1[i] * a[1][i] + m2[i] * a[2][i];                   \
+    }                                                                                         \
+  } while (0)
+  for (int m = 0; m < 3; ++m) PHYS_GRAD(u + m * N, g[m]); /* :116-135 */
+  for (int k = 0; k < 3; ++k) { /* :139-141 */
+    const double *m0 = MET(Xm, k, 0, N), *m1 = MET(Xm, k, 1, N), *m2 = MET(Xm, k, 2, N);
+    for (size_t i = 0; i < N; ++i)
+      uh[k * N + i] = m0[i] * u[i] + m1[i] * u[N + i] + m2[i] * u[2 * N + i];
+  }
+  PHYS_GRAD(q + 4 * N, pg); /* :144-149 */
+  /* friction :152-165 */
+  for (size_t i = 0; i < N; ++i) {
+    temp[i] = orc_T(c, q[i], q[4 * N + i]);
+    mu[i] = orc_mu(c, temp[i]);
+    mudiv[i] = -2.0 / 3.0 * mu[i] * (g[0][0][i] + g[1][1][i] + g[2][2][i]);
+    t11[i] = mudiv[i] + 2 * mu[i] * g[0][0][i];
+    t22[i] = mudiv[i] + 2 * mu[i] * g[1][1][i];
+    t33[i] = mudiv[i] + 2 * mu[i] * g[2][2][i];
+    t12[i] = mu[i] * (g[0][1][i] + g[1][0][i]);
+    t13[i] = mu[i] * (g[0][2][i] + g[2][0][i]);
+    t23[i] = mu[i] * (g[1][2][i] + g[2][1][i]);
+  }
+  /* heat :168-182 */
+  for (size_t i = 0; i < N; ++i) lam[i] = orc_lambda(c, mu[i]);
+  PHYS_GRAD(temp, tg);
+  for (size_t i = 0; i < N; ++i) {
+    lam[i] = lam[i] * Xj[i];
+    tg[0][i] = lam[i] * tg[0][i];
+    tg[1][i] = lam[i] * tg[1][i];
+    tg[2][i] = lam[i] * tg[2][i];
+  }
+#undef PHYS_GRAD
+  /* mass :184-190 */
+  for (int k = 0; k < 3; ++k) {
+    for (size_t i = 0; i < N; ++i) tmp[i] = q[i] * uh[k * N + i];
+    orc_diff3d(c, k, a[k], tmp, dxs[k]);
+  }
+  for (size_t i = 0; i < N; ++i) {
+    r[0][i] = -a[0][i] - a[1][i] - a[2][i];
+    r[0][i] = Xj[i] * r[0][i];
+  }
+  /* momentum :192-241 */
+  for (int m = 0; m < 3; ++m) {
+    for (int k = 0; k < 3; ++k) {
+      for (size_t i = 0; i < N; ++i) tmp[i] = q[(m + 1) * N + i] * uh[k * N + i];
+      orc_diff3d(c, k, a[k], tmp, dxs[k]);
+    }
+    for (size_t i = 0; i < N; ++i) r[m + 1][i] = -a[0][i] - a[1][i] - a[2][i] - pg[m][i];
+    for (int k = 0; k < 3; ++k) { /* tau_proj_mk = sum_j M(k,j) tau_mj */
+      const double *m0 = MET(Xm, k, 0, N), *m1 = MET(Xm, k, 1, N), *m2 = MET(Xm, k, 2, N);
+      for (size_t i = 0; i < N; ++i)
+        tmp[i] = m0[i] * tau[m][0][i] + m1[i] * tau[m][1][i] + m2[i] * tau[m][2][i];
+      orc_diff3d(c, k, a[k], tmp, dxs[k]);
+    }
+    for (size_t i = 0; i < N; ++i) {
+      r[m + 1][i] = r[m + 1][i] + a[0][i] + a[1][i] + a[2][i];
+      r[m + 1][i] = Xj[i] * r[m + 1][i];
+    }
+  }
+  /* pressure :243-276 */
+  for (int k = 0; k < 3; ++k) {
+    for (size_t i = 0; i < N; ++i) tmp[i] = q[4 * N + i] * uh[k * N + i];
+    orc_diff3d(c, k, a[k], tmp, dxs[k]);
+  }
+  const double gt = c->gamma / (c->gamma - 1.0);
+  for (size_t i = 0; i < N; ++i) {
+    r[4][i] = gt * (-a[0][i] - a[1][i] - a[2][i]);
+    r[4][i] = r[4][i] + u[i] * pg[0][i] + u[N + i] * pg[1][i] + u[2 * N + i] * pg[2][i];
+  }
+  for (int j = 0; j < 3; ++j) { /* heat flux in x_j (:252-268): outer Dx without h */
+    for (int k = 0; k < 3; ++k) orc_diff3d(c, k, a[k], tg[j], 1.0);
+    const double *m0 = MET(Xm, 0, j, N), *m1 = MET(Xm, 1, j, N), *m2 = MET(Xm, 2, j, N);
+    for (size_t i = 0; i < N; ++i)
+      r[4][i] = r[4][i] + m0[i] * a[0][i] + m1[i] * a[1][i] + m2[i] * a[2][i];
+  }
+  for (size_t i = 0; i < N; ++i) { /* :270-276 */
+    r[4][i] = r[4][i] + Xj[i] * (t11[i] * g[0][0][i] + t12[i] * g[0][1][i] + t13[i] * g[0][2][i]);
+    r[4][i] = r[4][i] + Xj[i] * (t12[i] * g[1][0][i] + t22[i] * g[1][1][i] + t23[i] * g[1][2][i]);
+    r[4][i] = r[4][i] + Xj[i] * (t13[i] * g[2][0][i] + t23[i] * g[2][1][i] + t33[i] * g[2][2][i]);
+    r[4][i] = Xj[i] * (c->gamma - 1.0) * r[4][i];
+  }
+  /* force (no-op), in-rhs sponge (empty), BC rhs :278-284 */
+  orc_bc_rhs(c, rhs, q, qref, shape);
+
+  free(q); free(u); free(uh);
+  for (int k = 0; k < 3; ++k) { free(a[k]); free(pg[k]); free(tg[k]); }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) free(g[i][j]);
   free(t11); free(t12); free(t13); free(t22); free(t23); free(t33);
   free(temp); free(mu); free(mudiv); free(lam); free(tmp);
 }

@@ -40,6 +40,11 @@ typedef struct orc_cfg {
   /* boundary_conditions.dat rows: type; i1s; i1e; i2s; i2e; i3s; i3e; k1; k2; k3 (1-based, inclusive) */
   int nbc;
   double bc[ORC_MAX_BC][10];
+  /* Deformed grid (data_geom X_metric(n1,n2,n3,3,3), X_jacobian(n1,n2,n3)); when
+   * X_metric is set the direct RHS is ns_rho_rhou_p_grid_transform's, as a
+   * case selects it through governing_equations.  NULL: Cartesian. */
+  const double *X_metric;
+  const double *X_jacobian;
 } orc_cfg;
 
 /* LAPACK dgttrf/dgttrs restatement (no transpose), 1 RHS at a time. */
@@ -59,6 +64,17 @@ void orc_filter4d(const orc_cfg *c, int dir, double *q, int nv);
 double orc_T(const orc_cfg *c, double rho, double p);
 double orc_mu(const orc_cfg *c, double T);
 double orc_lambda(const orc_cfg *c, double mu);
+
+/* Metric of a deformed grid, src/geometry/grid_metric.f90:16-188: X is the
+ * physical grid, X_calc the calculation space, both (n1,n2,n3,3); outputs
+ * X_metric(n1,n2,n3,3,3) (metric without the Jacobian) and X_jacobian. */
+void orc_calc_metric(const orc_cfg *c, const double *X, const double *X_calc, double *X_metric,
+                     double *X_jacobian);
+
+/* Direct RHS on a deformed grid, ns_rho_rhou_p_grid_transform.f90:31-330. */
+void orc_rhs_direct_gt(const orc_cfg *c, double *rhs, const double *q_in,
+                       const double *X_metric, const double *X_jacobian, const double *qref,
+                       const double *shape);
 
 /* Right-hand sides.  qref/shape may be NULL when no BC rows use them. */
 void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in,

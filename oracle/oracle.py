@@ -38,6 +38,8 @@ class OrcCfg(C.Structure):
         ("afilter_freq", C.c_int * 3),
         ("nbc", C.c_int),
         ("bc", (C.c_double * 10) * MAX_BC),
+        ("X_metric", C.c_void_p),
+        ("X_jacobian", C.c_void_p),
     ]
 
 
@@ -71,6 +73,8 @@ def lib():
         L.orc_lambda.argtypes = [P, C.c_double]
         L.orc_lambda.restype = C.c_double
         L.orc_rhs_direct.argtypes = [P, _dp, _dp, _vp, _vp, C.c_int]
+        L.orc_calc_metric.argtypes = [P, _dp, _dp, _dp, _dp]
+        L.orc_rhs_direct_gt.argtypes = [P, _dp, _dp, _dp, _dp, _vp, _vp]
         L.orc_rhs_adjoint.argtypes = [P, _dp, _dp, _dp, _vp, _vp]
         L.orc_bc_q.argtypes = [P, _dp, _vp, _vp]
         L.orc_bc_qs.argtypes = [P, _dp, _dp, _vp]
@@ -114,7 +118,9 @@ class Cfg:
 
     def __init__(self, n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0,
                  W=1.0, Pr=0.71, mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76,
-                 dt=1e-3, filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), bc=()):
+                 dt=1e-3, filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), bc=(), metric=None):
+        """metric: (X_metric (n1,n2,n3,3,3), X_jacobian (n1,n2,n3)) of a deformed
+        grid; the direct RHS is then the grid-transform one."""
         c = OrcCfg()
         for d in range(3):
             c.n[d] = int(n[d])
@@ -129,6 +135,13 @@ class Cfg:
         for r, row in enumerate(bc):
             for j in range(10):
                 c.bc[r][j] = float(row[j]) if j < len(row) else 0.0
+        self._metric = None
+        if metric is not None:
+            Xm, Xj = _f(metric[0]), _f(metric[1])
+            assert Xm.shape == tuple(n) + (3, 3) and Xj.shape == tuple(n)
+            self._metric = (Xm, Xj)  # kept alive for the pointers below
+            c.X_metric = Xm.ctypes.data
+            c.X_jacobian = Xj.ctypes.data
         self.c = c
 
     @property
@@ -184,6 +197,24 @@ def rhs_adjoint(cfg, qs, q, g=None, shape=None):
     g = None if g is None else _f(g)
     shape = None if shape is None else _f(shape)
     lib().orc_rhs_adjoint(cfg.ref(), rhs, qs, q, _ptr(g), _ptr(shape))
+    return rhs
+
+
+def calc_metric(cfg, X, X_calc):
+    """grid_metric::calc_metric: X, X_calc (n1,n2,n3,3) -> (X_metric (n1,n2,n3,3,3), X_jacobian)."""
+    X, X_calc = _f(X), _f(X_calc)
+    Xm = np.zeros(cfg.shape + (3, 3), order="F")
+    Xj = np.zeros(cfg.shape, order="F")
+    lib().orc_calc_metric(cfg.ref(), X, X_calc, Xm, Xj)
+    return Xm, Xj
+
+
+def rhs_direct_gt(cfg, q, X_metric, X_jacobian, qref=None, shape=None):
+    q = _f(q)
+    rhs = np.zeros_like(q, order="F")
+    qref = None if qref is None else _f(qref)
+    shape = None if shape is None else _f(shape)
+    lib().orc_rhs_direct_gt(cfg.ref(), rhs, q, _f(X_metric), _f(X_jacobian), _ptr(qref), _ptr(shape))
     return rhs
 
 

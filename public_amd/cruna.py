@@ -74,6 +74,7 @@ def lib():
         L.cruna_hip_get_state.argtypes = [vp, dp]
         L.cruna_hip_set_qref.argtypes = [vp, dp]
         L.cruna_hip_set_sponge_shape.argtypes = [vp, dp]
+        L.cruna_hip_set_metric.argtypes = [vp, dp, dp]
         L.cruna_hip_diff.argtypes = [vp, ip, dp, dp, C.c_double]
         L.cruna_hip_filter.argtypes = [vp, ip, dp, ip]
         L.cruna_hip_rhs_direct.argtypes = [vp, dp, dp, C.c_double]
@@ -96,7 +97,7 @@ def lib():
                   "filter", "rhs_direct", "rhs_adjoint", "time_stepper_direct", "time_stepper_adjoint",
                   "run_direct", "synchronize", "profile_enable", "profile_read", "profile_count",
                   "set_adjoint_state", "get_adjoint_state", "trajectory_alloc", "trajectory_set",
-                  "trajectory_get", "trajectory_record", "run_adjoint"):
+                  "trajectory_get", "trajectory_record", "run_adjoint", "set_metric"):
             getattr(L, "cruna_hip_" + f).restype = C.c_int
         _lib = L
     return _lib
@@ -249,6 +250,14 @@ class Cruna:
     def set_sponge_shape(self, shape):
         shape = _f64(shape, self.shape3)
         _check(lib().cruna_hip_set_sponge_shape(self._h, _dp(shape)))
+
+    def set_metric(self, X_metric, X_jacobian):
+        """Deformed grid: X_metric (n1,n2,n3,3,3) and X_jacobian (n1,n2,n3) of calc_metric."""
+        Xm = np.asfortranarray(X_metric, dtype=np.float64)
+        if Xm.shape != self.shape3 + (3, 3):
+            raise CrunaError(f"X_metric shape {Xm.shape} does not match {self.shape3 + (3, 3)}")
+        Xj = _f64(X_jacobian, self.shape3)
+        _check(lib().cruna_hip_set_metric(self._h, _dp(Xm), _dp(Xj)))
 
     def run_direct(self, nsteps, nt0=0):
         _check(lib().cruna_hip_run_direct(self._h, int(nt0), int(nsteps)))

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(NT) void k_asweep(const Q *qs, const double *__rest
                                                T *qs_out, double dt) {
   constexpr bool FIRST = DIR == 2, LAST = DIR == 0;
   __shared__ LineLds<T> lds;
-  const ThreadPos tp = thread_pos(g);
+  const ThreadPos tp = thread_pos<DIR>(g);
   const long B = tp.base, PS = g.PS;
   const T ih = T(inv_h), gam = T(gamma), gm1 = T(gamma - 1.0);
   T ud[PPT], irho[PPT], S[PPT], R5[PPT], f[PPT], x[PPT], r[PPT];

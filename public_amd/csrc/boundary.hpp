@@ -85,21 +85,25 @@ __global__ __launch_bounds__(256) void k_bc_sponge_q(Q *__restrict__ q,
 }
 
 // sponge_rhs with qref (sponge_layer_core.f90:34-66): rhs -= sigma (q - qref).
-// The pressure slot of the accumulator holds the bracket that the last sweep
-// multiplies by (gamma - 1), so its share enters divided by (gamma - 1).
+// The accumulator is finished by the last sweep, which multiplies the
+// pressure slot by (gamma - 1) and, on a deformed grid, every slot by the
+// Jacobian J; the sponge acts on the finished RHS, so its share enters
+// divided by those factors (J null: Cartesian grid).
 template <typename T, typename Q>
 __global__ __launch_bounds__(256) void k_sponge_rhs(T *__restrict__ rhs, const Q *__restrict__ q,
                                                      const double *__restrict__ qref,
                                                      const double *__restrict__ sigma, long N,
-                                                     double gm1) {
+                                                     double gm1, const T *__restrict__ J) {
   const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= N) return;
   const double s = sigma[o];
   if (s == 0.0) return;
+  const double jac = J ? double(J[o]) : 1.0;
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     double d = s * (double(q[v * N + o]) - qref[v * N + o]);
     if (v == 4) d = d / gm1;
+    if (J) d = d / jac;
     rhs[v * N + o] = T(double(rhs[v * N + o]) - d);
   }
 }

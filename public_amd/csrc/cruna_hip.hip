@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -34,11 +35,11 @@ int fail(int code, const std::string &msg) {
   } while (0)
 
 enum KernelId {
-  KID_S1X = 0, KID_S2Y, KID_S3Z, KID_S4Z, KID_S5X, KID_S6Y, KID_FX, KID_FY, KID_FZ, KID_DIFF,
+  KID_S1X = 0, KID_S2Y, KID_S3Z, KID_S4Z, KID_S5Y, KID_S6X, KID_FX, KID_FY, KID_FZ, KID_DIFF,
   KID_AX, KID_AY, KID_AZ, KID_COUNT
 };
 const char *kKernelNames[KID_COUNT] = {"sweep1_x", "sweep1_y",    "sweep1_z", "sweep2_z",
-                                       "sweep2_x", "sweep2_y_rk", "filter_x", "filter_y",
+                                       "sweep2_y", "sweep2_x_rk", "filter_x", "filter_y",
                                        "filter_z", "diff",        "asweep_x_rk", "asweep_y",
                                        "asweep_z"};
 
@@ -92,6 +93,11 @@ struct cruna_hip_ctx {
   // boundary_conditions.dat rows in file order (types 201, 400, 401)
   std::vector<std::pair<int, BoxRow>> bc;
   bool has_bcq = false, has_bcrhs = false;
+  // deformed grid (cruna_hip_set_metric): per-line metric entries and Jacobian
+  void *met = nullptr;
+  Metric<double> mt64{};
+  Metric<float> mt32{};
+  bool gt = false;
   // adjoint state qs^n and the forward trajectory Q(:,:,:,:,nt) (slot nt-1)
   double *qa = nullptr, *traj = nullptr, *gobj = nullptr;
   int traj_slots = 0;
@@ -225,6 +231,13 @@ DevOp<double> *fops<double>(cruna_hip_ctx *c) { return c->fop64; }
 template <>
 DevOp<float> *fops<float>(cruna_hip_ctx *c) { return c->fop32; }
 
+template <typename T>
+Metric<T> metric(cruna_hip_ctx *c);
+template <>
+Metric<double> metric<double>(cruna_hip_ctx *c) { return c->mt64; }
+template <>
+Metric<float> metric<float>(cruna_hip_ctx *c) { return c->mt32; }
+
 int need_bc_inputs(cruna_hip_ctx *c) {
   for (auto &r : c->bc) {
     if (!c->qref) return fail(CRUNA_HIP_EINVAL, "boundary rows need qref (cruna_hip_set_qref)");
@@ -262,95 +275,84 @@ int bc_rhs(cruna_hip_ctx *c, T *rhs, const Q *q) {
     if (r.first == 400) {
       hipLaunchKernelGGL((k_sponge_rhs<T, Q>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream,
                          rhs, q, (const double *)c->qref, (const double *)c->shape, c->N,
-                         c->cfg.gamma - 1.0);
+                         c->cfg.gamma - 1.0, c->gt ? metric<T>(c).J : (const T *)nullptr);
       if (int rc = check_launch()) return rc;
     }
   return CRUNA_HIP_OK;
 }
 
-// One RHS evaluation (mode 4) or RK stage (mode 0..3) from stage input `in`.
-template <typename T, typename Q>
-int rhs_stage(cruna_hip_ctx *c, const Q *in, int mode) {
-  const long N = c->N;
-  T *rhs = (T *)c->rhs, *G = (T *)c->G, *qs = (T *)c->qs;
-  DevOp<T> *D = dops<T>(c);
-  {
-    ProfScope p(c, KID_S1X);
-    if (c->cfg.scheme[0] == CRUNA_HIP_PERIODIC)
-      hipLaunchKernelGGL((k_sweep1<T, Q, 0, true>), dim3(c->geom[0].ntiles), dim3(c->wg[0]), 0,
-                         c->stream, in, rhs, G, D[0].op, c->geom[0], c->phys, N);
-    else
-      hipLaunchKernelGGL((k_sweep1<T, Q, 0, false>), dim3(c->geom[0].ntiles), dim3(c->wg[0]), 0,
-                         c->stream, in, rhs, G, D[0].op, c->geom[0], c->phys, N);
-  }
-  if (int rc = check_launch()) return rc;
-  {
-    ProfScope p(c, KID_S2Y);
-    if (c->cfg.scheme[1] == CRUNA_HIP_PERIODIC)
-      hipLaunchKernelGGL((k_sweep1<T, Q, 1, true>), dim3(c->geom[1].ntiles), dim3(c->wg[1]), 0,
-                         c->stream, in, rhs, G, D[1].op, c->geom[1], c->phys, N);
-    else
-      hipLaunchKernelGGL((k_sweep1<T, Q, 1, false>), dim3(c->geom[1].ntiles), dim3(c->wg[1]), 0,
-                         c->stream, in, rhs, G, D[1].op, c->geom[1], c->phys, N);
-  }
-  if (int rc = check_launch()) return rc;
-  {
-    ProfScope p(c, KID_S3Z);
-    if (c->cfg.scheme[2] == CRUNA_HIP_PERIODIC)
-      hipLaunchKernelGGL((k_sweep1<T, Q, 2, true>), dim3(c->geom[2].ntiles), dim3(c->wg[2]), 0,
-                         c->stream, in, rhs, G, D[2].op, c->geom[2], c->phys, N);
-    else
-      hipLaunchKernelGGL((k_sweep1<T, Q, 2, false>), dim3(c->geom[2].ntiles), dim3(c->wg[2]), 0,
-                         c->stream, in, rhs, G, D[2].op, c->geom[2], c->phys, N);
-  }
-  if (int rc = check_launch()) return rc;
-  {
-    ProfScope p(c, KID_S4Z);
-    if (c->cfg.scheme[2] == CRUNA_HIP_PERIODIC)
-      hipLaunchKernelGGL((k_sweep2<T, Q, 2, true>), dim3(c->geom[2].ntiles), dim3(c->wg[2]), 0,
-                         c->stream, in, rhs, (const T *)G, D[2].op, c->geom[2], c->phys, N, mode,
-                         c->qn, c->acc, qs, c->cfg.dt);
-    else
-      hipLaunchKernelGGL((k_sweep2<T, Q, 2, false>), dim3(c->geom[2].ntiles), dim3(c->wg[2]), 0,
-                         c->stream, in, rhs, (const T *)G, D[2].op, c->geom[2], c->phys, N, mode,
-                         c->qn, c->acc, qs, c->cfg.dt);
-  }
-  if (int rc = check_launch()) return rc;
-  {
-    ProfScope p(c, KID_S5X);
-    if (c->cfg.scheme[0] == CRUNA_HIP_PERIODIC)
-      hipLaunchKernelGGL((k_sweep2<T, Q, 0, true>), dim3(c->geom[0].ntiles), dim3(c->wg[0]), 0,
-                         c->stream, in, rhs, (const T *)G, D[0].op, c->geom[0], c->phys, N, mode,
-                         c->qn, c->acc, qs, c->cfg.dt);
-    else
-      hipLaunchKernelGGL((k_sweep2<T, Q, 0, false>), dim3(c->geom[0].ntiles), dim3(c->wg[0]), 0,
-                         c->stream, in, rhs, (const T *)G, D[0].op, c->geom[0], c->phys, N, mode,
-                         c->qn, c->acc, qs, c->cfg.dt);
-  }
-  if (int rc = check_launch()) return rc;
-  if (c->has_bcrhs)
-    if (int rc = bc_rhs<T, Q>(c, rhs, in)) return rc;
-  {
-    ProfScope p(c, KID_S6Y);
-    if (c->cfg.scheme[1] == CRUNA_HIP_PERIODIC)
-      hipLaunchKernelGGL((k_sweep2<T, Q, 1, true>), dim3(c->geom[1].ntiles), dim3(c->wg[1]), 0,
-                         c->stream, in, rhs, (const T *)G, D[1].op, c->geom[1], c->phys, N, mode,
-                         c->qn, c->acc, qs, c->cfg.dt);
-    else
-      hipLaunchKernelGGL((k_sweep2<T, Q, 1, false>), dim3(c->geom[1].ntiles), dim3(c->wg[1]), 0,
-                         c->stream, in, rhs, (const T *)G, D[1].op, c->geom[1], c->phys, N, mode,
-                         c->qn, c->acc, qs, c->cfg.dt);
-  }
+template <typename T, typename Q, int DIR, bool GT>
+void launch_sweep1_gt(cruna_hip_ctx *c, const Q *in) {
+  const dim3 grid(c->geom[DIR].ntiles), block(c->wg[DIR]);
+  T *rhs = (T *)c->rhs, *G = (T *)c->G;
+  const LineOp<T> &op = dops<T>(c)[DIR].op;
+  if (c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC)
+    hipLaunchKernelGGL((k_sweep1<T, Q, DIR, true, GT>), grid, block, 0, c->stream, in, rhs, G, op,
+                       c->geom[DIR], c->phys, c->N, metric<T>(c));
+  else
+    hipLaunchKernelGGL((k_sweep1<T, Q, DIR, false, GT>), grid, block, 0, c->stream, in, rhs, G, op,
+                       c->geom[DIR], c->phys, c->N, metric<T>(c));
+}
+
+template <typename T, typename Q, int DIR>
+int launch_sweep1(cruna_hip_ctx *c, const Q *in) {
+  ProfScope p(c, DIR == 0 ? KID_S1X : (DIR == 1 ? KID_S2Y : KID_S3Z));
+  if (c->gt)
+    launch_sweep1_gt<T, Q, DIR, true>(c, in);
+  else
+    launch_sweep1_gt<T, Q, DIR, false>(c, in);
   return check_launch();
+}
+
+template <typename T, int DIR, bool GT>
+void launch_sweep2_gt(cruna_hip_ctx *c, int mode, bool fuse_filter) {
+  const dim3 grid(c->geom[DIR].ntiles), block(c->wg[DIR]);
+  T *rhs = (T *)c->rhs, *qs = (T *)c->qs;
+  const T *G = (const T *)c->G;
+  const LineOp<T> &op = dops<T>(c)[DIR].op, &fop = fops<T>(c)[DIR].op;
+  const int ff = fuse_filter ? 1 : 0;
+  if (c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC)
+    hipLaunchKernelGGL((k_sweep2<T, DIR, true, GT>), grid, block, 0, c->stream, rhs, G, op, fop,
+                       c->geom[DIR], c->phys, c->N, mode, ff, c->qn, c->acc, qs, c->cfg.dt,
+                       metric<T>(c));
+  else
+    hipLaunchKernelGGL((k_sweep2<T, DIR, false, GT>), grid, block, 0, c->stream, rhs, G, op, fop,
+                       c->geom[DIR], c->phys, c->N, mode, ff, c->qn, c->acc, qs, c->cfg.dt,
+                       metric<T>(c));
+}
+
+template <typename T, int DIR>
+int launch_sweep2(cruna_hip_ctx *c, int mode, bool fuse_filter) {
+  ProfScope p(c, DIR == 2 ? KID_S4Z : (DIR == 1 ? KID_S5Y : KID_S6X));
+  if (c->gt)
+    launch_sweep2_gt<T, DIR, true>(c, mode, fuse_filter);
+  else
+    launch_sweep2_gt<T, DIR, false>(c, mode, fuse_filter);
+  return check_launch();
+}
+
+// One RHS evaluation (mode 4) or RK stage (mode 0..3) from stage input `in`:
+// first-derivative sweeps x, y, z, viscous sweeps z, y, x (the last one
+// finishes the stage).  fuse_filter: apply filter_i1 inside the final stage.
+template <typename T, typename Q>
+int rhs_stage(cruna_hip_ctx *c, const Q *in, int mode, bool fuse_filter = false) {
+  if (int rc = launch_sweep1<T, Q, 0>(c, in)) return rc;
+  if (int rc = launch_sweep1<T, Q, 1>(c, in)) return rc;
+  if (int rc = launch_sweep1<T, Q, 2>(c, in)) return rc;
+  if (int rc = launch_sweep2<T, 2>(c, mode, false)) return rc;
+  if (int rc = launch_sweep2<T, 1>(c, mode, false)) return rc;
+  if (c->has_bcrhs)
+    if (int rc = bc_rhs<T, Q>(c, (T *)c->rhs, in)) return rc;
+  return launch_sweep2<T, 0>(c, mode, fuse_filter);
 }
 
 // A stage from the FP64 state `from` (stage 0, or a single RHS evaluation) or,
 // with from == nullptr, from the stage input already in qs.  With q-type
 // boundary rows the RHS works on a private copy (ns_rho_rhou_p.f90:89-92).
 template <typename T>
-int run_stage(cruna_hip_ctx *c, const double *from, int mode) {
+int run_stage(cruna_hip_ctx *c, const double *from, int mode, bool fuse_filter = false) {
   T *qs = (T *)c->qs;
-  if (from && !c->has_bcq) return rhs_stage<T, double>(c, from, mode);
+  if (from && !c->has_bcq) return rhs_stage<T, double>(c, from, mode, fuse_filter);
   if (from) {
     const long n = 5 * c->N;
     hipLaunchKernelGGL((k_copy_state<T>), dim3((n + 255) / 256), dim3(256), 0, c->stream, qs, from, n);
@@ -358,14 +360,15 @@ int run_stage(cruna_hip_ctx *c, const double *from, int mode) {
   }
   if (c->has_bcq)
     if (int rc = bc_q<T>(c, qs)) return rc;
-  return rhs_stage<T, T>(c, (const T *)qs, mode);
+  return rhs_stage<T, T>(c, (const T *)qs, mode, fuse_filter);
 }
 
+// RK4 step on qn; with fuse_filter the last stage also applies filter_i1.
 template <typename T>
-int rk4_step(cruna_hip_ctx *c) {
+int rk4_step(cruna_hip_ctx *c, bool fuse_filter = false) {
   if (int rc = run_stage<T>(c, c->qn, 0)) return rc;
   for (int st = 1; st < 4; ++st)
-    if (int rc = run_stage<T>(c, nullptr, st)) return rc;
+    if (int rc = run_stage<T>(c, nullptr, st, st == 3 && fuse_filter)) return rc;
   return CRUNA_HIP_OK;
 }
 
@@ -378,12 +381,15 @@ int filter_dir(cruna_hip_ctx *c, double *q, int d, int nv) {
 
 int step_once(cruna_hip_ctx *c, int nt) {
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
-  int rc = mixed ? rk4_step<float>(c) : rk4_step<double>(c);
+  // filter_i1 directly follows the RK4 update unless q-type boundary rows
+  // come between them; then it runs inside the update's x sweep
+  const bool fuse_fx = !c->has_bcq && c->cfg.filter_freq[0] > 0 && nt % c->cfg.filter_freq[0] == 0;
+  int rc = mixed ? rk4_step<float>(c, fuse_fx) : rk4_step<double>(c, fuse_fx);
   if (rc) return rc;
   // set_boundary_condition_q(qn, outside_rhs=.true.) (cruna_hpc.f90:216)
   if (c->has_bcq)
     if ((rc = bc_q<double>(c, c->qn))) return rc;
-  for (int d = 0; d < 3; ++d) {
+  for (int d = fuse_fx ? 1 : 0; d < 3; ++d) {
     const int fq = c->cfg.filter_freq[d];
     if (fq > 0 && nt % fq == 0) {
       rc = mixed ? filter_dir<float>(c, c->qn, d, 5) : filter_dir<double>(c, c->qn, d, 5);
@@ -519,6 +525,28 @@ int ensure_adjoint(cruna_hip_ctx *c) {
 
 double *traj_slot(cruna_hip_ctx *c, int nt) { return c->traj + (size_t)(nt - 1) * 5 * c->N; }
 
+template <typename T>
+int upload_metric(cruna_hip_ctx *c, const std::vector<double> &m1, const std::vector<double> &m2,
+                  const std::vector<double> &m3, const double *Xj, Metric<T> &out) {
+  const size_t n1 = m1.size(), n2 = m2.size(), n3 = m3.size(), N = c->N;
+  std::vector<T> h(n1 + n2 + n3 + N);
+  size_t o = 0;
+  for (double v : m1) h[o++] = T(v);
+  for (double v : m2) h[o++] = T(v);
+  for (double v : m3) h[o++] = T(v);
+  for (size_t i = 0; i < N; ++i) h[o++] = T(Xj[i]);
+  if (c->met) HIP_TRY(hipFree(c->met));
+  c->met = nullptr;
+  HIP_TRY(hipMalloc(&c->met, h.size() * sizeof(T)));
+  HIP_TRY(hipMemcpy(c->met, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  T *b = (T *)c->met;
+  out.m[0] = b;
+  out.m[1] = b + n1;
+  out.m[2] = b + n1 + n2;
+  out.J = b + n1 + n2 + n3;
+  return CRUNA_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -621,6 +649,8 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   ph.visc_law = cfg->visc_law;
   ph.cp = cfg->gamma * (cfg->R / cfg->W) / (cfg->gamma - 1.0);
   ph.Pr = cfg->Pr;
+  ph.lam_coef = ph.cp / cfg->Pr;
+  ph.inv_T0 = 1.0 / cfg->T0;
   for (int d = 0; d < 3; ++d) {
     ph.inv_h[d] = 1.0 / cfg->dx[d];
     ph.heat_inv_h[d] = cfg->heat_unit_spacing ? 1.0 : 1.0 / cfg->dx[d];
@@ -630,7 +660,7 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   if (hipMalloc(&c->qn, 5 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->acc, 5 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->qs, 5 * N * ts) != hipSuccess || hipMalloc(&c->rhs, 5 * N * ts) != hipSuccess ||
-      hipMalloc(&c->G, 6 * N * ts) != hipSuccess) {
+      hipMalloc(&c->G, 7 * N * ts) != hipSuccess) {
     (void)hipGetLastError();
     cruna_hip_destroy(c.release());
     return fail(CRUNA_HIP_ENOMEM, "device allocation failed");
@@ -649,7 +679,7 @@ int cruna_hip_destroy(cruna_hip_ctx *c) {
     c->fop32[d].release();
   }
   for (void *p : {(void *)c->qn, (void *)c->acc, (void *)c->qref, (void *)c->shape, (void *)c->tmp,
-                  (void *)c->qa, (void *)c->traj, (void *)c->gobj, c->qs, c->rhs, c->G})
+                  (void *)c->qa, (void *)c->traj, (void *)c->gobj, c->met, c->qs, c->rhs, c->G})
     if (p) (void)hipFree(p);
   for (auto &p : c->pending) {
     (void)hipEventDestroy(p.second.first);
@@ -686,6 +716,49 @@ int cruna_hip_set_sponge_shape(cruna_hip_ctx *c, const double *shape) {
   if (!c || !shape) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (!c->shape) HIP_TRY(hipMalloc(&c->shape, c->N * sizeof(double)));
   HIP_TRY(hipMemcpy(c->shape, shape, c->N * sizeof(double), hipMemcpyHostToDevice));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_set_metric(cruna_hip_ctx *c, const double *X_metric, const double *X_jacobian) {
+  if (!c || !X_metric || !X_jacobian) return fail(CRUNA_HIP_EINVAL, "null argument");
+  const long n1 = c->cfg.n[0], n2 = c->cfg.n[1], n3 = c->cfg.n[2], N = c->N;
+  auto M = [&](int k, int j, long i, long jj, long kk) {
+    return X_metric[(size_t)N * (k + 3 * j) + i + n1 * (jj + n2 * kk)];
+  };
+  double scale = 0.0;
+  for (int d = 0; d < 3; ++d)
+    for (long p = 0; p < N; ++p) scale = std::max(scale, std::fabs(X_metric[(size_t)N * (4 * d) + p]));
+  if (!(scale > 0.0)) return fail(CRUNA_HIP_EINVAL, "metric with a zero diagonal");
+  for (int k = 0; k < 3; ++k)
+    for (int j = 0; j < 3; ++j) {
+      if (k == j) continue;
+      for (long p = 0; p < N; ++p)
+        if (std::fabs(X_metric[(size_t)N * (k + 3 * j) + p]) > 1e-12 * scale)
+          return fail(CRUNA_HIP_EINVAL,
+                      "X_metric has off-diagonal entries: general curvilinear grids are not "
+                      "supported on the device (tensor-product stretches are)");
+    }
+  // M(d,d) must be constant along the lines of direction d
+  std::vector<double> m1((size_t)n2 * n3), m2((size_t)n1 * n3), m3((size_t)n1 * n2);
+  for (long kk = 0; kk < n3; ++kk)
+    for (long jj = 0; jj < n2; ++jj)
+      for (long i = 0; i < n1; ++i) {
+        const double a = M(0, 0, i, jj, kk), b = M(1, 1, i, jj, kk), e = M(2, 2, i, jj, kk);
+        const double a0 = M(0, 0, 0, jj, kk), b0 = M(1, 1, i, 0, kk), e0 = M(2, 2, i, jj, 0);
+        if (std::fabs(a - a0) > 1e-13 * scale || std::fabs(b - b0) > 1e-13 * scale ||
+            std::fabs(e - e0) > 1e-13 * scale)
+          return fail(CRUNA_HIP_EINVAL,
+                      "X_metric(d,d) varies along direction d: only tensor-product stretched "
+                      "grids are supported on the device");
+        m1[jj + n2 * kk] = a0;
+        m2[i + n1 * kk] = b0;
+        m3[i + n1 * jj] = e0;
+      }
+  const int rc = c->cfg.precision == CRUNA_HIP_FP32_MIXED
+                     ? upload_metric<float>(c, m1, m2, m3, X_jacobian, c->mt32)
+                     : upload_metric<double>(c, m1, m2, m3, X_jacobian, c->mt64);
+  if (rc) return rc;
+  c->gt = true;
   return CRUNA_HIP_OK;
 }
 
@@ -751,6 +824,7 @@ int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const
                           const double *g, double t) {
   (void)t;  // the adjoint RHS of the reference does not depend on t
   if (!c || !rhs || !qs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (c->gt) return fail(CRUNA_HIP_EINVAL, "the deformed-grid adjoint is not implemented on the device");
   if (int rc = stage_adjoint_inputs(c, qs, q, g)) return rc;
   const double *q0 = c->tmp + 5 * c->N, *gd = g ? c->gobj : nullptr;
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
@@ -775,6 +849,7 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *
                                    const double *q0, const double *g, double t) {
   (void)t;
   if (!c || !qs1 || !qs0 || !q0) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (c->gt) return fail(CRUNA_HIP_EINVAL, "the deformed-grid adjoint is not implemented on the device");
   if (int rc = stage_adjoint_inputs(c, qs0, q0, g)) return rc;
   const double *qf = c->tmp + 5 * c->N, *gd = g ? c->gobj : nullptr;
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
@@ -846,6 +921,7 @@ int cruna_hip_trajectory_record(cruna_hip_ctx *c, int on) {
 
 int cruna_hip_run_adjoint(cruna_hip_ctx *c, int nt_end, int nsteps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (c->gt) return fail(CRUNA_HIP_EINVAL, "the deformed-grid adjoint is not implemented on the device");
   if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
   if (!c->qa) return fail(CRUNA_HIP_EINVAL, "no adjoint state (cruna_hip_set_adjoint_state)");
   if (c->traj_slots == 0) return fail(CRUNA_HIP_EINVAL, "no trajectory (cruna_hip_trajectory_alloc)");

@@ -42,9 +42,11 @@ struct SweepGeom {
 // Thread placement inside a tile.  The LDS slot of a (line, block) pair is the
 // thread index; BS / LS are the slot strides between blocks / lines.
 struct ThreadPos {
-  int l, b;       // line in tile, block in line
+  int l, b;        // line in tile, block in line
   int BS, LS;
-  long base;      // element offset of this thread's first point
+  long base;       // element offset of this thread's first point
+  long lidx;       // index of the line in the plane across it: j + n2 k (x), i + n1 k (y), i + n1 j (z)
+  bool wave_line;  // every line of the tile lies inside one wavefront
 };
 
 // Work-groups are dealt round-robin to the 8 XCDs; map consecutive tiles to the
@@ -54,26 +56,33 @@ __device__ inline int xcd_tile(int bid, int ntiles) {
   return (bid % 8) * (ntiles / 8) + bid / 8;
 }
 
+template <int DIR>
 __device__ inline ThreadPos thread_pos(const SweepGeom &g) {
   ThreadPos t;
   const int tid = threadIdx.x;
   const int tile = xcd_tile(blockIdx.x, g.ntiles);
-  if (g.dir == 0) {
+  if (DIR == 0) {
     t.b = tid % g.NB;
     t.l = tid / g.NB;
     t.BS = 1;
     t.LS = g.NB;
+    // x-lines take NB consecutive threads: with NB dividing the wavefront
+    // size no line straddles two wavefronts
+    t.wave_line = (64 % g.NB) == 0;
     const long line = (long)tile * g.TL + t.l;
     t.base = line * g.n1 + (long)t.b * PPT;
+    t.lidx = line;
   } else {
     t.l = tid % g.TL;
     t.b = tid / g.TL;
     t.BS = g.TL;
     t.LS = 1;
+    t.wave_line = false;
     const int i = (tile % g.ntiles_in) * g.TL + t.l;
     const long o = tile / g.ntiles_in;  // k for y-lines, j for z-lines
-    const long off = (g.dir == 1) ? (long)i + (long)g.n1 * g.n2 * o : (long)i + (long)g.n1 * o;
+    const long off = (DIR == 1) ? (long)i + (long)g.n1 * g.n2 * o : (long)i + (long)g.n1 * o;
     t.base = off + (long)t.b * PPT * g.PS;
+    t.lidx = (long)i + (long)g.n1 * o;
   }
   return t;
 }
@@ -85,6 +94,20 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Synchronisation point of the line engine: when each line lives inside one
+// wavefront, the LDS exchange only has to be ordered within the wavefront
+// (its LDS operations complete in issue order), so no work-group barrier is
+// needed.
+__device__ __forceinline__ void line_sync(bool wave_line) {
+  if (wave_line) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+  } else {
+    lds_barrier();
+  }
 }
 
 // LDS scratch of one work-group for the line engine.
@@ -124,7 +147,7 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T> &s, c
     s.halo[tid * 2 * HALO + m] = f[m];
     s.halo[tid * 2 * HALO + HALO + m] = f[PPT - HALO + m];
   }
-  lds_barrier();
+  line_sync(tp.wave_line);
   const bool hasL = PER || b > 0;
   const bool hasR = PER || b < NB - 1;
   const int bl = b == 0 ? NB - 1 : b - 1;
@@ -180,7 +203,7 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T> &s, c
   s.z[0][tid * 2] = z0;
   s.z[0][tid * 2 + 1] = z1;
   for (int k = 0; k < op.K; ++k) {
-    lds_barrier();
+    line_sync(tp.wave_line);
     const int st = 1 << k;
     const bool hm = PER || b - st >= 0;
     const bool hp = PER || b + st < NB;
@@ -216,7 +239,7 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T> &s, c
     s.z[cur][tid * 2] = z0;
     s.z[cur][tid * 2 + 1] = z1;
   }
-  lds_barrier();
+  line_sync(tp.wave_line);
   const T xl = hasL ? s.z[cur][sl * 2 + 1] : T(0);
   const T xf = hasR ? s.z[cur][sr * 2] : T(0);
   if (PER || ty == 1) {

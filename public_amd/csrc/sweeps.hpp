@@ -8,8 +8,8 @@
 //   S2 y : the same along y                                  (accumulates rhs)
 //   S3 z : the same along z
 //   S4 z : d/dz tau_{iz}
-//   S5 x : d/dx tau_{ix}
-//   S6 y : d/dy tau_{iy} and tau:grad(u), then the RK4 stage update
+//   S5 y : d/dy tau_{iy}
+//   S6 x : d/dx tau_{ix} and tau:grad(u), then the RK4 stage update
 //          (src/time/rk_o4_e.f90:45-50)
 // The pressure equation is accumulated as the bracket of ns_rho_rhou_p.f90:200-206
 // and multiplied by (gamma - 1) once, as the reference does.
@@ -18,10 +18,12 @@
 // Field layout in HBM: every field is f(n1,n2,n3) Fortran order; a state holds
 // 5 fields at a stride of N = n1*n2*n3 elements.  The viscous stress and the
 // dissipation use the velocity gradient only through its diagonal and the
-// symmetric sums, so G holds 6 fields:
-//   G0 = u_x, G1 = v_y, G2 = w_z, G3 = u_y + v_x, G4 = u_z + w_x, G5 = v_z + w_y
+// symmetric sums, so G holds 6 fields, and a seventh with the viscosity:
+//   G0 = u_x, G1 = v_y, G2 = w_z, G3 = u_y + v_x, G4 = u_z + w_x, G5 = v_z + w_y,
+//   G6 = mu(T) of the stage input
 // (each sum is completed by the second sweep that contributes to it; the
-// result equals the reference's u_y + v_x etc. bit for bit).
+// result equals the reference's u_y + v_x etc. bit for bit).  S1 x evaluates
+// the viscosity law once per point and stage; S2, S3 and S4-S6 read it.
 #pragma once
 
 #include "line_solve.hpp"
@@ -38,15 +40,28 @@ struct Phys {
   int visc_law;      // 0 Sutherland, 1 power law
   double cp;         // gamma (R/W) / (gamma - 1)
   double Pr;
+  double lam_coef;   // cp / Pr  (lambda = mu cp / Pr)
+  double inv_T0;     // 1 / T0
   double inv_h[3];       // 1/dx_d
   double heat_inv_h[3];  // 1/dx_d, or 1 for the reference's unit-spacing heat flux
+};
+
+// Deformed grid with a diagonal metric whose entries M(d,d) are constant along
+// the lines of direction d (every tensor-product stretch of a Cartesian grid:
+// ns_rho_rhou_p_grid_transform.f90 with X_metric from grid_metric.f90).
+// Stored per line of each direction and the Jacobian per point, in the
+// compute precision.
+template <typename T>
+struct Metric {
+  const T *m[3];  // M(d,d) of the line through (j,k) / (i,k) / (i,j): index ThreadPos::lidx
+  const T *J;     // X_jacobian(n1,n2,n3)
 };
 
 // give_mu, ns_rho_rhou_p_trafos.f90:119-142: mu0*(T/T0)**1.5*((T0+S1)/(T+S1)),
 // or the power law mu0*(T/T0)**exp of the case-level override.
 template <typename T>
 __device__ __forceinline__ T viscosity(const Phys &ph, T Tk) {
-  const T r = Tk / T(ph.T0);
+  const T r = Tk * T(ph.inv_T0);
   // r > 0, so r**e = 2**(e log2 r) without pow's special-case branches
   if (ph.visc_law == 1) return T(ph.mu0) * exp2(T(ph.visc_exp) * log2(r));
   const T S1 = T(110.4);
@@ -66,14 +81,15 @@ __device__ __forceinline__ void materialize(T (&a)[PPT]) {
   for (int m = 0; m < PPT; ++m) asm volatile("" : "+v"(a[m]));
 }
 
-// give_T (:57) and give_lambda (:169): T = (W/R) p / rho, lambda = mu cp / Pr
+// give_T (:57) and give_lambda (:169): T = (W/R) p / rho, lambda = mu cp / Pr,
+// with 1/rho formed once per point by the caller
 template <typename T>
-__device__ __forceinline__ T temperature(const Phys &ph, T rho, T p) {
-  return T(ph.w_inv_r) * p / rho;
+__device__ __forceinline__ T temperature(const Phys &ph, T irho, T p) {
+  return T(ph.w_inv_r) * p * irho;
 }
 template <typename T>
 __device__ __forceinline__ T lambda_of(const Phys &ph, T mu) {
-  return mu * T(ph.cp) / T(ph.Pr);
+  return mu * T(ph.lam_coef);
 }
 
 // A thread's block is PPT consecutive points of its line.  Along x those are
@@ -142,126 +158,140 @@ __device__ __forceinline__ void st(Q *__restrict__ p, long base, long PS, const 
 //   qs  : stage input state (5 fields, type Q)
 //   rhs : 5 accumulator fields (S1 writes, S2/S3 read-modify-write)
 //   G   : velocity-gradient fields (layout above)
-// The state is read once into registers; read-modify-write operands are
-// fetched just before the line solve whose result they meet, so their latency
-// hides behind the solve (the engine's barriers do not wait on global loads).
-template <typename T, typename Q, int DIR, bool PER>
+// The solves are ordered so that each state field is live only while needed:
+// the two transverse momentum components first (velocity gradient and
+// momentum flux of each), then the DIR component, mass, pressure gradient,
+// energy and heat flux with rho, rho u_DIR and p only.  Read-modify-write
+// operands are fetched just before the line solve whose result they meet, so
+// their latency hides behind the solve (the engine's barriers do not wait on
+// global loads).
+// GT: deformed grid (ns_rho_rhou_p_grid_transform.f90:31-330).  Every
+// derivative along DIR is scaled by the line's metric entry M(DIR,DIR)
+// (physical gradients, contravariant fluxes and the projected stresses all
+// reduce to that for a diagonal metric); the heat flux takes the temperature
+// gradient with h and lambda J inside, the outer derivative without h.
+template <typename T, typename Q, int DIR, bool PER, bool GT>
 __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__restrict__ rhs,
                                                T *__restrict__ G, LineOp<T> op, SweepGeom g,
-                                               Phys ph, long N) {
+                                               Phys ph, long N, Metric<T> mt) {
   __shared__ LineLds<T> lds;
-  const ThreadPos tp = thread_pos(g);
+  const ThreadPos tp = thread_pos<DIR>(g);
   const long B = tp.base, PS = g.PS;
-  const T ih = T(ph.inv_h[DIR]), hih = T(ph.heat_inv_h[DIR]);
-  // the state stays in registers as loaded; velocities are formed on use
-  T rho[PPT], m1[PPT], m2[PPT], m3[PPT], p[PPT];
-  ld<DIR>(qs, B, PS, rho);
-  ld<DIR>(qs + N, B, PS, m1);
-  ld<DIR>(qs + 2 * N, B, PS, m2);
-  ld<DIR>(qs + 3 * N, B, PS, m3);
-  ld<DIR>(qs + 4 * N, B, PS, p);
-  issue_loads();
-  T(&mom)[PPT] = DIR == 0 ? m1 : (DIR == 1 ? m2 : m3);
-  auto vel = [&](int i, int m) -> T { return (i == 0 ? m1[m] : (i == 1 ? m2[m] : m3[m])) / rho[m]; };
-  T f[PPT], x[PPT], r[PPT];
+  const T mline = GT ? mt.m[DIR][tp.lidx] : T(1);
+  const T ih = GT ? T(ph.inv_h[DIR]) * mline : T(ph.inv_h[DIR]);
+  // heat flux: inner (temperature) and outer derivative scales
+  const T hin = GT ? ih : T(ph.heat_inv_h[DIR]);
+  const T hout = GT ? mline : T(ph.heat_inv_h[DIR]);
+  // G slot of d(u_i)/dx_DIR and whether it completes a symmetric sum begun by
+  // an earlier sweep: x writes u_x, v_x, w_x; y adds u_y to v_x and writes
+  // v_y, w_y; z adds u_z to w_x, v_z to w_y and writes w_z
+  auto gslot = [](int i) -> int {
+    return DIR == 0 ? (i == 0 ? 0 : (i == 1 ? 3 : 4))
+                    : (DIR == 1 ? (i == 0 ? 3 : (i == 1 ? 1 : 5)) : (i == 0 ? 4 : (i == 1 ? 5 : 2)));
+  };
+  auto gsum = [](int i) -> bool { return (DIR == 1 && i == 0) || (DIR == 2 && i < 2); };
+  T irho[PPT], mom[PPT], p[PPT], f[PPT], x[PPT], r[PPT];
+  ld<DIR>(qs, B, PS, f);
+  ld<DIR>(qs + (1 + DIR) * N, B, PS, mom);
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) irho[m] = T(1) / f[m];
 
-  // velocity gradients u_d, v_d, w_d (ns_rho_rhou_p.f90:102-112) into the
-  // diagonal / symmetric-sum layout of G
-  {
-    T sa[PPT], sb[PPT];
-    if (DIR == 1) ld<DIR>(G + 3 * N, B, PS, sa);  // v_x
-    if (DIR == 2) {
-      ld<DIR>(G + 4 * N, B, PS, sa);  // w_x
-      ld<DIR>(G + 5 * N, B, PS, sb);  // w_y
-    }
+  // transverse components i != DIR: d(u_i)/dx_DIR (ns_rho_rhou_p.f90:102-112)
+  // and the momentum flux -d/dx_DIR (rho u_DIR u_i) (:138-172)
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = vel(0, m);
+  for (int k = 0; k < 2; ++k) {
+    const int i = k == 0 ? (DIR == 0 ? 1 : 0) : (DIR == 2 ? 1 : 2);
+    T ui[PPT], gs[PPT];
+    asm volatile("" ::: "memory");
+    ld<DIR>(qs + (1 + i) * N, B, PS, ui);
+    if (gsum(i)) ld<DIR>(G + gslot(i) * N, B, PS, gs);
+    issue_loads();
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) ui[m] = ui[m] * irho[m];
+    line_solve<T, PER>(op, lds, tp, ui, x);
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+      x[m] = x[m] * ih;
+      if (gsum(i)) x[m] = x[m] + gs[m];
+    }
+    st<DIR>(G + gslot(i) * N, B, PS, x);
+    if (DIR != 0) ld<DIR>(rhs + (1 + i) * N, B, PS, r);
+    if (k == 1) ld<DIR>(qs + 4 * N, B, PS, p);  // in flight for the pressure terms
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) f[m] = mom[m] * ui[m];
     line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) x[m] = x[m] * ih;
-    if (DIR == 0) st<DIR>(G + 0 * N, B, PS, x);  // u_x
-    if (DIR == 1) {
-#pragma unroll
-      for (int m = 0; m < PPT; ++m) sa[m] = x[m] + sa[m];
-      st<DIR>(G + 3 * N, B, PS, sa);  // u_y + v_x
-    }
-    if (DIR == 2) {
-#pragma unroll
-      for (int m = 0; m < PPT; ++m) sa[m] = x[m] + sa[m];
-      st<DIR>(G + 4 * N, B, PS, sa);  // u_z + w_x
-    }
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = vel(1, m);
-    line_solve<T, PER>(op, lds, tp, f, x);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) x[m] = x[m] * ih;
-    if (DIR == 0) st<DIR>(G + 3 * N, B, PS, x);  // v_x
-    if (DIR == 1) st<DIR>(G + 1 * N, B, PS, x);  // v_y
-    if (DIR == 2) {
-#pragma unroll
-      for (int m = 0; m < PPT; ++m) sb[m] = x[m] + sb[m];
-      st<DIR>(G + 5 * N, B, PS, sb);  // v_z + w_y
-    }
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = vel(2, m);
-    line_solve<T, PER>(op, lds, tp, f, x);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) x[m] = x[m] * ih;
-    if (DIR == 0) st<DIR>(G + 4 * N, B, PS, x);  // w_x
-    if (DIR == 1) st<DIR>(G + 5 * N, B, PS, x);  // w_y
-    if (DIR == 2) st<DIR>(G + 2 * N, B, PS, x);  // w_z
-
+    for (int m = 0; m < PPT; ++m) r[m] = DIR == 0 ? -(x[m] * ih) : r[m] - x[m] * ih;
+    st<DIR>(rhs + (1 + i) * N, B, PS, r);
   }
 
-  // mass: -d/dx_d (rho u_d) (:131-135)
+  // d(u_DIR)/dx_DIR
+  T ud[PPT];
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) ud[m] = mom[m] * irho[m];
+  line_solve<T, PER>(op, lds, tp, ud, x);
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) x[m] = x[m] * ih;
+  st<DIR>(G + DIR * N, B, PS, x);
+
+  // mass: -d/dx_DIR (rho u_DIR) (:131-135)
   if (DIR != 0) ld<DIR>(rhs, B, PS, r);
   line_solve<T, PER>(op, lds, tp, mom, x);
 #pragma unroll
   for (int m = 0; m < PPT; ++m) r[m] = DIR == 0 ? -(x[m] * ih) : r[m] - x[m] * ih;
   st<DIR>(rhs, B, PS, r);
 
-  // pressure gradient p_d
+  // pressure gradient p_DIR
   T pd[PPT];
   line_solve<T, PER>(op, lds, tp, p, pd);
 #pragma unroll
   for (int m = 0; m < PPT; ++m) pd[m] = pd[m] * ih;
 
-  // momentum: -d/dx_d (rho u_d u_i) - p_i (:138-172)
+  // momentum DIR: -d/dx_DIR (rho u_DIR u_DIR) - p_DIR
+  if (DIR != 0) ld<DIR>(rhs + (1 + DIR) * N, B, PS, r);
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    if (DIR != 0) ld<DIR>(rhs + (1 + i) * N, B, PS, r);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = mom[m] * vel(i, m);
-    line_solve<T, PER>(op, lds, tp, f, x);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-      r[m] = DIR == 0 ? -(x[m] * ih) : r[m] - x[m] * ih;
-      if (i == DIR) r[m] = r[m] - pd[m];
-    }
-    st<DIR>(rhs + (1 + i) * N, B, PS, r);
-  }
-
-  // pressure-equation bracket: gamma/(gamma-1) (-d(p u_d)) + u_d p_d (:180-187)
-  T e[PPT];
-#pragma unroll
-  for (int m = 0; m < PPT; ++m) f[m] = p[m] * vel(DIR, m);
-  line_solve<T, PER>(op, lds, tp, f, x);
-#pragma unroll
-  for (int m = 0; m < PPT; ++m)
-    e[m] = T(ph.gt) * (-(x[m] * ih)) + vel(DIR, m) * pd[m];
-
-  // heat flux d/dx_d (lambda d/dx_d T) (:189-199)
-  if (DIR != 0) ld<DIR>(rhs + 4 * N, B, PS, r);
-#pragma unroll
-  for (int m = 0; m < PPT; ++m) f[m] = temperature(ph, rho[m], p[m]);
-  line_solve<T, PER>(op, lds, tp, f, x);
-#pragma unroll
-  for (int m = 0; m < PPT; ++m)
-    f[m] = lambda_of(ph, viscosity(ph, temperature(ph, rho[m], p[m]))) * (x[m] * hih);
+  for (int m = 0; m < PPT; ++m) f[m] = mom[m] * ud[m];
   line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
   for (int m = 0; m < PPT; ++m) {
-    e[m] = e[m] + x[m] * hih;
+    r[m] = DIR == 0 ? -(x[m] * ih) : r[m] - x[m] * ih;
+    r[m] = r[m] - pd[m];
+  }
+  st<DIR>(rhs + (1 + DIR) * N, B, PS, r);
+
+  // pressure-equation bracket: gamma/(gamma-1) (-d(p u_DIR)) + u_DIR p_DIR (:180-187)
+  T e[PPT];
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) f[m] = p[m] * ud[m];
+  line_solve<T, PER>(op, lds, tp, f, x);
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) e[m] = T(ph.gt) * (-(x[m] * ih)) + ud[m] * pd[m];
+
+  // heat flux d/dx_DIR (lambda d/dx_DIR T) (:189-199); the x sweep evaluates
+  // the viscosity law and leaves mu in G6 for the later sweeps of the stage
+  T mu[PPT], Jv[PPT];
+  if (DIR != 0) {
+    ld<DIR>(rhs + 4 * N, B, PS, r);
+    ld<DIR>(G + 6 * N, B, PS, mu);
+  }
+  if (GT) ld<DIR>(mt.J, B, PS, Jv);
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) f[m] = temperature(ph, irho[m], p[m]);
+  if (DIR == 0) {
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) mu[m] = viscosity(ph, f[m]);
+    st<DIR>(G + 6 * N, B, PS, mu);
+  }
+  line_solve<T, PER>(op, lds, tp, f, x);
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) {
+    const T lam = GT ? lambda_of(ph, mu[m]) * Jv[m] : lambda_of(ph, mu[m]);
+    f[m] = lam * (x[m] * hin);
+  }
+  line_solve<T, PER>(op, lds, tp, f, x);
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) {
+    e[m] = e[m] + x[m] * hout;
     if (DIR != 0) e[m] = r[m] + e[m];
   }
   st<DIR>(rhs + 4 * N, B, PS, e);
@@ -304,37 +334,39 @@ __device__ __forceinline__ void rk_update(int mode, int v, const T (&k)[PPT], lo
 }
 
 // ----------------------------------------------------------------------------
-// S4 (z) / S5 (x) / S6 (y): viscous fluxes d/dx_d tau_{i d}; the y sweep also
+// S4 (z) / S5 (y) / S6 (x): viscous fluxes d/dx_d tau_{i d}; the x sweep also
 // adds tau:grad(u) (ns_rho_rhou_p.f90:202-205) and finishes the stage.
 // mode 0..3: RK4 stage update; mode 4: write the complete RHS into rhs.
-// qs_out may be the stage input itself (stages 1-3 update it in place, each
-// thread only its own points), so neither carries __restrict__.
-template <typename T, typename Q, int DIR, bool PER>
-__global__ __launch_bounds__(NT) void k_sweep2(const Q *qs, T *__restrict__ rhs,
-                                               const T *__restrict__ G, LineOp<T> op, SweepGeom g,
-                                               Phys ph, long N, int mode, double *__restrict__ qn,
-                                               double *__restrict__ acc, T *qs_out, double dt) {
+// With fuse_filter set, the last stage (mode 3) also applies filter_i1 to
+// the new state before storing it (the x filter of the step loop, fused
+// because the x sweep holds complete x-lines); fop is that filter.
+// The viscous sweeps read only G and rhs, so the next stage input (qs_out)
+// may overwrite the current one in place.
+// GT: deformed grid: d/dxi_d (M(d,d) tau_id) = M(d,d) d/dxi_d tau_id, and the
+// last sweep multiplies every equation by X_jacobian and the dissipation once
+// more (ns_rho_rhou_p_grid_transform.f90:190-276).
+template <typename T, int DIR, bool PER, bool GT>
+__global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__restrict__ G,
+                                               LineOp<T> op, LineOp<T> fop, SweepGeom g, Phys ph,
+                                               long N, int mode, int fuse_filter,
+                                               double *__restrict__ qn, double *__restrict__ acc,
+                                               T *__restrict__ qs_out, double dt, Metric<T> mt) {
+  constexpr bool LAST = DIR == 0;
   __shared__ LineLds<T> lds;
-  const ThreadPos tp = thread_pos(g);
+  const ThreadPos tp = thread_pos<DIR>(g);
   const long B = tp.base, PS = g.PS;
-  const T ih = T(ph.inv_h[DIR]);
+  const T ih = GT ? T(ph.inv_h[DIR]) * mt.m[DIR][tp.lidx] : T(ph.inv_h[DIR]);
   T t1[PPT], t2[PPT], t3[PPT], diss[PPT];
   {
     T mu[PPT], ux[PPT], vy[PPT], wz[PPT], sa[PPT], sb[PPT];
-    {
-      T rho[PPT], p[PPT];
-      ld<DIR>(qs, B, PS, rho);
-      ld<DIR>(qs + 4 * N, B, PS, p);
-      ld<DIR>(G + 0 * N, B, PS, ux);
-      ld<DIR>(G + 1 * N, B, PS, vy);
-      ld<DIR>(G + 2 * N, B, PS, wz);
-      // x: s12 = u_y + v_x, s13 = u_z + w_x;  y: s12, s23 = v_z + w_y;  z: s13, s23
-      ld<DIR>(G + (DIR == 2 ? 4 : 3) * N, B, PS, sa);
-      ld<DIR>(G + (DIR == 0 ? 4 : 5) * N, B, PS, sb);
-      issue_loads();
-#pragma unroll
-      for (int m = 0; m < PPT; ++m) mu[m] = viscosity(ph, temperature(ph, rho[m], p[m]));
-    }
+    ld<DIR>(G + 6 * N, B, PS, mu);
+    ld<DIR>(G + 0 * N, B, PS, ux);
+    ld<DIR>(G + 1 * N, B, PS, vy);
+    ld<DIR>(G + 2 * N, B, PS, wz);
+    // x: s12 = u_y + v_x, s13 = u_z + w_x;  y: s12, s23 = v_z + w_y;  z: s13, s23
+    ld<DIR>(G + (DIR == 2 ? 4 : 3) * N, B, PS, sa);
+    ld<DIR>(G + (DIR == 0 ? 4 : 5) * N, B, PS, sb);
+    issue_loads();
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
       const T md = -2.0 / 3.0 * mu[m] * (ux[m] + vy[m] + wz[m]);
@@ -343,66 +375,83 @@ __global__ __launch_bounds__(NT) void k_sweep2(const Q *qs, T *__restrict__ rhs,
         t1[m] = t11;            // tau_11
         t2[m] = mu[m] * sa[m];  // tau_21
         t3[m] = mu[m] * sb[m];  // tau_31
+        // tau_ij du_i/dx_j = tau_11 u_x + tau_22 v_y + tau_33 w_z + mu (s12^2 + s13^2 + s23^2)
+        diss[m] = t11 * ux[m] + t22 * vy[m] + t33 * wz[m] + t2[m] * sa[m] + t3[m] * sb[m];
       } else if (DIR == 1) {
         t1[m] = mu[m] * sa[m];  // tau_12
         t2[m] = t22;            // tau_22
         t3[m] = mu[m] * sb[m];  // tau_32
-        // tau_ij du_i/dx_j = tau_11 u_x + tau_22 v_y + tau_33 w_z + mu (s12^2 + s13^2 + s23^2)
-        diss[m] = t11 * ux[m] + t22 * vy[m] + t33 * wz[m] + t1[m] * sa[m] + t3[m] * sb[m];
       } else {
         t1[m] = mu[m] * sa[m];  // tau_13
         t2[m] = mu[m] * sb[m];  // tau_23
         t3[m] = t33;            // tau_33
       }
     }
-    if (DIR == 1) {
-      ld<DIR>(G + 4 * N, B, PS, sa);
+    if (LAST) {
+      ld<DIR>(G + 5 * N, B, PS, sa);
 #pragma unroll
       for (int m = 0; m < PPT; ++m) diss[m] = diss[m] + (mu[m] * sa[m]) * sa[m];
       materialize(diss);
     }
   }
-  T x1[PPT], x2[PPT], x3[PPT];
-  line_solve<T, PER>(op, lds, tp, t1, x1);
-  line_solve<T, PER>(op, lds, tp, t2, x2);
-  line_solve<T, PER>(op, lds, tp, t3, x3);
-  // keep the read-modify-write traffic below the solves: hoisting it above
-  // them (legal, the engine's barriers only order LDS) would hold every
-  // operand in registers across the solves
-  asm volatile("" ::: "memory");
+  T Jv[PPT];
+  if (LAST && GT) ld<DIR>(mt.J, B, PS, Jv);
+  // the last sweep's complete RHS of variable v: RK4 update, and on the
+  // final stage optionally the x filter of the new state
+  auto finish = [&](int v, T(&k)[PPT]) {
+    if (mode == 3 && fuse_filter) {
+      double q0[PPT], s[PPT];
+      ld<DIR>(qn + v * N, B, PS, q0);
+      ld<DIR>(acc + v * N, B, PS, s);
+      T f[PPT], x[PPT];
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) {
-    x1[m] = x1[m] * ih;
-    x2[m] = x2[m] * ih;
-    x3[m] = x3[m] * ih;
-  }
-  if (DIR != 1) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      T a[PPT];
-      ld<DIR>(rhs + (1 + i) * N, B, PS, a);
-#pragma unroll
-      for (int m = 0; m < PPT; ++m) a[m] = a[m] + (i == 0 ? x1[m] : (i == 1 ? x2[m] : x3[m]));
-      st<DIR>(rhs + (1 + i) * N, B, PS, a);
+      for (int m = 0; m < PPT; ++m) f[m] = T(q0[m] + dt / 6.0 * (s[m] + double(k[m])));
+      line_solve<T, PER>(fop, lds, tp, f, x);
+      st<DIR>(qn + v * N, B, PS, x);
+    } else {
+      rk_update<DIR, T>(mode, v, k, B, PS, N, rhs, qn, acc, qs_out, dt);
     }
-    return;
-  }
-  // complete RHS k_v and the RK4 update
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    // one variable at a time keeps the tail's loads from being hoisted together
-    if (v > 0) asm volatile("" ::: "memory");
+  };
+  // each solve's result is consumed at once (z/y: read-modify-write of the
+  // momentum accumulators; x: the complete RHS of that variable and its RK4
+  // update), so only the remaining stress fields stay live across a solve;
+  // the fences keep that traffic below the solve it depends on
+  if (LAST && (mode != 4 || GT)) {
+    // k_1 = rhs_1 (times J) is complete already: its stage update goes first
     T k[PPT];
-    ld<DIR>(rhs + v * N, B, PS, k);
+    ld<DIR>(rhs, B, PS, k);
+    if (GT) {
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) k[m] = Jv[m] * k[m];
+    }
+    finish(0, k);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    T x[PPT];
+    line_solve<T, PER>(op, lds, tp, i == 0 ? t1 : (i == 1 ? t2 : t3), x);
+    asm volatile("" ::: "memory");
+    T k[PPT];
+    ld<DIR>(rhs + (1 + i) * N, B, PS, k);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
-      if (v == 1) k[m] = k[m] + x1[m];
-      if (v == 2) k[m] = k[m] + x2[m];
-      if (v == 3) k[m] = k[m] + x3[m];
-      if (v == 4) k[m] = T(ph.gm1) * (k[m] + diss[m]);
+      k[m] = k[m] + x[m] * ih;
+      if (LAST && GT) k[m] = Jv[m] * k[m];
     }
-    if (mode == 4 && v == 0) continue;  // k_1 is already complete in rhs
-    rk_update<DIR, T>(mode, v, k, B, PS, N, rhs, qn, acc, qs_out, dt);
+    if (!LAST)
+      st<DIR>(rhs + (1 + i) * N, B, PS, k);
+    else
+      finish(1 + i, k);
+  }
+  if (LAST) {
+    // pressure: (gamma - 1) (bracket + tau:grad(u)) (ns_rho_rhou_p.f90:200-209)
+    asm volatile("" ::: "memory");
+    T k[PPT];
+    ld<DIR>(rhs + 4 * N, B, PS, k);
+#pragma unroll
+    for (int m = 0; m < PPT; ++m)
+      k[m] = GT ? (Jv[m] * T(ph.gm1)) * (k[m] + Jv[m] * diss[m]) : T(ph.gm1) * (k[m] + diss[m]);
+    finish(4, k);
   }
 }
 
@@ -412,7 +461,7 @@ template <typename T, int DIR, bool PER>
 __global__ __launch_bounds__(NT) void k_diff(const T *__restrict__ in, T *__restrict__ out,
                                              LineOp<T> op, SweepGeom g, double inv_h) {
   __shared__ LineLds<T> lds;
-  const ThreadPos tp = thread_pos(g);
+  const ThreadPos tp = thread_pos<DIR>(g);
   T f[PPT], x[PPT];
   ld<DIR>(in, tp.base, g.PS, f);
   line_solve<T, PER>(op, lds, tp, f, x);
@@ -426,7 +475,7 @@ template <typename T, int DIR, bool PER>
 __global__ __launch_bounds__(NT) void k_filter(double *__restrict__ q, LineOp<T> op, SweepGeom g,
                                                long N, int nv) {
   __shared__ LineLds<T> lds;
-  const ThreadPos tp = thread_pos(g);
+  const ThreadPos tp = thread_pos<DIR>(g);
   double a[PPT];
   ld<DIR>(q, tp.base, g.PS, a);
   for (int v = 0; v < nv; ++v) {
