@@ -113,6 +113,61 @@ def run_adjoint_gpu(q0, n, h, dt, steps, kw):
     return (t1 - t0), prof, bool(np.isfinite(qs).all())
 
 
+def run_shear_layer(steps, warmup, n=(512, 256, 128), seed=0):
+    """Temporal shear layer on a y-stretched grid, FP32-mixed: Lx, Ly, Lz = 60,
+    40, 30, tanh clustering factor 2 around y = 0, u = 0.5 tanh(2y) with seeded
+    white noise of amplitude 1e-3 windowed by exp(-y^2) on all three velocity
+    components, Mach 0.5 on the velocity difference (c = 2), uniform T and p,
+    Re 1000; periodic x/z, open y with characteristic rows (201) on both y
+    faces and a sponge on the RHS (400, length 4 in y, amplitude 1)."""
+    from public_amd import FP32_MIXED, OPEN, PERIODIC, SUTHERLAND, Cruna
+    from public_amd import geometry as geo
+    L = (60.0, 40.0, 30.0)
+    x0, x1 = (0.0, -L[1] / 2, 0.0), (L[0], L[1] / 2, L[2])
+    scheme = (PERIODIC, OPEN, PERIODIC)
+    Xc, dx = geo.cartesian_grid(n, x0, x1, scheme)
+    X = geo.tanh_stretch_y(Xc, x0[1], x1[1], 2.0)
+    rng = np.random.default_rng(seed)
+    y = X[..., 1]
+    win = np.exp(-y ** 2)
+    gamma = 1.4
+    q = np.zeros(n + (5,), order="F")
+    q[..., 0] = 1.0
+    q[..., 1] = 0.5 * np.tanh(2 * y) + 1e-3 * rng.standard_normal(n) * win
+    q[..., 2] = 1e-3 * rng.standard_normal(n) * win
+    q[..., 3] = 1e-3 * rng.standard_normal(n) * win
+    q[..., 4] = 4.0 / gamma
+    dt = geo.cfl_dt(q, X, 0.5, gamma)
+    n1, n2, n3 = n
+    bc = [(201, 1, n1, 1, 1, 1, n3, 0, -1, 0), (201, 1, n1, n2, n2, 1, n3, 0, 1, 0),
+          (400, 1, n1, 1, n2, 1, n3)]
+    kw = dict(scheme=scheme, gamma=gamma, R=1.0, W=1.0, Pr=0.71, mu0=1e-3, T0=4.0 / gamma,
+              visc_law=SUTHERLAND, dt=dt, filter_freq=(1, 1, 1), bc=bc)
+    with Cruna(n, dx, **kw) as g64:  # metric from FP64 compact derivatives
+        M, J = geo.calc_metric(g64, X, Xc)
+    sigma = geo.sponge_shape(X, x0, x1, (-1.0, 4.0, -1.0), 1.0)
+    g = Cruna(n, dx, precision=FP32_MIXED, **kw)
+    g.set_metric(M, J)
+    del M, J
+    g.set_qref(q)
+    g.set_sponge_shape(sigma)
+    g.set_state(q)
+    g.run_direct(warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run_direct(steps, nt0=warmup)
+    g.synchronize()
+    t1 = time.perf_counter()
+    qe = g.get_state()
+    g.close()
+    pts = float(n1) * n2 * n3
+    return {"value": round(pts * steps / (t1 - t0) / 1e6, 2), "unit": "Mpts/s",
+            "ms_per_step": round((t1 - t0) / steps * 1e3, 4), "steps": steps, "warmup": warmup,
+            "grid": list(n), "dt": dt, "dtype": "fp32-mixed (f32 RHS, f64 state/RK/boundaries)",
+            "finite": bool(np.isfinite(qe).all()),
+            "max_abs_v": float(np.abs(qe[..., 2] / qe[..., 0]).max())}
+
+
 def ke_history(q0, n, h, dt, precision, steps, every, kw):
     from public_amd import Cruna, ic
     g = Cruna((n,) * 3, (h,) * 3, precision=precision, dt=dt, filter_freq=(1, 1, 1), **kw)
@@ -190,6 +245,8 @@ def main():
     ap.add_argument("--ke-every", type=int, default=20)
     ap.add_argument("--adjoint-steps", type=int, default=20,
                     help="adjoint steps timed after a recorded forward run (0: skip)")
+    ap.add_argument("--shear-steps", type=int, default=50,
+                    help="timed steps of the stretched-grid shear layer (512x256x128, FP32-mixed; 0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -224,6 +281,7 @@ def main():
                                       "GBps": round(kernel_bytes_per_point(name, False) * float(n) ** 3
                                                     / (v[0] / v[1] / 1e3) / 1e9, 1)}
                                for name, v in prof_a.items()}}
+    shear = run_shear_layer(args.shear_steps, 2) if args.shear_steps > 0 else None
     pts = float(n) ** 3
     mpts64 = pts * args.steps / secs64 / 1e6
     mpts32 = pts * args.steps / secs32 / 1e6
@@ -299,6 +357,7 @@ def main():
         "fp32_mixed": {"value": round(mpts32, 2), "unit": "Mpts/s", "ms_per_step": round(step_ms32, 4),
                        "speedup_vs_fp64": round(mpts32 / mpts64, 3), "err_vs_fp64": errors},
         "adjoint_fp64": adjoint,
+        "shear_layer_fp32_mixed": shear,
         "roofline": roofline,
         "step_roofline": step_roofline,
         "kernels_fp64": kernels,
