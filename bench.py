@@ -40,15 +40,15 @@ SWEEP_WORDS = {
     "sweep1_z": 21.0,
     "sweep2_z": 12.0,
     "sweep2_y": 12.0,
-    "sweep2_x_rk": 29.5,  # mean over the four stages: 27, 32, 32, 27
+    "sweep2_x_rk": 28.25,  # mean over the four stages: 27, 32, 32, 22
     "filter_x": 10.0,
     "filter_y": 10.0,
     "filter_z": 10.0,
     # adjoint sweeps: q and qs read, rhs written (z) or read-modify-written;
-    # the x sweep also carries the RK4 tail (30, 35, 35, 30 words)
+    # the x sweep also carries the RK4 tail (30, 35, 35, 25 words)
     "asweep_z": 15.0,
     "asweep_y": 20.0,
-    "asweep_x_rk": 32.5,
+    "asweep_x_rk": 31.25,
 }
 # words of each sweep that are stage-state / RHS / gradient fields held in the
 # compute precision (the rest are FP64 state or accumulator words)

@@ -103,6 +103,9 @@ struct cruna_hip_ctx {
   int traj_slots = 0;
   bool record = false;
   void *qs = nullptr, *rhs = nullptr, *G = nullptr;  // compute precision
+  // qs holds qn in the compute precision (FP32-mixed: written by the step's
+  // last filter pass, read by the next step's first stage instead of qn)
+  bool qs_fresh = false;
   // profiling
   bool prof = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -352,6 +355,9 @@ int rhs_stage(cruna_hip_ctx *c, const Q *in, int mode, bool fuse_filter = false)
 template <typename T>
 int run_stage(cruna_hip_ctx *c, const double *from, int mode, bool fuse_filter = false) {
   T *qs = (T *)c->qs;
+  const bool fresh = from == c->qn && c->qs_fresh;
+  c->qs_fresh = false;  // the stage overwrites qs with the next stage input
+  if (fresh) from = nullptr;
   if (from && !c->has_bcq) return rhs_stage<T, double>(c, from, mode, fuse_filter);
   if (from) {
     const long n = 5 * c->N;
@@ -372,10 +378,11 @@ int rk4_step(cruna_hip_ctx *c, bool fuse_filter = false) {
   return CRUNA_HIP_OK;
 }
 
+// copy: also store the filtered state in the compute precision there (or null)
 template <typename T>
-int filter_dir(cruna_hip_ctx *c, double *q, int d, int nv) {
+int filter_dir(cruna_hip_ctx *c, double *q, int d, int nv, T *copy = nullptr) {
   ProfScope p(c, KID_FX + d);
-  DISPATCH_DIR(c, d, k_filter, T, q, fops<T>(c)[d].op, c->geom[d], c->N, nv);
+  DISPATCH_DIR(c, d, k_filter, T, q, fops<T>(c)[d].op, c->geom[d], c->N, nv, copy);
   return check_launch();
 }
 
@@ -389,13 +396,22 @@ int step_once(cruna_hip_ctx *c, int nt) {
   // set_boundary_condition_q(qn, outside_rhs=.true.) (cruna_hpc.f90:216)
   if (c->has_bcq)
     if ((rc = bc_q<double>(c, c->qn))) return rc;
+  int last = -1;
+  for (int d = fuse_fx ? 1 : 0; d < 3; ++d) {
+    const int fq = c->cfg.filter_freq[d];
+    if (fq > 0 && nt % fq == 0) last = d;
+  }
   for (int d = fuse_fx ? 1 : 0; d < 3; ++d) {
     const int fq = c->cfg.filter_freq[d];
     if (fq > 0 && nt % fq == 0) {
-      rc = mixed ? filter_dir<float>(c, c->qn, d, 5) : filter_dir<double>(c, c->qn, d, 5);
+      // in FP32-mixed mode the step's last filter pass also leaves the state
+      // in FP32 for the next step's first stage
+      rc = mixed ? filter_dir<float>(c, c->qn, d, 5, d == last ? (float *)c->qs : nullptr)
+                 : filter_dir<double>(c, c->qn, d, 5);
       if (rc) return rc;
     }
   }
+  c->qs_fresh = mixed && last >= 0;
   return CRUNA_HIP_OK;
 }
 
@@ -480,6 +496,7 @@ int arhs_stage(cruna_hip_ctx *c, const Q *in, const double *q0, const double *g,
 template <typename T>
 int arun_stage(cruna_hip_ctx *c, const double *from, const double *q0, const double *g, int mode) {
   T *qs = (T *)c->qs;
+  c->qs_fresh = false;
   if (from && !c->has_bcq) return arhs_stage<T, double>(c, from, q0, g, mode);
   if (from) {
     const long n = 5 * c->N;
@@ -693,6 +710,7 @@ int cruna_hip_destroy(cruna_hip_ctx *c) {
 
 int cruna_hip_set_state(cruna_hip_ctx *c, const double *q) {
   if (!c || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  c->qs_fresh = false;
   HIP_TRY(hipMemcpyAsync(c->qn, q, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
@@ -797,6 +815,7 @@ int cruna_hip_filter(cruna_hip_ctx *c, int dir, double *q, int nv) {
 int cruna_hip_rhs_direct(cruna_hip_ctx *c, double *rhs, const double *q, double t) {
   (void)t;  // the direct RHS of the reference does not depend on t
   if (!c || !rhs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  c->qs_fresh = false;
   if (int rc = ensure_tmp(c)) return rc;
   HIP_TRY(hipMemcpyAsync(c->tmp, q, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
@@ -836,6 +855,7 @@ int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const
 int cruna_hip_time_stepper_direct(cruna_hip_ctx *c, double *q1, const double *q0, double t) {
   (void)t;
   if (!c || !q1 || !q0) return fail(CRUNA_HIP_EINVAL, "null argument");
+  c->qs_fresh = false;
   HIP_TRY(hipMemcpyAsync(c->qn, q0, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
   int rc = mixed ? rk4_step<float>(c) : rk4_step<double>(c);

@@ -299,7 +299,10 @@ __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__re
 
 // RK4 stage update of variable v from its complete RHS k (rk_o4_e.f90:45-50,
 // and :90-95 for the adjoint with dt -> -dt): stage inputs q0 + dt/2 k1,
-// q0 + dt/2 k2, q0 + dt k3, then q1 = q0 + dt/6 (k1 + 2 k2 + 2 k3 + k4).
+// q0 + dt/2 k2, q0 + dt k3, and q1 = q0 + dt/6 (k1 + 2 k2 + 2 k3 + k4).  The
+// FP64 accumulator carries the partial new state q0 + dt/6 k1 + dt/3 k2 +
+// dt/3 k3, so the last stage reads it alone (not q0 as well); the sum is
+// associated differently from the reference's, a rounding-level change.
 // mode 4 writes k into rhs instead (a single RHS evaluation).
 template <int DIR, typename T>
 __device__ __forceinline__ void rk_update(int mode, int v, const T (&k)[PPT], long B, long PS,
@@ -309,28 +312,29 @@ __device__ __forceinline__ void rk_update(int mode, int v, const T (&k)[PPT], lo
     st<DIR>(rhs + v * N, B, PS, k);
     return;
   }
-  const double c_stage = mode == 2 ? dt : dt * 0.5;
-  double q0[PPT], s[PPT];
-  ld<DIR>(qn + v * N, B, PS, q0);
+  double a[PPT];
   if (mode == 3) {
-    ld<DIR>(acc + v * N, B, PS, s);
+    ld<DIR>(acc + v * N, B, PS, a);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) q0[m] = q0[m] + dt / 6.0 * (s[m] + double(k[m]));
-    st<DIR>(qn + v * N, B, PS, q0);
+    for (int m = 0; m < PPT; ++m) a[m] = a[m] + dt / 6.0 * double(k[m]);
+    st<DIR>(qn + v * N, B, PS, a);
     return;
   }
+  double q0[PPT];
+  ld<DIR>(qn + v * N, B, PS, q0);
   if (mode == 0) {
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) s[m] = double(k[m]);
+    for (int m = 0; m < PPT; ++m) a[m] = q0[m] + dt / 6.0 * double(k[m]);
   } else {
-    ld<DIR>(acc + v * N, B, PS, s);
+    ld<DIR>(acc + v * N, B, PS, a);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) s[m] = s[m] + 2 * double(k[m]);
+    for (int m = 0; m < PPT; ++m) a[m] = a[m] + dt / 3.0 * double(k[m]);
   }
-  st<DIR>(acc + v * N, B, PS, s);
+  st<DIR>(acc + v * N, B, PS, a);
+  const double c_stage = mode == 2 ? dt : dt * 0.5;
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) s[m] = q0[m] + c_stage * double(k[m]);
-  st<DIR>(qs_out + v * N, B, PS, s);
+  for (int m = 0; m < PPT; ++m) a[m] = q0[m] + c_stage * double(k[m]);
+  st<DIR>(qs_out + v * N, B, PS, a);
 }
 
 // ----------------------------------------------------------------------------
@@ -400,12 +404,11 @@ __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__r
   // final stage optionally the x filter of the new state
   auto finish = [&](int v, T(&k)[PPT]) {
     if (mode == 3 && fuse_filter) {
-      double q0[PPT], s[PPT];
-      ld<DIR>(qn + v * N, B, PS, q0);
-      ld<DIR>(acc + v * N, B, PS, s);
+      double a[PPT];
+      ld<DIR>(acc + v * N, B, PS, a);
       T f[PPT], x[PPT];
 #pragma unroll
-      for (int m = 0; m < PPT; ++m) f[m] = T(q0[m] + dt / 6.0 * (s[m] + double(k[m])));
+      for (int m = 0; m < PPT; ++m) f[m] = T(a[m] + dt / 6.0 * double(k[m]));
       line_solve<T, PER>(fop, lds, tp, f, x);
       st<DIR>(qn + v * N, B, PS, x);
     } else {
@@ -473,7 +476,7 @@ __global__ __launch_bounds__(NT) void k_diff(const T *__restrict__ in, T *__rest
 // Filter of nv fields in place (filter_4d): q = B_f^-1 A_f q per field.
 template <typename T, int DIR, bool PER>
 __global__ __launch_bounds__(NT) void k_filter(double *__restrict__ q, LineOp<T> op, SweepGeom g,
-                                               long N, int nv) {
+                                               long N, int nv, T *__restrict__ copy) {
   __shared__ LineLds<T> lds;
   const ThreadPos tp = thread_pos<DIR>(g);
   double a[PPT];
@@ -486,6 +489,9 @@ __global__ __launch_bounds__(NT) void k_filter(double *__restrict__ q, LineOp<T>
     if (v + 1 < nv) ld<DIR>(q + (v + 1) * N, tp.base, g.PS, a);
     line_solve<T, PER>(op, lds, tp, f, x);
     st<DIR>(q + v * N, tp.base, g.PS, x);
+    // the filtered state in the compute precision, when the next RK stage
+    // reads it (it equals the stored FP64 value converted back)
+    if (copy) st<DIR>(copy + v * N, tp.base, g.PS, x);
   }
 }
 

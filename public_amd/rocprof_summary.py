@@ -22,7 +22,7 @@ import sys
 
 _SWEEP_NAMES = {
     ("k_sweep1", "0"): "sweep1_x", ("k_sweep1", "1"): "sweep1_y", ("k_sweep1", "2"): "sweep1_z",
-    ("k_sweep2", "2"): "sweep2_z", ("k_sweep2", "0"): "sweep2_x", ("k_sweep2", "1"): "sweep2_y_rk",
+    ("k_sweep2", "2"): "sweep2_z", ("k_sweep2", "1"): "sweep2_y", ("k_sweep2", "0"): "sweep2_x_rk",
     ("k_filter", "0"): "filter_x", ("k_filter", "1"): "filter_y", ("k_filter", "2"): "filter_z",
     ("k_asweep", "0"): "asweep_x_rk", ("k_asweep", "1"): "asweep_y", ("k_asweep", "2"): "asweep_z",
     ("k_diff", "0"): "diff", ("k_diff", "1"): "diff", ("k_diff", "2"): "diff",
