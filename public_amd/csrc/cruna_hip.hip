@@ -85,6 +85,8 @@ struct cruna_hip_ctx {
   long N = 0;
   SweepGeom geom[3]{};
   int wg[3] = {0, 0, 0};
+  SweepGeom fgeom[3]{};  // filter passes (FP32: up to 32 y/z lines per work-group)
+  int fwg[3] = {0, 0, 0};
   DevOp<double> dop64[3], fop64[3];
   DevOp<float> dop32[3], fop32[3];
   Phys phys{};
@@ -116,41 +118,60 @@ struct cruna_hip_ctx {
 
 namespace {
 
-int build_geom(cruna_hip_ctx *c, int d) {
-  const int n1 = c->cfg.n[0], n2 = c->cfg.n[1], n3 = c->cfg.n[2];
-  const int n = c->cfg.n[d];
-  SweepGeom &g = c->geom[d];
+// Lines per work-group: TL lines of NB threads each (NB * TL <= wg_max); y/z
+// tiles are TL neighbouring x-positions, so their rows are TL elements long.
+SweepGeom make_geom(const cruna_hip_cfg &cfg, int d, int TL) {
+  SweepGeom g{};
+  const int n1 = cfg.n[0], n2 = cfg.n[1], n3 = cfg.n[2];
   g.dir = d;
-  g.n = n;
-  g.NB = n / PPT;
+  g.n = cfg.n[d];
+  g.NB = cfg.n[d] / PPT;
   g.n1 = n1;
   g.n2 = n2;
   g.n3 = n3;
-  const long lines_contig = d == 0 ? (long)n2 * n3 : n1;
-  int TL = c->cfg.tile_lines[d];
-  if (TL == 0) {
-    // x-lines are contiguous: up to 256 threads; y/z tiles take up to 16
-    // neighbouring x-lines (128-byte rows in FP64) in up to 512 threads
-    const int wg_max = d == 0 ? 256 : NT;
-    TL = 1;
-    while (g.NB * TL * 2 <= wg_max && lines_contig % (TL * 2) == 0 && (d == 0 || TL < 16)) TL *= 2;
-  } else if (TL < 0 || (TL & (TL - 1)) != 0 || lines_contig % TL != 0 || g.NB * TL > NT) {
-    return fail(CRUNA_HIP_EINVAL, "tile_lines[" + std::to_string(d) + "] = " + std::to_string(TL) +
-                                      ": needs a power of two dividing " +
-                                      std::to_string(lines_contig) + " with (n/8)*tile_lines <= " +
-                                      std::to_string(NT));
-  }
   g.TL = TL;
   if (d == 0) {
     g.PS = 1;
-    g.ntiles = (int)(lines_contig / TL);
+    g.ntiles = (int)((long)n2 * n3 / TL);
     g.ntiles_in = 1;
   } else {
     g.PS = d == 1 ? n1 : (long)n1 * n2;
     g.ntiles_in = n1 / TL;
     g.ntiles = g.ntiles_in * (d == 1 ? n3 : n2);
   }
-  c->wg[d] = g.NB * TL;
+  return g;
+}
+
+int auto_tl(const cruna_hip_cfg &cfg, int d, int wg_max, int tl_max) {
+  const long lines_contig = d == 0 ? (long)cfg.n[1] * cfg.n[2] : cfg.n[0];
+  const int NB = cfg.n[d] / PPT;
+  int TL = 1;
+  while (NB * TL * 2 <= wg_max && lines_contig % (TL * 2) == 0 && TL * 2 <= tl_max) TL *= 2;
+  return TL;
+}
+
+int build_geom(cruna_hip_ctx *c, int d) {
+  const cruna_hip_cfg &cfg = c->cfg;
+  const long lines_contig = d == 0 ? (long)cfg.n[1] * cfg.n[2] : cfg.n[0];
+  const int NB = cfg.n[d] / PPT;
+  int TL = cfg.tile_lines[d];
+  if (TL == 0) {
+    // x-lines are contiguous: up to 256 threads; y/z tiles take up to 16
+    // neighbouring x-lines (128-byte rows in FP64) in up to 512 threads
+    TL = d == 0 ? auto_tl(cfg, d, 256, 1 << 20) : auto_tl(cfg, d, NT, 16);
+  } else if (TL < 0 || (TL & (TL - 1)) != 0 || lines_contig % TL != 0 || NB * TL > NT) {
+    return fail(CRUNA_HIP_EINVAL, "tile_lines[" + std::to_string(d) + "] = " + std::to_string(TL) +
+                                      ": needs a power of two dividing " +
+                                      std::to_string(lines_contig) + " with (n/8)*tile_lines <= " +
+                                      std::to_string(NT));
+  }
+  c->geom[d] = make_geom(cfg, d, TL);
+  c->wg[d] = NB * TL;
+  // filter passes: FP32 y/z tiles up to 32 lines (128-byte rows of floats)
+  int FTL = TL;
+  if (d != 0 && cfg.precision == CRUNA_HIP_FP32_MIXED) FTL = auto_tl(cfg, d, NT_FILTER32, 32);
+  c->fgeom[d] = make_geom(cfg, d, FTL);
+  c->fwg[d] = NB * FTL;
   return CRUNA_HIP_OK;
 }
 
@@ -379,10 +400,24 @@ int rk4_step(cruna_hip_ctx *c, bool fuse_filter = false) {
 }
 
 // copy: also store the filtered state in the compute precision there (or null)
+template <typename T, int DIR>
+void launch_filter(cruna_hip_ctx *c, double *q, int nv, T *copy) {
+  const dim3 grid(c->fgeom[DIR].ntiles), block(c->fwg[DIR]);
+  const LineOp<T> &op = fops<T>(c)[DIR].op;
+  if (c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC)
+    hipLaunchKernelGGL((k_filter<T, DIR, true>), grid, block, 0, c->stream, q, op, c->fgeom[DIR],
+                       c->N, nv, copy);
+  else
+    hipLaunchKernelGGL((k_filter<T, DIR, false>), grid, block, 0, c->stream, q, op, c->fgeom[DIR],
+                       c->N, nv, copy);
+}
+
 template <typename T>
 int filter_dir(cruna_hip_ctx *c, double *q, int d, int nv, T *copy = nullptr) {
   ProfScope p(c, KID_FX + d);
-  DISPATCH_DIR(c, d, k_filter, T, q, fops<T>(c)[d].op, c->geom[d], c->N, nv, copy);
+  if (d == 0) launch_filter<T, 0>(c, q, nv, copy);
+  if (d == 1) launch_filter<T, 1>(c, q, nv, copy);
+  if (d == 2) launch_filter<T, 2>(c, q, nv, copy);
   return check_launch();
 }
 

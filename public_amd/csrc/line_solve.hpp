@@ -15,7 +15,9 @@
 
 namespace cruna {
 
-constexpr int NT = 512;  // max threads per work-group
+constexpr int NT = 512;  // max threads per work-group of the RHS sweeps
+// the FP32 filter's work-groups take up to 32 y/z lines (128-byte rows of floats)
+constexpr int NT_FILTER32 = 1024;
 
 template <typename T>
 struct LineOp {
@@ -110,11 +112,11 @@ __device__ __forceinline__ void line_sync(bool wave_line) {
   }
 }
 
-// LDS scratch of one work-group for the line engine.
-template <typename T>
+// LDS scratch of one work-group for the line engine, for up to NTM threads.
+template <typename T, int NTM = NT>
 struct LineLds {
-  T halo[NT * 2 * HALO];
-  T z[2][NT * 2];
+  T halo[NTM * 2 * HALO];
+  T z[2][NTM * 2];
 };
 
 // x = B^-1 A f on the thread's block (unit spacing; the caller scales by 1/h).
@@ -135,8 +137,8 @@ __device__ __forceinline__ cptr<T> launder_uniform(const T *p) {
   return (cptr<T>)(p);
 }
 
-template <typename T, bool PER>
-__device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T> &s, const ThreadPos &tp,
+template <typename T, bool PER, int NTM>
+__device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T, NTM> &s, const ThreadPos &tp,
                                            const T (&f)[PPT], T (&x)[PPT]) {
   const int tid = threadIdx.x;
   const int NB = op.NB;

@@ -474,10 +474,14 @@ __global__ __launch_bounds__(NT) void k_diff(const T *__restrict__ in, T *__rest
 }
 
 // Filter of nv fields in place (filter_4d): q = B_f^-1 A_f q per field.
+template <typename T>
+constexpr int filter_nt() { return sizeof(T) == 4 ? NT_FILTER32 : NT; }
+
 template <typename T, int DIR, bool PER>
-__global__ __launch_bounds__(NT) void k_filter(double *__restrict__ q, LineOp<T> op, SweepGeom g,
-                                               long N, int nv, T *__restrict__ copy) {
-  __shared__ LineLds<T> lds;
+__global__ __launch_bounds__(filter_nt<T>()) void k_filter(double *__restrict__ q, LineOp<T> op,
+                                                           SweepGeom g, long N, int nv,
+                                                           T *__restrict__ copy) {
+  __shared__ LineLds<T, filter_nt<T>()> lds;
   const ThreadPos tp = thread_pos<DIR>(g);
   double a[PPT];
   ld<DIR>(q, tp.base, g.PS, a);
