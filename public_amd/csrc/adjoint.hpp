@@ -49,19 +49,22 @@ __device__ __forceinline__ void finish(int v, T (&r)[PPT], long B, long PS, long
 
 // One adjoint sweep along DIR.
 //   qs   : adjoint stage input (5 fields, type Q)
-//   q    : forward state of the step, frozen over the stages (5 fields, FP64)
+//   q0p  : device cell holding the forward state of the step (5 fields, FP64),
+//          frozen over the stages; the cell lets one recorded launch sequence
+//          serve every step of a backward loop (the slot is set on the device)
 //   rhs  : 5 accumulator fields (written by the first sweep, z)
 //   gobj : objective gradient g (5 fields) or null; added by the x sweep
 // The x sweep (LAST) completes the stage: mode 0..3 RK4 update with step dt
 // (negative for the backward integration), mode 4 writes the RHS.
 template <typename T, typename Q, int DIR, bool PER>
-__global__ __launch_bounds__(NT) void k_asweep(const Q *qs, const double *__restrict__ q,
+__global__ __launch_bounds__(NT) void k_asweep(const Q *qs, const double *const *__restrict__ q0p,
                                                T *__restrict__ rhs, const double *__restrict__ gobj,
                                                LineOp<T> op, SweepGeom g, double gamma,
                                                double inv_h, long N, int mode,
                                                double *__restrict__ qn, double *__restrict__ acc,
                                                T *qs_out, double dt) {
   constexpr bool FIRST = DIR == 2, LAST = DIR == 0;
+  const double *__restrict__ q = *q0p;
   __shared__ LineLds<T> lds;
   const ThreadPos tp = thread_pos<DIR>(g);
   const long B = tp.base, PS = g.PS;
@@ -163,9 +166,10 @@ __global__ __launch_bounds__(NT) void k_asweep(const Q *qs, const double *__rest
 // points of one box: removes the incoming adjoint characteristics of qs.
 template <typename Q>
 __global__ __launch_bounds__(256) void k_bc_characteristic_qs(Q *__restrict__ qs,
-                                                               const double *__restrict__ q,
+                                                               const double *const *__restrict__ q0p,
                                                                BoxRow row, int n1, int n2, long N,
                                                                double gamma) {
+  const double *__restrict__ q = *q0p;
   const long nx = row.i1 - row.i0 + 1, ny = row.j1 - row.j0 + 1, nz = row.k1 - row.k0 + 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nx * ny * nz) return;
@@ -228,6 +232,22 @@ __global__ __launch_bounds__(256) void k_bc_sponge_qs(Q *__restrict__ qs,
     const double x = double(qs[v * N + o]);
     qs[v * N + o] = Q(x - s * x);
   }
+}
+
+// The forward state of the next backward step: the step counter *ntp (set
+// by the host before the loop) selects trajectory slot nt (or the only slot
+// of a one-slot trajectory), then counts down.  One thread writes.
+__global__ void k_adjoint_slot(const double **q0p, int *ntp, const double *traj, long stride,
+                               int nslots) {
+  if (threadIdx.x != 0) return;
+  const int nt = *ntp;
+  q0p[0] = traj + (long)(nslots > 1 ? nt - 1 : 0) * stride;
+  ntp[0] = nt - 1;
+}
+
+// Point the forward-state cell at a given state.  One thread writes.
+__global__ void k_set_state_ptr(const double **q0p, const double *q) {
+  if (threadIdx.x == 0) q0p[0] = q;
 }
 
 }  // namespace cruna

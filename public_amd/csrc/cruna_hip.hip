@@ -102,6 +102,10 @@ struct cruna_hip_ctx {
   bool gt = false;
   // adjoint state qs^n and the forward trajectory Q(:,:,:,:,nt) (slot nt-1)
   double *qa = nullptr, *traj = nullptr, *gobj = nullptr;
+  // device cells: the forward state of the current adjoint step and the
+  // backward step counter (k_adjoint_slot)
+  const double **q0p = nullptr;
+  int *ntp = nullptr;
   int traj_slots = 0;
   bool record = false;
   void *qs = nullptr, *rhs = nullptr, *G = nullptr;  // compute precision
@@ -479,13 +483,14 @@ int copy_rhs_out(cruna_hip_ctx *c, double *rhs) {
 // order: 201 characteristic_qs toward zero with the forward state q, 401
 // sponge_qs; 400 has no adjoint-state part.
 template <typename Q>
-int bc_qs(cruna_hip_ctx *c, Q *qs, const double *q) {
+int bc_qs(cruna_hip_ctx *c, Q *qs) {
   for (auto &r : c->bc) {
     if (r.first == 201) {
       const BoxRow &b = r.second;
       const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
       hipLaunchKernelGGL((k_bc_characteristic_qs<Q>), dim3((np + 255) / 256), dim3(256), 0,
-                         c->stream, qs, q, b, c->cfg.n[0], c->cfg.n[1], c->N, c->cfg.gamma);
+                         c->stream, qs, (const double *const *)c->q0p, b, c->cfg.n[0], c->cfg.n[1],
+                         c->N, c->cfg.gamma);
     } else if (r.first == 401) {
       if (!c->shape)
         return fail(CRUNA_HIP_EINVAL, "sponge rows need a shape (cruna_hip_set_sponge_shape)");
@@ -500,66 +505,73 @@ int bc_qs(cruna_hip_ctx *c, Q *qs, const double *q) {
 }
 
 template <typename T, typename Q, int DIR>
-int launch_asweep(cruna_hip_ctx *c, const Q *in, const double *q0, const double *g, int mode) {
+int launch_asweep(cruna_hip_ctx *c, const Q *in, const double *g, int mode) {
   ProfScope p(c, DIR == 0 ? KID_AX : (DIR == 1 ? KID_AY : KID_AZ));
   const dim3 grid(c->geom[DIR].ntiles), block(c->wg[DIR]);
   T *rhs = (T *)c->rhs, *qs = (T *)c->qs;
   const double dt = -c->cfg.dt;  // backward in time (rk_o4_e.f90:62-105)
   if (c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC)
-    hipLaunchKernelGGL((k_asweep<T, Q, DIR, true>), grid, block, 0, c->stream, in, q0, rhs, g,
+    hipLaunchKernelGGL((k_asweep<T, Q, DIR, true>), grid, block, 0, c->stream, in,
+                       (const double *const *)c->q0p, rhs, g,
                        dops<T>(c)[DIR].op, c->geom[DIR], c->cfg.gamma, c->phys.inv_h[DIR], c->N,
                        mode, c->qa, c->acc, qs, dt);
   else
-    hipLaunchKernelGGL((k_asweep<T, Q, DIR, false>), grid, block, 0, c->stream, in, q0, rhs, g,
+    hipLaunchKernelGGL((k_asweep<T, Q, DIR, false>), grid, block, 0, c->stream, in,
+                       (const double *const *)c->q0p, rhs, g,
                        dops<T>(c)[DIR].op, c->geom[DIR], c->cfg.gamma, c->phys.inv_h[DIR], c->N,
                        mode, c->qa, c->acc, qs, dt);
   return check_launch();
 }
 
-// right_hand_side_adjoint_3d from stage input `in` with forward state q0:
-// sweeps z, y, x; the x sweep adds g and finishes the stage (mode as rhs_stage).
+// right_hand_side_adjoint_3d from stage input `in` with the forward state in
+// the device cell q0p: sweeps z, y, x; the x sweep adds g and finishes the
+// stage (mode as rhs_stage).
 template <typename T, typename Q>
-int arhs_stage(cruna_hip_ctx *c, const Q *in, const double *q0, const double *g, int mode) {
-  if (int rc = launch_asweep<T, Q, 2>(c, in, q0, g, mode)) return rc;
-  if (int rc = launch_asweep<T, Q, 1>(c, in, q0, g, mode)) return rc;
-  return launch_asweep<T, Q, 0>(c, in, q0, g, mode);
+int arhs_stage(cruna_hip_ctx *c, const Q *in, const double *g, int mode) {
+  if (int rc = launch_asweep<T, Q, 2>(c, in, g, mode)) return rc;
+  if (int rc = launch_asweep<T, Q, 1>(c, in, g, mode)) return rc;
+  return launch_asweep<T, Q, 0>(c, in, g, mode);
 }
 
 // An adjoint stage from the FP64 state `from` or, with from == nullptr, from
 // the stage input in qs.  With 201/401 rows the RHS works on a private copy
 // to which set_boundary_condition_qs is applied (ns_rho_rhou_p.f90:288-290).
 template <typename T>
-int arun_stage(cruna_hip_ctx *c, const double *from, const double *q0, const double *g, int mode) {
+int arun_stage(cruna_hip_ctx *c, const double *from, const double *g, int mode) {
   T *qs = (T *)c->qs;
   c->qs_fresh = false;
-  if (from && !c->has_bcq) return arhs_stage<T, double>(c, from, q0, g, mode);
+  if (from && !c->has_bcq) return arhs_stage<T, double>(c, from, g, mode);
   if (from) {
     const long n = 5 * c->N;
     hipLaunchKernelGGL((k_copy_state<T>), dim3((n + 255) / 256), dim3(256), 0, c->stream, qs, from, n);
     if (int rc = check_launch()) return rc;
   }
   if (c->has_bcq)
-    if (int rc = bc_qs<T>(c, qs, q0)) return rc;
-  return arhs_stage<T, T>(c, (const T *)qs, q0, g, mode);
+    if (int rc = bc_qs<T>(c, qs)) return rc;
+  return arhs_stage<T, T>(c, (const T *)qs, g, mode);
 }
 
 // time_stepper_adjoint (rk_o4_e.f90:62-105) on the device adjoint state qa.
 template <typename T>
-int ark4_step(cruna_hip_ctx *c, const double *q0, const double *g) {
-  if (int rc = arun_stage<T>(c, c->qa, q0, g, 0)) return rc;
+int ark4_step(cruna_hip_ctx *c, const double *g) {
+  if (int rc = arun_stage<T>(c, c->qa, g, 0)) return rc;
   for (int st = 1; st < 4; ++st)
-    if (int rc = arun_stage<T>(c, nullptr, q0, g, st)) return rc;
+    if (int rc = arun_stage<T>(c, nullptr, g, st)) return rc;
   return CRUNA_HIP_OK;
 }
 
 // One step of the calc_adjoint loop (computation_adjoint.f90:72-99) with g = 0:
 // stepper -> BC on qs (outside rhs) -> adjoint filters where mod(nt, freq) == 0.
-int astep_once(cruna_hip_ctx *c, int nt, const double *q0) {
+int astep_once(cruna_hip_ctx *c, int nt) {
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
-  int rc = mixed ? ark4_step<float>(c, q0, nullptr) : ark4_step<double>(c, q0, nullptr);
+  // q0 = Q(:,:,:,:,nt) into the device cell; the counter then holds nt - 1
+  hipLaunchKernelGGL(k_adjoint_slot, dim3(1), dim3(64), 0, c->stream, c->q0p, c->ntp,
+                     (const double *)c->traj, 5 * c->N, c->traj_slots);
+  if (int rc = check_launch()) return rc;
+  int rc = mixed ? ark4_step<float>(c, nullptr) : ark4_step<double>(c, nullptr);
   if (rc) return rc;
   if (c->has_bcq)
-    if ((rc = bc_qs<double>(c, c->qa, q0))) return rc;
+    if ((rc = bc_qs<double>(c, c->qa))) return rc;
   for (int d = 0; d < 3; ++d) {
     const int fq = c->cfg.afilter_freq[d];
     if (fq > 0 && nt % fq == 0) {
@@ -572,7 +584,19 @@ int astep_once(cruna_hip_ctx *c, int nt, const double *q0) {
 
 int ensure_adjoint(cruna_hip_ctx *c) {
   if (!c->qa) HIP_TRY(hipMalloc(&c->qa, 5 * c->N * sizeof(double)));
+  if (!c->q0p) {
+    void *cells = nullptr;
+    HIP_TRY(hipMalloc(&cells, 256));  // one pointer, one counter
+    c->q0p = (const double **)cells;
+    c->ntp = (int *)((char *)cells + 128);
+  }
   return CRUNA_HIP_OK;
+}
+
+// The forward state of the next adjoint stages is the host-supplied state q.
+int point_q0(cruna_hip_ctx *c, const double *q) {
+  hipLaunchKernelGGL(k_set_state_ptr, dim3(1), dim3(64), 0, c->stream, c->q0p, q);
+  return check_launch();
 }
 
 double *traj_slot(cruna_hip_ctx *c, int nt) { return c->traj + (size_t)(nt - 1) * 5 * c->N; }
@@ -731,7 +755,7 @@ int cruna_hip_destroy(cruna_hip_ctx *c) {
     c->fop32[d].release();
   }
   for (void *p : {(void *)c->qn, (void *)c->acc, (void *)c->qref, (void *)c->shape, (void *)c->tmp,
-                  (void *)c->qa, (void *)c->traj, (void *)c->gobj, c->met, c->qs, c->rhs, c->G})
+                  (void *)c->qa, (void *)c->traj, (void *)c->gobj, c->met, (void *)c->q0p, c->qs, c->rhs, c->G})
     if (p) (void)hipFree(p);
   for (auto &p : c->pending) {
     (void)hipEventDestroy(p.second.first);
@@ -880,9 +904,10 @@ int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const
   if (!c || !rhs || !qs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->gt) return fail(CRUNA_HIP_EINVAL, "the deformed-grid adjoint is not implemented on the device");
   if (int rc = stage_adjoint_inputs(c, qs, q, g)) return rc;
-  const double *q0 = c->tmp + 5 * c->N, *gd = g ? c->gobj : nullptr;
+  const double *gd = g ? c->gobj : nullptr;
+  if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
-  int rc = mixed ? arun_stage<float>(c, c->qa, q0, gd, 4) : arun_stage<double>(c, c->qa, q0, gd, 4);
+  int rc = mixed ? arun_stage<float>(c, c->qa, gd, 4) : arun_stage<double>(c, c->qa, gd, 4);
   if (rc) return rc;
   return copy_rhs_out(c, rhs);
 }
@@ -906,9 +931,10 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *
   if (!c || !qs1 || !qs0 || !q0) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->gt) return fail(CRUNA_HIP_EINVAL, "the deformed-grid adjoint is not implemented on the device");
   if (int rc = stage_adjoint_inputs(c, qs0, q0, g)) return rc;
-  const double *qf = c->tmp + 5 * c->N, *gd = g ? c->gobj : nullptr;
+  const double *gd = g ? c->gobj : nullptr;
+  if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
-  int rc = mixed ? ark4_step<float>(c, qf, gd) : ark4_step<double>(c, qf, gd);
+  int rc = mixed ? ark4_step<float>(c, gd) : ark4_step<double>(c, gd);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(qs1, c->qa, 5 * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -983,8 +1009,10 @@ int cruna_hip_run_adjoint(cruna_hip_ctx *c, int nt_end, int nsteps) {
   // one slot: the linear adjoint about a fixed state (computation_adjoint.f90:69-73)
   if (c->traj_slots > 1 && nsteps > 0 && (nt_end > c->traj_slots || nt_end - nsteps + 1 < 1))
     return fail(CRUNA_HIP_EINVAL, "steps outside the trajectory");
+  HIP_TRY(hipMemcpyAsync(c->ntp, &nt_end, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));  // nt_end lives on this stack frame
   for (int nt = nt_end; nt > nt_end - nsteps; --nt)
-    if (int rc = astep_once(c, nt, traj_slot(c, c->traj_slots > 1 ? nt : 1))) return rc;
+    if (int rc = astep_once(c, nt)) return rc;
   return CRUNA_HIP_OK;
 }
 
