@@ -146,6 +146,40 @@ int cruna_hip_trajectory_record(cruna_hip_ctx *ctx, int on);
  * where mod(nt, afilter_freq) == 0.  Asynchronous on the context stream. */
 int cruna_hip_run_adjoint(cruna_hip_ctx *ctx, int nt_end, int nsteps);
 
+/* Tile-shape autotuner: times `steps` RK4 steps (after one warm-up step)
+ * on a saved copy of the state for each valid number of lines per
+ * work-group, one direction at a time, keeps the fastest in
+ * cfg.tile_lines and restores the state.  tile_lines (may be NULL) receives
+ * the choice.  Results do not depend on the tile shape. */
+int cruna_hip_autotune(cruna_hip_ctx *ctx, int steps, int tile_lines[3]);
+
+/* Ensembles: independent members on one device -- unevenly sized boxes,
+ * forward or adjoint, e.g. the seeds of a study or the direct and adjoint
+ * sweeps of an optimisation.  Each member is a context with its own stream,
+ * reachable through cruna_hip_ensemble_member for set_state, trajectories,
+ * etc.  cruna_hip_ensemble_run advances every member by nsteps: the
+ * launches of one period of steps of all members (the period after which
+ * the filter frequencies repeat) are recorded once into a HIP graph whose
+ * member branches run concurrently, and the graph is replayed; steps that
+ * do not fill a period, and the first period of a member, run eagerly.
+ * Forward members run calc_direct's step body from their step counter;
+ * adjoint members run calc_adjoint's backward from step nt over their
+ * trajectory.  Members must not be used on their own while a run is in
+ * progress. */
+#define CRUNA_HIP_FORWARD 0
+#define CRUNA_HIP_ADJOINT 1
+typedef struct cruna_hip_ensemble cruna_hip_ensemble;
+int cruna_hip_ensemble_create(const cruna_hip_cfg *cfgs, int nmembers, cruna_hip_ensemble **out);
+int cruna_hip_ensemble_destroy(cruna_hip_ensemble *ens);
+int cruna_hip_ensemble_member(cruna_hip_ensemble *ens, int i, cruna_hip_ctx **ctx);
+/* role CRUNA_HIP_FORWARD: nt = steps already done; CRUNA_HIP_ADJOINT: nt =
+ * the first (latest) step to run backward (needs the member's adjoint state
+ * and trajectory set). */
+int cruna_hip_ensemble_set_role(cruna_hip_ensemble *ens, int i, int role, int nt);
+int cruna_hip_ensemble_run(cruna_hip_ensemble *ens, int nsteps);
+/* cruna_hip_autotune on every member. */
+int cruna_hip_ensemble_autotune(cruna_hip_ensemble *ens, int steps);
+
 /* Per-kernel timing with HIP events on the context stream: enable, run,
  * then read the summed milliseconds and launch counts per kernel id. */
 int cruna_hip_profile_enable(cruna_hip_ctx *ctx, int on);

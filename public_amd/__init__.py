@@ -5,5 +5,5 @@ Python host mirror of the reference's interface and `ic` builds the seeded
 synthetic workloads.
 """
 from . import ic  # noqa: F401
-from .cruna import (FP32_MIXED, FP64, OPEN, PERIODIC, POWER, SUTHERLAND, Cruna,  # noqa: F401
-                    CrunaError, lib)
+from .cruna import (ADJOINT, FORWARD, FP32_MIXED, FP64, OPEN, PERIODIC, POWER,  # noqa: F401
+                    SUTHERLAND, Cruna, CrunaError, Ensemble, lib)

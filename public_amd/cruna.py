@@ -75,6 +75,13 @@ def lib():
         L.cruna_hip_set_qref.argtypes = [vp, dp]
         L.cruna_hip_set_sponge_shape.argtypes = [vp, dp]
         L.cruna_hip_set_metric.argtypes = [vp, dp, dp]
+        L.cruna_hip_autotune.argtypes = [vp, ip, C.POINTER(C.c_int)]
+        L.cruna_hip_ensemble_create.argtypes = [C.POINTER(Cfg), ip, C.POINTER(vp)]
+        L.cruna_hip_ensemble_destroy.argtypes = [vp]
+        L.cruna_hip_ensemble_member.argtypes = [vp, ip, C.POINTER(vp)]
+        L.cruna_hip_ensemble_set_role.argtypes = [vp, ip, ip, ip]
+        L.cruna_hip_ensemble_run.argtypes = [vp, ip]
+        L.cruna_hip_ensemble_autotune.argtypes = [vp, ip]
         L.cruna_hip_diff.argtypes = [vp, ip, dp, dp, C.c_double]
         L.cruna_hip_filter.argtypes = [vp, ip, dp, ip]
         L.cruna_hip_rhs_direct.argtypes = [vp, dp, dp, C.c_double]
@@ -97,7 +104,9 @@ def lib():
                   "filter", "rhs_direct", "rhs_adjoint", "time_stepper_direct", "time_stepper_adjoint",
                   "run_direct", "synchronize", "profile_enable", "profile_read", "profile_count",
                   "set_adjoint_state", "get_adjoint_state", "trajectory_alloc", "trajectory_set",
-                  "trajectory_get", "trajectory_record", "run_adjoint", "set_metric"):
+                  "trajectory_get", "trajectory_record", "run_adjoint", "set_metric", "autotune",
+                  "ensemble_create", "ensemble_destroy", "ensemble_member", "ensemble_set_role",
+                  "ensemble_run", "ensemble_autotune"):
             getattr(L, "cruna_hip_" + f).restype = C.c_int
         _lib = L
     return _lib
@@ -119,41 +128,51 @@ def _f64(a, shape):
     return a
 
 
+def make_cfg(n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0, W=1.0, Pr=0.71,
+             mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76, dt=1e-3,
+             filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), heat_unit_spacing=1, precision=FP64,
+             bc=(), tile_lines=(0, 0, 0)):
+    """cruna_hip_cfg from the reference's parameter names (defaults of the library)."""
+    cfg = Cfg()
+    lib().cruna_hip_default_cfg(C.byref(cfg))
+    for d in range(3):
+        cfg.n[d] = int(n[d])
+        cfg.dx[d] = float(dx[d])
+        cfg.scheme[d] = int(scheme[d])
+        cfg.filter_freq[d] = int(filter_freq[d])
+        cfg.afilter_freq[d] = int(afilter_freq[d])
+        cfg.tile_lines[d] = int(tile_lines[d])
+    cfg.filter_alpha = filter_alpha
+    cfg.gamma, cfg.R, cfg.W, cfg.Pr, cfg.mu0, cfg.T0 = gamma, R, W, Pr, mu0, T0
+    cfg.visc_law, cfg.visc_exp, cfg.dt = visc_law, visc_exp, dt
+    cfg.heat_unit_spacing, cfg.precision = heat_unit_spacing, precision
+    cfg.nbc = len(bc)
+    for r, row in enumerate(bc):
+        for j, v in enumerate(row):
+            cfg.bc[r][j] = float(v)
+    return cfg
+
+
 class Cruna:
     """One device context: grid, schemes, material constants, resident state."""
 
-    def __init__(self, n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0, W=1.0,
-                 Pr=0.71, mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76, dt=1e-3,
-                 filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), heat_unit_spacing=1, precision=FP64,
-                 bc=(), tile_lines=(0, 0, 0)):
-        L = lib()
-        cfg = Cfg()
-        L.cruna_hip_default_cfg(C.byref(cfg))
-        for d in range(3):
-            cfg.n[d] = int(n[d])
-            cfg.dx[d] = float(dx[d])
-            cfg.scheme[d] = int(scheme[d])
-            cfg.filter_freq[d] = int(filter_freq[d])
-            cfg.afilter_freq[d] = int(afilter_freq[d])
-            cfg.tile_lines[d] = int(tile_lines[d])
-        cfg.filter_alpha = filter_alpha
-        cfg.gamma, cfg.R, cfg.W, cfg.Pr, cfg.mu0, cfg.T0 = gamma, R, W, Pr, mu0, T0
-        cfg.visc_law, cfg.visc_exp, cfg.dt = visc_law, visc_exp, dt
-        cfg.heat_unit_spacing, cfg.precision = heat_unit_spacing, precision
-        cfg.nbc = len(bc)
-        for r, row in enumerate(bc):
-            for j, v in enumerate(row):
-                cfg.bc[r][j] = float(v)
+    def __init__(self, n, dx, _handle=None, **kw):
+        cfg = make_cfg(n, dx, **kw)
         self.cfg = cfg
         self.shape3 = tuple(int(v) for v in n)
         self.shape4 = self.shape3 + (5,)
+        self._owned = _handle is None
+        if _handle is not None:  # a member of an Ensemble, which owns it
+            self._h = _handle
+            return
         h = C.c_void_p()
-        _check(L.cruna_hip_create(C.byref(cfg), C.byref(h)))
+        _check(lib().cruna_hip_create(C.byref(cfg), C.byref(h)))
         self._h = h
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().cruna_hip_destroy(self._h)
+            if self._owned:
+                lib().cruna_hip_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -259,6 +278,14 @@ class Cruna:
         Xj = _f64(X_jacobian, self.shape3)
         _check(lib().cruna_hip_set_metric(self._h, _dp(Xm), _dp(Xj)))
 
+    def autotune(self, steps=2):
+        """Pick the lines per work-group of each direction by timing; returns them."""
+        out = (C.c_int * 3)()
+        _check(lib().cruna_hip_autotune(self._h, int(steps), out))
+        for d in range(3):
+            self.cfg.tile_lines[d] = out[d]
+        return tuple(out)
+
     def run_direct(self, nsteps, nt0=0):
         _check(lib().cruna_hip_run_direct(self._h, int(nt0), int(nsteps)))
 
@@ -309,3 +336,61 @@ class Cruna:
             if n.value:
                 out[name.value.decode()] = (ms.value, n.value)
         return out
+
+
+FORWARD, ADJOINT = 0, 1
+
+
+class Ensemble:
+    """Independent members (dicts of Cruna keyword arguments with n and dx) on
+    one device; run() advances all of them through a replayed HIP graph."""
+
+    def __init__(self, members):
+        cfgs = (Cfg * len(members))()
+        for i, kw in enumerate(members):
+            kw = dict(kw)
+            cfgs[i] = make_cfg(kw.pop("n"), kw.pop("dx"), **kw)
+        h = C.c_void_p()
+        _check(lib().cruna_hip_ensemble_create(cfgs, len(members), C.byref(h)))
+        self._h = h
+        self.members = []
+        for i, kw in enumerate(members):
+            mh = C.c_void_p()
+            _check(lib().cruna_hip_ensemble_member(h, i, C.byref(mh)))
+            kw = dict(kw)
+            self.members.append(Cruna(kw.pop("n"), kw.pop("dx"), _handle=mh, **kw))
+
+    def __len__(self):
+        return len(self.members)
+
+    def __getitem__(self, i):
+        return self.members[i]
+
+    def set_role(self, i, role, nt):
+        _check(lib().cruna_hip_ensemble_set_role(self._h, int(i), int(role), int(nt)))
+
+    def run(self, nsteps):
+        _check(lib().cruna_hip_ensemble_run(self._h, int(nsteps)))
+
+    def autotune(self, steps=2):
+        _check(lib().cruna_hip_ensemble_autotune(self._h, int(steps)))
+        return [tuple(m.cfg.tile_lines) for m in self.members]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for m in self.members:
+                m._h = None
+            lib().cruna_hip_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1029,6 +1029,70 @@ int cruna_hip_run_direct(cruna_hip_ctx *c, int nt0, int nsteps) {
   return CRUNA_HIP_OK;
 }
 
+int cruna_hip_autotune(cruna_hip_ctx *c, int steps, int tile_lines[3]) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (steps < 1) return fail(CRUNA_HIP_EINVAL, "steps must be >= 1");
+  if (int rc = ensure_tmp(c)) return rc;
+  // the trial steps run on a saved copy of the state, restored afterwards
+  const size_t bytes = 5 * (size_t)c->N * sizeof(double);
+  HIP_TRY(hipMemcpyAsync(c->tmp, c->qn, bytes, hipMemcpyDeviceToDevice, c->stream));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const bool prof = c->prof;
+  c->prof = false;
+  int best[3];
+  for (int d = 0; d < 3; ++d) best[d] = c->geom[d].TL;
+  int rc = CRUNA_HIP_OK;
+  auto trial = [&](float *ms) -> int {
+    for (int d = 0; d < 3; ++d) c->cfg.tile_lines[d] = best[d];
+    for (int d = 0; d < 3; ++d)
+      if (int r = build_geom(c, d)) return r;
+    HIP_TRY(hipMemcpyAsync(c->qn, c->tmp, bytes, hipMemcpyDeviceToDevice, c->stream));
+    c->qs_fresh = false;
+    if (int r = step_once(c, 1)) return r;  // warm-up
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int s = 0; s < steps; ++s)
+      if (int r = step_once(c, 2 + s)) return r;
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+    return CRUNA_HIP_OK;
+  };
+  // coordinate search: one direction at a time over the valid line counts
+  for (int d = 0; d < 3 && !rc; ++d) {
+    const long lines = d == 0 ? (long)c->cfg.n[1] * c->cfg.n[2] : c->cfg.n[0];
+    const int NB = c->cfg.n[d] / PPT;
+    int arg = best[d];
+    float tbest = 0.f;
+    bool first = true;
+    for (int TL = 1; NB * TL <= NT && TL <= 64; TL *= 2) {
+      if (lines % TL != 0 || NB * TL < 64) continue;  // at least one full wavefront
+      best[d] = TL;
+      float ms = 0.f;
+      if ((rc = trial(&ms))) break;
+      if (first || ms < tbest) {
+        tbest = ms;
+        arg = TL;
+        first = false;
+      }
+    }
+    best[d] = arg;
+  }
+  c->prof = prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int d = 0; d < 3; ++d) c->cfg.tile_lines[d] = best[d];
+  for (int d = 0; d < 3 && !rc; ++d) rc = build_geom(c, d);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(c->qn, c->tmp, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->qs_fresh = false;
+  if (tile_lines)
+    for (int d = 0; d < 3; ++d) tile_lines[d] = best[d];
+  return CRUNA_HIP_OK;
+}
+
 int cruna_hip_synchronize(cruna_hip_ctx *c) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1062,5 +1126,219 @@ int cruna_hip_profile_read(cruna_hip_ctx *c, int kid, double *ms, long *launches
 }
 
 int cruna_hip_profile_count(void) { return KID_COUNT; }
+
+// ---------------------------------------------------------------------------
+// Ensembles: independent members (forward or adjoint runs of any size) on one
+// device.  Each member keeps its own stream; a run records one period of
+// steps of every member into a HIP graph whose member branches run
+// concurrently, and replays it.
+
+}  // extern "C"
+
+struct cruna_hip_ensemble {
+  std::vector<cruna_hip_ctx *> m;
+  std::vector<int> role;  // CRUNA_HIP_FORWARD / CRUNA_HIP_ADJOINT
+  std::vector<int> nt;    // forward: steps done; adjoint: next step to run backward
+  std::vector<bool> warm;
+  hipStream_t main = nullptr;
+  std::vector<hipEvent_t> fork, join;
+  hipGraphExec_t exec = nullptr;
+  int period = 0;
+  bool dirty = true;  // the recorded graph no longer matches the members
+};
+
+namespace {
+
+int gcd_i(int a, int b) { return b ? gcd_i(b, a % b) : a; }
+int lcm_i(int a, int b) { return a / gcd_i(a, b) * b; }
+
+// Steps after which a member's launch sequence repeats: its filter
+// frequencies (direct or adjoint) decide which passes run at a step.
+int member_period(const cruna_hip_ctx *c, int role) {
+  int p = 1;
+  for (int d = 0; d < 3; ++d) {
+    const int f = role == CRUNA_HIP_ADJOINT ? c->cfg.afilter_freq[d] : c->cfg.filter_freq[d];
+    if (f > 0) p = lcm_i(p, f);
+  }
+  return p;
+}
+
+int member_steps(cruna_hip_ensemble *e, size_t i, int k) {
+  cruna_hip_ctx *c = e->m[i];
+  for (int s = 0; s < k; ++s) {
+    if (e->role[i] == CRUNA_HIP_FORWARD) {
+      if (int rc = step_once(c, ++e->nt[i])) return rc;
+    } else {
+      if (int rc = astep_once(c, e->nt[i]--)) return rc;
+    }
+  }
+  return CRUNA_HIP_OK;
+}
+
+void drop_graph(cruna_hip_ensemble *e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  e->exec = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cruna_hip_ensemble_create(const cruna_hip_cfg *cfgs, int nmembers, cruna_hip_ensemble **out) {
+  if (!cfgs || !out || nmembers < 1) return fail(CRUNA_HIP_EINVAL, "bad ensemble arguments");
+  *out = nullptr;
+  std::unique_ptr<cruna_hip_ensemble> e(new cruna_hip_ensemble());
+  for (int i = 0; i < nmembers; ++i) {
+    cruna_hip_ctx *c = nullptr;
+    if (int rc = cruna_hip_create(&cfgs[i], &c)) {
+      for (auto *x : e->m) cruna_hip_destroy(x);
+      return rc;
+    }
+    e->m.push_back(c);
+    e->role.push_back(CRUNA_HIP_FORWARD);
+    e->nt.push_back(0);
+    e->warm.push_back(false);
+  }
+  auto cleanup = [&](int rc) {
+    cruna_hip_ensemble_destroy(e.release());
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&e->main, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(CRUNA_HIP_EDEVICE, "stream creation failed"));
+  e->fork.resize(nmembers);
+  e->join.resize(nmembers);
+  for (int i = 0; i < nmembers; ++i)
+    if (hipEventCreateWithFlags(&e->fork[i], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) != hipSuccess)
+      return cleanup(fail(CRUNA_HIP_EDEVICE, "event creation failed"));
+  *out = e.release();
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_ensemble_destroy(cruna_hip_ensemble *e) {
+  if (!e) return CRUNA_HIP_OK;
+  if (e->main) (void)hipStreamSynchronize(e->main);
+  drop_graph(e);
+  for (auto ev : e->fork)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto ev : e->join)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto *c : e->m) cruna_hip_destroy(c);
+  if (e->main) (void)hipStreamDestroy(e->main);
+  delete e;
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_ensemble_member(cruna_hip_ensemble *e, int i, cruna_hip_ctx **ctx) {
+  if (!e || !ctx || i < 0 || i >= (int)e->m.size()) return fail(CRUNA_HIP_EINVAL, "bad member");
+  *ctx = e->m[i];
+  e->dirty = true;  // the caller may change the member (state, trajectory, role)
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_ensemble_set_role(cruna_hip_ensemble *e, int i, int role, int nt) {
+  if (!e || i < 0 || i >= (int)e->m.size()) return fail(CRUNA_HIP_EINVAL, "bad member");
+  if (role != CRUNA_HIP_FORWARD && role != CRUNA_HIP_ADJOINT) return fail(CRUNA_HIP_EINVAL, "bad role");
+  cruna_hip_ctx *c = e->m[i];
+  if (role == CRUNA_HIP_ADJOINT) {
+    if (c->gt) return fail(CRUNA_HIP_EINVAL, "the deformed-grid adjoint is not implemented on the device");
+    if (!c->qa || c->traj_slots == 0)
+      return fail(CRUNA_HIP_EINVAL, "an adjoint member needs its adjoint state and trajectory first");
+    if (c->traj_slots > 1 && (nt < 1 || nt > c->traj_slots))
+      return fail(CRUNA_HIP_EINVAL, "nt outside the trajectory");
+  }
+  if (role == CRUNA_HIP_FORWARD && c->record)
+    return fail(CRUNA_HIP_EINVAL, "forward members do not record trajectories");
+  e->role[i] = role;
+  e->nt[i] = nt;
+  e->warm[i] = false;
+  e->dirty = true;
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_ensemble_run(cruna_hip_ensemble *e, int nsteps) {
+  if (!e) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
+  const size_t M = e->m.size();
+  int P = 1;
+  for (size_t i = 0; i < M; ++i) {
+    P = lcm_i(P, member_period(e->m[i], e->role[i]));
+    if (e->role[i] == CRUNA_HIP_ADJOINT && e->m[i]->traj_slots > 1 && e->nt[i] - nsteps < 0)
+      return fail(CRUNA_HIP_EINVAL, "an adjoint member would run past its first step");
+  }
+  if (P > 64) return fail(CRUNA_HIP_EINVAL, "filter frequencies repeat only after more than 64 steps");
+  // adjoint step counters live on the device; set them for this run
+  for (size_t i = 0; i < M; ++i) {
+    cruna_hip_ctx *c = e->m[i];
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (e->role[i] == CRUNA_HIP_ADJOINT)
+      HIP_TRY(hipMemcpy(c->ntp, &e->nt[i], sizeof(int), hipMemcpyHostToDevice));
+    c->prof = false;
+  }
+  int left = nsteps;
+  // eager steps: members not yet in their repeating state, and the remainder
+  // that does not fill a period
+  bool all_warm = true;
+  for (size_t i = 0; i < M; ++i) all_warm = all_warm && e->warm[i];
+  int eager = all_warm ? left % P : std::min(left, P + left % P);
+  for (size_t i = 0; i < M; ++i) {
+    if (int rc = member_steps(e, i, eager)) return rc;
+    if (eager >= P) e->warm[i] = true;
+  }
+  left -= eager;
+  if (left == 0) {
+    for (auto *c : e->m) HIP_TRY(hipStreamSynchronize(c->stream));
+    return CRUNA_HIP_OK;
+  }
+  // record one period of every member as concurrent branches
+  if (e->dirty || !e->exec || e->period != P) {
+    drop_graph(e);
+    for (auto *c : e->m) HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipStreamBeginCapture(e->main, hipStreamCaptureModeThreadLocal));
+    std::vector<int> nt_save = e->nt;
+    int rc = CRUNA_HIP_OK;
+    for (size_t i = 0; i < M && !rc; ++i) {
+      cruna_hip_ctx *c = e->m[i];
+      if (hipEventRecord(e->fork[i], e->main) != hipSuccess ||
+          hipStreamWaitEvent(c->stream, e->fork[i], 0) != hipSuccess) {
+        rc = fail(CRUNA_HIP_EDEVICE, "graph fork failed");
+        break;
+      }
+      rc = member_steps(e, i, P);
+      if (!rc && (hipEventRecord(e->join[i], c->stream) != hipSuccess ||
+                  hipStreamWaitEvent(e->main, e->join[i], 0) != hipSuccess))
+        rc = fail(CRUNA_HIP_EDEVICE, "graph join failed");
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(e->main, &graph);
+    e->nt = nt_save;  // recording ran no step
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (ec != hipSuccess) return fail(CRUNA_HIP_EDEVICE, std::string("capture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      e->exec = nullptr;
+      return fail(CRUNA_HIP_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    }
+    e->period = P;
+    e->dirty = false;
+  }
+  for (int r = 0; r < left / P; ++r) HIP_TRY(hipGraphLaunch(e->exec, e->main));
+  HIP_TRY(hipStreamSynchronize(e->main));
+  for (size_t i = 0; i < M; ++i) e->nt[i] += e->role[i] == CRUNA_HIP_FORWARD ? left : -left;
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_ensemble_autotune(cruna_hip_ensemble *e, int steps) {
+  if (!e) return fail(CRUNA_HIP_EINVAL, "null argument");
+  for (auto *c : e->m)
+    if (int rc = cruna_hip_autotune(c, steps, nullptr)) return rc;
+  e->dirty = true;
+  return CRUNA_HIP_OK;
+}
+
 
 }  // extern "C"
