@@ -168,6 +168,81 @@ def run_shear_layer(steps, warmup, n=(512, 256, 128), seed=0):
             "max_abs_v": float(np.abs(qe[..., 2] / qe[..., 0]).max())}
 
 
+def run_ensemble(steps, nmembers=32, seed=0, autotune=True):
+    """Ensemble of periodic boxes with N drawn from {48, 64, 96, 128}^3 and Ma
+    from U[0.05, 0.3] (seeded): the first half forward RK4 runs, the second
+    half adjoint runs backward over their own forward trajectories of `steps`
+    steps (recorded on the device beforehand, untimed).  Timed: all members
+    advanced by `steps` steps, (a) one member after the other, eagerly, and
+    (b) together through the replayed graph of cruna_hip_ensemble_run."""
+    from public_amd import ADJOINT, FORWARD, Cruna, Ensemble, ic
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([48, 64, 96, 128], nmembers)
+    machs = rng.uniform(0.05, 0.3, nmembers)
+    specs, states = [], []
+    for i in range(nmembers):
+        n = int(sizes[i])
+        q = ic.periodic_box_state(n, seed + i, float(machs[i]))
+        h = ic.periodic_dx(n)
+        kw = dict(n=(n,) * 3, dx=(h,) * 3, mu0=1.0 / 1600.0, T0=1.0 / (1.4 * machs[i] ** 2),
+                  dt=ic.cfl_dt(q, h), afilter_freq=(1, 1, 1))
+        specs.append(kw)
+        states.append(q)
+    adj = set(range(nmembers // 2, nmembers))
+    pts = float(sum(int(s) ** 3 for s in sizes))
+
+    def prepare(get_member):
+        for i in range(nmembers):
+            g = get_member(i)
+            g.set_state(states[i])
+            if i in adj:
+                g.trajectory_alloc(steps)
+                g.trajectory_record(True)
+                g.run_direct(steps)
+                g.trajectory_record(False)
+                g.set_adjoint_state(np.asfortranarray(np.random.default_rng(seed + 100 + i)
+                                                      .standard_normal(states[i].shape)))
+                g.synchronize()
+
+    # (a) one after the other, eagerly
+    solo = []
+    for i in range(nmembers):
+        kw = dict(specs[i])
+        solo.append(Cruna(kw.pop("n"), kw.pop("dx"), **kw))
+    prepare(lambda i: solo[i])
+    t0 = time.perf_counter()
+    for i in range(nmembers):
+        if i in adj:
+            solo[i].run_adjoint(steps, steps)
+        else:
+            solo[i].run_direct(steps)
+        solo[i].synchronize()
+    t_serial = time.perf_counter() - t0
+    ref = [solo[i].get_adjoint_state() if i in adj else solo[i].get_state() for i in range(nmembers)]
+    for g in solo:
+        g.close()
+    # (b) the ensemble
+    with Ensemble(specs) as ens:
+        prepare(lambda i: ens[i])
+        tuned = ens.autotune(1) if autotune else None
+        for i in range(nmembers):
+            ens.set_role(i, ADJOINT if i in adj else FORWARD, steps if i in adj else 0)
+        t0 = time.perf_counter()
+        ens.run(steps)
+        t_graph = time.perf_counter() - t0
+        same = all(np.array_equal(ens[i].get_adjoint_state() if i in adj else ens[i].get_state(), ref[i])
+                   for i in range(nmembers))
+    return {"members": nmembers, "sizes": [int(s) for s in sizes], "forward": nmembers - len(adj),
+            "adjoint": len(adj), "steps": steps, "points_per_step": pts,
+            "serial_eager": {"value": round(pts * steps / t_serial / 1e6, 2), "unit": "Mpts/s",
+                             "seconds": round(t_serial, 4)},
+            "graph": {"value": round(pts * steps / t_graph / 1e6, 2), "unit": "Mpts/s",
+                      "seconds": round(t_graph, 4),
+                      "note": "includes the first (eager) step and recording the graph"},
+            "speedup": round(t_serial / t_graph, 3), "tile_lines": tuned,
+            "identical_to_serial": bool(same)}
+
+
 def ke_history(q0, n, h, dt, precision, steps, every, kw):
     from public_amd import Cruna, ic
     g = Cruna((n,) * 3, (h,) * 3, precision=precision, dt=dt, filter_freq=(1, 1, 1), **kw)
@@ -247,6 +322,8 @@ def main():
                     help="adjoint steps timed after a recorded forward run (0: skip)")
     ap.add_argument("--shear-steps", type=int, default=50,
                     help="timed steps of the stretched-grid shear layer (512x256x128, FP32-mixed; 0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=100,
+                    help="steps of the 32-member forward/adjoint ensemble (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -282,6 +359,7 @@ def main():
                                                     / (v[0] / v[1] / 1e3) / 1e9, 1)}
                                for name, v in prof_a.items()}}
     shear = run_shear_layer(args.shear_steps, 2) if args.shear_steps > 0 else None
+    ensemble = run_ensemble(args.ensemble_steps) if args.ensemble_steps > 0 else None
     pts = float(n) ** 3
     mpts64 = pts * args.steps / secs64 / 1e6
     mpts32 = pts * args.steps / secs32 / 1e6
@@ -358,6 +436,7 @@ def main():
                        "speedup_vs_fp64": round(mpts32 / mpts64, 3), "err_vs_fp64": errors},
         "adjoint_fp64": adjoint,
         "shear_layer_fp32_mixed": shear,
+        "ensemble_fp64": ensemble,
         "roofline": roofline,
         "step_roofline": step_roofline,
         "kernels_fp64": kernels,

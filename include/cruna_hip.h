@@ -158,10 +158,11 @@ int cruna_hip_autotune(cruna_hip_ctx *ctx, int steps, int tile_lines[3]);
  * sweeps of an optimisation.  Each member is a context with its own stream,
  * reachable through cruna_hip_ensemble_member for set_state, trajectories,
  * etc.  cruna_hip_ensemble_run advances every member by nsteps: the
- * launches of one period of steps of all members (the period after which
- * the filter frequencies repeat) are recorded once into a HIP graph whose
- * member branches run concurrently, and the graph is replayed; steps that
- * do not fill a period, and the first period of a member, run eagerly.
+ * launches of one period of steps of each member (the period after which
+ * the filter frequencies repeat) are recorded once into a HIP graph per
+ * member, and the members' graphs are replayed on their own streams so that
+ * members run concurrently; steps that do not fill a period, and the first
+ * period of a member, run eagerly.
  * Forward members run calc_direct's step body from their step counter;
  * adjoint members run calc_adjoint's backward from step nt over their
  * trajectory.  Members must not be used on their own while a run is in
@@ -177,8 +178,9 @@ int cruna_hip_ensemble_member(cruna_hip_ensemble *ens, int i, cruna_hip_ctx **ct
  * and trajectory set). */
 int cruna_hip_ensemble_set_role(cruna_hip_ensemble *ens, int i, int role, int nt);
 int cruna_hip_ensemble_run(cruna_hip_ensemble *ens, int nsteps);
-/* cruna_hip_autotune on every member. */
-int cruna_hip_ensemble_autotune(cruna_hip_ensemble *ens, int steps);
+/* cruna_hip_autotune on every member; tile_lines (may be NULL) receives 3
+ * values per member. */
+int cruna_hip_ensemble_autotune(cruna_hip_ensemble *ens, int steps, int *tile_lines);
 
 /* Per-kernel timing with HIP events on the context stream: enable, run,
  * then read the summed milliseconds and launch counts per kernel id. */

@@ -81,7 +81,7 @@ def lib():
         L.cruna_hip_ensemble_member.argtypes = [vp, ip, C.POINTER(vp)]
         L.cruna_hip_ensemble_set_role.argtypes = [vp, ip, ip, ip]
         L.cruna_hip_ensemble_run.argtypes = [vp, ip]
-        L.cruna_hip_ensemble_autotune.argtypes = [vp, ip]
+        L.cruna_hip_ensemble_autotune.argtypes = [vp, ip, C.POINTER(C.c_int)]
         L.cruna_hip_diff.argtypes = [vp, ip, dp, dp, C.c_double]
         L.cruna_hip_filter.argtypes = [vp, ip, dp, ip]
         L.cruna_hip_rhs_direct.argtypes = [vp, dp, dp, C.c_double]
@@ -373,8 +373,16 @@ class Ensemble:
         _check(lib().cruna_hip_ensemble_run(self._h, int(nsteps)))
 
     def autotune(self, steps=2):
-        _check(lib().cruna_hip_ensemble_autotune(self._h, int(steps)))
-        return [tuple(m.cfg.tile_lines) for m in self.members]
+        """Tile autotuner on every member; returns the lines per work-group per member."""
+        out = (C.c_int * (3 * len(self.members)))()
+        _check(lib().cruna_hip_ensemble_autotune(self._h, int(steps), out))
+        res = []
+        for i, m in enumerate(self.members):
+            t = tuple(out[3 * i + d] for d in range(3))
+            for d in range(3):
+                m.cfg.tile_lines[d] = t[d]
+            res.append(t)
+        return res
 
     def close(self):
         if getattr(self, "_h", None):

@@ -1130,8 +1130,11 @@ int cruna_hip_profile_count(void) { return KID_COUNT; }
 // ---------------------------------------------------------------------------
 // Ensembles: independent members (forward or adjoint runs of any size) on one
 // device.  Each member keeps its own stream; a run records one period of
-// steps of every member into a HIP graph whose member branches run
-// concurrently, and replays it.
+// steps of each member into a HIP graph and replays the members' graphs on
+// their own streams, so that members run concurrently on the device's
+// hardware queues (a single graph with one branch per member replays its
+// branches one after the other here, and small members alone leave most of
+// the device idle: 48^3 runs at ~290 Mpts/s, 128^3 at ~1350).
 
 }  // extern "C"
 
@@ -1140,9 +1143,7 @@ struct cruna_hip_ensemble {
   std::vector<int> role;  // CRUNA_HIP_FORWARD / CRUNA_HIP_ADJOINT
   std::vector<int> nt;    // forward: steps done; adjoint: next step to run backward
   std::vector<bool> warm;
-  hipStream_t main = nullptr;
-  std::vector<hipEvent_t> fork, join;
-  hipGraphExec_t exec = nullptr;
+  std::vector<hipGraphExec_t> exec;  // one period of steps of each member
   int period = 0;
   bool dirty = true;  // the recorded graph no longer matches the members
 };
@@ -1175,9 +1176,10 @@ int member_steps(cruna_hip_ensemble *e, size_t i, int k) {
   return CRUNA_HIP_OK;
 }
 
-void drop_graph(cruna_hip_ensemble *e) {
-  if (e->exec) (void)hipGraphExecDestroy(e->exec);
-  e->exec = nullptr;
+void drop_graphs(cruna_hip_ensemble *e) {
+  for (auto &x : e->exec)
+    if (x) (void)hipGraphExecDestroy(x);
+  e->exec.clear();
 }
 
 }  // namespace
@@ -1199,32 +1201,15 @@ int cruna_hip_ensemble_create(const cruna_hip_cfg *cfgs, int nmembers, cruna_hip
     e->nt.push_back(0);
     e->warm.push_back(false);
   }
-  auto cleanup = [&](int rc) {
-    cruna_hip_ensemble_destroy(e.release());
-    return rc;
-  };
-  if (hipStreamCreateWithFlags(&e->main, hipStreamNonBlocking) != hipSuccess)
-    return cleanup(fail(CRUNA_HIP_EDEVICE, "stream creation failed"));
-  e->fork.resize(nmembers);
-  e->join.resize(nmembers);
-  for (int i = 0; i < nmembers; ++i)
-    if (hipEventCreateWithFlags(&e->fork[i], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) != hipSuccess)
-      return cleanup(fail(CRUNA_HIP_EDEVICE, "event creation failed"));
   *out = e.release();
   return CRUNA_HIP_OK;
 }
 
 int cruna_hip_ensemble_destroy(cruna_hip_ensemble *e) {
   if (!e) return CRUNA_HIP_OK;
-  if (e->main) (void)hipStreamSynchronize(e->main);
-  drop_graph(e);
-  for (auto ev : e->fork)
-    if (ev) (void)hipEventDestroy(ev);
-  for (auto ev : e->join)
-    if (ev) (void)hipEventDestroy(ev);
+  for (auto *c : e->m) (void)hipStreamSynchronize(c->stream);
+  drop_graphs(e);
   for (auto *c : e->m) cruna_hip_destroy(c);
-  if (e->main) (void)hipStreamDestroy(e->main);
   delete e;
   return CRUNA_HIP_OK;
 }
@@ -1290,55 +1275,52 @@ int cruna_hip_ensemble_run(cruna_hip_ensemble *e, int nsteps) {
     for (auto *c : e->m) HIP_TRY(hipStreamSynchronize(c->stream));
     return CRUNA_HIP_OK;
   }
-  // record one period of every member as concurrent branches
-  if (e->dirty || !e->exec || e->period != P) {
-    drop_graph(e);
-    for (auto *c : e->m) HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipStreamBeginCapture(e->main, hipStreamCaptureModeThreadLocal));
+  // record one period of steps of each member (the recording runs nothing)
+  if (e->dirty || e->exec.size() != M || e->period != P) {
+    drop_graphs(e);
+    e->exec.assign(M, nullptr);
     std::vector<int> nt_save = e->nt;
-    int rc = CRUNA_HIP_OK;
-    for (size_t i = 0; i < M && !rc; ++i) {
+    for (size_t i = 0; i < M; ++i) {
       cruna_hip_ctx *c = e->m[i];
-      if (hipEventRecord(e->fork[i], e->main) != hipSuccess ||
-          hipStreamWaitEvent(c->stream, e->fork[i], 0) != hipSuccess) {
-        rc = fail(CRUNA_HIP_EDEVICE, "graph fork failed");
-        break;
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = member_steps(e, i, P);
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+      if (rc || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        e->nt = nt_save;
+        drop_graphs(e);
+        return rc ? rc : fail(CRUNA_HIP_EDEVICE, std::string("capture: ") + hipGetErrorString(ec));
       }
-      rc = member_steps(e, i, P);
-      if (!rc && (hipEventRecord(e->join[i], c->stream) != hipSuccess ||
-                  hipStreamWaitEvent(e->main, e->join[i], 0) != hipSuccess))
-        rc = fail(CRUNA_HIP_EDEVICE, "graph join failed");
+      const hipError_t ei = hipGraphInstantiate(&e->exec[i], graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        e->exec[i] = nullptr;
+        e->nt = nt_save;
+        drop_graphs(e);
+        return fail(CRUNA_HIP_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
+      }
     }
-    hipGraph_t graph = nullptr;
-    const hipError_t ec = hipStreamEndCapture(e->main, &graph);
-    e->nt = nt_save;  // recording ran no step
-    if (rc) {
-      if (graph) (void)hipGraphDestroy(graph);
-      return rc;
-    }
-    if (ec != hipSuccess) return fail(CRUNA_HIP_EDEVICE, std::string("capture: ") + hipGetErrorString(ec));
-    const hipError_t ei = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (ei != hipSuccess) {
-      e->exec = nullptr;
-      return fail(CRUNA_HIP_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
-    }
+    e->nt = nt_save;
     e->period = P;
     e->dirty = false;
   }
-  for (int r = 0; r < left / P; ++r) HIP_TRY(hipGraphLaunch(e->exec, e->main));
-  HIP_TRY(hipStreamSynchronize(e->main));
+  // replay: members interleaved so that their streams overlap
+  for (int r = 0; r < left / P; ++r)
+    for (size_t i = 0; i < M; ++i) HIP_TRY(hipGraphLaunch(e->exec[i], e->m[i]->stream));
+  for (auto *c : e->m) HIP_TRY(hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < M; ++i) e->nt[i] += e->role[i] == CRUNA_HIP_FORWARD ? left : -left;
   return CRUNA_HIP_OK;
 }
 
-int cruna_hip_ensemble_autotune(cruna_hip_ensemble *e, int steps) {
+int cruna_hip_ensemble_autotune(cruna_hip_ensemble *e, int steps, int *tile_lines) {
   if (!e) return fail(CRUNA_HIP_EINVAL, "null argument");
-  for (auto *c : e->m)
-    if (int rc = cruna_hip_autotune(c, steps, nullptr)) return rc;
+  for (size_t i = 0; i < e->m.size(); ++i)
+    if (int rc = cruna_hip_autotune(e->m[i], steps, tile_lines ? tile_lines + 3 * i : nullptr))
+      return rc;
   e->dirty = true;
   return CRUNA_HIP_OK;
 }
-
 
 }  // extern "C"
