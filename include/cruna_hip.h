@@ -105,6 +105,12 @@ int cruna_hip_filter(cruna_hip_ctx *ctx, int dir, double *q, int nv);
 /* right_hand_side_direct_3d(rhs,q_in,t) (src/equations/ns_rho_rhou_p.f90:31-253). */
 int cruna_hip_rhs_direct(cruna_hip_ctx *ctx, double *rhs, const double *q, double t);
 
+/* Box averages over f x f x f blocks of the device state q^n and (rhs_box
+ * not NULL) of its direct RHS, into host arrays (n1/f, n2/f, n3/f, 5): the
+ * filtered fine fields from which a closure's training data are made.  No
+ * counterpart in the reference (its solution output is whole fields). */
+int cruna_hip_box_filter_state(cruna_hip_ctx *ctx, int f, double *q_box, double *rhs_box);
+
 /* right_hand_side_adjoint_3d(rhs,qs_in,q,g,t) (ns_rho_rhou_p.f90:257-536); g may be NULL. */
 int cruna_hip_rhs_adjoint(cruna_hip_ctx *ctx, double *rhs, const double *qs, const double *q,
                           const double *g, double t);
@@ -181,6 +187,49 @@ int cruna_hip_ensemble_run(cruna_hip_ensemble *ens, int nsteps);
 /* cruna_hip_autotune on every member; tile_lines (may be NULL) receives 3
  * values per member. */
 int cruna_hip_ensemble_autotune(cruna_hip_ensemble *ens, int steps, int *tile_lines);
+
+/* Learned subfilter closure: a 3-D CNN (5 -> 32 -> 32 -> 32 -> 5 channels,
+ * 3x3x3 periodic kernels, GELU after the hidden layers) on a periodic S^3
+ * grid, trained and evaluated with BF16 matrix cores and FP32 accumulation.
+ * It is the additive force term of the direct RHS (apply_force,
+ * src/equations/ns_rho_rhou_p.f90:212 and src/opt/force.f90:28): with a
+ * closure set, every direct RHS evaluation of the context adds
+ * force = cnn(q) with the normalisation of cruna_hip_cnn_set_norm.
+ * Parameters (FP32) are laid out per layer as W[27][co][ci] then b[co] with
+ * the channel counts padded to (16,32), (32,32), (32,32), (32,16); taps
+ * t = 9 (dz+1) + 3 (dy+1) + (dx+1).  The closure's buffers are shared by
+ * every context it is set on: use those contexts from one host thread. */
+typedef struct cruna_hip_cnn cruna_hip_cnn;
+int cruna_hip_cnn_create(int S, cruna_hip_cnn **out);
+int cruna_hip_cnn_destroy(cruna_hip_cnn *cnn);
+long cruna_hip_cnn_num_params(const cruna_hip_cnn *cnn);
+int cruna_hip_cnn_init_params(cruna_hip_cnn *cnn, unsigned seed);
+int cruna_hip_cnn_set_params(cruna_hip_cnn *cnn, const float *params);
+int cruna_hip_cnn_get_params(cruna_hip_cnn *cnn, float *params);
+/* inputs: x_c = (q_c - mean of q_c over the grid) / in_std[c]; outputs:
+ * force_v = out_v out_std[v], the pressure equation's also times
+ * mean(p) / p_ref when p_ref > 0 (its residual scales with the mean
+ * pressure, i.e. 1 / Mach^2); defaults 1, 1 and 0 */
+int cruna_hip_cnn_set_norm(cruna_hip_cnn *cnn, const double in_std[5], const double out_std[5], double p_ref);
+/* closure (5 fields, FP64, field-major like q) of the state q */
+int cruna_hip_cnn_forward(cruna_hip_cnn *cnn, const double *q, double *closure);
+/* mean-square loss of the normalised output against target / out_std and
+ * its gradient with respect to every parameter (grads may be NULL) */
+int cruna_hip_cnn_gradients(cruna_hip_cnn *cnn, const double *q, const double *target, float *grads,
+                            float *loss);
+/* keep a (state, target) pair on the device for cruna_hip_cnn_train */
+int cruna_hip_cnn_add_sample(cruna_hip_cnn *cnn, const double *q, const double *target);
+int cruna_hip_cnn_num_samples(const cruna_hip_cnn *cnn);
+/* iters Adam steps (beta 0.9, 0.999, eps 1e-8) on batches drawn with the
+ * seed: `batch` windows of crop^3 points at random samples and offsets (the
+ * network runs on each window with a 4-point halo, so a window's values are
+ * those of the whole periodic sample), or, with crop 0, `batch` whole
+ * samples; losses (may be NULL) receives the loss of every step */
+int cruna_hip_cnn_train(cruna_hip_cnn *cnn, int iters, float lr, int batch, int crop, unsigned seed,
+                        float *losses);
+/* add the closure to the context's direct RHS (NULL removes it); the
+ * context's grid must be the periodic S^3 grid of the closure */
+int cruna_hip_set_closure(cruna_hip_ctx *ctx, cruna_hip_cnn *cnn);
 
 /* Per-kernel timing with HIP events on the context stream: enable, run,
  * then read the summed milliseconds and launch counts per kernel id. */

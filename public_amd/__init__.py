@@ -6,4 +6,4 @@ synthetic workloads.
 """
 from . import ic  # noqa: F401
 from .cruna import (ADJOINT, FORWARD, FP32_MIXED, FP64, OPEN, PERIODIC, POWER,  # noqa: F401
-                    SUTHERLAND, Cruna, CrunaError, Ensemble, lib)
+                    SUTHERLAND, Closure, Cruna, CrunaError, Ensemble, lib)

@@ -100,13 +100,31 @@ def lib():
         L.cruna_hip_profile_enable.argtypes = [vp, ip]
         L.cruna_hip_profile_read.argtypes = [vp, ip, dp, C.POINTER(C.c_long), C.c_char_p, C.c_size_t]
         L.cruna_hip_profile_count.argtypes = []
+        fp = C.POINTER(C.c_float)
+        L.cruna_hip_cnn_create.argtypes = [ip, C.POINTER(vp)]
+        L.cruna_hip_cnn_destroy.argtypes = [vp]
+        L.cruna_hip_cnn_num_params.argtypes = [vp]
+        L.cruna_hip_cnn_num_params.restype = C.c_long
+        L.cruna_hip_cnn_init_params.argtypes = [vp, C.c_uint]
+        L.cruna_hip_cnn_set_params.argtypes = [vp, fp]
+        L.cruna_hip_cnn_get_params.argtypes = [vp, fp]
+        L.cruna_hip_cnn_set_norm.argtypes = [vp, dp, dp, C.c_double]
+        L.cruna_hip_box_filter_state.argtypes = [vp, ip, dp, dp]
+        L.cruna_hip_cnn_forward.argtypes = [vp, dp, dp]
+        L.cruna_hip_cnn_gradients.argtypes = [vp, dp, dp, fp, fp]
+        L.cruna_hip_cnn_add_sample.argtypes = [vp, dp, dp]
+        L.cruna_hip_cnn_train.argtypes = [vp, ip, C.c_float, ip, ip, C.c_uint, fp]
+        L.cruna_hip_cnn_num_samples.argtypes = [vp]
+        L.cruna_hip_set_closure.argtypes = [vp, vp]
         for f in ("create", "destroy", "set_state", "get_state", "set_qref", "set_sponge_shape", "diff",
                   "filter", "rhs_direct", "rhs_adjoint", "time_stepper_direct", "time_stepper_adjoint",
                   "run_direct", "synchronize", "profile_enable", "profile_read", "profile_count",
                   "set_adjoint_state", "get_adjoint_state", "trajectory_alloc", "trajectory_set",
                   "trajectory_get", "trajectory_record", "run_adjoint", "set_metric", "autotune",
                   "ensemble_create", "ensemble_destroy", "ensemble_member", "ensemble_set_role",
-                  "ensemble_run", "ensemble_autotune"):
+                  "ensemble_run", "ensemble_autotune", "cnn_create", "cnn_destroy", "cnn_init_params",
+                  "cnn_set_params", "cnn_get_params", "cnn_set_norm", "cnn_forward", "cnn_gradients",
+                  "cnn_add_sample", "cnn_train", "set_closure", "box_filter_state"):
             getattr(L, "cruna_hip_" + f).restype = C.c_int
         _lib = L
     return _lib
@@ -323,6 +341,19 @@ class Cruna:
         nt_end = nsteps if nt_end is None else nt_end
         _check(lib().cruna_hip_run_adjoint(self._h, int(nt_end), int(nsteps)))
 
+    def box_filter_state(self, f, rhs=True):
+        """(q_box, rhs_box or None): f^3 box averages of the device state and of its direct RHS."""
+        shape = tuple(n // f for n in self.shape3) + (5,)
+        qb = np.zeros(shape, order="F")
+        rb = np.zeros(shape, order="F") if rhs else None
+        _check(lib().cruna_hip_box_filter_state(self._h, int(f), _dp(qb), None if rb is None else _dp(rb)))
+        return qb, rb
+
+    def set_closure(self, closure):
+        """Add a learned closure (a `Closure` of this grid) to the direct RHS; None removes it."""
+        _check(lib().cruna_hip_set_closure(self._h, None if closure is None else closure._h))
+        self._closure = closure
+
     def profile(self, on=True):
         _check(lib().cruna_hip_profile_enable(self._h, 1 if on else 0))
 
@@ -402,3 +433,84 @@ class Ensemble:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Closure:
+    """Learned subfilter closure: the 3-D CNN of cruna_hip_cnn on a periodic
+    S^3 grid (trained and evaluated on the device).  Fields are (S,S,S,5)
+    float64 arrays in Fortran order like the state."""
+
+    def __init__(self, S, seed=None):
+        h = C.c_void_p()
+        _check(lib().cruna_hip_cnn_create(int(S), C.byref(h)))
+        self._h = h
+        self.S = int(S)
+        self.shape = (self.S,) * 3 + (5,)
+        if seed is not None:
+            _check(lib().cruna_hip_cnn_init_params(h, int(seed)))
+
+    def num_params(self):
+        return int(lib().cruna_hip_cnn_num_params(self._h))
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != self.num_params():
+            raise CrunaError(f"expected {self.num_params()} parameters, got {p.size}")
+        _check(lib().cruna_hip_cnn_set_params(self._h, _fp(p)))
+
+    def get_params(self):
+        p = np.zeros(self.num_params(), dtype=np.float32)
+        _check(lib().cruna_hip_cnn_get_params(self._h, _fp(p)))
+        return p
+
+    def set_norm(self, in_std, out_std, p_ref=0.0):
+        """Input scales (about the sample's field means), output scales, and
+        the reference mean pressure of the pressure output (0: unscaled)."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (in_std, out_std)]
+        if any(x.shape != (5,) for x in a):
+            raise CrunaError("normalisation needs 5 values per argument")
+        _check(lib().cruna_hip_cnn_set_norm(self._h, _dp(a[0]), _dp(a[1]), float(p_ref)))
+
+    def forward(self, q):
+        q = _f64(q, self.shape)
+        out = np.zeros(self.shape, order="F")
+        _check(lib().cruna_hip_cnn_forward(self._h, _dp(q), _dp(out)))
+        return out
+
+    def gradients(self, q, target):
+        """(loss, gradient of every parameter) for one (state, target) pair."""
+        q, t = _f64(q, self.shape), _f64(target, self.shape)
+        g = np.zeros(self.num_params(), dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        _check(lib().cruna_hip_cnn_gradients(self._h, _dp(q), _dp(t), _fp(g), _fp(loss)))
+        return float(loss[0]), g
+
+    def add_sample(self, q, target):
+        q, t = _f64(q, self.shape), _f64(target, self.shape)
+        _check(lib().cruna_hip_cnn_add_sample(self._h, _dp(q), _dp(t)))
+
+    def num_samples(self):
+        return int(lib().cruna_hip_cnn_num_samples(self._h))
+
+    def train(self, iters, lr=1e-3, batch=8, crop=32, seed=0):
+        """Adam steps on batches of `batch` crop^3 windows of the stored
+        samples (crop 0: whole samples); returns the loss of every step."""
+        losses = np.zeros(int(iters), dtype=np.float32)
+        _check(lib().cruna_hip_cnn_train(self._h, int(iters), float(lr), int(batch), int(crop), int(seed),
+                                         _fp(losses)))
+        return losses
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cruna_hip_cnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

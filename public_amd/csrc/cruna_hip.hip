@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <random>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -15,6 +16,7 @@
 #include "boundary.hpp"
 #include "sweeps.hpp"
 #include "adjoint.hpp"
+#include "closure.hpp"
 
 using namespace cruna;
 
@@ -78,6 +80,7 @@ struct DevOp {
 
 }  // namespace
 
+struct cruna_hip_cnn;
 struct cruna_hip_ctx {
   cruna_hip_cfg cfg{};
   int device = 0;
@@ -109,6 +112,9 @@ struct cruna_hip_ctx {
   int traj_slots = 0;
   bool record = false;
   void *qs = nullptr, *rhs = nullptr, *G = nullptr;  // compute precision
+  // learned closure added to the direct RHS (cruna_hip_set_closure)
+  cruna_hip_cnn *cnn = nullptr;
+  void *force = nullptr;  // its term, 5 fields in the compute precision
   // qs holds qn in the compute precision (FP32-mixed: written by the step's
   // last filter pass, read by the next step's first stage instead of qn)
   bool qs_fresh = false;
@@ -309,6 +315,9 @@ int bc_rhs(cruna_hip_ctx *c, T *rhs, const Q *q) {
   return CRUNA_HIP_OK;
 }
 
+template <typename T, typename Q>
+int closure_term(cruna_hip_ctx *c, const Q *in);
+
 template <typename T, typename Q, int DIR, bool GT>
 void launch_sweep1_gt(cruna_hip_ctx *c, const Q *in) {
   const dim3 grid(c->geom[DIR].ntiles), block(c->wg[DIR]);
@@ -342,11 +351,11 @@ void launch_sweep2_gt(cruna_hip_ctx *c, int mode, bool fuse_filter) {
   if (c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC)
     hipLaunchKernelGGL((k_sweep2<T, DIR, true, GT>), grid, block, 0, c->stream, rhs, G, op, fop,
                        c->geom[DIR], c->phys, c->N, mode, ff, c->qn, c->acc, qs, c->cfg.dt,
-                       metric<T>(c));
+                       metric<T>(c), (const T *)(DIR == 0 && c->cnn ? c->force : nullptr));
   else
     hipLaunchKernelGGL((k_sweep2<T, DIR, false, GT>), grid, block, 0, c->stream, rhs, G, op, fop,
                        c->geom[DIR], c->phys, c->N, mode, ff, c->qn, c->acc, qs, c->cfg.dt,
-                       metric<T>(c));
+                       metric<T>(c), (const T *)(DIR == 0 && c->cnn ? c->force : nullptr));
 }
 
 template <typename T, int DIR>
@@ -371,6 +380,8 @@ int rhs_stage(cruna_hip_ctx *c, const Q *in, int mode, bool fuse_filter = false)
   if (int rc = launch_sweep2<T, 1>(c, mode, false)) return rc;
   if (c->has_bcrhs)
     if (int rc = bc_rhs<T, Q>(c, (T *)c->rhs, in)) return rc;
+  if (c->cnn)
+    if (int rc = closure_term<T, Q>(c, in)) return rc;
   return launch_sweep2<T, 0>(c, mode, fuse_filter);
 }
 
@@ -755,7 +766,8 @@ int cruna_hip_destroy(cruna_hip_ctx *c) {
     c->fop32[d].release();
   }
   for (void *p : {(void *)c->qn, (void *)c->acc, (void *)c->qref, (void *)c->shape, (void *)c->tmp,
-                  (void *)c->qa, (void *)c->traj, (void *)c->gobj, c->met, (void *)c->q0p, c->qs, c->rhs, c->G})
+                  (void *)c->qa, (void *)c->traj, (void *)c->gobj, c->met, (void *)c->q0p, c->force,
+                  c->qs, c->rhs, c->G})
     if (p) (void)hipFree(p);
   for (auto &p : c->pending) {
     (void)hipEventDestroy(p.second.first);
@@ -881,6 +893,35 @@ int cruna_hip_rhs_direct(cruna_hip_ctx *c, double *rhs, const double *q, double 
   int rc = mixed ? run_stage<float>(c, c->tmp, 4) : run_stage<double>(c, c->tmp, 4);
   if (rc) return rc;
   return copy_rhs_out(c, rhs);
+}
+
+int cruna_hip_box_filter_state(cruna_hip_ctx *c, int f, double *q_box, double *rhs_box) {
+  if (!c || !q_box) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (f < 1) return fail(CRUNA_HIP_EINVAL, "box factor must be >= 1");
+  for (int d = 0; d < 3; ++d)
+    if (c->cfg.n[d] % f) return fail(CRUNA_HIP_EINVAL, "box factor must divide every grid size");
+  if (int rc = ensure_tmp(c)) return rc;
+  const long M = c->N / ((long)f * f * f);
+  const unsigned nb = (unsigned)((5 * M + 255) / 256);
+  const int *n = c->cfg.n;
+  hipLaunchKernelGGL((k_box_average<double>), dim3(nb), dim3(256), 0, c->stream, (const double *)c->qn, n[0],
+                     n[1], n[2], f, c->tmp);
+  if (rhs_box) {
+    const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
+    if (int rc = mixed ? run_stage<float>(c, c->qn, 4) : run_stage<double>(c, c->qn, 4)) return rc;
+    if (mixed)
+      hipLaunchKernelGGL((k_box_average<float>), dim3(nb), dim3(256), 0, c->stream, (const float *)c->rhs, n[0],
+                         n[1], n[2], f, c->tmp + 5 * M);
+    else
+      hipLaunchKernelGGL((k_box_average<double>), dim3(nb), dim3(256), 0, c->stream, (const double *)c->rhs,
+                         n[0], n[1], n[2], f, c->tmp + 5 * M);
+  }
+  if (int rc = check_launch()) return rc;
+  HIP_TRY(hipMemcpyAsync(q_box, c->tmp, 5 * M * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (rhs_box)
+    HIP_TRY(hipMemcpyAsync(rhs_box, c->tmp + 5 * M, 5 * M * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CRUNA_HIP_OK;
 }
 
 // Host inputs of the adjoint entry points: qs -> qa, q -> second half of
@@ -1320,6 +1361,464 @@ int cruna_hip_ensemble_autotune(cruna_hip_ensemble *e, int steps, int *tile_line
     if (int rc = cruna_hip_autotune(e->m[i], steps, tile_lines ? tile_lines + 3 * i : nullptr))
       return rc;
   e->dirty = true;
+  return CRUNA_HIP_OK;
+}
+
+}  // extern "C"
+
+
+// ---------------------------------------------------------------------------
+// Learned subfilter closure (closure.hpp): a 3-D CNN on a periodic S^3 grid.
+
+struct cruna_hip_cnn {
+  int S = 0;
+  long N = 0;  // S^3
+  hipStream_t stream = nullptr;
+  static constexpr int L = 4;
+  static constexpr int HALO = 4;  // reach of the network: one point per layer
+  const int ci[L] = {16, 32, 32, 32}, co[L] = {32, 32, 32, 16};
+  long woff[L] = {}, boff[L] = {}, wpk[L] = {};
+  long nparams = 0;
+  float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
+  bf16 *wf = nullptr, *wt = nullptr;  // packed BF16 operands of all layers
+  CnnNorm hn = {};              // normalisation (host copy)
+  CnnNorm *dn = nullptr;        // its device copy
+  double *fmean = nullptr;      // field means of the sample in act[0]
+  double *stage = nullptr;      // host inputs of the staging calls (10 S^3)
+  long adam_t = 0;
+  // per-point work buffers, sized for `cap` points
+  long cap = 0;
+  bf16 *act[L] = {}, *z[L - 1] = {}, *dz_a = nullptr, *dz_b = nullptr;
+  float *out32 = nullptr, *tgt = nullptr, *lpart = nullptr, *part = nullptr;
+  long part_cap = 0;
+  CropSpec *spec = nullptr;
+  long spec_cap = 0;
+  std::vector<bf16 *> ds_feat;
+  std::vector<float *> ds_tgt;
+  std::vector<void *> fixed;  // parameter buffers
+  std::vector<void *> work;   // per-point buffers
+};
+
+namespace {
+
+constexpr int WGRAD_CHUNK = 4096;
+
+ConvGeom conv_geom(int X, int Y, int Z, bool periodic) {
+  ConvGeom g;
+  g.X = X;
+  g.Y = Y;
+  g.Z = Z;
+  g.periodic = periodic ? 1 : 0;
+  g.inv_x = 1.0 / X;
+  g.inv_xy = 1.0 / ((double)X * Y);
+  g.inv_xyz = 1.0 / ((double)X * Y * Z);
+  return g;
+}
+
+template <typename X>
+int cnn_alloc(std::vector<void *> &list, X **p, size_t count) {
+  void *q = nullptr;
+  if (hipMalloc(&q, count * sizeof(X)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(CRUNA_HIP_ENOMEM, "device allocation failed");
+  }
+  list.push_back(q);
+  *p = (X *)q;
+  return CRUNA_HIP_OK;
+}
+
+// work buffers for npts points (kept; grown when a larger batch comes)
+int cnn_reserve(cruna_hip_cnn *n, long npts) {
+  if (npts <= n->cap) return CRUNA_HIP_OK;
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  for (void *q : n->work) (void)hipFree(q);
+  n->work.clear();
+  n->cap = 0;
+  n->part_cap = 0;
+  int rc = CRUNA_HIP_OK;
+  for (int l = 0; l < n->L && !rc; ++l) rc = cnn_alloc(n->work, &n->act[l], npts * n->ci[l]);
+  for (int l = 0; l < n->L - 1 && !rc; ++l) rc = cnn_alloc(n->work, &n->z[l], npts * n->co[l]);
+  if (!rc) rc = cnn_alloc(n->work, &n->dz_a, npts * 32);
+  if (!rc) rc = cnn_alloc(n->work, &n->dz_b, npts * 32);
+  if (!rc) rc = cnn_alloc(n->work, &n->out32, npts * 16);
+  if (!rc) rc = cnn_alloc(n->work, &n->tgt, npts * 5);
+  if (!rc) rc = cnn_alloc(n->work, &n->lpart, (npts + 255) / 256);
+  const long nch = (npts + WGRAD_CHUNK - 1) / WGRAD_CHUNK;
+  if (!rc) rc = cnn_alloc(n->work, &n->part, (size_t)nch * 27 * 32 * 32);
+  if (rc) {
+    for (void *q : n->work) (void)hipFree(q);
+    n->work.clear();
+    return rc;
+  }
+  n->cap = npts;
+  n->part_cap = nch;
+  return CRUNA_HIP_OK;
+}
+
+int cnn_pack(cruna_hip_cnn *n, hipStream_t s) {
+  for (int l = 0; l < n->L; ++l) {
+    const long e = 27L * n->co[l] * n->ci[l];
+    hipLaunchKernelGGL(k_pack_weights, dim3((e + 255) / 256), dim3(256), 0, s, n->params + n->woff[l],
+                       n->co[l], n->ci[l], n->wf + n->wpk[l], n->wt + n->wpk[l]);
+  }
+  return check_launch();
+}
+
+// forward pass from act[0] to out32 over npts points of geometry geo; with
+// `train` the pre-activations are kept for the backward pass
+int cnn_forward(cruna_hip_cnn *n, hipStream_t s, const ConvGeom &geo, long npts, bool train) {
+  if (npts % 64 != 0 || npts > n->cap) return fail(CRUNA_HIP_EINVAL, "closure batch size");
+  const dim3 grid((unsigned)(npts / 64)), block(256);
+  const float *P = n->params;
+  hipLaunchKernelGGL((k_conv<16, 32, CONV_GELU>), grid, block, 0, s, n->act[0], n->wf + n->wpk[0],
+                     P + n->boff[0], geo, npts, (float *)nullptr, n->act[1], train ? n->z[0] : nullptr,
+                     (const bf16 *)nullptr);
+  hipLaunchKernelGGL((k_conv<32, 32, CONV_GELU>), grid, block, 0, s, n->act[1], n->wf + n->wpk[1],
+                     P + n->boff[1], geo, npts, (float *)nullptr, n->act[2], train ? n->z[1] : nullptr,
+                     (const bf16 *)nullptr);
+  hipLaunchKernelGGL((k_conv<32, 32, CONV_GELU>), grid, block, 0, s, n->act[2], n->wf + n->wpk[2],
+                     P + n->boff[2], geo, npts, (float *)nullptr, n->act[3], train ? n->z[2] : nullptr,
+                     (const bf16 *)nullptr);
+  hipLaunchKernelGGL((k_conv<32, 16, CONV_LINEAR>), grid, block, 0, s, n->act[3], n->wf + n->wpk[3],
+                     P + n->boff[3], geo, npts, n->out32, (bf16 *)nullptr, (bf16 *)nullptr,
+                     (const bf16 *)nullptr);
+  return check_launch();
+}
+
+template <int CI, int CO>
+void cnn_wgrad(cruna_hip_cnn *n, hipStream_t s, int l, const ConvGeom &geo, long npts, const bf16 *in,
+               const bf16 *dz) {
+  const int nch = (int)((npts + WGRAD_CHUNK - 1) / WGRAD_CHUNK);
+  hipLaunchKernelGGL((k_conv_wgrad<CI, CO>), dim3(nch, 27), dim3(256), 0, s, in, dz, geo, npts, WGRAD_CHUNK,
+                     n->part);
+  const long e = 27L * CO * CI;
+  hipLaunchKernelGGL(k_sum_chunks, dim3((e + 255) / 256), dim3(256), 0, s, (const float *)n->part, nch, e,
+                     n->grads + n->woff[l]);
+  hipLaunchKernelGGL(k_bias_grad, dim3(CO), dim3(256), 0, s, dz, npts, CO, n->grads + n->boff[l]);
+}
+
+// loss over the points at least `halo` from their box's faces (count of
+// them: counted) and the gradients of all parameters, after cnn_forward
+// with `train`; *loss (if set) receives the mean-square loss
+int cnn_backward(cruna_hip_cnn *n, hipStream_t s, const ConvGeom &geo, long npts, int halo, long counted,
+                 float *loss) {
+  const dim3 grid((unsigned)(npts / 64)), block(256);
+  const int nb = (int)((npts + 255) / 256);
+  const float scale = (float)(2.0 / ((double)counted * 5));
+  hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, s, (const float *)n->out32, (const float *)n->tgt, npts,
+                     5, geo, halo, scale, n->dz_a, n->lpart);
+  // layer 3 (32 -> 16, linear)
+  cnn_wgrad<32, 16>(n, s, 3, geo, npts, n->act[3], n->dz_a);
+  hipLaunchKernelGGL((k_conv<16, 32, CONV_BACKWARD>), grid, block, 0, s, n->dz_a, n->wt + n->wpk[3],
+                     (const float *)nullptr, geo, npts, (float *)nullptr, n->dz_b, (bf16 *)nullptr, n->z[2]);
+  // layer 2
+  cnn_wgrad<32, 32>(n, s, 2, geo, npts, n->act[2], n->dz_b);
+  hipLaunchKernelGGL((k_conv<32, 32, CONV_BACKWARD>), grid, block, 0, s, n->dz_b, n->wt + n->wpk[2],
+                     (const float *)nullptr, geo, npts, (float *)nullptr, n->dz_a, (bf16 *)nullptr, n->z[1]);
+  // layer 1
+  cnn_wgrad<32, 32>(n, s, 1, geo, npts, n->act[1], n->dz_a);
+  hipLaunchKernelGGL((k_conv<32, 32, CONV_BACKWARD>), grid, block, 0, s, n->dz_a, n->wt + n->wpk[1],
+                     (const float *)nullptr, geo, npts, (float *)nullptr, n->dz_b, (bf16 *)nullptr, n->z[0]);
+  // layer 0 (16 -> 32)
+  cnn_wgrad<16, 32>(n, s, 0, geo, npts, n->act[0], n->dz_b);
+  if (int rc = check_launch()) return rc;
+  if (loss) {
+    std::vector<float> h(nb);
+    HIP_TRY(hipMemcpyAsync(h.data(), n->lpart, nb * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    double sum = 0.0;
+    for (float x : h) sum += x;
+    *loss = (float)(sum / ((double)counted * 5));
+  }
+  return CRUNA_HIP_OK;
+}
+
+// closure evaluation on stream s: features of the state q (device, 5
+// fields of S^3), forward, force (5 fields of T) when `force` is set
+template <typename T, typename Q>
+int closure_eval(cruna_hip_cnn *n, hipStream_t s, const Q *q, T *force) {
+  const unsigned nb = (unsigned)((n->N + 255) / 256);
+  hipLaunchKernelGGL((k_field_means<Q>), dim3(1), dim3(1024), 0, s, q, n->N, n->fmean);
+  hipLaunchKernelGGL((k_pack_features<Q>), dim3(nb), dim3(256), 0, s, q, n->N, (const double *)n->fmean,
+                     (const CnnNorm *)n->dn, n->act[0]);
+  if (int rc = check_launch()) return rc;
+  if (int rc = cnn_forward(n, s, conv_geom(n->S, n->S, n->S, true), n->N, false)) return rc;
+  if (force)
+    hipLaunchKernelGGL((k_unpack_closure<T>), dim3(nb), dim3(256), 0, s, (const float *)n->out32, n->N,
+                       (const CnnNorm *)n->dn, (const double *)n->fmean, force);
+  return check_launch();
+}
+
+template <typename T, typename Q>
+int closure_term(cruna_hip_ctx *c, const Q *in) {
+  return closure_eval<T, Q>(c->cnn, c->stream, in, (T *)c->force);
+}
+
+// what cnn_stage does with a state from the host
+enum StageWhat { STAGE_FEATURES, STAGE_TRAIN, STAGE_FORCE };
+
+// stage a state (5 fields, FP64, host) as features in act[0] and the
+// normalised target, if given, in tgt; STAGE_TRAIN then runs the forward
+// pass keeping what the backward pass needs, STAGE_FORCE the closure term
+// (into the second half of `stage`)
+int cnn_stage(cruna_hip_cnn *n, const double *q, const double *target, StageWhat what) {
+  if (int rc = cnn_reserve(n, n->N)) return rc;
+  if (!n->stage) HIP_TRY(hipMalloc(&n->stage, 10 * n->N * sizeof(double)));
+  const unsigned nb = (unsigned)((n->N + 255) / 256);
+  double *qd = n->stage, *td = n->stage + 5 * n->N;
+  HIP_TRY(hipMemcpyAsync(qd, q, 5 * n->N * sizeof(double), hipMemcpyHostToDevice, n->stream));
+  if (target) HIP_TRY(hipMemcpyAsync(td, target, 5 * n->N * sizeof(double), hipMemcpyHostToDevice, n->stream));
+  if (what == STAGE_FORCE) {
+    if (int rc = closure_eval<double, double>(n, n->stream, qd, td)) return rc;
+  } else {
+    hipLaunchKernelGGL((k_field_means<double>), dim3(1), dim3(1024), 0, n->stream, (const double *)qd, n->N,
+                       n->fmean);
+    hipLaunchKernelGGL((k_pack_features<double>), dim3(nb), dim3(256), 0, n->stream, (const double *)qd, n->N,
+                       (const double *)n->fmean, (const CnnNorm *)n->dn, n->act[0]);
+    if (int rc = check_launch()) return rc;
+    if (what == STAGE_TRAIN)
+      if (int rc = cnn_forward(n, n->stream, conv_geom(n->S, n->S, n->S, true), n->N, true)) return rc;
+  }
+  if (target)
+    hipLaunchKernelGGL(k_scale_targets, dim3(nb), dim3(256), 0, n->stream, (const double *)td, n->N,
+                       (const CnnNorm *)n->dn, (const double *)n->fmean, n->tgt);
+  return check_launch();
+}
+
+}  // namespace
+
+extern "C" {
+
+int cruna_hip_cnn_create(int S, cruna_hip_cnn **out) {
+  if (!out) return fail(CRUNA_HIP_EINVAL, "null argument");
+  *out = nullptr;
+  if (S < 16 || S > 512 || (S & (S - 1)) != 0) return fail(CRUNA_HIP_EINVAL, "S must be a power of two in [16, 512]");
+  std::unique_ptr<cruna_hip_cnn> n(new cruna_hip_cnn());
+  n->S = S;
+  n->N = (long)S * S * S;
+  long o = 0, pk = 0;
+  for (int l = 0; l < n->L; ++l) {
+    n->woff[l] = o;
+    o += 27L * n->co[l] * n->ci[l];
+    n->boff[l] = o;
+    o += n->co[l];
+    n->wpk[l] = pk;
+    pk += 27L * n->co[l] * n->ci[l];
+  }
+  n->nparams = o;
+  if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(CRUNA_HIP_EDEVICE, "stream creation failed");
+  int rc = cnn_alloc(n->fixed, &n->params, o);
+  if (!rc) rc = cnn_alloc(n->fixed, &n->grads, o);
+  if (!rc) rc = cnn_alloc(n->fixed, &n->m, o);
+  if (!rc) rc = cnn_alloc(n->fixed, &n->v, o);
+  if (!rc) rc = cnn_alloc(n->fixed, &n->wf, pk);
+  if (!rc) rc = cnn_alloc(n->fixed, &n->wt, pk);
+  if (!rc) rc = cnn_alloc(n->fixed, &n->dn, 1);
+  if (!rc) rc = cnn_alloc(n->fixed, &n->fmean, 8);
+  if (!rc) rc = cnn_reserve(n.get(), n->N);
+  for (int k = 0; k < 5; ++k) n->hn.inv_std[k] = n->hn.out_std[k] = n->hn.out_inv_std[k] = 1.f;
+  n->hn.p_ref = 0.0;
+  if (!rc && (hipMemcpy(n->dn, &n->hn, sizeof n->hn, hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemset(n->params, 0, o * sizeof(float)) != hipSuccess ||
+              hipMemset(n->m, 0, o * sizeof(float)) != hipSuccess ||
+              hipMemset(n->v, 0, o * sizeof(float)) != hipSuccess))
+    rc = fail(CRUNA_HIP_EDEVICE, "initialisation failed");
+  if (!rc) rc = cnn_pack(n.get(), n->stream);
+  if (rc) {
+    cruna_hip_cnn *raw = n.release();
+    for (void *q : raw->fixed) (void)hipFree(q);
+    for (void *q : raw->work) (void)hipFree(q);
+    (void)hipStreamDestroy(raw->stream);
+    delete raw;
+    return rc;
+  }
+  *out = n.release();
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_cnn_destroy(cruna_hip_cnn *n) {
+  if (!n) return CRUNA_HIP_OK;
+  (void)hipStreamSynchronize(n->stream);
+  for (void *q : n->fixed) (void)hipFree(q);
+  for (void *q : n->work) (void)hipFree(q);
+  if (n->spec) (void)hipFree(n->spec);
+  if (n->stage) (void)hipFree(n->stage);
+  for (auto *q : n->ds_feat) (void)hipFree(q);
+  for (auto *q : n->ds_tgt) (void)hipFree(q);
+  (void)hipStreamDestroy(n->stream);
+  delete n;
+  return CRUNA_HIP_OK;
+}
+
+long cruna_hip_cnn_num_params(const cruna_hip_cnn *n) { return n ? n->nparams : 0; }
+
+int cruna_hip_cnn_set_params(cruna_hip_cnn *n, const float *params) {
+  if (!n || !params) return fail(CRUNA_HIP_EINVAL, "null argument");
+  HIP_TRY(hipMemcpyAsync(n->params, params, n->nparams * sizeof(float), hipMemcpyHostToDevice, n->stream));
+  HIP_TRY(hipMemsetAsync(n->m, 0, n->nparams * sizeof(float), n->stream));
+  HIP_TRY(hipMemsetAsync(n->v, 0, n->nparams * sizeof(float), n->stream));
+  n->adam_t = 0;
+  if (int rc = cnn_pack(n, n->stream)) return rc;
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_cnn_get_params(cruna_hip_cnn *n, float *params) {
+  if (!n || !params) return fail(CRUNA_HIP_EINVAL, "null argument");
+  HIP_TRY(hipMemcpyAsync(params, n->params, n->nparams * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_cnn_init_params(cruna_hip_cnn *n, unsigned seed) {
+  if (!n) return fail(CRUNA_HIP_EINVAL, "null argument");
+  const int real_in[4] = {5, 32, 32, 32}, real_out[4] = {32, 32, 32, 5};
+  std::vector<float> h(n->nparams, 0.f);
+  std::mt19937 gen(seed);
+  for (int l = 0; l < n->L; ++l) {
+    // He initialisation; the last layer starts small so that an untrained
+    // closure is a small perturbation of the RHS
+    std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (27.f * real_in[l])) * (l == 3 ? 0.1f : 1.f));
+    for (int t = 0; t < 27; ++t)
+      for (int o = 0; o < real_out[l]; ++o)
+        for (int i = 0; i < real_in[l]; ++i) h[n->woff[l] + ((long)t * n->co[l] + o) * n->ci[l] + i] = nd(gen);
+  }
+  return cruna_hip_cnn_set_params(n, h.data());
+}
+
+int cruna_hip_cnn_set_norm(cruna_hip_cnn *n, const double in_std[5], const double out_std[5], double p_ref) {
+  if (!n || !in_std || !out_std) return fail(CRUNA_HIP_EINVAL, "null argument");
+  for (int k = 0; k < 5; ++k)
+    if (!(in_std[k] > 0) || !(out_std[k] > 0)) return fail(CRUNA_HIP_EINVAL, "scales must be positive");
+  if (!(p_ref >= 0)) return fail(CRUNA_HIP_EINVAL, "p_ref must be >= 0");
+  for (int k = 0; k < 5; ++k) {
+    n->hn.inv_std[k] = (float)(1.0 / in_std[k]);
+    n->hn.out_std[k] = (float)out_std[k];
+    n->hn.out_inv_std[k] = (float)(1.0 / out_std[k]);
+  }
+  n->hn.p_ref = p_ref;
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  HIP_TRY(hipMemcpy(n->dn, &n->hn, sizeof n->hn, hipMemcpyHostToDevice));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_cnn_forward(cruna_hip_cnn *n, const double *q, double *closure) {
+  if (!n || !q || !closure) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (int rc = cnn_stage(n, q, nullptr, STAGE_FORCE)) return rc;
+  HIP_TRY(hipMemcpyAsync(closure, n->stage + 5 * n->N, 5 * n->N * sizeof(double), hipMemcpyDeviceToHost,
+                         n->stream));
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_cnn_gradients(cruna_hip_cnn *n, const double *q, const double *target, float *grads,
+                            float *loss) {
+  if (!n || !q || !target) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (int rc = cnn_stage(n, q, target, STAGE_TRAIN)) return rc;
+  const ConvGeom geo = conv_geom(n->S, n->S, n->S, true);
+  if (int rc = cnn_backward(n, n->stream, geo, n->N, 0, n->N, loss)) return rc;
+  if (grads)
+    HIP_TRY(hipMemcpyAsync(grads, n->grads, n->nparams * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_cnn_add_sample(cruna_hip_cnn *n, const double *q, const double *target) {
+  if (!n || !q || !target) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (int rc = cnn_stage(n, q, target, STAGE_FEATURES)) return rc;
+  bf16 *f = nullptr;
+  float *t = nullptr;
+  HIP_TRY(hipMalloc(&f, n->N * 16 * sizeof(bf16)));
+  if (hipMalloc(&t, n->N * 5 * sizeof(float)) != hipSuccess) {
+    (void)hipFree(f);
+    return fail(CRUNA_HIP_ENOMEM, "device allocation failed");
+  }
+  n->ds_feat.push_back(f);
+  n->ds_tgt.push_back(t);
+  HIP_TRY(hipMemcpyAsync(f, n->act[0], n->N * 16 * sizeof(bf16), hipMemcpyDeviceToDevice, n->stream));
+  HIP_TRY(hipMemcpyAsync(t, n->tgt, n->N * 5 * sizeof(float), hipMemcpyDeviceToDevice, n->stream));
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_cnn_num_samples(const cruna_hip_cnn *n) { return n ? (int)n->ds_feat.size() : 0; }
+
+int cruna_hip_cnn_train(cruna_hip_cnn *n, int iters, float lr, int batch, int crop, unsigned seed,
+                        float *losses) {
+  if (!n) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (n->ds_feat.empty()) return fail(CRUNA_HIP_EINVAL, "no training samples (cruna_hip_cnn_add_sample)");
+  if (iters < 0 || batch < 1 || batch > 64) return fail(CRUNA_HIP_EINVAL, "iters >= 0 and batch in [1, 64]");
+  if (crop < 0 || crop > n->S || (crop && crop % 4)) return fail(CRUNA_HIP_EINVAL, "crop: 0 or a multiple of 4 up to S");
+  // crop 0: whole periodic samples; else windows crop^3 with a halo of the
+  // network's reach, the loss taken on the crop^3 centre
+  const int H = crop ? n->HALO : 0;
+  const int X = crop ? crop + 2 * H : n->S;
+  const long per = (long)X * X * X;
+  const long npts = per * batch;
+  if (npts % 64 != 0) return fail(CRUNA_HIP_EINVAL, "batch points must be a multiple of 64");
+  const long counted = crop ? (long)crop * crop * crop * batch : npts;
+  const ConvGeom geo = conv_geom(X, X, X, crop == 0);
+  if (int rc = cnn_reserve(n, npts)) return rc;
+  // every step's batch drawn up front: sample and window offset per entry
+  std::mt19937 gen(seed);
+  std::uniform_int_distribution<int> pick(0, (int)n->ds_feat.size() - 1), off(0, n->S - 1);
+  std::vector<CropSpec> h((size_t)iters * batch);
+  for (auto &sp : h) {
+    const int s = pick(gen);
+    sp.feat = n->ds_feat[s];
+    sp.tgt = n->ds_tgt[s];
+    sp.ox = crop ? (off(gen) - H) & (n->S - 1) : 0;
+    sp.oy = crop ? (off(gen) - H) & (n->S - 1) : 0;
+    sp.oz = crop ? (off(gen) - H) & (n->S - 1) : 0;
+    sp.pad = 0;
+  }
+  if ((long)h.size() > n->spec_cap) {
+    if (n->spec) (void)hipFree(n->spec);
+    n->spec = nullptr;
+    n->spec_cap = 0;
+    HIP_TRY(hipMalloc(&n->spec, std::max<size_t>(h.size(), 1) * sizeof(CropSpec)));
+    n->spec_cap = (long)h.size();
+  }
+  if (!h.empty())
+    HIP_TRY(hipMemcpyAsync(n->spec, h.data(), h.size() * sizeof(CropSpec), hipMemcpyHostToDevice, n->stream));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const unsigned nbp = (unsigned)((n->nparams + 255) / 256);
+  const unsigned nbg = (unsigned)((npts + 255) / 256);
+  for (int it = 0; it < iters; ++it) {
+    hipLaunchKernelGGL(k_gather_crops, dim3(nbg), dim3(256), 0, n->stream,
+                       (const CropSpec *)(n->spec + (long)it * batch), n->S, geo, npts, n->act[0], n->tgt);
+    if (int rc = cnn_forward(n, n->stream, geo, npts, true)) return rc;
+    float loss = 0.f;
+    if (int rc = cnn_backward(n, n->stream, geo, npts, H, counted, losses ? &loss : nullptr)) return rc;
+    if (losses) losses[it] = loss;
+    ++n->adam_t;
+    const double bc1 = 1.0 - std::pow((double)b1, (double)n->adam_t);
+    const double bc2 = 1.0 - std::pow((double)b2, (double)n->adam_t);
+    const float lr_t = (float)(lr * std::sqrt(bc2) / bc1);
+    const float eps_t = (float)(eps * std::sqrt(bc2));
+    hipLaunchKernelGGL(k_adam, dim3(nbp), dim3(256), 0, n->stream, n->params, (const float *)n->grads, n->m, n->v,
+                       n->nparams, lr_t, b1, b2, eps_t);
+    if (int rc = cnn_pack(n, n->stream)) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_set_closure(cruna_hip_ctx *c, cruna_hip_cnn *n) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (n) {
+    for (int d = 0; d < 3; ++d)
+      if (c->cfg.n[d] != n->S || c->cfg.scheme[d] != CRUNA_HIP_PERIODIC)
+        return fail(CRUNA_HIP_EINVAL, "the closure needs the periodic S^3 grid it was built for");
+    if (int rc = cnn_reserve(n, n->N)) return rc;
+    if (!c->force) {
+      const size_t ts = c->cfg.precision == CRUNA_HIP_FP32_MIXED ? sizeof(float) : sizeof(double);
+      HIP_TRY(hipMalloc(&c->force, 5 * c->N * ts));
+    }
+    HIP_TRY(hipStreamSynchronize(n->stream));
+  }
+  c->cnn = n;
   return CRUNA_HIP_OK;
 }
 

@@ -354,7 +354,8 @@ __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__r
                                                LineOp<T> op, LineOp<T> fop, SweepGeom g, Phys ph,
                                                long N, int mode, int fuse_filter,
                                                double *__restrict__ qn, double *__restrict__ acc,
-                                               T *__restrict__ qs_out, double dt, Metric<T> mt) {
+                                               T *__restrict__ qs_out, double dt, Metric<T> mt,
+                                               const T *__restrict__ force) {
   constexpr bool LAST = DIR == 0;
   __shared__ LineLds<T> lds;
   const ThreadPos tp = thread_pos<DIR>(g);
@@ -419,7 +420,17 @@ __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__r
   // momentum accumulators; x: the complete RHS of that variable and its RK4
   // update), so only the remaining stress fields stay live across a solve;
   // the fences keep that traffic below the solve it depends on
-  if (LAST && (mode != 4 || GT)) {
+  // force (apply_force, ns_rho_rhou_p.f90:212): an additive term of the
+  // complete RHS, here the learned closure; null when there is none
+  auto add_force = [&](int v, T(&k)[PPT]) {
+    if (LAST && force) {
+      T fv[PPT];
+      ld<DIR>(force + v * N, B, PS, fv);
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) k[m] = k[m] + fv[m];
+    }
+  };
+  if (LAST && (mode != 4 || GT || force)) {
     // k_1 = rhs_1 (times J) is complete already: its stage update goes first
     T k[PPT];
     ld<DIR>(rhs, B, PS, k);
@@ -427,6 +438,7 @@ __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__r
 #pragma unroll
       for (int m = 0; m < PPT; ++m) k[m] = Jv[m] * k[m];
     }
+    add_force(0, k);
     finish(0, k);
   }
 #pragma unroll
@@ -441,10 +453,12 @@ __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__r
       k[m] = k[m] + x[m] * ih;
       if (LAST && GT) k[m] = Jv[m] * k[m];
     }
-    if (!LAST)
+    if (!LAST) {
       st<DIR>(rhs + (1 + i) * N, B, PS, k);
-    else
+    } else {
+      add_force(1 + i, k);
       finish(1 + i, k);
+    }
   }
   if (LAST) {
     // pressure: (gamma - 1) (bracket + tau:grad(u)) (ns_rho_rhou_p.f90:200-209)
@@ -454,6 +468,7 @@ __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__r
 #pragma unroll
     for (int m = 0; m < PPT; ++m)
       k[m] = GT ? (Jv[m] * T(ph.gm1)) * (k[m] + Jv[m] * diss[m]) : T(ph.gm1) * (k[m] + diss[m]);
+    add_force(4, k);
     finish(4, k);
   }
 }
