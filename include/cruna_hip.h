@@ -194,7 +194,9 @@ int cruna_hip_ensemble_autotune(cruna_hip_ensemble *ens, int steps, int *tile_li
  * It is the additive force term of the direct RHS (apply_force,
  * src/equations/ns_rho_rhou_p.f90:212 and src/opt/force.f90:28): with a
  * closure set, every direct RHS evaluation of the context adds
- * force = cnn(q) with the normalisation of cruna_hip_cnn_set_norm.
+ * force = cnn(q) with the normalisation of cruna_hip_cnn_set_norm, the terms
+ * of rho, rho u, rho v, rho w then shifted to zero domain mean (subfilter
+ * terms of conservation laws add nothing to the domain totals).
  * Parameters (FP32) are laid out per layer as W[27][co][ci] then b[co] with
  * the channel counts padded to (16,32), (32,32), (32,32), (32,16); taps
  * t = 9 (dz+1) + 3 (dy+1) + (dx+1).  The closure's buffers are shared by
@@ -211,6 +213,8 @@ int cruna_hip_cnn_get_params(cruna_hip_cnn *cnn, float *params);
  * mean(p) / p_ref when p_ref > 0 (its residual scales with the mean
  * pressure, i.e. 1 / Mach^2); defaults 1, 1 and 0 */
 int cruna_hip_cnn_set_norm(cruna_hip_cnn *cnn, const double in_std[5], const double out_std[5], double p_ref);
+/* scale of the closure term at inference (default 1; training unaffected) */
+int cruna_hip_cnn_set_gain(cruna_hip_cnn *cnn, double gain);
 /* closure (5 fields, FP64, field-major like q) of the state q */
 int cruna_hip_cnn_forward(cruna_hip_cnn *cnn, const double *q, double *closure);
 /* mean-square loss of the normalised output against target / out_std and

@@ -12,7 +12,8 @@ numpy, so that the device's BF16 matrix-core kernels can be checked:
             hidden:  x' = bf16(gelu(z)), z kept as bf16(z) for the backward pass
             last:    out = z (FP32)
   closure   force[v][p] = out[p][v] * s[v], s[v] = out_std[v], times
-            f32(m[4] / p_ref) for v = 4 when p_ref > 0
+            f32(m[4] / p_ref) for v = 4 when p_ref > 0; then, for v < 4,
+            minus the domain mean of force[v]
   loss      mean over p and the 5 real channels of (out - target / s)^2
 
 with periodic 3x3x3 taps t = 9 (dz+1) + 3 (dy+1) + (dx+1), o_t = (dx, dy, dz)
@@ -185,6 +186,8 @@ def closure(params, q, in_std, out_std, p_ref=0.0, bf16=True):
     for v in range(5):
         o = _ungrid(out[..., v])
         f[..., v] = (o.astype(np.float32) * s[v]).astype(np.float64) if bf16 else o * s[v]
+        if v < 4:
+            f[..., v] -= f[..., v].mean()
     return f
 
 

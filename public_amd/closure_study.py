@@ -1,0 +1,385 @@
+"""Learned subfilter closure for the periodic box: data, training, rollouts.
+
+A coarse grid (64^3) with the learned closure stands in for a fine one
+(256^3).  Everything runs through the library:
+
+* data: seeded fine runs (the plain solver, `Cruna.run_direct`) are box
+  filtered by the factor on the device (`Cruna.box_filter_state`), and the
+  target of each filtered state q_bar is the subfilter residual of the RHS,
+      target = box(RHS_fine(q_fine)) - RHS_coarse(q_bar),
+  for the 5 state variables (rho, rho u, rho v, rho w, p);
+* training: `Closure.train` (BF16 matrix-core forward and backward
+  kernels, Adam) on batches of 8 windows of 32^3 points;
+* rollouts: the coarse solver with the closure as its RHS force, from the
+  box-filtered fine initial state, at the coarse CFL step (factor times the
+  fine one), judged against the box-filtered plain fine solve by the
+  velocity energy spectrum and the L2 error.  The plain fine solve is also
+  the fallback: `ClosureModel.solve` uses the closure only when its
+  validation error is within the threshold.
+
+Run as a script it generates the data, trains, validates on held-out seeds
+and writes the parameters and a summary:
+    python -m public_amd.closure_study --out DIR
+"""
+import argparse
+import json
+import os
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from . import ic
+from .cruna import FP32_MIXED, FP64, POWER, Closure, Cruna
+
+GAMMA = 1.4
+P_REF = 1.0 / (GAMMA * 0.2 ** 2)  # mean pressure at Ma 0.2: the pressure output's reference
+
+
+def physics(mach):
+    """Periodic-box physics of the workload: Re 1600 on unit length and
+    velocity, Pr 0.71, power-law viscosity about the initial temperature,
+    compact filter every step.  The heat flux takes its derivatives with
+    the grid spacing (heat_unit_spacing 0), so that the fine and the coarse
+    grid solve the same equations (with unit spacing, as
+    ns_rho_rhou_p.f90:189-197 forms it, the conduction term would scale
+    with h^2 and differ 16-fold between 256^3 and 64^3)."""
+    return dict(gamma=GAMMA, R=1.0, W=1.0, Pr=0.71, mu0=1.0 / 1600.0, T0=1.0 / (GAMMA * mach ** 2),
+                visc_law=POWER, visc_exp=0.76, filter_alpha=0.35, filter_freq=(1, 1, 1), heat_unit_spacing=0)
+
+
+def fine_dt(q, n):
+    return ic.cfl_dt(q, ic.periodic_dx(n), GAMMA)
+
+
+def box_average(q, f):
+    """f^3 box average of (n, n, n, 5) Fortran-ordered fields (host)."""
+    n = q.shape[0]
+    m = n // f
+    return np.asfortranarray(q.reshape(f, m, f, m, f, m, 5, order="F").mean(axis=(0, 2, 4)))
+
+
+def velocity_spectrum(q):
+    """E(k) = 1/2 sum over the shell |k| in [k - 1/2, k + 1/2) of |u_hat|^2 for
+    the velocity (rho u / rho, ...) of a periodic state; k = 0 .. n/2."""
+    n = q.shape[0]
+    k1 = np.fft.fftfreq(n, d=1.0 / n)
+    kx, ky, kz = np.meshgrid(k1, k1, k1, indexing="ij")
+    shell = np.rint(np.sqrt(kx * kx + ky * ky + kz * kz)).astype(int)
+    E = np.zeros(n // 2 + 1)
+    for c in (1, 2, 3):
+        uh = np.fft.fftn(q[..., c] / q[..., 0]) / n ** 3
+        e = 0.5 * np.abs(uh) ** 2
+        m = shell <= n // 2
+        E += np.bincount(shell[m].ravel(), weights=e[m].ravel(), minlength=n // 2 + 1)
+    return E
+
+
+def spectrum_error(q, ref, kmax=None):
+    """Relative L2 difference of the velocity spectra over shells 1..kmax
+    (default: half the coarse grid's Nyquist wavenumber, the band the
+    coarse grid resolves; beyond it the box filter's aliased tail and the
+    coarse grid's own filter make the two incomparable)."""
+    a, b = velocity_spectrum(q), velocity_spectrum(ref)
+    kmax = kmax or (len(b) - 1) // 2
+    return float(np.linalg.norm(a[1:kmax + 1] - b[1:kmax + 1]) / np.linalg.norm(b[1:kmax + 1]))
+
+
+def state_error(q, ref):
+    """Relative L2 error of rho, rho u (all components) and p."""
+    out = {}
+    for name, sl in (("rho", [0]), ("rho_u", [1, 2, 3]), ("p", [4])):
+        a, b = q[..., sl], ref[..., sl]
+        if name == "p":
+            a, b = a - b.mean(), b - b.mean()
+        out[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# data
+
+def generate(closure, seeds, machs, n_fine=256, factor=4, steps=200, every=20, norm_runs=4, target="rhs",
+             log=print):
+    """Training samples of seeded fine runs into `closure` (the first
+    `norm_runs` runs also fix its normalisation).  Returns the statistics.
+
+    target "rhs": the subfilter residual of the RHS,
+        box(RHS_fine(q_f)) - RHS_coarse(box(q_f));
+    target "step": the subfilter residual of one coarse step (RK4 and
+    filter) per unit time,
+        (box(q_f(t + dt_c)) - Step_coarse(box(q_f(t)))) / dt_c,
+    i.e. the RHS residual of the time-discrete coarse dynamics, which also
+    carries what the coarse grid's filter and step size take away."""
+    m = n_fine // factor
+    hc = ic.periodic_dx(m)
+    pending, held = [], []
+    stats = None
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        nxt = pool.submit(ic.bandlimited_box_state, n_fine, seeds[0], machs[0])
+        for r, (seed, mach) in enumerate(zip(seeds, machs)):
+            q0 = nxt.result()
+            if r + 1 < len(seeds):
+                nxt = pool.submit(ic.bandlimited_box_state, n_fine, seeds[r + 1], machs[r + 1])
+            dt = fine_dt(q0, n_fine)
+            t0 = time.time()
+            kw = physics(mach)
+            with Cruna((n_fine,) * 3, (ic.periodic_dx(n_fine),) * 3, dt=dt, precision=FP32_MIXED, **kw) as fine, \
+                    Cruna((m,) * 3, (hc,) * 3, dt=dt * factor, **kw) as coarse:
+                fine.set_state(q0)
+                done = 0
+                for s in range(0, steps + 1, every):
+                    if s > done:
+                        fine.run_direct(s - done, nt0=done)
+                        done = s
+                    if target == "rhs":
+                        qb, rb = fine.box_filter_state(factor)
+                        t = np.asfortranarray(rb - coarse.right_hand_side_direct_3d(qb))
+                    else:
+                        if s + factor > steps:
+                            break
+                        qb, _ = fine.box_filter_state(factor, rhs=False)
+                        fine.run_direct(factor, nt0=done)
+                        done += factor
+                        qb1, _ = fine.box_filter_state(factor, rhs=False)
+                        coarse.set_state(qb)
+                        coarse.run_direct(1, nt0=0)
+                        t = np.asfortranarray((qb1 - coarse.get_state()) / (dt * factor))
+                    pending.append((qb, t))
+            if stats is None:
+                held += pending
+                pending = []
+                if r + 1 == min(norm_runs, len(seeds)):
+                    stats = normalisation(held)
+                    closure.set_norm(stats["in_std"], stats["out_std"], P_REF)
+                    for qb, t in held:
+                        closure.add_sample(qb, t)
+                    held = []
+            else:
+                for qb, t in pending:
+                    closure.add_sample(qb, t)
+                pending = []
+            log(f"closure data: run {r + 1}/{len(seeds)} seed {seed} Ma {mach:.3f} dt {dt:.3e} "
+                f"({time.time() - t0:.1f} s), samples {closure.num_samples()}")
+    return stats
+
+
+def normalisation(samples):
+    """Input scales (fluctuations about each sample's means) and output
+    scales (the pressure residual first divided by mean(p) / P_REF)."""
+    ins = np.zeros(5)
+    outs = np.zeros(5)
+    for qb, t in samples:
+        for c in range(5):
+            ins[c] += np.var(qb[..., c] - qb[..., c].mean())
+            scale = qb[..., 4].mean() / P_REF if c == 4 else 1.0
+            outs[c] += np.mean((t[..., c] / scale) ** 2)
+    k = len(samples)
+    in_std = np.sqrt(ins / k)
+    in_std[0] = max(in_std[0], 1e-6)
+    return {"in_std": in_std.tolist(), "out_std": np.sqrt(outs / k).tolist(), "p_ref": P_REF}
+
+
+# ---------------------------------------------------------------------------
+# rollouts
+
+def fine_reference(seed, mach, n_fine, factor, coarse_steps, every, precision=FP64, log=print):
+    """Box-filtered plain fine solve at coarse steps 0, every, ... and the
+    time the fine solver spent stepping (s)."""
+    q0 = ic.bandlimited_box_state(n_fine, seed, mach)
+    dt = fine_dt(q0, n_fine)
+    out, secs = [], 0.0
+    with Cruna((n_fine,) * 3, (ic.periodic_dx(n_fine),) * 3, dt=dt, precision=precision, **physics(mach)) as g:
+        g.set_state(q0)
+        out.append(g.box_filter_state(factor, rhs=False)[0])
+        for s in range(every, coarse_steps + 1, every):
+            t = time.perf_counter()
+            g.run_direct(every * factor, nt0=(s - every) * factor)
+            g.synchronize()
+            secs += time.perf_counter() - t
+            out.append(g.box_filter_state(factor, rhs=False)[0])
+            log(f"fine reference: seed {seed} coarse step {s}/{coarse_steps}")
+    return out, dt, secs
+
+
+def coarse_rollout(q0, mach, dt, steps, every, closure=None, precision=FP64):
+    """Coarse states at steps 0, every, ... and the stepping time (s)."""
+    m = q0.shape[0]
+    out, secs = [q0], 0.0
+    with Cruna((m,) * 3, (ic.periodic_dx(m),) * 3, dt=dt, precision=precision, **physics(mach)) as g:
+        if closure is not None:
+            g.set_closure(closure)
+        g.set_state(q0)
+        g.run_direct(1, nt0=0)  # warm-up of the launch path, then restart
+        g.synchronize()
+        g.set_state(q0)
+        for s in range(every, steps + 1, every):
+            t = time.perf_counter()
+            g.run_direct(every, nt0=s - every)
+            g.synchronize()
+            secs += time.perf_counter() - t
+            out.append(g.get_state())
+    return out, secs
+
+
+def evaluate(closure, seed, mach, n_fine=256, factor=4, steps=300, every=50, log=print, precision=FP64,
+             gains=(1.0,)):
+    """Held-out rollout without the closure and with it at each gain,
+    against the filtered fine solve."""
+    ref, dt_f, fine_secs = fine_reference(seed, mach, n_fine, factor, steps, every, log=log)
+    dt_c = dt_f * factor
+    res = {"seed": seed, "mach": mach, "coarse_steps": steps, "dt_coarse": dt_c, "fine_steps": steps * factor,
+           "fine_secs": fine_secs, "spectrum_ref": velocity_spectrum(ref[-1]).tolist()}
+    runs = [("no_closure", None, 1.0)] + [("closure" if g == 1.0 else f"closure_gain_{g:g}", closure, g)
+                                            for g in gains if closure is not None]
+    for name, cl, g in runs:
+        if cl is not None:
+            cl.set_gain(g)
+        states, secs = coarse_rollout(ref[0], mach, dt_c, steps, every, cl, precision)
+        res[name] = {
+            "secs": secs,
+            "spectrum_error": [spectrum_error(q, r) for q, r in zip(states, ref)],
+            "state_error": [state_error(q, r) for q, r in zip(states[1:], ref[1:])],
+            "finite": bool(np.all(np.isfinite(states[-1]))),
+            "spectrum": velocity_spectrum(states[-1]).tolist(),
+        }
+        res[name]["speedup_vs_fine"] = fine_secs / secs
+        log(f"rollout {name}: final spectrum error {res[name]['spectrum_error'][-1]:.4f}, "
+            f"{secs:.3f} s vs fine {fine_secs:.3f} s")
+    if closure is not None:
+        closure.set_gain(1.0)
+    return res
+
+
+class ClosureModel:
+    """A trained closure for one coarse grid with its inference gain and its
+    validation: the spectrum error of a validation rollout at coarse steps
+    0, every, 2 every, ...  `solve` runs the coarse grid with the closure
+    when the validated error at the requested horizon is within
+    `threshold`, else the plain fine solver (the fallback)."""
+
+    def __init__(self, path, threshold=0.10):
+        d = np.load(path)
+        self.S = int(d["S"])
+        self.factor = int(d["factor"])
+        self.params = d["params"].astype(np.float32)
+        self.in_std, self.out_std, self.p_ref = d["in_std"], d["out_std"], float(d["p_ref"])
+        self.gain = float(d["gain"])
+        self.val_steps = np.asarray(d["val_steps"], dtype=int)
+        self.val_errors = np.asarray(d["val_errors"], dtype=float)
+        self.threshold = threshold
+
+    def closure(self):
+        c = Closure(self.S)
+        c.set_norm(self.in_std, self.out_std, self.p_ref)
+        c.set_params(self.params)
+        c.set_gain(self.gain)
+        return c
+
+    def validated_error(self, coarse_steps):
+        """Validated spectrum error at the first checkpoint at or beyond
+        coarse_steps (inf beyond the validated horizon)."""
+        k = np.searchsorted(self.val_steps, coarse_steps)
+        return float(self.val_errors[k]) if k < len(self.val_steps) else np.inf
+
+    def usable(self, coarse_steps):
+        e = self.validated_error(coarse_steps)
+        return bool(np.isfinite(e) and e <= self.threshold)
+
+    def solve(self, q_fine0, mach, fine_steps):
+        """(box-filtered state after fine_steps fine steps' time, "closure" or
+        "fine"): the coarse closure solution where validated, else the
+        plain fine solution."""
+        n = q_fine0.shape[0]
+        dt = fine_dt(q_fine0, n)
+        coarse_steps = fine_steps // self.factor
+        if fine_steps % self.factor == 0 and n == self.S * self.factor and self.usable(coarse_steps):
+            cl = self.closure()
+            try:
+                states, _ = coarse_rollout(box_average(q_fine0, self.factor), mach, dt * self.factor,
+                                           coarse_steps, coarse_steps, cl)
+            finally:
+                cl.close()
+            return states[-1], "closure"
+        with Cruna((n,) * 3, (ic.periodic_dx(n),) * 3, dt=dt, **physics(mach)) as g:
+            g.set_state(q_fine0)
+            g.run_direct(fine_steps)
+            return g.box_filter_state(self.factor, rhs=False)[0], "fine"
+
+
+def save(path, closure, stats, factor, gain, val_steps, val_errors):
+    np.savez(path, S=closure.S, factor=factor, params=closure.get_params(), in_std=np.array(stats["in_std"]),
+             out_std=np.array(stats["out_std"]), p_ref=stats["p_ref"], gain=gain,
+             val_steps=np.asarray(val_steps), val_errors=np.asarray(val_errors))
+
+
+def main():
+    ap = argparse.ArgumentParser(description="generate closure data, train, validate")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--runs", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--every", type=int, default=20)
+    ap.add_argument("--iters", type=int, default=16000)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--val-seed", type=int, default=900)
+    ap.add_argument("--val-mach", type=float, default=0.25)
+    ap.add_argument("--holdout", type=int, nargs="*", default=[1001, 1002])
+    ap.add_argument("--holdout-mach", type=float, nargs="*", default=[0.3, 0.2])
+    ap.add_argument("--rollout-steps", type=int, default=300)
+    ap.add_argument("--check-every", type=int, default=25)
+    ap.add_argument("--n-fine", type=int, default=256)
+    ap.add_argument("--factor", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--target", choices=("rhs", "step"), default="step")
+    ap.add_argument("--gains", type=float, nargs="*", default=[0.5, 1.0, 1.5, 2.0])
+    args = ap.parse_args()
+    os.makedirs(args.out, exist_ok=True)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    rng = np.random.default_rng(args.seed)
+    seeds = list(range(100, 100 + args.runs))
+    machs = rng.uniform(0.1, 0.3, size=args.runs).tolist()
+    m = args.n_fine // args.factor
+    net = Closure(m, seed=args.seed)
+    t = time.time()
+    stats = generate(net, seeds, machs, args.n_fine, args.factor, args.steps, args.every, target=args.target,
+                     log=log)
+    gen_secs = time.time() - t
+    t = time.time()
+    losses = []
+    chunk = 500
+    for i in range(0, args.iters, chunk):
+        k = min(chunk, args.iters - i)
+        losses += net.train(k, lr=args.lr, batch=8, crop=32, seed=args.seed + i).tolist()
+        log(f"closure training: step {i + k}/{args.iters} loss {np.mean(losses[-k:]):.4f}")
+    train_secs = time.time() - t
+    # the gain: the best final spectrum error on the validation seed
+    val = evaluate(net, args.val_seed, args.val_mach, args.n_fine, args.factor, args.rollout_steps,
+                   every=args.check_every, log=log, gains=args.gains)
+    best = None
+    for g in args.gains:
+        r = val["closure" if g == 1.0 else f"closure_gain_{g:g}"]
+        err = r["spectrum_error"][-1]
+        if r["finite"] and np.isfinite(err) and (best is None or err < best[1]):
+            best = (g, err, r["spectrum_error"])
+    gain, _, val_errors = best if best else (0.0, np.inf, [np.inf] * (args.rollout_steps // args.check_every + 1))
+    val_steps = list(range(0, args.rollout_steps + 1, args.check_every))
+    log(f"closure gain {gain:g} (validation seed {args.val_seed})")
+    evals = [evaluate(net, s, mm, args.n_fine, args.factor, args.rollout_steps, every=args.check_every, log=log,
+                      gains=[gain]) for s, mm in zip(args.holdout, args.holdout_mach)]
+    save(os.path.join(args.out, "closure_params.npz"), net, stats, args.factor, gain, val_steps, val_errors)
+    summary = {"runs": args.runs, "steps": args.steps, "every": args.every, "target": args.target,
+               "samples": net.num_samples(), "data_secs": gen_secs, "train_secs": train_secs,
+               "iters": args.iters, "loss_first": float(np.mean(losses[:50])), "loss_last": float(np.mean(losses[-50:])),
+               "norm": stats, "gain": gain, "val_steps": val_steps, "validation": val, "holdout": evals}
+    with open(os.path.join(args.out, "closure_summary.json"), "w") as f:
+        json.dump(summary, f, indent=1)
+    np.save(os.path.join(args.out, "closure_losses.npy"), np.array(losses, dtype=np.float32))
+    log(json.dumps({k: v for k, v in summary.items() if k not in ("holdout", "validation")}))
+    net.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -110,6 +110,7 @@ def lib():
         L.cruna_hip_cnn_get_params.argtypes = [vp, fp]
         L.cruna_hip_cnn_set_norm.argtypes = [vp, dp, dp, C.c_double]
         L.cruna_hip_box_filter_state.argtypes = [vp, ip, dp, dp]
+        L.cruna_hip_cnn_set_gain.argtypes = [vp, C.c_double]
         L.cruna_hip_cnn_forward.argtypes = [vp, dp, dp]
         L.cruna_hip_cnn_gradients.argtypes = [vp, dp, dp, fp, fp]
         L.cruna_hip_cnn_add_sample.argtypes = [vp, dp, dp]
@@ -124,7 +125,7 @@ def lib():
                   "ensemble_create", "ensemble_destroy", "ensemble_member", "ensemble_set_role",
                   "ensemble_run", "ensemble_autotune", "cnn_create", "cnn_destroy", "cnn_init_params",
                   "cnn_set_params", "cnn_get_params", "cnn_set_norm", "cnn_forward", "cnn_gradients",
-                  "cnn_add_sample", "cnn_train", "set_closure", "box_filter_state"):
+                  "cnn_add_sample", "cnn_train", "set_closure", "box_filter_state", "cnn_set_gain"):
             getattr(L, "cruna_hip_" + f).restype = C.c_int
         _lib = L
     return _lib
@@ -474,6 +475,10 @@ class Closure:
         if any(x.shape != (5,) for x in a):
             raise CrunaError("normalisation needs 5 values per argument")
         _check(lib().cruna_hip_cnn_set_norm(self._h, _dp(a[0]), _dp(a[1]), float(p_ref)))
+
+    def set_gain(self, gain):
+        """Scale of the closure term at inference (training is unaffected)."""
+        _check(lib().cruna_hip_cnn_set_gain(self._h, float(gain)))
 
     def forward(self, q):
         q = _f64(q, self.shape)

@@ -31,15 +31,20 @@ __device__ __forceinline__ bf16 to_bf16(float f) {
 }
 __device__ __forceinline__ float from_bf16(bf16 h) { return __uint_as_float((unsigned)h << 16); }
 
-// GELU, tanh form: 0.5 z (1 + tanh(c (z + a z^3))), and its derivative
-__device__ __forceinline__ float gelu(float z) {
+// GELU, tanh form: 0.5 z (1 + tanh(u)), u = c (z + a z^3), written with the
+// logistic function s = 1 / (1 + exp(-2u)) (1 + tanh(u) = 2 s,
+// 1 - tanh(u)^2 = 4 s (1 - s)) on the hardware exp2 and reciprocal:
+//   gelu(z) = z s,   gelu'(z) = s + 2 z s (1 - s) c (1 + 3 a z^2)
+__device__ __forceinline__ float gelu_sigma(float z) {
   const float c = 0.7978845608028654f, a = 0.044715f;
-  return 0.5f * z * (1.f + tanhf(c * (z + a * z * z * z)));
+  const float u = c * (z + a * z * z * z);
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-2.8853900817779268f * u));  // 2 log2(e)
 }
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sigma(z); }
 __device__ __forceinline__ float gelu_grad(float z) {
   const float c = 0.7978845608028654f, a = 0.044715f;
-  const float t = tanhf(c * (z + a * z * z * z));
-  return 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * c * (1.f + 3.f * a * z * z);
+  const float s = gelu_sigma(z);
+  return s + 2.f * z * s * (1.f - s) * c * (1.f + 3.f * a * z * z);
 }
 
 // Geometry of a batch of boxes X x Y x Z, stored one after the other with
@@ -86,98 +91,242 @@ __device__ __forceinline__ long tap_neighbour(const Coord &c, int t, int sign, c
 
 enum ConvMode { CONV_GELU = 0, CONV_LINEAR = 1, CONV_BACKWARD = 2 };
 
-// One convolution as an implicit GEMM; each wavefront computes 16 points x CN
-// output channels.  CK: input (reduction) channels, CN: output channels.
+// Per-point neighbour offsets along one direction: off[0..2] for d = -1, 0,
+// +1 (wrapped on periodic boxes) and a validity bit per d (crops).
+struct AxisOffsets {
+  int off[3];
+  unsigned valid;  // bit d
+};
+__device__ __forceinline__ AxisOffsets axis_offsets(int i, int X, int stride, bool periodic, int sign) {
+  AxisOffsets a;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int ii = i + sign * (d - 1);
+    bool ok = true;
+    if (ii < 0) {
+      ok = periodic;
+      ii += X;
+    } else if (ii >= X) {
+      ok = periodic;
+      ii -= X;
+    }
+    a.off[d] = (ii - i) * stride;
+  }
+  a.valid = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int ii = i + sign * (d - 1);
+    if (periodic || (ii >= 0 && ii < X)) a.valid |= 1u << d;
+  }
+  return a;
+}
+
+// a[d] for a runtime d in 0..2 without indexing a register array
+__device__ __forceinline__ int pick3(const int (&a)[3], int d) { return d == 0 ? a[0] : (d == 1 ? a[1] : a[2]); }
+
+// One convolution as an implicit GEMM.  A work-group stages the layer's
+// weights in LDS once (rows padded by 8 elements against bank conflicts) and
+// then walks blocks of 256 consecutive points (grid-stride); each wavefront
+// takes 64 points as 4 tiles of 16, reuses every weight fragment for the 4
+// tiles, and finds the 27 neighbours of a point from per-axis offset tables.
+// CK: input (reduction) channels, CN: output channels.
 //   CONV_GELU     : z = conv + bias; zsave <- bf16(z) (if set); out16 <- bf16(gelu(z))
 //   CONV_LINEAR   : out32 <- conv + bias
 //   CONV_BACKWARD : taps mirrored, W given as [t][CN][CK] of the transposed
 //                   operator; out16 <- bf16(conv * gelu'(zin)) (zin: the
 //                   pre-activations of this layer's input, CN channels)
-// npts is a multiple of 64 (checked by the host).
+// npts is a multiple of 64 and npts * 32 < 2^31 (checked by the host).
+constexpr int CONV_TILES = 4;      // 16-point tiles per wavefront
+constexpr int CONV_BLOCK = 256;    // points per work-group block
+constexpr int CONV_XCDS = 8;       // XCDs of the MI355X, each with its own L2
+template <int CK, int CN>
+constexpr int conv_lds_row() { return CK + 8; }  // 16-byte aligned rows, staggered banks
+
 template <int CK, int CN, int MODE>
 __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in, const bf16 *__restrict__ W,
                                               const float *__restrict__ bias, ConvGeom geo, long npts,
                                               float *__restrict__ out32, bf16 *__restrict__ out16,
                                               bf16 *__restrict__ zsave, const bf16 *__restrict__ zin) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long p0 = ((long)blockIdx.x * 4 + wave) * 16;
-  if (p0 >= npts) return;
-  const int r = lane & 15, kg = lane >> 4;
-  const Coord pc = coord_of(p0 + r, geo);
-  f32x4 acc[CN / 16];
+  constexpr int ROW = conv_lds_row<CK, CN>();
+  __shared__ __align__(16) bf16 wl[27 * CN * ROW];
+  // stage W [t][CN][CK] -> wl [t][CN][ROW] in 16-byte pieces, every load
+  // issued before the first store
+  {
+    constexpr int PIECES = 27 * CN * CK / 8, BATCH = 4;
+    for (int base = 0; base < PIECES; base += 256 * BATCH) {
+      uint4 v[BATCH];
 #pragma unroll
-  for (int n = 0; n < CN / 16; ++n) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int sign = MODE == CONV_BACKWARD ? -1 : 1;
-  for (int t = 0; t < 27; ++t) {
-    const long nb = tap_neighbour(pc, t, sign, geo);
+      for (int u = 0; u < BATCH; ++u) {
+        const int e = base + threadIdx.x + 256 * u;
+        v[u] = e < PIECES ? *reinterpret_cast<const uint4 *>(W + (long)e * 8) : uint4{0u, 0u, 0u, 0u};
+      }
 #pragma unroll
-    for (int kc = 0; kc < CK / 16; ++kc) {
-      const bf16x4 a = nb >= 0 ? *reinterpret_cast<const bf16x4 *>(in + nb * CK + kc * 16 + kg * 4)
-                               : bf16x4{0, 0, 0, 0};
-#pragma unroll
-      for (int n = 0; n < CN / 16; ++n) {
-        const bf16x4 w =
-            *reinterpret_cast<const bf16x4 *>(W + ((long)(t * CN + n * 16 + r) * CK + kc * 16 + kg * 4));
-        acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w, acc[n], 0, 0, 0);
+      for (int u = 0; u < BATCH; ++u) {
+        const int e = base + threadIdx.x + 256 * u;
+        if (e < PIECES) *reinterpret_cast<uint4 *>(wl + (e / (CK / 8)) * ROW + (e % (CK / 8)) * 8) = v[u];
       }
     }
   }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, kg = lane >> 4;
+  const int sign = MODE == CONV_BACKWARD ? -1 : 1;
+  const bool per = geo.periodic != 0;
+  // work-groups are dealt to the 8 XCDs in turn (id % 8): give XCD x the
+  // contiguous range of blocks [x span, (x + 1) span), so that the
+  // neighbour planes a block gathers from are in that XCD's own L2
+  const int nblk = (int)((npts + CONV_BLOCK - 1) / CONV_BLOCK);
+  const int span = (nblk + CONV_XCDS - 1) / CONV_XCDS;
+  for (int v = blockIdx.x; v < span * CONV_XCDS; v += gridDim.x) {
+    const int blk = (v % CONV_XCDS) * span + v / CONV_XCDS;
+    if (blk >= nblk) continue;
+    const int p0 = blk * CONV_BLOCK + wave * 16 * CONV_TILES;
+    if (p0 >= npts) continue;  // npts is a multiple of 64: a wave's 64 points are all in or all out
+    int pt[CONV_TILES];
+    AxisOffsets ax[CONV_TILES], ay[CONV_TILES], az[CONV_TILES];
 #pragma unroll
-  for (int n = 0; n < CN / 16; ++n) {
-    const int ch = n * 16 + r;
-#pragma unroll
-    for (int rr = 0; rr < 4; ++rr) {
-      const long q = p0 + 4 * kg + rr;
-      const long o = q * CN + ch;
-      if (MODE == CONV_GELU) {
-        const float z = acc[n][rr] + bias[ch];
-        if (zsave) zsave[o] = to_bf16(z);
-        out16[o] = to_bf16(gelu(z));
-      } else if (MODE == CONV_LINEAR) {
-        out32[o] = acc[n][rr] + bias[ch];
-      } else {
-        out16[o] = to_bf16(acc[n][rr] * gelu_grad(from_bf16(zin[o])));
-      }
+    for (int j = 0; j < CONV_TILES; ++j) {
+      pt[j] = p0 + 16 * j + r;
+      const Coord cc = coord_of(pt[j], geo);
+      ax[j] = axis_offsets(cc.i, geo.X, 1, per, sign);
+      ay[j] = axis_offsets(cc.j, geo.Y, geo.X, per, sign);
+      az[j] = axis_offsets(cc.k, geo.Z, geo.X * geo.Y, per, sign);
     }
+    f32x4 acc[CONV_TILES][CN / 16];
+#pragma unroll
+    for (int j = 0; j < CONV_TILES; ++j)
+#pragma unroll
+      for (int n = 0; n < CN / 16; ++n) acc[j][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll 1
+      for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+          const int t = 9 * dz + 3 * dy + dx;
+          bf16x4 a[CONV_TILES][CK / 16];
+#pragma unroll
+          for (int j = 0; j < CONV_TILES; ++j) {
+            const int nb = pt[j] + ax[j].off[dx] + pick3(ay[j].off, dy) + pick3(az[j].off, dz);
+            const bool ok = per || ((ax[j].valid >> dx) & (ay[j].valid >> dy) & (az[j].valid >> dz) & 1u);
+#pragma unroll
+            for (int kc = 0; kc < CK / 16; ++kc)
+              a[j][kc] = ok ? *reinterpret_cast<const bf16x4 *>(in + nb * CK + kc * 16 + kg * 4)
+                            : bf16x4{0, 0, 0, 0};
+          }
+#pragma unroll
+          for (int n = 0; n < CN / 16; ++n)
+#pragma unroll
+            for (int kc = 0; kc < CK / 16; ++kc) {
+              const bf16x4 w =
+                  *reinterpret_cast<const bf16x4 *>(wl + (t * CN + n * 16 + r) * ROW + kc * 16 + kg * 4);
+#pragma unroll
+              for (int j = 0; j < CONV_TILES; ++j)
+                acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[j][kc], w, acc[j][n], 0, 0, 0);
+            }
+        }
+#pragma unroll
+    for (int j = 0; j < CONV_TILES; ++j)
+#pragma unroll
+      for (int n = 0; n < CN / 16; ++n) {
+        const int ch = n * 16 + r;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int o = (p0 + 16 * j + 4 * kg + rr) * CN + ch;
+          if (MODE == CONV_GELU) {
+            const float z = acc[j][n][rr] + bias[ch];
+            if (zsave) zsave[o] = to_bf16(z);
+            out16[o] = to_bf16(gelu(z));
+          } else if (MODE == CONV_LINEAR) {
+            out32[o] = acc[j][n][rr] + bias[ch];
+          } else {
+            out16[o] = to_bf16(acc[j][n][rr] * gelu_grad(from_bf16(zin[o])));
+          }
+        }
+      }
   }
 }
 
-// Weight gradient partials: block (chunk, tap) sums dz[p][co] in[p + o_t][ci]
-// over the chunk's points into part[chunk][t][co][ci].  Each wavefront takes
-// 16x16 (co, ci) tiles in turn.
+// Weight gradient partials of one tap: block (chunk, t) sums
+// dz[p][co] in[p + o_t][ci] over the chunk's points into
+// part[chunk][t][co][ci], as an MFMA GEMM with the points as reduction
+// index: sub-chunks of WG_SUB points of dz and of the tap-shifted input are
+// staged transposed in LDS ([channel][point]), so that every lane reads
+// its 4 consecutive points of one channel as one 8-byte word.  Blocks of
+// tap 0 also sum dz over their points into bpart[chunk][co] (bias
+// gradient partials).  Wavefront w takes the (co, ci) tile w (at most 4).
+constexpr int WG_SUB = 256;
 template <int CI, int CO>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16 *__restrict__ in, const bf16 *__restrict__ dz,
                                                     ConvGeom geo, long npts, int chunk,
-                                                    float *__restrict__ part) {
+                                                    float *__restrict__ part, float *__restrict__ bpart) {
+  constexpr int LD = WG_SUB + 8;
+  __shared__ __align__(16) bf16 dzT[CO * LD];
+  __shared__ __align__(16) bf16 inT[CI * LD];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 15, kg = lane >> 4;
   const int t = blockIdx.y;
-  const long c0 = (long)blockIdx.x * chunk;
-  const long c1 = c0 + chunk < npts ? c0 + chunk : npts;
-  constexpr int NT_CO = CO / 16, NT_CI = CI / 16;
-  for (int tile = wave; tile < NT_CO * NT_CI; tile += 4) {
-    const int mb = tile / NT_CI, nbk = tile % NT_CI;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (long pc = c0; pc < c1; pc += 16) {
-      bf16x4 a, b;
+  const int tx = t % 3, ty = (t / 3) % 3, tz = t / 9;
+  const int c0 = blockIdx.x * chunk;
+  const int c1 = c0 + chunk < npts ? c0 + chunk : (int)npts;
+  constexpr int NT_CI = CI / 16, NTILE = (CO / 16) * NT_CI;
+  static_assert(NTILE <= 4, "one tile per wavefront");
+  const bool has_tile = wave < NTILE;
+  const int mb = wave / NT_CI, nbk = wave % NT_CI;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
+  const bool per = geo.periodic != 0;
+  for (int sp = c0; sp < c1; sp += WG_SUB) {
+    const int np = c1 - sp < WG_SUB ? c1 - sp : WG_SUB;  // a multiple of 16
+    __syncthreads();
+    for (int e = threadIdx.x; e < np * (CO / 8); e += 256) {
+      const int q = e / (CO / 8), piece = e % (CO / 8);
+      const uint4 v = *reinterpret_cast<const uint4 *>(dz + (long)(sp + q) * CO + piece * 8);
+      const unsigned w4[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const long q = pc + 4 * kg + j;
-        const long nb = tap_neighbour(coord_of(q, geo), t, 1, geo);
-        a[j] = (short)dz[q * CO + mb * 16 + r];
-        b[j] = nb >= 0 ? (short)in[nb * CI + nbk * 16 + r] : (short)0;
+      for (int u = 0; u < 4; ++u) {
+        dzT[(piece * 8 + 2 * u) * LD + q] = (bf16)(w4[u] & 0xffffu);
+        dzT[(piece * 8 + 2 * u + 1) * LD + q] = (bf16)(w4[u] >> 16);
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
+    for (int e = threadIdx.x; e < np * (CI / 8); e += 256) {
+      const int q = e / (CI / 8), piece = e % (CI / 8);
+      const Coord cc = coord_of(sp + q, geo);
+      const AxisOffsets ax = axis_offsets(cc.i, geo.X, 1, per, 1);
+      const AxisOffsets ay = axis_offsets(cc.j, geo.Y, geo.X, per, 1);
+      const AxisOffsets az = axis_offsets(cc.k, geo.Z, geo.X * geo.Y, per, 1);
+      const bool ok = per || ((ax.valid >> tx) & (ay.valid >> ty) & (az.valid >> tz) & 1u);
+      const int nb = sp + q + pick3(ax.off, tx) + pick3(ay.off, ty) + pick3(az.off, tz);
+      uint4 v = {0u, 0u, 0u, 0u};
+      if (ok) v = *reinterpret_cast<const uint4 *>(in + (long)nb * CI + piece * 8);
+      const unsigned w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        inT[(piece * 8 + 2 * u) * LD + q] = (bf16)(w4[u] & 0xffffu);
+        inT[(piece * 8 + 2 * u + 1) * LD + q] = (bf16)(w4[u] >> 16);
+      }
+    }
+    __syncthreads();
+    if (has_tile)
+      for (int k = 0; k < np; k += 16) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(dzT + (mb * 16 + r) * LD + k + 4 * kg);
+        const bf16x4 b = *reinterpret_cast<const bf16x4 *>(inT + (nbk * 16 + r) * LD + k + 4 * kg);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+      }
+    if (t == 0 && (int)threadIdx.x < CO)
+      for (int k = 0; k < np; ++k) bsum += from_bf16(dzT[threadIdx.x * LD + k]);
+  }
+  if (has_tile) {
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
       const int co = mb * 16 + 4 * kg + rr, ci = nbk * 16 + r;
       part[(((long)blockIdx.x * 27 + t) * CO + co) * CI + ci] = acc[rr];
     }
   }
+  if (t == 0 && (int)threadIdx.x < CO) bpart[(long)blockIdx.x * CO + threadIdx.x] = bsum;
 }
 
-// dW[e] = sum over chunks of part[chunk][e] (e over 27 CO CI entries)
+// dW[e] = sum over chunks of part[chunk][e] (e over nelem entries)
 __global__ void k_sum_chunks(const float *__restrict__ part, int nchunks, long nelem,
                              float *__restrict__ dW) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -185,22 +334,6 @@ __global__ void k_sum_chunks(const float *__restrict__ part, int nchunks, long n
   float s = 0.f;
   for (int c = 0; c < nchunks; ++c) s += part[c * nelem + e];
   dW[e] = s;
-}
-
-// db[co] = sum_p dz[p][co]: one block per channel, fixed reduction order
-__global__ __launch_bounds__(256) void k_bias_grad(const bf16 *__restrict__ dz, long npts, int CO,
-                                                   float *__restrict__ db) {
-  __shared__ float red[256];
-  const int co = blockIdx.x;
-  float s = 0.f;
-  for (long p = threadIdx.x; p < npts; p += 256) s += from_bf16(dz[p * CO + co]);
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) db[co] = red[0];
 }
 
 // Mean-square loss over the nout real channels of out[p][16] against
@@ -295,17 +428,35 @@ __global__ void k_pack_weights(const float *__restrict__ w, int CO, int CI, bf16
   wt[((long)t * CI + ci) * CO + co] = h;
 }
 
-// Means of the 5 fields of a state (one work-group, fixed summation order):
-// the network sees every sample relative to its own means.
+// Means of the 5 fields of a state, in a fixed summation order: partial
+// sums of MEAN_BLOCKS grid-strided work-groups, then one work-group adds
+// them.  The network sees every sample relative to its own means.
+constexpr int MEAN_BLOCKS = 256;
 template <typename Q>
-__global__ __launch_bounds__(1024) void k_field_means(const Q *__restrict__ q, long N, double *__restrict__ mean) {
-  __shared__ double red[1024];
+__global__ __launch_bounds__(256) void k_field_partials(const Q *__restrict__ q, long N,
+                                                        double *__restrict__ part) {
+  __shared__ double red[256];
   for (int c = 0; c < 5; ++c) {
     double s = 0.0;
-    for (long p = threadIdx.x; p < N; p += 1024) s += (double)q[c * N + p];
+    for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < N; p += (long)gridDim.x * 256) s += (double)q[c * N + p];
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[c * gridDim.x + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_field_means(const double *__restrict__ part, int nparts, long N,
+                                                     double *__restrict__ mean) {
+  __shared__ double red[256];
+  for (int c = 0; c < 5; ++c) {
+    red[threadIdx.x] = (int)threadIdx.x < nparts ? part[c * nparts + threadIdx.x] : 0.0;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
       if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
       __syncthreads();
     }
@@ -320,6 +471,7 @@ __global__ __launch_bounds__(1024) void k_field_means(const Q *__restrict__ q, l
 // equation when p_ref > 0 (its residual scales with the mean pressure).
 struct CnnNorm {
   float inv_std[5], out_std[5], out_inv_std[5];
+  float gain;  // inference only: force = gain x the network's term
   double p_ref;
 };
 
@@ -346,7 +498,7 @@ __global__ void k_pack_features(const Q *__restrict__ q, long N, const double *_
   d[1] = uint4{0u, 0u, 0u, 0u};
 }
 
-// Closure term in the solver's layout: force[v][p] = out[p][v] out_std[v] (x pressure scale)
+// Closure term in the solver's layout: force[v][p] = out[p][v] out_std[v] (x pressure scale) x gain
 template <typename T>
 __global__ void k_unpack_closure(const float *__restrict__ out, long N, const CnnNorm *__restrict__ nm,
                                  const double *__restrict__ mean, T *__restrict__ force) {
@@ -356,8 +508,19 @@ __global__ void k_unpack_closure(const float *__restrict__ out, long N, const Cn
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const float s = v == 4 ? nm->out_std[4] * ps : nm->out_std[v];
-    force[v * N + p] = T(out[p * 16 + v] * s);
+    force[v * N + p] = T(out[p * 16 + v] * s * nm->gain);
   }
+}
+
+// Zero domain mean for the closure terms of the conserved variables (rho,
+// rho u, rho v, rho w): a subfilter term of a conservation law is the
+// divergence of a subfilter flux and adds nothing to the domain total.
+template <typename T>
+__global__ void k_remove_mean(T *__restrict__ force, long N, const double *__restrict__ mean) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) force[v * N + p] = force[v * N + p] - T(mean[v]);
 }
 
 // Normalised targets: tn[c][p] = target[c][p] out_inv_std[c] (/ pressure scale)

@@ -1389,7 +1389,7 @@ struct cruna_hip_cnn {
   // per-point work buffers, sized for `cap` points
   long cap = 0;
   bf16 *act[L] = {}, *z[L - 1] = {}, *dz_a = nullptr, *dz_b = nullptr;
-  float *out32 = nullptr, *tgt = nullptr, *lpart = nullptr, *part = nullptr;
+  float *out32 = nullptr, *tgt = nullptr, *lpart = nullptr, *part = nullptr, *bpart = nullptr;
   long part_cap = 0;
   CropSpec *spec = nullptr;
   long spec_cap = 0;
@@ -1401,7 +1401,15 @@ struct cruna_hip_cnn {
 
 namespace {
 
-constexpr int WGRAD_CHUNK = 4096;
+constexpr int WGRAD_CHUNK = 8192;
+
+// work-groups of k_conv: one per 256-point block up to two per CU (each
+// stages the weights once), a multiple of the XCD count so that a
+// work-group stays on its XCD's block range
+dim3 conv_grid(long npts) {
+  const long nblk = (npts + CONV_BLOCK - 1) / CONV_BLOCK;
+  return dim3((unsigned)std::min<long>((nblk + CONV_XCDS - 1) / CONV_XCDS * CONV_XCDS, 512));
+}
 
 ConvGeom conv_geom(int X, int Y, int Z, bool periodic) {
   ConvGeom g;
@@ -1430,6 +1438,7 @@ int cnn_alloc(std::vector<void *> &list, X **p, size_t count) {
 // work buffers for npts points (kept; grown when a larger batch comes)
 int cnn_reserve(cruna_hip_cnn *n, long npts) {
   if (npts <= n->cap) return CRUNA_HIP_OK;
+  if (npts * 32 >= (1L << 31)) return fail(CRUNA_HIP_EINVAL, "closure batch too large");
   HIP_TRY(hipStreamSynchronize(n->stream));
   for (void *q : n->work) (void)hipFree(q);
   n->work.clear();
@@ -1445,6 +1454,7 @@ int cnn_reserve(cruna_hip_cnn *n, long npts) {
   if (!rc) rc = cnn_alloc(n->work, &n->lpart, (npts + 255) / 256);
   const long nch = (npts + WGRAD_CHUNK - 1) / WGRAD_CHUNK;
   if (!rc) rc = cnn_alloc(n->work, &n->part, (size_t)nch * 27 * 32 * 32);
+  if (!rc) rc = cnn_alloc(n->work, &n->bpart, (size_t)nch * 32);
   if (rc) {
     for (void *q : n->work) (void)hipFree(q);
     n->work.clear();
@@ -1468,7 +1478,7 @@ int cnn_pack(cruna_hip_cnn *n, hipStream_t s) {
 // `train` the pre-activations are kept for the backward pass
 int cnn_forward(cruna_hip_cnn *n, hipStream_t s, const ConvGeom &geo, long npts, bool train) {
   if (npts % 64 != 0 || npts > n->cap) return fail(CRUNA_HIP_EINVAL, "closure batch size");
-  const dim3 grid((unsigned)(npts / 64)), block(256);
+  const dim3 grid = conv_grid(npts), block(256);
   const float *P = n->params;
   hipLaunchKernelGGL((k_conv<16, 32, CONV_GELU>), grid, block, 0, s, n->act[0], n->wf + n->wpk[0],
                      P + n->boff[0], geo, npts, (float *)nullptr, n->act[1], train ? n->z[0] : nullptr,
@@ -1490,11 +1500,12 @@ void cnn_wgrad(cruna_hip_cnn *n, hipStream_t s, int l, const ConvGeom &geo, long
                const bf16 *dz) {
   const int nch = (int)((npts + WGRAD_CHUNK - 1) / WGRAD_CHUNK);
   hipLaunchKernelGGL((k_conv_wgrad<CI, CO>), dim3(nch, 27), dim3(256), 0, s, in, dz, geo, npts, WGRAD_CHUNK,
-                     n->part);
+                     n->part, n->bpart);
   const long e = 27L * CO * CI;
   hipLaunchKernelGGL(k_sum_chunks, dim3((e + 255) / 256), dim3(256), 0, s, (const float *)n->part, nch, e,
                      n->grads + n->woff[l]);
-  hipLaunchKernelGGL(k_bias_grad, dim3(CO), dim3(256), 0, s, dz, npts, CO, n->grads + n->boff[l]);
+  hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, s, (const float *)n->bpart, nch, (long)CO,
+                     n->grads + n->boff[l]);
 }
 
 // loss over the points at least `halo` from their box's faces (count of
@@ -1502,7 +1513,7 @@ void cnn_wgrad(cruna_hip_cnn *n, hipStream_t s, int l, const ConvGeom &geo, long
 // with `train`; *loss (if set) receives the mean-square loss
 int cnn_backward(cruna_hip_cnn *n, hipStream_t s, const ConvGeom &geo, long npts, int halo, long counted,
                  float *loss) {
-  const dim3 grid((unsigned)(npts / 64)), block(256);
+  const dim3 grid = conv_grid(npts), block(256);
   const int nb = (int)((npts + 255) / 256);
   const float scale = (float)(2.0 / ((double)counted * 5));
   hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, s, (const float *)n->out32, (const float *)n->tgt, npts,
@@ -1538,14 +1549,23 @@ int cnn_backward(cruna_hip_cnn *n, hipStream_t s, const ConvGeom &geo, long npts
 template <typename T, typename Q>
 int closure_eval(cruna_hip_cnn *n, hipStream_t s, const Q *q, T *force) {
   const unsigned nb = (unsigned)((n->N + 255) / 256);
-  hipLaunchKernelGGL((k_field_means<Q>), dim3(1), dim3(1024), 0, s, q, n->N, n->fmean);
+  hipLaunchKernelGGL((k_field_partials<Q>), dim3(MEAN_BLOCKS), dim3(256), 0, s, q, n->N, n->fmean + 8);
+  hipLaunchKernelGGL(k_field_means, dim3(1), dim3(256), 0, s, (const double *)(n->fmean + 8), MEAN_BLOCKS, n->N,
+                     n->fmean);
   hipLaunchKernelGGL((k_pack_features<Q>), dim3(nb), dim3(256), 0, s, q, n->N, (const double *)n->fmean,
                      (const CnnNorm *)n->dn, n->act[0]);
   if (int rc = check_launch()) return rc;
   if (int rc = cnn_forward(n, s, conv_geom(n->S, n->S, n->S, true), n->N, false)) return rc;
-  if (force)
+  if (force) {
     hipLaunchKernelGGL((k_unpack_closure<T>), dim3(nb), dim3(256), 0, s, (const float *)n->out32, n->N,
                        (const CnnNorm *)n->dn, (const double *)n->fmean, force);
+    // domain means of the force (after the state's means are consumed)
+    hipLaunchKernelGGL((k_field_partials<T>), dim3(MEAN_BLOCKS), dim3(256), 0, s, (const T *)force, n->N,
+                       n->fmean + 8);
+    hipLaunchKernelGGL(k_field_means, dim3(1), dim3(256), 0, s, (const double *)(n->fmean + 8), MEAN_BLOCKS, n->N,
+                       n->fmean);
+    hipLaunchKernelGGL((k_remove_mean<T>), dim3(nb), dim3(256), 0, s, force, n->N, (const double *)n->fmean);
+  }
   return check_launch();
 }
 
@@ -1571,8 +1591,10 @@ int cnn_stage(cruna_hip_cnn *n, const double *q, const double *target, StageWhat
   if (what == STAGE_FORCE) {
     if (int rc = closure_eval<double, double>(n, n->stream, qd, td)) return rc;
   } else {
-    hipLaunchKernelGGL((k_field_means<double>), dim3(1), dim3(1024), 0, n->stream, (const double *)qd, n->N,
-                       n->fmean);
+    hipLaunchKernelGGL((k_field_partials<double>), dim3(MEAN_BLOCKS), dim3(256), 0, n->stream, (const double *)qd,
+                       n->N, n->fmean + 8);
+    hipLaunchKernelGGL(k_field_means, dim3(1), dim3(256), 0, n->stream, (const double *)(n->fmean + 8), MEAN_BLOCKS,
+                       n->N, n->fmean);
     hipLaunchKernelGGL((k_pack_features<double>), dim3(nb), dim3(256), 0, n->stream, (const double *)qd, n->N,
                        (const double *)n->fmean, (const CnnNorm *)n->dn, n->act[0]);
     if (int rc = check_launch()) return rc;
@@ -1615,9 +1637,10 @@ int cruna_hip_cnn_create(int S, cruna_hip_cnn **out) {
   if (!rc) rc = cnn_alloc(n->fixed, &n->wf, pk);
   if (!rc) rc = cnn_alloc(n->fixed, &n->wt, pk);
   if (!rc) rc = cnn_alloc(n->fixed, &n->dn, 1);
-  if (!rc) rc = cnn_alloc(n->fixed, &n->fmean, 8);
+  if (!rc) rc = cnn_alloc(n->fixed, &n->fmean, 8 + 5 * MEAN_BLOCKS);  // means, then partial sums
   if (!rc) rc = cnn_reserve(n.get(), n->N);
   for (int k = 0; k < 5; ++k) n->hn.inv_std[k] = n->hn.out_std[k] = n->hn.out_inv_std[k] = 1.f;
+  n->hn.gain = 1.f;
   n->hn.p_ref = 0.0;
   if (!rc && (hipMemcpy(n->dn, &n->hn, sizeof n->hn, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemset(n->params, 0, o * sizeof(float)) != hipSuccess ||
@@ -1698,6 +1721,15 @@ int cruna_hip_cnn_set_norm(cruna_hip_cnn *n, const double in_std[5], const doubl
     n->hn.out_inv_std[k] = (float)(1.0 / out_std[k]);
   }
   n->hn.p_ref = p_ref;
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  HIP_TRY(hipMemcpy(n->dn, &n->hn, sizeof n->hn, hipMemcpyHostToDevice));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_cnn_set_gain(cruna_hip_cnn *n, double gain) {
+  if (!n) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (!(gain >= 0.0)) return fail(CRUNA_HIP_EINVAL, "gain must be >= 0");
+  n->hn.gain = (float)gain;
   HIP_TRY(hipStreamSynchronize(n->stream));
   HIP_TRY(hipMemcpy(n->dn, &n->hn, sizeof n->hn, hipMemcpyHostToDevice));
   return CRUNA_HIP_OK;

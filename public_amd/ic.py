@@ -90,3 +90,51 @@ def kinetic_energy(q):
     """Volume-mean kinetic energy 0.5 <rho |u|^2>."""
     rho = q[..., 0]
     return float((0.5 * (q[..., 1] ** 2 + q[..., 2] ** 2 + q[..., 3] ** 2) / rho).mean())
+
+
+def bandlimited_velocity(n, seed, k0=4.0, kmax=16):
+    """(u, v, w) of the same spectrum, drawn only for wavenumbers in
+    [-kmax, kmax)^3 and summed on the grid by separable partial DFTs.
+
+    E(k) ~ k^4 exp(-2 (k/k0)^2) is below 1e-10 of its peak beyond k = 16, so
+    the truncation changes nothing measurable, and the field is the same
+    function on every grid finer than 2 kmax: a 256^3 state costs three
+    matrix products instead of full FFTs, and a coarse grid can be given
+    the very same field.  Same seed, other draws than `solenoidal_velocity`."""
+    rng = np.random.default_rng(seed)
+    k1 = np.arange(-kmax, kmax, dtype=np.float64)
+    kx, ky, kz = np.meshgrid(k1, k1, k1, indexing="ij")
+    kk = np.sqrt(kx * kx + ky * ky + kz * kz)
+    nz = kk > 0
+    amp = np.zeros_like(kk)
+    amp[nz] = np.sqrt(kk[nz] ** 4 * np.exp(-2.0 * (kk[nz] / k0) ** 2) / (4.0 * np.pi * kk[nz] ** 2))
+    uh = []
+    for _ in range(3):
+        phase = rng.uniform(0.0, 2.0 * np.pi, size=kk.shape)
+        mag = rng.standard_normal(size=kk.shape)
+        uh.append(amp * mag * np.exp(1j * phase))
+    kd = np.where(nz, kk * kk, 1.0)
+    dot = (kx * uh[0] + ky * uh[1] + kz * uh[2]) / kd
+    uh = [uh[0] - kx * dot, uh[1] - ky * dot, uh[2] - kz * dot]
+    x = 2.0 * np.pi * np.arange(n) / n
+    E = np.exp(1j * np.outer(x, k1))  # (n, 2 kmax)
+    vel = []
+    for c in uh:
+        a = c @ E.T                                   # (kx, ky, z)
+        a = np.einsum("ybz,jb->yjz", a, E)            # (kx, y, z)
+        a = (E @ a.reshape(a.shape[0], -1)).reshape(n, n, n)  # (x, y, z)
+        vel.append(np.real(a))
+    rms = np.sqrt((vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2).mean() / 3.0)
+    return [np.asfortranarray(c / rms) for c in vel]
+
+
+def bandlimited_box_state(n, seed, mach, gamma=1.4, kmax=16):
+    """q(n, n, n, 5) as `periodic_box_state` from `bandlimited_velocity`."""
+    u, v, w = bandlimited_velocity(n, seed, kmax=kmax)
+    q = np.zeros((n, n, n, 5), order="F")
+    q[..., 0] = 1.0
+    q[..., 1] = u
+    q[..., 2] = v
+    q[..., 3] = w
+    q[..., 4] = 1.0 / (gamma * mach * mach)
+    return q
