@@ -19,6 +19,11 @@ its algorithmic bytes per launch (compulsory reads and writes of the sweep)
 over its HIP-event-timed average duration, against 8 TB/s.  `cpu_baseline`
 times the C oracle on a bounded sample of the same workload (one RHS
 evaluation and the three filter passes of the 256^3 state, one host core).
+`closure_64_vs_256` runs the committed learned closure at 64^3 against the
+plain 256^3 FP64 solve over the same physical time from a held-out seed:
+wall-clock speedup, velocity-spectrum error against the box-filtered fine
+solve (with and without the closure), and the closure evaluation's
+HIP-event time against the BF16 matrix-core peak.
 """
 import argparse
 import json
@@ -32,6 +37,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+# closure network: multiply-adds per point of its four 3x3x3 convolutions
+# over the real channels (5 -> 32 -> 32 -> 32 -> 5)
+CLOSURE_MACS_PER_POINT = 27 * (5 * 32 + 32 * 32 + 32 * 32 + 32 * 5)
 # Algorithmic FP64-equivalent words per grid point and launch (compulsory
 # reads + writes of each sweep; FP32-mixed fields count as half words).
 SWEEP_WORDS = {
@@ -243,6 +252,76 @@ def run_ensemble(steps, nmembers=32, seed=0, autotune=True):
             "identical_to_serial": bool(same)}
 
 
+def run_closure(coarse_steps, seed=1001, mach=0.3):
+    """Learned closure at 64^3 against the plain 256^3 solve over the same
+    physical time (coarse_steps steps at 4x the fine step), from a held-out
+    seed: plain FP64 256^3 solve (timed), 64^3 FP64 with the committed
+    closure and without it (timed), velocity-spectrum errors against the
+    box-filtered fine solve, and the closure evaluation's timing on the
+    context stream (HIP events) against the BF16 matrix-core peak."""
+    from public_amd import Cruna, ic
+    from public_amd import closure_study as CS
+    model = CS.ClosureModel(os.path.join(ROOT, "public_amd", "data", "closure_64.npz"))
+    m, f = model.S, model.factor
+    nf = m * f
+    q0 = ic.bandlimited_box_state(nf, seed, mach)
+    dt_f = CS.fine_dt(q0, nf)
+    kw = CS.physics(mach)
+    with Cruna((nf,) * 3, (ic.periodic_dx(nf),) * 3, dt=dt_f, **kw) as g:
+        g.set_state(q0)
+        qb0 = g.box_filter_state(f, rhs=False)[0]
+        g.run_direct(2)
+        g.synchronize()
+        g.set_state(q0)
+        t0 = time.perf_counter()
+        g.run_direct(coarse_steps * f)
+        g.synchronize()
+        t_fine = time.perf_counter() - t0
+        ref = g.box_filter_state(f, rhs=False)[0]
+    out = {"seed": seed, "mach": mach, "coarse_grid": m, "fine_grid": nf, "coarse_steps": coarse_steps,
+           "fine_steps": coarse_steps * f, "fine_fp64_seconds": round(t_fine, 4)}
+    cl = model.closure()
+    try:
+        for name, net in (("no_closure", None), ("closure", cl)):
+            with Cruna((m,) * 3, (ic.periodic_dx(m),) * 3, dt=dt_f * f, **kw) as g:
+                if net is not None:
+                    g.set_closure(net)
+                g.set_state(qb0)
+                g.run_direct(1)
+                g.synchronize()
+                g.set_state(qb0)
+                t0 = time.perf_counter()
+                g.run_direct(coarse_steps)
+                g.synchronize()
+                secs = time.perf_counter() - t0
+                q = g.get_state()
+                res = {"seconds": round(secs, 4), "ms_per_step": round(secs / coarse_steps * 1e3, 4),
+                       "speedup_vs_fine_fp64": round(t_fine / secs, 2),
+                       "spectrum_error": round(CS.spectrum_error(q, ref), 4),
+                       "finite": bool(np.all(np.isfinite(q)))}
+                if net is not None:
+                    g.set_state(qb0)
+                    g.profile(True)
+                    g.run_direct(10)
+                    g.synchronize()
+                    ms, n = g.profile_read()["closure"]
+                    g.profile(False)
+                    avg = ms / n
+                    achieved = 2.0 * CLOSURE_MACS_PER_POINT * m ** 3 / (avg / 1e3) / 1e12
+                    res["evaluation_ms"] = round(avg, 4)
+                    res["roofline"] = {"bound": "mfma", "achieved": round(achieved, 2),
+                                       "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                       "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4),
+                                       "algorithmic_flops_per_evaluation": 2.0 * CLOSURE_MACS_PER_POINT * m ** 3}
+                out[name] = res
+    finally:
+        cl.close()
+    out["validated_error_at_horizon"] = model.validated_error(coarse_steps)
+    out["closure_used_by_fallback_rule"] = model.usable(coarse_steps)
+    out["gain"] = [float(x) for x in model.gain]
+    return out
+
+
 def ke_history(q0, n, h, dt, precision, steps, every, kw):
     from public_amd import Cruna, ic
     g = Cruna((n,) * 3, (h,) * 3, precision=precision, dt=dt, filter_freq=(1, 1, 1), **kw)
@@ -324,6 +403,8 @@ def main():
                     help="timed steps of the stretched-grid shear layer (512x256x128, FP32-mixed; 0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=100,
                     help="steps of the 32-member forward/adjoint ensemble (0: skip)")
+    ap.add_argument("--closure-steps", type=int, default=300,
+                    help="coarse steps of the 64^3 closure leg against the plain 256^3 solve (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -360,6 +441,7 @@ def main():
                                for name, v in prof_a.items()}}
     shear = run_shear_layer(args.shear_steps, 2) if args.shear_steps > 0 else None
     ensemble = run_ensemble(args.ensemble_steps) if args.ensemble_steps > 0 else None
+    closure = run_closure(args.closure_steps) if args.closure_steps > 0 else None
     pts = float(n) ** 3
     mpts64 = pts * args.steps / secs64 / 1e6
     mpts32 = pts * args.steps / secs32 / 1e6
@@ -437,6 +519,7 @@ def main():
         "adjoint_fp64": adjoint,
         "shear_layer_fp32_mixed": shear,
         "ensemble_fp64": ensemble,
+        "closure_64_vs_256": closure,
         "roofline": roofline,
         "step_roofline": step_roofline,
         "kernels_fp64": kernels,

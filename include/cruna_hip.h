@@ -213,8 +213,9 @@ int cruna_hip_cnn_get_params(cruna_hip_cnn *cnn, float *params);
  * mean(p) / p_ref when p_ref > 0 (its residual scales with the mean
  * pressure, i.e. 1 / Mach^2); defaults 1, 1 and 0 */
 int cruna_hip_cnn_set_norm(cruna_hip_cnn *cnn, const double in_std[5], const double out_std[5], double p_ref);
-/* scale of the closure term at inference (default 1; training unaffected) */
-int cruna_hip_cnn_set_gain(cruna_hip_cnn *cnn, double gain);
+/* scale of the closure term of each variable at inference (default 1;
+ * training is unaffected) */
+int cruna_hip_cnn_set_gain(cruna_hip_cnn *cnn, const double gain[5]);
 /* closure (5 fields, FP64, field-major like q) of the state q */
 int cruna_hip_cnn_forward(cruna_hip_cnn *cnn, const double *q, double *closure);
 /* mean-square loss of the normalised output against target / out_std and

@@ -222,16 +222,33 @@ def coarse_rollout(q0, mach, dt, steps, every, closure=None, precision=FP64):
     return out, secs
 
 
+def gain_label(g):
+    """Result name of a rollout with inference gain g (a value or 5 values)."""
+    g = np.broadcast_to(np.asarray(g, dtype=np.float64), (5,))
+    if np.all(g == 1.0):
+        return "closure"
+    if np.all(g == g[0]):
+        return f"closure_gain_{g[0]:g}"
+    return "closure_gain_" + "_".join(f"{x:g}" for x in g)
+
+
 def evaluate(closure, seed, mach, n_fine=256, factor=4, steps=300, every=50, log=print, precision=FP64,
-             gains=(1.0,)):
+             gains=(1.0,), fine_twin=False):
     """Held-out rollout without the closure and with it at each gain,
     against the filtered fine solve."""
     ref, dt_f, fine_secs = fine_reference(seed, mach, n_fine, factor, steps, every, log=log)
     dt_c = dt_f * factor
     res = {"seed": seed, "mach": mach, "coarse_steps": steps, "dt_coarse": dt_c, "fine_steps": steps * factor,
            "fine_secs": fine_secs, "spectrum_ref": velocity_spectrum(ref[-1]).tolist()}
-    runs = [("no_closure", None, 1.0)] + [("closure" if g == 1.0 else f"closure_gain_{g:g}", closure, g)
-                                            for g in gains if closure is not None]
+    if fine_twin:
+        # the same fine solve in FP32-mixed: its distance from the FP64 solve
+        # is how far two solutions that differ only by rounding drift apart
+        # (the chaotic growth of any difference), a floor for any model
+        twin, _, _ = fine_reference(seed, mach, n_fine, factor, steps, every, precision=FP32_MIXED, log=log)
+        res["fine_fp32_mixed"] = {"spectrum_error": [spectrum_error(q, r) for q, r in zip(twin, ref)],
+                                  "state_error": [state_error(q, r) for q, r in zip(twin[1:], ref[1:])]}
+        log(f"fine FP32-mixed vs FP64: final spectrum error {res['fine_fp32_mixed']['spectrum_error'][-1]:.4f}")
+    runs = [("no_closure", None, 1.0)] + [(gain_label(g), closure, g) for g in gains if closure is not None]
     for name, cl, g in runs:
         if cl is not None:
             cl.set_gain(g)
@@ -264,7 +281,7 @@ class ClosureModel:
         self.factor = int(d["factor"])
         self.params = d["params"].astype(np.float32)
         self.in_std, self.out_std, self.p_ref = d["in_std"], d["out_std"], float(d["p_ref"])
-        self.gain = float(d["gain"])
+        self.gain = np.broadcast_to(np.asarray(d["gain"], dtype=np.float64), (5,)).copy()
         self.val_steps = np.asarray(d["val_steps"], dtype=int)
         self.val_errors = np.asarray(d["val_errors"], dtype=float)
         self.threshold = threshold
@@ -331,7 +348,8 @@ def main():
     ap.add_argument("--factor", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--target", choices=("rhs", "step"), default="step")
-    ap.add_argument("--gains", type=float, nargs="*", default=[0.5, 1.0, 1.5, 2.0])
+    ap.add_argument("--gains", nargs="*", default=["1", "1,1.1,1.1,1.1,1", "1,1.2,1.2,1.2,1", "1,1.3,1.3,1.3,1"],
+                    help="inference gains to validate: a value or 5 comma-separated values each")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
 
@@ -356,19 +374,21 @@ def main():
         log(f"closure training: step {i + k}/{args.iters} loss {np.mean(losses[-k:]):.4f}")
     train_secs = time.time() - t
     # the gain: the best final spectrum error on the validation seed
+    gains = [[float(x) for x in s.split(",")] if "," in s else float(s) for s in args.gains]
     val = evaluate(net, args.val_seed, args.val_mach, args.n_fine, args.factor, args.rollout_steps,
-                   every=args.check_every, log=log, gains=args.gains)
+                   every=args.check_every, log=log, gains=gains)
     best = None
-    for g in args.gains:
-        r = val["closure" if g == 1.0 else f"closure_gain_{g:g}"]
+    for g in gains:
+        r = val[gain_label(g)]
         err = r["spectrum_error"][-1]
         if r["finite"] and np.isfinite(err) and (best is None or err < best[1]):
             best = (g, err, r["spectrum_error"])
     gain, _, val_errors = best if best else (0.0, np.inf, [np.inf] * (args.rollout_steps // args.check_every + 1))
     val_steps = list(range(0, args.rollout_steps + 1, args.check_every))
-    log(f"closure gain {gain:g} (validation seed {args.val_seed})")
+    log(f"closure gain {gain} (validation seed {args.val_seed})")
     evals = [evaluate(net, s, mm, args.n_fine, args.factor, args.rollout_steps, every=args.check_every, log=log,
-                      gains=[gain]) for s, mm in zip(args.holdout, args.holdout_mach)]
+                      gains=[gain], fine_twin=True) for s, mm in zip(args.holdout, args.holdout_mach)]
+    gain = np.broadcast_to(np.asarray(gain, dtype=np.float64), (5,)).tolist()
     save(os.path.join(args.out, "closure_params.npz"), net, stats, args.factor, gain, val_steps, val_errors)
     summary = {"runs": args.runs, "steps": args.steps, "every": args.every, "target": args.target,
                "samples": net.num_samples(), "data_secs": gen_secs, "train_secs": train_secs,

@@ -110,7 +110,7 @@ def lib():
         L.cruna_hip_cnn_get_params.argtypes = [vp, fp]
         L.cruna_hip_cnn_set_norm.argtypes = [vp, dp, dp, C.c_double]
         L.cruna_hip_box_filter_state.argtypes = [vp, ip, dp, dp]
-        L.cruna_hip_cnn_set_gain.argtypes = [vp, C.c_double]
+        L.cruna_hip_cnn_set_gain.argtypes = [vp, dp]
         L.cruna_hip_cnn_forward.argtypes = [vp, dp, dp]
         L.cruna_hip_cnn_gradients.argtypes = [vp, dp, dp, fp, fp]
         L.cruna_hip_cnn_add_sample.argtypes = [vp, dp, dp]
@@ -477,8 +477,10 @@ class Closure:
         _check(lib().cruna_hip_cnn_set_norm(self._h, _dp(a[0]), _dp(a[1]), float(p_ref)))
 
     def set_gain(self, gain):
-        """Scale of the closure term at inference (training is unaffected)."""
-        _check(lib().cruna_hip_cnn_set_gain(self._h, float(gain)))
+        """Scale of the closure term at inference, one value or one per
+        variable (training is unaffected)."""
+        g = np.ascontiguousarray(np.broadcast_to(np.asarray(gain, dtype=np.float64), (5,)))
+        _check(lib().cruna_hip_cnn_set_gain(self._h, _dp(g)))
 
     def forward(self, q):
         q = _f64(q, self.shape)

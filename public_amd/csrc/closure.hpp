@@ -471,7 +471,7 @@ __global__ __launch_bounds__(256) void k_field_means(const double *__restrict__ 
 // equation when p_ref > 0 (its residual scales with the mean pressure).
 struct CnnNorm {
   float inv_std[5], out_std[5], out_inv_std[5];
-  float gain;  // inference only: force = gain x the network's term
+  float gain[5];  // inference only: force_v = gain[v] x the network's term
   double p_ref;
 };
 
@@ -498,7 +498,7 @@ __global__ void k_pack_features(const Q *__restrict__ q, long N, const double *_
   d[1] = uint4{0u, 0u, 0u, 0u};
 }
 
-// Closure term in the solver's layout: force[v][p] = out[p][v] out_std[v] (x pressure scale) x gain
+// Closure term in the solver's layout: force[v][p] = out[p][v] out_std[v] (x pressure scale) x gain[v]
 template <typename T>
 __global__ void k_unpack_closure(const float *__restrict__ out, long N, const CnnNorm *__restrict__ nm,
                                  const double *__restrict__ mean, T *__restrict__ force) {
@@ -508,7 +508,7 @@ __global__ void k_unpack_closure(const float *__restrict__ out, long N, const Cn
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const float s = v == 4 ? nm->out_std[4] * ps : nm->out_std[v];
-    force[v * N + p] = T(out[p * 16 + v] * s * nm->gain);
+    force[v * N + p] = T(out[p * 16 + v] * s * nm->gain[v]);
   }
 }
 

@@ -38,12 +38,14 @@ int fail(int code, const std::string &msg) {
 
 enum KernelId {
   KID_S1X = 0, KID_S2Y, KID_S3Z, KID_S4Z, KID_S5Y, KID_S6X, KID_FX, KID_FY, KID_FZ, KID_DIFF,
-  KID_AX, KID_AY, KID_AZ, KID_COUNT
+  KID_AX, KID_AY, KID_AZ, KID_CLOSURE, KID_COUNT
 };
+// KID_CLOSURE times one whole closure evaluation (field means, features,
+// the four convolutions, the force and its mean projection)
 const char *kKernelNames[KID_COUNT] = {"sweep1_x", "sweep1_y",    "sweep1_z", "sweep2_z",
                                        "sweep2_y", "sweep2_x_rk", "filter_x", "filter_y",
                                        "filter_z", "diff",        "asweep_x_rk", "asweep_y",
-                                       "asweep_z"};
+                                       "asweep_z", "closure"};
 
 template <typename T>
 struct DevOp {
@@ -1571,6 +1573,7 @@ int closure_eval(cruna_hip_cnn *n, hipStream_t s, const Q *q, T *force) {
 
 template <typename T, typename Q>
 int closure_term(cruna_hip_ctx *c, const Q *in) {
+  ProfScope p(c, KID_CLOSURE);
   return closure_eval<T, Q>(c->cnn, c->stream, in, (T *)c->force);
 }
 
@@ -1639,8 +1642,7 @@ int cruna_hip_cnn_create(int S, cruna_hip_cnn **out) {
   if (!rc) rc = cnn_alloc(n->fixed, &n->dn, 1);
   if (!rc) rc = cnn_alloc(n->fixed, &n->fmean, 8 + 5 * MEAN_BLOCKS);  // means, then partial sums
   if (!rc) rc = cnn_reserve(n.get(), n->N);
-  for (int k = 0; k < 5; ++k) n->hn.inv_std[k] = n->hn.out_std[k] = n->hn.out_inv_std[k] = 1.f;
-  n->hn.gain = 1.f;
+  for (int k = 0; k < 5; ++k) n->hn.inv_std[k] = n->hn.out_std[k] = n->hn.out_inv_std[k] = n->hn.gain[k] = 1.f;
   n->hn.p_ref = 0.0;
   if (!rc && (hipMemcpy(n->dn, &n->hn, sizeof n->hn, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemset(n->params, 0, o * sizeof(float)) != hipSuccess ||
@@ -1726,10 +1728,11 @@ int cruna_hip_cnn_set_norm(cruna_hip_cnn *n, const double in_std[5], const doubl
   return CRUNA_HIP_OK;
 }
 
-int cruna_hip_cnn_set_gain(cruna_hip_cnn *n, double gain) {
-  if (!n) return fail(CRUNA_HIP_EINVAL, "null argument");
-  if (!(gain >= 0.0)) return fail(CRUNA_HIP_EINVAL, "gain must be >= 0");
-  n->hn.gain = (float)gain;
+int cruna_hip_cnn_set_gain(cruna_hip_cnn *n, const double gain[5]) {
+  if (!n || !gain) return fail(CRUNA_HIP_EINVAL, "null argument");
+  for (int k = 0; k < 5; ++k)
+    if (!(gain[k] >= 0.0)) return fail(CRUNA_HIP_EINVAL, "gains must be >= 0");
+  for (int k = 0; k < 5; ++k) n->hn.gain[k] = (float)gain[k];
   HIP_TRY(hipStreamSynchronize(n->stream));
   HIP_TRY(hipMemcpy(n->dn, &n->hn, sizeof n->hn, hipMemcpyHostToDevice));
   return CRUNA_HIP_OK;
