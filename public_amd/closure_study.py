@@ -338,10 +338,12 @@ def main():
     ap.add_argument("--every", type=int, default=20)
     ap.add_argument("--iters", type=int, default=16000)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--val-seed", type=int, default=900)
+    ap.add_argument("--lr-decay-at", type=float, default=0.75,
+                    help="fraction of the steps after which the learning rate drops tenfold")
+    ap.add_argument("--val-seed", type=int, default=2000)
     ap.add_argument("--val-mach", type=float, default=0.25)
-    ap.add_argument("--holdout", type=int, nargs="*", default=[1001, 1002])
-    ap.add_argument("--holdout-mach", type=float, nargs="*", default=[0.3, 0.2])
+    ap.add_argument("--holdout", type=int, nargs="*", default=[2001, 2002, 2003])
+    ap.add_argument("--holdout-mach", type=float, nargs="*", default=[0.3, 0.2, 0.15])
     ap.add_argument("--rollout-steps", type=int, default=300)
     ap.add_argument("--check-every", type=int, default=25)
     ap.add_argument("--n-fine", type=int, default=256)
@@ -357,7 +359,7 @@ def main():
         print(msg, flush=True)
 
     rng = np.random.default_rng(args.seed)
-    seeds = list(range(100, 100 + args.runs))
+    seeds = list(range(1000, 1000 + args.runs))
     machs = rng.uniform(0.1, 0.3, size=args.runs).tolist()
     m = args.n_fine // args.factor
     net = Closure(m, seed=args.seed)
@@ -370,7 +372,8 @@ def main():
     chunk = 500
     for i in range(0, args.iters, chunk):
         k = min(chunk, args.iters - i)
-        losses += net.train(k, lr=args.lr, batch=8, crop=32, seed=args.seed + i).tolist()
+        lr = args.lr if i < args.lr_decay_at * args.iters else 0.1 * args.lr
+        losses += net.train(k, lr=lr, batch=8, crop=32, seed=args.seed + i).tolist()
         log(f"closure training: step {i + k}/{args.iters} loss {np.mean(losses[-k:]):.4f}")
     train_secs = time.time() - t
     # the gain: the best final spectrum error on the validation seed

@@ -9,9 +9,10 @@
 //   forward       out[p][co] = b[co] + sum_t sum_ci in[p + o_t][ci] W[t][co][ci]
 //   input grad    din[p][ci] = gelu'(z[p][ci]) sum_t sum_co dz[p - o_t][co] W[t][co][ci]
 //   weight grad   dW[t][co][ci] = sum_p dz[p][co] in[p + o_t][ci]
-// v_mfma_f32_16x16x16_bf16 operand layout (lane l of 64): A row l%16 holds
-// K = 4(l/16)..+3; B column l%16 holds K = 4(l/16)..+3; D row 4(l/16)+r,
-// column l%16, r = 0..3.
+// MFMA operand layouts (lane l of 64; both checked on the device):
+// v_mfma_f32_16x16x16_bf16: A row l%16 holds K = 4(l/16)..+3, B column l%16
+// holds K = 4(l/16)..+3; v_mfma_f32_16x16x32_bf16: the same with
+// K = 8(l/16)..+7; both: D row 4(l/16)+r, column l%16, r = 0..3.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,6 +21,7 @@ namespace cruna {
 
 typedef unsigned short bf16;
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // float -> bf16, round to nearest even (NaN stays NaN)
@@ -129,6 +131,9 @@ __device__ __forceinline__ int pick3(const int (&a)[3], int d) { return d == 0 ?
 // then walks blocks of 256 consecutive points (grid-stride); each wavefront
 // takes 64 points as 4 tiles of 16, reuses every weight fragment for the 4
 // tiles, and finds the 27 neighbours of a point from per-axis offset tables.
+// With 32 input channels a lane loads 16 bytes (channels 8(l/16)..+7) of a
+// neighbour row per tap, the K = 32 operand of one v_mfma_f32_16x16x32_bf16;
+// with 16 it loads 8 bytes for v_mfma_f32_16x16x16_bf16.
 // CK: input (reduction) channels, CN: output channels.
 //   CONV_GELU     : z = conv + bias; zsave <- bf16(z) (if set); out16 <- bf16(gelu(z))
 //   CONV_LINEAR   : out32 <- conv + bias
@@ -204,26 +209,41 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in, const
 #pragma unroll
         for (int dx = 0; dx < 3; ++dx) {
           const int t = 9 * dz + 3 * dy + dx;
-          bf16x4 a[CONV_TILES][CK / 16];
+          int nb[CONV_TILES];
+          bool ok[CONV_TILES];
 #pragma unroll
           for (int j = 0; j < CONV_TILES; ++j) {
-            const int nb = pt[j] + ax[j].off[dx] + pick3(ay[j].off, dy) + pick3(az[j].off, dz);
-            const bool ok = per || ((ax[j].valid >> dx) & (ay[j].valid >> dy) & (az[j].valid >> dz) & 1u);
-#pragma unroll
-            for (int kc = 0; kc < CK / 16; ++kc)
-              a[j][kc] = ok ? *reinterpret_cast<const bf16x4 *>(in + nb * CK + kc * 16 + kg * 4)
-                            : bf16x4{0, 0, 0, 0};
+            nb[j] = pt[j] + ax[j].off[dx] + pick3(ay[j].off, dy) + pick3(az[j].off, dz);
+            ok[j] = per || ((ax[j].valid >> dx) & (ay[j].valid >> dy) & (az[j].valid >> dz) & 1u);
           }
+          if constexpr (CK == 32) {
+            uint4 a[CONV_TILES];
 #pragma unroll
-          for (int n = 0; n < CN / 16; ++n)
+            for (int j = 0; j < CONV_TILES; ++j)
+              a[j] = ok[j] ? *reinterpret_cast<const uint4 *>(in + nb[j] * CK + kg * 8) : uint4{0u, 0u, 0u, 0u};
 #pragma unroll
-            for (int kc = 0; kc < CK / 16; ++kc) {
-              const bf16x4 w =
-                  *reinterpret_cast<const bf16x4 *>(wl + (t * CN + n * 16 + r) * ROW + kc * 16 + kg * 4);
+            for (int n = 0; n < CN / 16; ++n) {
+              const bf16x8 w = __builtin_bit_cast(
+                  bf16x8, *reinterpret_cast<const uint4 *>(wl + (t * CN + n * 16 + r) * ROW + kg * 8));
 #pragma unroll
               for (int j = 0; j < CONV_TILES; ++j)
-                acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[j][kc], w, acc[j][n], 0, 0, 0);
+                acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[j]), w, acc[j][n],
+                                                                    0, 0, 0);
             }
+          } else {
+            static_assert(CK == 16, "16 or 32 input channels");
+            bf16x4 a[CONV_TILES];
+#pragma unroll
+            for (int j = 0; j < CONV_TILES; ++j)
+              a[j] = ok[j] ? *reinterpret_cast<const bf16x4 *>(in + nb[j] * CK + kg * 4) : bf16x4{0, 0, 0, 0};
+#pragma unroll
+            for (int n = 0; n < CN / 16; ++n) {
+              const bf16x4 w = *reinterpret_cast<const bf16x4 *>(wl + (t * CN + n * 16 + r) * ROW + kg * 4);
+#pragma unroll
+              for (int j = 0; j < CONV_TILES; ++j)
+                acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[j], w, acc[j][n], 0, 0, 0);
+            }
+          }
         }
 #pragma unroll
     for (int j = 0; j < CONV_TILES; ++j)
