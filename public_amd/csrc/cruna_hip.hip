@@ -349,15 +349,27 @@ void launch_sweep2_gt(cruna_hip_ctx *c, int mode, bool fuse_filter) {
   T *rhs = (T *)c->rhs, *qs = (T *)c->qs;
   const T *G = (const T *)c->G;
   const LineOp<T> &op = dops<T>(c)[DIR].op, &fop = fops<T>(c)[DIR].op;
-  const int ff = fuse_filter ? 1 : 0;
-  if (c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC)
-    hipLaunchKernelGGL((k_sweep2<T, DIR, true, GT>), grid, block, 0, c->stream, rhs, G, op, fop,
-                       c->geom[DIR], c->phys, c->N, mode, ff, c->qn, c->acc, qs, c->cfg.dt,
-                       metric<T>(c), (const T *)(DIR == 0 && c->cnn ? c->force : nullptr));
-  else
-    hipLaunchKernelGGL((k_sweep2<T, DIR, false, GT>), grid, block, 0, c->stream, rhs, G, op, fop,
-                       c->geom[DIR], c->phys, c->N, mode, ff, c->qn, c->acc, qs, c->cfg.dt,
-                       metric<T>(c), (const T *)(DIR == 0 && c->cnn ? c->force : nullptr));
+  const T *force = (const T *)(DIR == 0 && c->cnn ? c->force : nullptr);
+  const bool per = c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC;
+  // the fused x filter exists on the last stage of the x sweep only
+  if constexpr (DIR == 0) {
+    if (fuse_filter && mode == 3) {
+      if (per)
+        hipLaunchKernelGGL((k_sweep2<T, DIR, true, GT, true>), grid, block, 0, c->stream, rhs, G, op, fop,
+                           c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force);
+      else
+        hipLaunchKernelGGL((k_sweep2<T, DIR, false, GT, true>), grid, block, 0, c->stream, rhs, G, op, fop,
+                           c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force);
+      return;
+    }
+  }
+  if (per) {
+    hipLaunchKernelGGL((k_sweep2<T, DIR, true, GT, false>), grid, block, 0, c->stream, rhs, G, op, fop,
+                       c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force);
+  } else {
+    hipLaunchKernelGGL((k_sweep2<T, DIR, false, GT, false>), grid, block, 0, c->stream, rhs, G, op, fop,
+                       c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force);
+  }
 }
 
 template <typename T, int DIR>

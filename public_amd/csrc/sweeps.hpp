@@ -341,18 +341,20 @@ __device__ __forceinline__ void rk_update(int mode, int v, const T (&k)[PPT], lo
 // S4 (z) / S5 (y) / S6 (x): viscous fluxes d/dx_d tau_{i d}; the x sweep also
 // adds tau:grad(u) (ns_rho_rhou_p.f90:202-205) and finishes the stage.
 // mode 0..3: RK4 stage update; mode 4: write the complete RHS into rhs.
-// With fuse_filter set, the last stage (mode 3) also applies filter_i1 to
+// FF (x sweep of the last stage, mode 3, only): also apply filter_i1 to
 // the new state before storing it (the x filter of the step loop, fused
-// because the x sweep holds complete x-lines); fop is that filter.
+// because the x sweep holds complete x-lines); fop is that filter.  A
+// separate instance, so that the other stages do not carry the filter
+// solve's registers.
 // The viscous sweeps read only G and rhs, so the next stage input (qs_out)
 // may overwrite the current one in place.
 // GT: deformed grid: d/dxi_d (M(d,d) tau_id) = M(d,d) d/dxi_d tau_id, and the
 // last sweep multiplies every equation by X_jacobian and the dissipation once
 // more (ns_rho_rhou_p_grid_transform.f90:190-276).
-template <typename T, int DIR, bool PER, bool GT>
+template <typename T, int DIR, bool PER, bool GT, bool FF>
 __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__restrict__ G,
                                                LineOp<T> op, LineOp<T> fop, SweepGeom g, Phys ph,
-                                               long N, int mode, int fuse_filter,
+                                               long N, int mode,
                                                double *__restrict__ qn, double *__restrict__ acc,
                                                T *__restrict__ qs_out, double dt, Metric<T> mt,
                                                const T *__restrict__ force) {
@@ -404,7 +406,7 @@ __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__r
   // the last sweep's complete RHS of variable v: RK4 update, and on the
   // final stage optionally the x filter of the new state
   auto finish = [&](int v, T(&k)[PPT]) {
-    if (mode == 3 && fuse_filter) {
+    if constexpr (FF) {
       double a[PPT];
       ld<DIR>(acc + v * N, B, PS, a);
       T f[PPT], x[PPT];
@@ -441,6 +443,20 @@ __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__r
     add_force(0, k);
     finish(0, k);
   }
+  if (LAST) {
+    // pressure: (gamma - 1) (bracket + tau:grad(u)) (ns_rho_rhou_p.f90:200-209),
+    // complete before the momentum solves: finishing it here frees the
+    // dissipation's registers for them
+    asm volatile("" ::: "memory");
+    T k[PPT];
+    ld<DIR>(rhs + 4 * N, B, PS, k);
+#pragma unroll
+    for (int m = 0; m < PPT; ++m)
+      k[m] = GT ? (Jv[m] * T(ph.gm1)) * (k[m] + Jv[m] * diss[m]) : T(ph.gm1) * (k[m] + diss[m]);
+    add_force(4, k);
+    finish(4, k);
+    asm volatile("" ::: "memory");
+  }
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     T x[PPT];
@@ -459,17 +475,6 @@ __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__r
       add_force(1 + i, k);
       finish(1 + i, k);
     }
-  }
-  if (LAST) {
-    // pressure: (gamma - 1) (bracket + tau:grad(u)) (ns_rho_rhou_p.f90:200-209)
-    asm volatile("" ::: "memory");
-    T k[PPT];
-    ld<DIR>(rhs + 4 * N, B, PS, k);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m)
-      k[m] = GT ? (Jv[m] * T(ph.gm1)) * (k[m] + Jv[m] * diss[m]) : T(ph.gm1) * (k[m] + diss[m]);
-    add_force(4, k);
-    finish(4, k);
   }
 }
 
