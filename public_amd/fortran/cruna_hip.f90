@@ -168,7 +168,89 @@ module cruna_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: nt_end, nsteps
     end function
+    integer(c_int) function c_set_metric(ctx, xm, xj) bind(C, name="cruna_hip_set_metric")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: xm(*), xj(*)
+    end function
+    integer(c_int) function c_autotune(ctx, steps, tiles) bind(C, name="cruna_hip_autotune")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: steps
+      integer(c_int), intent(out) :: tiles(3)
+    end function
+    integer(c_int) function c_box_filter_state(ctx, f, qb, rb) bind(C, name="cruna_hip_box_filter_state")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, qb, rb
+      integer(c_int), value :: f
+    end function
+    integer(c_int) function c_ensemble_create(cfgs, n, ens) bind(C, name="cruna_hip_ensemble_create")
+      import :: c_int, c_ptr, cruna_hip_cfg
+      type(cruna_hip_cfg), intent(in) :: cfgs(*)
+      integer(c_int), value :: n
+      type(c_ptr), intent(out) :: ens
+    end function
+    integer(c_int) function c_ensemble_destroy(ens) bind(C, name="cruna_hip_ensemble_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+    end function
+    integer(c_int) function c_ensemble_member(ens, i, ctx) bind(C, name="cruna_hip_ensemble_member")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: i
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function c_ensemble_set_role(ens, i, role, nt) bind(C, name="cruna_hip_ensemble_set_role")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: i, role, nt
+    end function
+    integer(c_int) function c_ensemble_run(ens, nsteps) bind(C, name="cruna_hip_ensemble_run")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function c_ensemble_autotune(ens, steps, tiles) bind(C, name="cruna_hip_ensemble_autotune")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens, tiles
+      integer(c_int), value :: steps
+    end function
+    integer(c_int) function c_cnn_create(s, cnn) bind(C, name="cruna_hip_cnn_create")
+      import :: c_int, c_ptr
+      integer(c_int), value :: s
+      type(c_ptr), intent(out) :: cnn
+    end function
+    integer(c_int) function c_cnn_destroy(cnn) bind(C, name="cruna_hip_cnn_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: cnn
+    end function
+    integer(c_long) function c_cnn_num_params(cnn) bind(C, name="cruna_hip_cnn_num_params")
+      import :: c_long, c_ptr
+      type(c_ptr), value :: cnn
+    end function
+    integer(c_int) function c_cnn_set_params(cnn, params) bind(C, name="cruna_hip_cnn_set_params")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: cnn
+      real(c_float), intent(in) :: params(*)
+    end function
+    integer(c_int) function c_cnn_set_norm(cnn, in_std, out_std, p_ref) bind(C, name="cruna_hip_cnn_set_norm")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: cnn
+      real(c_double), intent(in) :: in_std(5), out_std(5)
+      real(c_double), value :: p_ref
+    end function
+    integer(c_int) function c_cnn_set_gain(cnn, gain) bind(C, name="cruna_hip_cnn_set_gain")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: cnn
+      real(c_double), intent(in) :: gain(5)
+    end function
+    integer(c_int) function c_set_closure(ctx, cnn) bind(C, name="cruna_hip_set_closure")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, cnn
+    end function
   end interface
+
+  integer, parameter, public :: CRUNA_HIP_FORWARD = 0, CRUNA_HIP_ADJOINT = 1
 
   public :: cruna_hip_default_cfg
   public :: hip_create, hip_destroy, hip_set_state, hip_get_state, hip_set_qref
@@ -177,6 +259,10 @@ module cruna_hip
   public :: hip_rhs_adjoint, hip_time_stepper_adjoint, hip_set_adjoint_state
   public :: hip_get_adjoint_state, hip_trajectory_alloc, hip_trajectory_set
   public :: hip_trajectory_get, hip_trajectory_record, hip_run_adjoint
+  public :: hip_set_metric, hip_autotune, hip_box_filter_state
+  public :: hip_ensemble_create, hip_ensemble_destroy, hip_ensemble_member
+  public :: hip_ensemble_set_role, hip_ensemble_run, hip_ensemble_autotune
+  public :: hip_closure_create, hip_closure_destroy, hip_set_closure
 
 contains
 
@@ -351,6 +437,103 @@ contains
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: nt_end, nsteps
     call check(c_run_adjoint(ctx, int(nt_end, c_int), int(nsteps, c_int)), "run_adjoint")
+  end subroutine
+
+  ! deformed grid: X_metric(n1,n2,n3,3,3), X_jacobian(n1,n2,n3) of data_geom
+  ! (grid_metric::calc_metric, src/geometry/grid_metric.f90:16-188)
+  subroutine hip_set_metric(ctx, X_metric, X_jacobian)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(in), contiguous :: X_metric(:,:,:,:,:), X_jacobian(:,:,:)
+    call check(c_set_metric(ctx, X_metric, X_jacobian), "set_metric")
+  end subroutine
+
+  ! lines per work-group per direction, tuned on a copy of the state
+  subroutine hip_autotune(ctx, steps, tiles)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: steps
+    integer(c_int), intent(out) :: tiles(3)
+    call check(c_autotune(ctx, int(steps, c_int), tiles), "autotune")
+  end subroutine
+
+  ! f^3 box averages of the device state (and of its direct RHS)
+  subroutine hip_box_filter_state(ctx, f, q_box, rhs_box)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: f
+    real(c_double), intent(out), contiguous, target :: q_box(:,:,:,:)
+    real(c_double), intent(out), contiguous, target, optional :: rhs_box(:,:,:,:)
+    type(c_ptr) :: rb
+    rb = c_null_ptr
+    if (present(rhs_box)) rb = c_loc(rhs_box)
+    call check(c_box_filter_state(ctx, int(f, c_int), c_loc(q_box), rb), "box_filter_state")
+  end subroutine
+
+  subroutine hip_ensemble_create(cfgs, ens)
+    type(cruna_hip_cfg), intent(in) :: cfgs(:)
+    type(c_ptr), intent(out) :: ens
+    call check(c_ensemble_create(cfgs, int(size(cfgs), c_int), ens), "ensemble_create")
+  end subroutine
+
+  subroutine hip_ensemble_destroy(ens)
+    type(c_ptr), intent(inout) :: ens
+    call check(c_ensemble_destroy(ens), "ensemble_destroy")
+    ens = c_null_ptr
+  end subroutine
+
+  ! member i (0-based) as a context for hip_set_state, trajectories, ...
+  subroutine hip_ensemble_member(ens, i, ctx)
+    type(c_ptr), intent(in) :: ens
+    integer, intent(in) :: i
+    type(c_ptr), intent(out) :: ctx
+    call check(c_ensemble_member(ens, int(i, c_int), ctx), "ensemble_member")
+  end subroutine
+
+  subroutine hip_ensemble_set_role(ens, i, role, nt)
+    type(c_ptr), intent(in) :: ens
+    integer, intent(in) :: i, role, nt
+    call check(c_ensemble_set_role(ens, int(i, c_int), int(role, c_int), int(nt, c_int)), "ensemble_set_role")
+  end subroutine
+
+  subroutine hip_ensemble_run(ens, nsteps)
+    type(c_ptr), intent(in) :: ens
+    integer, intent(in) :: nsteps
+    call check(c_ensemble_run(ens, int(nsteps, c_int)), "ensemble_run")
+  end subroutine
+
+  ! tiles(3, members) receives every member's lines per work-group
+  subroutine hip_ensemble_autotune(ens, steps, tiles)
+    type(c_ptr), intent(in) :: ens
+    integer, intent(in) :: steps
+    integer(c_int), intent(out), contiguous, target :: tiles(:,:)
+    call check(c_ensemble_autotune(ens, int(steps, c_int), c_loc(tiles)), "ensemble_autotune")
+  end subroutine
+
+  ! learned closure on the periodic S^3 grid from trained parameters
+  subroutine hip_closure_create(s, params, in_std, out_std, p_ref, gain, cnn)
+    integer, intent(in) :: s
+    real(c_float), intent(in) :: params(:)
+    real(c_double), intent(in) :: in_std(5), out_std(5), p_ref, gain(5)
+    type(c_ptr), intent(out) :: cnn
+    call check(c_cnn_create(int(s, c_int), cnn), "cnn_create")
+    if (size(params, kind=c_long) /= c_cnn_num_params(cnn)) then
+      write(*,*) "error in cruna_hip:closure_create: wrong number of parameters"
+      stop 1
+    end if
+    call check(c_cnn_set_norm(cnn, in_std, out_std, p_ref), "cnn_set_norm")
+    call check(c_cnn_set_params(cnn, params), "cnn_set_params")
+    call check(c_cnn_set_gain(cnn, gain), "cnn_set_gain")
+  end subroutine
+
+  subroutine hip_closure_destroy(cnn)
+    type(c_ptr), intent(inout) :: cnn
+    call check(c_cnn_destroy(cnn), "cnn_destroy")
+    cnn = c_null_ptr
+  end subroutine
+
+  ! the closure becomes the force term of the context's direct RHS
+  ! (apply_force, src/equations/ns_rho_rhou_p.f90:212); c_null_ptr removes it
+  subroutine hip_set_closure(ctx, cnn)
+    type(c_ptr), intent(in) :: ctx, cnn
+    call check(c_set_closure(ctx, cnn), "set_closure")
   end subroutine
 
 end module cruna_hip

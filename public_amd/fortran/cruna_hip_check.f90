@@ -2,10 +2,13 @@
 ! unit test usr/unit_test/diff/computation_direct.f90 (D(0) = 0, D(1) = 0,
 ! D sin(x_d) = cos(x_d), mean-absolute error per direction), followed by a
 ! device-resident run driven from Fortran:
-!   cruna_hip_check N NSTEPS DT MU0 T0 IN.bin OUT.bin
+!   cruna_hip_check N NSTEPS DT MU0 T0 IN.bin OUT.bin [CLOSURE.bin]
 ! reads q(N,N,N,5) (raw little-endian float64, Fortran order) from IN.bin,
 ! runs NSTEPS of the cruna_hpc step-loop body (RK4 -> filters) and writes the
 ! state to OUT.bin.  Without file arguments only the derivative test runs.
+! With CLOSURE.bin (16 float64: in_std(5), out_std(5), p_ref, gain(5); then
+! the float32 parameters) the learned closure on the N^3 grid is attached to
+! the direct RHS first.
 program cruna_hip_check
   use iso_c_binding
   use cruna_hip
@@ -15,7 +18,11 @@ program cruna_hip_check
   integer :: n, nsteps, i, j, k, d, u
   real(c_double) :: dt, mu0, t0, h, pi, err(3)
   real(c_double), allocatable :: y(:,:,:), y_x(:,:,:), x(:), q(:,:,:,:)
-  character(len=256) :: arg, fin, fout
+  character(len=256) :: arg, fin, fout, fcl
+  type(c_ptr) :: cnn
+  real(c_double) :: hdr(16)
+  real(c_float), allocatable :: params(:)
+  integer :: fsize
 
   pi = 4.0_c_double * atan(1.0_c_double)
   n = 32
@@ -25,6 +32,8 @@ program cruna_hip_check
   t0 = 1.0_c_double
   fin = ''
   fout = ''
+  fcl = ''
+  cnn = c_null_ptr
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg); read(arg, *) n
   end if
@@ -36,6 +45,7 @@ program cruna_hip_check
     call get_command_argument(6, fin)
     call get_command_argument(7, fout)
   end if
+  if (command_argument_count() >= 8) call get_command_argument(8, fcl)
 
   h = 2.0_c_double * pi / real(n, c_double)
   call cruna_hip_default_cfg(cfg)
@@ -99,6 +109,16 @@ program cruna_hip_check
     open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
     read(u) q
     close(u)
+    if (len_trim(fcl) > 0) then
+      open(newunit=u, file=trim(fcl), access='stream', form='unformatted', status='old')
+      inquire(unit=u, size=fsize)
+      allocate(params((fsize - 16 * 8) / 4))
+      read(u) hdr, params
+      close(u)
+      call hip_closure_create(n, params, hdr(1:5), hdr(6:10), hdr(11), hdr(12:16), cnn)
+      call hip_set_closure(ctx, cnn)
+      write(*, '(a, i0, a)') " closure: ", size(params), " parameters"
+    end if
     call hip_set_state(ctx, q)
     call hip_run_direct(ctx, 0, nsteps)
     call hip_synchronize(ctx)
@@ -109,4 +129,5 @@ program cruna_hip_check
     write(*, '(a, i0, a)') " run_direct: ", nsteps, " steps written"
   end if
   call hip_destroy(ctx)
+  if (c_associated(cnn)) call hip_closure_destroy(cnn)
 end program cruna_hip_check

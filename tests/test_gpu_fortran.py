@@ -47,3 +47,47 @@ def test_fortran_host_diff_and_run(tmp_path):
     ref = O.run_direct(O.Cfg((n,) * 3, (h,) * 3, mu0=mu0, T0=T0, dt=dt, filter_freq=(1, 1, 1)), q0, nsteps)
     err = np.linalg.norm(q1 - ref) / np.linalg.norm(ref)
     assert err < 1e-12, err
+
+
+def test_fortran_host_with_closure(tmp_path):
+    """The closure attached from Fortran (hip_closure_create +
+    hip_set_closure) gives the same device-resident run as the same closure
+    attached from Python, bit for bit (one library, one code path)."""
+    if not os.path.exists(EXE):
+        pytest.fail("public_amd/fortran/cruna_hip_check is not built (run __graft_entry__.build())")
+    from public_amd import Closure, Cruna, ic
+    n, nsteps, mach = 32, 4, 0.3
+    h = ic.periodic_dx(n)
+    q0 = ic.periodic_box_state(n, 5, mach)
+    dt = ic.cfl_dt(q0, h)
+    mu0, T0 = 1.0 / 1600.0, 1.0 / (1.4 * mach * mach)
+    in_std = np.array([0.03, 1.0, 1.0, 1.0, 1.1])
+    out_std = np.array([0.05, 0.6, 0.6, 0.6, 1.4])
+    p_ref, gain = 8.0, np.array([1.0, 1.2, 1.2, 1.2, 1.0])
+    net = Closure(n, seed=11)
+    params = net.get_params() * 3.0  # a closure strong enough to change the run
+    net.set_params(params)
+    net.set_norm(in_std, out_std, p_ref)
+    net.set_gain(gain)
+    with Cruna((n,) * 3, (h,) * 3, mu0=mu0, T0=T0, dt=dt, filter_freq=(1, 1, 1)) as g:
+        g.set_closure(net)
+        g.set_state(q0)
+        g.run_direct(nsteps)
+        ref = g.get_state()
+        g.set_closure(None)
+        g.set_state(q0)
+        g.run_direct(nsteps)
+        plain = g.get_state()
+    net.close()
+    assert np.linalg.norm(ref - plain) > 1e-6 * np.linalg.norm(plain)
+    fin, fout, fcl = tmp_path / "q0.bin", tmp_path / "q1.bin", tmp_path / "closure.bin"
+    np.asfortranarray(q0).ravel(order="F").astype("<f8").tofile(fin)
+    with open(fcl, "wb") as f:
+        np.concatenate([in_std, out_std, [p_ref], gain]).astype("<f8").tofile(f)
+        params.astype("<f4").tofile(f)
+    r = subprocess.run([EXE, str(n), str(nsteps), repr(dt), repr(mu0), repr(T0), str(fin), str(fout), str(fcl)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "closure:" in r.stdout
+    q1 = np.fromfile(fout, dtype="<f8").reshape((n, n, n, 5), order="F")
+    assert np.array_equal(q1, ref)
