@@ -172,7 +172,7 @@ int cruna_hip_autotune(cruna_hip_ctx *ctx, int steps, int tile_lines[3]);
  * Forward members run calc_direct's step body from their step counter;
  * adjoint members run calc_adjoint's backward from step nt over their
  * trajectory.  Members must not be used on their own while a run is in
- * progress. */
+ * progress, and cannot carry a learned closure (cruna_hip_set_closure). */
 #define CRUNA_HIP_FORWARD 0
 #define CRUNA_HIP_ADJOINT 1
 typedef struct cruna_hip_ensemble cruna_hip_ensemble;

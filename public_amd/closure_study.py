@@ -100,15 +100,15 @@ def state_error(q, ref):
 # data
 
 def generate(closure, seeds, machs, n_fine=256, factor=4, steps=200, every=20, norm_runs=4, target="rhs",
-             log=print):
+             target_steps=1, log=print):
     """Training samples of seeded fine runs into `closure` (the first
     `norm_runs` runs also fix its normalisation).  Returns the statistics.
 
     target "rhs": the subfilter residual of the RHS,
         box(RHS_fine(q_f)) - RHS_coarse(box(q_f));
-    target "step": the subfilter residual of one coarse step (RK4 and
-    filter) per unit time,
-        (box(q_f(t + dt_c)) - Step_coarse(box(q_f(t)))) / dt_c,
+    target "step": the subfilter residual of `target_steps` coarse steps
+    (RK4 and filter) per unit time,
+        (box(q_f(t + K dt_c)) - Step_coarse^K(box(q_f(t)))) / (K dt_c),
     i.e. the RHS residual of the time-discrete coarse dynamics, which also
     carries what the coarse grid's filter and step size take away."""
     m = n_fine // factor
@@ -136,15 +136,16 @@ def generate(closure, seeds, machs, n_fine=256, factor=4, steps=200, every=20, n
                         qb, rb = fine.box_filter_state(factor)
                         t = np.asfortranarray(rb - coarse.right_hand_side_direct_3d(qb))
                     else:
-                        if s + factor > steps:
+                        K = target_steps
+                        if s + K * factor > steps:
                             break
                         qb, _ = fine.box_filter_state(factor, rhs=False)
-                        fine.run_direct(factor, nt0=done)
-                        done += factor
+                        fine.run_direct(K * factor, nt0=done)
+                        done += K * factor
                         qb1, _ = fine.box_filter_state(factor, rhs=False)
                         coarse.set_state(qb)
-                        coarse.run_direct(1, nt0=0)
-                        t = np.asfortranarray((qb1 - coarse.get_state()) / (dt * factor))
+                        coarse.run_direct(K, nt0=0)
+                        t = np.asfortranarray((qb1 - coarse.get_state()) / (K * dt * factor))
                     pending.append((qb, t))
             if stats is None:
                 held += pending
@@ -350,6 +351,7 @@ def main():
     ap.add_argument("--factor", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--target", choices=("rhs", "step"), default="step")
+    ap.add_argument("--target-steps", type=int, default=1, help="coarse steps of a 'step' target")
     ap.add_argument("--gains", nargs="*", default=["1", "1,1.1,1.1,1.1,1", "1,1.2,1.2,1.2,1", "1,1.3,1.3,1.3,1"],
                     help="inference gains to validate: a value or 5 comma-separated values each")
     args = ap.parse_args()
@@ -365,7 +367,7 @@ def main():
     net = Closure(m, seed=args.seed)
     t = time.time()
     stats = generate(net, seeds, machs, args.n_fine, args.factor, args.steps, args.every, target=args.target,
-                     log=log)
+                     target_steps=args.target_steps, log=log)
     gen_secs = time.time() - t
     t = time.time()
     losses = []
@@ -394,6 +396,7 @@ def main():
     gain = np.broadcast_to(np.asarray(gain, dtype=np.float64), (5,)).tolist()
     save(os.path.join(args.out, "closure_params.npz"), net, stats, args.factor, gain, val_steps, val_errors)
     summary = {"runs": args.runs, "steps": args.steps, "every": args.every, "target": args.target,
+               "target_steps": args.target_steps,
                "samples": net.num_samples(), "data_secs": gen_secs, "train_secs": train_secs,
                "iters": args.iters, "loss_first": float(np.mean(losses[:50])), "loss_last": float(np.mean(losses[-50:])),
                "norm": stats, "gain": gain, "val_steps": val_steps, "validation": val, "holdout": evals}

@@ -1305,6 +1305,8 @@ int cruna_hip_ensemble_run(cruna_hip_ensemble *e, int nsteps) {
     P = lcm_i(P, member_period(e->m[i], e->role[i]));
     if (e->role[i] == CRUNA_HIP_ADJOINT && e->m[i]->traj_slots > 1 && e->nt[i] - nsteps < 0)
       return fail(CRUNA_HIP_EINVAL, "an adjoint member would run past its first step");
+    // a closure's work buffers can move when it trains, under a recorded graph
+    if (e->m[i]->cnn) return fail(CRUNA_HIP_EINVAL, "ensemble members cannot carry a learned closure");
   }
   if (P > 64) return fail(CRUNA_HIP_EINVAL, "filter frequencies repeat only after more than 64 steps");
   // adjoint step counters live on the device; set them for this run
