@@ -112,11 +112,14 @@ __device__ __forceinline__ void line_sync(bool wave_line) {
   }
 }
 
-// LDS scratch of one work-group for the line engine, for up to NTM threads.
+// LDS scratch of one work-group for the line engine, for up to NTM threads,
+// indexed [value][thread slot]: consecutive lanes touch consecutive words
+// (a thread-major [slot][value] layout put lanes 6 or 2 values apart, which
+// the bank-conflict counters showed stalling every exchange).
 template <typename T, int NTM = NT>
 struct LineLds {
-  T halo[NTM * 2 * HALO];
-  T z[2][NTM * 2];
+  T halo[2 * HALO][NTM];
+  T z[2][2][NTM];
 };
 
 // x = B^-1 A f on the thread's block (unit spacing; the caller scales by 1/h).
@@ -146,8 +149,8 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T, NTM> 
   // 1. halo exchange
 #pragma unroll
   for (int m = 0; m < HALO; ++m) {
-    s.halo[tid * 2 * HALO + m] = f[m];
-    s.halo[tid * 2 * HALO + HALO + m] = f[PPT - HALO + m];
+    s.halo[m][tid] = f[m];
+    s.halo[HALO + m][tid] = f[PPT - HALO + m];
   }
   line_sync(tp.wave_line);
   const bool hasL = PER || b > 0;
@@ -159,8 +162,8 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T, NTM> 
   T fe[FE];
 #pragma unroll
   for (int m = 0; m < HALO; ++m) {
-    fe[m] = hasL ? s.halo[sl * 2 * HALO + HALO + m] : T(0);
-    fe[HALO + PPT + m] = hasR ? s.halo[sr * 2 * HALO + m] : T(0);
+    fe[m] = hasL ? s.halo[HALO + m][sl] : T(0);
+    fe[HALO + PPT + m] = hasR ? s.halo[m][sr] : T(0);
   }
 #pragma unroll
   for (int m = 0; m < PPT; ++m) fe[HALO + m] = f[m];
@@ -202,8 +205,8 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T, NTM> 
   // 3. reduced 2x2 block system by cyclic reduction over blocks
   T z0 = y[0], z1 = y[PPT - 1];
   int cur = 0;
-  s.z[0][tid * 2] = z0;
-  s.z[0][tid * 2 + 1] = z1;
+  s.z[0][0][tid] = z0;
+  s.z[0][1][tid] = z1;
   for (int k = 0; k < op.K; ++k) {
     line_sync(tp.wave_line);
     const int st = 1 << k;
@@ -222,8 +225,8 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T, NTM> 
     }
     const int sm = tp.l * tp.LS + bm * tp.BS;
     const int sp = tp.l * tp.LS + bp * tp.BS;
-    const T m0 = hm ? s.z[cur][sm * 2] : T(0), m1 = hm ? s.z[cur][sm * 2 + 1] : T(0);
-    const T p0 = hp ? s.z[cur][sp * 2] : T(0), p1 = hp ? s.z[cur][sp * 2 + 1] : T(0);
+    const T m0 = hm ? s.z[cur][0][sm] : T(0), m1 = hm ? s.z[cur][1][sm] : T(0);
+    const T p0 = hp ? s.z[cur][0][sp] : T(0), p1 = hp ? s.z[cur][1][sp] : T(0);
     T w[12];
     if (PER) {
       const cptr<T> opW = launder_uniform(op.W);
@@ -238,12 +241,12 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T, NTM> 
     z0 = n0;
     z1 = n1;
     cur ^= 1;
-    s.z[cur][tid * 2] = z0;
-    s.z[cur][tid * 2 + 1] = z1;
+    s.z[cur][0][tid] = z0;
+    s.z[cur][1][tid] = z1;
   }
   line_sync(tp.wave_line);
-  const T xl = hasL ? s.z[cur][sl * 2 + 1] : T(0);
-  const T xf = hasR ? s.z[cur][sr * 2] : T(0);
+  const T xl = hasL ? s.z[cur][1][sl] : T(0);
+  const T xf = hasR ? s.z[cur][0][sr] : T(0);
   if (PER || ty == 1) {
     const cptr<T> opL = launder_uniform(op.L);
     const cptr<T> opR = launder_uniform(op.R);
