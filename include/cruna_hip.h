@@ -145,6 +145,26 @@ int cruna_hip_trajectory_set(cruna_hip_ctx *ctx, int nt, const double *q);
 int cruna_hip_trajectory_get(cruna_hip_ctx *ctx, int nt, double *q);
 int cruna_hip_trajectory_record(cruna_hip_ctx *ctx, int on);
 
+/* Subsets with checkpoints (time.subsets of calc_direct / calc_adjoint,
+ * src/main/computation_direct.f90:55-133, computation_adjoint.f90:45-135):
+ * the reference keeps each subset's forward trajectory Q(:,:,:,:,1..steps)
+ * in a cache ('data_direct_subsets_cache') and loads it subset by subset,
+ * last first, for the adjoint.  Here only the state at the start of every
+ * subset is kept on the device (a checkpoint), and the adjoint recomputes
+ * a subset's trajectory from its checkpoint into `steps` trajectory slots
+ * right before running backward over it: subsets + 1 + steps states in HBM
+ * instead of subsets * steps.
+ * alloc: checkpoints for `subsets` and a trajectory of `steps` slots.
+ * run_direct_subsets: for every subset, checkpoint q^n, then the step-loop
+ * body for nt = 1..steps (filters by mod(nt, freq) within the subset, as
+ * the reference counts nt); the final state is kept too.
+ * run_adjoint_subsets: for s = subsets..1: recompute subset s, then the
+ * adjoint loop nt = steps..1 over it; q^n is the final forward state again
+ * afterwards. */
+int cruna_hip_subsets_alloc(cruna_hip_ctx *ctx, int subsets, int steps);
+int cruna_hip_run_direct_subsets(cruna_hip_ctx *ctx, int subsets, int steps);
+int cruna_hip_run_adjoint_subsets(cruna_hip_ctx *ctx, int subsets, int steps);
+
 /* Adjoint step loop of calc_adjoint (src/main/computation_adjoint.f90:62-99)
  * with g = 0: for nt = nt_end down to nt_end-nsteps+1, with q0 = Q(nt)
  * (or the only slot when one is allocated: the linear adjoint):

@@ -111,6 +111,9 @@ def lib():
         L.cruna_hip_cnn_set_norm.argtypes = [vp, dp, dp, C.c_double]
         L.cruna_hip_box_filter_state.argtypes = [vp, ip, dp, dp]
         L.cruna_hip_cnn_set_gain.argtypes = [vp, dp]
+        L.cruna_hip_subsets_alloc.argtypes = [vp, ip, ip]
+        L.cruna_hip_run_direct_subsets.argtypes = [vp, ip, ip]
+        L.cruna_hip_run_adjoint_subsets.argtypes = [vp, ip, ip]
         L.cruna_hip_cnn_forward.argtypes = [vp, dp, dp]
         L.cruna_hip_cnn_gradients.argtypes = [vp, dp, dp, fp, fp]
         L.cruna_hip_cnn_add_sample.argtypes = [vp, dp, dp]
@@ -125,7 +128,8 @@ def lib():
                   "ensemble_create", "ensemble_destroy", "ensemble_member", "ensemble_set_role",
                   "ensemble_run", "ensemble_autotune", "cnn_create", "cnn_destroy", "cnn_init_params",
                   "cnn_set_params", "cnn_get_params", "cnn_set_norm", "cnn_forward", "cnn_gradients",
-                  "cnn_add_sample", "cnn_train", "set_closure", "box_filter_state", "cnn_set_gain"):
+                  "cnn_add_sample", "cnn_train", "set_closure", "box_filter_state", "cnn_set_gain", "subsets_alloc", "run_direct_subsets",
+                  "run_adjoint_subsets"):
             getattr(L, "cruna_hip_" + f).restype = C.c_int
         _lib = L
     return _lib
@@ -349,6 +353,18 @@ class Cruna:
         rb = np.zeros(shape, order="F") if rhs else None
         _check(lib().cruna_hip_box_filter_state(self._h, int(f), _dp(qb), None if rb is None else _dp(rb)))
         return qb, rb
+
+    def subsets_alloc(self, subsets, steps):
+        """Checkpoints for `subsets` subsets and a `steps`-slot trajectory."""
+        _check(lib().cruna_hip_subsets_alloc(self._h, int(subsets), int(steps)))
+
+    def run_direct_subsets(self, subsets, steps):
+        """calc_direct over subsets (nt = 1..steps in each), checkpointing every subset start."""
+        _check(lib().cruna_hip_run_direct_subsets(self._h, int(subsets), int(steps)))
+
+    def run_adjoint_subsets(self, subsets, steps):
+        """calc_adjoint over subsets, last first, recomputing each subset's trajectory."""
+        _check(lib().cruna_hip_run_adjoint_subsets(self._h, int(subsets), int(steps)))
 
     def set_closure(self, closure):
         """Add a learned closure (a `Closure` of this grid) to the direct RHS; None removes it."""

@@ -113,6 +113,9 @@ struct cruna_hip_ctx {
   int *ntp = nullptr;
   int traj_slots = 0;
   bool record = false;
+  // subset checkpoints: the state at the start of every subset and the final one
+  double *ckpt = nullptr;
+  int ckpt_n = 0;
   void *qs = nullptr, *rhs = nullptr, *G = nullptr;  // compute precision
   // learned closure added to the direct RHS (cruna_hip_set_closure)
   cruna_hip_cnn *cnn = nullptr;
@@ -781,6 +784,7 @@ int cruna_hip_destroy(cruna_hip_ctx *c) {
   }
   for (void *p : {(void *)c->qn, (void *)c->acc, (void *)c->qref, (void *)c->shape, (void *)c->tmp,
                   (void *)c->qa, (void *)c->traj, (void *)c->gobj, c->met, (void *)c->q0p, c->force,
+                  (void *)c->ckpt,
                   c->qs, c->rhs, c->G})
     if (p) (void)hipFree(p);
   for (auto &p : c->pending) {
@@ -1082,6 +1086,71 @@ int cruna_hip_run_direct(cruna_hip_ctx *c, int nt0, int nsteps) {
                              hipMemcpyDeviceToDevice, c->stream));
   }
   return CRUNA_HIP_OK;
+}
+
+int cruna_hip_subsets_alloc(cruna_hip_ctx *c, int subsets, int steps) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (subsets < 0 || steps < 0) return fail(CRUNA_HIP_EINVAL, "subsets and steps must be >= 0");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->ckpt) HIP_TRY(hipFree(c->ckpt));
+  c->ckpt = nullptr;
+  c->ckpt_n = 0;
+  if (subsets == 0) return cruna_hip_trajectory_alloc(c, 0);
+  if (hipMalloc(&c->ckpt, (size_t)(subsets + 1) * 5 * c->N * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    c->ckpt = nullptr;
+    return fail(CRUNA_HIP_ENOMEM, "subset checkpoints do not fit in device memory");
+  }
+  c->ckpt_n = subsets + 1;
+  return cruna_hip_trajectory_alloc(c, steps);
+}
+
+int cruna_hip_run_direct_subsets(cruna_hip_ctx *c, int subsets, int steps) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (subsets < 1 || steps < 0) return fail(CRUNA_HIP_EINVAL, "subsets >= 1 and steps >= 0");
+  if (c->ckpt_n < subsets + 1) return fail(CRUNA_HIP_EINVAL, "not enough subset checkpoints (cruna_hip_subsets_alloc)");
+  const size_t bytes = 5 * (size_t)c->N * sizeof(double);
+  const bool record = c->record;
+  c->record = false;
+  int rc = CRUNA_HIP_OK;
+  for (int s = 0; s < subsets && !rc; ++s) {
+    if (hipMemcpyAsync(c->ckpt + (size_t)s * 5 * c->N, c->qn, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+      rc = fail(CRUNA_HIP_EDEVICE, "checkpoint copy failed");
+    if (!rc) rc = cruna_hip_run_direct(c, 0, steps);  // nt = 1..steps within the subset
+  }
+  if (!rc && hipMemcpyAsync(c->ckpt + (size_t)subsets * 5 * c->N, c->qn, bytes, hipMemcpyDeviceToDevice,
+                            c->stream) != hipSuccess)
+    rc = fail(CRUNA_HIP_EDEVICE, "checkpoint copy failed");
+  c->record = record;
+  return rc;
+}
+
+int cruna_hip_run_adjoint_subsets(cruna_hip_ctx *c, int subsets, int steps) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (subsets < 1 || steps < 1) return fail(CRUNA_HIP_EINVAL, "subsets and steps must be >= 1");
+  if (c->ckpt_n < subsets + 1) return fail(CRUNA_HIP_EINVAL, "not enough subset checkpoints (cruna_hip_subsets_alloc)");
+  if (c->traj_slots < steps) return fail(CRUNA_HIP_EINVAL, "the trajectory holds fewer slots than steps");
+  if (!c->qa) return fail(CRUNA_HIP_EINVAL, "no adjoint state (cruna_hip_set_adjoint_state)");
+  const size_t bytes = 5 * (size_t)c->N * sizeof(double);
+  const bool record = c->record;
+  int rc = CRUNA_HIP_OK;
+  for (int s = subsets - 1; s >= 0 && !rc; --s) {
+    // recompute the subset's forward trajectory from its checkpoint
+    if (hipMemcpyAsync(c->qn, c->ckpt + (size_t)s * 5 * c->N, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+      rc = fail(CRUNA_HIP_EDEVICE, "checkpoint copy failed");
+    c->qs_fresh = false;
+    c->record = true;
+    if (!rc) rc = cruna_hip_run_direct(c, 0, steps);
+    c->record = false;
+    if (!rc) rc = cruna_hip_run_adjoint(c, steps, steps);
+  }
+  // the forward state is the final one again
+  if (!rc && hipMemcpyAsync(c->qn, c->ckpt + (size_t)subsets * 5 * c->N, bytes, hipMemcpyDeviceToDevice,
+                            c->stream) != hipSuccess)
+    rc = fail(CRUNA_HIP_EDEVICE, "checkpoint copy failed");
+  c->qs_fresh = false;
+  c->record = record;
+  return rc;
 }
 
 int cruna_hip_autotune(cruna_hip_ctx *c, int steps, int tile_lines[3]) {

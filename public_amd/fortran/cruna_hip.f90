@@ -168,6 +168,21 @@ module cruna_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: nt_end, nsteps
     end function
+    integer(c_int) function c_subsets_alloc(ctx, subsets, steps) bind(C, name="cruna_hip_subsets_alloc")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: subsets, steps
+    end function
+    integer(c_int) function c_run_direct_subsets(ctx, subsets, steps) bind(C, name="cruna_hip_run_direct_subsets")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: subsets, steps
+    end function
+    integer(c_int) function c_run_adjoint_subsets(ctx, subsets, steps) bind(C, name="cruna_hip_run_adjoint_subsets")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: subsets, steps
+    end function
     integer(c_int) function c_set_metric(ctx, xm, xj) bind(C, name="cruna_hip_set_metric")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
@@ -260,6 +275,7 @@ module cruna_hip
   public :: hip_get_adjoint_state, hip_trajectory_alloc, hip_trajectory_set
   public :: hip_trajectory_get, hip_trajectory_record, hip_run_adjoint
   public :: hip_set_metric, hip_autotune, hip_box_filter_state
+  public :: hip_subsets_alloc, hip_run_direct_subsets, hip_run_adjoint_subsets
   public :: hip_ensemble_create, hip_ensemble_destroy, hip_ensemble_member
   public :: hip_ensemble_set_role, hip_ensemble_run, hip_ensemble_autotune
   public :: hip_closure_create, hip_closure_destroy, hip_set_closure
@@ -437,6 +453,26 @@ contains
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: nt_end, nsteps
     call check(c_run_adjoint(ctx, int(nt_end, c_int), int(nsteps, c_int)), "run_adjoint")
+  end subroutine
+
+  ! time.subsets with device checkpoints (computation_direct.f90:55-133,
+  ! computation_adjoint.f90:45-135): checkpoints instead of the subset cache
+  subroutine hip_subsets_alloc(ctx, subsets, steps)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: subsets, steps
+    call check(c_subsets_alloc(ctx, int(subsets, c_int), int(steps, c_int)), "subsets_alloc")
+  end subroutine
+
+  subroutine hip_run_direct_subsets(ctx, subsets, steps)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: subsets, steps
+    call check(c_run_direct_subsets(ctx, int(subsets, c_int), int(steps, c_int)), "run_direct_subsets")
+  end subroutine
+
+  subroutine hip_run_adjoint_subsets(ctx, subsets, steps)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: subsets, steps
+    call check(c_run_adjoint_subsets(ctx, int(subsets, c_int), int(steps, c_int)), "run_adjoint_subsets")
   end subroutine
 
   ! deformed grid: X_metric(n1,n2,n3,3,3), X_jacobian(n1,n2,n3) of data_geom

@@ -506,3 +506,45 @@ def test_set_metric_rejects_general_curvilinear():
             Xm2, _ = O.calc_metric(O.Cfg(n, dx, **kw), X, Xc)
             g.set_metric(Xm2, Xj)
             g.time_stepper_adjoint(np.zeros(n + (5,)), _gt_state(X, 0))
+
+
+def test_adjoint_subsets_with_checkpoints_matches_oracle():
+    """time.subsets: the forward run checkpoints every subset start; the
+    adjoint recomputes each subset's trajectory from its checkpoint and runs
+    backward over it, last subset first; nt counts 1..steps within a subset
+    (filter and adjoint-filter frequencies 2 with 3 steps per subset make
+    that numbering matter).  Against the oracle's subset loops
+    (computation_direct.f90:55-133, computation_adjoint.f90:45-135)."""
+    Cruna = _gpu()
+    n, subsets, steps = 32, 3, 3
+    kw, h = _kw(0.3, n)
+    q0 = _box(n, 5, 0.3)
+    ff = (2, 1, 2)
+    cfg = O.Cfg((n,) * 3, (h,) * 3, filter_freq=ff, afilter_freq=ff, **kw)
+    Qs, q = [], q0
+    for s in range(subsets):
+        Q = np.zeros((n,) * 3 + (5, steps), order="F")
+        for nt in range(1, steps + 1):
+            q = O.run_direct(cfg, q, 1, nt0=nt - 1)
+            Q[..., nt - 1] = q
+        Qs.append(Q)
+    q_final = q
+    qs_T = np.asfortranarray(np.random.default_rng(9).standard_normal((n,) * 3 + (5,)))
+    ref = qs_T
+    for s in reversed(range(subsets)):
+        ref = O.run_adjoint(cfg, ref, Qs[s])
+    with Cruna((n,) * 3, (h,) * 3, filter_freq=ff, afilter_freq=ff, **kw) as g:
+        g.subsets_alloc(subsets, steps)
+        g.set_state(q0)
+        g.run_direct_subsets(subsets, steps)
+        g.synchronize()
+        fwd = g.get_state()
+        g.set_adjoint_state(qs_T)
+        g.run_adjoint_subsets(subsets, steps)
+        g.synchronize()
+        got = g.get_adjoint_state()
+        back = g.get_state()
+    assert rel(fwd, q_final) < 1e-12, rel(fwd, q_final)
+    assert np.array_equal(back, fwd)  # the final forward state is restored
+    for v in range(5):
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
