@@ -38,6 +38,9 @@ extern "C" {
 
 #define CRUNA_HIP_MAX_BC 32
 
+#define CRUNA_HIP_DIFF_O6_COMPACT 0  /* src/diff/diff_pen_core_o6_i_c_{p,o}.f90 */
+#define CRUNA_HIP_DIFF_O4_EXPLICIT 1 /* src/diff/diff_pen_core_o4_e_c_{p,o}.f90 */
+
 typedef struct cruna_hip_cfg {
   int n[3];              /* params%geom%n1..n3 (block = whole grid on one GPU) */
   double dx[3];          /* params%geom%dx1..dx3 */
@@ -62,6 +65,10 @@ typedef struct cruna_hip_cfg {
   /* boundary_conditions.dat rows: type; i1s; i1e; i2s; i2e; i3s; i3e; k1; k2; k3 */
   int nbc;
   double bc[CRUNA_HIP_MAX_BC][10];
+  int diff_kind[3];      /* derivative core per direction (discretisation_x_core):
+                            CRUNA_HIP_DIFF_O6_COMPACT (diff_pen_core_o6_i_c_*, default) or
+                            CRUNA_HIP_DIFF_O4_EXPLICIT (diff_pen_core_o4_e_c_*); the
+                            filter is the o6 compact one either way */
 } cruna_hip_cfg;
 
 typedef struct cruna_hip_ctx cruna_hip_ctx;

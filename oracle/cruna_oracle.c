@@ -266,6 +266,31 @@ void orc_diff_core(int scheme, int n, int nlines, double *q, double dx) {
     diff_open(n, nlines, q, dx);
 }
 
+/* 4th-order explicit cores: src/diff/diff_pen_core_o4_e_c_p.f90:53-82 and
+ * diff_pen_core_o4_e_c_o.f90:53-82 (8 (f_{i+1} - f_{i-1}) - (f_{i+2} - f_{i-2}),
+ * one-sided 5-point rows 1, 2, n-1, n on open lines; then / (12 dx)). */
+void orc_diff_core_o4(int scheme, int n, int nlines, double *q, double dx) {
+  double *t = xalloc(n);
+  for (int l = 0; l < nlines; ++l) {
+    double *x = q + (size_t)l * n;
+    for (int i = 0; i < n; ++i) {
+      if (scheme == ORC_PERIODIC || (i >= 2 && i <= n - 3)) {
+        const int ip1 = (i + 1) % n, ip2 = (i + 2) % n;
+        const int im1 = (i - 1 + n) % n, im2 = (i - 2 + 2 * n) % n;
+        t[i] = 8.0 * (x[ip1] - x[im1]) - 1.0 * (x[ip2] - x[im2]);
+      }
+    }
+    if (scheme != ORC_PERIODIC) {
+      t[0] = -25.0 * x[0] + 48.0 * x[1] - 36.0 * x[2] + 16.0 * x[3] - 3.0 * x[4];
+      t[1] = -3.0 * x[0] - 10.0 * x[1] + 18.0 * x[2] - 6.0 * x[3] + 1.0 * x[4];
+      t[n - 2] = -1.0 * x[n - 5] + 6.0 * x[n - 4] - 18.0 * x[n - 3] + 10.0 * x[n - 2] + 3.0 * x[n - 1];
+      t[n - 1] = 3.0 * x[n - 5] - 16.0 * x[n - 4] + 36.0 * x[n - 3] - 48.0 * x[n - 2] + 25.0 * x[n - 1];
+    }
+    for (int i = 0; i < n; ++i) x[i] = t[i] / (dx * 12.0);
+  }
+  free(t);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Filter cores.                                                            */
 
@@ -408,7 +433,10 @@ void orc_diff3d(const orc_cfg *c, int dir, double *out, const double *in, double
   const size_t N = npts(c);
   double *pen = xalloc(N);
   gather_pencils(c, dir, in, pen);
-  orc_diff_core(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, h);
+  if (c->diff_kind[dir] == ORC_DIFF_O4_EXPLICIT)
+    orc_diff_core_o4(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, h);
+  else
+    orc_diff_core(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, h);
   scatter_pencils(c, dir, pen, out);
   free(pen);
 }

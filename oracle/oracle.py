@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 MAX_BC = 32
 
 PERIODIC, OPEN = 0, 1
+DIFF_O6_COMPACT, DIFF_O4_EXPLICIT = 0, 1
 SUTHERLAND, POWER = 0, 1
 
 
@@ -40,6 +41,7 @@ class OrcCfg(C.Structure):
         ("bc", (C.c_double * 10) * MAX_BC),
         ("X_metric", C.c_void_p),
         ("X_jacobian", C.c_void_p),
+        ("diff_kind", C.c_int * 3),
     ]
 
 
@@ -62,6 +64,7 @@ def lib():
         L.orc_dgttrf.restype = C.c_int
         L.orc_dgttrs.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _ip, _dp, C.c_int]
         L.orc_diff_core.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double]
+        L.orc_diff_core_o4.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double]
         L.orc_filter_core.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double]
         L.orc_diff3d.argtypes = [P, C.c_int, _dp, _dp, C.c_double]
         L.orc_filter4d.argtypes = [P, C.c_int, _dp, C.c_int]
@@ -100,6 +103,7 @@ def ref_lib():
         L = C.CDLL(path)
         L.ref_init.argtypes = [C.c_double] * 7 + [C.c_int]
         L.ref_diff.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double]
+        L.ref_diff_o4.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double]
         L.ref_filter.argtypes = [C.c_int, _dp, C.c_int, C.c_int]
         L.ref_trafos.argtypes = [_dp, C.c_int, _dp, _dp, _dp]
         L.ref_characteristic_q.argtypes = [_dp, _dp, C.c_int, _dp]
@@ -118,9 +122,11 @@ class Cfg:
 
     def __init__(self, n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0,
                  W=1.0, Pr=0.71, mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76,
-                 dt=1e-3, filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), bc=(), metric=None):
+                 dt=1e-3, filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), bc=(), metric=None,
+                 diff_kind=(0, 0, 0)):
         """metric: (X_metric (n1,n2,n3,3,3), X_jacobian (n1,n2,n3)) of a deformed
-        grid; the direct RHS is then the grid-transform one."""
+        grid; the direct RHS is then the grid-transform one.  diff_kind per
+        direction: DIFF_O6_COMPACT or DIFF_O4_EXPLICIT."""
         c = OrcCfg()
         for d in range(3):
             c.n[d] = int(n[d])
@@ -128,6 +134,7 @@ class Cfg:
             c.scheme[d] = int(scheme[d])
             c.filter_freq[d] = int(filter_freq[d])
             c.afilter_freq[d] = int(afilter_freq[d])
+            c.diff_kind[d] = int(diff_kind[d])
         c.filter_alpha = filter_alpha
         c.gamma, c.R, c.W, c.Pr, c.mu0, c.T0 = gamma, R, W, Pr, mu0, T0
         c.visc_law, c.visc_exp, c.dt = visc_law, visc_exp, dt
@@ -156,9 +163,10 @@ def _f(a):
     return np.asfortranarray(a, dtype=np.float64)
 
 
-def diff_core(scheme, q, dx):
+def diff_core(scheme, q, dx, kind=0):
     q = _f(q).copy(order="F")
-    lib().orc_diff_core(scheme, q.shape[0], q.size // q.shape[0], q, dx)
+    f = lib().orc_diff_core_o4 if kind == DIFF_O4_EXPLICIT else lib().orc_diff_core
+    f(scheme, q.shape[0], q.size // q.shape[0], q, dx)
     return q
 
 

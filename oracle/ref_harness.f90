@@ -11,6 +11,8 @@ module ref_harness
   use parameter
   use diff_pen_core_o6_i_c_p, only: dp_i1 => diff_i1
   use diff_pen_core_o6_i_c_o, only: do_i1 => diff_i1
+  use diff_pen_core_o4_e_c_p, only: d4p_i1 => diff_i1
+  use diff_pen_core_o4_e_c_o, only: d4o_i1 => diff_i1
   use filter_pen_core_o6_i_c_p, only: fp_i1 => filter_i1
   use filter_pen_core_o6_i_c_o, only: fo_i1 => filter_i1
   use ns_rho_rhou_p_trafos, only: give_T, give_mu, give_lambda
@@ -46,6 +48,18 @@ contains
       call do_i1(q, dx)
     end if
   end subroutine ref_diff
+
+  ! the 4th-order explicit cores (no saved state)
+  subroutine ref_diff_o4(scheme, q, n, nlines, dx) bind(C, name="ref_diff_o4")
+    integer(c_int), value :: scheme, n, nlines
+    real(c_double), intent(inout) :: q(n, nlines)
+    real(c_double), value :: dx
+    if (scheme == 0) then
+      call d4p_i1(q, dx)
+    else
+      call d4o_i1(q, dx)
+    end if
+  end subroutine ref_diff_o4
 
   subroutine ref_filter(scheme, q, n, nlines) bind(C, name="ref_filter")
     integer(c_int), value :: scheme, n, nlines

@@ -5,5 +5,5 @@ Python host mirror of the reference's interface and `ic` builds the seeded
 synthetic workloads.
 """
 from . import ic  # noqa: F401
-from .cruna import (ADJOINT, FORWARD, FP32_MIXED, FP64, OPEN, PERIODIC, POWER,  # noqa: F401
-                    SUTHERLAND, Closure, Cruna, CrunaError, Ensemble, lib)
+from .cruna import (ADJOINT, DIFF_O4_EXPLICIT, DIFF_O6_COMPACT, FORWARD, FP32_MIXED, FP64,  # noqa: F401
+                    OPEN, PERIODIC, POWER, SUTHERLAND, Closure, Cruna, CrunaError, Ensemble, lib)

@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(HERE, "libcruna_hip.so")
 PERIODIC, OPEN = 0, 1
 SUTHERLAND, POWER = 0, 1
 FP64, FP32_MIXED = 0, 1
+DIFF_O6_COMPACT, DIFF_O4_EXPLICIT = 0, 1
 MAX_BC = 32
 
 
@@ -51,6 +52,7 @@ class Cfg(C.Structure):
         ("tile_lines", C.c_int * 3),
         ("nbc", C.c_int),
         ("bc", (C.c_double * 10) * MAX_BC),
+        ("diff_kind", C.c_int * 3),
     ]
 
 
@@ -154,7 +156,7 @@ def _f64(a, shape):
 def make_cfg(n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0, W=1.0, Pr=0.71,
              mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76, dt=1e-3,
              filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), heat_unit_spacing=1, precision=FP64,
-             bc=(), tile_lines=(0, 0, 0)):
+             bc=(), tile_lines=(0, 0, 0), diff_kind=(DIFF_O6_COMPACT,) * 3):
     """cruna_hip_cfg from the reference's parameter names (defaults of the library)."""
     cfg = Cfg()
     lib().cruna_hip_default_cfg(C.byref(cfg))
@@ -165,6 +167,7 @@ def make_cfg(n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0,
         cfg.filter_freq[d] = int(filter_freq[d])
         cfg.afilter_freq[d] = int(afilter_freq[d])
         cfg.tile_lines[d] = int(tile_lines[d])
+        cfg.diff_kind[d] = int(diff_kind[d])
     cfg.filter_alpha = filter_alpha
     cfg.gamma, cfg.R, cfg.W, cfg.Pr, cfg.mu0, cfg.T0 = gamma, R, W, Pr, mu0, T0
     cfg.visc_law, cfg.visc_exp, cfg.dt = visc_law, visc_exp, dt

@@ -5,6 +5,9 @@
 //   derivative (Lele 1992, 6th order):
 //     periodic  src/diff/diff_pen_core_o6_i_c_p.f90:87-228
 //     open      src/diff/diff_pen_core_o6_i_c_o.f90:85-199
+//   derivative, 4th-order explicit (B = I):
+//     periodic  src/diff/diff_pen_core_o4_e_c_p.f90:53-82
+//     open      src/diff/diff_pen_core_o4_e_c_o.f90:53-82
 //   filter (Gaitonde-Visbal 2000, 6th order):
 //     periodic  src/filter/filter_pen_core_o6_i_c_p.f90:84-249
 //     open      src/filter/filter_pen_core_o6_i_c_o.f90:79-240
@@ -36,7 +39,7 @@ constexpr int HALO = 3;          // explicit-stencil half width carried in M
 constexpr int FE = PPT + 2 * HALO;  // 14 = extended block
 constexpr int MAXK = 12;         // max reduced-system stages
 
-enum OpKind { OP_DIFF = 0, OP_FILTER = 1 };
+enum OpKind { OP_DIFF = 0, OP_FILTER = 1, OP_DIFF_O4E = 2 };
 
 // M, L, R are stored for three block types: 0 = first block, 1 = interior
 // blocks, 2 = last block (all equal on a periodic line).  W is stored per
@@ -133,7 +136,27 @@ inline void explicit_rows(OpKind kind, bool per, int n, double alpha,
     if (j < 0 || j >= n) return;
     A[i].push_back({j, c});
   };
-  if (kind == OP_DIFF) {
+  if (kind == OP_DIFF_O4E) {
+    // diff_pen_core_o4_e_c_{p,o}.f90 diff_main: (8 (f+1 - f-1) - (f+2 - f-2)) / 12,
+    // one-sided 5-point rows 1, 2, n-1, n on open lines
+    const double s = 1.0 / 12.0;
+    const int i0 = per ? 0 : 2, i1 = per ? n - 1 : n - 3;
+    for (int i = i0; i <= i1; ++i) {
+      add(i, i + 1, 8.0 * s);
+      add(i, i - 1, -8.0 * s);
+      add(i, i + 2, -1.0 * s);
+      add(i, i - 2, 1.0 * s);
+    }
+    if (!per) {
+      const double r0[5] = {-25.0, 48.0, -36.0, 16.0, -3.0}, r1[5] = {-3.0, -10.0, 18.0, -6.0, 1.0};
+      for (int k = 0; k < 5; ++k) {
+        add(0, k, r0[k] * s);
+        add(1, k, r1[k] * s);
+        add(n - 2, n - 1 - k, -r1[k] * s);
+        add(n - 1, n - 1 - k, -r0[k] * s);
+      }
+    }
+  } else if (kind == OP_DIFF) {
     // diff_pen_core_o6_i_c_{p,o}.f90 init_right_explicit_diff_part
     const double a = 14.00 / 9.0 / 2.0, b = 1.00 / 9.0 / 4.0;
     const int i0 = per ? 0 : 2, i1 = per ? n - 1 : n - 3;
@@ -188,10 +211,11 @@ inline void explicit_rows(OpKind kind, bool per, int n, double alpha,
 // Left matrix B as tridiagonal rows (sub, diag, super); cyclic corners for periodic.
 inline void implicit_rows(OpKind kind, bool per, int n, double alpha, std::vector<double> &lo,
                           std::vector<double> &di, std::vector<double> &up) {
-  const double al = kind == OP_DIFF ? 1.0 / 3.0 : alpha;
+  const double al = kind == OP_DIFF ? 1.0 / 3.0 : (kind == OP_DIFF_O4E ? 0.0 : alpha);
   lo.assign(n, al);
   di.assign(n, 1.0);
   up.assign(n, al);
+  if (kind == OP_DIFF_O4E) return;  // explicit: B = I
   if (!per) {
     lo[0] = 0.0;
     up[n - 1] = 0.0;

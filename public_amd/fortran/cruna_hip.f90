@@ -14,6 +14,7 @@ module cruna_hip
   integer, parameter, public :: CRUNA_HIP_VISC_SUTHERLAND = 0, CRUNA_HIP_VISC_POWER = 1
   integer, parameter, public :: CRUNA_HIP_FP64 = 0, CRUNA_HIP_FP32_MIXED = 1
   integer, parameter, public :: CRUNA_HIP_MAX_BC = 32
+  integer, parameter, public :: CRUNA_HIP_DIFF_O6_COMPACT = 0, CRUNA_HIP_DIFF_O4_EXPLICIT = 1
 
   ! cruna_hip_cfg (C layout; bc(1:10, r) is row r of boundary_conditions.dat)
   type, bind(C), public :: cruna_hip_cfg
@@ -37,6 +38,7 @@ module cruna_hip
     integer(c_int) :: tile_lines(3)
     integer(c_int) :: nbc
     real(c_double) :: bc(10, CRUNA_HIP_MAX_BC)
+    integer(c_int) :: diff_kind(3)
   end type cruna_hip_cfg
 
   interface

@@ -548,3 +548,59 @@ def test_adjoint_subsets_with_checkpoints_matches_oracle():
     assert np.array_equal(back, fwd)  # the final forward state is restored
     for v in range(5):
         assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+
+
+@pytest.mark.parametrize("scheme", [O.PERIODIC, O.OPEN])
+def test_o4_explicit_diff_matches_oracle(scheme):
+    """The 4th-order explicit cores (diff_pen_core_o4_e_c_{p,o}) on the device
+    line engine (B = I) along x, y and z of a non-cubic grid."""
+    Cruna = _gpu()
+    shape = (32, 24, 40)
+    rng = np.random.default_rng(17)
+    f = np.asfortranarray(rng.standard_normal(shape))
+    dx = (0.11, 0.23, 0.07)
+    cfg = O.Cfg(shape, dx, scheme=(scheme,) * 3, diff_kind=(O.DIFF_O4_EXPLICIT,) * 3)
+    with Cruna(shape, dx, scheme=(scheme,) * 3, diff_kind=(1, 1, 1)) as g:
+        for d in range(3):
+            got = g.Dx(d, f, dx[d])
+            ref = O.diff3d(cfg, d, f, dx[d])
+            assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max(), (d, np.abs(got - ref).max())
+
+
+def test_o4_explicit_rhs_and_run_match_oracle():
+    """Direct RHS and a 5-step device-resident run with the 4th-order explicit
+    derivative in every direction (the filter stays the o6 compact one), and
+    an open-y mixed case (o4 in x and y, o6 in z)."""
+    Cruna = _gpu()
+    n, mach = 32, 0.3
+    kw, h = _kw(mach, n)
+    q = _box(n, 8, mach)
+    o4 = (O.DIFF_O4_EXPLICIT,) * 3
+    cfg = O.Cfg((n,) * 3, (h,) * 3, filter_freq=(1, 1, 1), diff_kind=o4, **kw)
+    with Cruna((n,) * 3, (h,) * 3, filter_freq=(1, 1, 1), diff_kind=o4, **kw) as g:
+        r = g.right_hand_side_direct_3d(q)
+        g.set_state(q)
+        g.run_direct(5)
+        g.synchronize()
+        q5 = g.get_state()
+    rref = O.rhs_direct(cfg, q)
+    for v in range(5):
+        assert rel(r[..., v], rref[..., v]) < 1e-12, (v, rel(r[..., v], rref[..., v]))
+    qref = O.run_direct(cfg, q, 5)
+    assert rel(q5, qref) < 1e-12, rel(q5, qref)
+    # the o6 result differs: the option reaches the sweeps
+    assert rel(O.rhs_direct(O.Cfg((n,) * 3, (h,) * 3, **kw), q), rref) > 1e-6
+    shape = (32, 48, 16)
+    qs, dxs, _, _ = _shear_case(shape, seed=2)
+    scheme = (O.PERIODIC, O.OPEN, O.PERIODIC)
+    mixed = (O.DIFF_O4_EXPLICIT, O.DIFF_O4_EXPLICIT, O.DIFF_O6_COMPACT)
+    kw2 = dict(mu0=2e-3, T0=1.0 / (1.4 * 0.25), dt=2e-3, scheme=scheme)
+    cfg2 = O.Cfg(shape, dxs, filter_freq=(1, 1, 1), diff_kind=mixed, **kw2)
+    with Cruna(shape, dxs, filter_freq=(1, 1, 1), diff_kind=mixed, **kw2) as g:
+        g.set_state(qs)
+        g.run_direct(3)
+        g.synchronize()
+        got = g.get_state()
+    ref = O.run_direct(cfg2, qs, 3)
+    for v in range(5):
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))

@@ -75,9 +75,49 @@ def test_cores_vs_golden():
         kind, scheme, n = base.split("_")[0], int(base.split("_")[1]), int(base.split("_")[2])
         q = g[key]
         arg = float(g[base + "_arg"])
-        mine = O.diff_core(scheme, q, arg) if kind == "diff" else O.filter_core(scheme, q, arg)
+        if kind == "filter":
+            mine = O.filter_core(scheme, q, arg)
+        else:
+            mine = O.diff_core(scheme, q, arg, O.DIFF_O4_EXPLICIT if kind == "diffo4" else O.DIFF_O6_COMPACT)
         ref = g[base + "_out"]
         np.testing.assert_allclose(mine, ref, rtol=0, atol=1e-13 * np.abs(ref).max(), err_msg=base)
+
+
+@need_ref
+@pytest.mark.parametrize("scheme", [O.PERIODIC, O.OPEN])
+@pytest.mark.parametrize("n", [8, 24, 64])
+def test_o4_explicit_cores_vs_reference_fortran(scheme, n):
+    """diff_pen_core_o4_e_c_{p,o} restated in the oracle against the reference's own."""
+    q = _rand_lines(n, 5, 3 * n)
+    ref = refproc.run("core", "diff_o4", scheme, q, 0.17)
+    mine = O.diff_core(scheme, q, 0.17, O.DIFF_O4_EXPLICIT)
+    np.testing.assert_allclose(mine, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n", [16, 40])
+def test_o4_explicit_cores_closed_forms(n):
+    """4th order: every row is exact on quartics (the 5-point interior and
+    one-sided closures); periodic rows give the modified wavenumber
+    (8 sin w - sin 2w) / 6 on sine waves."""
+    h = 0.3
+    x = np.arange(n) * h
+    for p in range(5):
+        f = (x - 1.1) ** p
+        d = O.diff_core(O.OPEN, f[:, None], h, O.DIFF_O4_EXPLICIT)[:, 0]
+        exact = p * (x - 1.1) ** (p - 1) if p else np.zeros(n)
+        np.testing.assert_allclose(d, exact, atol=1e-9 * max(1.0, np.abs(exact).max()))
+    # a quintic is not differentiated exactly: the closures are 4th order, not more
+    f = (x - 1.1) ** 5
+    d = O.diff_core(O.OPEN, f[:, None], h, O.DIFF_O4_EXPLICIT)[:, 0]
+    assert np.abs(d - 5 * (x - 1.1) ** 4).max() > 1e-6
+    hp = 2 * np.pi / n
+    xp = np.arange(n) * hp
+    for k in (1, 3, n // 4):
+        f = np.sin(k * xp + 0.4)[:, None]
+        d = O.diff_core(O.PERIODIC, f, hp, O.DIFF_O4_EXPLICIT)[:, 0]
+        w = k * hp
+        exact = (8 * np.sin(w) - np.sin(2 * w)) / 6 / hp * np.cos(k * xp + 0.4)
+        np.testing.assert_allclose(d, exact, atol=1e-12 * k / hp)
 
 
 def _lele_modified_wavenumber(w):
