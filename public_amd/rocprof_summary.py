@@ -52,9 +52,13 @@ def _rows(db, query):
 
 
 def trace(db):
+    """Per (kernel, work-group count): dispatches, summed duration, grid, wg.
+    The same kernel on grids of other sizes (e.g. 64^3 and 256^3) gets a row
+    of its own."""
     agg = collections.OrderedDict()
     for name, dur, gx, wx in _rows(db, "select name, duration, grid_x, workgroup_x from kernels"):
-        a = agg.setdefault(kernel_name(name), [0, 0.0, gx // max(wx, 1), wx])
+        nb = gx // max(wx, 1)
+        a = agg.setdefault((kernel_name(name), nb), [0, 0.0, nb, wx])
         a[0] += 1
         a[1] += dur
     return agg
@@ -75,7 +79,7 @@ def counters(db, counter):
 def print_trace(db):
     agg = trace(db)
     print(f"{'kernel':34s} {'dispatches':>10s} {'avg_us':>10s} {'grid':>8s} {'wg':>5s}")
-    for name, (n, tot, g, w) in agg.items():
+    for (name, _), (n, tot, g, w) in agg.items():
         print(f"{name:34s} {n:10d} {tot / n / 1e3:10.2f} {g:8d} {w:5d}")
 
 
