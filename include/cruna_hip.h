@@ -230,6 +230,8 @@ int cruna_hip_ensemble_autotune(cruna_hip_ensemble *ens, int steps, int *tile_li
  * every context it is set on: use those contexts from one host thread. */
 typedef struct cruna_hip_cnn cruna_hip_cnn;
 int cruna_hip_cnn_create(int S, cruna_hip_cnn **out);
+/* EINVAL while the closure is attached to a context (detach it, or destroy
+ * the context, first) */
 int cruna_hip_cnn_destroy(cruna_hip_cnn *cnn);
 long cruna_hip_cnn_num_params(const cruna_hip_cnn *cnn);
 int cruna_hip_cnn_init_params(cruna_hip_cnn *cnn, unsigned seed);

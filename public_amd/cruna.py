@@ -531,8 +531,9 @@ class Closure:
         return losses
 
     def close(self):
+        """Free the closure; CrunaError while a context still uses it."""
         if getattr(self, "_h", None):
-            lib().cruna_hip_cnn_destroy(self._h)
+            _check(lib().cruna_hip_cnn_destroy(self._h))
             self._h = None
 
     def __del__(self):

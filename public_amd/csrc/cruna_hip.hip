@@ -83,6 +83,8 @@ struct DevOp {
 }  // namespace
 
 struct cruna_hip_cnn;
+// count of the contexts a closure is attached to (cruna_hip_set_closure)
+static void cnn_user(cruna_hip_cnn *n, int delta);
 struct cruna_hip_ctx {
   cruna_hip_cfg cfg{};
   int device = 0;
@@ -779,6 +781,7 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
 int cruna_hip_destroy(cruna_hip_ctx *c) {
   if (!c) return CRUNA_HIP_OK;
   (void)hipStreamSynchronize(c->stream);
+  if (c->cnn) cnn_user(c->cnn, -1);
   for (int d = 0; d < 3; ++d) {
     c->dop64[d].release();
     c->fop64[d].release();
@@ -1483,9 +1486,12 @@ struct cruna_hip_cnn {
   long spec_cap = 0;
   std::vector<bf16 *> ds_feat;
   std::vector<float *> ds_tgt;
+  int users = 0;              // contexts the closure is attached to
   std::vector<void *> fixed;  // parameter buffers
   std::vector<void *> work;   // per-point buffers
 };
+
+static void cnn_user(cruna_hip_cnn *n, int delta) { n->users += delta; }
 
 namespace {
 
@@ -1750,6 +1756,8 @@ int cruna_hip_cnn_create(int S, cruna_hip_cnn **out) {
 
 int cruna_hip_cnn_destroy(cruna_hip_cnn *n) {
   if (!n) return CRUNA_HIP_OK;
+  if (n->users > 0)
+    return fail(CRUNA_HIP_EINVAL, "the closure is still attached to a context (cruna_hip_set_closure(ctx, NULL))");
   (void)hipStreamSynchronize(n->stream);
   for (void *q : n->fixed) (void)hipFree(q);
   for (void *q : n->work) (void)hipFree(q);
@@ -1938,6 +1946,10 @@ int cruna_hip_set_closure(cruna_hip_ctx *c, cruna_hip_cnn *n) {
       HIP_TRY(hipMalloc(&c->force, 5 * c->N * ts));
     }
     HIP_TRY(hipStreamSynchronize(n->stream));
+  }
+  if (c->cnn != n) {
+    if (c->cnn) cnn_user(c->cnn, -1);
+    if (n) cnn_user(n, +1);
   }
   c->cnn = n;
   return CRUNA_HIP_OK;

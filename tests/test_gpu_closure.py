@@ -164,6 +164,22 @@ def test_box_filter_state():
     assert np.abs(rb - box(rhs)).max() < 1e-13 * np.abs(rhs).max()
 
 
+def test_attached_closure_cannot_be_freed():
+    """A closure in use by a context refuses to be destroyed; detaching it
+    (or destroying the context) releases it."""
+    from public_amd import Cruna, CrunaError, ic
+    net = _net(6)
+    h = ic.periodic_dx(S)
+    g = Cruna((S,) * 3, (h,) * 3)
+    g.set_closure(net)
+    with pytest.raises(CrunaError):
+        net.close()
+    g.set_closure(None)
+    g.set_closure(net)
+    g.close()
+    net.close()
+
+
 def test_closure_rejects_other_grids():
     from public_amd import OPEN, PERIODIC, Cruna, CrunaError
     net = _net(4)
