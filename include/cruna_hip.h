@@ -40,6 +40,7 @@ extern "C" {
 
 #define CRUNA_HIP_DIFF_O6_COMPACT 0  /* src/diff/diff_pen_core_o6_i_c_{p,o}.f90 */
 #define CRUNA_HIP_DIFF_O4_EXPLICIT 1 /* src/diff/diff_pen_core_o4_e_c_{p,o}.f90 */
+#define CRUNA_HIP_DIFF_O4_PADE 2     /* src/diff/diff_pen_core_o4_i_c_{p,o}_pade.f90 */
 
 typedef struct cruna_hip_cfg {
   int n[3];              /* params%geom%n1..n3 (block = whole grid on one GPU) */
@@ -66,9 +67,10 @@ typedef struct cruna_hip_cfg {
   int nbc;
   double bc[CRUNA_HIP_MAX_BC][10];
   int diff_kind[3];      /* derivative core per direction (discretisation_x_core):
-                            CRUNA_HIP_DIFF_O6_COMPACT (diff_pen_core_o6_i_c_*, default) or
-                            CRUNA_HIP_DIFF_O4_EXPLICIT (diff_pen_core_o4_e_c_*); the
-                            filter is the o6 compact one either way */
+                            CRUNA_HIP_DIFF_O6_COMPACT (diff_pen_core_o6_i_c_*, default),
+                            CRUNA_HIP_DIFF_O4_EXPLICIT (diff_pen_core_o4_e_c_*) or
+                            CRUNA_HIP_DIFF_O4_PADE (diff_pen_core_o4_i_c_*_pade); the
+                            filter is the o6 compact one in every case */
 } cruna_hip_cfg;
 
 typedef struct cruna_hip_ctx cruna_hip_ctx;

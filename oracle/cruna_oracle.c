@@ -136,10 +136,11 @@ static void lu_alloc(orc_lu *f, int n) {
   f->cyclic = 0;
 }
 
-/* Cyclic tridiag(alpha,1,alpha) with the Sherman-Morrison split used by
- * src/diff/diff_pen_core_o6_i_c_p.f90:182-228 and
- * src/filter/filter_pen_core_o6_i_c_p.f90:217-246. */
-static void lu_cyclic(orc_lu *f, int n, double alpha) {
+/* Cyclic tridiag(alpha,diag,alpha) with the Sherman-Morrison split used by
+ * src/diff/diff_pen_core_o6_i_c_p.f90:182-228 (diag 1),
+ * src/filter/filter_pen_core_o6_i_c_p.f90:217-246 (diag 1) and
+ * src/diff/diff_pen_core_o4_i_c_p_pade.f90:190-230 (diag 2/3). */
+static void lu_cyclic_d(orc_lu *f, int n, double alpha, double diag) {
   lu_alloc(f, n);
   f->cyclic = 1;
   double *fa = xalloc(n), *fb = xalloc(n);
@@ -148,7 +149,7 @@ static void lu_cyclic(orc_lu *f, int n, double alpha) {
     f->du[i] = alpha;
   }
   for (int i = 0; i < n; ++i) {
-    f->dc[i] = 1.0;
+    f->dc[i] = diag;
     fa[i] = 0.0;
     fb[i] = 0.0;
   }
@@ -167,6 +168,24 @@ static void lu_cyclic(orc_lu *f, int n, double alpha) {
   f->fbrat = fb[n - 1] / fb[0];
   free(fa);
   free(fb);
+}
+
+static void lu_cyclic(orc_lu *f, int n, double alpha) { lu_cyclic_d(f, n, alpha, 1.0); }
+
+/* Open-boundary B of the 4th-order Pade derivative,
+ * src/diff/diff_pen_core_o4_i_c_o_pade.f90:150-190: tridiag(1/6, 2/3, 1/6)
+ * with dc(1) = dc(n) = 1/3. */
+static void lu_pade_open(orc_lu *f, int n) {
+  lu_alloc(f, n);
+  for (int i = 0; i < n - 1; ++i) {
+    f->dl[i] = 1.0 / 6.0;
+    f->du[i] = 1.0 / 6.0;
+  }
+  for (int i = 0; i < n; ++i) f->dc[i] = 2.0 / 3.0;
+  f->dc[0] = 1.0 / 3.00;
+  f->dc[n - 1] = 1.0 / 3.00;
+  int info = orc_dgttrf(n, f->dl, f->dc, f->du, f->du2, f->ipiv);
+  if (info) fprintf(stderr, "oracle: dgttrf info %d\n", info);
 }
 
 /* Open-boundary B of the Lele o6 derivative, src/diff/diff_pen_core_o6_i_c_o.f90:159-199. */
@@ -289,6 +308,35 @@ void orc_diff_core_o4(int scheme, int n, int nlines, double *q, double dx) {
     for (int i = 0; i < n; ++i) x[i] = t[i] / (dx * 12.0);
   }
   free(t);
+}
+
+/* 4th-order Pade cores: src/diff/diff_pen_core_o4_i_c_p_pade.f90:96-140 and
+ * diff_pen_core_o4_i_c_o_pade.f90:81-124 (a (f_{i+1} - f_{i-1}), a = 1/2, with
+ * the one-sided rows -a f1 + a f2 and a fn - a f(n-1) on open lines; / dx;
+ * then the B solve). */
+void orc_diff_core_pade(int scheme, int n, int nlines, double *q, double dx) {
+  const double a = 1.0 / 2.0;
+  orc_lu f;
+  if (scheme == ORC_PERIODIC)
+    lu_cyclic_d(&f, n, 1.0 / 6.0, 2.0 / 3.0);
+  else
+    lu_pade_open(&f, n);
+  double *t = xalloc(n);
+  for (int l = 0; l < nlines; ++l) {
+    double *x = q + (size_t)l * n;
+    for (int i = 0; i < n; ++i) {
+      const int ip1 = (i + 1) % n, im1 = (i - 1 + n) % n;
+      t[i] = a * (x[ip1] - x[im1]);
+    }
+    if (scheme != ORC_PERIODIC) {
+      t[0] = -a * x[0] + a * (x[1]);
+      t[n - 1] = a * x[n - 1] - a * (x[n - 2]);
+    }
+    for (int i = 0; i < n; ++i) x[i] = t[i] / dx;
+  }
+  lu_solve(&f, q, nlines);
+  free(t);
+  lu_free(&f);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -435,6 +483,8 @@ void orc_diff3d(const orc_cfg *c, int dir, double *out, const double *in, double
   gather_pencils(c, dir, in, pen);
   if (c->diff_kind[dir] == ORC_DIFF_O4_EXPLICIT)
     orc_diff_core_o4(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, h);
+  else if (c->diff_kind[dir] == ORC_DIFF_O4_PADE)
+    orc_diff_core_pade(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, h);
   else
     orc_diff_core(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, h);
   scatter_pencils(c, dir, pen, out);

@@ -47,12 +47,14 @@ typedef struct orc_cfg {
   const double *X_jacobian;
   /* derivative core per direction (the case's discretisation_x_core choice):
    * ORC_DIFF_O6_COMPACT: diff_pen_core_o6_i_c_{p,o}; ORC_DIFF_O4_EXPLICIT:
-   * diff_pen_core_o4_e_c_{p,o}.  The filter is the o6 one either way. */
+   * diff_pen_core_o4_e_c_{p,o}; ORC_DIFF_O4_PADE: diff_pen_core_o4_i_c_{p,o}_pade.
+   * The filter is the o6 one either way. */
   int diff_kind[3];
 } orc_cfg;
 
 #define ORC_DIFF_O6_COMPACT 0
 #define ORC_DIFF_O4_EXPLICIT 1
+#define ORC_DIFF_O4_PADE 2
 
 /* LAPACK dgttrf/dgttrs restatement (no transpose), 1 RHS at a time. */
 int orc_dgttrf(int n, double *dl, double *d, double *du, double *du2, int *ipiv);
@@ -62,6 +64,7 @@ void orc_dgttrs(int n, int nrhs, const double *dl, const double *d, const double
 /* Pencil cores: q(n, nlines) in place, as diff_iN / filter_iN cores. */
 void orc_diff_core(int scheme, int n, int nlines, double *q, double dx);
 void orc_diff_core_o4(int scheme, int n, int nlines, double *q, double dx);
+void orc_diff_core_pade(int scheme, int n, int nlines, double *q, double dx);
 void orc_filter_core(int scheme, int n, int nlines, double *q, double alpha);
 
 /* 3-D wrappers (diff_3d / filter_4d): dir = 0,1,2. */

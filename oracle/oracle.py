@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 MAX_BC = 32
 
 PERIODIC, OPEN = 0, 1
-DIFF_O6_COMPACT, DIFF_O4_EXPLICIT = 0, 1
+DIFF_O6_COMPACT, DIFF_O4_EXPLICIT, DIFF_O4_PADE = 0, 1, 2
 SUTHERLAND, POWER = 0, 1
 
 
@@ -65,6 +65,7 @@ def lib():
         L.orc_dgttrs.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _ip, _dp, C.c_int]
         L.orc_diff_core.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double]
         L.orc_diff_core_o4.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double]
+        L.orc_diff_core_pade.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double]
         L.orc_filter_core.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double]
         L.orc_diff3d.argtypes = [P, C.c_int, _dp, _dp, C.c_double]
         L.orc_filter4d.argtypes = [P, C.c_int, _dp, C.c_int]
@@ -104,6 +105,7 @@ def ref_lib():
         L.ref_init.argtypes = [C.c_double] * 7 + [C.c_int]
         L.ref_diff.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double]
         L.ref_diff_o4.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double]
+        L.ref_diff_pade.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double]
         L.ref_filter.argtypes = [C.c_int, _dp, C.c_int, C.c_int]
         L.ref_trafos.argtypes = [_dp, C.c_int, _dp, _dp, _dp]
         L.ref_characteristic_q.argtypes = [_dp, _dp, C.c_int, _dp]
@@ -126,7 +128,7 @@ class Cfg:
                  diff_kind=(0, 0, 0)):
         """metric: (X_metric (n1,n2,n3,3,3), X_jacobian (n1,n2,n3)) of a deformed
         grid; the direct RHS is then the grid-transform one.  diff_kind per
-        direction: DIFF_O6_COMPACT or DIFF_O4_EXPLICIT."""
+        direction: DIFF_O6_COMPACT, DIFF_O4_EXPLICIT or DIFF_O4_PADE."""
         c = OrcCfg()
         for d in range(3):
             c.n[d] = int(n[d])
@@ -165,7 +167,8 @@ def _f(a):
 
 def diff_core(scheme, q, dx, kind=0):
     q = _f(q).copy(order="F")
-    f = lib().orc_diff_core_o4 if kind == DIFF_O4_EXPLICIT else lib().orc_diff_core
+    f = {DIFF_O4_EXPLICIT: lib().orc_diff_core_o4, DIFF_O4_PADE: lib().orc_diff_core_pade}.get(
+        kind, lib().orc_diff_core)
     f(scheme, q.shape[0], q.size // q.shape[0], q, dx)
     return q
 

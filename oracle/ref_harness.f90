@@ -13,6 +13,8 @@ module ref_harness
   use diff_pen_core_o6_i_c_o, only: do_i1 => diff_i1
   use diff_pen_core_o4_e_c_p, only: d4p_i1 => diff_i1
   use diff_pen_core_o4_e_c_o, only: d4o_i1 => diff_i1
+  use diff_pen_core_o4_i_c_p_pade, only: dpp_i1 => diff_i1
+  use diff_pen_core_o4_i_c_o_pade, only: dpo_i1 => diff_i1
   use filter_pen_core_o6_i_c_p, only: fp_i1 => filter_i1
   use filter_pen_core_o6_i_c_o, only: fo_i1 => filter_i1
   use ns_rho_rhou_p_trafos, only: give_T, give_mu, give_lambda
@@ -60,6 +62,18 @@ contains
       call d4o_i1(q, dx)
     end if
   end subroutine ref_diff_o4
+
+  ! the 4th-order Pade cores (factorisation saved on first use, like o6)
+  subroutine ref_diff_pade(scheme, q, n, nlines, dx) bind(C, name="ref_diff_pade")
+    integer(c_int), value :: scheme, n, nlines
+    real(c_double), intent(inout) :: q(n, nlines)
+    real(c_double), value :: dx
+    if (scheme == 0) then
+      call dpp_i1(q, dx)
+    else
+      call dpo_i1(q, dx)
+    end if
+  end subroutine ref_diff_pade
 
   subroutine ref_filter(scheme, q, n, nlines) bind(C, name="ref_filter")
     integer(c_int), value :: scheme, n, nlines

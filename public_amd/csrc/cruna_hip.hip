@@ -694,7 +694,8 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
       return fail(CRUNA_HIP_EINVAL, "grid size above 2048 points per line is not supported");
     if (cfg->scheme[d] != CRUNA_HIP_PERIODIC && cfg->scheme[d] != CRUNA_HIP_OPEN)
       return fail(CRUNA_HIP_EINVAL, "unknown scheme");
-    if (cfg->diff_kind[d] != CRUNA_HIP_DIFF_O6_COMPACT && cfg->diff_kind[d] != CRUNA_HIP_DIFF_O4_EXPLICIT)
+    if (cfg->diff_kind[d] != CRUNA_HIP_DIFF_O6_COMPACT && cfg->diff_kind[d] != CRUNA_HIP_DIFF_O4_EXPLICIT &&
+        cfg->diff_kind[d] != CRUNA_HIP_DIFF_O4_PADE)
       return fail(CRUNA_HIP_EINVAL, "unknown derivative core");
     if (!(cfg->dx[d] > 0)) return fail(CRUNA_HIP_EINVAL, "dx must be positive");
   }
@@ -735,7 +736,9 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   for (int d = 0; d < 3; ++d) {
     if (int rc = build_geom(c.get(), d)) return rc;
     try {
-      const OpKind dk = cfg->diff_kind[d] == CRUNA_HIP_DIFF_O4_EXPLICIT ? OP_DIFF_O4E : OP_DIFF;
+      const OpKind dk = cfg->diff_kind[d] == CRUNA_HIP_DIFF_O4_EXPLICIT ? OP_DIFF_O4E
+                        : cfg->diff_kind[d] == CRUNA_HIP_DIFF_O4_PADE   ? OP_DIFF_O4P
+                                                                        : OP_DIFF;
       HostLineOp dd = build_line_op(dk, cfg->scheme[d] == CRUNA_HIP_PERIODIC, cfg->n[d], 0.0);
       HostLineOp ff =
           build_line_op(OP_FILTER, cfg->scheme[d] == CRUNA_HIP_PERIODIC, cfg->n[d], cfg->filter_alpha);

@@ -39,7 +39,7 @@ constexpr int HALO = 3;          // explicit-stencil half width carried in M
 constexpr int FE = PPT + 2 * HALO;  // 14 = extended block
 constexpr int MAXK = 12;         // max reduced-system stages
 
-enum OpKind { OP_DIFF = 0, OP_FILTER = 1, OP_DIFF_O4E = 2 };
+enum OpKind { OP_DIFF = 0, OP_FILTER = 1, OP_DIFF_O4E = 2, OP_DIFF_O4P = 3 };
 
 // M, L, R are stored for three block types: 0 = first block, 1 = interior
 // blocks, 2 = last block (all equal on a periodic line).  W is stored per
@@ -156,6 +156,19 @@ inline void explicit_rows(OpKind kind, bool per, int n, double alpha,
         add(n - 1, n - 1 - k, -r0[k] * s);
       }
     }
+  } else if (kind == OP_DIFF_O4P) {
+    // diff_pen_core_o4_i_c_{p,o}_pade.f90 diff_main: (f+1 - f-1) / 2, with the
+    // one-sided rows (f2 - f1) / 2 and (fn - f(n-1)) / 2 on open lines
+    const double a = 0.5;
+    const int i0 = per ? 0 : 1, i1 = per ? n - 1 : n - 2;
+    for (int i = i0; i <= i1; ++i) {
+      add(i, i + 1, a);
+      add(i, i - 1, -a);
+    }
+    if (!per) {
+      add(0, 0, -a); add(0, 1, a);
+      add(n - 1, n - 1, a); add(n - 1, n - 2, -a);
+    }
   } else if (kind == OP_DIFF) {
     // diff_pen_core_o6_i_c_{p,o}.f90 init_right_explicit_diff_part
     const double a = 14.00 / 9.0 / 2.0, b = 1.00 / 9.0 / 4.0;
@@ -211,6 +224,20 @@ inline void explicit_rows(OpKind kind, bool per, int n, double alpha,
 // Left matrix B as tridiagonal rows (sub, diag, super); cyclic corners for periodic.
 inline void implicit_rows(OpKind kind, bool per, int n, double alpha, std::vector<double> &lo,
                           std::vector<double> &di, std::vector<double> &up) {
+  if (kind == OP_DIFF_O4P) {
+    // diff_pen_core_o4_i_c_p_pade.f90 (tridiag(1/6, 2/3, 1/6), cyclic) and
+    // diff_pen_core_o4_i_c_o_pade.f90 (the same with dc(1) = dc(n) = 1/3)
+    lo.assign(n, 1.0 / 6.0);
+    di.assign(n, 2.0 / 3.0);
+    up.assign(n, 1.0 / 6.0);
+    if (!per) {
+      lo[0] = 0.0;
+      up[n - 1] = 0.0;
+      di[0] = 1.0 / 3.0;
+      di[n - 1] = 1.0 / 3.0;
+    }
+    return;
+  }
   const double al = kind == OP_DIFF ? 1.0 / 3.0 : (kind == OP_DIFF_O4E ? 0.0 : alpha);
   lo.assign(n, al);
   di.assign(n, 1.0);

@@ -14,7 +14,8 @@ module cruna_hip
   integer, parameter, public :: CRUNA_HIP_VISC_SUTHERLAND = 0, CRUNA_HIP_VISC_POWER = 1
   integer, parameter, public :: CRUNA_HIP_FP64 = 0, CRUNA_HIP_FP32_MIXED = 1
   integer, parameter, public :: CRUNA_HIP_MAX_BC = 32
-  integer, parameter, public :: CRUNA_HIP_DIFF_O6_COMPACT = 0, CRUNA_HIP_DIFF_O4_EXPLICIT = 1
+  integer, parameter, public :: CRUNA_HIP_DIFF_O6_COMPACT = 0, CRUNA_HIP_DIFF_O4_EXPLICIT = 1, &
+                                CRUNA_HIP_DIFF_O4_PADE = 2
 
   ! cruna_hip_cfg (C layout; bc(1:10, r) is row r of boundary_conditions.dat)
   type, bind(C), public :: cruna_hip_cfg

@@ -78,20 +78,27 @@ int main() {
   std::mt19937_64 rng(7);
   std::normal_distribution<double> nd;
   const int ns[] = {16, 24, 48, 64, 96, 128, 256, 512};
-  for (int kind = 0; kind < 2; ++kind)
+  const OpKind kinds[] = {OP_DIFF, OP_FILTER, OP_DIFF_O4E, OP_DIFF_O4P};
+  const char *names[] = {"diff", "filter", "diffo4", "diffpade"};
+  for (int kind = 0; kind < 4; ++kind)
     for (int per = 1; per >= 0; --per)
       for (int n : ns) {
         const double alpha = 0.35;
-        HostLineOp op = build_line_op(kind == 0 ? OP_DIFF : OP_FILTER, per, n, alpha);
+        HostLineOp op = build_line_op(kinds[kind], per, n, alpha);
         double worst = 0.0;
         for (int t = 0; t < 4; ++t) {
           std::vector<double> f(n);
           for (auto &v : f) v = nd(rng);
           std::vector<double> ref = f;
+          const int sch = per ? ORC_PERIODIC : ORC_OPEN;
           if (kind == 0)
-            orc_diff_core(per ? ORC_PERIODIC : ORC_OPEN, n, 1, ref.data(), 1.0);
+            orc_diff_core(sch, n, 1, ref.data(), 1.0);
+          else if (kind == 1)
+            orc_filter_core(sch, n, 1, ref.data(), alpha);
+          else if (kind == 2)
+            orc_diff_core_o4(sch, n, 1, ref.data(), 1.0);
           else
-            orc_filter_core(per ? ORC_PERIODIC : ORC_OPEN, n, 1, ref.data(), alpha);
+            orc_diff_core_pade(sch, n, 1, ref.data(), 1.0);
           std::vector<double> x = emulate(op, f);
           double num = 0, den = 0;
           for (int i = 0; i < n; ++i) {
@@ -100,7 +107,7 @@ int main() {
           }
           worst = std::fmax(worst, num / den);
         }
-        std::printf("%s %s %d %d %.3e\n", kind == 0 ? "diff" : "filter", per ? "periodic" : "open", n,
+        std::printf("%s %s %d %d %.3e\n", names[kind], per ? "periodic" : "open", n,
                     op.K, worst);
       }
   return 0;

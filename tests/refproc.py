@@ -44,7 +44,7 @@ def _lib(alpha=0.35, gamma=1.4, R=1.0, W=1.0, Pr=0.71, mu0=1e-3, T0=1.0, npts=1)
 
 
 def core(kind, scheme, q, arg):
-    """kind 'diff' or 'diff_o4' (arg = dx) or 'filter' (arg = alpha) on q(n, nlines)."""
+    """kind 'diff', 'diff_o4' or 'diff_pade' (arg = dx) or 'filter' (arg = alpha) on q(n, nlines)."""
     import numpy as np
     q = np.asfortranarray(q, dtype=np.float64).copy(order="F")
     if kind == "diff":
@@ -53,6 +53,9 @@ def core(kind, scheme, q, arg):
     elif kind == "diff_o4":
         L = _lib()
         L.ref_diff_o4(scheme, q, q.shape[0], q.shape[1], arg)
+    elif kind == "diff_pade":
+        L = _lib()
+        L.ref_diff_pade(scheme, q, q.shape[0], q.shape[1], arg)
     else:
         L = _lib(alpha=arg)
         L.ref_filter(scheme, q, q.shape[0], q.shape[1])

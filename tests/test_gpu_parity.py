@@ -60,21 +60,23 @@ def test_filter_matches_oracle(shape, scheme):
 
 
 def test_dx_golden_vectors():
-    """Lines of the reference's own Fortran cores (tests/golden/cores.npz) placed along x."""
+    """Lines of the reference's own Fortran derivative cores (o6 compact, o4
+    explicit, o4 Pade; tests/golden/cores.npz) placed along x."""
     Cruna = _gpu()
     g = np.load(GOLDEN)
     for key in g.files:
         if not key.endswith("_in") or not key.startswith("diff"):
             continue
         base = key[:-3]
-        scheme, n = int(base.split("_")[1]), int(base.split("_")[2])
+        kind, scheme, n = base.split("_")[0], int(base.split("_")[1]), int(base.split("_")[2])
         if n < 16:
             continue
+        dk = {"diff": O.DIFF_O6_COMPACT, "diffo4": O.DIFF_O4_EXPLICIT, "diffpade": O.DIFF_O4_PADE}[kind]
         lines = g[key]
         h = float(g[base + "_arg"])
         f = _rand((n, 16, 16), 3)
         f[:, : lines.shape[1], 0] = lines
-        with Cruna((n, 16, 16), (h, 1, 1), scheme=(scheme, 0, 0)) as ctx:
+        with Cruna((n, 16, 16), (h, 1, 1), scheme=(scheme, 0, 0), diff_kind=(dk, 0, 0)) as ctx:
             got = ctx.Dx(0, f, h)[:, : lines.shape[1], 0]
         ref = g[base + "_out"]
         assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max(), base
@@ -550,42 +552,49 @@ def test_adjoint_subsets_with_checkpoints_matches_oracle():
         assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
 
 
+@pytest.mark.parametrize("kind", [O.DIFF_O4_EXPLICIT, O.DIFF_O4_PADE])
 @pytest.mark.parametrize("scheme", [O.PERIODIC, O.OPEN])
-def test_o4_explicit_diff_matches_oracle(scheme):
-    """The 4th-order explicit cores (diff_pen_core_o4_e_c_{p,o}) on the device
-    line engine (B = I) along x, y and z of a non-cubic grid."""
+def test_o4_diff_matches_oracle(scheme, kind):
+    """The 4th-order explicit cores (diff_pen_core_o4_e_c_{p,o}, B = I) and the
+    4th-order Pade cores (diff_pen_core_o4_i_c_{p,o}_pade, B = tridiag(1/6,
+    2/3, 1/6)) on the device line engine along x, y and z of a non-cubic grid."""
     Cruna = _gpu()
     shape = (32, 24, 40)
     rng = np.random.default_rng(17)
     f = np.asfortranarray(rng.standard_normal(shape))
     dx = (0.11, 0.23, 0.07)
-    cfg = O.Cfg(shape, dx, scheme=(scheme,) * 3, diff_kind=(O.DIFF_O4_EXPLICIT,) * 3)
-    with Cruna(shape, dx, scheme=(scheme,) * 3, diff_kind=(1, 1, 1)) as g:
+    cfg = O.Cfg(shape, dx, scheme=(scheme,) * 3, diff_kind=(kind,) * 3)
+    with Cruna(shape, dx, scheme=(scheme,) * 3, diff_kind=(kind,) * 3) as g:
         for d in range(3):
             got = g.Dx(d, f, dx[d])
             ref = O.diff3d(cfg, d, f, dx[d])
             assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max(), (d, np.abs(got - ref).max())
 
 
-def test_o4_explicit_rhs_and_run_match_oracle():
-    """Direct RHS and a 5-step device-resident run with the 4th-order explicit
-    derivative in every direction (the filter stays the o6 compact one), and
-    an open-y mixed case (o4 in x and y, o6 in z)."""
+@pytest.mark.parametrize("kind", [O.DIFF_O4_EXPLICIT, O.DIFF_O4_PADE])
+def test_o4_rhs_and_run_match_oracle(kind):
+    """Direct and adjoint RHS and a 5-step device-resident run with a
+    4th-order derivative in every direction (the filter stays the o6 compact
+    one), and an open-y mixed case (o4 in x and y, o6 in z)."""
     Cruna = _gpu()
     n, mach = 32, 0.3
     kw, h = _kw(mach, n)
     q = _box(n, 8, mach)
-    o4 = (O.DIFF_O4_EXPLICIT,) * 3
+    qa = _box(n, 9, mach) - _box(n, 8, mach)
+    o4 = (kind,) * 3
     cfg = O.Cfg((n,) * 3, (h,) * 3, filter_freq=(1, 1, 1), diff_kind=o4, **kw)
     with Cruna((n,) * 3, (h,) * 3, filter_freq=(1, 1, 1), diff_kind=o4, **kw) as g:
         r = g.right_hand_side_direct_3d(q)
+        ra = g.right_hand_side_adjoint_3d(qa, q)
         g.set_state(q)
         g.run_direct(5)
         g.synchronize()
         q5 = g.get_state()
     rref = O.rhs_direct(cfg, q)
+    raref = O.rhs_adjoint(cfg, qa, q)
     for v in range(5):
         assert rel(r[..., v], rref[..., v]) < 1e-12, (v, rel(r[..., v], rref[..., v]))
+        assert rel(ra[..., v], raref[..., v]) < 1e-12, (v, rel(ra[..., v], raref[..., v]))
     qref = O.run_direct(cfg, q, 5)
     assert rel(q5, qref) < 1e-12, rel(q5, qref)
     # the o6 result differs: the option reaches the sweeps
@@ -593,7 +602,7 @@ def test_o4_explicit_rhs_and_run_match_oracle():
     shape = (32, 48, 16)
     qs, dxs, _, _ = _shear_case(shape, seed=2)
     scheme = (O.PERIODIC, O.OPEN, O.PERIODIC)
-    mixed = (O.DIFF_O4_EXPLICIT, O.DIFF_O4_EXPLICIT, O.DIFF_O6_COMPACT)
+    mixed = (kind, kind, O.DIFF_O6_COMPACT)
     kw2 = dict(mu0=2e-3, T0=1.0 / (1.4 * 0.25), dt=2e-3, scheme=scheme)
     cfg2 = O.Cfg(shape, dxs, filter_freq=(1, 1, 1), diff_kind=mixed, **kw2)
     with Cruna(shape, dxs, filter_freq=(1, 1, 1), diff_kind=mixed, **kw2) as g:

@@ -78,7 +78,8 @@ def test_cores_vs_golden():
         if kind == "filter":
             mine = O.filter_core(scheme, q, arg)
         else:
-            mine = O.diff_core(scheme, q, arg, O.DIFF_O4_EXPLICIT if kind == "diffo4" else O.DIFF_O6_COMPACT)
+            dk = {"diffo4": O.DIFF_O4_EXPLICIT, "diffpade": O.DIFF_O4_PADE}.get(kind, O.DIFF_O6_COMPACT)
+            mine = O.diff_core(scheme, q, arg, dk)
         ref = g[base + "_out"]
         np.testing.assert_allclose(mine, ref, rtol=0, atol=1e-13 * np.abs(ref).max(), err_msg=base)
 
@@ -118,6 +119,32 @@ def test_o4_explicit_cores_closed_forms(n):
         w = k * hp
         exact = (8 * np.sin(w) - np.sin(2 * w)) / 6 / hp * np.cos(k * xp + 0.4)
         np.testing.assert_allclose(d, exact, atol=1e-12 * k / hp)
+
+
+@need_ref
+@pytest.mark.parametrize("scheme", [O.PERIODIC, O.OPEN])
+@pytest.mark.parametrize("n", [8, 24, 64])
+def test_pade_cores_vs_reference_fortran(scheme, n):
+    """diff_pen_core_o4_i_c_{p,o}_pade restated in the oracle against the reference's own."""
+    q = _rand_lines(n, 5, 5 * n)
+    ref = refproc.run("core", "diff_pade", scheme, q, 0.17)
+    mine = O.diff_core(scheme, q, 0.17, O.DIFF_O4_PADE)
+    np.testing.assert_allclose(mine, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n", [16, 48])
+def test_pade_periodic_closed_form(n):
+    """Periodic Pade: the modified wavenumber 3 sin w / (2 + cos w) on sine
+    waves, D(const) = 0."""
+    h = 2 * np.pi / n
+    x = np.arange(n) * h
+    for k in (1, 3, n // 4):
+        f = np.sin(k * x + 0.2)[:, None]
+        d = O.diff_core(O.PERIODIC, f, h, O.DIFF_O4_PADE)[:, 0]
+        w = k * h
+        exact = 3 * np.sin(w) / (2 + np.cos(w)) / h * np.cos(k * x + 0.2)
+        np.testing.assert_allclose(d, exact, atol=1e-12 * k / h)
+    assert np.abs(O.diff_core(O.PERIODIC, np.ones((n, 2)), h, O.DIFF_O4_PADE)).max() < 1e-13
 
 
 def _lele_modified_wavenumber(w):
