@@ -69,6 +69,18 @@ SWEEP_COMPUTE_WORDS = {
 # The step-level byte model of the reference's dependency graph (4624 B per
 # point per FP64 RK4 step incl. filtering; 2672 B in FP32-mixed).
 MODEL_BYTES = {"f64": 4624.0, "fp32-mixed": 2672.0}
+# General curvilinear grid (full metric per point): words per point and launch
+# of the curvilinear sweeps.  x/y/z first-derivative sweeps: state 5 + metric
+# row 3, six velocity-gradient, three temperature-gradient and five RHS
+# fields written (x, also mu) or read-modify-written (y, z); viscous z/y:
+# metric row 3 + J + mu + 3 temperature gradients + 6 velocity gradients, 4
+# RHS fields read-modify-written; viscous x: the same 14 + 5 RHS read + the
+# RK4 tail (mean of 15, 20, 20, 10 words over the stages).
+CURV_WORDS = {"sweep1_x": 23.0, "sweep1_y": 36.0, "sweep1_z": 36.0, "sweep2_z": 22.0,
+              "sweep2_y": 22.0, "sweep2_x_rk": 35.25, "filter_x": 10.0, "filter_y": 10.0,
+              "filter_z": 10.0}
+# per point and FP64 step: four stages of the six sweeps and three filter passes
+CURV_MODEL_BYTES = 8.0 * (4 * (23 + 36 + 36 + 22 + 22 + 35.25) + 30)
 
 
 def kernel_bytes_per_point(name, mixed):
@@ -175,6 +187,59 @@ def run_shear_layer(steps, warmup, n=(512, 256, 128), seed=0):
             "grid": list(n), "dt": dt, "dtype": "fp32-mixed (f32 RHS, f64 state/RK/boundaries)",
             "finite": bool(np.isfinite(qe).all()),
             "max_abs_v": float(np.abs(qe[..., 2] / qe[..., 0]).max())}
+
+
+def run_curvilinear(steps, warmup, n=256, seed=0, mach=0.3, amp=0.1):
+    """The periodic-box workload (same state, viscosity law and filters) on a
+    general curvilinear grid: the triply periodic wavy mesh
+    x = xi + amp sin(eta) sin(zeta), y = eta + amp sin(xi) sin(zeta),
+    z = zeta + amp sin(xi) sin(eta) on [0, 2pi)^3, metric from calc_metric's
+    conservative form with the library's own derivatives; FP64, RK4 with
+    the x, y, z filters every step."""
+    from public_amd import POWER, Cruna, ic
+    from public_amd import geometry as geo
+    gamma = 1.4
+    h = ic.periodic_dx(n)
+    xi = np.arange(n) * h
+    Xc = np.stack(np.meshgrid(xi, xi, xi, indexing="ij"), axis=-1)
+    s, t, u = Xc[..., 0], Xc[..., 1], Xc[..., 2]
+    X = np.stack([s + amp * np.sin(t) * np.sin(u), t + amp * np.sin(s) * np.sin(u),
+                  u + amp * np.sin(s) * np.sin(t)], axis=-1)
+    del s, t, u
+    X, Xc = np.asfortranarray(X), np.asfortranarray(Xc)
+    q0 = ic.periodic_box_state(n, seed, mach, gamma)
+    dt = geo.cfl_dt(q0, X, 0.5, gamma)
+    kw = dict(gamma=gamma, R=1.0, W=1.0, Pr=0.71, mu0=1.0 / 1600.0, T0=1.0 / (gamma * mach ** 2),
+              visc_law=POWER, visc_exp=0.76, filter_alpha=0.35, dt=dt, filter_freq=(1, 1, 1))
+    g = Cruna((n,) * 3, (h,) * 3, **kw)
+    M, J = geo.calc_metric(g, X, Xc)
+    del X, Xc
+    g.set_metric(M, J)
+    del M, J
+    g.set_state(q0)
+    g.run_direct(warmup)
+    g.synchronize()
+    g.profile(True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run_direct(steps, nt0=warmup)
+    g.synchronize()
+    t1 = time.perf_counter()
+    prof = g.profile_read()
+    g.profile(False)
+    qe = g.get_state()
+    g.close()
+    pts = float(n) ** 3
+    ms = (t1 - t0) / steps * 1e3
+    kernels = {k: {"avg_ms": round(v[0] / v[1], 4), "launches": v[1],
+                   "GBps": round(8.0 * CURV_WORDS[k] * pts / (v[0] / v[1] / 1e3) / 1e9, 1)}
+               for k, v in prof.items() if k in CURV_WORDS}
+    return {"value": round(pts * steps / (t1 - t0) / 1e6, 2), "unit": "Mpts/s",
+            "ms_per_step": round(ms, 4), "steps": steps, "warmup": warmup, "grid": [n, n, n],
+            "dt": dt, "dtype": "f64", "mesh": f"triply periodic wavy mesh, amplitude {amp}",
+            "model_bytes_per_point": CURV_MODEL_BYTES,
+            "step_frac_of_hbm": round(CURV_MODEL_BYTES * pts / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "kernels": kernels, "finite": bool(np.isfinite(qe).all())}
 
 
 def run_ensemble(steps, nmembers=32, seed=0, autotune=True):
@@ -401,6 +466,8 @@ def main():
                     help="adjoint steps timed after a recorded forward run (0: skip)")
     ap.add_argument("--shear-steps", type=int, default=50,
                     help="timed steps of the stretched-grid shear layer (512x256x128, FP32-mixed; 0: skip)")
+    ap.add_argument("--curvilinear-steps", type=int, default=20,
+                    help="timed steps of the 256^3 box on the wavy curvilinear mesh (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=100,
                     help="steps of the 32-member forward/adjoint ensemble (0: skip)")
     ap.add_argument("--closure-steps", type=int, default=300,
@@ -440,6 +507,7 @@ def main():
                                                     / (v[0] / v[1] / 1e3) / 1e9, 1)}
                                for name, v in prof_a.items()}}
     shear = run_shear_layer(args.shear_steps, 2) if args.shear_steps > 0 else None
+    curv = run_curvilinear(args.curvilinear_steps, 2, n) if args.curvilinear_steps > 0 else None
     ensemble = run_ensemble(args.ensemble_steps) if args.ensemble_steps > 0 else None
     closure = run_closure(args.closure_steps) if args.closure_steps > 0 else None
     pts = float(n) ** 3
@@ -518,6 +586,7 @@ def main():
                        "speedup_vs_fp64": round(mpts32 / mpts64, 3), "err_vs_fp64": errors},
         "adjoint_fp64": adjoint,
         "shear_layer_fp32_mixed": shear,
+        "curvilinear_fp64": curv,
         "ensemble_fp64": ensemble,
         "closure_64_vs_256": closure,
         "roofline": roofline,

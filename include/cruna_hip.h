@@ -97,9 +97,13 @@ int cruna_hip_set_sponge_shape(cruna_hip_ctx *ctx, const double *shape);
 /* Deformed grid: X_metric(n1,n2,n3,3,3) and X_jacobian(n1,n2,n3) of data_geom
  * as grid_metric::calc_metric leaves them (src/geometry/grid_metric.f90:16-188);
  * from then on the direct RHS is that of ns_rho_rhou_p_grid_transform.f90:31-330,
- * as a case selects it through governing_equations.  Supported: diagonal
- * metrics whose entry (d,d) is constant along direction d (every
- * tensor-product stretch of the Cartesian grid); others return EINVAL. */
+ * as a case selects it through governing_equations.  Any metric: a diagonal
+ * one whose entry (d,d) is constant along direction d (a tensor-product
+ * stretch) runs the per-line sweeps, every other (general curvilinear grids)
+ * the sweeps that read the full metric per point.  Non-finite entries return
+ * EINVAL.  The adjoint RHS of ns_rho_rhou_p_grid_transform.f90:334-613 is
+ * the Cartesian one (it uses neither X_metric nor X_jacobian), so the
+ * adjoint entry points run unchanged on a deformed grid. */
 int cruna_hip_set_metric(cruna_hip_ctx *ctx, const double *X_metric, const double *X_jacobian);
 
 /* Dx1/Dx2/Dx3 on a 3-D field: diff_iN_mpi_cartesian::diff_3d

@@ -1075,7 +1075,10 @@ void orc_rhs_direct_gt(const orc_cfg *c, double *rhs, const double *q_in, const 
 }
 
 /* ------------------------------------------------------------------------ */
-/* Adjoint (continuous Euler adjoint) RHS, ns_rho_rhou_p.f90:257-536.       */
+/* Adjoint (continuous Euler adjoint) RHS, ns_rho_rhou_p.f90:257-536.  The  */
+/* grid-transform module's adjoint (ns_rho_rhou_p_grid_transform.f90:       */
+/* 334-613) is the same body, without X_metric or X_jacobian, so a deformed */
+/* grid uses this one as well.                                              */
 void orc_rhs_adjoint(const orc_cfg *c, double *rhs, const double *qs_in, const double *q,
                      const double *g, const double *shape) {
   const size_t N = npts(c);

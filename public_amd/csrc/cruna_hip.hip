@@ -15,6 +15,7 @@
 #include "line_ops.hpp"
 #include "boundary.hpp"
 #include "sweeps.hpp"
+#include "curvilinear.hpp"
 #include "adjoint.hpp"
 #include "closure.hpp"
 
@@ -107,6 +108,12 @@ struct cruna_hip_ctx {
   Metric<double> mt64{};
   Metric<float> mt32{};
   bool gt = false;
+  // general curvilinear grid: the full metric and Jacobian per point in `met`
+  // (gt is set as well), and the physical temperature gradient (3 fields)
+  bool curv = false;
+  CurvMetric<double> cm64{};
+  CurvMetric<float> cm32{};
+  void *tg = nullptr;
   // adjoint state qs^n and the forward trajectory Q(:,:,:,:,nt) (slot nt-1)
   double *qa = nullptr, *traj = nullptr, *gobj = nullptr;
   // device cells: the forward state of the current adjoint step and the
@@ -279,6 +286,20 @@ Metric<double> metric<double>(cruna_hip_ctx *c) { return c->mt64; }
 template <>
 Metric<float> metric<float>(cruna_hip_ctx *c) { return c->mt32; }
 
+template <typename T>
+CurvMetric<T> curv_metric(cruna_hip_ctx *c);
+template <>
+CurvMetric<double> curv_metric<double>(cruna_hip_ctx *c) { return c->cm64; }
+template <>
+CurvMetric<float> curv_metric<float>(cruna_hip_ctx *c) { return c->cm32; }
+
+// X_jacobian in the compute precision on a deformed grid, else null
+template <typename T>
+const T *jacobian(cruna_hip_ctx *c) {
+  if (!c->gt) return nullptr;
+  return c->curv ? curv_metric<T>(c).J : metric<T>(c).J;
+}
+
 int need_bc_inputs(cruna_hip_ctx *c) {
   for (auto &r : c->bc) {
     if (!c->qref) return fail(CRUNA_HIP_EINVAL, "boundary rows need qref (cruna_hip_set_qref)");
@@ -316,7 +337,7 @@ int bc_rhs(cruna_hip_ctx *c, T *rhs, const Q *q) {
     if (r.first == 400) {
       hipLaunchKernelGGL((k_sponge_rhs<T, Q>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream,
                          rhs, q, (const double *)c->qref, (const double *)c->shape, c->N,
-                         c->cfg.gamma - 1.0, c->gt ? metric<T>(c).J : (const T *)nullptr);
+                         c->cfg.gamma - 1.0, jacobian<T>(c));
       if (int rc = check_launch()) return rc;
     }
   return CRUNA_HIP_OK;
@@ -339,9 +360,24 @@ void launch_sweep1_gt(cruna_hip_ctx *c, const Q *in) {
 }
 
 template <typename T, typename Q, int DIR>
+void launch_curv_sweep1(cruna_hip_ctx *c, const Q *in) {
+  const dim3 grid(c->geom[DIR].ntiles), block(c->wg[DIR]);
+  T *rhs = (T *)c->rhs, *G = (T *)c->G, *TG = (T *)c->tg;
+  const LineOp<T> &op = dops<T>(c)[DIR].op;
+  if (c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC)
+    hipLaunchKernelGGL((k_curv_sweep1<T, Q, DIR, true>), grid, block, 0, c->stream, in, rhs, G, TG, op,
+                       c->geom[DIR], c->phys, c->N, curv_metric<T>(c));
+  else
+    hipLaunchKernelGGL((k_curv_sweep1<T, Q, DIR, false>), grid, block, 0, c->stream, in, rhs, G, TG, op,
+                       c->geom[DIR], c->phys, c->N, curv_metric<T>(c));
+}
+
+template <typename T, typename Q, int DIR>
 int launch_sweep1(cruna_hip_ctx *c, const Q *in) {
   ProfScope p(c, DIR == 0 ? KID_S1X : (DIR == 1 ? KID_S2Y : KID_S3Z));
-  if (c->gt)
+  if (c->curv)
+    launch_curv_sweep1<T, Q, DIR>(c, in);
+  else if (c->gt)
     launch_sweep1_gt<T, Q, DIR, true>(c, in);
   else
     launch_sweep1_gt<T, Q, DIR, false>(c, in);
@@ -377,10 +413,41 @@ void launch_sweep2_gt(cruna_hip_ctx *c, int mode, bool fuse_filter) {
   }
 }
 
+template <typename T, int DIR, bool PER, bool FF>
+void launch_curv_sweep2_k(cruna_hip_ctx *c, int mode) {
+  const dim3 grid(c->geom[DIR].ntiles), block(c->wg[DIR]);
+  const T *force = (const T *)(DIR == 0 && c->cnn ? c->force : nullptr);
+  hipLaunchKernelGGL((k_curv_sweep2<T, DIR, PER, FF>), grid, block, 0, c->stream, (T *)c->rhs,
+                     (const T *)c->G, (const T *)c->tg, dops<T>(c)[DIR].op, fops<T>(c)[DIR].op,
+                     c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, (T *)c->qs, c->cfg.dt,
+                     curv_metric<T>(c), force);
+}
+
+template <typename T, int DIR>
+void launch_curv_sweep2(cruna_hip_ctx *c, int mode, bool fuse_filter) {
+  const bool per = c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC;
+  // the fused x filter exists on the last stage of the x sweep only
+  if constexpr (DIR == 0) {
+    if (fuse_filter && mode == 3) {
+      if (per)
+        launch_curv_sweep2_k<T, DIR, true, true>(c, mode);
+      else
+        launch_curv_sweep2_k<T, DIR, false, true>(c, mode);
+      return;
+    }
+  }
+  if (per)
+    launch_curv_sweep2_k<T, DIR, true, false>(c, mode);
+  else
+    launch_curv_sweep2_k<T, DIR, false, false>(c, mode);
+}
+
 template <typename T, int DIR>
 int launch_sweep2(cruna_hip_ctx *c, int mode, bool fuse_filter) {
   ProfScope p(c, DIR == 2 ? KID_S4Z : (DIR == 1 ? KID_S5Y : KID_S6X));
-  if (c->gt)
+  if (c->curv)
+    launch_curv_sweep2<T, DIR>(c, mode, fuse_filter);
+  else if (c->gt)
     launch_sweep2_gt<T, DIR, true>(c, mode, fuse_filter);
   else
     launch_sweep2_gt<T, DIR, false>(c, mode, fuse_filter);
@@ -653,6 +720,25 @@ int upload_metric(cruna_hip_ctx *c, const std::vector<double> &m1, const std::ve
   return CRUNA_HIP_OK;
 }
 
+// The full metric (9 fields, X_metric's order) and the Jacobian per point in
+// the compute precision, and the temperature-gradient fields of the
+// curvilinear sweeps.
+template <typename T>
+int upload_curv_metric(cruna_hip_ctx *c, const double *Xm, const double *Xj, CurvMetric<T> &out) {
+  const size_t N = c->N;
+  std::vector<T> h(10 * N);
+  for (size_t i = 0; i < 9 * N; ++i) h[i] = T(Xm[i]);
+  for (size_t i = 0; i < N; ++i) h[9 * N + i] = T(Xj[i]);
+  if (c->met) HIP_TRY(hipFree(c->met));
+  c->met = nullptr;
+  HIP_TRY(hipMalloc(&c->met, h.size() * sizeof(T)));
+  HIP_TRY(hipMemcpy(c->met, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (!c->tg) HIP_TRY(hipMalloc(&c->tg, 3 * N * sizeof(T)));
+  out.M = (const T *)c->met;
+  out.J = (const T *)c->met + 9 * N;
+  return CRUNA_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -793,7 +879,7 @@ int cruna_hip_destroy(cruna_hip_ctx *c) {
   }
   for (void *p : {(void *)c->qn, (void *)c->acc, (void *)c->qref, (void *)c->shape, (void *)c->tmp,
                   (void *)c->qa, (void *)c->traj, (void *)c->gobj, c->met, (void *)c->q0p, c->force,
-                  (void *)c->ckpt,
+                  (void *)c->ckpt, c->tg,
                   c->qs, c->rhs, c->G})
     if (p) (void)hipFree(p);
   for (auto &p : c->pending) {
@@ -842,38 +928,54 @@ int cruna_hip_set_metric(cruna_hip_ctx *c, const double *X_metric, const double 
     return X_metric[(size_t)N * (k + 3 * j) + i + n1 * (jj + n2 * kk)];
   };
   double scale = 0.0;
-  for (int d = 0; d < 3; ++d)
-    for (long p = 0; p < N; ++p) scale = std::max(scale, std::fabs(X_metric[(size_t)N * (4 * d) + p]));
-  if (!(scale > 0.0)) return fail(CRUNA_HIP_EINVAL, "metric with a zero diagonal");
-  for (int k = 0; k < 3; ++k)
-    for (int j = 0; j < 3; ++j) {
+  for (size_t p = 0; p < 9 * (size_t)N; ++p) {
+    if (!std::isfinite(X_metric[p])) return fail(CRUNA_HIP_EINVAL, "X_metric is not finite");
+    scale = std::max(scale, std::fabs(X_metric[p]));
+  }
+  for (long p = 0; p < N; ++p)
+    if (!std::isfinite(X_jacobian[p])) return fail(CRUNA_HIP_EINVAL, "X_jacobian is not finite");
+  if (!(scale > 0.0)) return fail(CRUNA_HIP_EINVAL, "zero metric");
+  // A diagonal metric whose entries M(d,d) are constant along the lines of
+  // direction d (every tensor-product stretch) takes the per-line sweeps;
+  // any other takes the general curvilinear sweeps with the metric per point.
+  bool tensor = true;
+  for (int k = 0; k < 3 && tensor; ++k)
+    for (int j = 0; j < 3 && tensor; ++j) {
       if (k == j) continue;
       for (long p = 0; p < N; ++p)
-        if (std::fabs(X_metric[(size_t)N * (k + 3 * j) + p]) > 1e-12 * scale)
-          return fail(CRUNA_HIP_EINVAL,
-                      "X_metric has off-diagonal entries: general curvilinear grids are not "
-                      "supported on the device (tensor-product stretches are)");
+        if (std::fabs(X_metric[(size_t)N * (k + 3 * j) + p]) > 1e-12 * scale) {
+          tensor = false;
+          break;
+        }
     }
-  // M(d,d) must be constant along the lines of direction d
   std::vector<double> m1((size_t)n2 * n3), m2((size_t)n1 * n3), m3((size_t)n1 * n2);
-  for (long kk = 0; kk < n3; ++kk)
-    for (long jj = 0; jj < n2; ++jj)
+  for (long kk = 0; kk < n3 && tensor; ++kk)
+    for (long jj = 0; jj < n2 && tensor; ++jj)
       for (long i = 0; i < n1; ++i) {
         const double a = M(0, 0, i, jj, kk), b = M(1, 1, i, jj, kk), e = M(2, 2, i, jj, kk);
         const double a0 = M(0, 0, 0, jj, kk), b0 = M(1, 1, i, 0, kk), e0 = M(2, 2, i, jj, 0);
         if (std::fabs(a - a0) > 1e-13 * scale || std::fabs(b - b0) > 1e-13 * scale ||
-            std::fabs(e - e0) > 1e-13 * scale)
-          return fail(CRUNA_HIP_EINVAL,
-                      "X_metric(d,d) varies along direction d: only tensor-product stretched "
-                      "grids are supported on the device");
+            std::fabs(e - e0) > 1e-13 * scale) {
+          tensor = false;
+          break;
+        }
         m1[jj + n2 * kk] = a0;
         m2[i + n1 * kk] = b0;
         m3[i + n1 * jj] = e0;
       }
-  const int rc = c->cfg.precision == CRUNA_HIP_FP32_MIXED
-                     ? upload_metric<float>(c, m1, m2, m3, X_jacobian, c->mt32)
-                     : upload_metric<double>(c, m1, m2, m3, X_jacobian, c->mt64);
-  if (rc) return rc;
+  const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
+  c->gt = false;
+  c->curv = false;
+  if (tensor) {
+    const int rc = mixed ? upload_metric<float>(c, m1, m2, m3, X_jacobian, c->mt32)
+                         : upload_metric<double>(c, m1, m2, m3, X_jacobian, c->mt64);
+    if (rc) return rc;
+  } else {
+    const int rc = mixed ? upload_curv_metric<float>(c, X_metric, X_jacobian, c->cm32)
+                         : upload_curv_metric<double>(c, X_metric, X_jacobian, c->cm64);
+    if (rc) return rc;
+    c->curv = true;
+  }
   c->gt = true;
   return CRUNA_HIP_OK;
 }
@@ -970,7 +1072,6 @@ int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const
                           const double *g, double t) {
   (void)t;  // the adjoint RHS of the reference does not depend on t
   if (!c || !rhs || !qs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
-  if (c->gt) return fail(CRUNA_HIP_EINVAL, "the deformed-grid adjoint is not implemented on the device");
   if (int rc = stage_adjoint_inputs(c, qs, q, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
   if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
@@ -997,7 +1098,6 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *
                                    const double *q0, const double *g, double t) {
   (void)t;
   if (!c || !qs1 || !qs0 || !q0) return fail(CRUNA_HIP_EINVAL, "null argument");
-  if (c->gt) return fail(CRUNA_HIP_EINVAL, "the deformed-grid adjoint is not implemented on the device");
   if (int rc = stage_adjoint_inputs(c, qs0, q0, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
   if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
@@ -1070,7 +1170,6 @@ int cruna_hip_trajectory_record(cruna_hip_ctx *c, int on) {
 
 int cruna_hip_run_adjoint(cruna_hip_ctx *c, int nt_end, int nsteps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
-  if (c->gt) return fail(CRUNA_HIP_EINVAL, "the deformed-grid adjoint is not implemented on the device");
   if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
   if (!c->qa) return fail(CRUNA_HIP_EINVAL, "no adjoint state (cruna_hip_set_adjoint_state)");
   if (c->traj_slots == 0) return fail(CRUNA_HIP_EINVAL, "no trajectory (cruna_hip_trajectory_alloc)");
@@ -1359,7 +1458,6 @@ int cruna_hip_ensemble_set_role(cruna_hip_ensemble *e, int i, int role, int nt) 
   if (role != CRUNA_HIP_FORWARD && role != CRUNA_HIP_ADJOINT) return fail(CRUNA_HIP_EINVAL, "bad role");
   cruna_hip_ctx *c = e->m[i];
   if (role == CRUNA_HIP_ADJOINT) {
-    if (c->gt) return fail(CRUNA_HIP_EINVAL, "the deformed-grid adjoint is not implemented on the device");
     if (!c->qa || c->traj_slots == 0)
       return fail(CRUNA_HIP_EINVAL, "an adjoint member needs its adjoint state and trajectory first");
     if (c->traj_slots > 1 && (nt < 1 || nt > c->traj_slots))

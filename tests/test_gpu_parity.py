@@ -492,22 +492,158 @@ def test_gt_fp32_mixed_close_to_fp64():
     assert err < 1e-5
 
 
-def test_set_metric_rejects_general_curvilinear():
+def _wavy(n, amp=(0.15, 0.2, 0.1, 0.05), ly=8.0):
+    """A general curvilinear grid: x = xi + a sin(zeta) cos(eta/2),
+    y = eta + b sin(xi) sin(zeta), z = zeta + c sin(xi) + e eta (x, z
+    periodic on [0, 2pi), y open); every metric entry varies everywhere."""
+    a, b, c, e = amp
+    n1, n2, n3 = n
+    dx = (2 * np.pi / n1, ly / (n2 - 1), 2 * np.pi / n3)
+    xi = np.arange(n1) * dx[0]
+    eta = -ly / 2 + np.arange(n2) * dx[1]
+    zeta = np.arange(n3) * dx[2]
+    Xc = np.stack(np.meshgrid(xi, eta, zeta, indexing="ij"), axis=-1)
+    s, t, u = Xc[..., 0], Xc[..., 1], Xc[..., 2]
+    X = np.stack([s + a * np.sin(u) * np.cos(0.5 * t), t + b * np.sin(s) * np.sin(u),
+                  u + c * np.sin(s) + e * t], axis=-1)
+    return dx, np.asfortranarray(X), np.asfortranarray(Xc)
+
+
+@pytest.mark.parametrize("visc", [O.SUTHERLAND, O.POWER])
+def test_rhs_direct_curvilinear_matches_oracle(visc):
+    """General curvilinear grid (full metric per point): the curvilinear
+    sweeps against ns_rho_rhou_p_grid_transform's restatement."""
+    Cruna = _gpu()
+    n = (32, 40, 16)
+    dx, X, Xc = _wavy(n)
+    scheme = (O.PERIODIC, O.OPEN, O.PERIODIC)
+    kw = dict(mu0=0.02, T0=1.0 / 1.4, dt=1e-3, scheme=scheme, visc_law=visc)
+    Xm, Xj = O.calc_metric(O.Cfg(n, dx, **kw), X, Xc)
+    assert np.abs(Xm[..., 0, 2]).max() > 0.1 and np.abs(Xm[..., 2, 0]).max() > 0.05
+    q = _gt_state(X, 1)
+    ref = O.rhs_direct_gt(O.Cfg(n, dx, **kw), q, Xm, Xj)
+    with Cruna(n, dx, **kw) as g:
+        g.set_metric(Xm, Xj)
+        got = g.right_hand_side_direct_3d(q)
+    for v in range(5):
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+
+
+@pytest.mark.parametrize("fuse", [True, False])
+def test_run_direct_curvilinear_with_boundaries(fuse):
+    """Wavy grid, RK4 + filters against the oracle.  fuse=True: a sponge on
+    the RHS (400) only, so the x filter runs inside the last stage's x
+    sweep; fuse=False: characteristic rows on both y faces and a sponge on
+    q (401), q-type rows that keep the x filter a pass of its own."""
+    Cruna = _gpu()
+    n = (32, 48, 16)
+    dx, X, Xc = _wavy(n)
+    n1, n2, n3 = n
+    if fuse:
+        bc = [(400, 1, n1, 1, n2, 1, n3)]
+    else:
+        bc = [(201, 1, n1, 1, 1, 1, n3, 0, -1, 0),
+              (201, 1, n1, n2, n2, 1, n3, 0, 1, 0),
+              (401, 1, n1, 1, n2, 1, n3)]
+    scheme = (O.PERIODIC, O.OPEN, O.PERIODIC)
+    kw = dict(mu0=2e-3, T0=1.0 / 1.4, dt=2e-3, scheme=scheme, bc=bc)
+    c0 = O.Cfg(n, dx, filter_freq=(1, 1, 1), **kw)
+    Xm, Xj = O.calc_metric(c0, X, Xc)
+    shape_fn = O.sponge_shape(c0, (0, -4.0, 0), (2 * np.pi, 4.0, 2 * np.pi), (-1.0, 1.5, -1.0), 1.0,
+                              np.asfortranarray(Xc[:, 0, 0, 0]), np.asfortranarray(Xc[0, :, 0, 1]),
+                              np.asfortranarray(Xc[0, 0, :, 2]))
+    q = _gt_state(X, 2)
+    qref = q.copy(order="F")
+    nsteps = 4
+    ref = O.run_direct(O.Cfg(n, dx, filter_freq=(1, 1, 1), metric=(Xm, Xj), **kw), q, nsteps,
+                       qref=qref, shape=shape_fn)
+    with Cruna(n, dx, filter_freq=(1, 1, 1), **kw) as g:
+        g.set_metric(Xm, Xj)
+        g.set_qref(qref)
+        g.set_sponge_shape(shape_fn)
+        g.set_state(q)
+        g.run_direct(nsteps)
+        g.synchronize()
+        got = g.get_state()
+    for v in range(5):
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+
+
+def test_curvilinear_fp32_mixed_close_to_fp64():
+    from public_amd import FP32_MIXED
+    Cruna = _gpu()
+    n = (32, 48, 16)
+    dx, X, Xc = _wavy(n)
+    scheme = (O.PERIODIC, O.OPEN, O.PERIODIC)
+    kw = dict(mu0=2e-3, T0=1.0 / 1.4, dt=2e-3, scheme=scheme)
+    Xm, Xj = O.calc_metric(O.Cfg(n, dx, **kw), X, Xc)
+    q = _gt_state(X, 3)
+    out = []
+    for prec in (0, FP32_MIXED):
+        with Cruna(n, dx, precision=prec, **kw) as g:
+            g.set_metric(Xm, Xj)
+            g.set_state(q)
+            g.run_direct(10)
+            g.synchronize()
+            out.append(g.get_state())
+    err = rel(out[1], out[0])
+    print(f"curvilinear grid fp32-mixed vs fp64 after 10 steps: {err:.3e}")
+    assert err < 1e-5
+
+
+def test_set_metric_rejects_nonfinite():
     from public_amd.cruna import CrunaError
     Cruna = _gpu()
     n = (16, 16, 16)
-    dx, X, Xc = _stretched(n)
+    dx, X, Xc = _wavy(n)
     kw = dict(scheme=(O.PERIODIC, O.OPEN, O.PERIODIC))
     Xm, Xj = O.calc_metric(O.Cfg(n, dx, **kw), X, Xc)
-    Xm = Xm.copy(order="F")
-    Xm[..., 0, 1] = 0.1  # a skewed grid
+    bad = Xm.copy(order="F")
+    bad[3, 4, 5, 1, 2] = np.nan
     with Cruna(n, dx, **kw) as g:
-        with pytest.raises(CrunaError, match="off-diagonal"):
-            g.set_metric(Xm, Xj)
-        with pytest.raises(CrunaError, match="not implemented"):
-            Xm2, _ = O.calc_metric(O.Cfg(n, dx, **kw), X, Xc)
-            g.set_metric(Xm2, Xj)
-            g.time_stepper_adjoint(np.zeros(n + (5,)), _gt_state(X, 0))
+        with pytest.raises(CrunaError, match="not finite"):
+            g.set_metric(bad, Xj)
+
+
+def test_adjoint_on_curvilinear_grid_matches_oracle():
+    """The grid-transform module's adjoint RHS is the Cartesian one
+    (ns_rho_rhou_p_grid_transform.f90:334-613 uses no metric): on the wavy
+    grid, the per-call adjoint RHS and an adjoint run over a trajectory the
+    curvilinear forward run recorded on the device, against the oracle."""
+    Cruna = _gpu()
+    n = (32, 40, 16)
+    dx, X, Xc = _wavy(n)
+    scheme = (O.PERIODIC, O.OPEN, O.PERIODIC)
+    kw = dict(mu0=2e-3, T0=1.0 / 1.4, dt=2e-3, scheme=scheme)
+    cfg = O.Cfg(n, dx, filter_freq=(1, 1, 1), afilter_freq=(1, 1, 1), **kw)
+    Xm, Xj = O.calc_metric(cfg, X, Xc)
+    cfg_m = O.Cfg(n, dx, filter_freq=(1, 1, 1), afilter_freq=(1, 1, 1), metric=(Xm, Xj), **kw)
+    q0 = _gt_state(X, 4)
+    qa = np.asfortranarray(np.random.default_rng(5).standard_normal(n + (5,)))
+    nsteps = 3
+    Q = np.zeros(n + (5, nsteps), order="F")
+    q = q0
+    for nt in range(1, nsteps + 1):
+        q = O.run_direct(cfg_m, q, 1, nt0=nt - 1)
+        Q[..., nt - 1] = q
+    ref = O.run_adjoint(cfg_m, qa, Q)
+    ref_r = O.rhs_adjoint(cfg_m, qa, q0)
+    with Cruna(n, dx, filter_freq=(1, 1, 1), afilter_freq=(1, 1, 1), **kw) as g:
+        g.set_metric(Xm, Xj)
+        r = g.right_hand_side_adjoint_3d(qa, q0)
+        g.trajectory_alloc(nsteps)
+        g.trajectory_record(True)
+        g.set_state(q0)
+        g.run_direct(nsteps)
+        g.synchronize()
+        assert rel(g.trajectory_get(nsteps), Q[..., -1]) < 1e-12
+        g.set_adjoint_state(qa)
+        g.run_adjoint(nsteps)
+        g.synchronize()
+        got = g.get_adjoint_state()
+    for v in range(5):
+        assert rel(r[..., v], ref_r[..., v]) < 1e-12, (v, rel(r[..., v], ref_r[..., v]))
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
 
 
 def test_adjoint_subsets_with_checkpoints_matches_oracle():
