@@ -106,6 +106,18 @@ int cruna_hip_set_sponge_shape(cruna_hip_ctx *ctx, const double *shape);
  * adjoint entry points run unchanged on a deformed grid. */
 int cruna_hip_set_metric(cruna_hip_ctx *ctx, const double *X_metric, const double *X_jacobian);
 
+/* Volume penalisation: phi(n1,n2,n3) and inv_phi(n1,n2,n3), the case's
+ * volume_penalization penalization(:,:,:,1) and (:,:,:,2) (porosity and the
+ * stored 1/phi, src/immersed/volume_penalization.f90 and the case's
+ * init_volume_penalization); from then on the direct RHS is that of
+ * ns_rho_rhou_p_ev_phi.f90:33-291, as a case selects it through
+ * governing_equations, with the reference's no-op apply_volume_penalization.
+ * Both null: back to ns_rho_rhou_p.  EINVAL: non-finite values, a deformed
+ * grid (ns_rho_rhou_p_ev_phi_grid_transform is not built), and afterwards
+ * every adjoint entry point (the reference's ev_phi adjoint stops,
+ * ns_rho_rhou_p_ev_phi.f90:295-309). */
+int cruna_hip_set_penalization(cruna_hip_ctx *ctx, const double *phi, const double *inv_phi);
+
 /* Dx1/Dx2/Dx3 on a 3-D field: diff_iN_mpi_cartesian::diff_3d
  * (src/diff/diff_i1_mpi_cartesian.f90:83-103).  dir = 0,1,2; h = spacing
  * (pass 1.0 for the optional-h default). */

@@ -740,6 +740,10 @@ void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const dou
     orc_rhs_direct_gt(c, rhs, q_in, c->X_metric, c->X_jacobian, qref, shape);
     return;
   }
+  if (c->penal_phi) { /* governing_equations => ns_rho_rhou_p_ev_phi */
+    orc_rhs_direct_ev(c, rhs, q_in, c->penal_phi, c->penal_inv_phi, qref, shape);
+    return;
+  }
   const size_t N = npts(c);
   const double dx1 = c->dx[0], dx2 = c->dx[1], dx3 = c->dx[2];
   const double hx1 = heat_unit_spacing ? 1.0 : dx1;
@@ -1072,6 +1076,107 @@ void orc_rhs_direct_gt(const orc_cfg *c, double *rhs, const double *q_in, const 
     for (int j = 0; j < 3; ++j) free(g[i][j]);
   free(t11); free(t12); free(t13); free(t22); free(t23); free(t33);
   free(temp); free(mu); free(mudiv); free(lam); free(tmp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Volume-penalised direct RHS, src/equations/ns_rho_rhou_p_ev_phi.f90:33-291:
+ * the fluxes carry the porosity phi inside the derivatives, the divergences
+ * are multiplied by div_1_phi (the stored 1/phi), the pressure gradient,
+ * u.grad(p) and tau:grad(u) are not; both heat-flux derivatives take no h
+ * (:212-220).  apply_volume_penalization (:237) is the reference's no-op
+ * (src/immersed/volume_penalization.f90), force a no-op, then the BC rows. */
+void orc_rhs_direct_ev(const orc_cfg *c, double *rhs, const double *q_in, const double *phi,
+                       const double *inv_phi, const double *qref, const double *shape) {
+  const size_t N = npts(c);
+  const double dxs[3] = {c->dx[0], c->dx[1], c->dx[2]};
+  double *q = xalloc(5 * N), *u = xalloc(3 * N), *pru = xalloc(3 * N), *php = xalloc(N);
+  double *a[3] = {xalloc(N), xalloc(N), xalloc(N)};
+  double *pg[3] = {xalloc(N), xalloc(N), xalloc(N)};
+  double *g[3][3]; /* g[i][j] = d(u_i)/dx_j */
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) g[i][j] = xalloc(N);
+  double *t11 = xalloc(N), *t12 = xalloc(N), *t13 = xalloc(N);
+  double *t22 = xalloc(N), *t23 = xalloc(N), *t33 = xalloc(N);
+  double *tau[3][3] = {{t11, t12, t13}, {t12, t22, t23}, {t13, t23, t33}};
+  double *temp = xalloc(N), *mu = xalloc(N), *mudiv = xalloc(N), *plam = xalloc(N);
+  double *tmp = xalloc(N);
+  double *r[5];
+  for (int v = 0; v < 5; ++v) r[v] = rhs + v * N;
+
+  memcpy(q, q_in, 5 * N * sizeof(double)); /* :99 */
+  orc_bc_q(c, q, qref, shape);              /* :102 */
+  for (int m = 0; m < 3; ++m)               /* :105-117 */
+    for (size_t i = 0; i < N; ++i) {
+      u[m * N + i] = q[(m + 1) * N + i] / q[i];
+      pru[m * N + i] = phi[i] * q[(m + 1) * N + i];
+    }
+  for (size_t i = 0; i < N; ++i) php[i] = phi[i] * q[4 * N + i];
+  for (int m = 0; m < 3; ++m) /* :121-131 */
+    for (int k = 0; k < 3; ++k) orc_diff3d(c, k, g[m][k], u + m * N, dxs[k]);
+  for (size_t i = 0; i < N; ++i) { /* :134-147 */
+    temp[i] = orc_T(c, q[i], q[4 * N + i]);
+    mu[i] = orc_mu(c, temp[i]);
+    mudiv[i] = -2.0 / 3.0 * mu[i] * (g[0][0][i] + g[1][1][i] + g[2][2][i]);
+    t11[i] = mudiv[i] + 2 * mu[i] * g[0][0][i];
+    t22[i] = mudiv[i] + 2 * mu[i] * g[1][1][i];
+    t33[i] = mudiv[i] + 2 * mu[i] * g[2][2][i];
+    t12[i] = mu[i] * (g[0][1][i] + g[1][0][i]);
+    t13[i] = mu[i] * (g[0][2][i] + g[2][0][i]);
+    t23[i] = mu[i] * (g[1][2][i] + g[2][1][i]);
+  }
+  for (int k = 0; k < 3; ++k) orc_diff3d(c, k, pg[k], q + 4 * N, dxs[k]); /* :149-151 */
+  /* mass :153-159 */
+  for (int k = 0; k < 3; ++k) orc_diff3d(c, k, a[k], pru + k * N, dxs[k]);
+  for (size_t i = 0; i < N; ++i) {
+    r[0][i] = -a[0][i] - a[1][i] - a[2][i];
+    r[0][i] = inv_phi[i] * r[0][i];
+  }
+  /* momentum :161-201 */
+  for (int m = 0; m < 3; ++m) {
+    for (int k = 0; k < 3; ++k) {
+      for (size_t i = 0; i < N; ++i) tmp[i] = pru[k * N + i] * u[m * N + i];
+      orc_diff3d(c, k, a[k], tmp, dxs[k]);
+    }
+    for (size_t i = 0; i < N; ++i) r[m + 1][i] = -a[0][i] - a[1][i] - a[2][i];
+    for (int k = 0; k < 3; ++k) {
+      for (size_t i = 0; i < N; ++i) tmp[i] = phi[i] * tau[m][k][i];
+      orc_diff3d(c, k, a[k], tmp, dxs[k]);
+    }
+    for (size_t i = 0; i < N; ++i) {
+      r[m + 1][i] = r[m + 1][i] + a[0][i] + a[1][i] + a[2][i];
+      r[m + 1][i] = inv_phi[i] * r[m + 1][i] - pg[m][i];
+    }
+  }
+  /* pressure :203-234 */
+  for (int k = 0; k < 3; ++k) {
+    for (size_t i = 0; i < N; ++i) tmp[i] = php[i] * u[k * N + i];
+    orc_diff3d(c, k, a[k], tmp, dxs[k]);
+  }
+  const double gt = c->gamma / (c->gamma - 1.0);
+  for (size_t i = 0; i < N; ++i) r[4][i] = gt * (-a[0][i] - a[1][i] - a[2][i]);
+  for (int k = 0; k < 3; ++k) orc_diff3d(c, k, a[k], temp, 1.0); /* d T, no h */
+  for (size_t i = 0; i < N; ++i) plam[i] = phi[i] * orc_lambda(c, mu[i]);
+  for (int k = 0; k < 3; ++k) {
+    for (size_t i = 0; i < N; ++i) tmp[i] = plam[i] * a[k][i];
+    orc_diff3d(c, k, a[k], tmp, 1.0);
+  }
+  for (size_t i = 0; i < N; ++i) {
+    r[4][i] = r[4][i] + a[0][i] + a[1][i] + a[2][i];
+    r[4][i] = inv_phi[i] * r[4][i];
+    r[4][i] = r[4][i] + u[i] * pg[0][i] + u[N + i] * pg[1][i] + u[2 * N + i] * pg[2][i];
+    r[4][i] = r[4][i] + t11[i] * g[0][0][i] + t12[i] * g[0][1][i] + t13[i] * g[0][2][i] +
+              t12[i] * g[1][0][i] + t22[i] * g[1][1][i] + t23[i] * g[1][2][i] +
+              t13[i] * g[2][0][i] + t23[i] * g[2][1][i] + t33[i] * g[2][2][i];
+    r[4][i] = (c->gamma - 1.0) * r[4][i];
+  }
+  orc_bc_rhs(c, rhs, q, qref, shape); /* :237-246 */
+
+  free(q); free(u); free(pru); free(php);
+  for (int k = 0; k < 3; ++k) { free(a[k]); free(pg[k]); }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) free(g[i][j]);
+  free(t11); free(t12); free(t13); free(t22); free(t23); free(t33);
+  free(temp); free(mu); free(mudiv); free(plam); free(tmp);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -50,6 +50,11 @@ typedef struct orc_cfg {
    * diff_pen_core_o4_e_c_{p,o}; ORC_DIFF_O4_PADE: diff_pen_core_o4_i_c_{p,o}_pade.
    * The filter is the o6 one either way. */
   int diff_kind[3];
+  /* Volume penalisation (volume_penalization's penalization(:,:,:,1) = phi
+   * and (:,:,:,2) = 1/phi as the case stores them); when set (and X_metric
+   * is not) the direct RHS is ns_rho_rhou_p_ev_phi's.  NULL: plain NS. */
+  const double *penal_phi;
+  const double *penal_inv_phi;
 } orc_cfg;
 
 #define ORC_DIFF_O6_COMPACT 0
@@ -81,6 +86,9 @@ double orc_lambda(const orc_cfg *c, double mu);
  * X_metric(n1,n2,n3,3,3) (metric without the Jacobian) and X_jacobian. */
 void orc_calc_metric(const orc_cfg *c, const double *X, const double *X_calc, double *X_metric,
                      double *X_jacobian);
+/* Direct RHS with volume penalisation, src/equations/ns_rho_rhou_p_ev_phi.f90:31-236. */
+void orc_rhs_direct_ev(const orc_cfg *c, double *rhs, const double *q_in, const double *phi,
+                       const double *inv_phi, const double *qref, const double *shape);
 
 /* Direct RHS on a deformed grid, ns_rho_rhou_p_grid_transform.f90:31-330. */
 void orc_rhs_direct_gt(const orc_cfg *c, double *rhs, const double *q_in,

@@ -42,6 +42,8 @@ class OrcCfg(C.Structure):
         ("X_metric", C.c_void_p),
         ("X_jacobian", C.c_void_p),
         ("diff_kind", C.c_int * 3),
+        ("penal_phi", C.c_void_p),
+        ("penal_inv_phi", C.c_void_p),
     ]
 
 
@@ -79,6 +81,7 @@ def lib():
         L.orc_rhs_direct.argtypes = [P, _dp, _dp, _vp, _vp, C.c_int]
         L.orc_calc_metric.argtypes = [P, _dp, _dp, _dp, _dp]
         L.orc_rhs_direct_gt.argtypes = [P, _dp, _dp, _dp, _dp, _vp, _vp]
+        L.orc_rhs_direct_ev.argtypes = [P, _dp, _dp, _dp, _dp, _vp, _vp]
         L.orc_rhs_adjoint.argtypes = [P, _dp, _dp, _dp, _vp, _vp]
         L.orc_bc_q.argtypes = [P, _dp, _vp, _vp]
         L.orc_bc_qs.argtypes = [P, _dp, _dp, _vp]
@@ -125,10 +128,12 @@ class Cfg:
     def __init__(self, n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0,
                  W=1.0, Pr=0.71, mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76,
                  dt=1e-3, filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), bc=(), metric=None,
-                 diff_kind=(0, 0, 0)):
+                 diff_kind=(0, 0, 0), penalization=None):
         """metric: (X_metric (n1,n2,n3,3,3), X_jacobian (n1,n2,n3)) of a deformed
         grid; the direct RHS is then the grid-transform one.  diff_kind per
-        direction: DIFF_O6_COMPACT, DIFF_O4_EXPLICIT or DIFF_O4_PADE."""
+        direction: DIFF_O6_COMPACT, DIFF_O4_EXPLICIT or DIFF_O4_PADE.
+        penalization: (phi, 1/phi) (n1,n2,n3) each; the direct RHS is then
+        ns_rho_rhou_p_ev_phi's."""
         c = OrcCfg()
         for d in range(3):
             c.n[d] = int(n[d])
@@ -151,6 +156,13 @@ class Cfg:
             self._metric = (Xm, Xj)  # kept alive for the pointers below
             c.X_metric = Xm.ctypes.data
             c.X_jacobian = Xj.ctypes.data
+        self._penal = None
+        if penalization is not None:
+            ph, ip = _f(penalization[0]), _f(penalization[1])
+            assert ph.shape == tuple(n) and ip.shape == tuple(n)
+            self._penal = (ph, ip)
+            c.penal_phi = ph.ctypes.data
+            c.penal_inv_phi = ip.ctypes.data
         self.c = c
 
     @property

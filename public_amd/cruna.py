@@ -77,6 +77,7 @@ def lib():
         L.cruna_hip_set_qref.argtypes = [vp, dp]
         L.cruna_hip_set_sponge_shape.argtypes = [vp, dp]
         L.cruna_hip_set_metric.argtypes = [vp, dp, dp]
+        L.cruna_hip_set_penalization.argtypes = [vp, vp, vp]
         L.cruna_hip_autotune.argtypes = [vp, ip, C.POINTER(C.c_int)]
         L.cruna_hip_ensemble_create.argtypes = [C.POINTER(Cfg), ip, C.POINTER(vp)]
         L.cruna_hip_ensemble_destroy.argtypes = [vp]
@@ -126,7 +127,7 @@ def lib():
                   "filter", "rhs_direct", "rhs_adjoint", "time_stepper_direct", "time_stepper_adjoint",
                   "run_direct", "synchronize", "profile_enable", "profile_read", "profile_count",
                   "set_adjoint_state", "get_adjoint_state", "trajectory_alloc", "trajectory_set",
-                  "trajectory_get", "trajectory_record", "run_adjoint", "set_metric", "autotune",
+                  "trajectory_get", "trajectory_record", "run_adjoint", "set_metric", "set_penalization", "autotune",
                   "ensemble_create", "ensemble_destroy", "ensemble_member", "ensemble_set_role",
                   "ensemble_run", "ensemble_autotune", "cnn_create", "cnn_destroy", "cnn_init_params",
                   "cnn_set_params", "cnn_get_params", "cnn_set_norm", "cnn_forward", "cnn_gradients",
@@ -303,6 +304,16 @@ class Cruna:
             raise CrunaError(f"X_metric shape {Xm.shape} does not match {self.shape3 + (3, 3)}")
         Xj = _f64(X_jacobian, self.shape3)
         _check(lib().cruna_hip_set_metric(self._h, _dp(Xm), _dp(Xj)))
+
+    def set_penalization(self, phi, inv_phi=None):
+        """Volume penalisation (ns_rho_rhou_p_ev_phi): porosity phi (n1,n2,n3) and
+        the stored 1/phi (default 1/phi); phi None removes it."""
+        if phi is None:
+            _check(lib().cruna_hip_set_penalization(self._h, None, None))
+            return
+        ph = _f64(phi, self.shape3)
+        ip = _f64(1.0 / ph if inv_phi is None else inv_phi, self.shape3)
+        _check(lib().cruna_hip_set_penalization(self._h, ph.ctypes.data, ip.ctypes.data))
 
     def autotune(self, steps=2):
         """Pick the lines per work-group of each direction by timing; returns them."""

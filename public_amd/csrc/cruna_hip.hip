@@ -114,6 +114,12 @@ struct cruna_hip_ctx {
   CurvMetric<double> cm64{};
   CurvMetric<float> cm32{};
   void *tg = nullptr;
+  // volume penalisation (cruna_hip_set_penalization): phi and 1/phi per
+  // point in the compute precision
+  bool ev = false;
+  void *pen = nullptr;
+  Penal<double> pe64{};
+  Penal<float> pe32{};
   // adjoint state qs^n and the forward trajectory Q(:,:,:,:,nt) (slot nt-1)
   double *qa = nullptr, *traj = nullptr, *gobj = nullptr;
   // device cells: the forward state of the current adjoint step and the
@@ -293,6 +299,18 @@ CurvMetric<double> curv_metric<double>(cruna_hip_ctx *c) { return c->cm64; }
 template <>
 CurvMetric<float> curv_metric<float>(cruna_hip_ctx *c) { return c->cm32; }
 
+template <typename T>
+Penal<T> penal(cruna_hip_ctx *c);
+template <>
+Penal<double> penal<double>(cruna_hip_ctx *c) { return c->pe64; }
+template <>
+Penal<float> penal<float>(cruna_hip_ctx *c) { return c->pe32; }
+
+// ns_rho_rhou_p_ev_phi.f90:295-309: the penalised module's adjoint RHS stops
+// with "not yet implemented"
+const char *kNoEvAdjoint =
+    "the adjoint of ns_rho_rhou_p_ev_phi is not implemented (the reference stops there too)";
+
 // X_jacobian in the compute precision on a deformed grid, else null
 template <typename T>
 const T *jacobian(cruna_hip_ctx *c) {
@@ -346,17 +364,17 @@ int bc_rhs(cruna_hip_ctx *c, T *rhs, const Q *q) {
 template <typename T, typename Q>
 int closure_term(cruna_hip_ctx *c, const Q *in);
 
-template <typename T, typename Q, int DIR, bool GT>
-void launch_sweep1_gt(cruna_hip_ctx *c, const Q *in) {
+template <typename T, typename Q, int DIR, int FORM>
+void launch_sweep1_form(cruna_hip_ctx *c, const Q *in) {
   const dim3 grid(c->geom[DIR].ntiles), block(c->wg[DIR]);
   T *rhs = (T *)c->rhs, *G = (T *)c->G;
   const LineOp<T> &op = dops<T>(c)[DIR].op;
   if (c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC)
-    hipLaunchKernelGGL((k_sweep1<T, Q, DIR, true, GT>), grid, block, 0, c->stream, in, rhs, G, op,
-                       c->geom[DIR], c->phys, c->N, metric<T>(c));
+    hipLaunchKernelGGL((k_sweep1<T, Q, DIR, true, FORM>), grid, block, 0, c->stream, in, rhs, G, op,
+                       c->geom[DIR], c->phys, c->N, metric<T>(c), penal<T>(c));
   else
-    hipLaunchKernelGGL((k_sweep1<T, Q, DIR, false, GT>), grid, block, 0, c->stream, in, rhs, G, op,
-                       c->geom[DIR], c->phys, c->N, metric<T>(c));
+    hipLaunchKernelGGL((k_sweep1<T, Q, DIR, false, FORM>), grid, block, 0, c->stream, in, rhs, G, op,
+                       c->geom[DIR], c->phys, c->N, metric<T>(c), penal<T>(c));
 }
 
 template <typename T, typename Q, int DIR>
@@ -378,14 +396,16 @@ int launch_sweep1(cruna_hip_ctx *c, const Q *in) {
   if (c->curv)
     launch_curv_sweep1<T, Q, DIR>(c, in);
   else if (c->gt)
-    launch_sweep1_gt<T, Q, DIR, true>(c, in);
+    launch_sweep1_form<T, Q, DIR, RHS_GT>(c, in);
+  else if (c->ev)
+    launch_sweep1_form<T, Q, DIR, RHS_EV>(c, in);
   else
-    launch_sweep1_gt<T, Q, DIR, false>(c, in);
+    launch_sweep1_form<T, Q, DIR, RHS_CART>(c, in);
   return check_launch();
 }
 
-template <typename T, int DIR, bool GT>
-void launch_sweep2_gt(cruna_hip_ctx *c, int mode, bool fuse_filter) {
+template <typename T, int DIR, int FORM>
+void launch_sweep2_form(cruna_hip_ctx *c, int mode, bool fuse_filter) {
   const dim3 grid(c->geom[DIR].ntiles), block(c->wg[DIR]);
   T *rhs = (T *)c->rhs, *qs = (T *)c->qs;
   const T *G = (const T *)c->G;
@@ -396,20 +416,24 @@ void launch_sweep2_gt(cruna_hip_ctx *c, int mode, bool fuse_filter) {
   if constexpr (DIR == 0) {
     if (fuse_filter && mode == 3) {
       if (per)
-        hipLaunchKernelGGL((k_sweep2<T, DIR, true, GT, true>), grid, block, 0, c->stream, rhs, G, op, fop,
-                           c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force);
+        hipLaunchKernelGGL((k_sweep2<T, DIR, true, FORM, true>), grid, block, 0, c->stream, rhs, G, op, fop,
+                           c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
+                           penal<T>(c));
       else
-        hipLaunchKernelGGL((k_sweep2<T, DIR, false, GT, true>), grid, block, 0, c->stream, rhs, G, op, fop,
-                           c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force);
+        hipLaunchKernelGGL((k_sweep2<T, DIR, false, FORM, true>), grid, block, 0, c->stream, rhs, G, op, fop,
+                           c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
+                           penal<T>(c));
       return;
     }
   }
   if (per) {
-    hipLaunchKernelGGL((k_sweep2<T, DIR, true, GT, false>), grid, block, 0, c->stream, rhs, G, op, fop,
-                       c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force);
+    hipLaunchKernelGGL((k_sweep2<T, DIR, true, FORM, false>), grid, block, 0, c->stream, rhs, G, op, fop,
+                       c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
+                           penal<T>(c));
   } else {
-    hipLaunchKernelGGL((k_sweep2<T, DIR, false, GT, false>), grid, block, 0, c->stream, rhs, G, op, fop,
-                       c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force);
+    hipLaunchKernelGGL((k_sweep2<T, DIR, false, FORM, false>), grid, block, 0, c->stream, rhs, G, op, fop,
+                       c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
+                           penal<T>(c));
   }
 }
 
@@ -448,9 +472,11 @@ int launch_sweep2(cruna_hip_ctx *c, int mode, bool fuse_filter) {
   if (c->curv)
     launch_curv_sweep2<T, DIR>(c, mode, fuse_filter);
   else if (c->gt)
-    launch_sweep2_gt<T, DIR, true>(c, mode, fuse_filter);
+    launch_sweep2_form<T, DIR, RHS_GT>(c, mode, fuse_filter);
+  else if (c->ev)
+    launch_sweep2_form<T, DIR, RHS_EV>(c, mode, fuse_filter);
   else
-    launch_sweep2_gt<T, DIR, false>(c, mode, fuse_filter);
+    launch_sweep2_form<T, DIR, RHS_CART>(c, mode, fuse_filter);
   return check_launch();
 }
 
@@ -879,7 +905,7 @@ int cruna_hip_destroy(cruna_hip_ctx *c) {
   }
   for (void *p : {(void *)c->qn, (void *)c->acc, (void *)c->qref, (void *)c->shape, (void *)c->tmp,
                   (void *)c->qa, (void *)c->traj, (void *)c->gobj, c->met, (void *)c->q0p, c->force,
-                  (void *)c->ckpt, c->tg,
+                  (void *)c->ckpt, c->tg, c->pen,
                   c->qs, c->rhs, c->G})
     if (p) (void)hipFree(p);
   for (auto &p : c->pending) {
@@ -923,6 +949,10 @@ int cruna_hip_set_sponge_shape(cruna_hip_ctx *c, const double *shape) {
 
 int cruna_hip_set_metric(cruna_hip_ctx *c, const double *X_metric, const double *X_jacobian) {
   if (!c || !X_metric || !X_jacobian) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (c->ev)
+    return fail(CRUNA_HIP_EINVAL,
+                "volume penalisation on a deformed grid (ns_rho_rhou_p_ev_phi_grid_transform) "
+                "is not implemented on the device");
   const long n1 = c->cfg.n[0], n2 = c->cfg.n[1], n3 = c->cfg.n[2], N = c->N;
   auto M = [&](int k, int j, long i, long jj, long kk) {
     return X_metric[(size_t)N * (k + 3 * j) + i + n1 * (jj + n2 * kk)];
@@ -977,6 +1007,48 @@ int cruna_hip_set_metric(cruna_hip_ctx *c, const double *X_metric, const double 
     c->curv = true;
   }
   c->gt = true;
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_set_penalization(cruna_hip_ctx *c, const double *phi, const double *inv_phi) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (!phi && !inv_phi) {  // back to ns_rho_rhou_p
+    (void)hipStreamSynchronize(c->stream);
+    if (c->pen) (void)hipFree(c->pen);
+    c->pen = nullptr;
+    c->ev = false;
+    c->pe64 = Penal<double>{};
+    c->pe32 = Penal<float>{};
+    return CRUNA_HIP_OK;
+  }
+  if (!phi || !inv_phi) return fail(CRUNA_HIP_EINVAL, "phi and inv_phi go together");
+  if (c->gt)
+    return fail(CRUNA_HIP_EINVAL,
+                "volume penalisation on a deformed grid (ns_rho_rhou_p_ev_phi_grid_transform) "
+                "is not implemented on the device");
+  const size_t N = c->N;
+  for (size_t i = 0; i < N; ++i)
+    if (!std::isfinite(phi[i]) || !std::isfinite(inv_phi[i]))
+      return fail(CRUNA_HIP_EINVAL, "phi or inv_phi is not finite");
+  const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
+  const size_t ts = mixed ? sizeof(float) : sizeof(double);
+  std::vector<char> h(2 * N * ts);
+  for (size_t i = 0; i < N; ++i) {
+    if (mixed) {
+      ((float *)h.data())[i] = float(phi[i]);
+      ((float *)h.data())[N + i] = float(inv_phi[i]);
+    } else {
+      ((double *)h.data())[i] = phi[i];
+      ((double *)h.data())[N + i] = inv_phi[i];
+    }
+  }
+  if (!c->pen) HIP_TRY(hipMalloc(&c->pen, h.size()));
+  HIP_TRY(hipMemcpy(c->pen, h.data(), h.size(), hipMemcpyHostToDevice));
+  if (mixed)
+    c->pe32 = Penal<float>{(const float *)c->pen, (const float *)c->pen + N};
+  else
+    c->pe64 = Penal<double>{(const double *)c->pen, (const double *)c->pen + N};
+  c->ev = true;
   return CRUNA_HIP_OK;
 }
 
@@ -1072,6 +1144,7 @@ int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const
                           const double *g, double t) {
   (void)t;  // the adjoint RHS of the reference does not depend on t
   if (!c || !rhs || !qs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (int rc = stage_adjoint_inputs(c, qs, q, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
   if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
@@ -1098,6 +1171,7 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *
                                    const double *q0, const double *g, double t) {
   (void)t;
   if (!c || !qs1 || !qs0 || !q0) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (int rc = stage_adjoint_inputs(c, qs0, q0, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
   if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
@@ -1170,6 +1244,7 @@ int cruna_hip_trajectory_record(cruna_hip_ctx *c, int on) {
 
 int cruna_hip_run_adjoint(cruna_hip_ctx *c, int nt_end, int nsteps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
   if (!c->qa) return fail(CRUNA_HIP_EINVAL, "no adjoint state (cruna_hip_set_adjoint_state)");
   if (c->traj_slots == 0) return fail(CRUNA_HIP_EINVAL, "no trajectory (cruna_hip_trajectory_alloc)");
@@ -1235,6 +1310,7 @@ int cruna_hip_run_direct_subsets(cruna_hip_ctx *c, int subsets, int steps) {
 
 int cruna_hip_run_adjoint_subsets(cruna_hip_ctx *c, int subsets, int steps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (subsets < 1 || steps < 1) return fail(CRUNA_HIP_EINVAL, "subsets and steps must be >= 1");
   if (c->ckpt_n < subsets + 1) return fail(CRUNA_HIP_EINVAL, "not enough subset checkpoints (cruna_hip_subsets_alloc)");
   if (c->traj_slots < steps) return fail(CRUNA_HIP_EINVAL, "the trajectory holds fewer slots than steps");
@@ -1458,6 +1534,7 @@ int cruna_hip_ensemble_set_role(cruna_hip_ensemble *e, int i, int role, int nt) 
   if (role != CRUNA_HIP_FORWARD && role != CRUNA_HIP_ADJOINT) return fail(CRUNA_HIP_EINVAL, "bad role");
   cruna_hip_ctx *c = e->m[i];
   if (role == CRUNA_HIP_ADJOINT) {
+    if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
     if (!c->qa || c->traj_slots == 0)
       return fail(CRUNA_HIP_EINVAL, "an adjoint member needs its adjoint state and trajectory first");
     if (c->traj_slots > 1 && (nt < 1 || nt > c->traj_slots))

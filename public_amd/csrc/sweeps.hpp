@@ -57,6 +57,19 @@ struct Metric {
   const T *J;     // X_jacobian(n1,n2,n3)
 };
 
+// Volume penalisation (ns_rho_rhou_p_ev_phi.f90): the porosity phi and the
+// stored 1/phi (volume_penalization's penalization(:,:,:,1:2)) per point.
+template <typename T>
+struct Penal {
+  const T *phi;
+  const T *inv_phi;
+};
+
+// Right-hand-side form of the sweeps: Cartesian (ns_rho_rhou_p), deformed
+// grid with a tensor-product metric (ns_rho_rhou_p_grid_transform), or
+// volume-penalised (ns_rho_rhou_p_ev_phi).
+enum RhsForm { RHS_CART = 0, RHS_GT = 1, RHS_EV = 2 };
+
 // give_mu, ns_rho_rhou_p_trafos.f90:119-142: mu0*(T/T0)**1.5*((T0+S1)/(T+S1)),
 // or the power law mu0*(T/T0)**exp of the case-level override.
 template <typename T>
@@ -170,18 +183,30 @@ __device__ __forceinline__ void st(Q *__restrict__ p, long base, long PS, const 
 // (physical gradients, contravariant fluxes and the projected stresses all
 // reduce to that for a diagonal metric); the heat flux takes the temperature
 // gradient with h and lambda J inside, the outer derivative without h.
-template <typename T, typename Q, int DIR, bool PER, bool GT>
+// EV: volume penalisation (ns_rho_rhou_p_ev_phi.f90:33-291): every flux
+// carries phi inside its derivative and its divergence is multiplied by
+// the stored 1/phi, distributed over the directions (the pressure
+// gradient and u.grad(p) are not); both heat-flux derivatives take no h.
+template <typename T, typename Q, int DIR, bool PER, int FORM>
 __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__restrict__ rhs,
                                                T *__restrict__ G, LineOp<T> op, SweepGeom g,
-                                               Phys ph, long N, Metric<T> mt) {
+                                               Phys ph, long N, Metric<T> mt, Penal<T> pe) {
+  constexpr bool GT = FORM == RHS_GT, EV = FORM == RHS_EV;
   __shared__ LineLds<T> lds;
   const ThreadPos tp = thread_pos<DIR>(g);
   const long B = tp.base, PS = g.PS;
   const T mline = GT ? mt.m[DIR][tp.lidx] : T(1);
   const T ih = GT ? T(ph.inv_h[DIR]) * mline : T(ph.inv_h[DIR]);
   // heat flux: inner (temperature) and outer derivative scales
-  const T hin = GT ? ih : T(ph.heat_inv_h[DIR]);
-  const T hout = GT ? mline : T(ph.heat_inv_h[DIR]);
+  const T hin = GT ? ih : (EV ? T(1) : T(ph.heat_inv_h[DIR]));
+  const T hout = GT ? mline : (EV ? T(1) : T(ph.heat_inv_h[DIR]));
+  // penalisation: phi inside the fluxes, the stored 1/phi on their divergence
+  T PH[PPT], IP[PPT];
+  if (EV) {
+    ld<DIR>(pe.phi, B, PS, PH);
+    ld<DIR>(pe.inv_phi, B, PS, IP);
+  }
+  auto div = [&](int m, T v) -> T { return EV ? IP[m] * v : v; };
   // G slot of d(u_i)/dx_DIR and whether it completes a symmetric sum begun by
   // an earlier sweep: x writes u_x, v_x, w_x; y adds u_y to v_x and writes
   // v_y, w_y; z adds u_z to w_x, v_z to w_y and writes w_z
@@ -218,10 +243,10 @@ __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__re
     if (DIR != 0) ld<DIR>(rhs + (1 + i) * N, B, PS, r);
     if (k == 1) ld<DIR>(qs + 4 * N, B, PS, p);  // in flight for the pressure terms
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = mom[m] * ui[m];
+    for (int m = 0; m < PPT; ++m) f[m] = EV ? (PH[m] * mom[m]) * ui[m] : mom[m] * ui[m];
     line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) r[m] = DIR == 0 ? -(x[m] * ih) : r[m] - x[m] * ih;
+    for (int m = 0; m < PPT; ++m) r[m] = DIR == 0 ? div(m, -(x[m] * ih)) : r[m] + div(m, -(x[m] * ih));
     st<DIR>(rhs + (1 + i) * N, B, PS, r);
   }
 
@@ -236,9 +261,15 @@ __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__re
 
   // mass: -d/dx_DIR (rho u_DIR) (:131-135)
   if (DIR != 0) ld<DIR>(rhs, B, PS, r);
-  line_solve<T, PER>(op, lds, tp, mom, x);
+  if (EV) {
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) r[m] = DIR == 0 ? -(x[m] * ih) : r[m] - x[m] * ih;
+    for (int m = 0; m < PPT; ++m) f[m] = PH[m] * mom[m];
+    line_solve<T, PER>(op, lds, tp, f, x);
+  } else {
+    line_solve<T, PER>(op, lds, tp, mom, x);
+  }
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) r[m] = DIR == 0 ? div(m, -(x[m] * ih)) : r[m] + div(m, -(x[m] * ih));
   st<DIR>(rhs, B, PS, r);
 
   // pressure gradient p_DIR
@@ -250,11 +281,11 @@ __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__re
   // momentum DIR: -d/dx_DIR (rho u_DIR u_DIR) - p_DIR
   if (DIR != 0) ld<DIR>(rhs + (1 + DIR) * N, B, PS, r);
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) f[m] = mom[m] * ud[m];
+  for (int m = 0; m < PPT; ++m) f[m] = EV ? (PH[m] * mom[m]) * ud[m] : mom[m] * ud[m];
   line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
   for (int m = 0; m < PPT; ++m) {
-    r[m] = DIR == 0 ? -(x[m] * ih) : r[m] - x[m] * ih;
+    r[m] = DIR == 0 ? div(m, -(x[m] * ih)) : r[m] + div(m, -(x[m] * ih));
     r[m] = r[m] - pd[m];
   }
   st<DIR>(rhs + (1 + DIR) * N, B, PS, r);
@@ -262,10 +293,10 @@ __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__re
   // pressure-equation bracket: gamma/(gamma-1) (-d(p u_DIR)) + u_DIR p_DIR (:180-187)
   T e[PPT];
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) f[m] = p[m] * ud[m];
+  for (int m = 0; m < PPT; ++m) f[m] = EV ? (PH[m] * p[m]) * ud[m] : p[m] * ud[m];
   line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) e[m] = T(ph.gt) * (-(x[m] * ih)) + ud[m] * pd[m];
+  for (int m = 0; m < PPT; ++m) e[m] = div(m, T(ph.gt) * (-(x[m] * ih))) + ud[m] * pd[m];
 
   // heat flux d/dx_DIR (lambda d/dx_DIR T) (:189-199); the x sweep evaluates
   // the viscosity law and leaves mu in G6 for the later sweeps of the stage
@@ -285,13 +316,14 @@ __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__re
   line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
   for (int m = 0; m < PPT; ++m) {
-    const T lam = GT ? lambda_of(ph, mu[m]) * Jv[m] : lambda_of(ph, mu[m]);
+    const T lam = GT ? lambda_of(ph, mu[m]) * Jv[m]
+                     : (EV ? PH[m] * lambda_of(ph, mu[m]) : lambda_of(ph, mu[m]));
     f[m] = lam * (x[m] * hin);
   }
   line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
   for (int m = 0; m < PPT; ++m) {
-    e[m] = e[m] + x[m] * hout;
+    e[m] = e[m] + div(m, x[m] * hout);
     if (DIR != 0) e[m] = r[m] + e[m];
   }
   st<DIR>(rhs + 4 * N, B, PS, e);
@@ -351,13 +383,15 @@ __device__ __forceinline__ void rk_update(int mode, int v, const T (&k)[PPT], lo
 // GT: deformed grid: d/dxi_d (M(d,d) tau_id) = M(d,d) d/dxi_d tau_id, and the
 // last sweep multiplies every equation by X_jacobian and the dissipation once
 // more (ns_rho_rhou_p_grid_transform.f90:190-276).
-template <typename T, int DIR, bool PER, bool GT, bool FF>
+// EV: d/dx_d (phi tau_id), times the stored 1/phi (ns_rho_rhou_p_ev_phi.f90:161-201).
+template <typename T, int DIR, bool PER, int FORM, bool FF>
 __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__restrict__ G,
                                                LineOp<T> op, LineOp<T> fop, SweepGeom g, Phys ph,
                                                long N, int mode,
                                                double *__restrict__ qn, double *__restrict__ acc,
                                                T *__restrict__ qs_out, double dt, Metric<T> mt,
-                                               const T *__restrict__ force) {
+                                               const T *__restrict__ force, Penal<T> pe) {
+  constexpr bool GT = FORM == RHS_GT, EV = FORM == RHS_EV;
   constexpr bool LAST = DIR == 0;
   __shared__ LineLds<T> lds;
   const ThreadPos tp = thread_pos<DIR>(g);
@@ -457,6 +491,18 @@ __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__r
     finish(4, k);
     asm volatile("" ::: "memory");
   }
+  T IP[PPT];
+  if (EV) {  // phi tau_id inside the derivative, 1/phi on it
+    T PH[PPT];
+    ld<DIR>(pe.phi, B, PS, PH);
+    ld<DIR>(pe.inv_phi, B, PS, IP);
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+      t1[m] = PH[m] * t1[m];
+      t2[m] = PH[m] * t2[m];
+      t3[m] = PH[m] * t3[m];
+    }
+  }
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     T x[PPT];
@@ -466,7 +512,7 @@ __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__r
     ld<DIR>(rhs + (1 + i) * N, B, PS, k);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
-      k[m] = k[m] + x[m] * ih;
+      k[m] = k[m] + (EV ? IP[m] * (x[m] * ih) : x[m] * ih);
       if (LAST && GT) k[m] = Jv[m] * k[m];
     }
     if (!LAST) {

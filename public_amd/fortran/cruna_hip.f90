@@ -191,6 +191,11 @@ module cruna_hip
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: xm(*), xj(*)
     end function
+    integer(c_int) function c_set_penalization(ctx, phi, inv_phi) bind(C, name="cruna_hip_set_penalization")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: phi(*), inv_phi(*)
+    end function
     integer(c_int) function c_autotune(ctx, steps, tiles) bind(C, name="cruna_hip_autotune")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
@@ -277,7 +282,7 @@ module cruna_hip
   public :: hip_rhs_adjoint, hip_time_stepper_adjoint, hip_set_adjoint_state
   public :: hip_get_adjoint_state, hip_trajectory_alloc, hip_trajectory_set
   public :: hip_trajectory_get, hip_trajectory_record, hip_run_adjoint
-  public :: hip_set_metric, hip_autotune, hip_box_filter_state
+  public :: hip_set_metric, hip_set_penalization, hip_autotune, hip_box_filter_state
   public :: hip_subsets_alloc, hip_run_direct_subsets, hip_run_adjoint_subsets
   public :: hip_ensemble_create, hip_ensemble_destroy, hip_ensemble_member
   public :: hip_ensemble_set_role, hip_ensemble_run, hip_ensemble_autotune
@@ -484,6 +489,14 @@ contains
     type(c_ptr), intent(in) :: ctx
     real(c_double), intent(in), contiguous :: X_metric(:,:,:,:,:), X_jacobian(:,:,:)
     call check(c_set_metric(ctx, X_metric, X_jacobian), "set_metric")
+  end subroutine
+
+  ! volume penalisation: penalization(:,:,:,1) (phi) and (:,:,:,2) (1/phi) of
+  ! the case's volume_penalization (src/immersed/volume_penalization.f90)
+  subroutine hip_set_penalization(ctx, phi, inv_phi)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(in), contiguous :: phi(:,:,:), inv_phi(:,:,:)
+    call check(c_set_penalization(ctx, phi, inv_phi), "set_penalization")
   end subroutine
 
   ! lines per work-group per direction, tuned on a copy of the state

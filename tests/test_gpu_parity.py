@@ -749,3 +749,115 @@ def test_o4_rhs_and_run_match_oracle(kind):
     ref = O.run_direct(cfg2, qs, 3)
     for v in range(5):
         assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+
+
+# volume penalisation (ns_rho_rhou_p_ev_phi)
+def _porosity(shape, dx, seed=0):
+    n1, n2, n3 = shape
+    x = np.arange(n1) * dx[0]
+    y = np.arange(n2) * dx[1]
+    z = np.arange(n3) * dx[2]
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.5, 1.5, 3)
+    phi = 0.7 + 0.25 * np.sin(a[0] * X) * np.cos(a[1] * Y) * np.sin(a[2] * Z + 0.3)
+    return np.asfortranarray(phi)
+
+
+@pytest.mark.parametrize("visc", [O.SUTHERLAND, O.POWER])
+def test_rhs_direct_ev_phi_matches_oracle(visc):
+    """ns_rho_rhou_p_ev_phi's RHS with a varying porosity on the periodic box;
+    the stored 1/phi is not exactly 1/phi here, so the test tells the two
+    fields apart."""
+    Cruna = _gpu()
+    n, mach = 32, 0.3
+    kw, h = _kw(mach, n, visc)
+    q = _box(n, 3, mach)
+    phi = _porosity((n,) * 3, (h,) * 3)
+    inv_phi = np.asfortranarray(1.0 / phi * (1 + 0.01 * np.cos(np.arange(n))[:, None, None]))
+    ref = O.rhs_direct(O.Cfg((n,) * 3, (h,) * 3, penalization=(phi, inv_phi), **kw), q)
+    with Cruna((n,) * 3, (h,) * 3, **kw) as g:
+        g.set_penalization(phi, inv_phi)
+        got = g.right_hand_side_direct_3d(q)
+        g.set_penalization(None)
+        plain = g.right_hand_side_direct_3d(q)
+    for v in range(5):
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+    ref_plain = O.rhs_direct(O.Cfg((n,) * 3, (h,) * 3, **kw), q)
+    assert rel(plain, ref_plain) < 1e-12
+    assert rel(ref_plain, ref) > 1e-3
+
+
+def test_run_direct_ev_phi_with_boundaries():
+    """Open-y shear layer with characteristic rows, a sponge on the RHS and a
+    porous slab: 4 RK4 steps with filters against the oracle."""
+    Cruna = _gpu()
+    shape = (32, 48, 16)
+    q, dx, (x, y, z), (Lx, Ly, Lz) = _shear_case(shape, seed=7)
+    n1, n2, n3 = shape
+    bc = [(201, 1, n1, 1, 1, 1, n3, 0, -1, 0),
+          (201, 1, n1, n2, n2, 1, n3, 0, 1, 0),
+          (400, 1, n1, 1, n2, 1, n3)]
+    scheme = (O.PERIODIC, O.OPEN, O.PERIODIC)
+    kw = dict(mu0=2e-3, T0=1.0 / (1.4 * 0.25), dt=2e-3, scheme=scheme, bc=bc)
+    cfg0 = O.Cfg(shape, dx, filter_freq=(1, 1, 1), **kw)
+    shape_fn = O.sponge_shape(cfg0, (0, -Ly / 2, 0), (Lx, Ly / 2, Lz), (-1.0, 0.2 * Ly, -1.0), 0.05,
+                              np.asfortranarray(x), np.asfortranarray(y), np.asfortranarray(z))
+    phi = _porosity(shape, dx, 1)
+    cfg = O.Cfg(shape, dx, filter_freq=(1, 1, 1), penalization=(phi, 1.0 / phi), **kw)
+    qref = q.copy(order="F")
+    ref = O.run_direct(cfg, q, 4, qref=qref, shape=shape_fn)
+    with Cruna(shape, dx, filter_freq=(1, 1, 1), **kw) as g:
+        g.set_penalization(phi)
+        g.set_qref(qref)
+        g.set_sponge_shape(shape_fn)
+        g.set_state(q)
+        g.run_direct(4)
+        g.synchronize()
+        got = g.get_state()
+    for v in range(5):
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+
+
+def test_ev_phi_fp32_mixed_close_to_fp64():
+    from public_amd import FP32_MIXED
+    Cruna = _gpu()
+    n, mach = 32, 0.3
+    kw, h = _kw(mach, n)
+    q = _box(n, 4, mach)
+    phi = _porosity((n,) * 3, (h,) * 3, 2)
+    out = []
+    for prec in (0, FP32_MIXED):
+        with Cruna((n,) * 3, (h,) * 3, precision=prec, filter_freq=(1, 1, 1), **kw) as g:
+            g.set_penalization(phi)
+            g.set_state(q)
+            g.run_direct(10)
+            g.synchronize()
+            out.append(g.get_state())
+    err = rel(out[1], out[0])
+    print(f"ev_phi fp32-mixed vs fp64 after 10 steps: {err:.3e}")
+    assert err < 1e-5
+
+
+def test_ev_phi_refuses_adjoint_and_deformed_grid():
+    from public_amd.cruna import CrunaError
+    Cruna = _gpu()
+    n = (16, 16, 16)
+    dx, X, Xc = _wavy(n)
+    kw = dict(scheme=(O.PERIODIC, O.OPEN, O.PERIODIC))
+    Xm, Xj = O.calc_metric(O.Cfg(n, dx, **kw), X, Xc)
+    phi = _porosity(n, dx)
+    with Cruna(n, dx, **kw) as g:
+        g.set_penalization(phi)
+        with pytest.raises(CrunaError, match="not implemented"):
+            g.time_stepper_adjoint(np.zeros(n + (5,)), _gt_state(X, 0))
+        with pytest.raises(CrunaError, match="deformed grid"):
+            g.set_metric(Xm, Xj)
+        bad = phi.copy(order="F")
+        bad[1, 2, 3] = np.inf
+        with pytest.raises(CrunaError, match="not finite"):
+            g.set_penalization(bad)
+        g.set_penalization(None)
+        g.set_metric(Xm, Xj)
+        with pytest.raises(CrunaError, match="deformed grid"):
+            g.set_penalization(phi)
