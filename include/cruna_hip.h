@@ -71,6 +71,10 @@ typedef struct cruna_hip_cfg {
                             CRUNA_HIP_DIFF_O4_EXPLICIT (diff_pen_core_o4_e_c_*) or
                             CRUNA_HIP_DIFF_O4_PADE (diff_pen_core_o4_i_c_*_pade); the
                             filter is the o6 compact one in every case */
+  int wfilter_freq[3];   /* filter.dwi1freq..dwi3freq of the cruna_hpc loop (0 = never):
+                            the weighted filter (filter_weighted_iN with the
+                            o2_e_c_p core, weight phi of the penalisation or 1,
+                            strength 1) after the compact filters of a step */
 } cruna_hip_cfg;
 
 typedef struct cruna_hip_ctx cruna_hip_ctx;
@@ -117,6 +121,14 @@ int cruna_hip_set_metric(cruna_hip_ctx *ctx, const double *X_metric, const doubl
  * every adjoint entry point (the reference's ev_phi adjoint stops,
  * ns_rho_rhou_p_ev_phi.f90:295-309). */
 int cruna_hip_set_penalization(cruna_hip_ctx *ctx, const double *phi, const double *inv_phi);
+
+/* filter_weighted_iN_mpi_cartesian::filter_4d on nv fields q(n1,n2,n3,nv) in
+ * place (src/filter/filter_weighted_i1_mpi_cartesian.f90:15-54) with the
+ * filter_weighted_pen_core_o2_e_c_p core (periodic along dir):
+ * q <- q - F(strength weight F(q)) / weight.  weight, strength: (n1,n2,n3)
+ * fields, or null for 1. */
+int cruna_hip_filter_weighted(cruna_hip_ctx *ctx, int dir, double *q, int nv, const double *weight,
+                              const double *strength);
 
 /* Dx1/Dx2/Dx3 on a 3-D field: diff_iN_mpi_cartesian::diff_3d
  * (src/diff/diff_i1_mpi_cartesian.f90:83-103).  dir = 0,1,2; h = spacing

@@ -1293,8 +1293,40 @@ void orc_time_stepper_adjoint(const orc_cfg *c, double *qs1, const double *qs0,
   free(k1); free(k2); free(k3); free(k4); free(s);
 }
 
-/* Step-loop body, src/main/computation_direct.f90:67-98 / cruna_hpc.f90:171-229:
- * stepper -> qn = q1 -> BC on q (outside rhs) -> filters i1, i2, i3 at their frequencies. */
+/* Weighted filter, src/filter/filter_weighted_i1_mpi_cartesian.f90:15-54 with
+ * filter_weighted_pen_core_o2_e_c_p.f90 filter_main (:75-108, periodic wrap of
+ * rows 1 and n) and init_stencil (:112-128, a0 = -2/4, a1 = 1/4):
+ * q_tilde = q; f = F(q); f = strength * weight * f; f = F(f); f = f / weight;
+ * q = q_tilde - f. */
+void orc_filter_weighted4d(const orc_cfg *c, int dir, double *q, int nv, const double *weight,
+                           const double *strength) {
+  const size_t N = npts(c);
+  const int n1 = c->n[0], n2 = c->n[1], n = c->n[dir];
+  const size_t st = dir == 0 ? 1 : (dir == 1 ? (size_t)n1 : (size_t)n1 * n2);
+  const double a0 = -2.0 * 0.25, a1 = 1.0 * 0.25;
+  double *f = xalloc(N), *g = xalloc(N);
+  size_t *ip = (size_t *)malloc(N * sizeof(size_t)), *im = (size_t *)malloc(N * sizeof(size_t));
+  for (size_t p = 0; p < N; ++p) {
+    const int i = dir == 0 ? (int)(p % n1) : (dir == 1 ? (int)((p / n1) % n2) : (int)(p / ((size_t)n1 * n2)));
+    ip[p] = i + 1 < n ? p + st : p - (size_t)(n - 1) * st;
+    im[p] = i > 0 ? p - st : p + (size_t)(n - 1) * st;
+  }
+  for (int v = 0; v < nv; ++v) {
+    double *x = q + (size_t)v * N;
+    for (size_t p = 0; p < N; ++p) f[p] = a0 * x[p] + a1 * (x[ip[p]] + x[im[p]]);
+    for (size_t p = 0; p < N; ++p)
+      f[p] = (strength ? strength[p] : 1.0) * (weight ? weight[p] : 1.0) * f[p];
+    for (size_t p = 0; p < N; ++p) g[p] = a0 * f[p] + a1 * (f[ip[p]] + f[im[p]]);
+    for (size_t p = 0; p < N; ++p) g[p] = g[p] / (weight ? weight[p] : 1.0);
+    for (size_t p = 0; p < N; ++p) x[p] = x[p] - g[p];
+  }
+  free(f); free(g); free(ip); free(im);
+}
+
+/* Step-loop body, src/main/computation_direct.f90:67-98 / cruna_hpc.f90:171-242:
+ * stepper -> qn = q1 -> BC on q (outside rhs) -> filters i1, i2, i3 at their
+ * frequencies -> weighted filters i1, i2, i3 at theirs (cruna_hpc.f90:231-242;
+ * the subset loop of computation_direct has none: leave wfilter_freq 0). */
 void orc_run_direct(const orc_cfg *c, double *q, int nt0, int nsteps, const double *qref,
                     const double *shape, int heat_unit_spacing) {
   const size_t M = 5 * npts(c);
@@ -1305,6 +1337,9 @@ void orc_run_direct(const orc_cfg *c, double *q, int nt0, int nsteps, const doub
     orc_bc_q(c, q, qref, shape);
     for (int d = 0; d < 3; ++d)
       if (c->filter_freq[d] > 0 && nt % c->filter_freq[d] == 0) orc_filter4d(c, d, q, 5);
+    for (int d = 0; d < 3; ++d)
+      if (c->wfilter_freq[d] > 0 && nt % c->wfilter_freq[d] == 0)
+        orc_filter_weighted4d(c, d, q, 5, c->penal_phi, NULL);
   }
   free(q1);
 }

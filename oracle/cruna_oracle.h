@@ -55,6 +55,9 @@ typedef struct orc_cfg {
    * is not) the direct RHS is ns_rho_rhou_p_ev_phi's.  NULL: plain NS. */
   const double *penal_phi;
   const double *penal_inv_phi;
+  /* filter.dwi1freq..dwi3freq (0 = never): the cruna_hpc loop's weighted
+   * filters with the o2_e_c_p core, weight penal_phi (or 1), strength 1 */
+  int wfilter_freq[3];
 } orc_cfg;
 
 #define ORC_DIFF_O6_COMPACT 0
@@ -86,7 +89,11 @@ double orc_lambda(const orc_cfg *c, double mu);
  * X_metric(n1,n2,n3,3,3) (metric without the Jacobian) and X_jacobian. */
 void orc_calc_metric(const orc_cfg *c, const double *X, const double *X_calc, double *X_metric,
                      double *X_jacobian);
-/* Direct RHS with volume penalisation, src/equations/ns_rho_rhou_p_ev_phi.f90:31-236. */
+/* filter_weighted_iN_mpi_cartesian::filter_4d with the o2_e_c_p core on
+ * q(n1,n2,n3,nv) along dir; weight / strength null mean 1. */
+void orc_filter_weighted4d(const orc_cfg *c, int dir, double *q, int nv, const double *weight,
+                           const double *strength);
+/* Direct RHS with volume penalisation, src/equations/ns_rho_rhou_p_ev_phi.f90:33-291. */
 void orc_rhs_direct_ev(const orc_cfg *c, double *rhs, const double *q_in, const double *phi,
                        const double *inv_phi, const double *qref, const double *shape);
 

@@ -44,6 +44,7 @@ class OrcCfg(C.Structure):
         ("diff_kind", C.c_int * 3),
         ("penal_phi", C.c_void_p),
         ("penal_inv_phi", C.c_void_p),
+        ("wfilter_freq", C.c_int * 3),
     ]
 
 
@@ -82,6 +83,7 @@ def lib():
         L.orc_calc_metric.argtypes = [P, _dp, _dp, _dp, _dp]
         L.orc_rhs_direct_gt.argtypes = [P, _dp, _dp, _dp, _dp, _vp, _vp]
         L.orc_rhs_direct_ev.argtypes = [P, _dp, _dp, _dp, _dp, _vp, _vp]
+        L.orc_filter_weighted4d.argtypes = [P, C.c_int, _dp, C.c_int, _vp, _vp]
         L.orc_rhs_adjoint.argtypes = [P, _dp, _dp, _dp, _vp, _vp]
         L.orc_bc_q.argtypes = [P, _dp, _vp, _vp]
         L.orc_bc_qs.argtypes = [P, _dp, _dp, _vp]
@@ -128,7 +130,7 @@ class Cfg:
     def __init__(self, n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0,
                  W=1.0, Pr=0.71, mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76,
                  dt=1e-3, filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), bc=(), metric=None,
-                 diff_kind=(0, 0, 0), penalization=None):
+                 diff_kind=(0, 0, 0), penalization=None, wfilter_freq=(0, 0, 0)):
         """metric: (X_metric (n1,n2,n3,3,3), X_jacobian (n1,n2,n3)) of a deformed
         grid; the direct RHS is then the grid-transform one.  diff_kind per
         direction: DIFF_O6_COMPACT, DIFF_O4_EXPLICIT or DIFF_O4_PADE.
@@ -142,6 +144,7 @@ class Cfg:
             c.filter_freq[d] = int(filter_freq[d])
             c.afilter_freq[d] = int(afilter_freq[d])
             c.diff_kind[d] = int(diff_kind[d])
+            c.wfilter_freq[d] = int(wfilter_freq[d])
         c.filter_alpha = filter_alpha
         c.gamma, c.R, c.W, c.Pr, c.mu0, c.T0 = gamma, R, W, Pr, mu0, T0
         c.visc_law, c.visc_exp, c.dt = visc_law, visc_exp, dt
@@ -221,6 +224,16 @@ def rhs_adjoint(cfg, qs, q, g=None, shape=None):
     shape = None if shape is None else _f(shape)
     lib().orc_rhs_adjoint(cfg.ref(), rhs, qs, q, _ptr(g), _ptr(shape))
     return rhs
+
+
+def filter_weighted4d(cfg, d, q, weight=None, strength=None):
+    """filter_weighted_iN::filter_4d (o2_e_c_p core) on a copy of q (n1,n2,n3[,nv])."""
+    q = _f(q).copy(order="F")
+    nv = 1 if q.ndim == 3 else q.shape[3]
+    w = None if weight is None else _f(weight)
+    s = None if strength is None else _f(strength)
+    lib().orc_filter_weighted4d(cfg.ref(), d, q, nv, _ptr(w), _ptr(s))
+    return q
 
 
 def calc_metric(cfg, X, X_calc):

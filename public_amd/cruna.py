@@ -53,6 +53,7 @@ class Cfg(C.Structure):
         ("nbc", C.c_int),
         ("bc", (C.c_double * 10) * MAX_BC),
         ("diff_kind", C.c_int * 3),
+        ("wfilter_freq", C.c_int * 3),
     ]
 
 
@@ -78,6 +79,7 @@ def lib():
         L.cruna_hip_set_sponge_shape.argtypes = [vp, dp]
         L.cruna_hip_set_metric.argtypes = [vp, dp, dp]
         L.cruna_hip_set_penalization.argtypes = [vp, vp, vp]
+        L.cruna_hip_filter_weighted.argtypes = [vp, ip, vp, ip, vp, vp]
         L.cruna_hip_autotune.argtypes = [vp, ip, C.POINTER(C.c_int)]
         L.cruna_hip_ensemble_create.argtypes = [C.POINTER(Cfg), ip, C.POINTER(vp)]
         L.cruna_hip_ensemble_destroy.argtypes = [vp]
@@ -127,7 +129,7 @@ def lib():
                   "filter", "rhs_direct", "rhs_adjoint", "time_stepper_direct", "time_stepper_adjoint",
                   "run_direct", "synchronize", "profile_enable", "profile_read", "profile_count",
                   "set_adjoint_state", "get_adjoint_state", "trajectory_alloc", "trajectory_set",
-                  "trajectory_get", "trajectory_record", "run_adjoint", "set_metric", "set_penalization", "autotune",
+                  "trajectory_get", "trajectory_record", "run_adjoint", "set_metric", "set_penalization", "filter_weighted", "autotune",
                   "ensemble_create", "ensemble_destroy", "ensemble_member", "ensemble_set_role",
                   "ensemble_run", "ensemble_autotune", "cnn_create", "cnn_destroy", "cnn_init_params",
                   "cnn_set_params", "cnn_get_params", "cnn_set_norm", "cnn_forward", "cnn_gradients",
@@ -157,7 +159,7 @@ def _f64(a, shape):
 def make_cfg(n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0, W=1.0, Pr=0.71,
              mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76, dt=1e-3,
              filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), heat_unit_spacing=1, precision=FP64,
-             bc=(), tile_lines=(0, 0, 0), diff_kind=(DIFF_O6_COMPACT,) * 3):
+             bc=(), tile_lines=(0, 0, 0), diff_kind=(DIFF_O6_COMPACT,) * 3, wfilter_freq=(0, 0, 0)):
     """cruna_hip_cfg from the reference's parameter names (defaults of the library)."""
     cfg = Cfg()
     lib().cruna_hip_default_cfg(C.byref(cfg))
@@ -169,6 +171,7 @@ def make_cfg(n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0,
         cfg.afilter_freq[d] = int(afilter_freq[d])
         cfg.tile_lines[d] = int(tile_lines[d])
         cfg.diff_kind[d] = int(diff_kind[d])
+        cfg.wfilter_freq[d] = int(wfilter_freq[d])
     cfg.filter_alpha = filter_alpha
     cfg.gamma, cfg.R, cfg.W, cfg.Pr, cfg.mu0, cfg.T0 = gamma, R, W, Pr, mu0, T0
     cfg.visc_law, cfg.visc_exp, cfg.dt = visc_law, visc_exp, dt
@@ -238,6 +241,20 @@ class Cruna:
         if q.shape[:3] != self.shape3:
             raise CrunaError("field does not match the grid")
         _check(lib().cruna_hip_filter(self._h, d, _dp(q), nv))
+        return q
+
+    def filter_weighted(self, d, q, weight=None, strength=None):
+        """filter_weighted_iN's filter_4d along direction d with the o2_e_c_p
+        core (on a copy, returned); weight/strength None mean 1."""
+        q = np.array(q, dtype=np.float64, order="F", copy=True)
+        nv = 1 if q.ndim == 3 else q.shape[3]
+        if q.shape[:3] != self.shape3:
+            raise CrunaError("field does not match the grid")
+        w = None if weight is None else _f64(weight, self.shape3)
+        s = None if strength is None else _f64(strength, self.shape3)
+        _check(lib().cruna_hip_filter_weighted(self._h, d, q.ctypes.data, nv,
+                                               None if w is None else w.ctypes.data,
+                                               None if s is None else s.ctypes.data))
         return q
 
     def filter_i1(self, q):

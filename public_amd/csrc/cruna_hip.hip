@@ -16,6 +16,7 @@
 #include "boundary.hpp"
 #include "sweeps.hpp"
 #include "curvilinear.hpp"
+#include "weighted_filter.hpp"
 #include "adjoint.hpp"
 #include "closure.hpp"
 
@@ -118,6 +119,9 @@ struct cruna_hip_ctx {
   // point in the compute precision
   bool ev = false;
   void *pen = nullptr;
+  // FP64 phi, the weight of the weighted filter when a penalisation is set
+  // (filter_weighted_pen_core_o2_e_c_p init_filter), and its output buffer
+  double *wphi = nullptr, *wtmp = nullptr, *wbuf = nullptr;
   Penal<double> pe64{};
   Penal<float> pe32{};
   // adjoint state qs^n and the forward trajectory Q(:,:,:,:,nt) (slot nt-1)
@@ -548,7 +552,30 @@ int filter_dir(cruna_hip_ctx *c, double *q, int d, int nv, T *copy = nullptr) {
   return check_launch();
 }
 
-int step_once(cruna_hip_ctx *c, int nt) {
+// filter_weighted_iN on nv fields of the FP64 state q in place (through out)
+int filter_weighted_dir(cruna_hip_ctx *c, double *q, double *out, int d, int nv, const double *w,
+                        const double *s) {
+  const dim3 grid((c->N + 255) / 256), block(256);
+  const int n1 = c->cfg.n[0], n2 = c->cfg.n[1], n3 = c->cfg.n[2];
+  if (d == 0)
+    hipLaunchKernelGGL(k_filter_weighted<0>, grid, block, 0, c->stream, q, out, nv, w, s, n1, n2, n3, c->N);
+  else if (d == 1)
+    hipLaunchKernelGGL(k_filter_weighted<1>, grid, block, 0, c->stream, q, out, nv, w, s, n1, n2, n3, c->N);
+  else
+    hipLaunchKernelGGL(k_filter_weighted<2>, grid, block, 0, c->stream, q, out, nv, w, s, n1, n2, n3, c->N);
+  if (int rc = check_launch()) return rc;
+  HIP_TRY(hipMemcpyAsync(q, out, (size_t)nv * c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  return CRUNA_HIP_OK;
+}
+
+}  // namespace
+// the direct step loop of cruna_hpc (weighted filters) or computation_direct (none)
+int run_direct_loop(cruna_hip_ctx *c, int nt0, int nsteps, bool weighted);
+namespace {
+
+// weighted: the cruna_hpc loop's filter_weighted_i1..i3 after the filters
+// (cruna_hpc.f90:231-242); the subset loop (computation_direct.f90) has none.
+int step_once(cruna_hip_ctx *c, int nt, bool weighted = true) {
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
   // filter_i1 directly follows the RK4 update unless q-type boundary rows
   // come between them; then it runs inside the update's x sweep
@@ -574,6 +601,17 @@ int step_once(cruna_hip_ctx *c, int nt) {
     }
   }
   c->qs_fresh = mixed && last >= 0;
+  // weight phi of the penalisation (or 1), strength 1: init_filter of
+  // filter_weighted_pen_core_o2_e_c_p.f90:18-37
+  for (int d = 0; d < 3 && weighted; ++d) {
+    const int fq = c->cfg.wfilter_freq[d];
+    if (fq > 0 && nt % fq == 0) {
+      if (!c->wtmp) HIP_TRY(hipMalloc(&c->wtmp, 5 * c->N * sizeof(double)));
+      if ((rc = filter_weighted_dir(c, c->qn, c->wtmp, d, 5, c->ev ? c->wphi : nullptr, nullptr)))
+        return rc;
+      c->qs_fresh = false;  // the FP32 copy of the compact filter pass is stale now
+    }
+  }
   return CRUNA_HIP_OK;
 }
 
@@ -810,6 +848,7 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
         cfg->diff_kind[d] != CRUNA_HIP_DIFF_O4_PADE)
       return fail(CRUNA_HIP_EINVAL, "unknown derivative core");
     if (!(cfg->dx[d] > 0)) return fail(CRUNA_HIP_EINVAL, "dx must be positive");
+    if (cfg->wfilter_freq[d] < 0) return fail(CRUNA_HIP_EINVAL, "wfilter_freq must be >= 0");
   }
   if ((double)cfg->n[0] * cfg->n[1] * cfg->n[2] * sizeof(double) >= 4294967296.0)
     return fail(CRUNA_HIP_EINVAL, "one FP64 field must fit in 4 GiB (32-bit device offsets)");
@@ -905,7 +944,7 @@ int cruna_hip_destroy(cruna_hip_ctx *c) {
   }
   for (void *p : {(void *)c->qn, (void *)c->acc, (void *)c->qref, (void *)c->shape, (void *)c->tmp,
                   (void *)c->qa, (void *)c->traj, (void *)c->gobj, c->met, (void *)c->q0p, c->force,
-                  (void *)c->ckpt, c->tg, c->pen,
+                  (void *)c->ckpt, c->tg, c->pen, (void *)c->wphi, (void *)c->wtmp, (void *)c->wbuf,
                   c->qs, c->rhs, c->G})
     if (p) (void)hipFree(p);
   for (auto &p : c->pending) {
@@ -1015,7 +1054,9 @@ int cruna_hip_set_penalization(cruna_hip_ctx *c, const double *phi, const double
   if (!phi && !inv_phi) {  // back to ns_rho_rhou_p
     (void)hipStreamSynchronize(c->stream);
     if (c->pen) (void)hipFree(c->pen);
+    if (c->wphi) (void)hipFree(c->wphi);
     c->pen = nullptr;
+    c->wphi = nullptr;
     c->ev = false;
     c->pe64 = Penal<double>{};
     c->pe32 = Penal<float>{};
@@ -1044,11 +1085,38 @@ int cruna_hip_set_penalization(cruna_hip_ctx *c, const double *phi, const double
   }
   if (!c->pen) HIP_TRY(hipMalloc(&c->pen, h.size()));
   HIP_TRY(hipMemcpy(c->pen, h.data(), h.size(), hipMemcpyHostToDevice));
+  if (!c->wphi) HIP_TRY(hipMalloc(&c->wphi, N * sizeof(double)));
+  HIP_TRY(hipMemcpy(c->wphi, phi, N * sizeof(double), hipMemcpyHostToDevice));
   if (mixed)
     c->pe32 = Penal<float>{(const float *)c->pen, (const float *)c->pen + N};
   else
     c->pe64 = Penal<double>{(const double *)c->pen, (const double *)c->pen + N};
   c->ev = true;
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_filter_weighted(cruna_hip_ctx *c, int dir, double *q, int nv, const double *weight,
+                              const double *strength) {
+  if (!c || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (dir < 0 || dir > 2) return fail(CRUNA_HIP_EINVAL, "dir must be 0, 1 or 2");
+  if (nv < 1 || nv > 5) return fail(CRUNA_HIP_EINVAL, "nv must be 1..5");
+  if (int rc = ensure_tmp(c)) return rc;
+  const size_t N = c->N;
+  if (!c->wbuf) HIP_TRY(hipMalloc(&c->wbuf, 2 * N * sizeof(double)));
+  double *in = c->tmp, *out = c->tmp + 5 * N;
+  HIP_TRY(hipMemcpyAsync(in, q, nv * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const double *w = nullptr, *s = nullptr;
+  if (weight) {
+    HIP_TRY(hipMemcpyAsync(c->wbuf, weight, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    w = c->wbuf;
+  }
+  if (strength) {
+    HIP_TRY(hipMemcpyAsync(c->wbuf + N, strength, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    s = c->wbuf + N;
+  }
+  if (int rc = filter_weighted_dir(c, in, out, dir, nv, w, s)) return rc;
+  HIP_TRY(hipMemcpyAsync(q, in, nv * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
 
@@ -1259,10 +1327,14 @@ int cruna_hip_run_adjoint(cruna_hip_ctx *c, int nt_end, int nsteps) {
 }
 
 int cruna_hip_run_direct(cruna_hip_ctx *c, int nt0, int nsteps) {
+  return run_direct_loop(c, nt0, nsteps, true);
+}
+
+extern "C++" int run_direct_loop(cruna_hip_ctx *c, int nt0, int nsteps, bool weighted) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
   for (int nt = nt0 + 1; nt <= nt0 + nsteps; ++nt) {
-    if (int rc = step_once(c, nt)) return rc;
+    if (int rc = step_once(c, nt, weighted)) return rc;
     // Q(:,:,:,:,nt) = qn (computation_direct.f90:100-103)
     if (c->record && nt >= 1 && nt <= c->traj_slots)
       HIP_TRY(hipMemcpyAsync(traj_slot(c, nt), c->qn, 5 * c->N * sizeof(double),
@@ -1299,7 +1371,7 @@ int cruna_hip_run_direct_subsets(cruna_hip_ctx *c, int subsets, int steps) {
   for (int s = 0; s < subsets && !rc; ++s) {
     if (hipMemcpyAsync(c->ckpt + (size_t)s * 5 * c->N, c->qn, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
       rc = fail(CRUNA_HIP_EDEVICE, "checkpoint copy failed");
-    if (!rc) rc = cruna_hip_run_direct(c, 0, steps);  // nt = 1..steps within the subset
+    if (!rc) rc = run_direct_loop(c, 0, steps, false);  // nt = 1..steps within the subset
   }
   if (!rc && hipMemcpyAsync(c->ckpt + (size_t)subsets * 5 * c->N, c->qn, bytes, hipMemcpyDeviceToDevice,
                             c->stream) != hipSuccess)
@@ -1324,7 +1396,7 @@ int cruna_hip_run_adjoint_subsets(cruna_hip_ctx *c, int subsets, int steps) {
       rc = fail(CRUNA_HIP_EDEVICE, "checkpoint copy failed");
     c->qs_fresh = false;
     c->record = true;
-    if (!rc) rc = cruna_hip_run_direct(c, 0, steps);
+    if (!rc) rc = run_direct_loop(c, 0, steps, false);
     c->record = false;
     if (!rc) rc = cruna_hip_run_adjoint(c, steps, steps);
   }

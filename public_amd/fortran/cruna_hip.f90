@@ -40,6 +40,7 @@ module cruna_hip
     integer(c_int) :: nbc
     real(c_double) :: bc(10, CRUNA_HIP_MAX_BC)
     integer(c_int) :: diff_kind(3)
+    integer(c_int) :: wfilter_freq(3)
   end type cruna_hip_cfg
 
   interface
@@ -92,6 +93,14 @@ module cruna_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: dir, nv
       real(c_double), intent(inout) :: q(*)
+    end function
+    integer(c_int) function c_filter_weighted(ctx, dir, q, nv, weight, strength) &
+        bind(C, name="cruna_hip_filter_weighted")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: dir, nv
+      real(c_double), intent(inout) :: q(*)
+      real(c_double), intent(in) :: weight(*), strength(*)
     end function
     integer(c_int) function c_rhs_direct(ctx, rhs, q, t) bind(C, name="cruna_hip_rhs_direct")
       import :: c_int, c_ptr, c_double
@@ -277,7 +286,7 @@ module cruna_hip
 
   public :: cruna_hip_default_cfg
   public :: hip_create, hip_destroy, hip_set_state, hip_get_state, hip_set_qref
-  public :: hip_set_sponge_shape, hip_diff, hip_filter, hip_rhs_direct
+  public :: hip_set_sponge_shape, hip_diff, hip_filter, hip_filter_weighted, hip_rhs_direct
   public :: hip_time_stepper_direct, hip_run_direct, hip_synchronize
   public :: hip_rhs_adjoint, hip_time_stepper_adjoint, hip_set_adjoint_state
   public :: hip_get_adjoint_state, hip_trajectory_alloc, hip_trajectory_set
@@ -362,6 +371,17 @@ contains
     integer, intent(in) :: dir
     real(c_double), intent(inout), contiguous :: q(:,:,:,:)
     call check(c_filter(ctx, int(dir - 1, c_int), q, int(size(q, 4), c_int)), "filter")
+  end subroutine
+
+  ! filter_weighted_iN_mpi_cartesian::filter_4d(q, weight, strength) with the
+  ! o2_e_c_p core (src/filter/filter_weighted_i1_mpi_cartesian.f90:15-54)
+  subroutine hip_filter_weighted(ctx, dir, q, weight, strength)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: dir
+    real(c_double), intent(inout), contiguous :: q(:,:,:,:)
+    real(c_double), intent(in), contiguous :: weight(:,:,:), strength(:,:,:)
+    call check(c_filter_weighted(ctx, int(dir - 1, c_int), q, int(size(q, 4), c_int), weight, strength), &
+               "filter_weighted")
   end subroutine
 
   subroutine hip_rhs_direct(ctx, rhs, q, t)

@@ -107,3 +107,44 @@ def test_ev_phi_rhs_converges_to_penalised_equations():
     plain = O.rhs_direct(O.Cfg((n,) * 3, (h,) * 3, mu0=mu0, T0=T0), q, heat_unit_spacing=1)
     ev = O.rhs_direct(O.Cfg((n,) * 3, (h,) * 3, mu0=mu0, T0=T0, penalization=(phi, 1.0 / phi)), q)
     assert np.linalg.norm(plain - ev) / np.linalg.norm(ev) > 1e-2
+
+
+# weighted filter (filter_weighted_iN with the o2_e_c_p core)
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_weighted_filter_transfer_function(d):
+    """weight = strength = 1: q - F(F(q)) with F the 3-point stencil
+    (q+ - 2 q + q-)/4, so a sine wave of step angle w is damped by
+    1 - sin(w/2)^4 (the Nyquist mode is removed, constants are kept)."""
+    shape = (16, 24, 32)
+    cfg = O.Cfg(shape, (1.0, 1.0, 1.0))
+    n = shape[d]
+    for k in (1, 3, n // 4, n // 2):
+        w = 2 * np.pi * k / n
+        i = np.arange(n) * w + 0.3
+        line = np.cos(i) if k != n // 2 else np.cos(np.arange(n) * np.pi)
+        q = np.broadcast_to(line.reshape([-1 if a == d else 1 for a in range(3)]), shape).copy(order="F")
+        got = O.filter_weighted4d(cfg, d, q)
+        np.testing.assert_allclose(got, (1 - np.sin(w / 2) ** 4) * q, atol=1e-14)
+    one = np.full(shape, 2.5, order="F")
+    assert np.abs(O.filter_weighted4d(cfg, d, one) - one).max() < 1e-14
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_weighted_filter_conserves_the_weighted_sum(d):
+    """sum(weight q) is kept along every line (F sums to zero over a periodic
+    line, a0 + 2 a1 = 0: the pressure-tight property of the filter), while q
+    changes; strength 0 leaves q unchanged."""
+    shape = (16, 24, 32)
+    rng = np.random.default_rng(3 + d)
+    q = np.asfortranarray(rng.standard_normal(shape + (2,)))
+    w = np.asfortranarray(rng.uniform(0.2, 1.0, shape))
+    s = np.asfortranarray(rng.uniform(0.0, 1.0, shape))
+    cfg = O.Cfg(shape, (1.0, 1.0, 1.0))
+    got = O.filter_weighted4d(cfg, d, q, w, s)
+    for v in range(2):
+        before = (w * q[..., v]).sum(axis=d)
+        after = (w * got[..., v]).sum(axis=d)
+        assert np.abs(after - before).max() < 1e-12 * np.abs(w * q[..., v]).sum(axis=d).max()
+    assert np.abs(got - q).max() > 1e-2
+    zero = np.zeros(shape, order="F")
+    assert np.array_equal(O.filter_weighted4d(cfg, d, q, w, zero), q)
