@@ -47,3 +47,16 @@ def test_cfg_struct_matches_header():
     body = text[text.index("typedef struct cruna_hip_cfg {"):text.index("} cruna_hip_cfg;")]
     names = re.findall(r"\b(?:int|double)\s+([a-zA-Z_0-9]+)", body)
     assert names == [f[0] for f in Cfg._fields_]
+
+
+IO_HEADER = os.path.join(ROOT, "include", "cruna_io.h")
+IO_LIB = os.path.join(ROOT, "public_amd", "libcruna_io.so")
+
+
+@pytest.mark.skipif(not os.path.exists(IO_LIB), reason="IO library not built (no HDF5)")
+def test_io_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(IO_LIB)
+    syms = sorted(set(re.findall(r"\b(cruna_io_[a-z_0-9]+)\s*\(", open(IO_HEADER).read())))
+    assert len(syms) >= 8
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
