@@ -639,11 +639,55 @@ static void bc_unsupported(int type, const char *where) {
 
 /* set_boundary_condition_q, src/boundaries/boundary_conditions.f90:52-136
  * (types 0, 201, 400, 401). */
+/* characteristic_rhs_k1d, src/boundaries/nonreflecting.f90:237-341 (no
+ * shape): dq = prim(aim) - prim(q) + (rhs dt, rhs_m dt / rho, rhs_5 dt), keep
+ * the incoming waves, rhs <- dq / dt (momentum: times the updated rho). */
+void orc_characteristic_rhs_k1d(int np, int sv, double *rhs, const double *q, const double *aim,
+                                const double k[3], double gamma, double dt) {
+  const double inv_dt = 1.0 / dt;
+  for (int m = 0; m < np; ++m) {
+    double *F = rhs + m;
+    const double *Q = q + m, *A = aim + m;
+    double r = Q[0];                                                /* give_r */
+    const double u[3] = {Q[sv] / Q[0], Q[2 * sv] / Q[0], Q[3 * sv] / Q[0]}; /* give_u */
+    const double c = sqrt(gamma * Q[4 * sv] / Q[0]);                /* give_c */
+    double dq[5];
+    dq[0] = A[0] - r; /* :274-278 */
+    for (int d = 0; d < 3; ++d) dq[1 + d] = A[(1 + d) * sv] / A[0] - u[d];
+    dq[4] = A[4 * sv] - Q[4 * sv];
+    double tmp = dt / r; /* :280-285 */
+    dq[0] = dq[0] + F[0] * dt;
+    for (int d = 0; d < 3; ++d) dq[1 + d] = dq[1 + d] + F[(1 + d) * sv] * tmp;
+    dq[4] = dq[4] + F[4 * sv] * dt;
+    const double kx = -k[0], ky = -k[1], kz = -k[2]; /* :288-290 */
+    double R1 = kx * dq[0] + kz * dq[2] - ky * dq[3] - kx * dq[4] / (c * c); /* :293-297 */
+    double R2 = ky * dq[0] - kz * dq[1] + kx * dq[3] - ky * dq[4] / (c * c);
+    double R3 = kz * dq[0] + ky * dq[1] - kx * dq[2] - kz * dq[4] / (c * c);
+    double R4 = kx * dq[1] + ky * dq[2] + kz * dq[3] + dq[4] / (r * c);
+    double R5 = -kx * dq[1] - ky * dq[2] - kz * dq[3] + dq[4] / (r * c);
+    const double lamS = kx * u[0] + ky * u[1] + kz * u[2]; /* :299-310 */
+    const double lamP = lamS + c, lamM = lamS - c;
+    if (lamS < 0.0) { R1 = 0.0; R2 = 0.0; R3 = 0.0; }
+    if (lamP < 0.0) R4 = 0.0;
+    if (lamM < 0.0) R5 = 0.0;
+    dq[0] = kx * R1 + ky * R2 + kz * R3 + r / (2.0 * c) * R4 + r / (2.0 * c) * R5; /* :313-318 */
+    dq[1] = -kz * R2 + ky * R3 + 0.5 * kx * R4 - 0.5 * kx * R5;
+    dq[2] = kz * R1 - kx * R3 + 0.5 * ky * R4 - 0.5 * ky * R5;
+    dq[3] = -ky * R1 + kx * R2 + 0.5 * kz * R4 - 0.5 * kz * R5;
+    dq[4] = +0.5 * r * c * R4 + 0.5 * r * c * R5;
+    r = r + dq[0]; /* :321 */
+    F[0] = dq[0] * inv_dt; /* :333-339 */
+    tmp = inv_dt * r;
+    for (int d = 0; d < 3; ++d) F[(1 + d) * sv] = dq[1 + d] * tmp;
+    F[4 * sv] = dq[4] * inv_dt;
+  }
+}
+
 void orc_bc_q(const orc_cfg *c, double *q, const double *qref, const double *shape) {
   const size_t N = npts(c);
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
-    if (t == 0 || t == 400) continue;
+    if (t == 0 || t == 100 || t == 204 || t == 400) continue; /* RHS rows */
     if (t == 201) {
       const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
       FOR_BOX(c, r, idx) orc_characteristic_q_k1d(1, (int)N, q + idx, qref + idx, k, c->gamma);
@@ -663,7 +707,7 @@ void orc_bc_qs(const orc_cfg *c, double *qs, const double *q, const double *shap
   const size_t N = npts(c);
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
-    if (t == 0 || t == 400) continue;
+    if (t == 0 || t == 100 || t == 204 || t == 400) continue; /* RHS rows */
     if (t == 201) {
       const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
       FOR_BOX(c, r, idx) orc_characteristic_qs_k1d(1, (int)N, qs + idx, q + idx, k, c->gamma);
@@ -677,7 +721,7 @@ void orc_bc_qs(const orc_cfg *c, double *qs, const double *q, const double *shap
   }
 }
 
-/* set_boundary_condition_rhs, boundary_conditions.f90:228-298 (types 0, 201, 400, 401). */
+/* set_boundary_condition_rhs, boundary_conditions.f90:228-298 (types 0, 100, 201, 204, 400, 401). */
 void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qref,
                 const double *shape) {
   const size_t N = npts(c);
@@ -689,6 +733,14 @@ void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qr
       for (int v = 0; v < NV; ++v)
         for (size_t i = 0; i < N; ++i)
           rhs[v * N + i] = rhs[v * N + i] - shape[i] * (q[v * N + i] - qref[v * N + i]);
+    } else if (t == 100) {
+      /* wall_slip_no_thermodynamics(rhs(box), nint(row(8))), wall_boundary_conditions.f90:13-23 */
+      const int v = (int)lround(c->bc[r][7]) - 1;
+      FOR_BOX(c, r, idx) rhs[v * N + idx] = 0.0;
+    } else if (t == 204) {
+      /* characteristic_rhs_k1d(rhs(box), q(box), qref(box), row(8:10)) */
+      const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
+      FOR_BOX(c, r, idx) orc_characteristic_rhs_k1d(1, (int)N, rhs + idx, q + idx, qref + idx, k, c->gamma, c->dt);
     } else {
       bc_unsupported(t, "set_boundary_condition_rhs");
     }
@@ -700,7 +752,7 @@ void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qr
 static void bc_rhss(const orc_cfg *c) {
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
-    if (t == 0 || t == 201 || t == 400 || t == 401) continue;
+    if (t == 0 || t == 201 || t == 204 || t == 400 || t == 401) continue; /* 204: commented out */
     bc_unsupported(t, "set_boundary_condition_rhss");
   }
 }

@@ -113,6 +113,8 @@ void orc_bc_q(const orc_cfg *c, double *q, const double *qref, const double *sha
 void orc_bc_qs(const orc_cfg *c, double *qs, const double *q, const double *shape);
 void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qref,
                 const double *shape);
+void orc_characteristic_rhs_k1d(int np, int sv, double *rhs, const double *q, const double *aim,
+                                const double k[3], double gamma, double dt);
 void orc_characteristic_q_k1d(int npts, int stride_v, double *q, const double *aim,
                               const double k[3], double gamma);
 void orc_characteristic_qs_k1d(int npts, int stride_v, double *qs, const double *q,

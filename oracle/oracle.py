@@ -84,6 +84,8 @@ def lib():
         L.orc_rhs_direct_gt.argtypes = [P, _dp, _dp, _dp, _dp, _vp, _vp]
         L.orc_rhs_direct_ev.argtypes = [P, _dp, _dp, _dp, _dp, _vp, _vp]
         L.orc_filter_weighted4d.argtypes = [P, C.c_int, _dp, C.c_int, _vp, _vp]
+        L.orc_characteristic_rhs_k1d.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_double,
+                                                 C.c_double]
         L.orc_rhs_adjoint.argtypes = [P, _dp, _dp, _dp, _vp, _vp]
         L.orc_bc_q.argtypes = [P, _dp, _vp, _vp]
         L.orc_bc_qs.argtypes = [P, _dp, _dp, _vp]
@@ -223,6 +225,14 @@ def rhs_adjoint(cfg, qs, q, g=None, shape=None):
     g = None if g is None else _f(g)
     shape = None if shape is None else _f(shape)
     lib().orc_rhs_adjoint(cfg.ref(), rhs, qs, q, _ptr(g), _ptr(shape))
+    return rhs
+
+
+def characteristic_rhs(rhs, q, aim, k, gamma, dt):
+    """characteristic_rhs_k1d on m points: rhs, q, aim (m, 5); returns the new rhs."""
+    rhs = _f(rhs).copy(order="F")
+    m = rhs.shape[0]
+    lib().orc_characteristic_rhs_k1d(m, m, rhs, _f(q), _f(aim), _f(np.asarray(k, float)), gamma, dt)
     return rhs
 
 

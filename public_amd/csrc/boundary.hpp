@@ -1,6 +1,7 @@
 // Boundary conditions of src/boundaries/boundary_conditions.f90 used by the
-// hot path: type 201 (characteristic, nonreflecting toward qref), 400 (sponge
-// on the RHS) and 401 (sponge on q).  Point-wise kernels, one thread per
+// hot path: type 201 (characteristic, nonreflecting toward qref), 204 (the
+// same on the RHS), 100 (slip wall on the RHS), 400 (sponge on the RHS) and
+// 401 (sponge on q).  Point-wise kernels, one thread per
 // point; the characteristic decomposition always runs in FP64, also in the
 // FP32-mixed mode.
 #pragma once
@@ -65,6 +66,76 @@ __global__ __launch_bounds__(256) void k_bc_characteristic_q(Q *__restrict__ q,
   q[2 * N + o] = Q(r * u2);
   q[3 * N + o] = Q(r * u3);
   q[4 * N + o] = Q(p);
+}
+
+// characteristic_rhs_k1d (src/boundaries/nonreflecting.f90:237-341, type 204)
+// on the complete RHS of the points of one box: the change that the RHS
+// makes over dt plus the change toward aim, in primitive variables, keeps
+// only its incoming waves and becomes the RHS again.
+template <typename T, typename Q>
+__global__ __launch_bounds__(256) void k_bc_characteristic_rhs(T *__restrict__ rhs,
+                                                                const Q *__restrict__ q,
+                                                                const double *__restrict__ aim,
+                                                                BoxRow row, int n1, int n2, long N,
+                                                                double gamma, double dt) {
+  const long nx = row.i1 - row.i0 + 1, ny = row.j1 - row.j0 + 1, nz = row.k1 - row.k0 + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nx * ny * nz) return;
+  const long i = row.i0 + t % nx, j = row.j0 + (t / nx) % ny, k = row.k0 + t / (nx * ny);
+  const long o = i + (long)n1 * (j + (long)n2 * k);
+  const double inv_dt = 1.0 / dt;
+  double r = double(q[o]);
+  const double u1 = double(q[N + o]) / double(q[o]);
+  const double u2 = double(q[2 * N + o]) / double(q[o]);
+  const double u3 = double(q[3 * N + o]) / double(q[o]);
+  const double c = sqrt(gamma * double(q[4 * N + o]) / double(q[o]));
+  // change required by aim (:274-278), plus the RHS over dt (:280-285)
+  double dq1 = aim[o] - r;
+  double dq2 = aim[N + o] / aim[o] - u1;
+  double dq3 = aim[2 * N + o] / aim[o] - u2;
+  double dq4 = aim[3 * N + o] / aim[o] - u3;
+  double dq5 = aim[4 * N + o] - double(q[4 * N + o]);
+  double tmp = dt / r;
+  dq1 = dq1 + double(rhs[o]) * dt;
+  dq2 = dq2 + double(rhs[N + o]) * tmp;
+  dq3 = dq3 + double(rhs[2 * N + o]) * tmp;
+  dq4 = dq4 + double(rhs[3 * N + o]) * tmp;
+  dq5 = dq5 + double(rhs[4 * N + o]) * dt;
+  const double kx = -row.k[0], ky = -row.k[1], kz = -row.k[2];
+  double R1 = kx * dq1 + kz * dq3 - ky * dq4 - kx * dq5 / (c * c);
+  double R2 = ky * dq1 - kz * dq2 + kx * dq4 - ky * dq5 / (c * c);
+  double R3 = kz * dq1 + ky * dq2 - kx * dq3 - kz * dq5 / (c * c);
+  double R4 = kx * dq2 + ky * dq3 + kz * dq4 + dq5 / (r * c);
+  double R5 = -kx * dq2 - ky * dq3 - kz * dq4 + dq5 / (r * c);
+  const double lamS = kx * u1 + ky * u2 + kz * u3;
+  if (lamS < 0.0) R1 = R2 = R3 = 0.0;
+  if (lamS + c < 0.0) R4 = 0.0;
+  if (lamS - c < 0.0) R5 = 0.0;
+  dq1 = kx * R1 + ky * R2 + kz * R3 + r / (2.0 * c) * R4 + r / (2.0 * c) * R5;
+  dq2 = -kz * R2 + ky * R3 + 0.5 * kx * R4 - 0.5 * kx * R5;
+  dq3 = kz * R1 - kx * R3 + 0.5 * ky * R4 - 0.5 * ky * R5;
+  dq4 = -ky * R1 + kx * R2 + 0.5 * kz * R4 - 0.5 * kz * R5;
+  dq5 = +0.5 * r * c * R4 + 0.5 * r * c * R5;
+  // back to the RHS (:321-339, no shape)
+  r = r + dq1;
+  rhs[o] = T(dq1 * inv_dt);
+  tmp = inv_dt * r;
+  rhs[N + o] = T(dq2 * tmp);
+  rhs[2 * N + o] = T(dq3 * tmp);
+  rhs[3 * N + o] = T(dq4 * tmp);
+  rhs[4 * N + o] = T(dq5 * inv_dt);
+}
+
+// wall_slip_no_thermodynamics (src/boundaries/wall_boundary_conditions.f90:13-23,
+// type 100): the RHS of one variable (v, 0-based) is zero on the box.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bc_wall_rhs(T *__restrict__ rhs, BoxRow row, int v, int n1,
+                                                      int n2, long N) {
+  const long nx = row.i1 - row.i0 + 1, ny = row.j1 - row.j0 + 1, nz = row.k1 - row.k0 + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nx * ny * nz) return;
+  const long i = row.i0 + t % nx, j = row.j0 + (t / nx) % ny, k = row.k0 + t / (nx * ny);
+  rhs[(long)v * N + i + (long)n1 * (j + (long)n2 * k)] = T(0);
 }
 
 // sponge_q with qref (src/boundaries/sponge_layer_core.f90:70-103):

@@ -104,6 +104,9 @@ struct cruna_hip_ctx {
   // boundary_conditions.dat rows in file order (types 201, 400, 401)
   std::vector<std::pair<int, BoxRow>> bc;
   bool has_bcq = false, has_bcrhs = false;
+  // rows that act on the complete RHS (100 wall, 204 characteristic): the
+  // last sweep then writes the RHS, the rows follow, then the RK4 update
+  bool has_bclate = false, has_wall = false;
   // deformed grid (cruna_hip_set_metric): per-line metric entries and Jacobian
   void *met = nullptr;
   Metric<double> mt64{};
@@ -312,6 +315,9 @@ Penal<float> penal<float>(cruna_hip_ctx *c) { return c->pe32; }
 
 // ns_rho_rhou_p_ev_phi.f90:295-309: the penalised module's adjoint RHS stops
 // with "not yet implemented"
+// set_boundary_condition_rhss zeroes a variable of the adjoint RHS on wall
+// rows (100); the device adjoint does not apply rows to its complete RHS
+const char *kNoWallAdjoint = "wall rows (100) are not implemented in the device adjoint";
 const char *kNoEvAdjoint =
     "the adjoint of ns_rho_rhou_p_ev_phi is not implemented (the reference stops there too)";
 
@@ -351,18 +357,59 @@ int bc_q(cruna_hip_ctx *c, Q *q) {
   return CRUNA_HIP_OK;
 }
 
-// set_boundary_condition_rhs (boundary_conditions.f90:228-298): type 400.
+// set_boundary_condition_rhs (boundary_conditions.f90:228-298), rows in order.
+// Without 100/204 rows only the sponge (400) is there; it is linear and runs
+// before the last sweep, its share pre-divided by what that sweep multiplies
+// in.  With them (late), every row acts on the complete RHS.
 template <typename T, typename Q>
-int bc_rhs(cruna_hip_ctx *c, T *rhs, const Q *q) {
+int bc_rhs(cruna_hip_ctx *c, T *rhs, const Q *q, bool late = false) {
   if (int rc = need_bc_inputs(c)) return rc;
-  for (auto &r : c->bc)
+  const int n1 = c->cfg.n[0], n2 = c->cfg.n[1];
+  for (auto &r : c->bc) {
+    const BoxRow &b = r.second;
+    const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
     if (r.first == 400) {
       hipLaunchKernelGGL((k_sponge_rhs<T, Q>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream,
                          rhs, q, (const double *)c->qref, (const double *)c->shape, c->N,
-                         c->cfg.gamma - 1.0, jacobian<T>(c));
-      if (int rc = check_launch()) return rc;
+                         late ? 1.0 : c->cfg.gamma - 1.0, late ? nullptr : jacobian<T>(c));
+    } else if (late && r.first == 204) {
+      hipLaunchKernelGGL((k_bc_characteristic_rhs<T, Q>), dim3((np + 255) / 256), dim3(256), 0,
+                         c->stream, rhs, q, (const double *)c->qref, b, n1, n2, c->N, c->cfg.gamma,
+                         c->cfg.dt);
+    } else if (late && r.first == 100) {
+      hipLaunchKernelGGL((k_bc_wall_rhs<T>), dim3((np + 255) / 256), dim3(256), 0, c->stream, rhs, b,
+                         (int)b.k[0], n1, n2, c->N);
+    } else {
+      continue;
     }
+    if (int rc = check_launch()) return rc;
+  }
   return CRUNA_HIP_OK;
+}
+
+// RK4 stage update of all five variables from the complete RHS in rhs, point
+// by point: rk_update's arithmetic, for stages whose RHS rows need the
+// complete RHS first.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rk_stage(int mode, const T *__restrict__ rhs,
+                                                   double *__restrict__ qn, double *__restrict__ acc,
+                                                   T *__restrict__ qs_out, double dt, long N) {
+  const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= N) return;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const long p = v * N + o;
+    const double k = double(rhs[p]);
+    if (mode == 3) {
+      qn[p] = acc[p] + dt / 6.0 * k;
+      continue;
+    }
+    const double q0 = qn[p];
+    const double a = mode == 0 ? q0 + dt / 6.0 * k : acc[p] + dt / 3.0 * k;
+    acc[p] = a;
+    const double c_stage = mode == 2 ? dt : dt * 0.5;
+    qs_out[p] = T(q0 + c_stage * k);
+  }
 }
 
 template <typename T, typename Q>
@@ -494,11 +541,18 @@ int rhs_stage(cruna_hip_ctx *c, const Q *in, int mode, bool fuse_filter = false)
   if (int rc = launch_sweep1<T, Q, 2>(c, in)) return rc;
   if (int rc = launch_sweep2<T, 2>(c, mode, false)) return rc;
   if (int rc = launch_sweep2<T, 1>(c, mode, false)) return rc;
-  if (c->has_bcrhs)
+  if (c->has_bcrhs && !c->has_bclate)
     if (int rc = bc_rhs<T, Q>(c, (T *)c->rhs, in)) return rc;
   if (c->cnn)
     if (int rc = closure_term<T, Q>(c, in)) return rc;
-  return launch_sweep2<T, 0>(c, mode, fuse_filter);
+  if (!c->has_bclate) return launch_sweep2<T, 0>(c, mode, fuse_filter);
+  // complete RHS, the rows on it, then the stage update
+  if (int rc = launch_sweep2<T, 0>(c, 4, false)) return rc;
+  if (int rc = bc_rhs<T, Q>(c, (T *)c->rhs, in, true)) return rc;
+  if (mode == 4) return CRUNA_HIP_OK;
+  hipLaunchKernelGGL((k_rk_stage<T>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream, mode,
+                     (const T *)c->rhs, c->qn, c->acc, (T *)c->qs, c->cfg.dt, c->N);
+  return check_launch();
 }
 
 // A stage from the FP64 state `from` (stage 0, or a single RHS evaluation) or,
@@ -579,7 +633,8 @@ int step_once(cruna_hip_ctx *c, int nt, bool weighted = true) {
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
   // filter_i1 directly follows the RK4 update unless q-type boundary rows
   // come between them; then it runs inside the update's x sweep
-  const bool fuse_fx = !c->has_bcq && c->cfg.filter_freq[0] > 0 && nt % c->cfg.filter_freq[0] == 0;
+  const bool fuse_fx =
+      !c->has_bcq && !c->has_bclate && c->cfg.filter_freq[0] > 0 && nt % c->cfg.filter_freq[0] == 0;
   int rc = mixed ? rk4_step<float>(c, fuse_fx) : rk4_step<double>(c, fuse_fx);
   if (rc) return rc;
   // set_boundary_condition_q(qn, outside_rhs=.true.) (cruna_hpc.f90:216)
@@ -857,7 +912,7 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   if (cfg->nbc < 0 || cfg->nbc > CRUNA_HIP_MAX_BC) return fail(CRUNA_HIP_EINVAL, "bad nbc");
   for (int r = 0; r < cfg->nbc; ++r) {
     const int t = (int)lround(cfg->bc[r][0]);
-    if (t != 0 && t != 201 && t != 400 && t != 401)
+    if (t != 0 && t != 100 && t != 201 && t != 204 && t != 400 && t != 401)
       return fail(CRUNA_HIP_EINVAL,
                   "boundary type " + std::to_string(t) + " not implemented on the device");
   }
@@ -880,9 +935,16 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
     }
     if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) continue;
     b.i0 = lo[0]; b.i1 = hi[0]; b.j0 = lo[1]; b.j1 = hi[1]; b.k0 = lo[2]; b.k1 = hi[2];
+    if (t == 100) {  // the variable whose RHS is zero (1-based, in column 8)
+      const int v = (int)lround(cfg->bc[r][7]);
+      if (v < 1 || v > 5) return fail(CRUNA_HIP_EINVAL, "wall row (100): column 8 must name a variable 1..5");
+      b.k[0] = v - 1;
+    }
     c->bc.push_back({t, b});
     if (t == 201 || t == 401) c->has_bcq = true;
     if (t == 400) c->has_bcrhs = true;
+    if (t == 100 || t == 204) c->has_bclate = true;
+    if (t == 100) c->has_wall = true;
   }
   for (int d = 0; d < 3; ++d) {
     if (int rc = build_geom(c.get(), d)) return rc;
@@ -1213,6 +1275,7 @@ int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const
   (void)t;  // the adjoint RHS of the reference does not depend on t
   if (!c || !rhs || !qs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
+  if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
   if (int rc = stage_adjoint_inputs(c, qs, q, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
   if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
@@ -1240,6 +1303,7 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *
   (void)t;
   if (!c || !qs1 || !qs0 || !q0) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
+  if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
   if (int rc = stage_adjoint_inputs(c, qs0, q0, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
   if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
@@ -1313,6 +1377,7 @@ int cruna_hip_trajectory_record(cruna_hip_ctx *c, int on) {
 int cruna_hip_run_adjoint(cruna_hip_ctx *c, int nt_end, int nsteps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
+  if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
   if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
   if (!c->qa) return fail(CRUNA_HIP_EINVAL, "no adjoint state (cruna_hip_set_adjoint_state)");
   if (c->traj_slots == 0) return fail(CRUNA_HIP_EINVAL, "no trajectory (cruna_hip_trajectory_alloc)");
@@ -1383,6 +1448,7 @@ int cruna_hip_run_direct_subsets(cruna_hip_ctx *c, int subsets, int steps) {
 int cruna_hip_run_adjoint_subsets(cruna_hip_ctx *c, int subsets, int steps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
+  if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
   if (subsets < 1 || steps < 1) return fail(CRUNA_HIP_EINVAL, "subsets and steps must be >= 1");
   if (c->ckpt_n < subsets + 1) return fail(CRUNA_HIP_EINVAL, "not enough subset checkpoints (cruna_hip_subsets_alloc)");
   if (c->traj_slots < steps) return fail(CRUNA_HIP_EINVAL, "the trajectory holds fewer slots than steps");
@@ -1607,6 +1673,7 @@ int cruna_hip_ensemble_set_role(cruna_hip_ensemble *e, int i, int role, int nt) 
   cruna_hip_ctx *c = e->m[i];
   if (role == CRUNA_HIP_ADJOINT) {
     if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
+    if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
     if (!c->qa || c->traj_slots == 0)
       return fail(CRUNA_HIP_EINVAL, "an adjoint member needs its adjoint state and trajectory first");
     if (c->traj_slots > 1 && (nt < 1 || nt > c->traj_slots))

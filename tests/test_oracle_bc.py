@@ -1,0 +1,116 @@
+"""Pins of the oracle's RHS boundary rows: characteristic_rhs_k1d (type 204,
+src/boundaries/nonreflecting.f90:237-341) and the slip wall (type 100).
+
+The wave transform and its back-transform must be inverses: when every wave
+enters the domain nothing is removed, and the new RHS is the closed form
+rho_t = (rho_aim - rho)/dt + rho_t, u_t = ((u_aim - u)/dt + m_t/rho) (rho + d rho)
+(momentum slots), p_t = (p_aim - p)/dt + p_t, for any unit normal; when
+every wave leaves it, the RHS is zero; aim = q with a zero RHS stays zero."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GAMMA, DT = 1.4, 2e-3
+
+
+def _states(m, seed, speed):
+    rng = np.random.default_rng(seed)
+    q = np.zeros((m, 5), order="F")
+    rho = 1 + 0.1 * rng.random(m)
+    p = 1 / 1.4 * (1 + 0.1 * rng.random(m))
+    c = np.sqrt(GAMMA * p / rho)
+    return q, rho, p, c, rng
+
+
+def _conservative(rho, u, p):
+    q = np.zeros((rho.size, 5), order="F")
+    q[:, 0] = rho
+    q[:, 1:4] = rho[:, None] * u
+    q[:, 4] = p
+    return q
+
+
+@pytest.mark.parametrize("k", [(0.0, -1.0, 0.0), (0.6, 0.0, 0.8), (-0.48, 0.6, 0.64)])
+def test_all_incoming_keeps_every_wave(k):
+    m = 16
+    _, rho, p, c, rng = _states(m, 1, 0)
+    k = np.asarray(k) / np.linalg.norm(k)
+    # velocity along -k (into the domain) faster than sound: every eigenvalue of (-k).u >= 0
+    u = (-k)[None, :] * (c * 1.5)[:, None] + 0.05 * rng.standard_normal((m, 3))
+    q = _conservative(rho, u, p)
+    aim = _conservative(rho * 1.02, u * 0.97, p * 1.01)
+    rhs = np.asfortranarray(rng.standard_normal((m, 5)) * 0.1)
+    got = O.characteristic_rhs(rhs, q, aim, k, GAMMA, DT)
+    d_rho = aim[:, 0] - rho + rhs[:, 0] * DT
+    exp = np.zeros_like(got)
+    exp[:, 0] = d_rho / DT
+    u_aim = aim[:, 1:4] / aim[:, :1]
+    du = (u_aim - u) + rhs[:, 1:4] * (DT / rho)[:, None]
+    exp[:, 1:4] = du * ((rho + d_rho) / DT)[:, None]
+    exp[:, 4] = (aim[:, 4] - p + rhs[:, 4] * DT) / DT
+    np.testing.assert_allclose(got, exp, rtol=1e-11, atol=1e-11)
+
+
+def test_all_outgoing_removes_every_wave():
+    m = 8
+    _, rho, p, c, rng = _states(m, 2, 0)
+    k = np.array([1.0, 0.0, 0.0])
+    u = k[None, :] * (c * 1.3)[:, None]  # along +k: out of the domain, supersonic
+    q = _conservative(rho, u, p)
+    aim = _conservative(rho * 1.05, u * 0.9, p * 1.02)
+    rhs = np.asfortranarray(rng.standard_normal((m, 5)))
+    got = O.characteristic_rhs(rhs, q, aim, k, GAMMA, DT)
+    assert np.abs(got).max() < 1e-12
+
+
+def test_aim_equal_to_state_and_zero_rhs_stays_zero():
+    m = 8
+    _, rho, p, c, rng = _states(m, 3, 0)
+    u = 0.3 * rng.standard_normal((m, 3))
+    q = _conservative(rho, u, p)
+    got = O.characteristic_rhs(np.zeros((m, 5), order="F"), q, q, (0.0, 0.0, -1.0), GAMMA, DT)
+    assert np.abs(got).max() < 1e-13
+
+
+def test_subsonic_outflow_keeps_only_the_upstream_acoustic_wave():
+    """Subsonic outflow through +x: the inward normal is -x, so lamS = -u < 0
+    and lamM = -u - c < 0 remove R1..R3 and R5, and only R4 (lamP = c - u > 0)
+    survives: the new RHS is R4's eigenvector (rho/(2c), -1/2, 0, 0, rho c/2)
+    times R4 over dt (momentum: times rho + d rho)."""
+    m = 6
+    _, rho, p, c, rng = _states(m, 4, 0)
+    u = np.zeros((m, 3))
+    u[:, 0] = 0.4 * c
+    q = _conservative(rho, u, p)
+    aim = _conservative(rho * 1.01, u * 1.03, p * 0.99)
+    rhs = np.asfortranarray(rng.standard_normal((m, 5)) * 0.05)
+    got = O.characteristic_rhs(rhs, q, aim, (1.0, 0.0, 0.0), GAMMA, DT)
+    du = aim[:, 1:4] / aim[:, :1] - u + rhs[:, 1:4] * (DT / rho)[:, None]
+    dp = aim[:, 4] - p + rhs[:, 4] * DT
+    R4 = -du[:, 0] + dp / (rho * c)
+    d_rho = rho / (2 * c) * R4
+    exp = np.zeros_like(got)
+    exp[:, 0] = d_rho / DT
+    exp[:, 1] = -0.5 * R4 * (rho + d_rho) / DT
+    exp[:, 4] = 0.5 * rho * c * R4 / DT
+    np.testing.assert_allclose(got, exp, rtol=1e-11, atol=1e-12)
+
+
+def test_wall_and_characteristic_rows_in_the_rhs():
+    """Rows 100 and 204 act on the finished RHS through orc_bc_rhs, in row order,
+    only inside their boxes."""
+    n = (16, 16, 16)
+    rng = np.random.default_rng(5)
+    q = np.zeros(n + (5,), order="F")
+    q[..., 0] = 1 + 0.05 * rng.random(n)
+    q[..., 1:4] = 0.1 * rng.standard_normal(n + (3,))
+    q[..., 4] = 1 / 1.4
+    rhs = np.asfortranarray(rng.standard_normal(n + (5,)))
+    cfg = O.Cfg(n, (0.1,) * 3, dt=DT, bc=[(100, 1, 16, 1, 1, 1, 16, 3), (204, 1, 16, 16, 16, 1, 16, 0, 1, 0)])
+    got = O.bc_rhs(cfg, rhs, q, qref=q)
+    assert np.array_equal(got[:, 0, :, 2], np.zeros((16, 16)))
+    assert np.array_equal(got[:, 1:15], rhs[:, 1:15])
+    face = O.characteristic_rhs(rhs[:, 15].reshape(-1, 5, order="F"), q[:, 15].reshape(-1, 5, order="F"),
+                                q[:, 15].reshape(-1, 5, order="F"), (0, 1, 0), 1.4, DT)
+    np.testing.assert_allclose(got[:, 15].reshape(-1, 5, order="F"), face, rtol=0, atol=0)
