@@ -65,7 +65,9 @@ typedef struct cruna_hip_cfg {
                             line count with (n/8) * tile_lines <= 512 */
   /* boundary_conditions.dat rows: type; i1s; i1e; i2s; i2e; i3s; i3e; k1; k2; k3.
      Types 0, 100 (slip wall: column 8 names the variable whose RHS is zero),
-     201, 204 (characteristic on q / on the RHS), 400, 401 (sponge on the RHS / on q) */
+     200 (characteristic toward the inner neighbour; box one point thick along
+     the normal), 201, 204 (characteristic on q / on the RHS), 400, 401
+     (sponge on the RHS / on q) */
   int nbc;
   double bc[CRUNA_HIP_MAX_BC][10];
   int diff_kind[3];      /* derivative core per direction (discretisation_x_core):

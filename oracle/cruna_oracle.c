@@ -683,6 +683,20 @@ void orc_characteristic_rhs_k1d(int np, int sv, double *rhs, const double *q, co
   }
 }
 
+/* characteristic_zero_grad_init, src/boundaries/nonreflecting.f90:665-728:
+ * offset of the inner neighbour (one point against the sign of each
+ * component of the normalised kvector). */
+static long zero_grad_shift(const orc_cfg *c, int r) {
+  const double *kv = &c->bc[r][7];
+  const double kn = sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+  long s[3];
+  for (int d = 0; d < 3; ++d) {
+    const double k = kv[d] / kn;
+    s[d] = k > 0 ? -1 : (k < 0 ? 1 : 0);
+  }
+  return s[0] + (long)c->n[0] * (s[1] + (long)c->n[1] * s[2]);
+}
+
 void orc_bc_q(const orc_cfg *c, double *q, const double *qref, const double *shape) {
   const size_t N = npts(c);
   for (int r = 0; r < c->nbc; ++r) {
@@ -691,6 +705,11 @@ void orc_bc_q(const orc_cfg *c, double *q, const double *qref, const double *sha
     if (t == 201) {
       const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
       FOR_BOX(c, r, idx) orc_characteristic_q_k1d(1, (int)N, q + idx, qref + idx, k, c->gamma);
+    } else if (t == 200) {
+      /* characteristic_q(q(box), q(box of inner neighbours), row(8:10)), boundary_conditions.f90:80-83 */
+      const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
+      const long sh = zero_grad_shift(c, r);
+      FOR_BOX(c, r, idx) orc_characteristic_q_k1d(1, (int)N, q + idx, q + idx + sh, k, c->gamma);
     } else if (t == 401) {
       /* sponge_q with qref, src/boundaries/sponge_layer_core.f90:70-103 */
       for (int v = 0; v < NV; ++v)
@@ -711,6 +730,11 @@ void orc_bc_qs(const orc_cfg *c, double *qs, const double *q, const double *shap
     if (t == 201) {
       const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
       FOR_BOX(c, r, idx) orc_characteristic_qs_k1d(1, (int)N, qs + idx, q + idx, k, c->gamma);
+    } else if (t == 200) {
+      /* characteristic_qs(qs(box), q(box of inner neighbours), row(8:10)), boundary_conditions.f90:167-170 */
+      const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
+      const long sh = zero_grad_shift(c, r);
+      FOR_BOX(c, r, idx) orc_characteristic_qs_k1d(1, (int)N, qs + idx, q + idx + sh, k, c->gamma);
     } else if (t == 401) {
       /* sponge_qs, sponge_layer_core.f90:155-167 */
       for (int v = 0; v < NV; ++v)
@@ -727,7 +751,7 @@ void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qr
   const size_t N = npts(c);
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
-    if (t == 0 || t == 201 || t == 401) continue;
+    if (t == 0 || t == 200 || t == 201 || t == 401) continue;
     if (t == 400) {
       /* sponge_rhs with qref, sponge_layer_core.f90:34-66 */
       for (int v = 0; v < NV; ++v)
@@ -752,7 +776,7 @@ void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qr
 static void bc_rhss(const orc_cfg *c) {
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
-    if (t == 0 || t == 201 || t == 204 || t == 400 || t == 401) continue; /* 204: commented out */
+    if (t == 0 || t == 200 || t == 201 || t == 204 || t == 400 || t == 401) continue; /* 204: commented out */
     bc_unsupported(t, "set_boundary_condition_rhss");
   }
 }

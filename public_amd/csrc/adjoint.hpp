@@ -169,7 +169,8 @@ __global__ __launch_bounds__(256) void k_bc_characteristic_qs(Q *__restrict__ qs
                                                                const double *const *__restrict__ q0p,
                                                                BoxRow row, int n1, int n2, long N,
                                                                double gamma) {
-  const double *__restrict__ q = *q0p;
+  // forward state at the point (201) or at its inner neighbour (200)
+  const double *__restrict__ q = *q0p + row.aim_shift;
   const long nx = row.i1 - row.i0 + 1, ny = row.j1 - row.j0 + 1, nz = row.k1 - row.k0 + 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nx * ny * nz) return;

@@ -13,31 +13,35 @@ namespace cruna {
 struct BoxRow {
   int i0, i1, j0, j1, k0, k1;  // 0-based inclusive
   double k[3];                 // outward normal (kvector), as given in the row
+  long aim_shift;              // type 200: offset of the inner neighbour (zero-gradient aim)
 };
 
 // characteristic_q_k1d (src/boundaries/nonreflecting.f90:46-137) on the
 // points of one box: q <- q + (incoming part of prim(aim) - prim(q)).
-template <typename Q>
-__global__ __launch_bounds__(256) void k_bc_characteristic_q(Q *__restrict__ q,
-                                                              const double *__restrict__ aim,
-                                                              BoxRow row, int n1, int n2, long N,
-                                                              double gamma) {
+// Type 201: aim = qref at the same point (row.aim_shift 0); type 200: aim =
+// q at the inner neighbour (characteristic_zero_grad_init, :665-728), a point
+// outside the one-point-thick box, so the in-place update reads no value it
+// writes.
+template <typename Q, typename A>
+__global__ __launch_bounds__(256) void k_bc_characteristic_q(Q *q, const A *aim_base, BoxRow row,
+                                                              int n1, int n2, long N, double gamma) {
   const long nx = row.i1 - row.i0 + 1, ny = row.j1 - row.j0 + 1, nz = row.k1 - row.k0 + 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nx * ny * nz) return;
   const long i = row.i0 + t % nx, j = row.j0 + (t / nx) % ny, k = row.k0 + t / (nx * ny);
   const long o = i + (long)n1 * (j + (long)n2 * k);
+  const A *aim = aim_base + row.aim_shift;
   double r = double(q[o]);
   double u1 = double(q[N + o]) / double(q[o]);
   double u2 = double(q[2 * N + o]) / double(q[o]);
   double u3 = double(q[3 * N + o]) / double(q[o]);
   double p = double(q[4 * N + o]);
   const double c = sqrt(gamma * double(q[4 * N + o]) / double(q[o]));
-  double dq1 = aim[o] - r;
-  double dq2 = aim[N + o] / aim[o] - u1;
-  double dq3 = aim[2 * N + o] / aim[o] - u2;
-  double dq4 = aim[3 * N + o] / aim[o] - u3;
-  double dq5 = aim[4 * N + o] - p;
+  double dq1 = double(aim[o]) - r;
+  double dq2 = double(aim[N + o]) / double(aim[o]) - u1;
+  double dq3 = double(aim[2 * N + o]) / double(aim[o]) - u2;
+  double dq4 = double(aim[3 * N + o]) / double(aim[o]) - u3;
+  double dq5 = double(aim[4 * N + o]) - p;
   const double kx = -row.k[0], ky = -row.k[1], kz = -row.k[2];
   // primitive -> characteristic waves (:97-101)
   double R1 = kx * dq1 + kz * dq3 - ky * dq4 - kx * dq5 / (c * c);

@@ -345,9 +345,14 @@ int bc_q(cruna_hip_ctx *c, Q *q) {
     if (r.first == 201) {
       const BoxRow &b = r.second;
       const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
-      hipLaunchKernelGGL((k_bc_characteristic_q<Q>), dim3((np + 255) / 256), dim3(256), 0,
+      hipLaunchKernelGGL((k_bc_characteristic_q<Q, double>), dim3((np + 255) / 256), dim3(256), 0,
                          c->stream, q, (const double *)c->qref, b, c->cfg.n[0], c->cfg.n[1], c->N,
                          c->cfg.gamma);
+    } else if (r.first == 200) {
+      const BoxRow &b = r.second;
+      const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
+      hipLaunchKernelGGL((k_bc_characteristic_q<Q, Q>), dim3((np + 255) / 256), dim3(256), 0,
+                         c->stream, q, (const Q *)q, b, c->cfg.n[0], c->cfg.n[1], c->N, c->cfg.gamma);
     } else if (r.first == 401) {
       hipLaunchKernelGGL((k_bc_sponge_q<Q>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream, q,
                          (const double *)c->qref, (const double *)c->shape, c->N);
@@ -701,7 +706,7 @@ int copy_rhs_out(cruna_hip_ctx *c, double *rhs) {
 template <typename Q>
 int bc_qs(cruna_hip_ctx *c, Q *qs) {
   for (auto &r : c->bc) {
-    if (r.first == 201) {
+    if (r.first == 201 || r.first == 200) {
       const BoxRow &b = r.second;
       const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
       hipLaunchKernelGGL((k_bc_characteristic_qs<Q>), dim3((np + 255) / 256), dim3(256), 0,
@@ -912,7 +917,7 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   if (cfg->nbc < 0 || cfg->nbc > CRUNA_HIP_MAX_BC) return fail(CRUNA_HIP_EINVAL, "bad nbc");
   for (int r = 0; r < cfg->nbc; ++r) {
     const int t = (int)lround(cfg->bc[r][0]);
-    if (t != 0 && t != 100 && t != 201 && t != 204 && t != 400 && t != 401)
+    if (t != 0 && t != 100 && t != 200 && t != 201 && t != 204 && t != 400 && t != 401)
       return fail(CRUNA_HIP_EINVAL,
                   "boundary type " + std::to_string(t) + " not implemented on the device");
   }
@@ -935,13 +940,32 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
     }
     if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) continue;
     b.i0 = lo[0]; b.i1 = hi[0]; b.j0 = lo[1]; b.j1 = hi[1]; b.k0 = lo[2]; b.k1 = hi[2];
+    b.aim_shift = 0;
+    if (t == 200) {
+      // characteristic_zero_grad_init (nonreflecting.f90:665-728): the aim is
+      // the inner neighbour, one point against each nonzero normal component
+      // (its sign decides); the box must be one point thick along them so
+      // that no aim point is itself updated
+      const double kn = std::sqrt(b.k[0] * b.k[0] + b.k[1] * b.k[1] + b.k[2] * b.k[2]);
+      if (!(kn > 0)) return fail(CRUNA_HIP_EINVAL, "type 200 row needs a nonzero normal");
+      long shift[3];
+      for (int d = 0; d < 3; ++d) {
+        const double kd = b.k[d] / kn;
+        shift[d] = kd > 0 ? -1 : (kd < 0 ? 1 : 0);
+        if (shift[d] != 0 && lo[d] != hi[d])
+          return fail(CRUNA_HIP_EINVAL, "type 200 row: the box must be one point thick along its normal");
+        if (lo[d] + shift[d] < 0 || hi[d] + shift[d] > cfg->n[d] - 1)
+          return fail(CRUNA_HIP_EINVAL, "type 200 row: inner neighbour outside the grid");
+      }
+      b.aim_shift = shift[0] + (long)cfg->n[0] * (shift[1] + (long)cfg->n[1] * shift[2]);
+    }
     if (t == 100) {  // the variable whose RHS is zero (1-based, in column 8)
       const int v = (int)lround(cfg->bc[r][7]);
       if (v < 1 || v > 5) return fail(CRUNA_HIP_EINVAL, "wall row (100): column 8 must name a variable 1..5");
       b.k[0] = v - 1;
     }
     c->bc.push_back({t, b});
-    if (t == 201 || t == 401) c->has_bcq = true;
+    if (t == 200 || t == 201 || t == 401) c->has_bcq = true;
     if (t == 400) c->has_bcrhs = true;
     if (t == 100 || t == 204) c->has_bclate = true;
     if (t == 100) c->has_wall = true;

@@ -1019,3 +1019,48 @@ def test_adjoint_with_rhs_rows():
         g.set_sponge_shape(shape_fn)
         with pytest.raises(CrunaError, match="wall rows"):
             g.right_hand_side_adjoint_3d(qs, q)
+
+
+def test_zero_gradient_rows_direct_and_adjoint_match_oracle():
+    """Type 200 on both y faces (characteristic toward the inner neighbour) in
+    a 4-step direct run and a 3-step adjoint run over the recorded
+    trajectory, against the oracle; a box thicker than one point along its
+    normal is refused."""
+    from public_amd.cruna import CrunaError
+    Cruna = _gpu()
+    shape = (32, 48, 16)
+    q, dx, (x, y, z), (Lx, Ly, Lz) = _shear_case(shape, seed=10)
+    n1, n2, n3 = shape
+    bc = [(200, 1, n1, 1, 1, 1, n3, 0, -1, 0), (200, 1, n1, n2, n2, 1, n3, 0, 1, 0)]
+    scheme = (O.PERIODIC, O.OPEN, O.PERIODIC)
+    kw = dict(mu0=2e-3, T0=1.0 / (1.4 * 0.25), dt=2e-3, scheme=scheme, bc=bc)
+    cfg = O.Cfg(shape, dx, filter_freq=(1, 1, 1), afilter_freq=(1, 1, 1), **kw)
+    nsteps = 3
+    Q = np.zeros(shape + (5, nsteps), order="F")
+    s = q
+    for nt in range(1, nsteps + 1):
+        s = O.run_direct(cfg, s, 1, nt0=nt - 1)
+        Q[..., nt - 1] = s
+    ref4 = O.run_direct(cfg, q, 4)
+    qs_T = np.asfortranarray(np.random.default_rng(4).standard_normal(shape + (5,)))
+    ref_a = O.run_adjoint(cfg, qs_T, Q)
+    with Cruna(shape, dx, filter_freq=(1, 1, 1), afilter_freq=(1, 1, 1), **kw) as g:
+        g.set_qref(q)
+        g.set_state(q)
+        g.run_direct(4)
+        g.synchronize()
+        got4 = g.get_state()
+        g.trajectory_alloc(nsteps)
+        g.trajectory_record(True)
+        g.set_state(q)
+        g.run_direct(nsteps)
+        g.set_adjoint_state(qs_T)
+        g.run_adjoint(nsteps)
+        g.synchronize()
+        got_a = g.get_adjoint_state()
+    for v in range(5):
+        assert rel(got4[..., v], ref4[..., v]) < 1e-12, (v, rel(got4[..., v], ref4[..., v]))
+        assert rel(got_a[..., v], ref_a[..., v]) < 1e-12, (v, rel(got_a[..., v], ref_a[..., v]))
+    thick = [(200, 1, n1, 1, 2, 1, n3, 0, -1, 0)]
+    with pytest.raises(CrunaError, match="one point thick"):
+        Cruna(shape, dx, **dict(kw, bc=thick))

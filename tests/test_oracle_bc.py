@@ -114,3 +114,29 @@ def test_wall_and_characteristic_rows_in_the_rhs():
     face = O.characteristic_rhs(rhs[:, 15].reshape(-1, 5, order="F"), q[:, 15].reshape(-1, 5, order="F"),
                                 q[:, 15].reshape(-1, 5, order="F"), (0, 1, 0), 1.4, DT)
     np.testing.assert_allclose(got[:, 15].reshape(-1, 5, order="F"), face, rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("face", ["lo", "hi"])
+def test_zero_gradient_row_is_the_characteristic_row_toward_the_inner_neighbour(face):
+    """Type 200 (characteristic_zero_grad_init + characteristic_q/_qs) equals
+    type 201 with the target (q) or the linearisation state (qs) taken one
+    point inside, against the normal: the index shift of the zero-gradient
+    aim, pinned on both y faces."""
+    n = (16, 16, 16)
+    rng = np.random.default_rng(7)
+    q = np.zeros(n + (5,), order="F")
+    q[..., 0] = 1 + 0.05 * rng.random(n)
+    q[..., 1:4] = 0.2 * rng.standard_normal(n + (3,))
+    q[..., 4] = 1 / 1.4 * (1 + 0.05 * rng.random(n))
+    j, k2, jn = (1, -1.0, 2) if face == "lo" else (16, 1.0, 15)
+    row = (1, 16, j, j, 1, 16, 0, k2, 0)
+    shifted = q.copy(order="F")
+    shifted[:, j - 1] = q[:, jn - 1]
+    got = O.bc_q(O.Cfg(n, (0.1,) * 3, bc=[(200,) + row]), q)
+    ref = O.bc_q(O.Cfg(n, (0.1,) * 3, bc=[(201,) + row]), q, qref=shifted)
+    assert np.array_equal(got, ref)
+    assert np.abs(got - q).max() > 1e-3
+    qs = np.asfortranarray(rng.standard_normal(n + (5,)))
+    got_s = O.bc_qs(O.Cfg(n, (0.1,) * 3, bc=[(200,) + row]), qs, q)
+    ref_s = O.bc_qs(O.Cfg(n, (0.1,) * 3, bc=[(201,) + row]), qs, shifted)
+    assert np.array_equal(got_s, ref_s)
