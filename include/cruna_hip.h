@@ -41,6 +41,7 @@ extern "C" {
 #define CRUNA_HIP_DIFF_O6_COMPACT 0  /* src/diff/diff_pen_core_o6_i_c_{p,o}.f90 */
 #define CRUNA_HIP_DIFF_O4_EXPLICIT 1 /* src/diff/diff_pen_core_o4_e_c_{p,o}.f90 */
 #define CRUNA_HIP_DIFF_O4_PADE 2     /* src/diff/diff_pen_core_o4_i_c_{p,o}_pade.f90 */
+#define CRUNA_HIP_DIFF_O2_EXPLICIT 3 /* src/diff/diff_pen_core_o2_e_c_o.f90 (open lines only) */
 
 typedef struct cruna_hip_cfg {
   int n[3];              /* params%geom%n1..n3 (block = whole grid on one GPU) */
@@ -73,7 +74,8 @@ typedef struct cruna_hip_cfg {
   int diff_kind[3];      /* derivative core per direction (discretisation_x_core):
                             CRUNA_HIP_DIFF_O6_COMPACT (diff_pen_core_o6_i_c_*, default),
                             CRUNA_HIP_DIFF_O4_EXPLICIT (diff_pen_core_o4_e_c_*) or
-                            CRUNA_HIP_DIFF_O4_PADE (diff_pen_core_o4_i_c_*_pade); the
+                            CRUNA_HIP_DIFF_O4_PADE (diff_pen_core_o4_i_c_*_pade) or
+                            CRUNA_HIP_DIFF_O2_EXPLICIT (diff_pen_core_o2_e_c_o, open only); the
                             filter is the o6 compact one in every case */
   int wfilter_freq[3];   /* filter.dwi1freq..dwi3freq of the cruna_hpc loop (0 = never):
                             the weighted filter (filter_weighted_iN with the

@@ -310,6 +310,22 @@ void orc_diff_core_o4(int scheme, int n, int nlines, double *q, double dx) {
   free(t);
 }
 
+/* 2nd-order explicit open core, src/diff/diff_pen_core_o2_e_c_o.f90 diff_main:
+ * (f+1 - f-1)/2 inside, one-sided rows 1, 2, n-1, n; then / dx. */
+void orc_diff_core_o2(int n, int nlines, double *q, double dx) {
+  double *t = xalloc(n);
+  for (int l = 0; l < nlines; ++l) {
+    double *x = q + (size_t)l * n;
+    t[0] = -1.5 * x[0] + 2.0 * x[1] - 0.5 * x[2];
+    t[1] = -0.5 * x[0] + 0.5 * x[2];
+    for (int i = 2; i < n - 2; ++i) t[i] = 0.5 * (x[i + 1] - x[i - 1]);
+    t[n - 2] = -0.5 * x[n - 3] + 0.5 * x[n - 1];
+    t[n - 1] = +0.5 * x[n - 3] - 2.0 * x[n - 2] + 1.5 * x[n - 1];
+    for (int i = 0; i < n; ++i) x[i] = t[i] / dx;
+  }
+  free(t);
+}
+
 /* 4th-order Pade cores: src/diff/diff_pen_core_o4_i_c_p_pade.f90:96-140 and
  * diff_pen_core_o4_i_c_o_pade.f90:81-124 (a (f_{i+1} - f_{i-1}), a = 1/2, with
  * the one-sided rows -a f1 + a f2 and a fn - a f(n-1) on open lines; / dx;
@@ -485,6 +501,8 @@ void orc_diff3d(const orc_cfg *c, int dir, double *out, const double *in, double
     orc_diff_core_o4(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, h);
   else if (c->diff_kind[dir] == ORC_DIFF_O4_PADE)
     orc_diff_core_pade(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, h);
+  else if (c->diff_kind[dir] == ORC_DIFF_O2_EXPLICIT)
+    orc_diff_core_o2(c->n[dir], (int)(N / c->n[dir]), pen, h);
   else
     orc_diff_core(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, h);
   scatter_pencils(c, dir, pen, out);

@@ -63,6 +63,7 @@ typedef struct orc_cfg {
 #define ORC_DIFF_O6_COMPACT 0
 #define ORC_DIFF_O4_EXPLICIT 1
 #define ORC_DIFF_O4_PADE 2
+#define ORC_DIFF_O2_EXPLICIT 3 /* diff_pen_core_o2_e_c_o (open lines only) */
 
 /* LAPACK dgttrf/dgttrs restatement (no transpose), 1 RHS at a time. */
 int orc_dgttrf(int n, double *dl, double *d, double *du, double *du2, int *ipiv);
@@ -72,6 +73,7 @@ void orc_dgttrs(int n, int nrhs, const double *dl, const double *d, const double
 /* Pencil cores: q(n, nlines) in place, as diff_iN / filter_iN cores. */
 void orc_diff_core(int scheme, int n, int nlines, double *q, double dx);
 void orc_diff_core_o4(int scheme, int n, int nlines, double *q, double dx);
+void orc_diff_core_o2(int n, int nlines, double *q, double dx);
 void orc_diff_core_pade(int scheme, int n, int nlines, double *q, double dx);
 void orc_filter_core(int scheme, int n, int nlines, double *q, double alpha);
 

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 MAX_BC = 32
 
 PERIODIC, OPEN = 0, 1
-DIFF_O6_COMPACT, DIFF_O4_EXPLICIT, DIFF_O4_PADE = 0, 1, 2
+DIFF_O6_COMPACT, DIFF_O4_EXPLICIT, DIFF_O4_PADE, DIFF_O2_EXPLICIT = 0, 1, 2, 3
 SUTHERLAND, POWER = 0, 1
 
 
@@ -69,6 +69,7 @@ def lib():
         L.orc_diff_core.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double]
         L.orc_diff_core_o4.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double]
         L.orc_diff_core_pade.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double]
+        L.orc_diff_core_o2.argtypes = [C.c_int, C.c_int, _dp, C.c_double]
         L.orc_filter_core.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_double]
         L.orc_diff3d.argtypes = [P, C.c_int, _dp, _dp, C.c_double]
         L.orc_filter4d.argtypes = [P, C.c_int, _dp, C.c_int]
@@ -113,6 +114,7 @@ def ref_lib():
         L.ref_diff.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double]
         L.ref_diff_o4.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double]
         L.ref_diff_pade.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double]
+        L.ref_diff_o2.argtypes = [_dp, C.c_int, C.c_int, C.c_double]
         L.ref_filter.argtypes = [C.c_int, _dp, C.c_int, C.c_int]
         L.ref_trafos.argtypes = [_dp, C.c_int, _dp, _dp, _dp]
         L.ref_characteristic_q.argtypes = [_dp, _dp, C.c_int, _dp]
@@ -184,6 +186,9 @@ def _f(a):
 
 def diff_core(scheme, q, dx, kind=0):
     q = _f(q).copy(order="F")
+    if kind == DIFF_O2_EXPLICIT:  # open lines only
+        lib().orc_diff_core_o2(q.shape[0], q.size // q.shape[0], q, dx)
+        return q
     f = {DIFF_O4_EXPLICIT: lib().orc_diff_core_o4, DIFF_O4_PADE: lib().orc_diff_core_pade}.get(
         kind, lib().orc_diff_core)
     f(scheme, q.shape[0], q.size // q.shape[0], q, dx)

@@ -13,6 +13,7 @@ module ref_harness
   use diff_pen_core_o6_i_c_o, only: do_i1 => diff_i1
   use diff_pen_core_o4_e_c_p, only: d4p_i1 => diff_i1
   use diff_pen_core_o4_e_c_o, only: d4o_i1 => diff_i1
+  use diff_pen_core_o2_e_c_o, only: d2o_i1 => diff_i1
   use diff_pen_core_o4_i_c_p_pade, only: dpp_i1 => diff_i1
   use diff_pen_core_o4_i_c_o_pade, only: dpo_i1 => diff_i1
   use filter_pen_core_o6_i_c_p, only: fp_i1 => filter_i1
@@ -62,6 +63,14 @@ contains
       call d4o_i1(q, dx)
     end if
   end subroutine ref_diff_o4
+
+  ! the 2nd-order explicit open core
+  subroutine ref_diff_o2(q, n, nlines, dx) bind(C, name="ref_diff_o2")
+    integer(c_int), value :: n, nlines
+    real(c_double), intent(inout) :: q(n, nlines)
+    real(c_double), value :: dx
+    call d2o_i1(q, dx)
+  end subroutine ref_diff_o2
 
   ! the 4th-order Pade cores (factorisation saved on first use, like o6)
   subroutine ref_diff_pade(scheme, q, n, nlines, dx) bind(C, name="ref_diff_pade")

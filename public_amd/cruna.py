@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "libcruna_hip.so")
 PERIODIC, OPEN = 0, 1
 SUTHERLAND, POWER = 0, 1
 FP64, FP32_MIXED = 0, 1
-DIFF_O6_COMPACT, DIFF_O4_EXPLICIT, DIFF_O4_PADE = 0, 1, 2
+DIFF_O6_COMPACT, DIFF_O4_EXPLICIT, DIFF_O4_PADE, DIFF_O2_EXPLICIT = 0, 1, 2, 3
 MAX_BC = 32
 
 

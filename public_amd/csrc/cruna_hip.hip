@@ -905,8 +905,10 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
     if (cfg->scheme[d] != CRUNA_HIP_PERIODIC && cfg->scheme[d] != CRUNA_HIP_OPEN)
       return fail(CRUNA_HIP_EINVAL, "unknown scheme");
     if (cfg->diff_kind[d] != CRUNA_HIP_DIFF_O6_COMPACT && cfg->diff_kind[d] != CRUNA_HIP_DIFF_O4_EXPLICIT &&
-        cfg->diff_kind[d] != CRUNA_HIP_DIFF_O4_PADE)
+        cfg->diff_kind[d] != CRUNA_HIP_DIFF_O4_PADE && cfg->diff_kind[d] != CRUNA_HIP_DIFF_O2_EXPLICIT)
       return fail(CRUNA_HIP_EINVAL, "unknown derivative core");
+    if (cfg->diff_kind[d] == CRUNA_HIP_DIFF_O2_EXPLICIT && cfg->scheme[d] != CRUNA_HIP_OPEN)
+      return fail(CRUNA_HIP_EINVAL, "the 2nd-order explicit core (diff_pen_core_o2_e_c_o) is for open lines only");
     if (!(cfg->dx[d] > 0)) return fail(CRUNA_HIP_EINVAL, "dx must be positive");
     if (cfg->wfilter_freq[d] < 0) return fail(CRUNA_HIP_EINVAL, "wfilter_freq must be >= 0");
   }
@@ -975,6 +977,7 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
     try {
       const OpKind dk = cfg->diff_kind[d] == CRUNA_HIP_DIFF_O4_EXPLICIT ? OP_DIFF_O4E
                         : cfg->diff_kind[d] == CRUNA_HIP_DIFF_O4_PADE   ? OP_DIFF_O4P
+                        : cfg->diff_kind[d] == CRUNA_HIP_DIFF_O2_EXPLICIT ? OP_DIFF_O2E
                                                                         : OP_DIFF;
       HostLineOp dd = build_line_op(dk, cfg->scheme[d] == CRUNA_HIP_PERIODIC, cfg->n[d], 0.0);
       HostLineOp ff =

@@ -39,7 +39,7 @@ constexpr int HALO = 3;          // explicit-stencil half width carried in M
 constexpr int FE = PPT + 2 * HALO;  // 14 = extended block
 constexpr int MAXK = 12;         // max reduced-system stages
 
-enum OpKind { OP_DIFF = 0, OP_FILTER = 1, OP_DIFF_O4E = 2, OP_DIFF_O4P = 3 };
+enum OpKind { OP_DIFF = 0, OP_FILTER = 1, OP_DIFF_O4E = 2, OP_DIFF_O4P = 3, OP_DIFF_O2E = 4 };
 
 // M, L, R are stored for three block types: 0 = first block, 1 = interior
 // blocks, 2 = last block (all equal on a periodic line).  W is stored per
@@ -156,6 +156,17 @@ inline void explicit_rows(OpKind kind, bool per, int n, double alpha,
         add(n - 1, n - 1 - k, -r0[k] * s);
       }
     }
+  } else if (kind == OP_DIFF_O2E) {
+    // diff_pen_core_o2_e_c_o.f90 diff_main (open lines only): (f+1 - f-1) / 2,
+    // one-sided rows 1, 2, n-1, n
+    for (int i = 2; i <= n - 3; ++i) {
+      add(i, i + 1, 0.5);
+      add(i, i - 1, -0.5);
+    }
+    add(0, 0, -1.5); add(0, 1, 2.0); add(0, 2, -0.5);
+    add(1, 0, -0.5); add(1, 2, 0.5);
+    add(n - 2, n - 3, -0.5); add(n - 2, n - 1, 0.5);
+    add(n - 1, n - 3, 0.5); add(n - 1, n - 2, -2.0); add(n - 1, n - 1, 1.5);
   } else if (kind == OP_DIFF_O4P) {
     // diff_pen_core_o4_i_c_{p,o}_pade.f90 diff_main: (f+1 - f-1) / 2, with the
     // one-sided rows (f2 - f1) / 2 and (fn - f(n-1)) / 2 on open lines
@@ -238,11 +249,12 @@ inline void implicit_rows(OpKind kind, bool per, int n, double alpha, std::vecto
     }
     return;
   }
-  const double al = kind == OP_DIFF ? 1.0 / 3.0 : (kind == OP_DIFF_O4E ? 0.0 : alpha);
+  const bool expl = kind == OP_DIFF_O4E || kind == OP_DIFF_O2E;
+  const double al = kind == OP_DIFF ? 1.0 / 3.0 : (expl ? 0.0 : alpha);
   lo.assign(n, al);
   di.assign(n, 1.0);
   up.assign(n, al);
-  if (kind == OP_DIFF_O4E) return;  // explicit: B = I
+  if (expl) return;  // explicit: B = I
   if (!per) {
     lo[0] = 0.0;
     up[n - 1] = 0.0;

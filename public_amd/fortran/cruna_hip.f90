@@ -15,7 +15,7 @@ module cruna_hip
   integer, parameter, public :: CRUNA_HIP_FP64 = 0, CRUNA_HIP_FP32_MIXED = 1
   integer, parameter, public :: CRUNA_HIP_MAX_BC = 32
   integer, parameter, public :: CRUNA_HIP_DIFF_O6_COMPACT = 0, CRUNA_HIP_DIFF_O4_EXPLICIT = 1, &
-                                CRUNA_HIP_DIFF_O4_PADE = 2
+                                CRUNA_HIP_DIFF_O4_PADE = 2, CRUNA_HIP_DIFF_O2_EXPLICIT = 3
 
   ! cruna_hip_cfg (C layout; bc(1:10, r) is row r of boundary_conditions.dat)
   type, bind(C), public :: cruna_hip_cfg

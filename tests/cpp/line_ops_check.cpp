@@ -78,11 +78,12 @@ int main() {
   std::mt19937_64 rng(7);
   std::normal_distribution<double> nd;
   const int ns[] = {16, 24, 48, 64, 96, 128, 256, 512};
-  const OpKind kinds[] = {OP_DIFF, OP_FILTER, OP_DIFF_O4E, OP_DIFF_O4P};
-  const char *names[] = {"diff", "filter", "diffo4", "diffpade"};
-  for (int kind = 0; kind < 4; ++kind)
+  const OpKind kinds[] = {OP_DIFF, OP_FILTER, OP_DIFF_O4E, OP_DIFF_O4P, OP_DIFF_O2E};
+  const char *names[] = {"diff", "filter", "diffo4", "diffpade", "diffo2"};
+  for (int kind = 0; kind < 5; ++kind)
     for (int per = 1; per >= 0; --per)
       for (int n : ns) {
+        if (kinds[kind] == OP_DIFF_O2E && per) continue;  // open lines only
         const double alpha = 0.35;
         HostLineOp op = build_line_op(kinds[kind], per, n, alpha);
         double worst = 0.0;
@@ -97,8 +98,10 @@ int main() {
             orc_filter_core(sch, n, 1, ref.data(), alpha);
           else if (kind == 2)
             orc_diff_core_o4(sch, n, 1, ref.data(), 1.0);
-          else
+          else if (kind == 3)
             orc_diff_core_pade(sch, n, 1, ref.data(), 1.0);
+          else
+            orc_diff_core_o2(n, 1, ref.data(), 1.0);
           std::vector<double> x = emulate(op, f);
           double num = 0, den = 0;
           for (int i = 0; i < n; ++i) {

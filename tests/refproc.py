@@ -56,6 +56,9 @@ def core(kind, scheme, q, arg):
     elif kind == "diff_pade":
         L = _lib()
         L.ref_diff_pade(scheme, q, q.shape[0], q.shape[1], arg)
+    elif kind == "diff_o2":
+        L = _lib()
+        L.ref_diff_o2(q, q.shape[0], q.shape[1], arg)
     else:
         L = _lib(alpha=arg)
         L.ref_filter(scheme, q, q.shape[0], q.shape[1])

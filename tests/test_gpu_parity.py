@@ -61,7 +61,7 @@ def test_filter_matches_oracle(shape, scheme):
 
 def test_dx_golden_vectors():
     """Lines of the reference's own Fortran derivative cores (o6 compact, o4
-    explicit, o4 Pade; tests/golden/cores.npz) placed along x."""
+    explicit, o4 Pade, o2 explicit; tests/golden/cores.npz) placed along x."""
     Cruna = _gpu()
     g = np.load(GOLDEN)
     for key in g.files:
@@ -71,7 +71,8 @@ def test_dx_golden_vectors():
         kind, scheme, n = base.split("_")[0], int(base.split("_")[1]), int(base.split("_")[2])
         if n < 16:
             continue
-        dk = {"diff": O.DIFF_O6_COMPACT, "diffo4": O.DIFF_O4_EXPLICIT, "diffpade": O.DIFF_O4_PADE}[kind]
+        dk = {"diff": O.DIFF_O6_COMPACT, "diffo4": O.DIFF_O4_EXPLICIT, "diffpade": O.DIFF_O4_PADE,
+              "diffo2": O.DIFF_O2_EXPLICIT}[kind]
         lines = g[key]
         h = float(g[base + "_arg"])
         f = _rand((n, 16, 16), 3)
@@ -1064,3 +1065,30 @@ def test_zero_gradient_rows_direct_and_adjoint_match_oracle():
     thick = [(200, 1, n1, 1, 2, 1, n3, 0, -1, 0)]
     with pytest.raises(CrunaError, match="one point thick"):
         Cruna(shape, dx, **dict(kw, bc=thick))
+
+
+def test_o2_explicit_open_direction_run_matches_oracle():
+    """diff_pen_core_o2_e_c_o along the open y direction (o6 compact in the
+    periodic x and z): 3 RK4 steps with filters and characteristic rows, and
+    the periodic use of the open-only core refused."""
+    from public_amd.cruna import CrunaError
+    Cruna = _gpu()
+    shape = (32, 48, 16)
+    qs, dxs, _, _ = _shear_case(shape, seed=3)
+    n1, n2, n3 = shape
+    scheme = (O.PERIODIC, O.OPEN, O.PERIODIC)
+    kinds = (O.DIFF_O6_COMPACT, O.DIFF_O2_EXPLICIT, O.DIFF_O6_COMPACT)
+    bc = [(201, 1, n1, 1, 1, 1, n3, 0, -1, 0), (201, 1, n1, n2, n2, 1, n3, 0, 1, 0)]
+    kw = dict(mu0=2e-3, T0=1.0 / (1.4 * 0.25), dt=2e-3, scheme=scheme, bc=bc)
+    cfg = O.Cfg(shape, dxs, filter_freq=(1, 1, 1), diff_kind=kinds, **kw)
+    ref = O.run_direct(cfg, qs, 3, qref=qs)
+    with Cruna(shape, dxs, filter_freq=(1, 1, 1), diff_kind=kinds, **kw) as g:
+        g.set_qref(qs)
+        g.set_state(qs)
+        g.run_direct(3)
+        g.synchronize()
+        got = g.get_state()
+    for v in range(5):
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+    with pytest.raises(CrunaError, match="open lines only"):
+        Cruna(shape, dxs, scheme=scheme, diff_kind=(O.DIFF_O2_EXPLICIT, 0, 0))

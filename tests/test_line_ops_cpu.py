@@ -1,6 +1,6 @@
 """CPU check of the device line-solve algorithm: the host tables of
 public_amd/csrc/line_ops.hpp, applied by a CPU emulation of the block engine,
-reproduce the oracle's LU-based derivative (o6 compact, o4 explicit, o4 Pade)
+reproduce the oracle's LU-based derivative (o6 compact, o4 explicit, o4 Pade, o2 explicit)
 and filter cores to rounding."""
 import os
 import subprocess
@@ -19,6 +19,6 @@ def test_block_engine_matches_lu_cores(tmp_path):
     subprocess.run(["g++", "-O2", "-std=c++17", src, str(obj), "-o", str(exe), "-lm"], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
     rows = [line.split() for line in out.strip().splitlines()]
-    assert len(rows) == 64
+    assert len(rows) == 72
     for kind, scheme, n, K, err in rows:
         assert float(err) < 1e-14, (kind, scheme, n, K, err)

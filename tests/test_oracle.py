@@ -78,7 +78,8 @@ def test_cores_vs_golden():
         if kind == "filter":
             mine = O.filter_core(scheme, q, arg)
         else:
-            dk = {"diffo4": O.DIFF_O4_EXPLICIT, "diffpade": O.DIFF_O4_PADE}.get(kind, O.DIFF_O6_COMPACT)
+            dk = {"diffo4": O.DIFF_O4_EXPLICIT, "diffpade": O.DIFF_O4_PADE,
+                  "diffo2": O.DIFF_O2_EXPLICIT}.get(kind, O.DIFF_O6_COMPACT)
             mine = O.diff_core(scheme, q, arg, dk)
         ref = g[base + "_out"]
         np.testing.assert_allclose(mine, ref, rtol=0, atol=1e-13 * np.abs(ref).max(), err_msg=base)
@@ -130,6 +131,26 @@ def test_pade_cores_vs_reference_fortran(scheme, n):
     ref = refproc.run("core", "diff_pade", scheme, q, 0.17)
     mine = O.diff_core(scheme, q, 0.17, O.DIFF_O4_PADE)
     np.testing.assert_allclose(mine, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+
+
+@need_ref
+@pytest.mark.parametrize("n", [8, 24, 64])
+def test_o2_core_vs_reference_fortran(n):
+    """diff_pen_core_o2_e_c_o restated in the oracle against the reference's own."""
+    q = _rand_lines(n, 5, 7 * n)
+    ref = refproc.run("core", "diff_o2", 1, q, 0.21)
+    mine = O.diff_core(O.OPEN, q, 0.21, O.DIFF_O2_EXPLICIT)
+    np.testing.assert_allclose(mine, ref, rtol=0, atol=1e-14 * np.abs(ref).max())
+
+
+def test_o2_core_closed_form():
+    """Exact on quadratics in every row (central and one-sided), not on cubics."""
+    n, h = 20, 0.3
+    x = np.arange(n) * h
+    d = O.diff_core(O.OPEN, (1.5 - 0.7 * x + 0.4 * x ** 2)[:, None], h, O.DIFF_O2_EXPLICIT)[:, 0]
+    np.testing.assert_allclose(d, -0.7 + 0.8 * x, atol=1e-12)
+    d3 = O.diff_core(O.OPEN, (x ** 3)[:, None], h, O.DIFF_O2_EXPLICIT)[:, 0]
+    assert np.abs(d3 - 3 * x ** 2).max() > 1e-3
 
 
 @pytest.mark.parametrize("n", [16, 48])
