@@ -120,12 +120,14 @@ int cruna_hip_set_metric(cruna_hip_ctx *ctx, const double *X_metric, const doubl
  * volume_penalization penalization(:,:,:,1) and (:,:,:,2) (porosity and the
  * stored 1/phi, src/immersed/volume_penalization.f90 and the case's
  * init_volume_penalization); from then on the direct RHS is that of
- * ns_rho_rhou_p_ev_phi.f90:33-291, as a case selects it through
- * governing_equations, with the reference's no-op apply_volume_penalization.
- * Both null: back to ns_rho_rhou_p.  EINVAL: non-finite values, a deformed
- * grid (ns_rho_rhou_p_ev_phi_grid_transform is not built), and afterwards
- * every adjoint entry point (the reference's ev_phi adjoint stops,
- * ns_rho_rhou_p_ev_phi.f90:295-309). */
+ * ns_rho_rhou_p_ev_phi.f90:33-291 (Cartesian) or of
+ * ns_rho_rhou_p_ev_phi_grid_transform.f90 (with a metric), as a case selects
+ * it through governing_equations, with the reference's no-op
+ * apply_volume_penalization.  Both null: back to the unpenalised RHS.
+ * EINVAL: non-finite values; a tensor-product metric set before (set the
+ * penalisation first: the penalised deformed grid keeps the full metric per
+ * point); and afterwards every adjoint entry point (the reference's ev_phi
+ * adjoint stops, ns_rho_rhou_p_ev_phi.f90:295-309). */
 int cruna_hip_set_penalization(cruna_hip_ctx *ctx, const double *phi, const double *inv_phi);
 
 /* filter_weighted_iN_mpi_cartesian::filter_4d on nv fields q(n1,n2,n3,nv) in

@@ -830,6 +830,11 @@ void orc_sponge_shape(const orc_cfg *c, const double x0[3], const double x1[3],
 /* Direct NS right-hand side, src/equations/ns_rho_rhou_p.f90:31-253.       */
 void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const double *qref,
                     const double *shape, int heat_unit_spacing) {
+  if (c->X_metric && c->penal_phi) { /* => ns_rho_rhou_p_ev_phi_grid_transform */
+    orc_rhs_direct_ev_gt(c, rhs, q_in, c->X_metric, c->X_jacobian, c->penal_phi, c->penal_inv_phi,
+                         qref, shape);
+    return;
+  }
   if (c->X_metric) { /* governing_equations => ns_rho_rhou_p_grid_transform */
     orc_rhs_direct_gt(c, rhs, q_in, c->X_metric, c->X_jacobian, qref, shape);
     return;
@@ -1047,8 +1052,30 @@ void orc_calc_metric(const orc_cfg *c, const double *X, const double *X_calc, do
  * fluxes with the contravariant velocity u_hat_k = sum_j M(k,j) u_j, and
  * every equation is multiplied by X_jacobian at the end; the heat flux's
  * outer derivatives take no h (:252-268). */
+static void rhs_direct_gt_impl(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
+                               const double *Xj, const double *phi, const double *inv_phi,
+                               const double *qref, const double *shape);
+
 void orc_rhs_direct_gt(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
                        const double *Xj, const double *qref, const double *shape) {
+  rhs_direct_gt_impl(c, rhs, q_in, Xm, Xj, NULL, NULL, qref, shape);
+}
+
+/* Volume-penalised deformed-grid RHS, src/equations/ns_rho_rhou_p_ev_phi_grid_transform.f90:
+ * the grid-transform RHS with phi inside every flux derivative (phi rho u_hat,
+ * phi rho u_i u_hat, phi tau_proj, phi p u_hat, phi lambda J T_j :126-202) and
+ * div_1_phi on the divergences (:210, :227, :292), not on the pressure
+ * gradient, u.grad(p) or J tau:grad(u). */
+void orc_rhs_direct_ev_gt(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
+                          const double *Xj, const double *phi, const double *inv_phi,
+                          const double *qref, const double *shape) {
+  rhs_direct_gt_impl(c, rhs, q_in, Xm, Xj, phi, inv_phi, qref, shape);
+}
+
+/* phi == NULL: ns_rho_rhou_p_grid_transform; else its penalised form. */
+static void rhs_direct_gt_impl(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
+                               const double *Xj, const double *phi, const double *inv_phi,
+                               const double *qref, const double *shape) {
   const size_t N = npts(c);
   const double dxs[3] = {c->dx[0], c->dx[1], c->dx[2]};
   double *q = xalloc(5 * N), *u = xalloc(3 * N), *uh = xalloc(3 * N);
@@ -1106,7 +1133,7 @@ void orc_rhs_direct_gt(const orc_cfg *c, double *rhs, const double *q_in, const 
   for (size_t i = 0; i < N; ++i) lam[i] = orc_lambda(c, mu[i]);
   PHYS_GRAD(temp, tg);
   for (size_t i = 0; i < N; ++i) {
-    lam[i] = lam[i] * Xj[i];
+    lam[i] = phi ? phi[i] * lam[i] * Xj[i] : lam[i] * Xj[i]; /* ev: phi lambda J (:199) */
     tg[0][i] = lam[i] * tg[0][i];
     tg[1][i] = lam[i] * tg[1][i];
     tg[2][i] = lam[i] * tg[2][i];
@@ -1114,40 +1141,45 @@ void orc_rhs_direct_gt(const orc_cfg *c, double *rhs, const double *q_in, const 
 #undef PHYS_GRAD
   /* mass :184-190 */
   for (int k = 0; k < 3; ++k) {
-    for (size_t i = 0; i < N; ++i) tmp[i] = q[i] * uh[k * N + i];
+    for (size_t i = 0; i < N; ++i) tmp[i] = (phi ? phi[i] * q[i] : q[i]) * uh[k * N + i];
     orc_diff3d(c, k, a[k], tmp, dxs[k]);
   }
   for (size_t i = 0; i < N; ++i) {
     r[0][i] = -a[0][i] - a[1][i] - a[2][i];
-    r[0][i] = Xj[i] * r[0][i];
+    r[0][i] = phi ? Xj[i] * inv_phi[i] * r[0][i] : Xj[i] * r[0][i];
   }
   /* momentum :192-241 */
   for (int m = 0; m < 3; ++m) {
     for (int k = 0; k < 3; ++k) {
-      for (size_t i = 0; i < N; ++i) tmp[i] = q[(m + 1) * N + i] * uh[k * N + i];
+      for (size_t i = 0; i < N; ++i)
+        tmp[i] = (phi ? phi[i] * q[(m + 1) * N + i] : q[(m + 1) * N + i]) * uh[k * N + i];
       orc_diff3d(c, k, a[k], tmp, dxs[k]);
     }
-    for (size_t i = 0; i < N; ++i) r[m + 1][i] = -a[0][i] - a[1][i] - a[2][i] - pg[m][i];
+    for (size_t i = 0; i < N; ++i)
+      r[m + 1][i] = phi ? -a[0][i] - a[1][i] - a[2][i] : -a[0][i] - a[1][i] - a[2][i] - pg[m][i];
     for (int k = 0; k < 3; ++k) { /* tau_proj_mk = sum_j M(k,j) tau_mj */
       const double *m0 = MET(Xm, k, 0, N), *m1 = MET(Xm, k, 1, N), *m2 = MET(Xm, k, 2, N);
-      for (size_t i = 0; i < N; ++i)
+      for (size_t i = 0; i < N; ++i) {
         tmp[i] = m0[i] * tau[m][0][i] + m1[i] * tau[m][1][i] + m2[i] * tau[m][2][i];
+        if (phi) tmp[i] = phi[i] * tmp[i]; /* ev: phi tau_proj (:222-224) */
+      }
       orc_diff3d(c, k, a[k], tmp, dxs[k]);
     }
     for (size_t i = 0; i < N; ++i) {
       r[m + 1][i] = r[m + 1][i] + a[0][i] + a[1][i] + a[2][i];
+      if (phi) r[m + 1][i] = inv_phi[i] * r[m + 1][i] - pg[m][i]; /* ev (:227) */
       r[m + 1][i] = Xj[i] * r[m + 1][i];
     }
   }
   /* pressure :243-276 */
   for (int k = 0; k < 3; ++k) {
-    for (size_t i = 0; i < N; ++i) tmp[i] = q[4 * N + i] * uh[k * N + i];
+    for (size_t i = 0; i < N; ++i) tmp[i] = (phi ? phi[i] * q[4 * N + i] : q[4 * N + i]) * uh[k * N + i];
     orc_diff3d(c, k, a[k], tmp, dxs[k]);
   }
   const double gt = c->gamma / (c->gamma - 1.0);
   for (size_t i = 0; i < N; ++i) {
     r[4][i] = gt * (-a[0][i] - a[1][i] - a[2][i]);
-    r[4][i] = r[4][i] + u[i] * pg[0][i] + u[N + i] * pg[1][i] + u[2 * N + i] * pg[2][i];
+    if (!phi) r[4][i] = r[4][i] + u[i] * pg[0][i] + u[N + i] * pg[1][i] + u[2 * N + i] * pg[2][i];
   }
   for (int j = 0; j < 3; ++j) { /* heat flux in x_j (:252-268): outer Dx without h */
     for (int k = 0; k < 3; ++k) orc_diff3d(c, k, a[k], tg[j], 1.0);
@@ -1156,6 +1188,10 @@ void orc_rhs_direct_gt(const orc_cfg *c, double *rhs, const double *q_in, const 
       r[4][i] = r[4][i] + m0[i] * a[0][i] + m1[i] * a[1][i] + m2[i] * a[2][i];
   }
   for (size_t i = 0; i < N; ++i) { /* :270-276 */
+    if (phi) { /* ev: div_1_phi on the bracket, then u.grad(p) (:292-293) */
+      r[4][i] = inv_phi[i] * r[4][i];
+      r[4][i] = r[4][i] + u[i] * pg[0][i] + u[N + i] * pg[1][i] + u[2 * N + i] * pg[2][i];
+    }
     r[4][i] = r[4][i] + Xj[i] * (t11[i] * g[0][0][i] + t12[i] * g[0][1][i] + t13[i] * g[0][2][i]);
     r[4][i] = r[4][i] + Xj[i] * (t12[i] * g[1][0][i] + t22[i] * g[1][1][i] + t23[i] * g[1][2][i]);
     r[4][i] = r[4][i] + Xj[i] * (t13[i] * g[2][0][i] + t23[i] * g[2][1][i] + t33[i] * g[2][2][i]);

@@ -91,6 +91,10 @@ double orc_lambda(const orc_cfg *c, double mu);
  * X_metric(n1,n2,n3,3,3) (metric without the Jacobian) and X_jacobian. */
 void orc_calc_metric(const orc_cfg *c, const double *X, const double *X_calc, double *X_metric,
                      double *X_jacobian);
+/* Volume-penalised deformed-grid RHS, src/equations/ns_rho_rhou_p_ev_phi_grid_transform.f90. */
+void orc_rhs_direct_ev_gt(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
+                          const double *Xj, const double *phi, const double *inv_phi,
+                          const double *qref, const double *shape);
 /* filter_weighted_iN_mpi_cartesian::filter_4d with the o2_e_c_p core on
  * q(n1,n2,n3,nv) along dir; weight / strength null mean 1. */
 void orc_filter_weighted4d(const orc_cfg *c, int dir, double *q, int nv, const double *weight,

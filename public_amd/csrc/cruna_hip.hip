@@ -438,12 +438,19 @@ void launch_curv_sweep1(cruna_hip_ctx *c, const Q *in) {
   const dim3 grid(c->geom[DIR].ntiles), block(c->wg[DIR]);
   T *rhs = (T *)c->rhs, *G = (T *)c->G, *TG = (T *)c->tg;
   const LineOp<T> &op = dops<T>(c)[DIR].op;
-  if (c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC)
-    hipLaunchKernelGGL((k_curv_sweep1<T, Q, DIR, true>), grid, block, 0, c->stream, in, rhs, G, TG, op,
-                       c->geom[DIR], c->phys, c->N, curv_metric<T>(c));
+  const bool per = c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC;
+  if (per && c->ev)
+    hipLaunchKernelGGL((k_curv_sweep1<T, Q, DIR, true, true>), grid, block, 0, c->stream, in, rhs, G, TG, op,
+                       c->geom[DIR], c->phys, c->N, curv_metric<T>(c), penal<T>(c));
+  else if (per)
+    hipLaunchKernelGGL((k_curv_sweep1<T, Q, DIR, true, false>), grid, block, 0, c->stream, in, rhs, G, TG, op,
+                       c->geom[DIR], c->phys, c->N, curv_metric<T>(c), penal<T>(c));
+  else if (c->ev)
+    hipLaunchKernelGGL((k_curv_sweep1<T, Q, DIR, false, true>), grid, block, 0, c->stream, in, rhs, G, TG, op,
+                       c->geom[DIR], c->phys, c->N, curv_metric<T>(c), penal<T>(c));
   else
-    hipLaunchKernelGGL((k_curv_sweep1<T, Q, DIR, false>), grid, block, 0, c->stream, in, rhs, G, TG, op,
-                       c->geom[DIR], c->phys, c->N, curv_metric<T>(c));
+    hipLaunchKernelGGL((k_curv_sweep1<T, Q, DIR, false, false>), grid, block, 0, c->stream, in, rhs, G, TG, op,
+                       c->geom[DIR], c->phys, c->N, curv_metric<T>(c), penal<T>(c));
 }
 
 template <typename T, typename Q, int DIR>
@@ -497,10 +504,16 @@ template <typename T, int DIR, bool PER, bool FF>
 void launch_curv_sweep2_k(cruna_hip_ctx *c, int mode) {
   const dim3 grid(c->geom[DIR].ntiles), block(c->wg[DIR]);
   const T *force = (const T *)(DIR == 0 && c->cnn ? c->force : nullptr);
-  hipLaunchKernelGGL((k_curv_sweep2<T, DIR, PER, FF>), grid, block, 0, c->stream, (T *)c->rhs,
-                     (const T *)c->G, (const T *)c->tg, dops<T>(c)[DIR].op, fops<T>(c)[DIR].op,
-                     c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, (T *)c->qs, c->cfg.dt,
-                     curv_metric<T>(c), force);
+  if (c->ev)
+    hipLaunchKernelGGL((k_curv_sweep2<T, DIR, PER, FF, true>), grid, block, 0, c->stream, (T *)c->rhs,
+                       (const T *)c->G, (const T *)c->tg, dops<T>(c)[DIR].op, fops<T>(c)[DIR].op,
+                       c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, (T *)c->qs, c->cfg.dt,
+                       curv_metric<T>(c), force, penal<T>(c));
+  else
+    hipLaunchKernelGGL((k_curv_sweep2<T, DIR, PER, FF, false>), grid, block, 0, c->stream, (T *)c->rhs,
+                       (const T *)c->G, (const T *)c->tg, dops<T>(c)[DIR].op, fops<T>(c)[DIR].op,
+                       c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, (T *)c->qs, c->cfg.dt,
+                       curv_metric<T>(c), force, penal<T>(c));
 }
 
 template <typename T, int DIR>
@@ -1012,7 +1025,11 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   if (hipMalloc(&c->qn, 5 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->acc, 5 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->qs, 5 * N * ts) != hipSuccess || hipMalloc(&c->rhs, 5 * N * ts) != hipSuccess ||
-      hipMalloc(&c->G, 7 * N * ts) != hipSuccess) {
+      hipMalloc(&c->G, 7 * N * ts) != hipSuccess ||
+      // the weighted filter's output buffer, allocated here so that a step
+      // loop under graph capture allocates nothing
+      ((cfg->wfilter_freq[0] > 0 || cfg->wfilter_freq[1] > 0 || cfg->wfilter_freq[2] > 0) &&
+       hipMalloc(&c->wtmp, 5 * N * sizeof(double)) != hipSuccess)) {
     (void)hipGetLastError();
     cruna_hip_destroy(c.release());
     return fail(CRUNA_HIP_ENOMEM, "device allocation failed");
@@ -1077,10 +1094,7 @@ int cruna_hip_set_sponge_shape(cruna_hip_ctx *c, const double *shape) {
 
 int cruna_hip_set_metric(cruna_hip_ctx *c, const double *X_metric, const double *X_jacobian) {
   if (!c || !X_metric || !X_jacobian) return fail(CRUNA_HIP_EINVAL, "null argument");
-  if (c->ev)
-    return fail(CRUNA_HIP_EINVAL,
-                "volume penalisation on a deformed grid (ns_rho_rhou_p_ev_phi_grid_transform) "
-                "is not implemented on the device");
+
   const long n1 = c->cfg.n[0], n2 = c->cfg.n[1], n3 = c->cfg.n[2], N = c->N;
   auto M = [&](int k, int j, long i, long jj, long kk) {
     return X_metric[(size_t)N * (k + 3 * j) + i + n1 * (jj + n2 * kk)];
@@ -1124,6 +1138,7 @@ int cruna_hip_set_metric(cruna_hip_ctx *c, const double *X_metric, const double 
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
   c->gt = false;
   c->curv = false;
+  if (c->ev) tensor = false;  // the penalised form runs on the per-point sweeps
   if (tensor) {
     const int rc = mixed ? upload_metric<float>(c, m1, m2, m3, X_jacobian, c->mt32)
                          : upload_metric<double>(c, m1, m2, m3, X_jacobian, c->mt64);
@@ -1152,10 +1167,12 @@ int cruna_hip_set_penalization(cruna_hip_ctx *c, const double *phi, const double
     return CRUNA_HIP_OK;
   }
   if (!phi || !inv_phi) return fail(CRUNA_HIP_EINVAL, "phi and inv_phi go together");
-  if (c->gt)
+  // on a deformed grid the penalised RHS (ns_rho_rhou_p_ev_phi_grid_transform)
+  // runs on the sweeps with the full metric per point
+  if (c->gt && !c->curv)
     return fail(CRUNA_HIP_EINVAL,
-                "volume penalisation on a deformed grid (ns_rho_rhou_p_ev_phi_grid_transform) "
-                "is not implemented on the device");
+                "set the penalisation before the metric: a penalised deformed grid keeps the full "
+                "metric per point, which a tensor-product metric set before it did not");
   const size_t N = c->N;
   for (size_t i = 0; i < N; ++i)
     if (!std::isfinite(phi[i]) || !std::isfinite(inv_phi[i]))

@@ -31,11 +31,14 @@ struct CurvMetric {
   const T *J;  // X_jacobian
 };
 
-template <typename T, typename Q, int DIR, bool PER>
+// EV: volume penalisation (ns_rho_rhou_p_ev_phi_grid_transform.f90): phi
+// inside every flux derivative, the stored 1/phi on each direction's share of
+// the divergences (not on the pressure gradient, u.grad(p) or tau:grad(u)).
+template <typename T, typename Q, int DIR, bool PER, bool EV>
 __global__ __launch_bounds__(NT) void k_curv_sweep1(const Q *__restrict__ qs, T *__restrict__ rhs,
                                                     T *__restrict__ G, T *__restrict__ TG,
                                                     LineOp<T> op, SweepGeom g, Phys ph, long N,
-                                                    CurvMetric<T> mt) {
+                                                    CurvMetric<T> mt, Penal<T> pe) {
   constexpr bool FIRST = DIR == 0;
   __shared__ LineLds<T> lds;
   const ThreadPos tp = thread_pos<DIR>(g);
@@ -135,14 +138,20 @@ __global__ __launch_bounds__(NT) void k_curv_sweep1(const Q *__restrict__ qs, T 
   for (int m = 0; m < PPT; ++m) ap[m] = ap[m] * ih;
 
   // convective fluxes with the contravariant velocity (:184-247)
+  T PH[PPT], IP[PPT];
+  if (EV) {
+    ld<DIR>(pe.phi, B, PS, PH);
+    ld<DIR>(pe.inv_phi, B, PS, IP);
+  }
+  auto div = [&](int m, T v) -> T { return EV ? IP[m] * v : v; };
   {  // mass
     T rho[PPT];
     ld<DIR>(qs, B, PS, rho);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = rho[m] * uh[m];
+    for (int m = 0; m < PPT; ++m) f[m] = EV ? (PH[m] * rho[m]) * uh[m] : rho[m] * uh[m];
     line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) x[m] = -(x[m] * ih);
+    for (int m = 0; m < PPT; ++m) x[m] = div(m, -(x[m] * ih));
     acc(rhs, x);
   }
 #pragma unroll
@@ -150,32 +159,33 @@ __global__ __launch_bounds__(NT) void k_curv_sweep1(const Q *__restrict__ qs, T 
     T mom[PPT];
     ld<DIR>(qs + (1 + i) * N, B, PS, mom);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = mom[m] * uh[m];
+    for (int m = 0; m < PPT; ++m) f[m] = EV ? (PH[m] * mom[m]) * uh[m] : mom[m] * uh[m];
     line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
       const T c = i == 0 ? M0[m] : (i == 1 ? M1[m] : M2[m]);
-      x[m] = -(x[m] * ih) - c * ap[m];
+      x[m] = div(m, -(x[m] * ih)) - c * ap[m];
     }
     acc(rhs + (1 + i) * N, x);
   }
   // pressure bracket: gamma/(gamma-1) (-a(p u_hat)) + u_hat a(p) (:245-250)
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) f[m] = p[m] * uh[m];
+  for (int m = 0; m < PPT; ++m) f[m] = EV ? (PH[m] * p[m]) * uh[m] : p[m] * uh[m];
   line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) x[m] = T(ph.gt) * (-(x[m] * ih)) + uh[m] * ap[m];
+  for (int m = 0; m < PPT; ++m) x[m] = div(m, T(ph.gt) * (-(x[m] * ih))) + uh[m] * ap[m];
   acc(rhs + 4 * N, x);
 }
 
-template <typename T, int DIR, bool PER, bool FF>
+template <typename T, int DIR, bool PER, bool FF, bool EV>
 __global__ __launch_bounds__(NT) void k_curv_sweep2(T *__restrict__ rhs, const T *__restrict__ G,
                                                     const T *__restrict__ TG, LineOp<T> op,
                                                     LineOp<T> fop, SweepGeom g, Phys ph, long N,
                                                     int mode, double *__restrict__ qn,
                                                     double *__restrict__ acc,
                                                     T *__restrict__ qs_out, double dt,
-                                                    CurvMetric<T> mt, const T *__restrict__ force) {
+                                                    CurvMetric<T> mt, const T *__restrict__ force,
+                                                    Penal<T> pe) {
   constexpr bool LAST = DIR == 0;
   __shared__ LineLds<T> lds;
   const ThreadPos tp = thread_pos<DIR>(g);
@@ -187,6 +197,11 @@ __global__ __launch_bounds__(NT) void k_curv_sweep2(T *__restrict__ rhs, const T
   ld<DIR>(mt.M + (DIR + 6) * N, B, PS, M2);
   ld<DIR>(mt.J, B, PS, Jv);
   ld<DIR>(G + 6 * N, B, PS, mu);
+  T PH[PPT], IP[PPT];
+  if (EV) {
+    ld<DIR>(pe.phi, B, PS, PH);
+    ld<DIR>(pe.inv_phi, B, PS, IP);
+  }
   // heat flux along DIR first (the metric row and mu are live for it and the
   // stresses only): sum_j M(DIR,j) D_DIR(lambda J T_j), no h (:252-268)
   T hs[PPT];
@@ -195,13 +210,18 @@ __global__ __launch_bounds__(NT) void k_curv_sweep2(T *__restrict__ rhs, const T
     T f[PPT], x[PPT];
     ld<DIR>(TG + j * N, B, PS, f);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) f[m] = (lambda_of(ph, mu[m]) * Jv[m]) * f[m];
+    for (int m = 0; m < PPT; ++m)
+      f[m] = (EV ? (PH[m] * lambda_of(ph, mu[m])) * Jv[m] : lambda_of(ph, mu[m]) * Jv[m]) * f[m];
     line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
       const T c = j == 0 ? M0[m] : (j == 1 ? M1[m] : M2[m]);
       hs[m] = j == 0 ? c * x[m] : hs[m] + c * x[m];
     }
+  }
+  if (EV) {
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) hs[m] = IP[m] * hs[m];
   }
   if (!LAST) {
     T k[PPT];
@@ -270,6 +290,14 @@ __global__ __launch_bounds__(NT) void k_curv_sweep2(T *__restrict__ rhs, const T
       rk_update<DIR, T>(mode, v, k, B, PS, N, rhs, qn, acc, qs_out, dt);
     }
   };
+  if (EV) {  // phi tau_proj inside the derivative
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+      t1[m] = PH[m] * t1[m];
+      t2[m] = PH[m] * t2[m];
+      t3[m] = PH[m] * t3[m];
+    }
+  }
   if (LAST) {  // pressure: J (gamma - 1) (bracket + heat + J tau:grad(u)) (:270-276)
     T k[PPT];
     ld<DIR>(rhs + 4 * N, B, PS, k);
@@ -296,7 +324,7 @@ __global__ __launch_bounds__(NT) void k_curv_sweep2(T *__restrict__ rhs, const T
     ld<DIR>(rhs + (1 + i) * N, B, PS, k);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
-      k[m] = k[m] + x[m] * ih;
+      k[m] = k[m] + (EV ? IP[m] * (x[m] * ih) : x[m] * ih);
       if (LAST) k[m] = Jv[m] * k[m];
     }
     if (LAST)

@@ -148,3 +148,54 @@ def test_weighted_filter_conserves_the_weighted_sum(d):
     assert np.abs(got - q).max() > 1e-2
     zero = np.zeros(shape, order="F")
     assert np.array_equal(O.filter_weighted4d(cfg, d, q, w, zero), q)
+
+
+# penalised deformed-grid RHS (ns_rho_rhou_p_ev_phi_grid_transform)
+def test_ev_phi_gt_with_unit_porosity_is_the_grid_transform_rhs():
+    from test_oracle_gt import _smooth_state, cfg as gt_cfg, wavy_grid
+    n = (16, 24, 16)
+    dx, X, Xc = wavy_grid(n)
+    c0 = gt_cfg(n, dx, mu0=0.02, T0=1.0 / 1.4)
+    Xm, Xj = O.calc_metric(c0, X, Xc)
+    q = _smooth_state(X)
+    one = np.ones(n, order="F")
+    ref = O.rhs_direct_gt(c0, q, Xm, Xj)
+    got = O.rhs_direct(gt_cfg(n, dx, mu0=0.02, T0=1.0 / 1.4, metric=(Xm, Xj), penalization=(one, one)), q)
+    for v in range(5):
+        e = np.linalg.norm(got[..., v] - ref[..., v]) / np.linalg.norm(ref[..., v])
+        assert e < 1e-13, (v, e)
+
+
+def _penalised_euler_physical(gamma):
+    x, y, z = sp.symbols("x y z", real=True)
+    X = (x, y, z)
+    from test_oracle_gt import _smooth_fields
+    rho, U, p = _smooth_fields(x, y, z)
+    phi = 0.7 + 0.2 * sp.sin(x) * sp.cos(0.4 * y) * sp.sin(z)
+    r = [-sum(sp.diff(phi * rho * U[j], X[j]) for j in range(3)) / phi]
+    for i in range(3):
+        r.append(-sum(sp.diff(phi * rho * U[i] * U[j], X[j]) for j in range(3)) / phi - sp.diff(p, X[i]))
+    r.append((gamma - 1) * (-(gamma / (gamma - 1)) * sum(sp.diff(phi * p * U[j], X[j]) for j in range(3)) / phi
+                            + sum(U[j] * sp.diff(p, X[j]) for j in range(3))))
+    return [sp.lambdify((x, y, z), e, "numpy") for e in r], sp.lambdify((x, y, z), phi, "numpy")
+
+
+def test_ev_phi_gt_inviscid_converges_to_penalised_euler():
+    """mu0 = 0 on the wavy grid with a varying porosity: the penalised
+    grid-transform RHS converges at high order to the penalised Euler
+    equations in physical coordinates (rho_t = -(1/phi) div(phi rho u), ...),
+    which know nothing of the mapping."""
+    from test_oracle_gt import _converges, _smooth_state, cfg as gt_cfg, wavy_grid
+    fns, phi_fn = _penalised_euler_physical(1.4)
+    pairs = []
+    for n in [(16, 33, 16), (32, 65, 32)]:
+        dx, X, Xc = wavy_grid(n)
+        c0 = gt_cfg(n, dx, mu0=0.0, T0=1.0 / 1.4)
+        Xm, Xj = O.calc_metric(c0, X, Xc)
+        phi = np.asfortranarray(np.broadcast_to(phi_fn(X[..., 0], X[..., 1], X[..., 2]), n))
+        q = _smooth_state(X)
+        got = O.rhs_direct(gt_cfg(n, dx, mu0=0.0, T0=1.0 / 1.4, metric=(Xm, Xj),
+                                  penalization=(phi, 1.0 / phi)), q)
+        exact = [np.broadcast_to(f(X[..., 0], X[..., 1], X[..., 2]), n) for f in fns]
+        pairs.append((got, exact))
+    _converges(pairs, 2e-3, 12)
