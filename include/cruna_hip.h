@@ -43,6 +43,9 @@ extern "C" {
 #define CRUNA_HIP_DIFF_O4_PADE 2     /* src/diff/diff_pen_core_o4_i_c_{p,o}_pade.f90 */
 #define CRUNA_HIP_DIFF_O2_EXPLICIT 3 /* src/diff/diff_pen_core_o2_e_c_o.f90 (open lines only) */
 
+#define CRUNA_HIP_EQ_NS 0    /* src/equations/ns_rho_rhou_p*.f90 (default) */
+#define CRUNA_HIP_EQ_EULER 1 /* src/equations/euler_rho_rhou_p.f90 (Cartesian) */
+
 typedef struct cruna_hip_cfg {
   int n[3];              /* params%geom%n1..n3 (block = whole grid on one GPU) */
   double dx[3];          /* params%geom%dx1..dx3 */
@@ -81,6 +84,10 @@ typedef struct cruna_hip_cfg {
                             the weighted filter (filter_weighted_iN with the
                             o2_e_c_p core, weight phi of the penalisation or 1,
                             strength 1) after the compact filters of a step */
+  int equations;         /* governing_equations: CRUNA_HIP_EQ_NS or CRUNA_HIP_EQ_EULER
+                            (euler_rho_rhou_p: the NS sweeps with mu = lambda = 0;
+                            Cartesian grids without penalisation; its adjoint with
+                            sponge rows (400) returns EINVAL) */
 } cruna_hip_cfg;
 
 typedef struct cruna_hip_ctx cruna_hip_ctx;

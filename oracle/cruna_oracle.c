@@ -827,9 +827,53 @@ void orc_sponge_shape(const orc_cfg *c, const double x0[3], const double x1[3],
 }
 
 /* ------------------------------------------------------------------------ */
+/* Euler right-hand side, src/equations/euler_rho_rhou_p.f90:30-122 (body :53-108): mass
+ * -D(rho u_d); momentum -D(rho u_d u_i) - p_i; pressure
+ * -gamma D(p u_d) + (gamma - 1) u.grad(p); force (no-op), BC rows. */
+void orc_rhs_direct_euler(const orc_cfg *c, double *rhs, const double *q_in, const double *qref,
+                          const double *shape) {
+  const size_t N = npts(c);
+  const double dxs[3] = {c->dx[0], c->dx[1], c->dx[2]};
+  double *q = xalloc(5 * N), *u = xalloc(3 * N), *tmp = xalloc(N);
+  double *a[3] = {xalloc(N), xalloc(N), xalloc(N)};
+  double *pg[3] = {xalloc(N), xalloc(N), xalloc(N)};
+  double *r[5];
+  for (int v = 0; v < 5; ++v) r[v] = rhs + v * N;
+  memcpy(q, q_in, 5 * N * sizeof(double)); /* :53 */
+  orc_bc_q(c, q, qref, shape);              /* :56 */
+  for (int m = 0; m < 3; ++m)               /* :59-63 */
+    for (size_t i = 0; i < N; ++i) u[m * N + i] = q[(m + 1) * N + i] / q[i];
+  for (int k = 0; k < 3; ++k) orc_diff3d(c, k, a[k], q + (k + 1) * N, dxs[k]); /* mass :66-69 */
+  for (size_t i = 0; i < N; ++i) r[0][i] = -a[0][i] - a[1][i] - a[2][i];
+  for (int m = 0; m < 3; ++m) { /* momentum :73-95 */
+    for (int k = 0; k < 3; ++k) {
+      for (size_t i = 0; i < N; ++i) tmp[i] = q[(k + 1) * N + i] * u[m * N + i];
+      orc_diff3d(c, k, a[k], tmp, dxs[k]);
+    }
+    orc_diff3d(c, m, pg[m], q + 4 * N, dxs[m]);
+    for (size_t i = 0; i < N; ++i) r[m + 1][i] = -a[0][i] - a[1][i] - a[2][i] - pg[m][i];
+  }
+  for (int k = 0; k < 3; ++k) { /* pressure :97-102 */
+    for (size_t i = 0; i < N; ++i) tmp[i] = q[4 * N + i] * u[k * N + i];
+    orc_diff3d(c, k, a[k], tmp, dxs[k]);
+  }
+  for (size_t i = 0; i < N; ++i) {
+    r[4][i] = -c->gamma * (a[0][i] + a[1][i] + a[2][i]);
+    r[4][i] = r[4][i] + (c->gamma - 1.0) * (u[i] * pg[0][i] + u[N + i] * pg[1][i] + u[2 * N + i] * pg[2][i]);
+  }
+  orc_bc_rhs(c, rhs, q, qref, shape); /* force (no-op) :105, BC rows :108 */
+  free(q); free(u); free(tmp);
+  for (int k = 0; k < 3; ++k) { free(a[k]); free(pg[k]); }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Direct NS right-hand side, src/equations/ns_rho_rhou_p.f90:31-253.       */
 void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const double *qref,
                     const double *shape, int heat_unit_spacing) {
+  if (c->equations == ORC_EQ_EULER) { /* governing_equations => euler_rho_rhou_p */
+    orc_rhs_direct_euler(c, rhs, q_in, qref, shape);
+    return;
+  }
   if (c->X_metric && c->penal_phi) { /* => ns_rho_rhou_p_ev_phi_grid_transform */
     orc_rhs_direct_ev_gt(c, rhs, q_in, c->X_metric, c->X_jacobian, c->penal_phi, c->penal_inv_phi,
                          qref, shape);

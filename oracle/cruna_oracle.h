@@ -58,7 +58,13 @@ typedef struct orc_cfg {
   /* filter.dwi1freq..dwi3freq (0 = never): the cruna_hpc loop's weighted
    * filters with the o2_e_c_p core, weight penal_phi (or 1), strength 1 */
   int wfilter_freq[3];
+  /* governing_equations: ORC_EQ_NS (ns_rho_rhou_p*) or ORC_EQ_EULER
+   * (euler_rho_rhou_p, Cartesian) */
+  int equations;
 } orc_cfg;
+
+#define ORC_EQ_NS 0
+#define ORC_EQ_EULER 1
 
 #define ORC_DIFF_O6_COMPACT 0
 #define ORC_DIFF_O4_EXPLICIT 1
@@ -91,6 +97,9 @@ double orc_lambda(const orc_cfg *c, double mu);
  * X_metric(n1,n2,n3,3,3) (metric without the Jacobian) and X_jacobian. */
 void orc_calc_metric(const orc_cfg *c, const double *X, const double *X_calc, double *X_metric,
                      double *X_jacobian);
+/* Euler RHS, src/equations/euler_rho_rhou_p.f90:30-122. */
+void orc_rhs_direct_euler(const orc_cfg *c, double *rhs, const double *q_in, const double *qref,
+                          const double *shape);
 /* Volume-penalised deformed-grid RHS, src/equations/ns_rho_rhou_p_ev_phi_grid_transform.f90. */
 void orc_rhs_direct_ev_gt(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
                           const double *Xj, const double *phi, const double *inv_phi,

@@ -17,6 +17,7 @@ MAX_BC = 32
 
 PERIODIC, OPEN = 0, 1
 DIFF_O6_COMPACT, DIFF_O4_EXPLICIT, DIFF_O4_PADE, DIFF_O2_EXPLICIT = 0, 1, 2, 3
+EQ_NS, EQ_EULER = 0, 1
 SUTHERLAND, POWER = 0, 1
 
 
@@ -45,6 +46,7 @@ class OrcCfg(C.Structure):
         ("penal_phi", C.c_void_p),
         ("penal_inv_phi", C.c_void_p),
         ("wfilter_freq", C.c_int * 3),
+        ("equations", C.c_int),
     ]
 
 
@@ -134,7 +136,7 @@ class Cfg:
     def __init__(self, n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0,
                  W=1.0, Pr=0.71, mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76,
                  dt=1e-3, filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), bc=(), metric=None,
-                 diff_kind=(0, 0, 0), penalization=None, wfilter_freq=(0, 0, 0)):
+                 diff_kind=(0, 0, 0), penalization=None, wfilter_freq=(0, 0, 0), equations=0):
         """metric: (X_metric (n1,n2,n3,3,3), X_jacobian (n1,n2,n3)) of a deformed
         grid; the direct RHS is then the grid-transform one.  diff_kind per
         direction: DIFF_O6_COMPACT, DIFF_O4_EXPLICIT or DIFF_O4_PADE.
@@ -152,6 +154,7 @@ class Cfg:
         c.filter_alpha = filter_alpha
         c.gamma, c.R, c.W, c.Pr, c.mu0, c.T0 = gamma, R, W, Pr, mu0, T0
         c.visc_law, c.visc_exp, c.dt = visc_law, visc_exp, dt
+        c.equations = int(equations)
         c.nbc = len(bc)
         for r, row in enumerate(bc):
             for j in range(10):

@@ -6,4 +6,4 @@ synthetic workloads.
 """
 from . import ic  # noqa: F401
 from .cruna import (ADJOINT, DIFF_O2_EXPLICIT, DIFF_O4_EXPLICIT, DIFF_O4_PADE, DIFF_O6_COMPACT, FORWARD, FP32_MIXED, FP64,  # noqa: F401
-                    OPEN, PERIODIC, POWER, SUTHERLAND, Closure, Cruna, CrunaError, Ensemble, lib)
+                    EQ_EULER, EQ_NS, OPEN, PERIODIC, POWER, SUTHERLAND, Closure, Cruna, CrunaError, Ensemble, lib)

@@ -22,6 +22,7 @@ PERIODIC, OPEN = 0, 1
 SUTHERLAND, POWER = 0, 1
 FP64, FP32_MIXED = 0, 1
 DIFF_O6_COMPACT, DIFF_O4_EXPLICIT, DIFF_O4_PADE, DIFF_O2_EXPLICIT = 0, 1, 2, 3
+EQ_NS, EQ_EULER = 0, 1
 MAX_BC = 32
 
 
@@ -54,6 +55,7 @@ class Cfg(C.Structure):
         ("bc", (C.c_double * 10) * MAX_BC),
         ("diff_kind", C.c_int * 3),
         ("wfilter_freq", C.c_int * 3),
+        ("equations", C.c_int),
     ]
 
 
@@ -159,7 +161,8 @@ def _f64(a, shape):
 def make_cfg(n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0, W=1.0, Pr=0.71,
              mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76, dt=1e-3,
              filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), heat_unit_spacing=1, precision=FP64,
-             bc=(), tile_lines=(0, 0, 0), diff_kind=(DIFF_O6_COMPACT,) * 3, wfilter_freq=(0, 0, 0)):
+             bc=(), tile_lines=(0, 0, 0), diff_kind=(DIFF_O6_COMPACT,) * 3, wfilter_freq=(0, 0, 0),
+             equations=0):
     """cruna_hip_cfg from the reference's parameter names (defaults of the library)."""
     cfg = Cfg()
     lib().cruna_hip_default_cfg(C.byref(cfg))
@@ -176,6 +179,7 @@ def make_cfg(n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0,
     cfg.gamma, cfg.R, cfg.W, cfg.Pr, cfg.mu0, cfg.T0 = gamma, R, W, Pr, mu0, T0
     cfg.visc_law, cfg.visc_exp, cfg.dt = visc_law, visc_exp, dt
     cfg.heat_unit_spacing, cfg.precision = heat_unit_spacing, precision
+    cfg.equations = int(equations)
     cfg.nbc = len(bc)
     for r, row in enumerate(bc):
         for j, v in enumerate(row):

@@ -318,6 +318,14 @@ Penal<float> penal<float>(cruna_hip_ctx *c) { return c->pe32; }
 // set_boundary_condition_rhss zeroes a variable of the adjoint RHS on wall
 // rows (100); the device adjoint does not apply rows to its complete RHS
 const char *kNoWallAdjoint = "wall rows (100) are not implemented in the device adjoint";
+// euler_rho_rhou_p's variants on deformed grids and with penalisation are
+// separate modules the device does not build
+const char *kEulerCartesian =
+    "the Euler equations are built for Cartesian grids without penalisation only";
+// euler_rho_rhou_p.f90:173 passes q to set_boundary_condition_rhss, so its
+// adjoint applies sponge_rhss on 400 rows, which the device adjoint does not
+const char *kNoEulerSpongeAdjoint =
+    "the Euler adjoint with sponge rows (400) is not implemented on the device";
 const char *kNoEvAdjoint =
     "the adjoint of ns_rho_rhou_p_ev_phi is not implemented (the reference stops there too)";
 
@@ -925,6 +933,8 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
     if (!(cfg->dx[d] > 0)) return fail(CRUNA_HIP_EINVAL, "dx must be positive");
     if (cfg->wfilter_freq[d] < 0) return fail(CRUNA_HIP_EINVAL, "wfilter_freq must be >= 0");
   }
+  if (cfg->equations != CRUNA_HIP_EQ_NS && cfg->equations != CRUNA_HIP_EQ_EULER)
+    return fail(CRUNA_HIP_EINVAL, "unknown equations");
   if ((double)cfg->n[0] * cfg->n[1] * cfg->n[2] * sizeof(double) >= 4294967296.0)
     return fail(CRUNA_HIP_EINVAL, "one FP64 field must fit in 4 GiB (32-bit device offsets)");
   if (cfg->precision != CRUNA_HIP_FP64 && cfg->precision != CRUNA_HIP_FP32_MIXED)
@@ -1008,7 +1018,9 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   ph.gm1 = cfg->gamma - 1.0;
   ph.gt = cfg->gamma / (cfg->gamma - 1.0);
   ph.w_inv_r = cfg->W / cfg->R;
-  ph.mu0 = cfg->mu0;
+  // euler_rho_rhou_p (governing_equations): the NS sweeps without viscosity
+  // and conduction (mu = lambda = 0 make every viscous and heat term zero)
+  ph.mu0 = cfg->equations == CRUNA_HIP_EQ_EULER ? 0.0 : cfg->mu0;
   ph.T0 = cfg->T0;
   ph.visc_exp = cfg->visc_exp;
   ph.visc_law = cfg->visc_law;
@@ -1094,6 +1106,7 @@ int cruna_hip_set_sponge_shape(cruna_hip_ctx *c, const double *shape) {
 
 int cruna_hip_set_metric(cruna_hip_ctx *c, const double *X_metric, const double *X_jacobian) {
   if (!c || !X_metric || !X_jacobian) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (c->cfg.equations == CRUNA_HIP_EQ_EULER) return fail(CRUNA_HIP_EINVAL, kEulerCartesian);
 
   const long n1 = c->cfg.n[0], n2 = c->cfg.n[1], n3 = c->cfg.n[2], N = c->N;
   auto M = [&](int k, int j, long i, long jj, long kk) {
@@ -1167,6 +1180,7 @@ int cruna_hip_set_penalization(cruna_hip_ctx *c, const double *phi, const double
     return CRUNA_HIP_OK;
   }
   if (!phi || !inv_phi) return fail(CRUNA_HIP_EINVAL, "phi and inv_phi go together");
+  if (c->cfg.equations == CRUNA_HIP_EQ_EULER) return fail(CRUNA_HIP_EINVAL, kEulerCartesian);
   // on a deformed grid the penalised RHS (ns_rho_rhou_p_ev_phi_grid_transform)
   // runs on the sweeps with the full metric per point
   if (c->gt && !c->curv)
@@ -1320,6 +1334,7 @@ int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const
   if (!c || !rhs || !qs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
+  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (int rc = stage_adjoint_inputs(c, qs, q, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
   if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
@@ -1348,6 +1363,7 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *
   if (!c || !qs1 || !qs0 || !q0) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
+  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (int rc = stage_adjoint_inputs(c, qs0, q0, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
   if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
@@ -1422,6 +1438,7 @@ int cruna_hip_run_adjoint(cruna_hip_ctx *c, int nt_end, int nsteps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
+  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
   if (!c->qa) return fail(CRUNA_HIP_EINVAL, "no adjoint state (cruna_hip_set_adjoint_state)");
   if (c->traj_slots == 0) return fail(CRUNA_HIP_EINVAL, "no trajectory (cruna_hip_trajectory_alloc)");
@@ -1493,6 +1510,7 @@ int cruna_hip_run_adjoint_subsets(cruna_hip_ctx *c, int subsets, int steps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
+  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (subsets < 1 || steps < 1) return fail(CRUNA_HIP_EINVAL, "subsets and steps must be >= 1");
   if (c->ckpt_n < subsets + 1) return fail(CRUNA_HIP_EINVAL, "not enough subset checkpoints (cruna_hip_subsets_alloc)");
   if (c->traj_slots < steps) return fail(CRUNA_HIP_EINVAL, "the trajectory holds fewer slots than steps");
@@ -1718,6 +1736,7 @@ int cruna_hip_ensemble_set_role(cruna_hip_ensemble *e, int i, int role, int nt) 
   if (role == CRUNA_HIP_ADJOINT) {
     if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
     if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
+    if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
     if (!c->qa || c->traj_slots == 0)
       return fail(CRUNA_HIP_EINVAL, "an adjoint member needs its adjoint state and trajectory first");
     if (c->traj_slots > 1 && (nt < 1 || nt > c->traj_slots))

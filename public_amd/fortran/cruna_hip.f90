@@ -16,6 +16,7 @@ module cruna_hip
   integer, parameter, public :: CRUNA_HIP_MAX_BC = 32
   integer, parameter, public :: CRUNA_HIP_DIFF_O6_COMPACT = 0, CRUNA_HIP_DIFF_O4_EXPLICIT = 1, &
                                 CRUNA_HIP_DIFF_O4_PADE = 2, CRUNA_HIP_DIFF_O2_EXPLICIT = 3
+  integer, parameter, public :: CRUNA_HIP_EQ_NS = 0, CRUNA_HIP_EQ_EULER = 1
 
   ! cruna_hip_cfg (C layout; bc(1:10, r) is row r of boundary_conditions.dat)
   type, bind(C), public :: cruna_hip_cfg
@@ -41,6 +42,7 @@ module cruna_hip
     real(c_double) :: bc(10, CRUNA_HIP_MAX_BC)
     integer(c_int) :: diff_kind(3)
     integer(c_int) :: wfilter_freq(3)
+    integer(c_int) :: equations
   end type cruna_hip_cfg
 
   interface

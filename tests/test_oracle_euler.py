@@ -1,0 +1,68 @@
+"""Pins of the oracle's Euler RHS (euler_rho_rhou_p.f90 right_hand_side_direct_3d):
+  * it converges at high order to the compressible Euler equations on a
+    periodic box (derived symbolically, in conservative form for mass and
+    momentum and in the (gamma, u.grad p) form of the pressure equation);
+  * it equals the NS RHS with mu0 = 0 to rounding (gamma D(p u) against
+    gamma/(gamma-1) (gamma-1) D(p u)), which is how the device runs it."""
+import numpy as np
+import sympy as sp
+
+from oracle import oracle as O
+
+
+def _box(n):
+    h = 2 * np.pi / n
+    x = np.arange(n) * h
+    return h, np.stack(np.meshgrid(x, x, x, indexing="ij"), axis=-1)
+
+
+def _fields(x, y, z, lib=np):
+    rho = 1 + 0.1 * lib.sin(x) * lib.cos(z) + 0.05 * lib.cos(y)
+    u = (0.3 * lib.sin(y) + 0.05 * lib.sin(x + z), 0.05 * lib.cos(x) * lib.sin(z), 0.05 * lib.sin(z) * lib.cos(y))
+    p = 1 / 1.4 + 0.05 * lib.cos(x) * lib.sin(z) * lib.cos(y)
+    return rho, u, p
+
+
+def _state(X):
+    rho, u, p = _fields(X[..., 0], X[..., 1], X[..., 2])
+    q = np.zeros(X.shape[:3] + (5,), order="F")
+    q[..., 0] = rho
+    for i in range(3):
+        q[..., 1 + i] = rho * u[i]
+    q[..., 4] = p
+    return q
+
+
+def test_euler_rhs_converges_to_euler_equations():
+    gamma = 1.4
+    x, y, z = sp.symbols("x y z", real=True)
+    X = (x, y, z)
+    rho, U, p = _fields(x, y, z, lib=sp)
+    r = [-sum(sp.diff(rho * U[j], X[j]) for j in range(3))]
+    for i in range(3):
+        r.append(-sum(sp.diff(rho * U[i] * U[j], X[j]) for j in range(3)) - sp.diff(p, X[i]))
+    r.append(-gamma * sum(sp.diff(p * U[j], X[j]) for j in range(3))
+             + (gamma - 1) * sum(U[j] * sp.diff(p, X[j]) for j in range(3)))
+    fns = [sp.lambdify((x, y, z), e, "numpy") for e in r]
+    errs = []
+    for n in (16, 32):
+        h, G = _box(n)
+        got = O.rhs_direct(O.Cfg((n,) * 3, (h,) * 3, equations=O.EQ_EULER), _state(G))
+        errs.append([np.linalg.norm(got[..., v] - np.broadcast_to(fns[v](G[..., 0], G[..., 1], G[..., 2]), (n,) * 3))
+                     / np.linalg.norm(np.broadcast_to(fns[v](G[..., 0], G[..., 1], G[..., 2]), (n,) * 3))
+                     for v in range(5)])
+    errs = np.array(errs)
+    assert (errs[1] < 1e-4).all(), errs
+    assert (errs[0] / errs[1] > 30).all(), errs
+
+
+def test_euler_rhs_is_ns_without_viscosity():
+    n = 24
+    h, G = _box(n)
+    q = _state(G)
+    eu = O.rhs_direct(O.Cfg((n,) * 3, (h,) * 3, equations=O.EQ_EULER), q)
+    ns = O.rhs_direct(O.Cfg((n,) * 3, (h,) * 3, mu0=0.0), q)
+    for v in range(5):
+        assert np.linalg.norm(eu[..., v] - ns[..., v]) <= 1e-13 * np.linalg.norm(eu[..., v]), v
+    visc = O.rhs_direct(O.Cfg((n,) * 3, (h,) * 3, mu0=0.05), q)
+    assert np.linalg.norm(visc - eu) > 1e-4 * np.linalg.norm(eu)
