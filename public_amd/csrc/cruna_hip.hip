@@ -122,6 +122,9 @@ struct cruna_hip_ctx {
   // point in the compute precision
   bool ev = false;
   void *pen = nullptr;
+  // bumped by every setter that changes device buffers, kernel choices or
+  // launch shapes, so that an ensemble re-records graphs made before
+  unsigned gen = 0;
   // FP64 phi, the weight of the weighted filter when a penalisation is set
   // (filter_weighted_pen_core_o2_e_c_p init_filter), and its output buffer
   double *wphi = nullptr, *wtmp = nullptr, *wbuf = nullptr;
@@ -1092,6 +1095,7 @@ int cruna_hip_get_state(cruna_hip_ctx *c, double *q) {
 
 int cruna_hip_set_qref(cruna_hip_ctx *c, const double *qref) {
   if (!c || !qref) return fail(CRUNA_HIP_EINVAL, "null argument");
+  ++c->gen;
   if (!c->qref) HIP_TRY(hipMalloc(&c->qref, 5 * c->N * sizeof(double)));
   HIP_TRY(hipMemcpy(c->qref, qref, 5 * c->N * sizeof(double), hipMemcpyHostToDevice));
   return CRUNA_HIP_OK;
@@ -1099,6 +1103,7 @@ int cruna_hip_set_qref(cruna_hip_ctx *c, const double *qref) {
 
 int cruna_hip_set_sponge_shape(cruna_hip_ctx *c, const double *shape) {
   if (!c || !shape) return fail(CRUNA_HIP_EINVAL, "null argument");
+  ++c->gen;
   if (!c->shape) HIP_TRY(hipMalloc(&c->shape, c->N * sizeof(double)));
   HIP_TRY(hipMemcpy(c->shape, shape, c->N * sizeof(double), hipMemcpyHostToDevice));
   return CRUNA_HIP_OK;
@@ -1106,6 +1111,7 @@ int cruna_hip_set_sponge_shape(cruna_hip_ctx *c, const double *shape) {
 
 int cruna_hip_set_metric(cruna_hip_ctx *c, const double *X_metric, const double *X_jacobian) {
   if (!c || !X_metric || !X_jacobian) return fail(CRUNA_HIP_EINVAL, "null argument");
+  ++c->gen;
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER) return fail(CRUNA_HIP_EINVAL, kEulerCartesian);
 
   const long n1 = c->cfg.n[0], n2 = c->cfg.n[1], n3 = c->cfg.n[2], N = c->N;
@@ -1168,6 +1174,7 @@ int cruna_hip_set_metric(cruna_hip_ctx *c, const double *X_metric, const double 
 
 int cruna_hip_set_penalization(cruna_hip_ctx *c, const double *phi, const double *inv_phi) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  ++c->gen;
   if (!phi && !inv_phi) {  // back to ns_rho_rhou_p
     (void)hipStreamSynchronize(c->stream);
     if (c->pen) (void)hipFree(c->pen);
@@ -1393,6 +1400,7 @@ int cruna_hip_get_adjoint_state(cruna_hip_ctx *c, double *qs) {
 
 int cruna_hip_trajectory_alloc(cruna_hip_ctx *c, int nslots) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  ++c->gen;
   if (nslots < 0) return fail(CRUNA_HIP_EINVAL, "nslots must be >= 0");
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (c->traj) HIP_TRY(hipFree(c->traj));
@@ -1429,6 +1437,7 @@ int cruna_hip_trajectory_get(cruna_hip_ctx *c, int nt, double *q) {
 
 int cruna_hip_trajectory_record(cruna_hip_ctx *c, int on) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  ++c->gen;
   if (on && c->traj_slots == 0) return fail(CRUNA_HIP_EINVAL, "no trajectory (cruna_hip_trajectory_alloc)");
   c->record = on != 0;
   return CRUNA_HIP_OK;
@@ -1471,6 +1480,7 @@ extern "C++" int run_direct_loop(cruna_hip_ctx *c, int nt0, int nsteps, bool wei
 
 int cruna_hip_subsets_alloc(cruna_hip_ctx *c, int subsets, int steps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  ++c->gen;
   if (subsets < 0 || steps < 0) return fail(CRUNA_HIP_EINVAL, "subsets and steps must be >= 0");
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (c->ckpt) HIP_TRY(hipFree(c->ckpt));
@@ -1539,6 +1549,7 @@ int cruna_hip_run_adjoint_subsets(cruna_hip_ctx *c, int subsets, int steps) {
 
 int cruna_hip_autotune(cruna_hip_ctx *c, int steps, int tile_lines[3]) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  ++c->gen;
   if (steps < 1) return fail(CRUNA_HIP_EINVAL, "steps must be >= 1");
   if (int rc = ensure_tmp(c)) return rc;
   // the trial steps run on a saved copy of the state, restored afterwards
@@ -1652,6 +1663,7 @@ struct cruna_hip_ensemble {
   std::vector<int> nt;    // forward: steps done; adjoint: next step to run backward
   std::vector<bool> warm;
   std::vector<hipGraphExec_t> exec;  // one period of steps of each member
+  std::vector<unsigned> gen;         // each member's setter generation when recorded
   int period = 0;
   bool dirty = true;  // the recorded graph no longer matches the members
 };
@@ -1787,6 +1799,10 @@ int cruna_hip_ensemble_run(cruna_hip_ensemble *e, int nsteps) {
     for (auto *c : e->m) HIP_TRY(hipStreamSynchronize(c->stream));
     return CRUNA_HIP_OK;
   }
+  // a member changed through a setter since the recording (its buffers,
+  // kernels or launch shapes may differ): record again
+  if (e->gen.size() != M) e->dirty = true;
+  for (size_t i = 0; i < M && !e->dirty; ++i) e->dirty = e->m[i]->gen != e->gen[i];
   // record one period of steps of each member (the recording runs nothing)
   if (e->dirty || e->exec.size() != M || e->period != P) {
     drop_graphs(e);
@@ -1816,6 +1832,8 @@ int cruna_hip_ensemble_run(cruna_hip_ensemble *e, int nsteps) {
     }
     e->nt = nt_save;
     e->period = P;
+    e->gen.resize(M);
+    for (size_t i = 0; i < M; ++i) e->gen[i] = e->m[i]->gen;
     e->dirty = false;
   }
   // replay: members interleaved so that their streams overlap
@@ -2316,6 +2334,7 @@ int cruna_hip_cnn_train(cruna_hip_cnn *n, int iters, float lr, int batch, int cr
 
 int cruna_hip_set_closure(cruna_hip_ctx *c, cruna_hip_cnn *n) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  ++c->gen;
   if (n) {
     for (int d = 0; d < 3; ++d)
       if (c->cfg.n[d] != n->S || c->cfg.scheme[d] != CRUNA_HIP_PERIODIC)

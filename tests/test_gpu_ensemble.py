@@ -117,3 +117,45 @@ def test_autotune_keeps_results():
         g.synchronize()
         got = g.get_state()
     assert np.array_equal(got, _standalone_direct(m, q, 3))
+
+
+def test_ensemble_rerecords_after_member_setters_and_runs_new_paths():
+    """Graphs recorded by a first run are recorded again when a member changes
+    through a setter (here a penalisation set on member 0 between runs: its
+    buffers and kernels change); member 1 runs the complete-RHS boundary row
+    (204) with its point-wise RK4 update and the weighted filter's copies
+    inside the graph.  Both equal the same runs made eagerly, bit for bit."""
+    from public_amd import FORWARD, Cruna, Ensemble
+    m0, q0 = _fwd_member(32, 11, 0.3)
+    m1, q1 = _fwd_member(32, 12, 0.2, wfilter_freq=(1, 1, 1), bc=[(204, 1, 32, 1, 1, 1, 32, 0, -1, 0)])
+    n = 32
+    x = np.arange(n) * m0["dx"][0]
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    phi = np.asfortranarray(0.7 + 0.2 * np.sin(X) * np.cos(Y) * np.sin(Z))
+    with Ensemble([m0, m1]) as ens:
+        ens[0].set_state(q0)
+        ens[1].set_qref(q1)
+        ens[1].set_state(q1)
+        ens.set_role(0, FORWARD, 0)
+        ens.set_role(1, FORWARD, 0)
+        ens.run(4)
+        ens[0].set_penalization(phi)
+        ens.run(4)
+        out = [ens[i].get_state() for i in range(2)]
+    kw0 = dict(m0)
+    with Cruna(kw0.pop("n"), kw0.pop("dx"), **kw0) as g:
+        g.set_state(q0)
+        g.run_direct(4)
+        g.set_penalization(phi)
+        g.run_direct(4, nt0=4)
+        g.synchronize()
+        ref0 = g.get_state()
+    kw1 = dict(m1)
+    with Cruna(kw1.pop("n"), kw1.pop("dx"), **kw1) as g:
+        g.set_qref(q1)
+        g.set_state(q1)
+        g.run_direct(8)
+        g.synchronize()
+        ref1 = g.get_state()
+    assert np.array_equal(out[0], ref0)
+    assert np.array_equal(out[1], ref1)
