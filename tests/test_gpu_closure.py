@@ -196,7 +196,7 @@ def test_trained_closure_gains_over_plain_coarse_grid():
     """The committed closure (trained on the GPU by the library's own
     kernels on box-filtered 256^3 solves) against the same 64^3 solver
     without it, from a held-out seed over 300 coarse steps (1200 fine
-    steps, the plain FP64 256^3 solve as the judge): the velocity-spectrum
+    steps, the plain FP64 256^3 solve as the reference): the velocity-spectrum
     error must fall well below the plain coarse grid's (measured at seed
     2001, Ma 0.3: 0.25 against 0.58)."""
     import os
