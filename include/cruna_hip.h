@@ -70,7 +70,8 @@ typedef struct cruna_hip_cfg {
   /* boundary_conditions.dat rows: type; i1s; i1e; i2s; i2e; i3s; i3e; k1; k2; k3.
      Types 0, 100 (slip wall: column 8 names the variable whose RHS is zero),
      200 (characteristic toward the inner neighbour; box one point thick along
-     the normal), 201, 204 (characteristic on q / on the RHS), 400, 401
+     the normal), 201, 204 (characteristic on q / on the RHS), 202, 205 (the
+     same with the normals per point, cruna_hip_set_kvector), 400, 401
      (sponge on the RHS / on q) */
   int nbc;
   double bc[CRUNA_HIP_MAX_BC][10];
@@ -110,6 +111,12 @@ int cruna_hip_get_state(cruna_hip_ctx *ctx, double *q);
  * the sponge shape sigma(n1,n2,n3) (shape_functions.f90:69-150). */
 int cruna_hip_set_qref(cruna_hip_ctx *ctx, const double *qref);
 int cruna_hip_set_sponge_shape(cruna_hip_ctx *ctx, const double *shape);
+
+/* kvector(n1,n2,n3,3): the outward boundary normal per point that rows of
+ * types 202 and 205 use (characteristic_q_k4d / characteristic_rhs_k4d /
+ * characteristic_qs_k4d, src/boundaries/nonreflecting.f90; the reference
+ * loads it as the "kvector" field, io_helper.f90 get_fname "kvector"). */
+int cruna_hip_set_kvector(cruna_hip_ctx *ctx, const double *kvector);
 
 /* Deformed grid: X_metric(n1,n2,n3,3,3) and X_jacobian(n1,n2,n3) of data_geom
  * as grid_metric::calc_metric leaves them (src/geometry/grid_metric.f90:16-188);

@@ -719,10 +719,17 @@ void orc_bc_q(const orc_cfg *c, double *q, const double *qref, const double *sha
   const size_t N = npts(c);
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
-    if (t == 0 || t == 100 || t == 204 || t == 400) continue; /* RHS rows */
+    if (t == 0 || t == 100 || t == 204 || t == 205 || t == 400) continue; /* RHS rows */
     if (t == 201) {
       const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
       FOR_BOX(c, r, idx) orc_characteristic_q_k1d(1, (int)N, q + idx, qref + idx, k, c->gamma);
+    } else if (t == 202) {
+      /* characteristic_q_k4d(q(box), qref(box), kvector(box,:)), boundary_conditions.f90:89-91 */
+      const double *kv = c->kvector;
+      FOR_BOX(c, r, idx) {
+        const double k[3] = {kv[idx], kv[N + idx], kv[2 * N + idx]};
+        orc_characteristic_q_k1d(1, (int)N, q + idx, qref + idx, k, c->gamma);
+      }
     } else if (t == 200) {
       /* characteristic_q(q(box), q(box of inner neighbours), row(8:10)), boundary_conditions.f90:80-83 */
       const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
@@ -744,10 +751,17 @@ void orc_bc_qs(const orc_cfg *c, double *qs, const double *q, const double *shap
   const size_t N = npts(c);
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
-    if (t == 0 || t == 100 || t == 204 || t == 400) continue; /* RHS rows */
+    if (t == 0 || t == 100 || t == 204 || t == 205 || t == 400) continue; /* RHS rows */
     if (t == 201) {
       const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
       FOR_BOX(c, r, idx) orc_characteristic_qs_k1d(1, (int)N, qs + idx, q + idx, k, c->gamma);
+    } else if (t == 202) {
+      /* characteristic_qs(qs(box), q(box), kvector(box,:)), boundary_conditions.f90:176-179 */
+      const double *kv = c->kvector;
+      FOR_BOX(c, r, idx) {
+        const double k[3] = {kv[idx], kv[N + idx], kv[2 * N + idx]};
+        orc_characteristic_qs_k1d(1, (int)N, qs + idx, q + idx, k, c->gamma);
+      }
     } else if (t == 200) {
       /* characteristic_qs(qs(box), q(box of inner neighbours), row(8:10)), boundary_conditions.f90:167-170 */
       const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
@@ -769,7 +783,7 @@ void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qr
   const size_t N = npts(c);
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
-    if (t == 0 || t == 200 || t == 201 || t == 401) continue;
+    if (t == 0 || t == 200 || t == 201 || t == 202 || t == 401) continue;
     if (t == 400) {
       /* sponge_rhs with qref, sponge_layer_core.f90:34-66 */
       for (int v = 0; v < NV; ++v)
@@ -779,6 +793,13 @@ void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qr
       /* wall_slip_no_thermodynamics(rhs(box), nint(row(8))), wall_boundary_conditions.f90:13-23 */
       const int v = (int)lround(c->bc[r][7]) - 1;
       FOR_BOX(c, r, idx) rhs[v * N + idx] = 0.0;
+    } else if (t == 205) {
+      /* characteristic_rhs_k4d(rhs(box), q(box), qref(box), kvector(box,:)), boundary_conditions.f90:272-274 */
+      const double *kv = c->kvector;
+      FOR_BOX(c, r, idx) {
+        const double k[3] = {kv[idx], kv[N + idx], kv[2 * N + idx]};
+        orc_characteristic_rhs_k1d(1, (int)N, rhs + idx, q + idx, qref + idx, k, c->gamma, c->dt);
+      }
     } else if (t == 204) {
       /* characteristic_rhs_k1d(rhs(box), q(box), qref(box), row(8:10)) */
       const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
@@ -794,7 +815,8 @@ void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qr
 static void bc_rhss(const orc_cfg *c) {
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
-    if (t == 0 || t == 200 || t == 201 || t == 204 || t == 400 || t == 401) continue; /* 204: commented out */
+    if (t == 0 || t == 200 || t == 201 || t == 202 || t == 204 || t == 205 || t == 400 || t == 401)
+      continue; /* 204, 205: commented out */
     bc_unsupported(t, "set_boundary_condition_rhss");
   }
 }

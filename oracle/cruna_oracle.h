@@ -61,6 +61,8 @@ typedef struct orc_cfg {
   /* governing_equations: ORC_EQ_NS (ns_rho_rhou_p*) or ORC_EQ_EULER
    * (euler_rho_rhou_p, Cartesian) */
   int equations;
+  /* kvector(n1,n2,n3,3): boundary normals per point (rows 202, 205) */
+  const double *kvector;
 } orc_cfg;
 
 #define ORC_EQ_NS 0

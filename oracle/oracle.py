@@ -47,6 +47,7 @@ class OrcCfg(C.Structure):
         ("penal_inv_phi", C.c_void_p),
         ("wfilter_freq", C.c_int * 3),
         ("equations", C.c_int),
+        ("kvector", C.c_void_p),
     ]
 
 
@@ -136,7 +137,8 @@ class Cfg:
     def __init__(self, n, dx, scheme=(PERIODIC,) * 3, filter_alpha=0.35, gamma=1.4, R=1.0,
                  W=1.0, Pr=0.71, mu0=1.0 / 1600.0, T0=1.0, visc_law=SUTHERLAND, visc_exp=0.76,
                  dt=1e-3, filter_freq=(1, 1, 1), afilter_freq=(0, 0, 0), bc=(), metric=None,
-                 diff_kind=(0, 0, 0), penalization=None, wfilter_freq=(0, 0, 0), equations=0):
+                 diff_kind=(0, 0, 0), penalization=None, wfilter_freq=(0, 0, 0), equations=0,
+                 kvector=None):
         """metric: (X_metric (n1,n2,n3,3,3), X_jacobian (n1,n2,n3)) of a deformed
         grid; the direct RHS is then the grid-transform one.  diff_kind per
         direction: DIFF_O6_COMPACT, DIFF_O4_EXPLICIT or DIFF_O4_PADE.
@@ -166,6 +168,12 @@ class Cfg:
             self._metric = (Xm, Xj)  # kept alive for the pointers below
             c.X_metric = Xm.ctypes.data
             c.X_jacobian = Xj.ctypes.data
+        self._kvec = None
+        if kvector is not None:
+            kv = _f(kvector)
+            assert kv.shape == tuple(n) + (3,)
+            self._kvec = kv
+            c.kvector = kv.ctypes.data
         self._penal = None
         if penalization is not None:
             ph, ip = _f(penalization[0]), _f(penalization[1])

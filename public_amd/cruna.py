@@ -79,6 +79,7 @@ def lib():
         L.cruna_hip_get_state.argtypes = [vp, dp]
         L.cruna_hip_set_qref.argtypes = [vp, dp]
         L.cruna_hip_set_sponge_shape.argtypes = [vp, dp]
+        L.cruna_hip_set_kvector.argtypes = [vp, dp]
         L.cruna_hip_set_metric.argtypes = [vp, dp, dp]
         L.cruna_hip_set_penalization.argtypes = [vp, vp, vp]
         L.cruna_hip_filter_weighted.argtypes = [vp, ip, vp, ip, vp, vp]
@@ -127,7 +128,7 @@ def lib():
         L.cruna_hip_cnn_train.argtypes = [vp, ip, C.c_float, ip, ip, C.c_uint, fp]
         L.cruna_hip_cnn_num_samples.argtypes = [vp]
         L.cruna_hip_set_closure.argtypes = [vp, vp]
-        for f in ("create", "destroy", "set_state", "get_state", "set_qref", "set_sponge_shape", "diff",
+        for f in ("create", "destroy", "set_state", "get_state", "set_qref", "set_sponge_shape", "set_kvector", "diff",
                   "filter", "rhs_direct", "rhs_adjoint", "time_stepper_direct", "time_stepper_adjoint",
                   "run_direct", "synchronize", "profile_enable", "profile_read", "profile_count",
                   "set_adjoint_state", "get_adjoint_state", "trajectory_alloc", "trajectory_set",
@@ -317,6 +318,13 @@ class Cruna:
     def set_sponge_shape(self, shape):
         shape = _f64(shape, self.shape3)
         _check(lib().cruna_hip_set_sponge_shape(self._h, _dp(shape)))
+
+    def set_kvector(self, kvector):
+        """Boundary normals per point, kvector (n1,n2,n3,3), for rows 202 / 205."""
+        k = np.asfortranarray(kvector, dtype=np.float64)
+        if k.shape != self.shape3 + (3,):
+            raise CrunaError(f"kvector shape {k.shape} does not match {self.shape3 + (3,)}")
+        _check(lib().cruna_hip_set_kvector(self._h, _dp(k)))
 
     def set_metric(self, X_metric, X_jacobian):
         """Deformed grid: X_metric (n1,n2,n3,3,3) and X_jacobian (n1,n2,n3) of calc_metric."""

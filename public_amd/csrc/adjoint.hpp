@@ -168,7 +168,8 @@ template <typename Q>
 __global__ __launch_bounds__(256) void k_bc_characteristic_qs(Q *__restrict__ qs,
                                                                const double *const *__restrict__ q0p,
                                                                BoxRow row, int n1, int n2, long N,
-                                                               double gamma) {
+                                                               double gamma,
+                                                               const double *__restrict__ kfield) {
   // forward state at the point (201) or at its inner neighbour (200)
   const double *__restrict__ q = *q0p + row.aim_shift;
   const long nx = row.i1 - row.i0 + 1, ny = row.j1 - row.j0 + 1, nz = row.k1 - row.k0 + 1;
@@ -178,7 +179,9 @@ __global__ __launch_bounds__(256) void k_bc_characteristic_qs(Q *__restrict__ qs
   const long o = i + (long)n1 * (j + (long)n2 * k);
   const double s1 = double(qs[o]), s2 = double(qs[N + o]), s3 = double(qs[2 * N + o]),
                s4 = double(qs[3 * N + o]), s5 = double(qs[4 * N + o]);
-  const double kx = -row.k[0], ky = -row.k[1], kz = -row.k[2];
+  // the row's normal, or the point's (202: characteristic_qs_k4d)
+  const double kx = -(kfield ? kfield[o] : row.k[0]), ky = -(kfield ? kfield[N + o] : row.k[1]),
+               kz = -(kfield ? kfield[2 * N + o] : row.k[2]);
   const double gm1 = gamma - 1.0;
   const double c = sqrt(gamma * (q[4 * N + o] / q[o]));
   const double g_c = gm1 / c;

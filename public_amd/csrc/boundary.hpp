@@ -22,9 +22,12 @@ struct BoxRow {
 // q at the inner neighbour (characteristic_zero_grad_init, :665-728), a point
 // outside the one-point-thick box, so the in-place update reads no value it
 // writes.
+// kfield (types 202/205, characteristic_*_k4d): the normal per point,
+// kvector(n1,n2,n3,3), in place of the row's; null: the row's (201/204).
 template <typename Q, typename A>
 __global__ __launch_bounds__(256) void k_bc_characteristic_q(Q *q, const A *aim_base, BoxRow row,
-                                                              int n1, int n2, long N, double gamma) {
+                                                              int n1, int n2, long N, double gamma,
+                                                              const double *__restrict__ kfield) {
   const long nx = row.i1 - row.i0 + 1, ny = row.j1 - row.j0 + 1, nz = row.k1 - row.k0 + 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nx * ny * nz) return;
@@ -42,7 +45,8 @@ __global__ __launch_bounds__(256) void k_bc_characteristic_q(Q *q, const A *aim_
   double dq3 = double(aim[2 * N + o]) / double(aim[o]) - u2;
   double dq4 = double(aim[3 * N + o]) / double(aim[o]) - u3;
   double dq5 = double(aim[4 * N + o]) - p;
-  const double kx = -row.k[0], ky = -row.k[1], kz = -row.k[2];
+  const double kx = -(kfield ? kfield[o] : row.k[0]), ky = -(kfield ? kfield[N + o] : row.k[1]),
+               kz = -(kfield ? kfield[2 * N + o] : row.k[2]);
   // primitive -> characteristic waves (:97-101)
   double R1 = kx * dq1 + kz * dq3 - ky * dq4 - kx * dq5 / (c * c);
   double R2 = ky * dq1 - kz * dq2 + kx * dq4 - ky * dq5 / (c * c);
@@ -81,7 +85,8 @@ __global__ __launch_bounds__(256) void k_bc_characteristic_rhs(T *__restrict__ r
                                                                 const Q *__restrict__ q,
                                                                 const double *__restrict__ aim,
                                                                 BoxRow row, int n1, int n2, long N,
-                                                                double gamma, double dt) {
+                                                                double gamma, double dt,
+                                                                const double *__restrict__ kfield) {
   const long nx = row.i1 - row.i0 + 1, ny = row.j1 - row.j0 + 1, nz = row.k1 - row.k0 + 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nx * ny * nz) return;
@@ -105,7 +110,8 @@ __global__ __launch_bounds__(256) void k_bc_characteristic_rhs(T *__restrict__ r
   dq3 = dq3 + double(rhs[2 * N + o]) * tmp;
   dq4 = dq4 + double(rhs[3 * N + o]) * tmp;
   dq5 = dq5 + double(rhs[4 * N + o]) * dt;
-  const double kx = -row.k[0], ky = -row.k[1], kz = -row.k[2];
+  const double kx = -(kfield ? kfield[o] : row.k[0]), ky = -(kfield ? kfield[N + o] : row.k[1]),
+               kz = -(kfield ? kfield[2 * N + o] : row.k[2]);
   double R1 = kx * dq1 + kz * dq3 - ky * dq4 - kx * dq5 / (c * c);
   double R2 = ky * dq1 - kz * dq2 + kx * dq4 - ky * dq5 / (c * c);
   double R3 = kz * dq1 + ky * dq2 - kx * dq3 - kz * dq5 / (c * c);

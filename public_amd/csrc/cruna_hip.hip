@@ -107,6 +107,9 @@ struct cruna_hip_ctx {
   // rows that act on the complete RHS (100 wall, 204 characteristic): the
   // last sweep then writes the RHS, the rows follow, then the RK4 update
   bool has_bclate = false, has_wall = false;
+  // boundary normals per point, kvector(n1,n2,n3,3) (types 202, 205)
+  bool needs_kvec = false;
+  double *kvec = nullptr;
   // deformed grid (cruna_hip_set_metric): per-line metric entries and Jacobian
   void *met = nullptr;
   Metric<double> mt64{};
@@ -340,6 +343,8 @@ const T *jacobian(cruna_hip_ctx *c) {
 }
 
 int need_bc_inputs(cruna_hip_ctx *c) {
+  if (c->needs_kvec && !c->kvec)
+    return fail(CRUNA_HIP_EINVAL, "rows of types 202/205 need the normals per point (cruna_hip_set_kvector)");
   for (auto &r : c->bc) {
     if (!c->qref) return fail(CRUNA_HIP_EINVAL, "boundary rows need qref (cruna_hip_set_qref)");
     if ((r.first == 400 || r.first == 401) && !c->shape)
@@ -358,12 +363,19 @@ int bc_q(cruna_hip_ctx *c, Q *q) {
       const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
       hipLaunchKernelGGL((k_bc_characteristic_q<Q, double>), dim3((np + 255) / 256), dim3(256), 0,
                          c->stream, q, (const double *)c->qref, b, c->cfg.n[0], c->cfg.n[1], c->N,
-                         c->cfg.gamma);
+                         c->cfg.gamma, (const double *)nullptr);
+    } else if (r.first == 202) {
+      const BoxRow &b = r.second;
+      const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
+      hipLaunchKernelGGL((k_bc_characteristic_q<Q, double>), dim3((np + 255) / 256), dim3(256), 0,
+                         c->stream, q, (const double *)c->qref, b, c->cfg.n[0], c->cfg.n[1], c->N,
+                         c->cfg.gamma, (const double *)c->kvec);
     } else if (r.first == 200) {
       const BoxRow &b = r.second;
       const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
       hipLaunchKernelGGL((k_bc_characteristic_q<Q, Q>), dim3((np + 255) / 256), dim3(256), 0,
-                         c->stream, q, (const Q *)q, b, c->cfg.n[0], c->cfg.n[1], c->N, c->cfg.gamma);
+                         c->stream, q, (const Q *)q, b, c->cfg.n[0], c->cfg.n[1], c->N, c->cfg.gamma,
+                         (const double *)nullptr);
     } else if (r.first == 401) {
       hipLaunchKernelGGL((k_bc_sponge_q<Q>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream, q,
                          (const double *)c->qref, (const double *)c->shape, c->N);
@@ -388,10 +400,10 @@ int bc_rhs(cruna_hip_ctx *c, T *rhs, const Q *q, bool late = false) {
       hipLaunchKernelGGL((k_sponge_rhs<T, Q>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream,
                          rhs, q, (const double *)c->qref, (const double *)c->shape, c->N,
                          late ? 1.0 : c->cfg.gamma - 1.0, late ? nullptr : jacobian<T>(c));
-    } else if (late && r.first == 204) {
+    } else if (late && (r.first == 204 || r.first == 205)) {
       hipLaunchKernelGGL((k_bc_characteristic_rhs<T, Q>), dim3((np + 255) / 256), dim3(256), 0,
                          c->stream, rhs, q, (const double *)c->qref, b, n1, n2, c->N, c->cfg.gamma,
-                         c->cfg.dt);
+                         c->cfg.dt, r.first == 205 ? (const double *)c->kvec : (const double *)nullptr);
     } else if (late && r.first == 100) {
       hipLaunchKernelGGL((k_bc_wall_rhs<T>), dim3((np + 255) / 256), dim3(256), 0, c->stream, rhs, b,
                          (int)b.k[0], n1, n2, c->N);
@@ -730,12 +742,15 @@ int copy_rhs_out(cruna_hip_ctx *c, double *rhs) {
 template <typename Q>
 int bc_qs(cruna_hip_ctx *c, Q *qs) {
   for (auto &r : c->bc) {
-    if (r.first == 201 || r.first == 200) {
+    if (r.first == 201 || r.first == 200 || r.first == 202) {
       const BoxRow &b = r.second;
       const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
+      if (r.first == 202 && !c->kvec)
+        return fail(CRUNA_HIP_EINVAL, "rows of types 202/205 need the normals per point (cruna_hip_set_kvector)");
       hipLaunchKernelGGL((k_bc_characteristic_qs<Q>), dim3((np + 255) / 256), dim3(256), 0,
                          c->stream, qs, (const double *const *)c->q0p, b, c->cfg.n[0], c->cfg.n[1],
-                         c->N, c->cfg.gamma);
+                         c->N, c->cfg.gamma,
+                         r.first == 202 ? (const double *)c->kvec : (const double *)nullptr);
     } else if (r.first == 401) {
       if (!c->shape)
         return fail(CRUNA_HIP_EINVAL, "sponge rows need a shape (cruna_hip_set_sponge_shape)");
@@ -945,7 +960,8 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   if (cfg->nbc < 0 || cfg->nbc > CRUNA_HIP_MAX_BC) return fail(CRUNA_HIP_EINVAL, "bad nbc");
   for (int r = 0; r < cfg->nbc; ++r) {
     const int t = (int)lround(cfg->bc[r][0]);
-    if (t != 0 && t != 100 && t != 200 && t != 201 && t != 204 && t != 400 && t != 401)
+    if (t != 0 && t != 100 && t != 200 && t != 201 && t != 202 && t != 204 && t != 205 && t != 400 &&
+        t != 401)
       return fail(CRUNA_HIP_EINVAL,
                   "boundary type " + std::to_string(t) + " not implemented on the device");
   }
@@ -993,9 +1009,10 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
       b.k[0] = v - 1;
     }
     c->bc.push_back({t, b});
-    if (t == 200 || t == 201 || t == 401) c->has_bcq = true;
+    if (t == 200 || t == 201 || t == 202 || t == 401) c->has_bcq = true;
+    if (t == 202 || t == 205) c->needs_kvec = true;
     if (t == 400) c->has_bcrhs = true;
-    if (t == 100 || t == 204) c->has_bclate = true;
+    if (t == 100 || t == 204 || t == 205) c->has_bclate = true;
     if (t == 100) c->has_wall = true;
   }
   for (int d = 0; d < 3; ++d) {
@@ -1066,6 +1083,7 @@ int cruna_hip_destroy(cruna_hip_ctx *c) {
   for (void *p : {(void *)c->qn, (void *)c->acc, (void *)c->qref, (void *)c->shape, (void *)c->tmp,
                   (void *)c->qa, (void *)c->traj, (void *)c->gobj, c->met, (void *)c->q0p, c->force,
                   (void *)c->ckpt, c->tg, c->pen, (void *)c->wphi, (void *)c->wtmp, (void *)c->wbuf,
+                  (void *)c->kvec,
                   c->qs, c->rhs, c->G})
     if (p) (void)hipFree(p);
   for (auto &p : c->pending) {
@@ -1098,6 +1116,16 @@ int cruna_hip_set_qref(cruna_hip_ctx *c, const double *qref) {
   ++c->gen;
   if (!c->qref) HIP_TRY(hipMalloc(&c->qref, 5 * c->N * sizeof(double)));
   HIP_TRY(hipMemcpy(c->qref, qref, 5 * c->N * sizeof(double), hipMemcpyHostToDevice));
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_set_kvector(cruna_hip_ctx *c, const double *kvector) {
+  if (!c || !kvector) return fail(CRUNA_HIP_EINVAL, "null argument");
+  ++c->gen;
+  for (long i = 0; i < 3 * c->N; ++i)
+    if (!std::isfinite(kvector[i])) return fail(CRUNA_HIP_EINVAL, "kvector is not finite");
+  if (!c->kvec) HIP_TRY(hipMalloc(&c->kvec, 3 * c->N * sizeof(double)));
+  HIP_TRY(hipMemcpy(c->kvec, kvector, 3 * c->N * sizeof(double), hipMemcpyHostToDevice));
   return CRUNA_HIP_OK;
 }
 

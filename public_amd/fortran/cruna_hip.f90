@@ -82,6 +82,11 @@ module cruna_hip
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: s(*)
     end function
+    integer(c_int) function c_set_kvector(ctx, k) bind(C, name="cruna_hip_set_kvector")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: k(*)
+    end function
     integer(c_int) function c_diff(ctx, dir, q_x, q, h) bind(C, name="cruna_hip_diff")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
@@ -288,7 +293,7 @@ module cruna_hip
 
   public :: cruna_hip_default_cfg
   public :: hip_create, hip_destroy, hip_set_state, hip_get_state, hip_set_qref
-  public :: hip_set_sponge_shape, hip_diff, hip_filter, hip_filter_weighted, hip_rhs_direct
+  public :: hip_set_sponge_shape, hip_set_kvector, hip_diff, hip_filter, hip_filter_weighted, hip_rhs_direct
   public :: hip_time_stepper_direct, hip_run_direct, hip_synchronize
   public :: hip_rhs_adjoint, hip_time_stepper_adjoint, hip_set_adjoint_state
   public :: hip_get_adjoint_state, hip_trajectory_alloc, hip_trajectory_set
@@ -352,6 +357,13 @@ contains
     type(c_ptr), intent(in) :: ctx
     real(c_double), intent(in), contiguous :: s(:,:,:)
     call check(c_set_sponge_shape(ctx, s), "set_sponge_shape")
+  end subroutine
+
+  ! kvector(n1,n2,n3,3): boundary normals per point for rows 202 / 205
+  subroutine hip_set_kvector(ctx, k)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(in), contiguous :: k(:,:,:,:)
+    call check(c_set_kvector(ctx, k), "set_kvector")
   end subroutine
 
   ! diff_3d(q_x, q, h) along direction dir = 1, 2, 3 (h optional, default 1)

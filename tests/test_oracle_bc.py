@@ -140,3 +140,42 @@ def test_zero_gradient_row_is_the_characteristic_row_toward_the_inner_neighbour(
     got_s = O.bc_qs(O.Cfg(n, (0.1,) * 3, bc=[(200,) + row]), qs, q)
     ref_s = O.bc_qs(O.Cfg(n, (0.1,) * 3, bc=[(201,) + row]), qs, shifted)
     assert np.array_equal(got_s, ref_s)
+
+
+def test_per_point_normal_rows_match_their_row_normal_forms():
+    """Types 202 / 205 (characteristic_*_k4d with kvector(box,:)) equal 201 /
+    204 applied point by point with each point's normal as the row normal;
+    with a constant field equal to the row's normal they equal 201 / 204."""
+    n = (16, 16, 16)
+    rng = np.random.default_rng(9)
+    q = np.zeros(n + (5,), order="F")
+    q[..., 0] = 1 + 0.05 * rng.random(n)
+    q[..., 1:4] = 0.2 * rng.standard_normal(n + (3,))
+    q[..., 4] = 1 / 1.4 * (1 + 0.05 * rng.random(n))
+    qref = q.copy(order="F")
+    qref[..., 0] *= 1.02
+    kv = np.zeros(n + (3,), order="F")
+    kv[..., 1] = -1.0
+    kv[..., 0] = 0.3 * np.sin(np.arange(16))[:, None, None]
+    kv /= np.linalg.norm(kv, axis=-1, keepdims=True)
+    box = (1, 16, 1, 1, 1, 16)
+    got = O.bc_q(O.Cfg(n, (0.1,) * 3, bc=[(202,) + box], kvector=kv), q, qref=qref)
+    ref = q.copy(order="F")
+    for i in range(16):
+        for kk in range(16):
+            row = (201, i + 1, i + 1, 1, 1, kk + 1, kk + 1) + tuple(kv[i, 0, kk])
+            ref = O.bc_q(O.Cfg(n, (0.1,) * 3, bc=[row]), ref, qref=qref)
+    assert np.array_equal(got, ref)
+    const = np.zeros(n + (3,), order="F")
+    const[..., 1] = -1.0
+    a = O.bc_q(O.Cfg(n, (0.1,) * 3, bc=[(202,) + box], kvector=const), q, qref=qref)
+    b = O.bc_q(O.Cfg(n, (0.1,) * 3, bc=[(201,) + box + (0, -1, 0)]), q, qref=qref)
+    assert np.array_equal(a, b)
+    rhs = np.asfortranarray(rng.standard_normal(n + (5,)))
+    a = O.bc_rhs(O.Cfg(n, (0.1,) * 3, dt=DT, bc=[(205,) + box], kvector=const), rhs, q, qref=qref)
+    b = O.bc_rhs(O.Cfg(n, (0.1,) * 3, dt=DT, bc=[(204,) + box + (0, -1, 0)]), rhs, q, qref=qref)
+    assert np.array_equal(a, b)
+    qs = np.asfortranarray(rng.standard_normal(n + (5,)))
+    a = O.bc_qs(O.Cfg(n, (0.1,) * 3, bc=[(202,) + box], kvector=const), qs, q)
+    b = O.bc_qs(O.Cfg(n, (0.1,) * 3, bc=[(201,) + box + (0, -1, 0)]), qs, q)
+    assert np.array_equal(a, b)
