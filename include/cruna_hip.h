@@ -71,8 +71,12 @@ typedef struct cruna_hip_cfg {
      Types 0, 100 (slip wall: column 8 names the variable whose RHS is zero),
      200 (characteristic toward the inner neighbour; box one point thick along
      the normal), 201, 204 (characteristic on q / on the RHS), 202, 205 (the
-     same with the normals per point, cruna_hip_set_kvector), 400, 401
-     (sponge on the RHS / on q) */
+     same with the normals per point, cruna_hip_set_kvector), 203, 206 (zonal:
+     the same over the whole domain, weighted by the sponge shape; box columns
+     unused; a 203 row's column 8 is -1 / 1 / 0 for acting only after the RK
+     step / only inside the RHS / in both, the last 203 row deciding), 400,
+     401 (sponge on the RHS / on q).  The adjoint refuses rows 204-206 as the
+     reference's set_boundary_condition_rhss stops on them */
   int nbc;
   double bc[CRUNA_HIP_MAX_BC][10];
   int diff_kind[3];      /* derivative core per direction (discretisation_x_core):
@@ -113,7 +117,7 @@ int cruna_hip_set_qref(cruna_hip_ctx *ctx, const double *qref);
 int cruna_hip_set_sponge_shape(cruna_hip_ctx *ctx, const double *shape);
 
 /* kvector(n1,n2,n3,3): the outward boundary normal per point that rows of
- * types 202 and 205 use (characteristic_q_k4d / characteristic_rhs_k4d /
+ * types 202, 203, 205 and 206 use (characteristic_q_k4d / characteristic_rhs_k4d /
  * characteristic_qs_k4d, src/boundaries/nonreflecting.f90; the reference
  * loads it as the "kvector" field, io_helper.f90 get_fname "kvector"). */
 int cruna_hip_set_kvector(cruna_hip_ctx *ctx, const double *kvector);

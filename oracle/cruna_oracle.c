@@ -558,8 +558,10 @@ static int bc_type(const orc_cfg *c, int r) { return (int)lround(c->bc[r][0]); }
 
 /* src/boundaries/nonreflecting.f90:46-137 on one point (stride_v = distance
  * between variables).  npts points starting at q, contiguous. */
-void orc_characteristic_q_k1d(int np, int sv, double *q, const double *aim, const double k[3],
-                              double gamma) {
+/* sc scales the change toward aim before the decomposition: the shape of
+ * characteristic_q_k4d (nonreflecting.f90:180-185, type 203), 1 otherwise. */
+static void char_q(int np, int sv, double *q, const double *aim, const double k[3], double gamma,
+                   double sc) {
   for (int m = 0; m < np; ++m) {
     double *Q = q + m;
     const double *A = aim + m;
@@ -572,6 +574,11 @@ void orc_characteristic_q_k1d(int np, int sv, double *q, const double *aim, cons
     double dq3 = A[2 * sv] / A[0] - u2;
     double dq4 = A[3 * sv] / A[0] - u3;
     double dq5 = A[4 * sv] - p;
+    dq1 *= sc;
+    dq2 *= sc;
+    dq3 *= sc;
+    dq4 *= sc;
+    dq5 *= sc;
     const double kx = -k[0], ky = -k[1], kz = -k[2];
     double R1 = kx * dq1 + kz * dq3 - ky * dq4 - kx * dq5 / (c * c);
     double R2 = ky * dq1 - kz * dq2 + kx * dq4 - ky * dq5 / (c * c);
@@ -601,13 +608,20 @@ void orc_characteristic_q_k1d(int np, int sv, double *q, const double *aim, cons
   }
 }
 
-/* src/boundaries/nonreflecting.f90:453-554 (adjoint, target zero). */
-void orc_characteristic_qs_k1d(int np, int sv, double *qs, const double *q, const double k[3],
-                               double gamma) {
+void orc_characteristic_q_k1d(int np, int sv, double *q, const double *aim, const double k[3],
+                              double gamma) {
+  char_q(np, sv, q, aim, k, gamma, 1.0);
+}
+
+/* src/boundaries/nonreflecting.f90:453-554 (adjoint, target zero); sc: the
+ * shape of characteristic_qs_k4d (:596-601, type 203), 1 otherwise. */
+static void char_qs(int np, int sv, double *qs, const double *q, const double k[3], double gamma,
+                    double sc) {
   for (int m = 0; m < np; ++m) {
     double *S = qs + m;
     const double *Q = q + m;
-    double ds1 = -S[0], ds2 = -S[sv], ds3 = -S[2 * sv], ds4 = -S[3 * sv], ds5 = -S[4 * sv];
+    double ds1 = -S[0] * sc, ds2 = -S[sv] * sc, ds3 = -S[2 * sv] * sc, ds4 = -S[3 * sv] * sc,
+           ds5 = -S[4 * sv] * sc;
     const double kx = -k[0], ky = -k[1], kz = -k[2];
     const double gm1 = gamma - 1.0;
     double c = sqrt(gamma * (Q[4 * sv] / Q[0]));
@@ -650,6 +664,11 @@ void orc_characteristic_qs_k1d(int np, int sv, double *qs, const double *q, cons
   }
 }
 
+void orc_characteristic_qs_k1d(int np, int sv, double *qs, const double *q, const double k[3],
+                               double gamma) {
+  char_qs(np, sv, qs, q, k, gamma, 1.0);
+}
+
 static void bc_unsupported(int type, const char *where) {
   fprintf(stderr, "oracle: boundary type %d not implemented in %s\n", type, where);
   abort();
@@ -660,8 +679,10 @@ static void bc_unsupported(int type, const char *where) {
 /* characteristic_rhs_k1d, src/boundaries/nonreflecting.f90:237-341 (no
  * shape): dq = prim(aim) - prim(q) + (rhs dt, rhs_m dt / rho, rhs_5 dt), keep
  * the incoming waves, rhs <- dq / dt (momentum: times the updated rho). */
-void orc_characteristic_rhs_k1d(int np, int sv, double *rhs, const double *q, const double *aim,
-                                const double k[3], double gamma, double dt) {
+/* sh: the shape of characteristic_rhs_k4d (nonreflecting.f90:431-439, type
+ * 206), which blends (1 - shape) rhs + shape (new rhs); null: no shape. */
+static void char_rhs(int np, int sv, double *rhs, const double *q, const double *aim,
+                     const double k[3], double gamma, double dt, const double *sh) {
   const double inv_dt = 1.0 / dt;
   for (int m = 0; m < np; ++m) {
     double *F = rhs + m;
@@ -694,11 +715,35 @@ void orc_characteristic_rhs_k1d(int np, int sv, double *rhs, const double *q, co
     dq[3] = -ky * R1 + kx * R2 + 0.5 * kz * R4 - 0.5 * kz * R5;
     dq[4] = +0.5 * r * c * R4 + 0.5 * r * c * R5;
     r = r + dq[0]; /* :321 */
+    if (sh) {
+      const double s = sh[m];
+      tmp = s * inv_dt;
+      F[0] = (1.0 - s) * F[0] + dq[0] * tmp;
+      F[4 * sv] = (1.0 - s) * F[4 * sv] + dq[4] * tmp;
+      tmp = tmp * r;
+      for (int d = 0; d < 3; ++d) F[(1 + d) * sv] = (1.0 - s) * F[(1 + d) * sv] + dq[1 + d] * tmp;
+      continue;
+    }
     F[0] = dq[0] * inv_dt; /* :333-339 */
     tmp = inv_dt * r;
     for (int d = 0; d < 3; ++d) F[(1 + d) * sv] = dq[1 + d] * tmp;
     F[4 * sv] = dq[4] * inv_dt;
   }
+}
+
+void orc_characteristic_rhs_k1d(int np, int sv, double *rhs, const double *q, const double *aim,
+                                const double k[3], double gamma, double dt) {
+  char_rhs(np, sv, rhs, q, aim, k, gamma, dt, NULL);
+}
+
+/* init_boundary_conditions (boundary_conditions.f90:434-438): the last type
+ * 203 row's column 8 picks where its rows act: -1 only in the call after the
+ * RK step (outside_rhs), 1 only in the RHS's own call, 0 both. */
+static int zone_applies(const orc_cfg *c, int outside) {
+  int flag = 0;
+  for (int r = 0; r < c->nbc; ++r)
+    if ((int)lround(c->bc[r][0]) == 203) flag = (int)lround(c->bc[r][7]);
+  return outside ? flag != 1 : flag != -1;
 }
 
 /* characteristic_zero_grad_init, src/boundaries/nonreflecting.f90:665-728:
@@ -715,11 +760,24 @@ static long zero_grad_shift(const orc_cfg *c, int r) {
   return s[0] + (long)c->n[0] * (s[1] + (long)c->n[1] * s[2]);
 }
 
-void orc_bc_q(const orc_cfg *c, double *q, const double *qref, const double *shape) {
+/* outside: the call after the RK step (set_boundary_condition_q(qn,
+ * outside_rhs=.true.)), which type 203 rows tell apart. */
+static void bc_q_at(const orc_cfg *c, double *q, const double *qref, const double *shape,
+                    int outside) {
   const size_t N = npts(c);
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
-    if (t == 0 || t == 100 || t == 204 || t == 205 || t == 400) continue; /* RHS rows */
+    if (t == 0 || t == 100 || t == 204 || t == 205 || t == 206 || t == 400) continue; /* RHS rows */
+    if (t == 203) {
+      /* characteristic_q_k4d(q, qref, kvector, shape) on the whole domain, :93-108 */
+      if (!zone_applies(c, outside)) continue;
+      const double *kv = c->kvector;
+      for (size_t i = 0; i < N; ++i) {
+        const double k[3] = {kv[i], kv[N + i], kv[2 * N + i]};
+        char_q(1, (int)N, q + i, qref + i, k, c->gamma, shape[i]);
+      }
+      continue;
+    }
     if (t == 201) {
       const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
       FOR_BOX(c, r, idx) orc_characteristic_q_k1d(1, (int)N, q + idx, qref + idx, k, c->gamma);
@@ -746,12 +804,27 @@ void orc_bc_q(const orc_cfg *c, double *q, const double *qref, const double *sha
   }
 }
 
+void orc_bc_q(const orc_cfg *c, double *q, const double *qref, const double *shape) {
+  bc_q_at(c, q, qref, shape, 0);
+}
+
 /* set_boundary_condition_qs, boundary_conditions.f90:140-224 (types 0, 201, 400, 401). */
-void orc_bc_qs(const orc_cfg *c, double *qs, const double *q, const double *shape) {
+static void bc_qs_at(const orc_cfg *c, double *qs, const double *q, const double *shape,
+                     int outside) {
   const size_t N = npts(c);
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
-    if (t == 0 || t == 100 || t == 204 || t == 205 || t == 400) continue; /* RHS rows */
+    if (t == 0 || t == 100 || t == 204 || t == 205 || t == 206 || t == 400) continue; /* RHS rows */
+    if (t == 203) {
+      /* characteristic_qs_k4d(qs, q, kvector, shape) on the whole domain, :181-194 */
+      if (!zone_applies(c, outside)) continue;
+      const double *kv = c->kvector;
+      for (size_t i = 0; i < N; ++i) {
+        const double k[3] = {kv[i], kv[N + i], kv[2 * N + i]};
+        char_qs(1, (int)N, qs + i, q + i, k, c->gamma, shape[i]);
+      }
+      continue;
+    }
     if (t == 201) {
       const double k[3] = {c->bc[r][7], c->bc[r][8], c->bc[r][9]};
       FOR_BOX(c, r, idx) orc_characteristic_qs_k1d(1, (int)N, qs + idx, q + idx, k, c->gamma);
@@ -777,13 +850,26 @@ void orc_bc_qs(const orc_cfg *c, double *qs, const double *q, const double *shap
   }
 }
 
+void orc_bc_qs(const orc_cfg *c, double *qs, const double *q, const double *shape) {
+  bc_qs_at(c, qs, q, shape, 0);
+}
+
 /* set_boundary_condition_rhs, boundary_conditions.f90:228-298 (types 0, 100, 201, 204, 400, 401). */
 void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qref,
                 const double *shape) {
   const size_t N = npts(c);
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
-    if (t == 0 || t == 200 || t == 201 || t == 202 || t == 401) continue;
+    if (t == 0 || t == 200 || t == 201 || t == 202 || t == 203 || t == 401) continue;
+    if (t == 206) {
+      /* characteristic_rhs_k4d(rhs, q, qref, kvector, shape) on the whole domain, :276-279 */
+      const double *kv = c->kvector;
+      for (size_t i = 0; i < N; ++i) {
+        const double k[3] = {kv[i], kv[N + i], kv[2 * N + i]};
+        char_rhs(1, (int)N, rhs + i, q + i, qref + i, k, c->gamma, c->dt, shape + i);
+      }
+      continue;
+    }
     if (t == 400) {
       /* sponge_rhs with qref, sponge_layer_core.f90:34-66 */
       for (int v = 0; v < NV; ++v)
@@ -815,8 +901,9 @@ void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qr
 static void bc_rhss(const orc_cfg *c) {
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
-    if (t == 0 || t == 200 || t == 201 || t == 202 || t == 204 || t == 205 || t == 400 || t == 401)
-      continue; /* 204, 205: commented out */
+    /* the cases 204-206 are commented out there (:339-353), so those rows
+     * reach case default, which stops the run */
+    if (t == 0 || t == 200 || t == 201 || t == 202 || t == 203 || t == 400 || t == 401) continue;
     bc_unsupported(t, "set_boundary_condition_rhss");
   }
 }
@@ -1530,7 +1617,7 @@ void orc_run_direct(const orc_cfg *c, double *q, int nt0, int nsteps, const doub
   for (int nt = nt0 + 1; nt <= nt0 + nsteps; ++nt) {
     orc_time_stepper_direct(c, q1, q, qref, shape, heat_unit_spacing);
     memcpy(q, q1, M * sizeof(double));
-    orc_bc_q(c, q, qref, shape);
+    bc_q_at(c, q, qref, shape, 1);
     for (int d = 0; d < 3; ++d)
       if (c->filter_freq[d] > 0 && nt % c->filter_freq[d] == 0) orc_filter4d(c, d, q, 5);
     for (int d = 0; d < 3; ++d)
@@ -1549,7 +1636,7 @@ void orc_run_adjoint(const orc_cfg *c, double *qs, const double *Q, int nsteps,
     const double *q0 = Q + (size_t)(nt - 1) * M;
     orc_time_stepper_adjoint(c, qs1, qs, q0, NULL, shape);
     memcpy(qs, qs1, M * sizeof(double));
-    orc_bc_qs(c, qs, q0, shape);
+    bc_qs_at(c, qs, q0, shape, 1);
     for (int d = 0; d < 3; ++d)
       if (c->afilter_freq[d] > 0 && nt % c->afilter_freq[d] == 0) orc_filter4d(c, d, qs, 5);
   }

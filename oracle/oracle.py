@@ -187,6 +187,15 @@ class Cfg:
     def shape(self):
         return tuple(self.c.n)
 
+    def check_adjoint_rows(self):
+        """set_boundary_condition_rhss (boundary_conditions.f90:302-377) has
+        its cases 204-206 commented out: such rows reach case default, which
+        stops the run.  Raised here instead of the C oracle's abort."""
+        for r in range(self.c.nbc):
+            t = int(round(self.c.bc[r][0]))
+            if t in (204, 205, 206):
+                raise ValueError(f"boundary type {t} stops set_boundary_condition_rhss")
+
     def ref(self):
         return C.byref(self.c)
 
@@ -236,6 +245,7 @@ def rhs_direct(cfg, q, qref=None, shape=None, heat_unit_spacing=1):
 
 
 def rhs_adjoint(cfg, qs, q, g=None, shape=None):
+    cfg.check_adjoint_rows()
     qs, q = _f(qs), _f(q)
     rhs = np.zeros_like(q, order="F")
     g = None if g is None else _f(g)
@@ -290,6 +300,7 @@ def time_stepper_direct(cfg, q0, qref=None, shape=None, heat_unit_spacing=1):
 
 
 def time_stepper_adjoint(cfg, qs0, q0, g=None, shape=None):
+    cfg.check_adjoint_rows()
     qs0, q0 = _f(qs0), _f(q0)
     qs1 = np.zeros_like(q0, order="F")
     g = None if g is None else _f(g)
@@ -308,6 +319,7 @@ def run_direct(cfg, q, nsteps, nt0=0, qref=None, shape=None, heat_unit_spacing=1
 
 def run_adjoint(cfg, qs, Q, shape=None):
     """Q: direct trajectory, shape (n1, n2, n3, 5, nsteps); Q[..., nt-1] is the state after step nt."""
+    cfg.check_adjoint_rows()
     qs = _f(qs).copy(order="F")
     Q = _f(Q)
     shape = None if shape is None else _f(shape)

@@ -320,7 +320,7 @@ class Cruna:
         _check(lib().cruna_hip_set_sponge_shape(self._h, _dp(shape)))
 
     def set_kvector(self, kvector):
-        """Boundary normals per point, kvector (n1,n2,n3,3), for rows 202 / 205."""
+        """Boundary normals per point, kvector (n1,n2,n3,3), for rows 202 / 203 / 205 / 206."""
         k = np.asfortranarray(kvector, dtype=np.float64)
         if k.shape != self.shape3 + (3,):
             raise CrunaError(f"kvector shape {k.shape} does not match {self.shape3 + (3,)}")

@@ -164,12 +164,15 @@ __global__ __launch_bounds__(NT) void k_asweep(const Q *qs, const double *const 
 
 // characteristic_qs_k1d (src/boundaries/nonreflecting.f90:453-554) on the
 // points of one box: removes the incoming adjoint characteristics of qs.
+// shape (type 203, characteristic_qs_k4d, :596-601): the change toward zero
+// is scaled by shape at the point.
 template <typename Q>
 __global__ __launch_bounds__(256) void k_bc_characteristic_qs(Q *__restrict__ qs,
                                                                const double *const *__restrict__ q0p,
                                                                BoxRow row, int n1, int n2, long N,
                                                                double gamma,
-                                                               const double *__restrict__ kfield) {
+                                                               const double *__restrict__ kfield,
+                                                               const double *__restrict__ shape) {
   // forward state at the point (201) or at its inner neighbour (200)
   const double *__restrict__ q = *q0p + row.aim_shift;
   const long nx = row.i1 - row.i0 + 1, ny = row.j1 - row.j0 + 1, nz = row.k1 - row.k0 + 1;
@@ -187,7 +190,8 @@ __global__ __launch_bounds__(256) void k_bc_characteristic_qs(Q *__restrict__ qs
   const double g_c = gm1 / c;
   const double u = q[N + o] / q[o], v = q[2 * N + o] / q[o], w = q[3 * N + o] / q[o];
   // increments toward zero adjoint in the incoming waves
-  const double d1 = -s1, d2 = -s2, d3 = -s3, d4 = -s4, d5 = -s5;
+  const double sh = shape ? shape[o] : 1.0;
+  const double d1 = -s1 * sh, d2 = -s2 * sh, d3 = -s3 * sh, d4 = -s4 * sh, d5 = -s5 * sh;
   const double ku = kx * u + ky * v + kz * w;
   const double kus = kx * d2 + ky * d3 + kz * d4;
   const double ruus = (d1 + u * d2 + v * d3 + w * d4) / c;

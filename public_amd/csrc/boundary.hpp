@@ -1,7 +1,7 @@
 // Boundary conditions of src/boundaries/boundary_conditions.f90 used by the
-// hot path: type 201 (characteristic, nonreflecting toward qref), 204 (the
-// same on the RHS), 100 (slip wall on the RHS), 400 (sponge on the RHS) and
-// 401 (sponge on q).  Point-wise kernels, one thread per
+// hot path: types 200-203 (characteristic, nonreflecting toward qref or the
+// inner neighbour), 204-206 (the same on the RHS), 100 (slip wall on the
+// RHS), 400 (sponge on the RHS) and 401 (sponge on q).  Point-wise kernels, one thread per
 // point; the characteristic decomposition always runs in FP64, also in the
 // FP32-mixed mode.
 #pragma once
@@ -24,10 +24,13 @@ struct BoxRow {
 // writes.
 // kfield (types 202/205, characteristic_*_k4d): the normal per point,
 // kvector(n1,n2,n3,3), in place of the row's; null: the row's (201/204).
+// shape (type 203, characteristic_q_k4d with shape, :180-185): the change
+// toward aim is scaled by shape at the point before the decomposition.
 template <typename Q, typename A>
 __global__ __launch_bounds__(256) void k_bc_characteristic_q(Q *q, const A *aim_base, BoxRow row,
                                                               int n1, int n2, long N, double gamma,
-                                                              const double *__restrict__ kfield) {
+                                                              const double *__restrict__ kfield,
+                                                              const double *__restrict__ shape) {
   const long nx = row.i1 - row.i0 + 1, ny = row.j1 - row.j0 + 1, nz = row.k1 - row.k0 + 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nx * ny * nz) return;
@@ -45,6 +48,14 @@ __global__ __launch_bounds__(256) void k_bc_characteristic_q(Q *q, const A *aim_
   double dq3 = double(aim[2 * N + o]) / double(aim[o]) - u2;
   double dq4 = double(aim[3 * N + o]) / double(aim[o]) - u3;
   double dq5 = double(aim[4 * N + o]) - p;
+  if (shape) {
+    const double sh = shape[o];
+    dq1 *= sh;
+    dq2 *= sh;
+    dq3 *= sh;
+    dq4 *= sh;
+    dq5 *= sh;
+  }
   const double kx = -(kfield ? kfield[o] : row.k[0]), ky = -(kfield ? kfield[N + o] : row.k[1]),
                kz = -(kfield ? kfield[2 * N + o] : row.k[2]);
   // primitive -> characteristic waves (:97-101)
@@ -79,14 +90,17 @@ __global__ __launch_bounds__(256) void k_bc_characteristic_q(Q *q, const A *aim_
 // characteristic_rhs_k1d (src/boundaries/nonreflecting.f90:237-341, type 204)
 // on the complete RHS of the points of one box: the change that the RHS
 // makes over dt plus the change toward aim, in primitive variables, keeps
-// only its incoming waves and becomes the RHS again.
+// only its incoming waves and becomes the RHS again.  With shape (type 206,
+// characteristic_rhs_k4d, :431-439) the result is blended with the RHS as
+// (1 - shape) rhs + shape (new rhs).
 template <typename T, typename Q>
 __global__ __launch_bounds__(256) void k_bc_characteristic_rhs(T *__restrict__ rhs,
                                                                 const Q *__restrict__ q,
                                                                 const double *__restrict__ aim,
                                                                 BoxRow row, int n1, int n2, long N,
                                                                 double gamma, double dt,
-                                                                const double *__restrict__ kfield) {
+                                                                const double *__restrict__ kfield,
+                                                                const double *__restrict__ shape) {
   const long nx = row.i1 - row.i0 + 1, ny = row.j1 - row.j0 + 1, nz = row.k1 - row.k0 + 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nx * ny * nz) return;
@@ -105,11 +119,13 @@ __global__ __launch_bounds__(256) void k_bc_characteristic_rhs(T *__restrict__ r
   double dq4 = aim[3 * N + o] / aim[o] - u3;
   double dq5 = aim[4 * N + o] - double(q[4 * N + o]);
   double tmp = dt / r;
-  dq1 = dq1 + double(rhs[o]) * dt;
-  dq2 = dq2 + double(rhs[N + o]) * tmp;
-  dq3 = dq3 + double(rhs[2 * N + o]) * tmp;
-  dq4 = dq4 + double(rhs[3 * N + o]) * tmp;
-  dq5 = dq5 + double(rhs[4 * N + o]) * dt;
+  const double f1 = double(rhs[o]), f2 = double(rhs[N + o]), f3 = double(rhs[2 * N + o]),
+               f4 = double(rhs[3 * N + o]), f5 = double(rhs[4 * N + o]);
+  dq1 = dq1 + f1 * dt;
+  dq2 = dq2 + f2 * tmp;
+  dq3 = dq3 + f3 * tmp;
+  dq4 = dq4 + f4 * tmp;
+  dq5 = dq5 + f5 * dt;
   const double kx = -(kfield ? kfield[o] : row.k[0]), ky = -(kfield ? kfield[N + o] : row.k[1]),
                kz = -(kfield ? kfield[2 * N + o] : row.k[2]);
   double R1 = kx * dq1 + kz * dq3 - ky * dq4 - kx * dq5 / (c * c);
@@ -126,8 +142,19 @@ __global__ __launch_bounds__(256) void k_bc_characteristic_rhs(T *__restrict__ r
   dq3 = kz * R1 - kx * R3 + 0.5 * ky * R4 - 0.5 * ky * R5;
   dq4 = -ky * R1 + kx * R2 + 0.5 * kz * R4 - 0.5 * kz * R5;
   dq5 = +0.5 * r * c * R4 + 0.5 * r * c * R5;
-  // back to the RHS (:321-339, no shape)
   r = r + dq1;
+  if (shape) {  // :431-439
+    const double sh = shape[o], keep = 1.0 - sh;
+    tmp = sh * inv_dt;
+    rhs[o] = T(keep * f1 + dq1 * tmp);
+    rhs[4 * N + o] = T(keep * f5 + dq5 * tmp);
+    tmp = tmp * r;
+    rhs[N + o] = T(keep * f2 + dq2 * tmp);
+    rhs[2 * N + o] = T(keep * f3 + dq3 * tmp);
+    rhs[3 * N + o] = T(keep * f4 + dq4 * tmp);
+    return;
+  }
+  // back to the RHS (:321-339, no shape)
   rhs[o] = T(dq1 * inv_dt);
   tmp = inv_dt * r;
   rhs[N + o] = T(dq2 * tmp);

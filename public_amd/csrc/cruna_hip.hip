@@ -107,8 +107,16 @@ struct cruna_hip_ctx {
   // rows that act on the complete RHS (100 wall, 204 characteristic): the
   // last sweep then writes the RHS, the rows follow, then the RK4 update
   bool has_bclate = false, has_wall = false;
-  // boundary normals per point, kvector(n1,n2,n3,3) (types 202, 205)
+  // boundary normals per point, kvector(n1,n2,n3,3) (types 202, 203, 205, 206)
   bool needs_kvec = false;
+  // zonal rows (203, 206) weight by the sponge shape over the whole domain
+  bool needs_shape = false;
+  // column 8 of the last 203 row (init_boundary_conditions,
+  // boundary_conditions.f90:434-438): -1 acts only in the call after the RK
+  // step, 1 only in the RHS's own call, 0 in both
+  int zone_flag = 0;
+  // rows 204-206 stop the reference's adjoint (set_boundary_condition_rhss)
+  bool has_char_rhs = false;
   double *kvec = nullptr;
   // deformed grid (cruna_hip_set_metric): per-line metric entries and Jacobian
   void *met = nullptr;
@@ -332,6 +340,10 @@ const char *kEulerCartesian =
 // adjoint applies sponge_rhss on 400 rows, which the device adjoint does not
 const char *kNoEulerSpongeAdjoint =
     "the Euler adjoint with sponge rows (400) is not implemented on the device";
+// set_boundary_condition_rhss (boundary_conditions.f90:339-353) has its
+// cases 204-206 commented out, so the adjoint reaches case default and stops
+const char *kNoCharRhsAdjoint =
+    "rows of types 204-206 stop the reference's adjoint (set_boundary_condition_rhss has no case for them)";
 const char *kNoEvAdjoint =
     "the adjoint of ns_rho_rhou_p_ev_phi is not implemented (the reference stops there too)";
 
@@ -344,7 +356,9 @@ const T *jacobian(cruna_hip_ctx *c) {
 
 int need_bc_inputs(cruna_hip_ctx *c) {
   if (c->needs_kvec && !c->kvec)
-    return fail(CRUNA_HIP_EINVAL, "rows of types 202/205 need the normals per point (cruna_hip_set_kvector)");
+    return fail(CRUNA_HIP_EINVAL, "rows of types 202/203/205/206 need the normals per point (cruna_hip_set_kvector)");
+  if (c->needs_shape && !c->shape)
+    return fail(CRUNA_HIP_EINVAL, "zonal rows (203/206) need a shape (cruna_hip_set_sponge_shape)");
   for (auto &r : c->bc) {
     if (!c->qref) return fail(CRUNA_HIP_EINVAL, "boundary rows need qref (cruna_hip_set_qref)");
     if ((r.first == 400 || r.first == 401) && !c->shape)
@@ -353,9 +367,15 @@ int need_bc_inputs(cruna_hip_ctx *c) {
   return CRUNA_HIP_OK;
 }
 
-// set_boundary_condition_q (boundary_conditions.f90:52-136) on q, rows in order.
+// does a 203 row act in this call (outside: the one after the RK step)?
+bool zone_applies(const cruna_hip_ctx *c, bool outside) {
+  return outside ? c->zone_flag != 1 : c->zone_flag != -1;
+}
+
+// set_boundary_condition_q (boundary_conditions.f90:52-136) on q, rows in
+// order; outside: the call with outside_rhs=.true. after the RK step.
 template <typename Q>
-int bc_q(cruna_hip_ctx *c, Q *q) {
+int bc_q(cruna_hip_ctx *c, Q *q, bool outside = false) {
   if (int rc = need_bc_inputs(c)) return rc;
   for (auto &r : c->bc) {
     if (r.first == 201) {
@@ -363,19 +383,21 @@ int bc_q(cruna_hip_ctx *c, Q *q) {
       const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
       hipLaunchKernelGGL((k_bc_characteristic_q<Q, double>), dim3((np + 255) / 256), dim3(256), 0,
                          c->stream, q, (const double *)c->qref, b, c->cfg.n[0], c->cfg.n[1], c->N,
-                         c->cfg.gamma, (const double *)nullptr);
-    } else if (r.first == 202) {
+                         c->cfg.gamma, (const double *)nullptr, (const double *)nullptr);
+    } else if (r.first == 202 || r.first == 203) {
+      if (r.first == 203 && !zone_applies(c, outside)) continue;
       const BoxRow &b = r.second;
       const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
       hipLaunchKernelGGL((k_bc_characteristic_q<Q, double>), dim3((np + 255) / 256), dim3(256), 0,
                          c->stream, q, (const double *)c->qref, b, c->cfg.n[0], c->cfg.n[1], c->N,
-                         c->cfg.gamma, (const double *)c->kvec);
+                         c->cfg.gamma, (const double *)c->kvec,
+                         r.first == 203 ? (const double *)c->shape : (const double *)nullptr);
     } else if (r.first == 200) {
       const BoxRow &b = r.second;
       const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
       hipLaunchKernelGGL((k_bc_characteristic_q<Q, Q>), dim3((np + 255) / 256), dim3(256), 0,
                          c->stream, q, (const Q *)q, b, c->cfg.n[0], c->cfg.n[1], c->N, c->cfg.gamma,
-                         (const double *)nullptr);
+                         (const double *)nullptr, (const double *)nullptr);
     } else if (r.first == 401) {
       hipLaunchKernelGGL((k_bc_sponge_q<Q>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream, q,
                          (const double *)c->qref, (const double *)c->shape, c->N);
@@ -400,10 +422,11 @@ int bc_rhs(cruna_hip_ctx *c, T *rhs, const Q *q, bool late = false) {
       hipLaunchKernelGGL((k_sponge_rhs<T, Q>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream,
                          rhs, q, (const double *)c->qref, (const double *)c->shape, c->N,
                          late ? 1.0 : c->cfg.gamma - 1.0, late ? nullptr : jacobian<T>(c));
-    } else if (late && (r.first == 204 || r.first == 205)) {
+    } else if (late && (r.first == 204 || r.first == 205 || r.first == 206)) {
       hipLaunchKernelGGL((k_bc_characteristic_rhs<T, Q>), dim3((np + 255) / 256), dim3(256), 0,
                          c->stream, rhs, q, (const double *)c->qref, b, n1, n2, c->N, c->cfg.gamma,
-                         c->cfg.dt, r.first == 205 ? (const double *)c->kvec : (const double *)nullptr);
+                         c->cfg.dt, r.first == 204 ? (const double *)nullptr : (const double *)c->kvec,
+                         r.first == 206 ? (const double *)c->shape : (const double *)nullptr);
     } else if (late && r.first == 100) {
       hipLaunchKernelGGL((k_bc_wall_rhs<T>), dim3((np + 255) / 256), dim3(256), 0, c->stream, rhs, b,
                          (int)b.k[0], n1, n2, c->N);
@@ -680,7 +703,7 @@ int step_once(cruna_hip_ctx *c, int nt, bool weighted = true) {
   if (rc) return rc;
   // set_boundary_condition_q(qn, outside_rhs=.true.) (cruna_hpc.f90:216)
   if (c->has_bcq)
-    if ((rc = bc_q<double>(c, c->qn))) return rc;
+    if ((rc = bc_q<double>(c, c->qn, true))) return rc;
   int last = -1;
   for (int d = fuse_fx ? 1 : 0; d < 3; ++d) {
     const int fq = c->cfg.filter_freq[d];
@@ -740,17 +763,21 @@ int copy_rhs_out(cruna_hip_ctx *c, double *rhs) {
 // order: 201 characteristic_qs toward zero with the forward state q, 401
 // sponge_qs; 400 has no adjoint-state part.
 template <typename Q>
-int bc_qs(cruna_hip_ctx *c, Q *qs) {
+int bc_qs(cruna_hip_ctx *c, Q *qs, bool outside = false) {
   for (auto &r : c->bc) {
-    if (r.first == 201 || r.first == 200 || r.first == 202) {
+    if (r.first == 201 || r.first == 200 || r.first == 202 || r.first == 203) {
+      if (r.first == 203 && !zone_applies(c, outside)) continue;
       const BoxRow &b = r.second;
       const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
-      if (r.first == 202 && !c->kvec)
-        return fail(CRUNA_HIP_EINVAL, "rows of types 202/205 need the normals per point (cruna_hip_set_kvector)");
+      if (r.first >= 202 && !c->kvec)
+        return fail(CRUNA_HIP_EINVAL, "rows of types 202/203/205/206 need the normals per point (cruna_hip_set_kvector)");
+      if (r.first == 203 && !c->shape)
+        return fail(CRUNA_HIP_EINVAL, "zonal rows (203/206) need a shape (cruna_hip_set_sponge_shape)");
       hipLaunchKernelGGL((k_bc_characteristic_qs<Q>), dim3((np + 255) / 256), dim3(256), 0,
                          c->stream, qs, (const double *const *)c->q0p, b, c->cfg.n[0], c->cfg.n[1],
                          c->N, c->cfg.gamma,
-                         r.first == 202 ? (const double *)c->kvec : (const double *)nullptr);
+                         r.first >= 202 ? (const double *)c->kvec : (const double *)nullptr,
+                         r.first == 203 ? (const double *)c->shape : (const double *)nullptr);
     } else if (r.first == 401) {
       if (!c->shape)
         return fail(CRUNA_HIP_EINVAL, "sponge rows need a shape (cruna_hip_set_sponge_shape)");
@@ -831,7 +858,7 @@ int astep_once(cruna_hip_ctx *c, int nt) {
   int rc = mixed ? ark4_step<float>(c, nullptr) : ark4_step<double>(c, nullptr);
   if (rc) return rc;
   if (c->has_bcq)
-    if ((rc = bc_qs<double>(c, c->qa))) return rc;
+    if ((rc = bc_qs<double>(c, c->qa, true))) return rc;
   for (int d = 0; d < 3; ++d) {
     const int fq = c->cfg.afilter_freq[d];
     if (fq > 0 && nt % fq == 0) {
@@ -960,8 +987,7 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   if (cfg->nbc < 0 || cfg->nbc > CRUNA_HIP_MAX_BC) return fail(CRUNA_HIP_EINVAL, "bad nbc");
   for (int r = 0; r < cfg->nbc; ++r) {
     const int t = (int)lround(cfg->bc[r][0]);
-    if (t != 0 && t != 100 && t != 200 && t != 201 && t != 202 && t != 204 && t != 205 && t != 400 &&
-        t != 401)
+    if (t != 0 && t != 100 && (t < 200 || t > 206) && t != 400 && t != 401)
       return fail(CRUNA_HIP_EINVAL,
                   "boundary type " + std::to_string(t) + " not implemented on the device");
   }
@@ -981,6 +1007,21 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
       lo[d] = std::max(1, (int)lround(cfg->bc[r][1 + 2 * d])) - 1;
       hi[d] = std::min(cfg->n[d], (int)lround(cfg->bc[r][2 + 2 * d])) - 1;
       b.k[d] = cfg->bc[r][7 + d];
+    }
+    if (t == 203 || t == 206) {
+      // zonal rows act on the whole domain; their box columns are unused
+      // (boundary_conditions.f90:93-108, 276-279)
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = 0;
+        hi[d] = cfg->n[d] - 1;
+        b.k[d] = 0.0;
+      }
+      if (t == 203) {
+        const int f = (int)lround(cfg->bc[r][7]);
+        if (f < -1 || f > 1) return fail(CRUNA_HIP_EINVAL, "type 203 row: column 8 must be -1, 0 or 1");
+        c->zone_flag = f;
+      }
+      c->needs_shape = true;
     }
     if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) continue;
     b.i0 = lo[0]; b.i1 = hi[0]; b.j0 = lo[1]; b.j1 = hi[1]; b.k0 = lo[2]; b.k1 = hi[2];
@@ -1009,10 +1050,11 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
       b.k[0] = v - 1;
     }
     c->bc.push_back({t, b});
-    if (t == 200 || t == 201 || t == 202 || t == 401) c->has_bcq = true;
-    if (t == 202 || t == 205) c->needs_kvec = true;
+    if (t == 200 || t == 201 || t == 202 || t == 203 || t == 401) c->has_bcq = true;
+    if (t == 202 || t == 203 || t == 205 || t == 206) c->needs_kvec = true;
     if (t == 400) c->has_bcrhs = true;
-    if (t == 100 || t == 204 || t == 205) c->has_bclate = true;
+    if (t == 100 || t == 204 || t == 205 || t == 206) c->has_bclate = true;
+    if (t == 204 || t == 205 || t == 206) c->has_char_rhs = true;
     if (t == 100) c->has_wall = true;
   }
   for (int d = 0; d < 3; ++d) {
@@ -1369,6 +1411,7 @@ int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const
   if (!c || !rhs || !qs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
+  if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (int rc = stage_adjoint_inputs(c, qs, q, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
@@ -1398,6 +1441,7 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *
   if (!c || !qs1 || !qs0 || !q0) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
+  if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (int rc = stage_adjoint_inputs(c, qs0, q0, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
@@ -1475,6 +1519,7 @@ int cruna_hip_run_adjoint(cruna_hip_ctx *c, int nt_end, int nsteps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
+  if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
   if (!c->qa) return fail(CRUNA_HIP_EINVAL, "no adjoint state (cruna_hip_set_adjoint_state)");
@@ -1548,6 +1593,7 @@ int cruna_hip_run_adjoint_subsets(cruna_hip_ctx *c, int subsets, int steps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
+  if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (subsets < 1 || steps < 1) return fail(CRUNA_HIP_EINVAL, "subsets and steps must be >= 1");
   if (c->ckpt_n < subsets + 1) return fail(CRUNA_HIP_EINVAL, "not enough subset checkpoints (cruna_hip_subsets_alloc)");
@@ -1776,6 +1822,7 @@ int cruna_hip_ensemble_set_role(cruna_hip_ensemble *e, int i, int role, int nt) 
   if (role == CRUNA_HIP_ADJOINT) {
     if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
     if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
+    if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
     if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
     if (!c->qa || c->traj_slots == 0)
       return fail(CRUNA_HIP_EINVAL, "an adjoint member needs its adjoint state and trajectory first");

@@ -359,7 +359,7 @@ contains
     call check(c_set_sponge_shape(ctx, s), "set_sponge_shape")
   end subroutine
 
-  ! kvector(n1,n2,n3,3): boundary normals per point for rows 202 / 205
+  ! kvector(n1,n2,n3,3): boundary normals per point for rows 202 / 203 / 205 / 206
   subroutine hip_set_kvector(ctx, k)
     type(c_ptr), intent(in) :: ctx
     real(c_double), intent(in), contiguous :: k(:,:,:,:)

@@ -179,3 +179,100 @@ def test_per_point_normal_rows_match_their_row_normal_forms():
     a = O.bc_qs(O.Cfg(n, (0.1,) * 3, bc=[(202,) + box], kvector=const), qs, q)
     b = O.bc_qs(O.Cfg(n, (0.1,) * 3, bc=[(201,) + box + (0, -1, 0)]), qs, q)
     assert np.array_equal(a, b)
+
+
+def _zonal_case(n, seed):
+    rng = np.random.default_rng(seed)
+    q = np.zeros(n + (5,), order="F")
+    q[..., 0] = 1 + 0.05 * rng.random(n)
+    q[..., 1:4] = 0.3 * rng.standard_normal(n + (3,))
+    q[..., 4] = 1 / 1.4 * (1 + 0.05 * rng.random(n))
+    qref = q.copy(order="F")
+    qref[..., 0] *= 1.02
+    qref[..., 4] *= 0.99
+    qref[..., 1:4] += 0.05
+    kv = np.asfortranarray(rng.standard_normal(n + (3,)))
+    kv /= np.linalg.norm(kv, axis=-1, keepdims=True)
+    shape = np.asfortranarray(rng.random(n))
+    return q, qref, kv, shape, rng
+
+
+def _prim(q):
+    return np.concatenate([q[..., :1], q[..., 1:4] / q[..., :1], q[..., 4:]], axis=-1)
+
+
+def test_zonal_rows_are_the_per_point_rows_over_the_domain_scaled_by_the_shape():
+    """Types 203 / 206 (characteristic_*_k4d(q, qref, kvector, shape) on the
+    whole domain, boundary_conditions.f90:93-108, 276-279): with shape 1 they
+    equal 202 / 205 on a box covering the domain, whatever box columns the
+    row holds; the change of the primitive state that 203 makes is linear in
+    the shape (the wave mask depends on the state only), shape 0 leaves q,
+    qs and the RHS as they are; 206 is affine in the shape between the RHS
+    (shape 0) and 205's (shape 1); the adjoint 203 scales qs's change."""
+    n = (12, 10, 8)
+    q, qref, kv, shape, rng = _zonal_case(n, 21)
+    dx = (0.1,) * 3
+    whole = (1, n[0], 1, n[1], 1, n[2])
+    one, zero = np.ones(n, order="F"), np.zeros(n, order="F")
+    z = O.Cfg(n, dx, dt=DT, bc=[(203, 0, 0, 0, 0, 0, 0)], kvector=kv)
+    p = O.Cfg(n, dx, dt=DT, bc=[(202,) + whole], kvector=kv)
+    assert np.array_equal(O.bc_q(z, q, qref=qref, shape=one), O.bc_q(p, q, qref=qref))
+    base = _prim(q)
+    d1 = _prim(O.bc_q(z, q, qref=qref, shape=one)) - base
+    ds = _prim(O.bc_q(z, q, qref=qref, shape=shape)) - base
+    assert np.abs(ds - shape[..., None] * d1).max() < 1e-14 * np.abs(base).max()
+    assert np.abs(d1).max() > 1e-3  # the rows do change the state
+    assert np.allclose(O.bc_q(z, q, qref=qref, shape=zero), q, rtol=1e-15, atol=0)
+
+    rhs = np.asfortranarray(rng.standard_normal(n + (5,)))
+    z6 = O.Cfg(n, dx, dt=DT, bc=[(206, 0, 0, 0, 0, 0, 0)], kvector=kv)
+    p5 = O.Cfg(n, dx, dt=DT, bc=[(205,) + whole], kvector=kv)
+    r1 = O.bc_rhs(z6, rhs, q, qref=qref, shape=one)
+    assert np.array_equal(r1, O.bc_rhs(p5, rhs, q, qref=qref))
+    assert np.array_equal(O.bc_rhs(z6, rhs, q, qref=qref, shape=zero), rhs)
+    rs = O.bc_rhs(z6, rhs, q, qref=qref, shape=shape)
+    s = shape[..., None]
+    assert np.abs(rs - ((1 - s) * rhs + s * r1)).max() < 1e-13 * np.abs(r1).max()
+
+    qs = np.asfortranarray(rng.standard_normal(n + (5,)))
+    a1 = O.bc_qs(z, qs, q, shape=one)
+    assert np.array_equal(a1, O.bc_qs(p, qs, q))
+    a_s = O.bc_qs(z, qs, q, shape=shape)
+    assert np.abs((a_s - qs) - s * (a1 - qs)).max() < 1e-14 * np.abs(qs).max()
+
+
+@pytest.mark.parametrize("flag", [-1, 0, 1])
+def test_zonal_row_flag_picks_the_calls_it_acts_in(flag):
+    """Column 8 of a 203 row (init_boundary_conditions,
+    boundary_conditions.f90:434-438): -1 acts only in the call after the RK
+    step (outside_rhs), 1 only in the RHS's own calls, 0 in both.  A step
+    with flag 1 is the RK step of a flag-0 row; a step with flag -1 is the
+    RK step without the row followed by the row once."""
+    n = (12, 10, 8)
+    q, qref, kv, shape, _ = _zonal_case(n, 22)
+    q[..., 1:4] *= 0.3
+    kw = dict(dt=DT, mu0=1e-2, filter_freq=(0, 0, 0), kvector=kv)
+    dx = (0.1,) * 3
+    row = (203, 0, 0, 0, 0, 0, 0, flag)
+    got = O.run_direct(O.Cfg(n, dx, bc=[row], **kw), q, 1, qref=qref, shape=shape)
+    both = O.Cfg(n, dx, bc=[(203, 0, 0, 0, 0, 0, 0, 0)], **kw)
+    if flag == 1:
+        ref = O.time_stepper_direct(both, q, qref=qref, shape=shape)
+    else:
+        inner = O.time_stepper_direct(both if flag == 0 else O.Cfg(n, dx, **kw), q, qref=qref, shape=shape)
+        ref = O.bc_q(both, inner, qref=qref, shape=shape)
+    assert np.array_equal(got, ref)
+    other = O.run_direct(O.Cfg(n, dx, bc=[(203, 0, 0, 0, 0, 0, 0, -flag or 1)], **kw), q, 1,
+                         qref=qref, shape=shape)
+    assert not np.array_equal(got, other)
+
+
+def test_adjoint_stops_on_characteristic_rhs_rows():
+    """set_boundary_condition_rhss has its cases 204-206 commented out
+    (boundary_conditions.f90:339-353): the adjoint stops on such rows."""
+    n = (8, 8, 8)
+    q, qref, kv, shape, rng = _zonal_case(n, 23)
+    for row in [(204, 1, 8, 1, 1, 1, 8, 0, -1, 0), (205, 1, 8, 1, 1, 1, 8), (206,)]:
+        cfg = O.Cfg(n, (0.1,) * 3, bc=[row], kvector=kv)
+        with pytest.raises(ValueError, match="stops"):
+            O.rhs_adjoint(cfg, q, q, shape=shape)
