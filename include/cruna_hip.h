@@ -44,7 +44,7 @@ extern "C" {
 #define CRUNA_HIP_DIFF_O2_EXPLICIT 3 /* src/diff/diff_pen_core_o2_e_c_o.f90 (open lines only) */
 
 #define CRUNA_HIP_EQ_NS 0    /* src/equations/ns_rho_rhou_p*.f90 (default) */
-#define CRUNA_HIP_EQ_EULER 1 /* src/equations/euler_rho_rhou_p.f90 (Cartesian) */
+#define CRUNA_HIP_EQ_EULER 1 /* src/equations/euler_rho_rhou_p.f90, _grid_transform.f90 on a metric */
 
 typedef struct cruna_hip_cfg {
   int n[3];              /* params%geom%n1..n3 (block = whole grid on one GPU) */
@@ -90,9 +90,11 @@ typedef struct cruna_hip_cfg {
                             o2_e_c_p core, weight phi of the penalisation or 1,
                             strength 1) after the compact filters of a step */
   int equations;         /* governing_equations: CRUNA_HIP_EQ_NS or CRUNA_HIP_EQ_EULER
-                            (euler_rho_rhou_p: the NS sweeps with mu = lambda = 0;
-                            Cartesian grids without penalisation; its adjoint with
-                            sponge rows (400) returns EINVAL) */
+                            (euler_rho_rhou_p, or euler_rho_rhou_p_grid_transform
+                            after cruna_hip_set_metric: the NS sweeps with
+                            mu = lambda = 0; without penalisation; its adjoint
+                            with sponge rows (400) or on a deformed grid returns
+                            EINVAL) */
 } cruna_hip_cfg;
 
 typedef struct cruna_hip_ctx cruna_hip_ctx;

@@ -975,10 +975,76 @@ void orc_rhs_direct_euler(const orc_cfg *c, double *rhs, const double *q_in, con
   for (int k = 0; k < 3; ++k) { free(a[k]); free(pg[k]); }
 }
 
+/* Euler right-hand side on a deformed grid,
+ * src/equations/euler_rho_rhou_p_grid_transform.f90:33-140: projected
+ * velocities uh_d = sum_j M(d,j) u_j (:75-77), physical pressure gradient
+ * p_j = sum_d M(d,j) D_d p (:80-85); mass -J sum_d D_d(rho uh_d); momentum
+ * -J (sum_d D_d(rho u_i uh_d) + p_i); pressure
+ * J (gamma - 1) (gamma/(gamma - 1) (-sum_d D_d(p uh_d)) + u.grad p)
+ * (:118-128); force (no-op), sponge (empty), BC rows (:133-139). */
+#define MET_E(Xm, a, b, N) ((Xm) + (size_t)(N) * ((a) + 3 * (b)))
+void orc_rhs_direct_euler_gt(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
+                             const double *Xj, const double *qref, const double *shape) {
+  const size_t N = npts(c);
+  const double dxs[3] = {c->dx[0], c->dx[1], c->dx[2]};
+  double *q = xalloc(5 * N), *u = xalloc(3 * N), *uh = xalloc(3 * N), *tmp = xalloc(N);
+  double *a[3] = {xalloc(N), xalloc(N), xalloc(N)};
+  double *pg[3] = {xalloc(N), xalloc(N), xalloc(N)};
+  double *r[5];
+  for (int v = 0; v < 5; ++v) r[v] = rhs + v * N;
+  memcpy(q, q_in, 5 * N * sizeof(double)); /* :59 */
+  orc_bc_q(c, q, qref, shape);              /* :62 */
+  for (int m = 0; m < 3; ++m)               /* :65-69 */
+    for (size_t i = 0; i < N; ++i) u[m * N + i] = q[(m + 1) * N + i] / q[i];
+  for (int d = 0; d < 3; ++d) {             /* :75-77 */
+    const double *m0 = MET_E(Xm, d, 0, N), *m1 = MET_E(Xm, d, 1, N), *m2 = MET_E(Xm, d, 2, N);
+    for (size_t i = 0; i < N; ++i) uh[d * N + i] = m0[i] * u[i] + m1[i] * u[N + i] + m2[i] * u[2 * N + i];
+  }
+  for (int d = 0; d < 3; ++d) orc_diff3d(c, d, a[d], q + 4 * N, dxs[d]); /* :80-85 */
+  for (int j = 0; j < 3; ++j) {
+    const double *m0 = MET_E(Xm, 0, j, N), *m1 = MET_E(Xm, 1, j, N), *m2 = MET_E(Xm, 2, j, N);
+    for (size_t i = 0; i < N; ++i) pg[j][i] = m0[i] * a[0][i] + m1[i] * a[1][i] + m2[i] * a[2][i];
+  }
+  for (int d = 0; d < 3; ++d) { /* mass :88-93 */
+    for (size_t i = 0; i < N; ++i) tmp[i] = q[i] * uh[d * N + i];
+    orc_diff3d(c, d, a[d], tmp, dxs[d]);
+  }
+  for (size_t i = 0; i < N; ++i) r[0][i] = Xj[i] * (-a[0][i] - a[1][i] - a[2][i]);
+  for (int m = 0; m < 3; ++m) { /* momentum :96-117 */
+    for (int d = 0; d < 3; ++d) {
+      for (size_t i = 0; i < N; ++i) tmp[i] = q[i] * u[m * N + i] * uh[d * N + i];
+      orc_diff3d(c, d, a[d], tmp, dxs[d]);
+    }
+    for (size_t i = 0; i < N; ++i) r[m + 1][i] = Xj[i] * (-a[0][i] - a[1][i] - a[2][i] - pg[m][i]);
+  }
+  for (int d = 0; d < 3; ++d) { /* pressure :120-128 */
+    for (size_t i = 0; i < N; ++i) tmp[i] = q[4 * N + i] * uh[d * N + i];
+    orc_diff3d(c, d, a[d], tmp, dxs[d]);
+  }
+  const double gt = c->gamma / (c->gamma - 1.0), gm1 = c->gamma - 1.0;
+  for (size_t i = 0; i < N; ++i) {
+    double v = gt * (-a[0][i] - a[1][i] - a[2][i]);
+    v = v + u[i] * pg[0][i] + u[N + i] * pg[1][i] + u[2 * N + i] * pg[2][i];
+    r[4][i] = Xj[i] * gm1 * v;
+  }
+  orc_bc_rhs(c, rhs, q, qref, shape); /* :133-139 */
+  free(q); free(u); free(uh); free(tmp);
+  for (int k = 0; k < 3; ++k) { free(a[k]); free(pg[k]); }
+}
+#undef MET_E
+
 /* ------------------------------------------------------------------------ */
 /* Direct NS right-hand side, src/equations/ns_rho_rhou_p.f90:31-253.       */
 void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const double *qref,
                     const double *shape, int heat_unit_spacing) {
+  if (c->equations == ORC_EQ_EULER && c->penal_phi) {
+    fprintf(stderr, "oracle: the penalised Euler modules are not restated\n");
+    abort();
+  }
+  if (c->equations == ORC_EQ_EULER && c->X_metric) { /* => euler_rho_rhou_p_grid_transform */
+    orc_rhs_direct_euler_gt(c, rhs, q_in, c->X_metric, c->X_jacobian, qref, shape);
+    return;
+  }
   if (c->equations == ORC_EQ_EULER) { /* governing_equations => euler_rho_rhou_p */
     orc_rhs_direct_euler(c, rhs, q_in, qref, shape);
     return;

@@ -86,6 +86,7 @@ def lib():
         L.orc_rhs_direct.argtypes = [P, _dp, _dp, _vp, _vp, C.c_int]
         L.orc_calc_metric.argtypes = [P, _dp, _dp, _dp, _dp]
         L.orc_rhs_direct_gt.argtypes = [P, _dp, _dp, _dp, _dp, _vp, _vp]
+        L.orc_rhs_direct_euler_gt.argtypes = [P, _dp, _dp, _dp, _dp, _vp, _vp]
         L.orc_rhs_direct_ev.argtypes = [P, _dp, _dp, _dp, _dp, _vp, _vp]
         L.orc_filter_weighted4d.argtypes = [P, C.c_int, _dp, C.c_int, _vp, _vp]
         L.orc_characteristic_rhs_k1d.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_double,
@@ -157,6 +158,8 @@ class Cfg:
         c.gamma, c.R, c.W, c.Pr, c.mu0, c.T0 = gamma, R, W, Pr, mu0, T0
         c.visc_law, c.visc_exp, c.dt = visc_law, visc_exp, dt
         c.equations = int(equations)
+        if c.equations == EQ_EULER and penalization is not None:
+            raise ValueError("the penalised Euler modules are not restated")
         c.nbc = len(bc)
         for r, row in enumerate(bc):
             for j in range(10):
@@ -287,6 +290,15 @@ def rhs_direct_gt(cfg, q, X_metric, X_jacobian, qref=None, shape=None):
     qref = None if qref is None else _f(qref)
     shape = None if shape is None else _f(shape)
     lib().orc_rhs_direct_gt(cfg.ref(), rhs, q, _f(X_metric), _f(X_jacobian), _ptr(qref), _ptr(shape))
+    return rhs
+
+
+def rhs_direct_euler_gt(cfg, q, X_metric, X_jacobian, qref=None, shape=None):
+    q = _f(q)
+    rhs = np.zeros_like(q, order="F")
+    qref = None if qref is None else _f(qref)
+    shape = None if shape is None else _f(shape)
+    lib().orc_rhs_direct_euler_gt(cfg.ref(), rhs, q, _f(X_metric), _f(X_jacobian), _ptr(qref), _ptr(shape))
     return rhs
 
 

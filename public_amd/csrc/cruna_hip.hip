@@ -332,10 +332,12 @@ Penal<float> penal<float>(cruna_hip_ctx *c) { return c->pe32; }
 // set_boundary_condition_rhss zeroes a variable of the adjoint RHS on wall
 // rows (100); the device adjoint does not apply rows to its complete RHS
 const char *kNoWallAdjoint = "wall rows (100) are not implemented in the device adjoint";
-// euler_rho_rhou_p's variants on deformed grids and with penalisation are
-// separate modules the device does not build
-const char *kEulerCartesian =
-    "the Euler equations are built for Cartesian grids without penalisation only";
+// euler_rho_rhou_p's penalised variants (euler_*_ev_phi*) are separate
+// modules the device does not build
+const char *kEulerCartesian = "the penalised Euler equations are not built on the device";
+// euler_rho_rhou_p_grid_transform.f90:158-509 has its own metric adjoint,
+// which the device does not build (the direct RHS is the NS one with mu = 0)
+const char *kNoEulerGtAdjoint = "the Euler adjoint on a deformed grid is not built on the device";
 // euler_rho_rhou_p.f90:173 passes q to set_boundary_condition_rhss, so its
 // adjoint applies sponge_rhss on 400 rows, which the device adjoint does not
 const char *kNoEulerSpongeAdjoint =
@@ -1182,7 +1184,6 @@ int cruna_hip_set_sponge_shape(cruna_hip_ctx *c, const double *shape) {
 int cruna_hip_set_metric(cruna_hip_ctx *c, const double *X_metric, const double *X_jacobian) {
   if (!c || !X_metric || !X_jacobian) return fail(CRUNA_HIP_EINVAL, "null argument");
   ++c->gen;
-  if (c->cfg.equations == CRUNA_HIP_EQ_EULER) return fail(CRUNA_HIP_EINVAL, kEulerCartesian);
 
   const long n1 = c->cfg.n[0], n2 = c->cfg.n[1], n3 = c->cfg.n[2], N = c->N;
   auto M = [&](int k, int j, long i, long jj, long kk) {
@@ -1412,6 +1413,7 @@ int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
   if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
+  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (int rc = stage_adjoint_inputs(c, qs, q, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
@@ -1442,6 +1444,7 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
   if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
+  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (int rc = stage_adjoint_inputs(c, qs0, q0, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
@@ -1520,6 +1523,7 @@ int cruna_hip_run_adjoint(cruna_hip_ctx *c, int nt_end, int nsteps) {
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
   if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
+  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
   if (!c->qa) return fail(CRUNA_HIP_EINVAL, "no adjoint state (cruna_hip_set_adjoint_state)");
@@ -1594,6 +1598,7 @@ int cruna_hip_run_adjoint_subsets(cruna_hip_ctx *c, int subsets, int steps) {
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
   if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
+  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (subsets < 1 || steps < 1) return fail(CRUNA_HIP_EINVAL, "subsets and steps must be >= 1");
   if (c->ckpt_n < subsets + 1) return fail(CRUNA_HIP_EINVAL, "not enough subset checkpoints (cruna_hip_subsets_alloc)");
@@ -1823,6 +1828,7 @@ int cruna_hip_ensemble_set_role(cruna_hip_ensemble *e, int i, int role, int nt) 
     if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
     if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
     if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
+    if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
     if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
     if (!c->qa || c->traj_slots == 0)
       return fail(CRUNA_HIP_EINVAL, "an adjoint member needs its adjoint state and trajectory first");

@@ -1344,3 +1344,40 @@ def test_zonal_rows_match_oracle(flag):
         got_a = g.get_adjoint_state()
     for v in range(5):
         assert rel(got_a[..., v], ref_a[..., v]) < 1e-12, (v, rel(got_a[..., v], ref_a[..., v]))
+
+
+@pytest.mark.parametrize("grid", ["wavy", "flat"])
+def test_euler_on_deformed_grids_matches_oracle(grid):
+    """euler_rho_rhou_p_grid_transform (the grid_transform_ns_and_euler
+    unit-test family's 03 case runs it on an undeformed grid: "flat", the
+    tensor-product path) and a wavy curvilinear grid: the RHS and a 4-step
+    run with characteristic rows on both y faces against the oracle's
+    restatement; the Euler adjoint on such a grid is refused."""
+    from public_amd import EQ_EULER
+    from public_amd.cruna import CrunaError
+    Cruna = _gpu()
+    n = (32, 40, 16)
+    dx, X, Xc = _wavy(n, amp=(0.15, 0.2, 0.1, 0.05) if grid == "wavy" else (0.0, 0.0, 0.0, 0.0))
+    n1, n2, n3 = n
+    bc = [(201, 1, n1, 1, 1, 1, n3, 0, -1, 0), (201, 1, n1, n2, n2, 1, n3, 0, 1, 0)]
+    scheme = (O.PERIODIC, O.OPEN, O.PERIODIC)
+    kw = dict(T0=1.0 / 1.4, dt=2e-3, scheme=scheme, bc=bc)
+    c0 = O.Cfg(n, dx, filter_freq=(1, 1, 1), equations=O.EQ_EULER, **kw)
+    Xm, Xj = O.calc_metric(c0, X, Xc)
+    cfg = O.Cfg(n, dx, filter_freq=(1, 1, 1), equations=O.EQ_EULER, metric=(Xm, Xj), **kw)
+    q = _gt_state(X, 3)
+    ref_r = O.rhs_direct(cfg, q, qref=q)
+    ref = O.run_direct(cfg, q, 4, qref=q)
+    with Cruna(n, dx, filter_freq=(1, 1, 1), equations=EQ_EULER, **kw) as g:
+        g.set_metric(Xm, Xj)
+        g.set_qref(q)
+        got_r = g.right_hand_side_direct_3d(q)
+        g.set_state(q)
+        g.run_direct(4)
+        g.synchronize()
+        got = g.get_state()
+        with pytest.raises(CrunaError, match="deformed grid"):
+            g.right_hand_side_adjoint_3d(q, q)
+    for v in range(5):
+        assert rel(got_r[..., v], ref_r[..., v]) < 1e-12, (v, rel(got_r[..., v], ref_r[..., v]))
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))

@@ -66,3 +66,41 @@ def test_euler_rhs_is_ns_without_viscosity():
         assert np.linalg.norm(eu[..., v] - ns[..., v]) <= 1e-13 * np.linalg.norm(eu[..., v]), v
     visc = O.rhs_direct(O.Cfg((n,) * 3, (h,) * 3, mu0=0.05), q)
     assert np.linalg.norm(visc - eu) > 1e-4 * np.linalg.norm(eu)
+
+
+def test_euler_gt_rhs_pins():
+    """The grid-transform Euler RHS (euler_rho_rhou_p_grid_transform.f90:33-140):
+    on an undeformed grid (calc_metric of the computational grid: metric the
+    identity, Jacobian 1) it equals the Cartesian Euler RHS; on a wavy
+    deformed grid it equals the grid-transform NS RHS with mu0 = 0 (pinned
+    by its own convergence tests) to rounding; a uniform stream stays at
+    rest; a metric in the config selects it."""
+    from tests.test_oracle_gt import _smooth_state, cfg, wavy_grid
+
+    n = (16, 17, 16)
+    dx = (2 * np.pi / 16, 0.5, 2 * np.pi / 16)
+    xs = [np.arange(n[0]) * dx[0], -4 + np.arange(n[1]) * dx[1], np.arange(n[2]) * dx[2]]
+    Xc = np.asfortranarray(np.stack(np.meshgrid(*xs, indexing="ij"), axis=-1))
+    q = _smooth_state(Xc)
+    c = cfg(n, dx, equations=O.EQ_EULER)
+    Xm, Xj = O.calc_metric(c, Xc, Xc)
+    ref = O.rhs_direct(c, q)
+    got = O.rhs_direct_euler_gt(c, q, Xm, Xj)
+    for v in range(5):
+        e = np.linalg.norm(got[..., v] - ref[..., v]) / np.linalg.norm(ref[..., v])
+        assert e < 1e-12, (v, e)
+
+    n = (16, 24, 16)
+    dx, X, Xc = wavy_grid(n)
+    c = cfg(n, dx, equations=O.EQ_EULER)
+    Xm, Xj = O.calc_metric(c, X, Xc)
+    q = _smooth_state(X)
+    got = O.rhs_direct_euler_gt(c, q, Xm, Xj)
+    ns = O.rhs_direct_gt(cfg(n, dx, mu0=0.0), q, Xm, Xj)
+    for v in range(5):
+        e = np.linalg.norm(got[..., v] - ns[..., v]) / np.linalg.norm(ns[..., v])
+        assert e < 1e-12, (v, e)
+    assert np.array_equal(O.rhs_direct(cfg(n, dx, equations=O.EQ_EULER, metric=(Xm, Xj)), q), got)
+    q0 = np.zeros(n + (5,), order="F")
+    q0[..., 0], q0[..., 1], q0[..., 2], q0[..., 3], q0[..., 4] = 1.2, 0.3, -0.2, 0.1, 0.9
+    assert np.abs(O.rhs_direct_euler_gt(c, q0, Xm, Xj)).max() < 1e-12
