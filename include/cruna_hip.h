@@ -92,9 +92,11 @@ typedef struct cruna_hip_cfg {
   int equations;         /* governing_equations: CRUNA_HIP_EQ_NS or CRUNA_HIP_EQ_EULER
                             (euler_rho_rhou_p, or euler_rho_rhou_p_grid_transform
                             after cruna_hip_set_metric: the NS sweeps with
-                            mu = lambda = 0; without penalisation; its adjoint
-                            with sponge rows (400) or on a deformed grid returns
-                            EINVAL) */
+                            mu = lambda = 0; penalised only with a metric,
+                            euler_rho_rhou_p_ev_phi_grid_transform, as the
+                            reference has no Cartesian penalised Euler module;
+                            its adjoint with sponge rows (400) or on a deformed
+                            grid returns EINVAL) */
 } cruna_hip_cfg;
 
 typedef struct cruna_hip_ctx cruna_hip_ctx;

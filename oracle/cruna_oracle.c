@@ -981,10 +981,15 @@ void orc_rhs_direct_euler(const orc_cfg *c, double *rhs, const double *q_in, con
  * p_j = sum_d M(d,j) D_d p (:80-85); mass -J sum_d D_d(rho uh_d); momentum
  * -J (sum_d D_d(rho u_i uh_d) + p_i); pressure
  * J (gamma - 1) (gamma/(gamma - 1) (-sum_d D_d(p uh_d)) + u.grad p)
- * (:118-128); force (no-op), sponge (empty), BC rows (:133-139). */
+ * (:118-128); force (no-op), sponge (empty), BC rows (:133-139).
+ * With phi, euler_rho_rhou_p_ev_phi_grid_transform.f90:33-160: phi inside the
+ * flux derivatives (phi rho uh_d, phi rho u_i uh_d, phi p uh_d) and 1/phi on
+ * their sums; pressure J (-gamma/phi sum_d D_d(phi p uh_d) + (gamma - 1)
+ * u.grad p); apply_volume_penalization is the reference's no-op. */
 #define MET_E(Xm, a, b, N) ((Xm) + (size_t)(N) * ((a) + 3 * (b)))
-void orc_rhs_direct_euler_gt(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
-                             const double *Xj, const double *qref, const double *shape) {
+static void euler_gt_impl(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
+                          const double *Xj, const double *phi, const double *inv_phi,
+                          const double *qref, const double *shape) {
   const size_t N = npts(c);
   const double dxs[3] = {c->dx[0], c->dx[1], c->dx[2]};
   double *q = xalloc(5 * N), *u = xalloc(3 * N), *uh = xalloc(3 * N), *tmp = xalloc(N);
@@ -1005,27 +1010,40 @@ void orc_rhs_direct_euler_gt(const orc_cfg *c, double *rhs, const double *q_in, 
     const double *m0 = MET_E(Xm, 0, j, N), *m1 = MET_E(Xm, 1, j, N), *m2 = MET_E(Xm, 2, j, N);
     for (size_t i = 0; i < N; ++i) pg[j][i] = m0[i] * a[0][i] + m1[i] * a[1][i] + m2[i] * a[2][i];
   }
-  for (int d = 0; d < 3; ++d) { /* mass :88-93 */
-    for (size_t i = 0; i < N; ++i) tmp[i] = q[i] * uh[d * N + i];
+#define PHI_(i) (phi ? phi[i] : 1.0)
+  for (int d = 0; d < 3; ++d) { /* mass :88-93 (ev: phi_rho uh) */
+    for (size_t i = 0; i < N; ++i) tmp[i] = phi ? phi[i] * q[i] * uh[d * N + i] : q[i] * uh[d * N + i];
     orc_diff3d(c, d, a[d], tmp, dxs[d]);
   }
-  for (size_t i = 0; i < N; ++i) r[0][i] = Xj[i] * (-a[0][i] - a[1][i] - a[2][i]);
-  for (int m = 0; m < 3; ++m) { /* momentum :96-117 */
+  for (size_t i = 0; i < N; ++i)
+    r[0][i] = phi ? Xj[i] * inv_phi[i] * (-a[0][i] - a[1][i] - a[2][i]) : Xj[i] * (-a[0][i] - a[1][i] - a[2][i]);
+  for (int m = 0; m < 3; ++m) { /* momentum :96-117 (ev: phi_rho_u_m uh) */
     for (int d = 0; d < 3; ++d) {
-      for (size_t i = 0; i < N; ++i) tmp[i] = q[i] * u[m * N + i] * uh[d * N + i];
+      for (size_t i = 0; i < N; ++i)
+        tmp[i] = phi ? PHI_(i) * q[(m + 1) * N + i] * uh[d * N + i] : q[i] * u[m * N + i] * uh[d * N + i];
       orc_diff3d(c, d, a[d], tmp, dxs[d]);
     }
-    for (size_t i = 0; i < N; ++i) r[m + 1][i] = Xj[i] * (-a[0][i] - a[1][i] - a[2][i] - pg[m][i]);
+    for (size_t i = 0; i < N; ++i)
+      r[m + 1][i] = phi ? Xj[i] * (inv_phi[i] * (-a[0][i] - a[1][i] - a[2][i]) - pg[m][i])
+                        : Xj[i] * (-a[0][i] - a[1][i] - a[2][i] - pg[m][i]);
   }
-  for (int d = 0; d < 3; ++d) { /* pressure :120-128 */
-    for (size_t i = 0; i < N; ++i) tmp[i] = q[4 * N + i] * uh[d * N + i];
+  for (int d = 0; d < 3; ++d) { /* pressure :120-128 (ev: phi_p uh) */
+    for (size_t i = 0; i < N; ++i) tmp[i] = PHI_(i) * q[4 * N + i] * uh[d * N + i];
     orc_diff3d(c, d, a[d], tmp, dxs[d]);
   }
+#undef PHI_
   const double gt = c->gamma / (c->gamma - 1.0), gm1 = c->gamma - 1.0;
   for (size_t i = 0; i < N; ++i) {
-    double v = gt * (-a[0][i] - a[1][i] - a[2][i]);
-    v = v + u[i] * pg[0][i] + u[N + i] * pg[1][i] + u[2 * N + i] * pg[2][i];
-    r[4][i] = Xj[i] * gm1 * v;
+    const double up = u[i] * pg[0][i] + u[N + i] * pg[1][i] + u[2 * N + i] * pg[2][i];
+    if (phi) {
+      double v = -c->gamma * inv_phi[i] * (a[0][i] + a[1][i] + a[2][i]);
+      v = v + gm1 * up;
+      r[4][i] = v * Xj[i];
+    } else {
+      double v = gt * (-a[0][i] - a[1][i] - a[2][i]);
+      v = v + up;
+      r[4][i] = Xj[i] * gm1 * v;
+    }
   }
   orc_bc_rhs(c, rhs, q, qref, shape); /* :133-139 */
   free(q); free(u); free(uh); free(tmp);
@@ -1033,12 +1051,30 @@ void orc_rhs_direct_euler_gt(const orc_cfg *c, double *rhs, const double *q_in, 
 }
 #undef MET_E
 
+void orc_rhs_direct_euler_gt(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
+                             const double *Xj, const double *qref, const double *shape) {
+  euler_gt_impl(c, rhs, q_in, Xm, Xj, NULL, NULL, qref, shape);
+}
+
+void orc_rhs_direct_euler_ev_gt(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
+                                const double *Xj, const double *phi, const double *inv_phi,
+                                const double *qref, const double *shape) {
+  euler_gt_impl(c, rhs, q_in, Xm, Xj, phi, inv_phi, qref, shape);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Direct NS right-hand side, src/equations/ns_rho_rhou_p.f90:31-253.       */
 void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const double *qref,
                     const double *shape, int heat_unit_spacing) {
+  if (c->equations == ORC_EQ_EULER && c->penal_phi && c->X_metric) {
+    /* => euler_rho_rhou_p_ev_phi_grid_transform */
+    orc_rhs_direct_euler_ev_gt(c, rhs, q_in, c->X_metric, c->X_jacobian, c->penal_phi,
+                               c->penal_inv_phi, qref, shape);
+    return;
+  }
   if (c->equations == ORC_EQ_EULER && c->penal_phi) {
-    fprintf(stderr, "oracle: the penalised Euler modules are not restated\n");
+    /* the reference has no Cartesian euler_rho_rhou_p_ev_phi module */
+    fprintf(stderr, "oracle: no penalised Euler module for Cartesian grids\n");
     abort();
   }
   if (c->equations == ORC_EQ_EULER && c->X_metric) { /* => euler_rho_rhou_p_grid_transform */

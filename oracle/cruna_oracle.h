@@ -105,6 +105,10 @@ void orc_rhs_direct_euler(const orc_cfg *c, double *rhs, const double *q_in, con
 /* euler_rho_rhou_p_grid_transform.f90:33-140 on the deformed grid (Xm, Xj). */
 void orc_rhs_direct_euler_gt(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
                              const double *Xj, const double *qref, const double *shape);
+/* euler_rho_rhou_p_ev_phi_grid_transform.f90:33-160 (penalised, phi and 1/phi). */
+void orc_rhs_direct_euler_ev_gt(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
+                                const double *Xj, const double *phi, const double *inv_phi,
+                                const double *qref, const double *shape);
 /* Volume-penalised deformed-grid RHS, src/equations/ns_rho_rhou_p_ev_phi_grid_transform.f90. */
 void orc_rhs_direct_ev_gt(const orc_cfg *c, double *rhs, const double *q_in, const double *Xm,
                           const double *Xj, const double *phi, const double *inv_phi,

@@ -158,8 +158,8 @@ class Cfg:
         c.gamma, c.R, c.W, c.Pr, c.mu0, c.T0 = gamma, R, W, Pr, mu0, T0
         c.visc_law, c.visc_exp, c.dt = visc_law, visc_exp, dt
         c.equations = int(equations)
-        if c.equations == EQ_EULER and penalization is not None:
-            raise ValueError("the penalised Euler modules are not restated")
+        if c.equations == EQ_EULER and penalization is not None and metric is None:
+            raise ValueError("the reference has no penalised Euler module for Cartesian grids")
         c.nbc = len(bc)
         for r, row in enumerate(bc):
             for j in range(10):

@@ -332,9 +332,11 @@ Penal<float> penal<float>(cruna_hip_ctx *c) { return c->pe32; }
 // set_boundary_condition_rhss zeroes a variable of the adjoint RHS on wall
 // rows (100); the device adjoint does not apply rows to its complete RHS
 const char *kNoWallAdjoint = "wall rows (100) are not implemented in the device adjoint";
-// euler_rho_rhou_p's penalised variants (euler_*_ev_phi*) are separate
-// modules the device does not build
-const char *kEulerCartesian = "the penalised Euler equations are not built on the device";
+// the reference's penalised Euler module exists for deformed grids only
+// (euler_rho_rhou_p_ev_phi_grid_transform; there is no euler_rho_rhou_p_ev_phi)
+const char *kEulerCartesian =
+    "the penalised Euler equations need a metric (the reference has no Cartesian "
+    "euler_rho_rhou_p_ev_phi module)";
 // euler_rho_rhou_p_grid_transform.f90:158-509 has its own metric adjoint,
 // which the device does not build (the direct RHS is the NS one with mu = 0)
 const char *kNoEulerGtAdjoint = "the Euler adjoint on a deformed grid is not built on the device";
@@ -602,6 +604,8 @@ int launch_sweep2(cruna_hip_ctx *c, int mode, bool fuse_filter) {
 // finishes the stage).  fuse_filter: apply filter_i1 inside the final stage.
 template <typename T, typename Q>
 int rhs_stage(cruna_hip_ctx *c, const Q *in, int mode, bool fuse_filter = false) {
+  if (c->ev && !c->gt && c->cfg.equations == CRUNA_HIP_EQ_EULER)
+    return fail(CRUNA_HIP_EINVAL, kEulerCartesian);
   if (int rc = launch_sweep1<T, Q, 0>(c, in)) return rc;
   if (int rc = launch_sweep1<T, Q, 1>(c, in)) return rc;
   if (int rc = launch_sweep1<T, Q, 2>(c, in)) return rc;
@@ -1258,7 +1262,6 @@ int cruna_hip_set_penalization(cruna_hip_ctx *c, const double *phi, const double
     return CRUNA_HIP_OK;
   }
   if (!phi || !inv_phi) return fail(CRUNA_HIP_EINVAL, "phi and inv_phi go together");
-  if (c->cfg.equations == CRUNA_HIP_EQ_EULER) return fail(CRUNA_HIP_EINVAL, kEulerCartesian);
   // on a deformed grid the penalised RHS (ns_rho_rhou_p_ev_phi_grid_transform)
   // runs on the sweeps with the full metric per point
   if (c->gt && !c->curv)

@@ -104,3 +104,34 @@ def test_euler_gt_rhs_pins():
     q0 = np.zeros(n + (5,), order="F")
     q0[..., 0], q0[..., 1], q0[..., 2], q0[..., 3], q0[..., 4] = 1.2, 0.3, -0.2, 0.1, 0.9
     assert np.abs(O.rhs_direct_euler_gt(c, q0, Xm, Xj)).max() < 1e-12
+
+
+def test_euler_ev_phi_gt_rhs_pins():
+    """The penalised grid-transform Euler RHS
+    (euler_rho_rhou_p_ev_phi_grid_transform.f90:33-160): with a varying
+    porosity on a wavy grid it equals the penalised grid-transform NS RHS
+    with mu0 = 0 (pinned by convergence to the penalised Euler equations)
+    to rounding; with porosity 1 it equals the grid-transform Euler RHS; the
+    reference has no Cartesian penalised Euler module."""
+    import pytest
+    from tests.test_oracle_gt import _smooth_state, cfg, wavy_grid
+
+    n = (16, 24, 16)
+    dx, X, Xc = wavy_grid(n)
+    Xm, Xj = O.calc_metric(cfg(n, dx), X, Xc)
+    q = _smooth_state(X)
+    phi = np.asfortranarray(0.7 + 0.2 * np.sin(X[..., 0]) * np.cos(0.4 * X[..., 1]) * np.sin(X[..., 2]))
+    pen = (phi, 1.0 / phi)
+    got = O.rhs_direct(cfg(n, dx, equations=O.EQ_EULER, metric=(Xm, Xj), penalization=pen), q)
+    ns = O.rhs_direct(cfg(n, dx, mu0=0.0, metric=(Xm, Xj), penalization=pen), q)
+    for v in range(5):
+        e = np.linalg.norm(got[..., v] - ns[..., v]) / np.linalg.norm(ns[..., v])
+        assert e < 1e-12, (v, e)
+    one = np.ones(n, order="F")
+    got1 = O.rhs_direct(cfg(n, dx, equations=O.EQ_EULER, metric=(Xm, Xj), penalization=(one, one)), q)
+    ref1 = O.rhs_direct_euler_gt(cfg(n, dx, equations=O.EQ_EULER), q, Xm, Xj)
+    for v in range(5):
+        e = np.linalg.norm(got1[..., v] - ref1[..., v]) / np.linalg.norm(ref1[..., v])
+        assert e < 1e-13, (v, e)
+    with pytest.raises(ValueError, match="Cartesian"):
+        cfg(n, dx, equations=O.EQ_EULER, penalization=pen)
