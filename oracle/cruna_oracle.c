@@ -897,13 +897,21 @@ void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qr
 }
 
 /* set_boundary_condition_rhss called without q (ns_rho_rhou_p.f90:518):
- * type 400 resolves to the empty sponge_rhss (sponge_layer_core.f90:189-196). */
-static void bc_rhss(const orc_cfg *c) {
+ * type 400 resolves to the empty sponge_rhss (sponge_layer_core.f90:189-196);
+ * type 100 zeroes its variable of the adjoint RHS on its box (:319-320). */
+static void bc_rhss(const orc_cfg *c, double *rhs) {
+  const size_t N = npts(c);
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
     /* the cases 204-206 are commented out there (:339-353), so those rows
      * reach case default, which stops the run */
     if (t == 0 || t == 200 || t == 201 || t == 202 || t == 203 || t == 400 || t == 401) continue;
+    if (t == 100) {
+      /* wall_slip_no_thermodynamics(rhss(box), nint(row(8))), wall_boundary_conditions.f90:13-23 */
+      const int v = (int)lround(c->bc[r][7]) - 1;
+      FOR_BOX(c, r, idx) rhs[v * N + idx] = 0.0;
+      continue;
+    }
     bc_unsupported(t, "set_boundary_condition_rhss");
   }
 }
@@ -1638,7 +1646,7 @@ void orc_rhs_adjoint(const orc_cfg *c, double *rhs, const double *qs_in, const d
   if (g)
     for (size_t i = 0; i < 5 * N; ++i) rhs[i] = rhs[i] + g[i]; /* :512 */
   /* sponge_adjoint is the empty in-rhs sponge; rhss BCs without q */
-  bc_rhss(c);
+  bc_rhss(c, rhs);
   free(qs); free(u); free(gdp); free(irho); free(a); free(b); free(cc); free(d); free(e);
   free(tmp);
 }

@@ -329,9 +329,6 @@ Penal<float> penal<float>(cruna_hip_ctx *c) { return c->pe32; }
 
 // ns_rho_rhou_p_ev_phi.f90:295-309: the penalised module's adjoint RHS stops
 // with "not yet implemented"
-// set_boundary_condition_rhss zeroes a variable of the adjoint RHS on wall
-// rows (100); the device adjoint does not apply rows to its complete RHS
-const char *kNoWallAdjoint = "wall rows (100) are not implemented in the device adjoint";
 // the reference's penalised Euler module exists for deformed grids only
 // (euler_rho_rhou_p_ev_phi_grid_transform; there is no euler_rho_rhou_p_ev_phi)
 const char *kEulerCartesian =
@@ -819,11 +816,29 @@ int launch_asweep(cruna_hip_ctx *c, const Q *in, const double *g, int mode) {
 // right_hand_side_adjoint_3d from stage input `in` with the forward state in
 // the device cell q0p: sweeps z, y, x; the x sweep adds g and finishes the
 // stage (mode as rhs_stage).
+// With wall rows (100) the x sweep writes the complete RHS (g added), the
+// rows zero their variable on it (set_boundary_condition_rhss,
+// boundary_conditions.f90:319-320), then the stage update runs point-wise.
 template <typename T, typename Q>
 int arhs_stage(cruna_hip_ctx *c, const Q *in, const double *g, int mode) {
   if (int rc = launch_asweep<T, Q, 2>(c, in, g, mode)) return rc;
   if (int rc = launch_asweep<T, Q, 1>(c, in, g, mode)) return rc;
-  return launch_asweep<T, Q, 0>(c, in, g, mode);
+  if (!c->has_wall) return launch_asweep<T, Q, 0>(c, in, g, mode);
+  if (int rc = launch_asweep<T, Q, 0>(c, in, g, 4)) return rc;
+  const int n1 = c->cfg.n[0], n2 = c->cfg.n[1];
+  for (auto &r : c->bc) {
+    if (r.first != 100) continue;
+    const BoxRow &b = r.second;
+    const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
+    hipLaunchKernelGGL((k_bc_wall_rhs<T>), dim3((np + 255) / 256), dim3(256), 0, c->stream, (T *)c->rhs,
+                       b, (int)b.k[0], n1, n2, c->N);
+    if (int rc = check_launch()) return rc;
+  }
+  if (mode == 4) return CRUNA_HIP_OK;
+  // backward in time, as the sweeps' own update (rk_o4_e.f90:62-105)
+  hipLaunchKernelGGL((k_rk_stage<T>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream, mode,
+                     (const T *)c->rhs, c->qa, c->acc, (T *)c->qs, -c->cfg.dt, c->N);
+  return check_launch();
 }
 
 // An adjoint stage from the FP64 state `from` or, with from == nullptr, from
@@ -1414,7 +1429,6 @@ int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const
   (void)t;  // the adjoint RHS of the reference does not depend on t
   if (!c || !rhs || !qs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
-  if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
   if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
@@ -1445,7 +1459,6 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *
   (void)t;
   if (!c || !qs1 || !qs0 || !q0) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
-  if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
   if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
@@ -1524,7 +1537,6 @@ int cruna_hip_trajectory_record(cruna_hip_ctx *c, int on) {
 int cruna_hip_run_adjoint(cruna_hip_ctx *c, int nt_end, int nsteps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
-  if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
   if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
@@ -1599,7 +1611,6 @@ int cruna_hip_run_direct_subsets(cruna_hip_ctx *c, int subsets, int steps) {
 int cruna_hip_run_adjoint_subsets(cruna_hip_ctx *c, int subsets, int steps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
-  if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
   if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
@@ -1829,7 +1840,6 @@ int cruna_hip_ensemble_set_role(cruna_hip_ensemble *e, int i, int role, int nt) 
   cruna_hip_ctx *c = e->m[i];
   if (role == CRUNA_HIP_ADJOINT) {
     if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
-    if (c->has_wall) return fail(CRUNA_HIP_EINVAL, kNoWallAdjoint);
     if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
     if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
     if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);

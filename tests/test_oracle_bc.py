@@ -276,3 +276,20 @@ def test_adjoint_stops_on_characteristic_rhs_rows():
         cfg = O.Cfg(n, (0.1,) * 3, bc=[row], kvector=kv)
         with pytest.raises(ValueError, match="stops"):
             O.rhs_adjoint(cfg, q, q, shape=shape)
+
+
+def test_adjoint_wall_row_zeroes_its_variable_after_everything_else():
+    """Type 100 in set_boundary_condition_rhss (boundary_conditions.f90:319-320):
+    the adjoint RHS with a wall row is the one without it, with the named
+    variable zero on the row's box only, g included (the row acts last)."""
+    n = (12, 10, 8)
+    q, qref, kv, shape, rng = _zonal_case(n, 24)
+    qs = np.asfortranarray(rng.standard_normal(n + (5,)))
+    g = np.asfortranarray(rng.standard_normal(n + (5,)))
+    kw = dict(dt=DT, scheme=(O.PERIODIC, O.OPEN, O.PERIODIC))
+    base = O.rhs_adjoint(O.Cfg(n, (0.1,) * 3, **kw), qs, q, g=g)
+    got = O.rhs_adjoint(O.Cfg(n, (0.1,) * 3, bc=[(100, 1, 12, 1, 1, 1, 8, 3)], **kw), qs, q, g=g)
+    ref = base.copy(order="F")
+    ref[:, 0, :, 2] = 0.0
+    assert np.array_equal(got, ref)
+    assert np.abs(base[:, 0, :, 2]).max() > 0
