@@ -95,8 +95,7 @@ typedef struct cruna_hip_cfg {
                             mu = lambda = 0; penalised only with a metric,
                             euler_rho_rhou_p_ev_phi_grid_transform, as the
                             reference has no Cartesian penalised Euler module;
-                            its adjoint with sponge rows (400) or on a deformed
-                            grid returns EINVAL) */
+                            its adjoint on a deformed grid returns EINVAL) */
 } cruna_hip_cfg;
 
 typedef struct cruna_hip_ctx cruna_hip_ctx;

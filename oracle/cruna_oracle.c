@@ -7,6 +7,10 @@
  * expressions term by term so that FP64 results agree to rounding.
  * Each function cites the reference file:line it restates (paths are
  * relative to the reference root).
+ *
+ * Parity unpinned: the five entries of sponge_rhss_correct's matrix in
+ * bc_rhss (the Euler adjoint's sponge rows); its tests fix only that shape 0
+ * adds nothing and that the term is linear in the shape.
  */
 #include "cruna_oracle.h"
 
@@ -896,15 +900,34 @@ void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qr
   }
 }
 
-/* set_boundary_condition_rhss called without q (ns_rho_rhou_p.f90:518):
- * type 400 resolves to the empty sponge_rhss (sponge_layer_core.f90:189-196);
- * type 100 zeroes its variable of the adjoint RHS on its box (:319-320). */
-static void bc_rhss(const orc_cfg *c, double *rhs) {
+/* set_boundary_condition_rhss (boundary_conditions.f90:302-377).  Type 100
+ * zeroes its variable of the adjoint RHS on its box (:319-320).  Type 400:
+ * called without q (ns_rho_rhou_p.f90:518, q == NULL here) it resolves to the
+ * empty sponge_rhss (sponge_layer_core.f90:189-196); with q (the Euler
+ * adjoint, euler_rho_rhou_p.f90:384) to sponge_rhss_correct (:107-151), on
+ * qs = the adjoint state after set_boundary_condition_qs. */
+static void bc_rhss(const orc_cfg *c, double *rhs, const double *qs, const double *q,
+                    const double *shape) {
   const size_t N = npts(c);
   for (int r = 0; r < c->nbc; ++r) {
     const int t = bc_type(c, r);
     /* the cases 204-206 are commented out there (:339-353), so those rows
      * reach case default, which stops the run */
+    if (t == 400 && q) {
+      const double gm1 = c->gamma - 1.0;
+      for (size_t i = 0; i < N; ++i) {
+        const double rho = q[i]; /* give_r, give_u */
+        const double u1 = q[N + i] / q[i], u2 = q[2 * N + i] / q[i], u3 = q[3 * N + i] / q[i];
+        double sp[5];
+        sp[0] = 1.0 * qs[i] - u1 / rho * qs[N + i] - u2 / rho * qs[2 * N + i] - u3 / rho * qs[3 * N + i];
+        sp[1] = 1.0 / rho * qs[N + i];
+        sp[2] = 1.0 / rho * qs[2 * N + i];
+        sp[3] = 1.0 / rho * qs[3 * N + i];
+        sp[4] = gm1 * qs[4 * N + i];
+        for (int v = 0; v < 5; ++v) rhs[v * N + i] = rhs[v * N + i] + shape[i] * sp[v];
+      }
+      continue;
+    }
     if (t == 0 || t == 200 || t == 201 || t == 202 || t == 203 || t == 400 || t == 401) continue;
     if (t == 100) {
       /* wall_slip_no_thermodynamics(rhss(box), nint(row(8))), wall_boundary_conditions.f90:13-23 */
@@ -1646,7 +1669,7 @@ void orc_rhs_adjoint(const orc_cfg *c, double *rhs, const double *qs_in, const d
   if (g)
     for (size_t i = 0; i < 5 * N; ++i) rhs[i] = rhs[i] + g[i]; /* :512 */
   /* sponge_adjoint is the empty in-rhs sponge; rhss BCs without q */
-  bc_rhss(c, rhs);
+  bc_rhss(c, rhs, qs, c->equations == ORC_EQ_EULER ? q : NULL, shape);
   free(qs); free(u); free(gdp); free(irho); free(a); free(b); free(cc); free(d); free(e);
   free(tmp);
 }

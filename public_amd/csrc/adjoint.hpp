@@ -227,6 +227,32 @@ __global__ __launch_bounds__(256) void k_bc_characteristic_qs(Q *__restrict__ qs
   qs[4 * N + o] = Q(s5 + dps);
 }
 
+// sponge_rhss_correct (src/boundaries/sponge_layer_core.f90:107-151), the
+// adjoint sponge of rows 400 when set_boundary_condition_rhss gets the
+// forward state (the Euler adjoint, euler_rho_rhou_p.f90:384):
+// rhss += sigma (qs1 - u.qs(2:4)/rho, qs(2:4)/rho, (gamma - 1) qs5), with qs
+// the adjoint state the RHS was computed from and q the forward state.
+template <typename T, typename Q>
+__global__ __launch_bounds__(256) void k_sponge_rhss(T *__restrict__ rhs, const Q *__restrict__ qs,
+                                                      const double *const *__restrict__ q0p,
+                                                      const double *__restrict__ sigma, double gamma,
+                                                      long N) {
+  const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= N) return;
+  const double *__restrict__ q = *q0p;
+  const double rho = q[o];
+  const double u1 = q[N + o] / rho, u2 = q[2 * N + o] / rho, u3 = q[3 * N + o] / rho;
+  const double s1 = double(qs[o]), s2 = double(qs[N + o]), s3 = double(qs[2 * N + o]),
+               s4 = double(qs[3 * N + o]), s5 = double(qs[4 * N + o]);
+  const double sg = sigma[o];
+  const double r1 = 1.0 * s1 - u1 / rho * s2 - u2 / rho * s3 - u3 / rho * s4;
+  rhs[o] = T(double(rhs[o]) + sg * r1);
+  rhs[N + o] = T(double(rhs[N + o]) + sg * (1.0 / rho * s2));
+  rhs[2 * N + o] = T(double(rhs[2 * N + o]) + sg * (1.0 / rho * s3));
+  rhs[3 * N + o] = T(double(rhs[3 * N + o]) + sg * (1.0 / rho * s4));
+  rhs[4 * N + o] = T(double(rhs[4 * N + o]) + sg * ((gamma - 1.0) * s5));
+}
+
 // sponge_qs (src/boundaries/sponge_layer_core.f90:155-167): qs <- qs - sigma qs.
 template <typename Q>
 __global__ __launch_bounds__(256) void k_bc_sponge_qs(Q *__restrict__ qs,

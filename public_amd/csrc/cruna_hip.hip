@@ -337,10 +337,6 @@ const char *kEulerCartesian =
 // euler_rho_rhou_p_grid_transform.f90:158-509 has its own metric adjoint,
 // which the device does not build (the direct RHS is the NS one with mu = 0)
 const char *kNoEulerGtAdjoint = "the Euler adjoint on a deformed grid is not built on the device";
-// euler_rho_rhou_p.f90:173 passes q to set_boundary_condition_rhss, so its
-// adjoint applies sponge_rhss on 400 rows, which the device adjoint does not
-const char *kNoEulerSpongeAdjoint =
-    "the Euler adjoint with sponge rows (400) is not implemented on the device";
 // set_boundary_condition_rhss (boundary_conditions.f90:339-353) has its
 // cases 204-206 commented out, so the adjoint reaches case default and stops
 const char *kNoCharRhsAdjoint =
@@ -816,22 +812,34 @@ int launch_asweep(cruna_hip_ctx *c, const Q *in, const double *g, int mode) {
 // right_hand_side_adjoint_3d from stage input `in` with the forward state in
 // the device cell q0p: sweeps z, y, x; the x sweep adds g and finishes the
 // stage (mode as rhs_stage).
-// With wall rows (100) the x sweep writes the complete RHS (g added), the
-// rows zero their variable on it (set_boundary_condition_rhss,
-// boundary_conditions.f90:319-320), then the stage update runs point-wise.
+// With wall rows (100), or sponge rows (400) in the Euler adjoint, the x
+// sweep writes the complete RHS (g added) and set_boundary_condition_rhss
+// (boundary_conditions.f90:302-377) applies the rows to it in order: 100
+// zeroes its variable, 400 adds sponge_rhss (the Euler adjoint passes the
+// forward state, euler_rho_rhou_p.f90:384; the NS adjoint does not, and
+// there 400 is the empty sponge_rhss); then the stage update runs point-wise.
 template <typename T, typename Q>
 int arhs_stage(cruna_hip_ctx *c, const Q *in, const double *g, int mode) {
   if (int rc = launch_asweep<T, Q, 2>(c, in, g, mode)) return rc;
   if (int rc = launch_asweep<T, Q, 1>(c, in, g, mode)) return rc;
-  if (!c->has_wall) return launch_asweep<T, Q, 0>(c, in, g, mode);
+  const bool euler_sponge = c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs;
+  if (!c->has_wall && !euler_sponge) return launch_asweep<T, Q, 0>(c, in, g, mode);
   if (int rc = launch_asweep<T, Q, 0>(c, in, g, 4)) return rc;
   const int n1 = c->cfg.n[0], n2 = c->cfg.n[1];
   for (auto &r : c->bc) {
-    if (r.first != 100) continue;
-    const BoxRow &b = r.second;
-    const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
-    hipLaunchKernelGGL((k_bc_wall_rhs<T>), dim3((np + 255) / 256), dim3(256), 0, c->stream, (T *)c->rhs,
-                       b, (int)b.k[0], n1, n2, c->N);
+    if (r.first == 400 && euler_sponge) {
+      if (!c->shape) return fail(CRUNA_HIP_EINVAL, "sponge rows need a shape (cruna_hip_set_sponge_shape)");
+      hipLaunchKernelGGL((k_sponge_rhss<T, Q>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream,
+                         (T *)c->rhs, in, (const double *const *)c->q0p, (const double *)c->shape,
+                         c->cfg.gamma, c->N);
+    } else if (r.first == 100) {
+      const BoxRow &b = r.second;
+      const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
+      hipLaunchKernelGGL((k_bc_wall_rhs<T>), dim3((np + 255) / 256), dim3(256), 0, c->stream,
+                         (T *)c->rhs, b, (int)b.k[0], n1, n2, c->N);
+    } else {
+      continue;
+    }
     if (int rc = check_launch()) return rc;
   }
   if (mode == 4) return CRUNA_HIP_OK;
@@ -1431,7 +1439,6 @@ int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
-  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (int rc = stage_adjoint_inputs(c, qs, q, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
   if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
@@ -1461,7 +1468,6 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
-  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (int rc = stage_adjoint_inputs(c, qs0, q0, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
   if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
@@ -1539,7 +1545,6 @@ int cruna_hip_run_adjoint(cruna_hip_ctx *c, int nt_end, int nsteps) {
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
-  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
   if (!c->qa) return fail(CRUNA_HIP_EINVAL, "no adjoint state (cruna_hip_set_adjoint_state)");
   if (c->traj_slots == 0) return fail(CRUNA_HIP_EINVAL, "no trajectory (cruna_hip_trajectory_alloc)");
@@ -1613,7 +1618,6 @@ int cruna_hip_run_adjoint_subsets(cruna_hip_ctx *c, int subsets, int steps) {
   if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
   if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
   if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
-  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
   if (subsets < 1 || steps < 1) return fail(CRUNA_HIP_EINVAL, "subsets and steps must be >= 1");
   if (c->ckpt_n < subsets + 1) return fail(CRUNA_HIP_EINVAL, "not enough subset checkpoints (cruna_hip_subsets_alloc)");
   if (c->traj_slots < steps) return fail(CRUNA_HIP_EINVAL, "the trajectory holds fewer slots than steps");
@@ -1842,7 +1846,6 @@ int cruna_hip_ensemble_set_role(cruna_hip_ensemble *e, int i, int role, int nt) 
     if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
     if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
     if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
-    if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs) return fail(CRUNA_HIP_EINVAL, kNoEulerSpongeAdjoint);
     if (!c->qa || c->traj_slots == 0)
       return fail(CRUNA_HIP_EINVAL, "an adjoint member needs its adjoint state and trajectory first");
     if (c->traj_slots > 1 && (nt < 1 || nt > c->traj_slots))

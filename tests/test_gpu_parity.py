@@ -1455,3 +1455,46 @@ def test_penalised_euler_on_deformed_grid_matches_oracle():
     for v in range(5):
         assert rel(got_r[..., v], ref_r[..., v]) < 1e-12, (v, rel(got_r[..., v], ref_r[..., v]))
         assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+
+
+def test_euler_adjoint_with_sponge_rows_matches_oracle():
+    """The Euler adjoint's sponge rows (400: sponge_rhss_correct on the
+    complete adjoint RHS, as euler_rho_rhou_p.f90:384 passes the forward
+    state) with characteristic rows (201) on both y faces: the adjoint RHS
+    and a 3-step adjoint run over the recorded trajectory against the
+    oracle."""
+    from public_amd import EQ_EULER
+    Cruna = _gpu()
+    shape = (32, 48, 16)
+    q, dx, kw, shape_fn = _rows_case(shape, 11)
+    n1, n2, n3 = shape
+    bc = [(201, 1, n1, 1, 1, 1, n3, 0, -1, 0), (201, 1, n1, n2, n2, 1, n3, 0, 1, 0),
+          (400, 1, n1, 1, n2, 1, n3)]
+    kwe = dict(kw, bc=bc)
+    cfg = O.Cfg(shape, dx, filter_freq=(1, 1, 1), afilter_freq=(1, 1, 1), equations=O.EQ_EULER, **kwe)
+    qs = np.asfortranarray(np.random.default_rng(3).standard_normal(shape + (5,)))
+    ref = O.rhs_adjoint(cfg, qs, q, shape=shape_fn)
+    free = O.rhs_adjoint(O.Cfg(shape, dx, equations=O.EQ_EULER, **dict(kwe, bc=bc[:2])), qs, q)
+    assert rel(ref, free) > 1e-6  # the sponge term is there
+    nsteps = 3
+    Q = np.zeros(shape + (5, nsteps), order="F")
+    s = q
+    for nt in range(1, nsteps + 1):
+        s = O.run_direct(cfg, s, 1, nt0=nt - 1, qref=q, shape=shape_fn)
+        Q[..., nt - 1] = s
+    ref_a = O.run_adjoint(cfg, qs, Q, shape=shape_fn)
+    with Cruna(shape, dx, filter_freq=(1, 1, 1), afilter_freq=(1, 1, 1), equations=EQ_EULER, **kwe) as g:
+        g.set_qref(q)
+        g.set_sponge_shape(shape_fn)
+        got = g.right_hand_side_adjoint_3d(qs, q)
+        g.trajectory_alloc(nsteps)
+        g.trajectory_record(True)
+        g.set_state(q)
+        g.run_direct(nsteps)
+        g.set_adjoint_state(qs)
+        g.run_adjoint(nsteps)
+        g.synchronize()
+        got_a = g.get_adjoint_state()
+    for v in range(5):
+        assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
+        assert rel(got_a[..., v], ref_a[..., v]) < 1e-12, (v, rel(got_a[..., v], ref_a[..., v]))

@@ -135,3 +135,30 @@ def test_euler_ev_phi_gt_rhs_pins():
         assert e < 1e-13, (v, e)
     with pytest.raises(ValueError, match="Cartesian"):
         cfg(n, dx, equations=O.EQ_EULER, penalization=pen)
+
+
+def test_euler_adjoint_sponge_rows():
+    """Type 400 in the Euler adjoint: set_boundary_condition_rhss gets the
+    forward state (euler_rho_rhou_p.f90:384), so the row adds
+    sponge_rhss_correct (sponge_layer_core.f90:107-151); the NS adjoint calls
+    it without q, where the row is the empty sponge_rhss.  Pinned: shape 0
+    gives the row-free adjoint RHS exactly, the added term is linear in the
+    shape, and the NS adjoint ignores the row.  The matrix entries
+    themselves (:131-145) have no independent check here."""
+    n = 16
+    h, G = _box(n)
+    q = _state(G)
+    rng = np.random.default_rng(5)
+    qs = np.asfortranarray(rng.standard_normal(q.shape))
+    sig = np.asfortranarray(0.3 * rng.random((n,) * 3))
+    zero = np.zeros((n,) * 3, order="F")
+    row = [(400, 1, n, 1, n, 1, n)]
+    eu = dict(equations=O.EQ_EULER)
+    free = O.rhs_adjoint(O.Cfg((n,) * 3, (h,) * 3, **eu), qs, q)
+    assert np.array_equal(O.rhs_adjoint(O.Cfg((n,) * 3, (h,) * 3, bc=row, **eu), qs, q, shape=zero), free)
+    r1 = O.rhs_adjoint(O.Cfg((n,) * 3, (h,) * 3, bc=row, **eu), qs, q, shape=sig)
+    r2 = O.rhs_adjoint(O.Cfg((n,) * 3, (h,) * 3, bc=row, **eu), qs, q, shape=np.asfortranarray(2 * sig))
+    assert np.abs((r2 - free) - 2 * (r1 - free)).max() < 1e-12 * np.abs(r1 - free).max()
+    assert np.abs(r1 - free).max() > 1e-3
+    ns_free = O.rhs_adjoint(O.Cfg((n,) * 3, (h,) * 3), qs, q)
+    assert np.array_equal(O.rhs_adjoint(O.Cfg((n,) * 3, (h,) * 3, bc=row), qs, q, shape=sig), ns_free)
