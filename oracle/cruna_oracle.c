@@ -6,11 +6,11 @@
  * cpu_baseline leg).  Expression order follows the Fortran array
  * expressions term by term so that FP64 results agree to rounding.
  * Each function cites the reference file:line it restates (paths are
- * relative to the reference root).
- *
- * Parity unpinned: the five entries of sponge_rhss_correct's matrix in
- * bc_rhss (the Euler adjoint's sponge rows); its tests fix only that shape 0
- * adds nothing and that the term is linear in the shape.
+ * relative to the reference root).  Every function is pinned by a test in
+ * tests/test_oracle*.py: against the reference's own compiled leaf modules
+ * where they build here (cores, closures, characteristic and sponge rows
+ * including sponge_rhss_correct), otherwise by closed forms, invariants and
+ * convergence to an independent discretisation.
  */
 #include "cruna_oracle.h"
 

@@ -144,7 +144,8 @@ def test_euler_adjoint_sponge_rows():
     it without q, where the row is the empty sponge_rhss.  Pinned: shape 0
     gives the row-free adjoint RHS exactly, the added term is linear in the
     shape, and the NS adjoint ignores the row.  The matrix entries
-    themselves (:131-145) have no independent check here."""
+    themselves (:131-145) are pinned against the reference's compiled
+    routine in test_oracle_pins.py."""
     n = 16
     h, G = _box(n)
     q = _state(G)
