@@ -141,7 +141,7 @@ def run_shear_layer(steps, warmup, n=(512, 256, 128), seed=0):
     components, Mach 0.5 on the velocity difference (c = 2), uniform T and p,
     Re 1000; periodic x/z, open y with characteristic rows (201) on both y
     faces and a sponge on the RHS (400, length 4 in y, amplitude 1)."""
-    from public_amd import FP32_MIXED, OPEN, PERIODIC, SUTHERLAND, Cruna
+    from public_amd import FP32_MIXED, FP64, OPEN, PERIODIC, SUTHERLAND, Cruna, ic
     from public_amd import geometry as geo
     L = (60.0, 40.0, 30.0)
     x0, x1 = (0.0, -L[1] / 2, 0.0), (L[0], L[1] / 2, L[2])
@@ -167,12 +167,16 @@ def run_shear_layer(steps, warmup, n=(512, 256, 128), seed=0):
     with Cruna(n, dx, **kw) as g64:  # metric from FP64 compact derivatives
         M, J = geo.calc_metric(g64, X, Xc)
     sigma = geo.sponge_shape(X, x0, x1, (-1.0, 4.0, -1.0), 1.0)
-    g = Cruna(n, dx, precision=FP32_MIXED, **kw)
-    g.set_metric(M, J)
-    del M, J
-    g.set_qref(q)
-    g.set_sponge_shape(sigma)
-    g.set_state(q)
+
+    def make(precision):
+        g = Cruna(n, dx, precision=precision, **kw)
+        g.set_metric(M, J)
+        g.set_qref(q)
+        g.set_sponge_shape(sigma)
+        g.set_state(q)
+        return g
+
+    g = make(FP32_MIXED)
     g.run_direct(warmup)
     g.synchronize()
     t0 = time.perf_counter()
@@ -181,12 +185,33 @@ def run_shear_layer(steps, warmup, n=(512, 256, 128), seed=0):
     t1 = time.perf_counter()
     qe = g.get_state()
     g.close()
+    # the same run in FP64 (untimed) for the error of the mixed mode, with
+    # the kinetic-energy history of both sampled every 10 steps
+    total = warmup + steps
+    hist = {}
+    for prec in (FP64, FP32_MIXED):
+        g = make(prec)
+        ke, done = [ic.kinetic_energy(q)], 0
+        while done < total:
+            k = min(10, total - done)
+            g.run_direct(k, nt0=done)
+            done += k
+            ke.append(ic.kinetic_energy(g.get_state()))
+        hist[prec] = (np.array(ke), g.get_state())
+        g.close()
+    ref = hist[FP64][1]
+    err = rel_errors(qe, ref, gamma)
+    ke64, ke32 = hist[FP64][0], hist[FP32_MIXED][0]
+    err["kinetic_energy_history"] = float(np.linalg.norm(ke32 - ke64) / np.linalg.norm(ke64))
+    err["after_steps"] = total
+    err["same_as_timed_run"] = bool(np.array_equal(hist[FP32_MIXED][1], qe))
     pts = float(n1) * n2 * n3
     return {"value": round(pts * steps / (t1 - t0) / 1e6, 2), "unit": "Mpts/s",
             "ms_per_step": round((t1 - t0) / steps * 1e3, 4), "steps": steps, "warmup": warmup,
             "grid": list(n), "dt": dt, "dtype": "fp32-mixed (f32 RHS, f64 state/RK/boundaries)",
             "finite": bool(np.isfinite(qe).all()),
-            "max_abs_v": float(np.abs(qe[..., 2] / qe[..., 0]).max())}
+            "max_abs_v": float(np.abs(qe[..., 2] / qe[..., 0]).max()),
+            "err_vs_fp64": err}
 
 
 def run_curvilinear(steps, warmup, n=256, seed=0, mach=0.3, amp=0.1):
@@ -242,40 +267,31 @@ def run_curvilinear(steps, warmup, n=256, seed=0, mach=0.3, amp=0.1):
             "kernels": kernels, "finite": bool(np.isfinite(qe).all())}
 
 
-def run_ensemble(steps, nmembers=32, seed=0, autotune=True):
-    """Ensemble of periodic boxes with N drawn from {48, 64, 96, 128}^3 and Ma
-    from U[0.05, 0.3] (seeded): the first half forward RK4 runs, the second
-    half adjoint runs backward over their own forward trajectories of `steps`
-    steps (recorded on the device beforehand, untimed).  Timed: all members
-    advanced by `steps` steps, (a) one member after the other, eagerly, and
-    (b) together through the replayed graph of cruna_hip_ensemble_run."""
-    from public_amd import ADJOINT, FORWARD, Cruna, Ensemble, ic
-    rng = np.random.default_rng(seed)
-    sizes = rng.choice([48, 64, 96, 128], nmembers)
-    machs = rng.uniform(0.05, 0.3, nmembers)
-    specs, states = [], []
-    for i in range(nmembers):
-        n = int(sizes[i])
-        q = ic.periodic_box_state(n, seed + i, float(machs[i]))
-        h = ic.periodic_dx(n)
-        kw = dict(n=(n,) * 3, dx=(h,) * 3, mu0=1.0 / 1600.0, T0=1.0 / (1.4 * machs[i] ** 2),
-                  dt=ic.cfl_dt(q, h), afilter_freq=(1, 1, 1))
-        specs.append(kw)
-        states.append(q)
-    adj = set(range(nmembers // 2, nmembers))
+def run_ensemble(steps, nmembers=32, seed=0, autotune=True, subset_steps=10):
+    """The batched workload of public_amd.workloads: periodic boxes with N
+    drawn from {48, 64, 96, 128}^3 and Ma from U[0.05, 0.3] (seeded); the
+    first half forward RK4 runs, the second half adjoint runs backward over
+    forward trajectories checkpointed every `subset_steps` steps (the
+    checkpoints are written by an untimed forward run; the adjoint recomputes
+    each subset from its checkpoint inside the timed region).  Timed: all
+    members advanced by `steps` steps, (a) one member after the other,
+    eagerly, and (b) together through cruna_hip_ensemble_run's replayed
+    member graphs.  Mpts/s counts N^3 per member and step (an adjoint step's
+    recompute is not counted as work)."""
+    from public_amd import ADJOINT, FORWARD, Cruna, Ensemble, workloads
+    specs, states, adj = workloads.ensemble_members(nmembers, seed)
+    sizes = [s["n"][0] for s in specs]
     pts = float(sum(int(s) ** 3 for s in sizes))
+    subsets = steps // subset_steps
 
     def prepare(get_member):
         for i in range(nmembers):
             g = get_member(i)
             g.set_state(states[i])
             if i in adj:
-                g.trajectory_alloc(steps)
-                g.trajectory_record(True)
-                g.run_direct(steps)
-                g.trajectory_record(False)
-                g.set_adjoint_state(np.asfortranarray(np.random.default_rng(seed + 100 + i)
-                                                      .standard_normal(states[i].shape)))
+                g.subsets_alloc(subsets, subset_steps)
+                g.run_direct_subsets(subsets, subset_steps)
+                g.set_adjoint_state(workloads.final_adjoint_state(states[i].shape, seed, i))
                 g.synchronize()
 
     # (a) one after the other, eagerly
@@ -287,7 +303,7 @@ def run_ensemble(steps, nmembers=32, seed=0, autotune=True):
     t0 = time.perf_counter()
     for i in range(nmembers):
         if i in adj:
-            solo[i].run_adjoint(steps, steps)
+            solo[i].run_adjoint_subsets(subsets, subset_steps)
         else:
             solo[i].run_direct(steps)
         solo[i].synchronize()
@@ -307,12 +323,13 @@ def run_ensemble(steps, nmembers=32, seed=0, autotune=True):
         same = all(np.array_equal(ens[i].get_adjoint_state() if i in adj else ens[i].get_state(), ref[i])
                    for i in range(nmembers))
     return {"members": nmembers, "sizes": [int(s) for s in sizes], "forward": nmembers - len(adj),
-            "adjoint": len(adj), "steps": steps, "points_per_step": pts,
+            "adjoint": len(adj), "steps": steps, "checkpoint_every": subset_steps,
+            "points_per_step": pts,
             "serial_eager": {"value": round(pts * steps / t_serial / 1e6, 2), "unit": "Mpts/s",
                              "seconds": round(t_serial, 4)},
             "graph": {"value": round(pts * steps / t_graph / 1e6, 2), "unit": "Mpts/s",
                       "seconds": round(t_graph, 4),
-                      "note": "includes the first (eager) step and recording the graph"},
+                      "note": "includes the first (eager) period and recording the graphs"},
             "speedup": round(t_serial / t_graph, 3), "tile_lines": tuned,
             "identical_to_serial": bool(same)}
 

@@ -196,6 +196,19 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *ctx, double *qs1, const double
 int cruna_hip_run_direct(cruna_hip_ctx *ctx, int nt0, int nsteps);
 int cruna_hip_synchronize(cruna_hip_ctx *ctx);
 
+/* Convergence check of the cruna_hpc loop (src/main/cruna_hpc.f90:185-208,
+ * run.cfreq and run.ctol read at :81-82): at steps with mod(nt, cfreq) == 0,
+ * residual = maxval(abs((qn - q1)/dt)) over every point and variable right
+ * after the RK4 step; when residual <= ctol, cruna_hip_run_direct ends after
+ * that step with the unfiltered q1 as the device state (the state the
+ * reference stores as its snapshot before stop_cruna).  cfreq 0 (the
+ * default) disables the check.  Each check waits for the step to finish (the
+ * reference's allreduce).  get_residual: the last residual computed (-1 if
+ * none), the step it was computed at, and the step at which the last
+ * cruna_hip_run_direct reached the tolerance (0 if it did not). */
+int cruna_hip_set_convergence(cruna_hip_ctx *ctx, int cfreq, double ctol);
+int cruna_hip_get_residual(cruna_hip_ctx *ctx, double *residual, int *nt_checked, int *nt_converged);
+
 /* Device-resident adjoint state qs^n (layout as q). */
 int cruna_hip_set_adjoint_state(cruna_hip_ctx *ctx, const double *qs);
 int cruna_hip_get_adjoint_state(cruna_hip_ctx *ctx, double *qs);

@@ -15,6 +15,9 @@
 #include "cruna_oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -470,13 +473,17 @@ static void gather_pencils(const orc_cfg *c, int dir, const double *in, double *
     memcpy(pen, in, (size_t)n1 * n2 * n3 * sizeof(double));
     return;
   }
+  (void)line;
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < (dir == 2 ? n2 : n3); ++k)
-    for (int i = 0; i < n1; ++i, ++line)
+    for (int i = 0; i < n1; ++i) {
+      const size_t ln = (size_t)k * n1 + i;
       for (int p = 0; p < nd; ++p) {
         size_t idx = (dir == 1) ? (size_t)i + (size_t)n1 * (p + (size_t)n2 * k)
                                 : (size_t)i + (size_t)n1 * (k + (size_t)n2 * p);
-        pen[line * nd + p] = in[idx];
+        pen[ln * nd + p] = in[idx];
       }
+    }
 }
 
 static void scatter_pencils(const orc_cfg *c, int dir, const double *pen, double *out) {
@@ -487,13 +494,17 @@ static void scatter_pencils(const orc_cfg *c, int dir, const double *pen, double
     memcpy(out, pen, (size_t)n1 * n2 * n3 * sizeof(double));
     return;
   }
+  (void)line;
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < (dir == 2 ? n2 : n3); ++k)
-    for (int i = 0; i < n1; ++i, ++line)
+    for (int i = 0; i < n1; ++i) {
+      const size_t ln = (size_t)k * n1 + i;
       for (int p = 0; p < nd; ++p) {
         size_t idx = (dir == 1) ? (size_t)i + (size_t)n1 * (p + (size_t)n2 * k)
                                 : (size_t)i + (size_t)n1 * (k + (size_t)n2 * p);
-        out[idx] = pen[line * nd + p];
+        out[idx] = pen[ln * nd + p];
       }
+    }
 }
 
 /* src/diff/diff_i1_mpi_cartesian.f90:83-103 (and i2/i3). */
@@ -501,14 +512,30 @@ void orc_diff3d(const orc_cfg *c, int dir, double *out, const double *in, double
   const size_t N = npts(c);
   double *pen = xalloc(N);
   gather_pencils(c, dir, in, pen);
-  if (c->diff_kind[dir] == ORC_DIFF_O4_EXPLICIT)
-    orc_diff_core_o4(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, h);
-  else if (c->diff_kind[dir] == ORC_DIFF_O4_PADE)
-    orc_diff_core_pade(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, h);
-  else if (c->diff_kind[dir] == ORC_DIFF_O2_EXPLICIT)
-    orc_diff_core_o2(c->n[dir], (int)(N / c->n[dir]), pen, h);
-  else
-    orc_diff_core(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, h);
+  const int nd = c->n[dir];
+  const long nl = (long)(N / nd);
+  /* the lines are independent: each thread runs the core on its own share */
+#pragma omp parallel
+  {
+    long l0 = 0, l1 = nl;
+#ifdef _OPENMP
+    const long nt = omp_get_num_threads(), t = omp_get_thread_num();
+    l0 = nl * t / nt;
+    l1 = nl * (t + 1) / nt;
+#endif
+    double *p = pen + (size_t)l0 * nd;
+    const int m = (int)(l1 - l0);
+    if (m > 0) {
+      if (c->diff_kind[dir] == ORC_DIFF_O4_EXPLICIT)
+        orc_diff_core_o4(c->scheme[dir], nd, m, p, h);
+      else if (c->diff_kind[dir] == ORC_DIFF_O4_PADE)
+        orc_diff_core_pade(c->scheme[dir], nd, m, p, h);
+      else if (c->diff_kind[dir] == ORC_DIFF_O2_EXPLICIT)
+        orc_diff_core_o2(nd, m, p, h);
+      else
+        orc_diff_core(c->scheme[dir], nd, m, p, h);
+    }
+  }
   scatter_pencils(c, dir, pen, out);
   free(pen);
 }
@@ -519,7 +546,19 @@ void orc_filter4d(const orc_cfg *c, int dir, double *q, int nv) {
   double *pen = xalloc(N);
   for (int v = 0; v < nv; ++v) {
     gather_pencils(c, dir, q + v * N, pen);
-    orc_filter_core(c->scheme[dir], c->n[dir], (int)(N / c->n[dir]), pen, c->filter_alpha);
+    const int nd = c->n[dir];
+    const long nl = (long)(N / nd);
+#pragma omp parallel
+    {
+      long l0 = 0, l1 = nl;
+#ifdef _OPENMP
+      const long nt = omp_get_num_threads(), t = omp_get_thread_num();
+      l0 = nl * t / nt;
+      l1 = nl * (t + 1) / nt;
+#endif
+      if (l1 > l0)
+        orc_filter_core(c->scheme[dir], nd, (int)(l1 - l0), pen + (size_t)l0 * nd, c->filter_alpha);
+    }
     scatter_pencils(c, dir, pen, q + v * N);
   }
   free(pen);
@@ -776,6 +815,7 @@ static void bc_q_at(const orc_cfg *c, double *q, const double *qref, const doubl
       /* characteristic_q_k4d(q, qref, kvector, shape) on the whole domain, :93-108 */
       if (!zone_applies(c, outside)) continue;
       const double *kv = c->kvector;
+      #pragma omp parallel for schedule(static)
       for (size_t i = 0; i < N; ++i) {
         const double k[3] = {kv[i], kv[N + i], kv[2 * N + i]};
         char_q(1, (int)N, q + i, qref + i, k, c->gamma, shape[i]);
@@ -800,6 +840,7 @@ static void bc_q_at(const orc_cfg *c, double *q, const double *qref, const doubl
     } else if (t == 401) {
       /* sponge_q with qref, src/boundaries/sponge_layer_core.f90:70-103 */
       for (int v = 0; v < NV; ++v)
+        #pragma omp parallel for schedule(static)
         for (size_t i = 0; i < N; ++i)
           q[v * N + i] = q[v * N + i] - shape[i] * (q[v * N + i] - qref[v * N + i]);
     } else {
@@ -823,6 +864,7 @@ static void bc_qs_at(const orc_cfg *c, double *qs, const double *q, const double
       /* characteristic_qs_k4d(qs, q, kvector, shape) on the whole domain, :181-194 */
       if (!zone_applies(c, outside)) continue;
       const double *kv = c->kvector;
+      #pragma omp parallel for schedule(static)
       for (size_t i = 0; i < N; ++i) {
         const double k[3] = {kv[i], kv[N + i], kv[2 * N + i]};
         char_qs(1, (int)N, qs + i, q + i, k, c->gamma, shape[i]);
@@ -847,6 +889,7 @@ static void bc_qs_at(const orc_cfg *c, double *qs, const double *q, const double
     } else if (t == 401) {
       /* sponge_qs, sponge_layer_core.f90:155-167 */
       for (int v = 0; v < NV; ++v)
+        #pragma omp parallel for schedule(static)
         for (size_t i = 0; i < N; ++i) qs[v * N + i] = qs[v * N + i] - shape[i] * qs[v * N + i];
     } else {
       bc_unsupported(t, "set_boundary_condition_qs");
@@ -868,6 +911,7 @@ void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qr
     if (t == 206) {
       /* characteristic_rhs_k4d(rhs, q, qref, kvector, shape) on the whole domain, :276-279 */
       const double *kv = c->kvector;
+      #pragma omp parallel for schedule(static)
       for (size_t i = 0; i < N; ++i) {
         const double k[3] = {kv[i], kv[N + i], kv[2 * N + i]};
         char_rhs(1, (int)N, rhs + i, q + i, qref + i, k, c->gamma, c->dt, shape + i);
@@ -877,6 +921,7 @@ void orc_bc_rhs(const orc_cfg *c, double *rhs, const double *q, const double *qr
     if (t == 400) {
       /* sponge_rhs with qref, sponge_layer_core.f90:34-66 */
       for (int v = 0; v < NV; ++v)
+        #pragma omp parallel for schedule(static)
         for (size_t i = 0; i < N; ++i)
           rhs[v * N + i] = rhs[v * N + i] - shape[i] * (q[v * N + i] - qref[v * N + i]);
     } else if (t == 100) {
@@ -915,6 +960,7 @@ static void bc_rhss(const orc_cfg *c, double *rhs, const double *qs, const doubl
      * reach case default, which stops the run */
     if (t == 400 && q) {
       const double gm1 = c->gamma - 1.0;
+      #pragma omp parallel for schedule(static)
       for (size_t i = 0; i < N; ++i) {
         const double rho = q[i]; /* give_r, give_u */
         const double u1 = q[N + i] / q[i], u2 = q[2 * N + i] / q[i], u3 = q[3 * N + i] / q[i];
@@ -982,21 +1028,27 @@ void orc_rhs_direct_euler(const orc_cfg *c, double *rhs, const double *q_in, con
   memcpy(q, q_in, 5 * N * sizeof(double)); /* :53 */
   orc_bc_q(c, q, qref, shape);              /* :56 */
   for (int m = 0; m < 3; ++m)               /* :59-63 */
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) u[m * N + i] = q[(m + 1) * N + i] / q[i];
   for (int k = 0; k < 3; ++k) orc_diff3d(c, k, a[k], q + (k + 1) * N, dxs[k]); /* mass :66-69 */
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) r[0][i] = -a[0][i] - a[1][i] - a[2][i];
   for (int m = 0; m < 3; ++m) { /* momentum :73-95 */
     for (int k = 0; k < 3; ++k) {
+      #pragma omp parallel for schedule(static)
       for (size_t i = 0; i < N; ++i) tmp[i] = q[(k + 1) * N + i] * u[m * N + i];
       orc_diff3d(c, k, a[k], tmp, dxs[k]);
     }
     orc_diff3d(c, m, pg[m], q + 4 * N, dxs[m]);
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) r[m + 1][i] = -a[0][i] - a[1][i] - a[2][i] - pg[m][i];
   }
   for (int k = 0; k < 3; ++k) { /* pressure :97-102 */
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) tmp[i] = q[4 * N + i] * u[k * N + i];
     orc_diff3d(c, k, a[k], tmp, dxs[k]);
   }
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) {
     r[4][i] = -c->gamma * (a[0][i] + a[1][i] + a[2][i]);
     r[4][i] = r[4][i] + (c->gamma - 1.0) * (u[i] * pg[0][i] + u[N + i] * pg[1][i] + u[2 * N + i] * pg[2][i]);
@@ -1031,39 +1083,48 @@ static void euler_gt_impl(const orc_cfg *c, double *rhs, const double *q_in, con
   memcpy(q, q_in, 5 * N * sizeof(double)); /* :59 */
   orc_bc_q(c, q, qref, shape);              /* :62 */
   for (int m = 0; m < 3; ++m)               /* :65-69 */
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) u[m * N + i] = q[(m + 1) * N + i] / q[i];
   for (int d = 0; d < 3; ++d) {             /* :75-77 */
     const double *m0 = MET_E(Xm, d, 0, N), *m1 = MET_E(Xm, d, 1, N), *m2 = MET_E(Xm, d, 2, N);
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) uh[d * N + i] = m0[i] * u[i] + m1[i] * u[N + i] + m2[i] * u[2 * N + i];
   }
   for (int d = 0; d < 3; ++d) orc_diff3d(c, d, a[d], q + 4 * N, dxs[d]); /* :80-85 */
   for (int j = 0; j < 3; ++j) {
     const double *m0 = MET_E(Xm, 0, j, N), *m1 = MET_E(Xm, 1, j, N), *m2 = MET_E(Xm, 2, j, N);
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) pg[j][i] = m0[i] * a[0][i] + m1[i] * a[1][i] + m2[i] * a[2][i];
   }
 #define PHI_(i) (phi ? phi[i] : 1.0)
   for (int d = 0; d < 3; ++d) { /* mass :88-93 (ev: phi_rho uh) */
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) tmp[i] = phi ? phi[i] * q[i] * uh[d * N + i] : q[i] * uh[d * N + i];
     orc_diff3d(c, d, a[d], tmp, dxs[d]);
   }
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i)
     r[0][i] = phi ? Xj[i] * inv_phi[i] * (-a[0][i] - a[1][i] - a[2][i]) : Xj[i] * (-a[0][i] - a[1][i] - a[2][i]);
   for (int m = 0; m < 3; ++m) { /* momentum :96-117 (ev: phi_rho_u_m uh) */
     for (int d = 0; d < 3; ++d) {
+      #pragma omp parallel for schedule(static)
       for (size_t i = 0; i < N; ++i)
         tmp[i] = phi ? PHI_(i) * q[(m + 1) * N + i] * uh[d * N + i] : q[i] * u[m * N + i] * uh[d * N + i];
       orc_diff3d(c, d, a[d], tmp, dxs[d]);
     }
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i)
       r[m + 1][i] = phi ? Xj[i] * (inv_phi[i] * (-a[0][i] - a[1][i] - a[2][i]) - pg[m][i])
                         : Xj[i] * (-a[0][i] - a[1][i] - a[2][i] - pg[m][i]);
   }
   for (int d = 0; d < 3; ++d) { /* pressure :120-128 (ev: phi_p uh) */
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) tmp[i] = PHI_(i) * q[4 * N + i] * uh[d * N + i];
     orc_diff3d(c, d, a[d], tmp, dxs[d]);
   }
 #undef PHI_
   const double gt = c->gamma / (c->gamma - 1.0), gm1 = c->gamma - 1.0;
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) {
     const double up = u[i] * pg[0][i] + u[N + i] * pg[1][i] + u[2 * N + i] * pg[2][i];
     if (phi) {
@@ -1150,6 +1211,7 @@ void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const dou
   orc_bc_q(c, q, qref, shape);              /* :92 */
   const double *q1 = q, *q2 = q + N, *q3 = q + 2 * N, *q4 = q + 3 * N, *q5 = q + 4 * N;
   double *u1 = u, *u2 = u + N, *u3 = u + 2 * N;
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) { /* :95-99 */
     u1[i] = q2[i] / q1[i];
     u2[i] = q3[i] / q1[i];
@@ -1166,6 +1228,7 @@ void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const dou
   orc_diff3d(c, 1, wy, u3, dx2);
   orc_diff3d(c, 2, wz, u3, dx3);
   /* :115-128 */
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) {
     temp[i] = orc_T(c, q1[i], q5[i]);
     mu[i] = orc_mu(c, temp[i]);
@@ -1181,6 +1244,7 @@ void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const dou
   orc_diff3d(c, 0, ax, q2, dx1);
   orc_diff3d(c, 1, ay, q3, dx2);
   orc_diff3d(c, 2, az, q4, dx3);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) r1[i] = -ax[i] - ay[i] - az[i];
 
   /* momentum :138-177 */
@@ -1193,12 +1257,15 @@ void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const dou
     const double *um = u + m * N;
     double *a[3] = {ax, ay, az};
     for (int d = 0; d < 3; ++d) {
+      #pragma omp parallel for schedule(static)
       for (size_t i = 0; i < N; ++i) tmp[i] = qd[d][i] * um[i];
       orc_diff3d(c, d, a[d], tmp, dxs[d]);
     }
     orc_diff3d(c, m, pd[m], q5, dxs[m]);
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) rm[m][i] = -ax[i] - ay[i] - az[i] - pd[m][i];
     for (int d = 0; d < 3; ++d) orc_diff3d(c, d, a[d], tau[m][d], dxs[d]);
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) rm[m][i] = rm[m][i] + ax[i] + ay[i] + az[i];
   }
 
@@ -1207,10 +1274,12 @@ void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const dou
     double *a[3] = {ax, ay, az};
     for (int d = 0; d < 3; ++d) {
       const double *ud = u + d * N;
+      #pragma omp parallel for schedule(static)
       for (size_t i = 0; i < N; ++i) tmp[i] = q5[i] * ud[i];
       orc_diff3d(c, d, a[d], tmp, dxs[d]);
     }
     const double gt = c->gamma / (c->gamma - 1.0);
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) {
       r5[i] = gt * (-ax[i] - ay[i] - az[i]);
       r5[i] = r5[i] + u1[i] * px[i] + u2[i] * py[i] + u3[i] * pz[i];
@@ -1219,12 +1288,15 @@ void orc_rhs_direct(const orc_cfg *c, double *rhs, const double *q_in, const dou
     orc_diff3d(c, 0, ax, temp, hx1);
     orc_diff3d(c, 1, ay, temp, hx2);
     orc_diff3d(c, 2, az, temp, hx3);
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) lam[i] = orc_lambda(c, mu[i]);
     const double hx[3] = {hx1, hx2, hx3};
     for (int d = 0; d < 3; ++d) {
+      #pragma omp parallel for schedule(static)
       for (size_t i = 0; i < N; ++i) tmp[i] = lam[i] * a[d][i];
       orc_diff3d(c, d, a[d], tmp, hx[d]);
     }
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) {
       r5[i] = r5[i] + ax[i] + ay[i] + az[i];
       r5[i] = r5[i] + t11[i] * ux[i] + t12[i] * uy[i] + t13[i] * uz[i] + t12[i] * vx[i] +
@@ -1258,6 +1330,7 @@ void orc_calc_metric(const orc_cfg *c, const double *X, const double *X_calc, do
   double *Xt = xalloc(3 * N); /* :68-70 */
   const double cc[3] = {c1, c2, c3};
   for (int d = 0; d < 3; ++d)
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) Xt[d * N + i] = X[d * N + i] - cc[d] * X_calc[d * N + i];
   const double *X1 = Xt, *X2 = Xt + N, *X3 = Xt + 2 * N;
   double *d11 = xalloc(N), *d12 = xalloc(N), *d13 = xalloc(N); /* dx1dxi1, dx1dxi2, dx1dxi3 */
@@ -1280,51 +1353,61 @@ void orc_calc_metric(const orc_cfg *c, const double *X, const double *X_calc, do
   PROD(d22, X3); orc_diff3d(c, 2, t1, pr, dx3);
   PROD(d23, X3); orc_diff3d(c, 1, t2, pr, dx2);
   M = MET(X_metric, 0, 0, N);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] + c2 * d33[i] + c3 * d22[i] + c2 * c3;
   /* m12 :124-126 */
   PROD(d13, X3); orc_diff3d(c, 1, t1, pr, dx2);
   PROD(d12, X3); orc_diff3d(c, 2, t2, pr, dx3);
   M = MET(X_metric, 0, 1, N);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] - c3 * d12[i];
   /* m13 :132-134 */
   PROD(d12, X2); orc_diff3d(c, 2, t1, pr, dx3);
   PROD(d13, X2); orc_diff3d(c, 1, t2, pr, dx2);
   M = MET(X_metric, 0, 2, N);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] - c2 * d13[i];
   /* m21 :139-141 */
   PROD(d23, X3); orc_diff3d(c, 0, t1, pr, dx1);
   PROD(d21, X3); orc_diff3d(c, 2, t2, pr, dx3);
   M = MET(X_metric, 1, 0, N);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] - c3 * d21[i];
   /* m22 :147-149 */
   PROD(d11, X3); orc_diff3d(c, 2, t1, pr, dx3);
   PROD(d13, X3); orc_diff3d(c, 0, t2, pr, dx1);
   M = MET(X_metric, 1, 1, N);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] + c1 * d33[i] + c3 * d11[i] + c1 * c3;
   /* m23 :155-157 */
   PROD(d13, X2); orc_diff3d(c, 0, t1, pr, dx1);
   PROD(d11, X2); orc_diff3d(c, 2, t2, pr, dx3);
   M = MET(X_metric, 1, 2, N);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] - c1 * d23[i];
   /* m31 :162-164 */
   PROD(X3, d21); orc_diff3d(c, 1, t1, pr, dx2);
   PROD(X3, d22); orc_diff3d(c, 0, t2, pr, dx1);
   M = MET(X_metric, 2, 0, N);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] - c2 * d31[i];
   /* m32 :169-171 */
   PROD(d12, X3); orc_diff3d(c, 0, t1, pr, dx1);
   PROD(d11, X3); orc_diff3d(c, 1, t2, pr, dx2);
   M = MET(X_metric, 2, 1, N);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] - c1 * d32[i];
   /* m33 :176-178 */
   PROD(d11, X2); orc_diff3d(c, 1, t1, pr, dx2);
   PROD(d12, X2); orc_diff3d(c, 0, t2, pr, dx1);
   M = MET(X_metric, 2, 2, N);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) M[i] = t1[i] - t2[i] + c1 * d22[i] + d11[i] * c2 + c1 * c2;
 #undef PROD
   /* Jacobian :182-185 */
   const double *M31 = MET(X_metric, 2, 0, N), *M32 = MET(X_metric, 2, 1, N),
                *M33 = MET(X_metric, 2, 2, N);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) {
     X_jacobian[i] = M31[i] * d13[i] + M32[i] * d23[i] + M33[i] * (d33[i] + c3);
     X_jacobian[i] = 1.0 / X_jacobian[i];
@@ -1385,6 +1468,7 @@ static void rhs_direct_gt_impl(const orc_cfg *c, double *rhs, const double *q_in
   memcpy(q, q_in, 5 * N * sizeof(double)); /* :102 */
   orc_bc_q(c, q, qref, shape);              /* :105 */
   for (int m = 0; m < 3; ++m) /* :108-112 */
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) u[m * N + i] = q[(m + 1) * N + i] / q[i];
   /* gradient of f in physical coordinates without Jacobian: out_j = sum_k M(k,j) D_k f */
 #define PHYS_GRAD(f, out)                                                                     \
@@ -1392,6 +1476,7 @@ static void rhs_direct_gt_impl(const orc_cfg *c, double *rhs, const double *q_in
     for (int k_ = 0; k_ < 3; ++k_) orc_diff3d(c, k_, a[k_], (f), dxs[k_]);                    \
     for (int j_ = 0; j_ < 3; ++j_) {                                                          \
       const double *m0 = MET(Xm, 0, j_, N), *m1 = MET(Xm, 1, j_, N), *m2 = MET(Xm, 2, j_, N); \
+      _Pragma("omp parallel for schedule(static)")                                            \
       for (size_t i = 0; i < N; ++i)                                                          \
         (out)[j_][i] = m0[i] * a[0][i] + m1[i] * a[1][i] + m2[i] * a[2][i];                   \
     }                                                                                         \
@@ -1399,11 +1484,13 @@ static void rhs_direct_gt_impl(const orc_cfg *c, double *rhs, const double *q_in
   for (int m = 0; m < 3; ++m) PHYS_GRAD(u + m * N, g[m]); /* :116-135 */
   for (int k = 0; k < 3; ++k) { /* :139-141 */
     const double *m0 = MET(Xm, k, 0, N), *m1 = MET(Xm, k, 1, N), *m2 = MET(Xm, k, 2, N);
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i)
       uh[k * N + i] = m0[i] * u[i] + m1[i] * u[N + i] + m2[i] * u[2 * N + i];
   }
   PHYS_GRAD(q + 4 * N, pg); /* :144-149 */
   /* friction :152-165 */
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) {
     temp[i] = orc_T(c, q[i], q[4 * N + i]);
     mu[i] = orc_mu(c, temp[i]);
@@ -1416,8 +1503,10 @@ static void rhs_direct_gt_impl(const orc_cfg *c, double *rhs, const double *q_in
     t23[i] = mu[i] * (g[1][2][i] + g[2][1][i]);
   }
   /* heat :168-182 */
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) lam[i] = orc_lambda(c, mu[i]);
   PHYS_GRAD(temp, tg);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) {
     lam[i] = phi ? phi[i] * lam[i] * Xj[i] : lam[i] * Xj[i]; /* ev: phi lambda J (:199) */
     tg[0][i] = lam[i] * tg[0][i];
@@ -1427,9 +1516,11 @@ static void rhs_direct_gt_impl(const orc_cfg *c, double *rhs, const double *q_in
 #undef PHYS_GRAD
   /* mass :184-190 */
   for (int k = 0; k < 3; ++k) {
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) tmp[i] = (phi ? phi[i] * q[i] : q[i]) * uh[k * N + i];
     orc_diff3d(c, k, a[k], tmp, dxs[k]);
   }
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) {
     r[0][i] = -a[0][i] - a[1][i] - a[2][i];
     r[0][i] = phi ? Xj[i] * inv_phi[i] * r[0][i] : Xj[i] * r[0][i];
@@ -1437,20 +1528,24 @@ static void rhs_direct_gt_impl(const orc_cfg *c, double *rhs, const double *q_in
   /* momentum :192-241 */
   for (int m = 0; m < 3; ++m) {
     for (int k = 0; k < 3; ++k) {
+      #pragma omp parallel for schedule(static)
       for (size_t i = 0; i < N; ++i)
         tmp[i] = (phi ? phi[i] * q[(m + 1) * N + i] : q[(m + 1) * N + i]) * uh[k * N + i];
       orc_diff3d(c, k, a[k], tmp, dxs[k]);
     }
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i)
       r[m + 1][i] = phi ? -a[0][i] - a[1][i] - a[2][i] : -a[0][i] - a[1][i] - a[2][i] - pg[m][i];
     for (int k = 0; k < 3; ++k) { /* tau_proj_mk = sum_j M(k,j) tau_mj */
       const double *m0 = MET(Xm, k, 0, N), *m1 = MET(Xm, k, 1, N), *m2 = MET(Xm, k, 2, N);
+      #pragma omp parallel for schedule(static)
       for (size_t i = 0; i < N; ++i) {
         tmp[i] = m0[i] * tau[m][0][i] + m1[i] * tau[m][1][i] + m2[i] * tau[m][2][i];
         if (phi) tmp[i] = phi[i] * tmp[i]; /* ev: phi tau_proj (:222-224) */
       }
       orc_diff3d(c, k, a[k], tmp, dxs[k]);
     }
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) {
       r[m + 1][i] = r[m + 1][i] + a[0][i] + a[1][i] + a[2][i];
       if (phi) r[m + 1][i] = inv_phi[i] * r[m + 1][i] - pg[m][i]; /* ev (:227) */
@@ -1459,10 +1554,12 @@ static void rhs_direct_gt_impl(const orc_cfg *c, double *rhs, const double *q_in
   }
   /* pressure :243-276 */
   for (int k = 0; k < 3; ++k) {
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) tmp[i] = (phi ? phi[i] * q[4 * N + i] : q[4 * N + i]) * uh[k * N + i];
     orc_diff3d(c, k, a[k], tmp, dxs[k]);
   }
   const double gt = c->gamma / (c->gamma - 1.0);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) {
     r[4][i] = gt * (-a[0][i] - a[1][i] - a[2][i]);
     if (!phi) r[4][i] = r[4][i] + u[i] * pg[0][i] + u[N + i] * pg[1][i] + u[2 * N + i] * pg[2][i];
@@ -1470,9 +1567,11 @@ static void rhs_direct_gt_impl(const orc_cfg *c, double *rhs, const double *q_in
   for (int j = 0; j < 3; ++j) { /* heat flux in x_j (:252-268): outer Dx without h */
     for (int k = 0; k < 3; ++k) orc_diff3d(c, k, a[k], tg[j], 1.0);
     const double *m0 = MET(Xm, 0, j, N), *m1 = MET(Xm, 1, j, N), *m2 = MET(Xm, 2, j, N);
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i)
       r[4][i] = r[4][i] + m0[i] * a[0][i] + m1[i] * a[1][i] + m2[i] * a[2][i];
   }
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) { /* :270-276 */
     if (phi) { /* ev: div_1_phi on the bracket, then u.grad(p) (:292-293) */
       r[4][i] = inv_phi[i] * r[4][i];
@@ -1522,13 +1621,16 @@ void orc_rhs_direct_ev(const orc_cfg *c, double *rhs, const double *q_in, const 
   memcpy(q, q_in, 5 * N * sizeof(double)); /* :99 */
   orc_bc_q(c, q, qref, shape);              /* :102 */
   for (int m = 0; m < 3; ++m)               /* :105-117 */
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) {
       u[m * N + i] = q[(m + 1) * N + i] / q[i];
       pru[m * N + i] = phi[i] * q[(m + 1) * N + i];
     }
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) php[i] = phi[i] * q[4 * N + i];
   for (int m = 0; m < 3; ++m) /* :121-131 */
     for (int k = 0; k < 3; ++k) orc_diff3d(c, k, g[m][k], u + m * N, dxs[k]);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) { /* :134-147 */
     temp[i] = orc_T(c, q[i], q[4 * N + i]);
     mu[i] = orc_mu(c, temp[i]);
@@ -1543,6 +1645,7 @@ void orc_rhs_direct_ev(const orc_cfg *c, double *rhs, const double *q_in, const 
   for (int k = 0; k < 3; ++k) orc_diff3d(c, k, pg[k], q + 4 * N, dxs[k]); /* :149-151 */
   /* mass :153-159 */
   for (int k = 0; k < 3; ++k) orc_diff3d(c, k, a[k], pru + k * N, dxs[k]);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) {
     r[0][i] = -a[0][i] - a[1][i] - a[2][i];
     r[0][i] = inv_phi[i] * r[0][i];
@@ -1550,14 +1653,18 @@ void orc_rhs_direct_ev(const orc_cfg *c, double *rhs, const double *q_in, const 
   /* momentum :161-201 */
   for (int m = 0; m < 3; ++m) {
     for (int k = 0; k < 3; ++k) {
+      #pragma omp parallel for schedule(static)
       for (size_t i = 0; i < N; ++i) tmp[i] = pru[k * N + i] * u[m * N + i];
       orc_diff3d(c, k, a[k], tmp, dxs[k]);
     }
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) r[m + 1][i] = -a[0][i] - a[1][i] - a[2][i];
     for (int k = 0; k < 3; ++k) {
+      #pragma omp parallel for schedule(static)
       for (size_t i = 0; i < N; ++i) tmp[i] = phi[i] * tau[m][k][i];
       orc_diff3d(c, k, a[k], tmp, dxs[k]);
     }
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) {
       r[m + 1][i] = r[m + 1][i] + a[0][i] + a[1][i] + a[2][i];
       r[m + 1][i] = inv_phi[i] * r[m + 1][i] - pg[m][i];
@@ -1565,17 +1672,22 @@ void orc_rhs_direct_ev(const orc_cfg *c, double *rhs, const double *q_in, const 
   }
   /* pressure :203-234 */
   for (int k = 0; k < 3; ++k) {
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) tmp[i] = php[i] * u[k * N + i];
     orc_diff3d(c, k, a[k], tmp, dxs[k]);
   }
   const double gt = c->gamma / (c->gamma - 1.0);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) r[4][i] = gt * (-a[0][i] - a[1][i] - a[2][i]);
   for (int k = 0; k < 3; ++k) orc_diff3d(c, k, a[k], temp, 1.0); /* d T, no h */
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) plam[i] = phi[i] * orc_lambda(c, mu[i]);
   for (int k = 0; k < 3; ++k) {
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) tmp[i] = plam[i] * a[k][i];
     orc_diff3d(c, k, a[k], tmp, 1.0);
   }
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) {
     r[4][i] = r[4][i] + a[0][i] + a[1][i] + a[2][i];
     r[4][i] = inv_phi[i] * r[4][i];
@@ -1613,12 +1725,14 @@ void orc_rhs_adjoint(const orc_cfg *c, double *rhs, const double *qs_in, const d
   const double *q1 = q, *q5 = q + 4 * N;
   double *r[5];
   for (int v = 0; v < 5; ++v) r[v] = rhs + v * N;
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; ++i) {
     gdp[i] = gam * q5[i] / (q1[i] * gm1);
     irho[i] = 1.0 / q1[i];
   }
   memset(rhs, 0, 5 * N * sizeof(double));
   for (int m = 0; m < 3; ++m)
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) u[m * N + i] = q[(m + 1) * N + i] / q1[i];
   const double *u1 = u, *u2 = u + N, *u3 = u + 2 * N;
   const double *qs5 = qs + 4 * N;
@@ -1631,6 +1745,7 @@ void orc_rhs_adjoint(const orc_cfg *c, double *rhs, const double *qs_in, const d
     orc_diff3d(c, dd, d, qs + 3 * N, h);
     orc_diff3d(c, dd, e, qs + 4 * N, h);
     const double *ud = u + dd * N;
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) {
       if (dd == 0) { /* :317-339 */
         r[0][i] = r[0][i] + (u1[i] * u1[i]) * b[i] + (u1[i] * u2[i]) * cc[i] +
@@ -1656,11 +1771,14 @@ void orc_rhs_adjoint(const orc_cfg *c, double *rhs, const double *qs_in, const d
       }
     }
     /* -(A^T)^-1 (C^T qs)_x : :355-357, :443-445, :512-514 */
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) tmp[i] = -ud[i] * qs5[i];
     orc_diff3d(c, dd, a, tmp, h);
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) r[4][i] = r[4][i] - gm1 * a[i];
     /* +(A^T)^-1 C~ c_x : :366-372 etc. */
     orc_diff3d(c, dd, a, q5, h);
+    #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; ++i) {
       r[0][i] = r[0][i] + qs5[i] * ud[i] * irho[i] * a[i];
       r[dd + 1][i] = r[dd + 1][i] - qs5[i] * irho[i] * a[i];
@@ -1682,12 +1800,16 @@ void orc_time_stepper_direct(const orc_cfg *c, double *q1, const double *q0,
   const double dt = c->dt, dth = c->dt * (0.5);
   double *k1 = xalloc(M), *k2 = xalloc(M), *k3 = xalloc(M), *k4 = xalloc(M), *s = xalloc(M);
   orc_rhs_direct(c, k1, q0, qref, shape, heat_unit_spacing);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < M; ++i) s[i] = q0[i] + dth * k1[i];
   orc_rhs_direct(c, k2, s, qref, shape, heat_unit_spacing);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < M; ++i) s[i] = q0[i] + dth * k2[i];
   orc_rhs_direct(c, k3, s, qref, shape, heat_unit_spacing);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < M; ++i) s[i] = q0[i] + dt * k3[i];
   orc_rhs_direct(c, k4, s, qref, shape, heat_unit_spacing);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < M; ++i) q1[i] = q0[i] + dt / 6.0 * (k1[i] + 2 * k2[i] + 2 * k3[i] + k4[i]);
   free(k1); free(k2); free(k3); free(k4); free(s);
 }
@@ -1699,12 +1821,16 @@ void orc_time_stepper_adjoint(const orc_cfg *c, double *qs1, const double *qs0,
   const double dt = -c->dt, dth = -c->dt * (0.5);
   double *k1 = xalloc(M), *k2 = xalloc(M), *k3 = xalloc(M), *k4 = xalloc(M), *s = xalloc(M);
   orc_rhs_adjoint(c, k1, qs0, q0, g, shape);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < M; ++i) s[i] = qs0[i] + dth * k1[i];
   orc_rhs_adjoint(c, k2, s, q0, g, shape);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < M; ++i) s[i] = qs0[i] + dth * k2[i];
   orc_rhs_adjoint(c, k3, s, q0, g, shape);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < M; ++i) s[i] = qs0[i] + dt * k3[i];
   orc_rhs_adjoint(c, k4, s, q0, g, shape);
+  #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < M; ++i) qs1[i] = qs0[i] + dt / 6.0 * (k1[i] + 2 * k2[i] + 2 * k3[i] + k4[i]);
   free(k1); free(k2); free(k3); free(k4); free(s);
 }
@@ -1722,6 +1848,7 @@ void orc_filter_weighted4d(const orc_cfg *c, int dir, double *q, int nv, const d
   const double a0 = -2.0 * 0.25, a1 = 1.0 * 0.25;
   double *f = xalloc(N), *g = xalloc(N);
   size_t *ip = (size_t *)malloc(N * sizeof(size_t)), *im = (size_t *)malloc(N * sizeof(size_t));
+  #pragma omp parallel for schedule(static)
   for (size_t p = 0; p < N; ++p) {
     const int i = dir == 0 ? (int)(p % n1) : (dir == 1 ? (int)((p / n1) % n2) : (int)(p / ((size_t)n1 * n2)));
     ip[p] = i + 1 < n ? p + st : p - (size_t)(n - 1) * st;
@@ -1729,11 +1856,16 @@ void orc_filter_weighted4d(const orc_cfg *c, int dir, double *q, int nv, const d
   }
   for (int v = 0; v < nv; ++v) {
     double *x = q + (size_t)v * N;
+    #pragma omp parallel for schedule(static)
     for (size_t p = 0; p < N; ++p) f[p] = a0 * x[p] + a1 * (x[ip[p]] + x[im[p]]);
+    #pragma omp parallel for schedule(static)
     for (size_t p = 0; p < N; ++p)
       f[p] = (strength ? strength[p] : 1.0) * (weight ? weight[p] : 1.0) * f[p];
+    #pragma omp parallel for schedule(static)
     for (size_t p = 0; p < N; ++p) g[p] = a0 * f[p] + a1 * (f[ip[p]] + f[im[p]]);
+    #pragma omp parallel for schedule(static)
     for (size_t p = 0; p < N; ++p) g[p] = g[p] / (weight ? weight[p] : 1.0);
+    #pragma omp parallel for schedule(static)
     for (size_t p = 0; p < N; ++p) x[p] = x[p] - g[p];
   }
   free(f); free(g); free(ip); free(im);
@@ -1749,6 +1881,46 @@ void orc_run_direct(const orc_cfg *c, double *q, int nt0, int nsteps, const doub
   double *q1 = xalloc(M);
   for (int nt = nt0 + 1; nt <= nt0 + nsteps; ++nt) {
     orc_time_stepper_direct(c, q1, q, qref, shape, heat_unit_spacing);
+    memcpy(q, q1, M * sizeof(double));
+    bc_q_at(c, q, qref, shape, 1);
+    for (int d = 0; d < 3; ++d)
+      if (c->filter_freq[d] > 0 && nt % c->filter_freq[d] == 0) orc_filter4d(c, d, q, 5);
+    for (int d = 0; d < 3; ++d)
+      if (c->wfilter_freq[d] > 0 && nt % c->wfilter_freq[d] == 0)
+        orc_filter_weighted4d(c, d, q, 5, c->penal_phi, NULL);
+  }
+  free(q1);
+}
+
+/* The cruna_hpc loop with its convergence check, src/main/cruna_hpc.f90:171-242
+ * (run.cfreq, run.ctol read at :81-82): after the stepper, at steps with
+ * mod(nt, cfreq) == 0, residual = maxval(abs((qn - q1)/dt)) (:185-186); when
+ * residual <= ctol the loop stops with q1 (the unfiltered new state the
+ * reference stores as its snapshot, :191) and *nt_conv = nt, else qn = q1 and
+ * the BC and filters follow as in orc_run_direct.  *res: the last residual
+ * computed (-1 if none), *nt_conv: 0 if the tolerance was not reached. */
+void orc_run_direct_conv(const orc_cfg *c, double *q, int nt0, int nsteps, int cfreq, double ctol,
+                         const double *qref, const double *shape, int heat_unit_spacing,
+                         double *res, int *nt_conv) {
+  const size_t M = 5 * npts(c);
+  double *q1 = xalloc(M);
+  *res = -1.0;
+  *nt_conv = 0;
+  for (int nt = nt0 + 1; nt <= nt0 + nsteps; ++nt) {
+    orc_time_stepper_direct(c, q1, q, qref, shape, heat_unit_spacing);
+    if (cfreq > 0 && nt % cfreq == 0) {
+      double m = 0.0;
+      for (size_t i = 0; i < M; ++i) {
+        const double d = fabs((q[i] - q1[i]) / c->dt);
+        if (d > m) m = d;
+      }
+      *res = m;
+      if (m <= ctol) {
+        memcpy(q, q1, M * sizeof(double));
+        *nt_conv = nt;
+        break;
+      }
+    }
     memcpy(q, q1, M * sizeof(double));
     bc_q_at(c, q, qref, shape, 1);
     for (int d = 0; d < 3; ++d)

@@ -157,6 +157,10 @@ void orc_time_stepper_adjoint(const orc_cfg *c, double *qs1, const double *qs0,
 void orc_run_direct(const orc_cfg *c, double *q, int nt0, int nsteps, const double *qref,
                     const double *shape, int heat_unit_spacing);
 /* Q holds nsteps direct states (state after step nt at Q + (nt-1)*N*5). */
+/* cruna_hpc's loop with the residual check of run.cfreq / run.ctol */
+void orc_run_direct_conv(const orc_cfg *c, double *q, int nt0, int nsteps, int cfreq, double ctol,
+                         const double *qref, const double *shape, int heat_unit_spacing,
+                         double *res, int *nt_conv);
 void orc_run_adjoint(const orc_cfg *c, double *qs, const double *Q, int nsteps,
                      const double *shape);
 

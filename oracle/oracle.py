@@ -102,6 +102,8 @@ def lib():
         L.orc_time_stepper_adjoint.argtypes = [P, _dp, _dp, _dp, _vp, _vp]
         L.orc_run_direct.argtypes = [P, _dp, C.c_int, C.c_int, _vp, _vp, C.c_int]
         L.orc_run_adjoint.argtypes = [P, _dp, _dp, C.c_int, _vp]
+        L.orc_run_direct_conv.argtypes = [P, _dp, C.c_int, C.c_int, C.c_int, C.c_double, _vp, _vp, C.c_int,
+                                          C.POINTER(C.c_double), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -327,6 +329,18 @@ def run_direct(cfg, q, nsteps, nt0=0, qref=None, shape=None, heat_unit_spacing=1
     shape = None if shape is None else _f(shape)
     lib().orc_run_direct(cfg.ref(), q, nt0, nsteps, _ptr(qref), _ptr(shape), heat_unit_spacing)
     return q
+
+
+def run_direct_conv(cfg, q, nsteps, cfreq, ctol, nt0=0, qref=None, shape=None, heat_unit_spacing=1):
+    """cruna_hpc's loop with the convergence check: (q, last residual, step
+    at which the tolerance was reached or 0)."""
+    q = _f(q).copy(order="F")
+    qref = None if qref is None else _f(qref)
+    shape = None if shape is None else _f(shape)
+    res, ntc = C.c_double(), C.c_int()
+    lib().orc_run_direct_conv(cfg.ref(), q, nt0, nsteps, cfreq, ctol, _ptr(qref), _ptr(shape),
+                              heat_unit_spacing, C.byref(res), C.byref(ntc))
+    return q, res.value, ntc.value
 
 
 def run_adjoint(cfg, qs, Q, shape=None):

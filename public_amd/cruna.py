@@ -128,6 +128,8 @@ def lib():
         L.cruna_hip_cnn_train.argtypes = [vp, ip, C.c_float, ip, ip, C.c_uint, fp]
         L.cruna_hip_cnn_num_samples.argtypes = [vp]
         L.cruna_hip_set_closure.argtypes = [vp, vp]
+        L.cruna_hip_set_convergence.argtypes = [vp, ip, C.c_double]
+        L.cruna_hip_get_residual.argtypes = [vp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         for f in ("create", "destroy", "set_state", "get_state", "set_qref", "set_sponge_shape", "set_kvector", "diff",
                   "filter", "rhs_direct", "rhs_adjoint", "time_stepper_direct", "time_stepper_adjoint",
                   "run_direct", "synchronize", "profile_enable", "profile_read", "profile_count",
@@ -137,7 +139,7 @@ def lib():
                   "ensemble_run", "ensemble_autotune", "cnn_create", "cnn_destroy", "cnn_init_params",
                   "cnn_set_params", "cnn_get_params", "cnn_set_norm", "cnn_forward", "cnn_gradients",
                   "cnn_add_sample", "cnn_train", "set_closure", "box_filter_state", "cnn_set_gain", "subsets_alloc", "run_direct_subsets",
-                  "run_adjoint_subsets"):
+                  "run_adjoint_subsets", "set_convergence", "get_residual"):
             getattr(L, "cruna_hip_" + f).restype = C.c_int
         _lib = L
     return _lib
@@ -357,6 +359,18 @@ class Cruna:
 
     def synchronize(self):
         _check(lib().cruna_hip_synchronize(self._h))
+
+    def set_convergence(self, cfreq, ctol):
+        """run.cfreq / run.ctol of cruna_hpc: stop run_direct once
+        max|qn - q1|/dt <= ctol at a step with mod(nt, cfreq) == 0 (0: off)."""
+        _check(lib().cruna_hip_set_convergence(self._h, int(cfreq), float(ctol)))
+
+    def residual(self):
+        """(last residual or -1, step it was computed at, step at which the
+        last run_direct reached the tolerance or 0)."""
+        r, ntc, ntv = C.c_double(), C.c_int(), C.c_int()
+        _check(lib().cruna_hip_get_residual(self._h, C.byref(r), C.byref(ntc), C.byref(ntv)))
+        return r.value, ntc.value, ntv.value
 
     # ----- adjoint loop -----
     def set_adjoint_state(self, qs):

@@ -22,6 +22,8 @@
 // dt -> -dt).
 #pragma once
 
+#include <climits>
+
 #include "boundary.hpp"
 #include "sweeps.hpp"
 
@@ -277,6 +279,23 @@ __global__ void k_adjoint_slot(const double **q0p, int *ntp, const double *traj,
   const int nt = *ntp;
   q0p[0] = traj + (long)(nslots > 1 ? nt - 1 : 0) * stride;
   ntp[0] = nt - 1;
+}
+
+// Subset checkpoints inside a recorded period: the subset counter *sp
+// (device cell) names the checkpoint to restore; every thread copies its
+// share of that state into q (grid-stride, 16-byte pairs; the stride of a
+// state, 5 N, is even or handled by the tail thread).
+__global__ __launch_bounds__(256) void k_restore_ckpt(double *__restrict__ q, const double *__restrict__ ckpt,
+                                                      const int *__restrict__ sp, long M) {
+  const int s = *sp;
+  const double *src = ckpt + (long)s * M;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < M; i += (long)gridDim.x * 256) q[i] = src[i];
+}
+
+// Set an int cell (counter) from the stream, so that a recorded graph can
+// reset it without a memset node.
+__global__ void k_set_int(int *p, int v, int delta) {
+  if (threadIdx.x == 0) p[0] = v == INT_MIN ? p[0] + delta : v;
 }
 
 // Point the forward-state cell at a given state.  One thread writes.

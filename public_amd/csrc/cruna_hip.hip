@@ -19,6 +19,7 @@
 #include "weighted_filter.hpp"
 #include "adjoint.hpp"
 #include "closure.hpp"
+#include "residual.hpp"
 
 using namespace cruna;
 
@@ -147,11 +148,15 @@ struct cruna_hip_ctx {
   // backward step counter (k_adjoint_slot)
   const double **q0p = nullptr;
   int *ntp = nullptr;
+  // subset counter of an ensemble member running the subset adjoint loop
+  // inside recorded graphs: the checkpoint to restore next
+  int *sckp = nullptr;
   int traj_slots = 0;
   bool record = false;
   // subset checkpoints: the state at the start of every subset and the final one
   double *ckpt = nullptr;
   int ckpt_n = 0;
+  int sub_steps = 0;  // steps per subset (cruna_hip_subsets_alloc)
   void *qs = nullptr, *rhs = nullptr, *G = nullptr;  // compute precision
   // learned closure added to the direct RHS (cruna_hip_set_closure)
   cruna_hip_cnn *cnn = nullptr;
@@ -159,6 +164,13 @@ struct cruna_hip_ctx {
   // qs holds qn in the compute precision (FP32-mixed: written by the step's
   // last filter pass, read by the next step's first stage instead of qn)
   bool qs_fresh = false;
+  // convergence check of the cruna_hpc loop (run.cfreq, run.ctol): the state
+  // before the step (q0 copy, 5 fields), partial maxima and the residual
+  int cfreq = 0;
+  double ctol = 0.0;
+  double *q0copy = nullptr, *res_buf = nullptr;
+  double last_res = -1.0;
+  int res_nt = 0, conv_nt = 0;
   // profiling
   bool prof = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -343,6 +355,25 @@ const char *kNoCharRhsAdjoint =
     "rows of types 204-206 stop the reference's adjoint (set_boundary_condition_rhss has no case for them)";
 const char *kNoEvAdjoint =
     "the adjoint of ns_rho_rhou_p_ev_phi is not implemented (the reference stops there too)";
+
+// Configurations the adjoint entry points refuse, checked before anything is
+// launched: the penalised adjoint, rows 204-206, the Euler adjoint on a
+// deformed grid, and inputs that rows of the adjoint need (the sponge shape
+// of 400/401 rows, whose Euler-adjoint RHS term and adjoint-state relaxation
+// read it; the normals per point of rows 202/203).
+int adjoint_checks(cruna_hip_ctx *c) {
+  if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
+  if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
+  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
+  for (auto &r : c->bc) {
+    const bool euler_sponge = r.first == 400 && c->cfg.equations == CRUNA_HIP_EQ_EULER;
+    if ((euler_sponge || r.first == 401 || r.first == 203) && !c->shape)
+      return fail(CRUNA_HIP_EINVAL, "sponge rows need a shape (cruna_hip_set_sponge_shape)");
+    if ((r.first == 202 || r.first == 203) && !c->kvec)
+      return fail(CRUNA_HIP_EINVAL, "rows of types 202/203/205/206 need the normals per point (cruna_hip_set_kvector)");
+  }
+  return CRUNA_HIP_OK;
+}
 
 // X_jacobian in the compute precision on a deformed grid, else null
 template <typename T>
@@ -692,14 +723,42 @@ namespace {
 
 // weighted: the cruna_hpc loop's filter_weighted_i1..i3 after the filters
 // (cruna_hpc.f90:231-242); the subset loop (computation_direct.f90) has none.
-int step_once(cruna_hip_ctx *c, int nt, bool weighted = true) {
+// converged (may be null): the convergence check of the cruna_hpc loop
+// (cruna_hpc.f90:185-208) at steps with mod(nt, cfreq) == 0; when the
+// residual max|qn - q1|/dt is <= ctol the step ends with the unfiltered q1 in
+// qn (the state the reference stores before it stops) and *converged is set.
+int step_once(cruna_hip_ctx *c, int nt, bool weighted = true, bool *converged = nullptr) {
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
+  const bool check = converged && c->cfreq > 0 && nt % c->cfreq == 0;
+  const size_t sbytes = 5 * (size_t)c->N * sizeof(double);
+  if (check) HIP_TRY(hipMemcpyAsync(c->q0copy, c->qn, sbytes, hipMemcpyDeviceToDevice, c->stream));
   // filter_i1 directly follows the RK4 update unless q-type boundary rows
-  // come between them; then it runs inside the update's x sweep
-  const bool fuse_fx =
-      !c->has_bcq && !c->has_bclate && c->cfg.filter_freq[0] > 0 && nt % c->cfg.filter_freq[0] == 0;
+  // come between them (or the residual needs the unfiltered new state);
+  // then it runs inside the update's x sweep
+  const bool fuse_fx = !check && !c->has_bcq && !c->has_bclate && c->cfg.filter_freq[0] > 0 &&
+                       nt % c->cfg.filter_freq[0] == 0;
   int rc = mixed ? rk4_step<float>(c, fuse_fx) : rk4_step<double>(c, fuse_fx);
   if (rc) return rc;
+  if (check) {
+    const long M = 5 * c->N;
+    const int parts = (int)std::min<long>(RES_MAX_PARTS, (M / 2 + RES_THREADS - 1) / RES_THREADS);
+    hipLaunchKernelGGL(k_absdiff_max, dim3(std::max(parts, 1)), dim3(RES_THREADS), 0, c->stream,
+                       (const double *)c->q0copy, (const double *)c->qn, M, c->res_buf + 1);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(RES_THREADS), 0, c->stream,
+                       (const double *)(c->res_buf + 1), std::max(parts, 1), c->cfg.dt, c->res_buf);
+    if ((rc = check_launch())) return rc;
+    double res = 0.0;
+    HIP_TRY(hipMemcpyAsync(&res, c->res_buf, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->last_res = res;
+    c->res_nt = nt;
+    if (res <= c->ctol) {
+      c->conv_nt = nt;
+      c->qs_fresh = false;
+      *converged = true;
+      return CRUNA_HIP_OK;
+    }
+  }
   // set_boundary_condition_q(qn, outside_rhs=.true.) (cruna_hpc.f90:216)
   if (c->has_bcq)
     if ((rc = bc_q<double>(c, c->qn, true))) return rc;
@@ -902,9 +961,10 @@ int ensure_adjoint(cruna_hip_ctx *c) {
   if (!c->qa) HIP_TRY(hipMalloc(&c->qa, 5 * c->N * sizeof(double)));
   if (!c->q0p) {
     void *cells = nullptr;
-    HIP_TRY(hipMalloc(&cells, 256));  // one pointer, one counter
+    HIP_TRY(hipMalloc(&cells, 256));  // one pointer, two counters
     c->q0p = (const double **)cells;
     c->ntp = (int *)((char *)cells + 128);
+    c->sckp = (int *)((char *)cells + 160);
   }
   return CRUNA_HIP_OK;
 }
@@ -1154,7 +1214,7 @@ int cruna_hip_destroy(cruna_hip_ctx *c) {
   for (void *p : {(void *)c->qn, (void *)c->acc, (void *)c->qref, (void *)c->shape, (void *)c->tmp,
                   (void *)c->qa, (void *)c->traj, (void *)c->gobj, c->met, (void *)c->q0p, c->force,
                   (void *)c->ckpt, c->tg, c->pen, (void *)c->wphi, (void *)c->wtmp, (void *)c->wbuf,
-                  (void *)c->kvec,
+                  (void *)c->kvec, (void *)c->q0copy, (void *)c->res_buf,
                   c->qs, c->rhs, c->G})
     if (p) (void)hipFree(p);
   for (auto &p : c->pending) {
@@ -1436,9 +1496,7 @@ int cruna_hip_rhs_adjoint(cruna_hip_ctx *c, double *rhs, const double *qs, const
                           const double *g, double t) {
   (void)t;  // the adjoint RHS of the reference does not depend on t
   if (!c || !rhs || !qs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
-  if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
-  if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
-  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
+  if (int rc = adjoint_checks(c)) return rc;
   if (int rc = stage_adjoint_inputs(c, qs, q, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
   if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
@@ -1465,9 +1523,7 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *
                                    const double *q0, const double *g, double t) {
   (void)t;
   if (!c || !qs1 || !qs0 || !q0) return fail(CRUNA_HIP_EINVAL, "null argument");
-  if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
-  if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
-  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
+  if (int rc = adjoint_checks(c)) return rc;
   if (int rc = stage_adjoint_inputs(c, qs0, q0, g)) return rc;
   const double *gd = g ? c->gobj : nullptr;
   if (int rc = point_q0(c, c->tmp + 5 * c->N)) return rc;
@@ -1542,9 +1598,7 @@ int cruna_hip_trajectory_record(cruna_hip_ctx *c, int on) {
 
 int cruna_hip_run_adjoint(cruna_hip_ctx *c, int nt_end, int nsteps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
-  if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
-  if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
-  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
+  if (int rc = adjoint_checks(c)) return rc;
   if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
   if (!c->qa) return fail(CRUNA_HIP_EINVAL, "no adjoint state (cruna_hip_set_adjoint_state)");
   if (c->traj_slots == 0) return fail(CRUNA_HIP_EINVAL, "no trajectory (cruna_hip_trajectory_alloc)");
@@ -1562,11 +1616,44 @@ int cruna_hip_run_direct(cruna_hip_ctx *c, int nt0, int nsteps) {
   return run_direct_loop(c, nt0, nsteps, true);
 }
 
+int cruna_hip_set_convergence(cruna_hip_ctx *c, int cfreq, double ctol) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (cfreq < 0) return fail(CRUNA_HIP_EINVAL, "cfreq must be >= 0");
+  ++c->gen;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (cfreq > 0 && !c->q0copy) {
+    if (hipMalloc(&c->q0copy, 5 * (size_t)c->N * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->res_buf, (1 + RES_MAX_PARTS) * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(CRUNA_HIP_ENOMEM, "convergence buffers do not fit in device memory");
+    }
+  }
+  c->cfreq = cfreq;
+  c->ctol = ctol;
+  c->last_res = -1.0;
+  c->res_nt = 0;
+  c->conv_nt = 0;
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_get_residual(cruna_hip_ctx *c, double *residual, int *nt_checked, int *nt_converged) {
+  if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (residual) *residual = c->last_res;
+  if (nt_checked) *nt_checked = c->res_nt;
+  if (nt_converged) *nt_converged = c->conv_nt;
+  return CRUNA_HIP_OK;
+}
+
 extern "C++" int run_direct_loop(cruna_hip_ctx *c, int nt0, int nsteps, bool weighted) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (nsteps < 0) return fail(CRUNA_HIP_EINVAL, "nsteps must be >= 0");
+  // the convergence check belongs to the cruna_hpc loop (weighted filters);
+  // calc_direct's subset loop has none
+  bool conv = false;
+  if (weighted) c->conv_nt = 0;
   for (int nt = nt0 + 1; nt <= nt0 + nsteps; ++nt) {
-    if (int rc = step_once(c, nt, weighted)) return rc;
+    if (int rc = step_once(c, nt, weighted, weighted ? &conv : nullptr)) return rc;
+    if (conv) break;  // stop_cruna after the snapshot of q1 (cruna_hpc.f90:190-206)
     // Q(:,:,:,:,nt) = qn (computation_direct.f90:100-103)
     if (c->record && nt >= 1 && nt <= c->traj_slots)
       HIP_TRY(hipMemcpyAsync(traj_slot(c, nt), c->qn, 5 * c->N * sizeof(double),
@@ -1583,6 +1670,7 @@ int cruna_hip_subsets_alloc(cruna_hip_ctx *c, int subsets, int steps) {
   if (c->ckpt) HIP_TRY(hipFree(c->ckpt));
   c->ckpt = nullptr;
   c->ckpt_n = 0;
+  c->sub_steps = 0;
   if (subsets == 0) return cruna_hip_trajectory_alloc(c, 0);
   if (hipMalloc(&c->ckpt, (size_t)(subsets + 1) * 5 * c->N * sizeof(double)) != hipSuccess) {
     (void)hipGetLastError();
@@ -1590,6 +1678,7 @@ int cruna_hip_subsets_alloc(cruna_hip_ctx *c, int subsets, int steps) {
     return fail(CRUNA_HIP_ENOMEM, "subset checkpoints do not fit in device memory");
   }
   c->ckpt_n = subsets + 1;
+  c->sub_steps = steps;
   return cruna_hip_trajectory_alloc(c, steps);
 }
 
@@ -1615,9 +1704,7 @@ int cruna_hip_run_direct_subsets(cruna_hip_ctx *c, int subsets, int steps) {
 
 int cruna_hip_run_adjoint_subsets(cruna_hip_ctx *c, int subsets, int steps) {
   if (!c) return fail(CRUNA_HIP_EINVAL, "null argument");
-  if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
-  if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
-  if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
+  if (int rc = adjoint_checks(c)) return rc;
   if (subsets < 1 || steps < 1) return fail(CRUNA_HIP_EINVAL, "subsets and steps must be >= 1");
   if (c->ckpt_n < subsets + 1) return fail(CRUNA_HIP_EINVAL, "not enough subset checkpoints (cruna_hip_subsets_alloc)");
   if (c->traj_slots < steps) return fail(CRUNA_HIP_EINVAL, "the trajectory holds fewer slots than steps");
@@ -1758,6 +1845,11 @@ struct cruna_hip_ensemble {
   std::vector<cruna_hip_ctx *> m;
   std::vector<int> role;  // CRUNA_HIP_FORWARD / CRUNA_HIP_ADJOINT
   std::vector<int> nt;    // forward: steps done; adjoint: next step to run backward
+  // adjoint members with subset checkpoints (time.subsets): the subset loop
+  // of calc_adjoint, each subset's trajectory recomputed from its checkpoint;
+  // pos = backward steps left in the recomputed subset (0: recompute next)
+  std::vector<bool> subm;
+  std::vector<int> pos;
   std::vector<bool> warm;
   std::vector<hipGraphExec_t> exec;  // one period of steps of each member
   std::vector<unsigned> gen;         // each member's setter generation when recorded
@@ -1771,14 +1863,50 @@ int gcd_i(int a, int b) { return b ? gcd_i(b, a % b) : a; }
 int lcm_i(int a, int b) { return a / gcd_i(a, b) * b; }
 
 // Steps after which a member's launch sequence repeats: its filter
-// frequencies (direct or adjoint) decide which passes run at a step.
-int member_period(const cruna_hip_ctx *c, int role) {
+// frequencies (direct: the compact and the weighted filters; adjoint: the
+// adjoint filters) decide which passes run at a step; a subset adjoint
+// member repeats with every subset (step numbers restart at 1 in each).
+int member_period(const cruna_hip_ctx *c, int role, bool subset) {
+  if (role == CRUNA_HIP_ADJOINT && subset) return c->sub_steps;
   int p = 1;
   for (int d = 0; d < 3; ++d) {
     const int f = role == CRUNA_HIP_ADJOINT ? c->cfg.afilter_freq[d] : c->cfg.filter_freq[d];
     if (f > 0) p = lcm_i(p, f);
+    if (role == CRUNA_HIP_FORWARD && c->cfg.wfilter_freq[d] > 0) p = lcm_i(p, c->cfg.wfilter_freq[d]);
   }
   return p;
+}
+
+// One backward step of a subset adjoint member: at the start of a subset
+// restore its checkpoint (named by the device counter, which then counts
+// down) and recompute the subset's trajectory (calc_direct's subset loop,
+// nt = 1..steps, no weighted filters), then the adjoint step over it.
+int subset_astep(cruna_hip_ensemble *e, size_t i) {
+  cruna_hip_ctx *c = e->m[i];
+  const int n = c->sub_steps;
+  if (e->pos[i] == 0) {
+    const long M = 5 * c->N;
+    hipLaunchKernelGGL(k_restore_ckpt, dim3((unsigned)std::min<long>((M + 255) / 256, 4096)), dim3(256), 0,
+                       c->stream, c->qn, (const double *)c->ckpt, (const int *)c->sckp, M);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->sckp, INT_MIN, -1);
+    if (int rc = check_launch()) return rc;
+    c->qs_fresh = false;
+    const bool rec = c->record;
+    c->record = true;
+    int rc = run_direct_loop(c, 0, n, false);
+    c->record = rec;
+    if (rc) return rc;
+    // backward from the subset's last step, or from within it right after
+    // set_role named such a step
+    const int p = (e->nt[i] - 1) % n + 1;
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->ntp, p, 0);
+    if ((rc = check_launch())) return rc;
+    e->pos[i] = p;
+  }
+  if (int rc = astep_once(c, e->pos[i])) return rc;
+  --e->pos[i];
+  --e->nt[i];
+  return CRUNA_HIP_OK;
 }
 
 int member_steps(cruna_hip_ensemble *e, size_t i, int k) {
@@ -1786,6 +1914,8 @@ int member_steps(cruna_hip_ensemble *e, size_t i, int k) {
   for (int s = 0; s < k; ++s) {
     if (e->role[i] == CRUNA_HIP_FORWARD) {
       if (int rc = step_once(c, ++e->nt[i])) return rc;
+    } else if (e->subm[i]) {
+      if (int rc = subset_astep(e, i)) return rc;
     } else {
       if (int rc = astep_once(c, e->nt[i]--)) return rc;
     }
@@ -1816,6 +1946,8 @@ int cruna_hip_ensemble_create(const cruna_hip_cfg *cfgs, int nmembers, cruna_hip
     e->m.push_back(c);
     e->role.push_back(CRUNA_HIP_FORWARD);
     e->nt.push_back(0);
+    e->subm.push_back(false);
+    e->pos.push_back(0);
     e->warm.push_back(false);
   }
   *out = e.release();
@@ -1843,18 +1975,26 @@ int cruna_hip_ensemble_set_role(cruna_hip_ensemble *e, int i, int role, int nt) 
   if (role != CRUNA_HIP_FORWARD && role != CRUNA_HIP_ADJOINT) return fail(CRUNA_HIP_EINVAL, "bad role");
   cruna_hip_ctx *c = e->m[i];
   if (role == CRUNA_HIP_ADJOINT) {
-    if (c->ev) return fail(CRUNA_HIP_EINVAL, kNoEvAdjoint);
-    if (c->has_char_rhs) return fail(CRUNA_HIP_EINVAL, kNoCharRhsAdjoint);
-    if (c->cfg.equations == CRUNA_HIP_EQ_EULER && c->gt) return fail(CRUNA_HIP_EINVAL, kNoEulerGtAdjoint);
+    if (int rc = adjoint_checks(c)) return rc;
     if (!c->qa || c->traj_slots == 0)
       return fail(CRUNA_HIP_EINVAL, "an adjoint member needs its adjoint state and trajectory first");
-    if (c->traj_slots > 1 && (nt < 1 || nt > c->traj_slots))
+    if (c->ckpt_n >= 2) {
+      // subset checkpoints: nt counts the backward steps over all subsets
+      const int total = (c->ckpt_n - 1) * c->sub_steps;
+      if (nt < 1 || nt > total) return fail(CRUNA_HIP_EINVAL, "nt outside the subsets' steps");
+      const int sidx = (nt - 1) / c->sub_steps;
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      HIP_TRY(hipMemcpy(c->sckp, &sidx, sizeof(int), hipMemcpyHostToDevice));
+    } else if (c->traj_slots > 1 && (nt < 1 || nt > c->traj_slots)) {
       return fail(CRUNA_HIP_EINVAL, "nt outside the trajectory");
+    }
   }
   if (role == CRUNA_HIP_FORWARD && c->record)
     return fail(CRUNA_HIP_EINVAL, "forward members do not record trajectories");
   e->role[i] = role;
   e->nt[i] = nt;
+  e->subm[i] = role == CRUNA_HIP_ADJOINT && c->ckpt_n >= 2;
+  e->pos[i] = 0;
   e->warm[i] = false;
   e->dirty = true;
   return CRUNA_HIP_OK;
@@ -1866,18 +2006,19 @@ int cruna_hip_ensemble_run(cruna_hip_ensemble *e, int nsteps) {
   const size_t M = e->m.size();
   int P = 1;
   for (size_t i = 0; i < M; ++i) {
-    P = lcm_i(P, member_period(e->m[i], e->role[i]));
-    if (e->role[i] == CRUNA_HIP_ADJOINT && e->m[i]->traj_slots > 1 && e->nt[i] - nsteps < 0)
+    P = lcm_i(P, member_period(e->m[i], e->role[i], e->subm[i]));
+    if (e->role[i] == CRUNA_HIP_ADJOINT && (e->m[i]->traj_slots > 1 || e->subm[i]) && e->nt[i] - nsteps < 0)
       return fail(CRUNA_HIP_EINVAL, "an adjoint member would run past its first step");
     // a closure's work buffers can move when it trains, under a recorded graph
     if (e->m[i]->cnn) return fail(CRUNA_HIP_EINVAL, "ensemble members cannot carry a learned closure");
   }
   if (P > 64) return fail(CRUNA_HIP_EINVAL, "filter frequencies repeat only after more than 64 steps");
-  // adjoint step counters live on the device; set them for this run
+  // adjoint step counters live on the device; set them for this run (the
+  // counters of subset members are kept by their own kernels)
   for (size_t i = 0; i < M; ++i) {
     cruna_hip_ctx *c = e->m[i];
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (e->role[i] == CRUNA_HIP_ADJOINT)
+    if (e->role[i] == CRUNA_HIP_ADJOINT && !e->subm[i])
       HIP_TRY(hipMemcpy(c->ntp, &e->nt[i], sizeof(int), hipMemcpyHostToDevice));
     c->prof = false;
   }
@@ -1904,7 +2045,7 @@ int cruna_hip_ensemble_run(cruna_hip_ensemble *e, int nsteps) {
   if (e->dirty || e->exec.size() != M || e->period != P) {
     drop_graphs(e);
     e->exec.assign(M, nullptr);
-    std::vector<int> nt_save = e->nt;
+    std::vector<int> nt_save = e->nt, pos_save = e->pos;
     for (size_t i = 0; i < M; ++i) {
       cruna_hip_ctx *c = e->m[i];
       HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1915,6 +2056,7 @@ int cruna_hip_ensemble_run(cruna_hip_ensemble *e, int nsteps) {
       if (rc || ec != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
         e->nt = nt_save;
+        e->pos = pos_save;
         drop_graphs(e);
         return rc ? rc : fail(CRUNA_HIP_EDEVICE, std::string("capture: ") + hipGetErrorString(ec));
       }
@@ -1923,11 +2065,13 @@ int cruna_hip_ensemble_run(cruna_hip_ensemble *e, int nsteps) {
       if (ei != hipSuccess) {
         e->exec[i] = nullptr;
         e->nt = nt_save;
+        e->pos = pos_save;
         drop_graphs(e);
         return fail(CRUNA_HIP_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
       }
     }
     e->nt = nt_save;
+    e->pos = pos_save;
     e->period = P;
     e->gen.resize(M);
     for (size_t i = 0; i < M; ++i) e->gen[i] = e->m[i]->gen;

@@ -133,6 +133,18 @@ module cruna_hip
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function c_set_convergence(ctx, cfreq, ctol) bind(C, name="cruna_hip_set_convergence")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: cfreq
+      real(c_double), value :: ctol
+    end function
+    integer(c_int) function c_get_residual(ctx, res, nt_checked, nt_conv) bind(C, name="cruna_hip_get_residual")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: res
+      integer(c_int), intent(out) :: nt_checked, nt_conv
+    end function
     integer(c_int) function c_rhs_adjoint(ctx, rhs, qs, q, g, t) bind(C, name="cruna_hip_rhs_adjoint")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
@@ -295,6 +307,7 @@ module cruna_hip
   public :: hip_create, hip_destroy, hip_set_state, hip_get_state, hip_set_qref
   public :: hip_set_sponge_shape, hip_set_kvector, hip_diff, hip_filter, hip_filter_weighted, hip_rhs_direct
   public :: hip_time_stepper_direct, hip_run_direct, hip_synchronize
+  public :: hip_set_convergence, hip_get_residual
   public :: hip_rhs_adjoint, hip_time_stepper_adjoint, hip_set_adjoint_state
   public :: hip_get_adjoint_state, hip_trajectory_alloc, hip_trajectory_set
   public :: hip_trajectory_get, hip_trajectory_record, hip_run_adjoint
@@ -418,6 +431,26 @@ contains
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: nt0, nsteps
     call check(c_run_direct(ctx, int(nt0, c_int), int(nsteps, c_int)), "run_direct")
+  end subroutine
+
+  ! run.cfreq / run.ctol of the cruna_hpc loop (cruna_hpc.f90:81-82, 185-208)
+  subroutine hip_set_convergence(ctx, cfreq, ctol)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: cfreq
+    real(c_double), intent(in) :: ctol
+    call check(c_set_convergence(ctx, int(cfreq, c_int), ctol), "set_convergence")
+  end subroutine
+
+  ! last residual, the step it was computed at, and the step at which the
+  ! last hip_run_direct reached run.ctol (0: not reached)
+  subroutine hip_get_residual(ctx, res, nt_checked, nt_conv)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(out) :: res
+    integer, intent(out) :: nt_checked, nt_conv
+    integer(c_int) :: a, b
+    call check(c_get_residual(ctx, res, a, b), "get_residual")
+    nt_checked = a
+    nt_conv = b
   end subroutine
 
   subroutine hip_synchronize(ctx)

@@ -300,3 +300,27 @@ def test_initial_condition_matches_its_specification():
     assert abs(slope) < 0.2, slope
     # k^4 exp(-2 (k/4)^2) peaks where 4 k^3 = k^5 / 4, at k = 4
     assert 3 <= int(np.argmax(spec[:12])) <= 5
+
+
+# --------------------------------------------------------------------------- convergence check
+def test_run_direct_conv_is_the_loop_with_its_residual():
+    """cruna_hpc.f90:185-208: the residual at a checked step is
+    max|qn - q1|/dt of that step's stepper call, and reaching run.ctol ends the
+    loop with the unfiltered q1; without a check the loop is orc_run_direct."""
+    n = 12
+    h = 2 * np.pi / n
+    q = ic.periodic_box_state(n, 3, 0.3)
+    kw = dict(mu0=0.01, T0=1 / (1.4 * 0.09), dt=ic.cfl_dt(q, h), filter_freq=(1, 1, 1))
+    cfg = O.Cfg((n,) * 3, (h,) * 3, **kw)
+    q6, res, ntc = O.run_direct_conv(cfg, q, 6, 0, 0.0)
+    assert np.array_equal(q6, O.run_direct(cfg, q, 6)) and res == -1.0 and ntc == 0
+    q4 = O.run_direct(cfg, q, 4)
+    q5 = O.time_stepper_direct(cfg, q4)
+    r5 = np.abs((q4 - q5) / kw["dt"]).max()
+    assert r5 > 1.0
+    # checks at 5 (and 10): tolerance above the residual -> stop at 5 with q1
+    got, res, ntc = O.run_direct_conv(cfg, q, 10, 5, 1.01 * r5)
+    assert ntc == 5 and res == r5 and np.array_equal(got, q5)
+    # tolerance below it: no stop, the residual of the last check is reported
+    got, res, ntc = O.run_direct_conv(cfg, q, 6, 5, 0.5 * r5)
+    assert ntc == 0 and res == r5 and np.array_equal(got, q6)
