@@ -75,6 +75,7 @@ struct DevOp {
     op.NB = h.NB;
     op.K = h.K;
     op.per = h.per;
+    op.spec2 = h.spec2;
     return CRUNA_HIP_OK;
   }
   void release() {
@@ -92,6 +93,11 @@ struct cruna_hip_ctx {
   cruna_hip_cfg cfg{};
   int device = 0;
   hipStream_t stream = nullptr;
+  // device layout: an open direction whose length is not a multiple of 8 is
+  // stored padded to np[d] points with pf[d] of them in front of the real
+  // ones (decoupled rows of the line operators); N counts the padded points
+  int np[3] = {0, 0, 0}, pf[3] = {0, 0, 0};
+  bool padded = false;
   long N = 0;
   SweepGeom geom[3]{};
   int wg[3] = {0, 0, 0};
@@ -183,12 +189,12 @@ namespace {
 
 // Lines per work-group: TL lines of NB threads each (NB * TL <= wg_max); y/z
 // tiles are TL neighbouring x-positions, so their rows are TL elements long.
-SweepGeom make_geom(const cruna_hip_cfg &cfg, int d, int TL) {
+SweepGeom make_geom(const int *np, int d, int TL) {
   SweepGeom g{};
-  const int n1 = cfg.n[0], n2 = cfg.n[1], n3 = cfg.n[2];
+  const int n1 = np[0], n2 = np[1], n3 = np[2];
   g.dir = d;
-  g.n = cfg.n[d];
-  g.NB = cfg.n[d] / PPT;
+  g.n = np[d];
+  g.NB = np[d] / PPT;
   g.n1 = n1;
   g.n2 = n2;
   g.n3 = n3;
@@ -205,9 +211,9 @@ SweepGeom make_geom(const cruna_hip_cfg &cfg, int d, int TL) {
   return g;
 }
 
-int auto_tl(const cruna_hip_cfg &cfg, int d, int wg_max, int tl_max) {
-  const long lines_contig = d == 0 ? (long)cfg.n[1] * cfg.n[2] : cfg.n[0];
-  const int NB = cfg.n[d] / PPT;
+int auto_tl(const int *np, int d, int wg_max, int tl_max) {
+  const long lines_contig = d == 0 ? (long)np[1] * np[2] : np[0];
+  const int NB = np[d] / PPT;
   int TL = 1;
   while (NB * TL * 2 <= wg_max && lines_contig % (TL * 2) == 0 && TL * 2 <= tl_max) TL *= 2;
   return TL;
@@ -215,25 +221,26 @@ int auto_tl(const cruna_hip_cfg &cfg, int d, int wg_max, int tl_max) {
 
 int build_geom(cruna_hip_ctx *c, int d) {
   const cruna_hip_cfg &cfg = c->cfg;
-  const long lines_contig = d == 0 ? (long)cfg.n[1] * cfg.n[2] : cfg.n[0];
-  const int NB = cfg.n[d] / PPT;
+  const int *np = c->np;
+  const long lines_contig = d == 0 ? (long)np[1] * np[2] : np[0];
+  const int NB = np[d] / PPT;
   int TL = cfg.tile_lines[d];
   if (TL == 0) {
     // x-lines are contiguous: up to 256 threads; y/z tiles take up to 16
     // neighbouring x-lines (128-byte rows in FP64) in up to 512 threads
-    TL = d == 0 ? auto_tl(cfg, d, 256, 1 << 20) : auto_tl(cfg, d, NT, 16);
+    TL = d == 0 ? auto_tl(np, d, 256, 1 << 20) : auto_tl(np, d, NT, 16);
   } else if (TL < 0 || (TL & (TL - 1)) != 0 || lines_contig % TL != 0 || NB * TL > NT) {
     return fail(CRUNA_HIP_EINVAL, "tile_lines[" + std::to_string(d) + "] = " + std::to_string(TL) +
                                       ": needs a power of two dividing " +
                                       std::to_string(lines_contig) + " with (n/8)*tile_lines <= " +
                                       std::to_string(NT));
   }
-  c->geom[d] = make_geom(cfg, d, TL);
+  c->geom[d] = make_geom(np, d, TL);
   c->wg[d] = NB * TL;
   // filter passes: FP32 y/z tiles up to 32 lines (128-byte rows of floats)
   int FTL = TL;
-  if (d != 0 && cfg.precision == CRUNA_HIP_FP32_MIXED) FTL = auto_tl(cfg, d, NT_FILTER32, 32);
-  c->fgeom[d] = make_geom(cfg, d, FTL);
+  if (d != 0 && cfg.precision == CRUNA_HIP_FP32_MIXED) FTL = auto_tl(np, d, NT_FILTER32, 32);
+  c->fgeom[d] = make_geom(np, d, FTL);
   c->fwg[d] = NB * FTL;
   return CRUNA_HIP_OK;
 }
@@ -410,21 +417,21 @@ int bc_q(cruna_hip_ctx *c, Q *q, bool outside = false) {
       const BoxRow &b = r.second;
       const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
       hipLaunchKernelGGL((k_bc_characteristic_q<Q, double>), dim3((np + 255) / 256), dim3(256), 0,
-                         c->stream, q, (const double *)c->qref, b, c->cfg.n[0], c->cfg.n[1], c->N,
+                         c->stream, q, (const double *)c->qref, b, c->np[0], c->np[1], c->N,
                          c->cfg.gamma, (const double *)nullptr, (const double *)nullptr);
     } else if (r.first == 202 || r.first == 203) {
       if (r.first == 203 && !zone_applies(c, outside)) continue;
       const BoxRow &b = r.second;
       const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
       hipLaunchKernelGGL((k_bc_characteristic_q<Q, double>), dim3((np + 255) / 256), dim3(256), 0,
-                         c->stream, q, (const double *)c->qref, b, c->cfg.n[0], c->cfg.n[1], c->N,
+                         c->stream, q, (const double *)c->qref, b, c->np[0], c->np[1], c->N,
                          c->cfg.gamma, (const double *)c->kvec,
                          r.first == 203 ? (const double *)c->shape : (const double *)nullptr);
     } else if (r.first == 200) {
       const BoxRow &b = r.second;
       const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
       hipLaunchKernelGGL((k_bc_characteristic_q<Q, Q>), dim3((np + 255) / 256), dim3(256), 0,
-                         c->stream, q, (const Q *)q, b, c->cfg.n[0], c->cfg.n[1], c->N, c->cfg.gamma,
+                         c->stream, q, (const Q *)q, b, c->np[0], c->np[1], c->N, c->cfg.gamma,
                          (const double *)nullptr, (const double *)nullptr);
     } else if (r.first == 401) {
       hipLaunchKernelGGL((k_bc_sponge_q<Q>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream, q,
@@ -442,7 +449,7 @@ int bc_q(cruna_hip_ctx *c, Q *q, bool outside = false) {
 template <typename T, typename Q>
 int bc_rhs(cruna_hip_ctx *c, T *rhs, const Q *q, bool late = false) {
   if (int rc = need_bc_inputs(c)) return rc;
-  const int n1 = c->cfg.n[0], n2 = c->cfg.n[1];
+  const int n1 = c->np[0], n2 = c->np[1];
   for (auto &r : c->bc) {
     const BoxRow &b = r.second;
     const long np = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
@@ -704,7 +711,7 @@ int filter_dir(cruna_hip_ctx *c, double *q, int d, int nv, T *copy = nullptr) {
 int filter_weighted_dir(cruna_hip_ctx *c, double *q, double *out, int d, int nv, const double *w,
                         const double *s) {
   const dim3 grid((c->N + 255) / 256), block(256);
-  const int n1 = c->cfg.n[0], n2 = c->cfg.n[1], n3 = c->cfg.n[2];
+  const int n1 = c->np[0], n2 = c->np[1], n3 = c->np[2];
   if (d == 0)
     hipLaunchKernelGGL(k_filter_weighted<0>, grid, block, 0, c->stream, q, out, nv, w, s, n1, n2, n3, c->N);
   else if (d == 1)
@@ -792,6 +799,80 @@ int step_once(cruna_hip_ctx *c, int nt, bool weighted = true, bool *converged = 
   return CRUNA_HIP_OK;
 }
 
+// Host arrays hold the reference's layout f(n1,n2,n3[,nv]); device fields the
+// padded one, real point (i,j,k) at (i+pf1, j+pf2, k+pf3) of (np1,np2,np3).
+// The padding points hold a uniform state, which the line operators keep
+// away from the real points and which stays constant (a zero RHS).
+const double kRestState[5] = {1.0, 0.0, 0.0, 0.0, 1.0};
+
+// nv host fields in the padded layout: the real points copied, the padding
+// filled with fill[v] (fill null: 0) or, with replicate, with the value of
+// the nearest real point (keeps a field constant along lines constant).
+std::vector<double> pad_host(const cruna_hip_ctx *c, const double *h, int nv, const double *fill,
+                             bool replicate = false) {
+  const long n1 = c->cfg.n[0], n2 = c->cfg.n[1], n3 = c->cfg.n[2];
+  const long p1 = c->np[0], p2 = c->np[1], p3 = c->np[2], N = c->N, Nr = n1 * n2 * n3;
+  std::vector<double> out((size_t)nv * N);
+  for (int v = 0; v < nv; ++v)
+    for (long k = 0; k < p3; ++k)
+      for (long j = 0; j < p2; ++j)
+        for (long i = 0; i < p1; ++i) {
+          long ri = i - c->pf[0], rj = j - c->pf[1], rk = k - c->pf[2];
+          const bool real = ri >= 0 && ri < n1 && rj >= 0 && rj < n2 && rk >= 0 && rk < n3;
+          double val;
+          if (real || replicate) {
+            ri = std::min(std::max(ri, 0L), n1 - 1);
+            rj = std::min(std::max(rj, 0L), n2 - 1);
+            rk = std::min(std::max(rk, 0L), n3 - 1);
+            val = h[v * Nr + ri + n1 * (rj + n2 * rk)];
+          } else {
+            val = fill ? fill[v] : 0.0;
+          }
+          out[(size_t)v * N + i + p1 * (j + p2 * k)] = val;
+        }
+  return out;
+}
+
+// the real points of nv padded device-layout fields into a host array
+void unpad_host(const cruna_hip_ctx *c, const std::vector<double> &d, double *h, int nv) {
+  const long n1 = c->cfg.n[0], n2 = c->cfg.n[1], n3 = c->cfg.n[2];
+  const long p1 = c->np[0], p2 = c->np[1], N = c->N, Nr = n1 * n2 * n3;
+  for (int v = 0; v < nv; ++v)
+    for (long k = 0; k < n3; ++k)
+      for (long j = 0; j < n2; ++j)
+        for (long i = 0; i < n1; ++i)
+          h[v * Nr + i + n1 * (j + n2 * k)] =
+              d[(size_t)v * N + (i + c->pf[0]) + p1 * ((j + c->pf[1]) + p2 * (k + c->pf[2]))];
+}
+
+// host -> device fields (asynchronous on the context stream when unpadded,
+// as the plain copies were; waits for the staging copy when padded)
+int put_fields(cruna_hip_ctx *c, double *dev, const double *host, int nv, const double *fill) {
+  const size_t bytes = (size_t)nv * c->N * sizeof(double);
+  if (!c->padded) {
+    HIP_TRY(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream));
+    return CRUNA_HIP_OK;
+  }
+  std::vector<double> v = pad_host(c, host, nv, fill);
+  HIP_TRY(hipMemcpyAsync(dev, v.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CRUNA_HIP_OK;
+}
+
+// device -> host fields (the real points)
+int get_fields(cruna_hip_ctx *c, double *host, const double *dev, int nv) {
+  const size_t bytes = (size_t)nv * c->N * sizeof(double);
+  if (!c->padded) {
+    HIP_TRY(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+    return CRUNA_HIP_OK;
+  }
+  std::vector<double> v((size_t)nv * c->N);
+  HIP_TRY(hipMemcpyAsync(v.data(), dev, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  unpad_host(c, v, host, nv);
+  return CRUNA_HIP_OK;
+}
+
 // Staging area of the host-pointer entry points: two states.
 int ensure_tmp(cruna_hip_ctx *c) {
   if (!c->tmp) HIP_TRY(hipMalloc(&c->tmp, 10 * c->N * sizeof(double)));
@@ -805,10 +886,14 @@ int copy_rhs_out(cruna_hip_ctx *c, double *rhs) {
     HIP_TRY(hipMemcpyAsync(h.data(), c->rhs, 5 * c->N * sizeof(float), hipMemcpyDeviceToHost,
                            c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    for (long i = 0; i < 5 * c->N; ++i) rhs[i] = h[i];
+    if (!c->padded) {
+      for (long i = 0; i < 5 * c->N; ++i) rhs[i] = h[i];
+    } else {
+      std::vector<double> d(h.begin(), h.end());
+      unpad_host(c, d, rhs, 5);
+    }
   } else {
-    HIP_TRY(hipMemcpyAsync(rhs, c->rhs, 5 * c->N * sizeof(double), hipMemcpyDeviceToHost,
-                           c->stream));
+    if (int rc = get_fields(c, rhs, (const double *)c->rhs, 5)) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
   }
   return CRUNA_HIP_OK;
@@ -832,7 +917,7 @@ int bc_qs(cruna_hip_ctx *c, Q *qs, bool outside = false) {
       if (r.first == 203 && !c->shape)
         return fail(CRUNA_HIP_EINVAL, "zonal rows (203/206) need a shape (cruna_hip_set_sponge_shape)");
       hipLaunchKernelGGL((k_bc_characteristic_qs<Q>), dim3((np + 255) / 256), dim3(256), 0,
-                         c->stream, qs, (const double *const *)c->q0p, b, c->cfg.n[0], c->cfg.n[1],
+                         c->stream, qs, (const double *const *)c->q0p, b, c->np[0], c->np[1],
                          c->N, c->cfg.gamma,
                          r.first >= 202 ? (const double *)c->kvec : (const double *)nullptr,
                          r.first == 203 ? (const double *)c->shape : (const double *)nullptr);
@@ -884,7 +969,7 @@ int arhs_stage(cruna_hip_ctx *c, const Q *in, const double *g, int mode) {
   const bool euler_sponge = c->cfg.equations == CRUNA_HIP_EQ_EULER && c->has_bcrhs;
   if (!c->has_wall && !euler_sponge) return launch_asweep<T, Q, 0>(c, in, g, mode);
   if (int rc = launch_asweep<T, Q, 0>(c, in, g, 4)) return rc;
-  const int n1 = c->cfg.n[0], n2 = c->cfg.n[1];
+  const int n1 = c->np[0], n2 = c->np[1];
   for (auto &r : c->bc) {
     if (r.first == 400 && euler_sponge) {
       if (!c->shape) return fail(CRUNA_HIP_EINVAL, "sponge rows need a shape (cruna_hip_set_sponge_shape)");
@@ -977,6 +1062,8 @@ int point_q0(cruna_hip_ctx *c, const double *q) {
 
 double *traj_slot(cruna_hip_ctx *c, int nt) { return c->traj + (size_t)(nt - 1) * 5 * c->N; }
 
+
+
 template <typename T>
 int upload_metric(cruna_hip_ctx *c, const std::vector<double> &m1, const std::vector<double> &m2,
                   const std::vector<double> &m3, const double *Xj, Metric<T> &out) {
@@ -1051,14 +1138,17 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   if (!cfg || !out) return fail(CRUNA_HIP_EINVAL, "null argument");
   *out = nullptr;
   for (int d = 0; d < 3; ++d) {
-    if (cfg->n[d] < 16 || cfg->n[d] % PPT != 0)
-      return fail(CRUNA_HIP_EINVAL, "n" + std::to_string(d + 1) + " = " +
-                                        std::to_string(cfg->n[d]) +
-                                        ": each grid size must be a multiple of 8 and >= 16");
-    if (cfg->n[d] / PPT > 256)
-      return fail(CRUNA_HIP_EINVAL, "grid size above 2048 points per line is not supported");
     if (cfg->scheme[d] != CRUNA_HIP_PERIODIC && cfg->scheme[d] != CRUNA_HIP_OPEN)
       return fail(CRUNA_HIP_EINVAL, "unknown scheme");
+    const bool per = cfg->scheme[d] == CRUNA_HIP_PERIODIC;
+    if (per && (cfg->n[d] < 16 || cfg->n[d] % PPT != 0))
+      return fail(CRUNA_HIP_EINVAL, "n" + std::to_string(d + 1) + " = " + std::to_string(cfg->n[d]) +
+                                        ": a periodic direction needs a multiple of 8, >= 16");
+    if (!per && cfg->n[d] < PPT + 1)
+      return fail(CRUNA_HIP_EINVAL, "n" + std::to_string(d + 1) + " = " + std::to_string(cfg->n[d]) +
+                                        ": an open direction needs at least 9 points");
+    if (padded_length(per, cfg->n[d]) / PPT > 256)
+      return fail(CRUNA_HIP_EINVAL, "grid size above 2048 points per line is not supported");
     if (cfg->diff_kind[d] != CRUNA_HIP_DIFF_O6_COMPACT && cfg->diff_kind[d] != CRUNA_HIP_DIFF_O4_EXPLICIT &&
         cfg->diff_kind[d] != CRUNA_HIP_DIFF_O4_PADE && cfg->diff_kind[d] != CRUNA_HIP_DIFF_O2_EXPLICIT)
       return fail(CRUNA_HIP_EINVAL, "unknown derivative core");
@@ -1066,11 +1156,17 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
       return fail(CRUNA_HIP_EINVAL, "the 2nd-order explicit core (diff_pen_core_o2_e_c_o) is for open lines only");
     if (!(cfg->dx[d] > 0)) return fail(CRUNA_HIP_EINVAL, "dx must be positive");
     if (cfg->wfilter_freq[d] < 0) return fail(CRUNA_HIP_EINVAL, "wfilter_freq must be >= 0");
+    if (cfg->wfilter_freq[d] > 0 && padded_length(per, cfg->n[d]) != cfg->n[d])
+      return fail(CRUNA_HIP_EINVAL, "the weighted filter's core is periodic: its direction needs a multiple of 8 points");
   }
   if (cfg->equations != CRUNA_HIP_EQ_NS && cfg->equations != CRUNA_HIP_EQ_EULER)
     return fail(CRUNA_HIP_EINVAL, "unknown equations");
-  if ((double)cfg->n[0] * cfg->n[1] * cfg->n[2] * sizeof(double) >= 4294967296.0)
-    return fail(CRUNA_HIP_EINVAL, "one FP64 field must fit in 4 GiB (32-bit device offsets)");
+  {
+    double pts = 1.0;
+    for (int d = 0; d < 3; ++d) pts *= padded_length(cfg->scheme[d] == CRUNA_HIP_PERIODIC, cfg->n[d]);
+    if (pts * sizeof(double) >= 4294967296.0)
+      return fail(CRUNA_HIP_EINVAL, "one FP64 field must fit in 4 GiB (32-bit device offsets)");
+  }
   if (cfg->precision != CRUNA_HIP_FP64 && cfg->precision != CRUNA_HIP_FP32_MIXED)
     return fail(CRUNA_HIP_EINVAL, "unknown precision");
   if (cfg->nbc < 0 || cfg->nbc > CRUNA_HIP_MAX_BC) return fail(CRUNA_HIP_EINVAL, "bad nbc");
@@ -1085,7 +1181,13 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   c->cfg = *cfg;
   HIP_TRY(hipGetDevice(&c->device));
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  c->N = (long)cfg->n[0] * cfg->n[1] * cfg->n[2];
+  for (int d = 0; d < 3; ++d) {
+    const bool per = cfg->scheme[d] == CRUNA_HIP_PERIODIC;
+    c->np[d] = padded_length(per, cfg->n[d]);
+    c->pf[d] = front_padding(per, cfg->n[d]);
+    c->padded = c->padded || c->np[d] != cfg->n[d];
+  }
+  c->N = (long)c->np[0] * c->np[1] * c->np[2];
   for (int r = 0; r < cfg->nbc; ++r) {
     const int t = (int)lround(cfg->bc[r][0]);
     if (t == 0) continue;
@@ -1113,6 +1215,11 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
       c->needs_shape = true;
     }
     if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) continue;
+    // device indices of the padded layout
+    for (int d = 0; d < 3; ++d) {
+      lo[d] += c->pf[d];
+      hi[d] += c->pf[d];
+    }
     b.i0 = lo[0]; b.i1 = hi[0]; b.j0 = lo[1]; b.j1 = hi[1]; b.k0 = lo[2]; b.k1 = hi[2];
     b.aim_shift = 0;
     if (t == 200) {
@@ -1128,10 +1235,10 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
         shift[d] = kd > 0 ? -1 : (kd < 0 ? 1 : 0);
         if (shift[d] != 0 && lo[d] != hi[d])
           return fail(CRUNA_HIP_EINVAL, "type 200 row: the box must be one point thick along its normal");
-        if (lo[d] + shift[d] < 0 || hi[d] + shift[d] > cfg->n[d] - 1)
+        if (lo[d] - c->pf[d] + shift[d] < 0 || hi[d] - c->pf[d] + shift[d] > cfg->n[d] - 1)
           return fail(CRUNA_HIP_EINVAL, "type 200 row: inner neighbour outside the grid");
       }
-      b.aim_shift = shift[0] + (long)cfg->n[0] * (shift[1] + (long)cfg->n[1] * shift[2]);
+      b.aim_shift = shift[0] + (long)c->np[0] * (shift[1] + (long)c->np[1] * shift[2]);
     }
     if (t == 100) {  // the variable whose RHS is zero (1-based, in column 8)
       const int v = (int)lround(cfg->bc[r][7]);
@@ -1230,14 +1337,14 @@ int cruna_hip_destroy(cruna_hip_ctx *c) {
 int cruna_hip_set_state(cruna_hip_ctx *c, const double *q) {
   if (!c || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
   c->qs_fresh = false;
-  HIP_TRY(hipMemcpyAsync(c->qn, q, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (int rc = put_fields(c, c->qn, q, 5, kRestState)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
 
 int cruna_hip_get_state(cruna_hip_ctx *c, double *q) {
   if (!c || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
-  HIP_TRY(hipMemcpyAsync(q, c->qn, 5 * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (int rc = get_fields(c, q, c->qn, 5)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
@@ -1246,7 +1353,8 @@ int cruna_hip_set_qref(cruna_hip_ctx *c, const double *qref) {
   if (!c || !qref) return fail(CRUNA_HIP_EINVAL, "null argument");
   ++c->gen;
   if (!c->qref) HIP_TRY(hipMalloc(&c->qref, 5 * c->N * sizeof(double)));
-  HIP_TRY(hipMemcpy(c->qref, qref, 5 * c->N * sizeof(double), hipMemcpyHostToDevice));
+  if (int rc = put_fields(c, c->qref, qref, 5, kRestState)) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
 
@@ -1256,7 +1364,9 @@ int cruna_hip_set_kvector(cruna_hip_ctx *c, const double *kvector) {
   for (long i = 0; i < 3 * c->N; ++i)
     if (!std::isfinite(kvector[i])) return fail(CRUNA_HIP_EINVAL, "kvector is not finite");
   if (!c->kvec) HIP_TRY(hipMalloc(&c->kvec, 3 * c->N * sizeof(double)));
-  HIP_TRY(hipMemcpy(c->kvec, kvector, 3 * c->N * sizeof(double), hipMemcpyHostToDevice));
+  const double kfill[3] = {1.0, 0.0, 0.0};
+  if (int rc = put_fields(c, c->kvec, kvector, 3, kfill)) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
 
@@ -1264,7 +1374,8 @@ int cruna_hip_set_sponge_shape(cruna_hip_ctx *c, const double *shape) {
   if (!c || !shape) return fail(CRUNA_HIP_EINVAL, "null argument");
   ++c->gen;
   if (!c->shape) HIP_TRY(hipMalloc(&c->shape, c->N * sizeof(double)));
-  HIP_TRY(hipMemcpy(c->shape, shape, c->N * sizeof(double), hipMemcpyHostToDevice));
+  if (int rc = put_fields(c, c->shape, shape, 1, nullptr)) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
 
@@ -1272,7 +1383,16 @@ int cruna_hip_set_metric(cruna_hip_ctx *c, const double *X_metric, const double 
   if (!c || !X_metric || !X_jacobian) return fail(CRUNA_HIP_EINVAL, "null argument");
   ++c->gen;
 
-  const long n1 = c->cfg.n[0], n2 = c->cfg.n[1], n3 = c->cfg.n[2], N = c->N;
+  // the padded layout replicates the nearest real point's metric, which
+  // keeps a metric that is constant along lines constant along them
+  std::vector<double> xm_pad, xj_pad;
+  if (c->padded) {
+    xm_pad = pad_host(c, X_metric, 9, nullptr, true);
+    xj_pad = pad_host(c, X_jacobian, 1, nullptr, true);
+    X_metric = xm_pad.data();
+    X_jacobian = xj_pad.data();
+  }
+  const long n1 = c->np[0], n2 = c->np[1], n3 = c->np[2], N = c->N;
   auto M = [&](int k, int j, long i, long jj, long kk) {
     return X_metric[(size_t)N * (k + 3 * j) + i + n1 * (jj + n2 * kk)];
   };
@@ -1351,6 +1471,14 @@ int cruna_hip_set_penalization(cruna_hip_ctx *c, const double *phi, const double
     return fail(CRUNA_HIP_EINVAL,
                 "set the penalisation before the metric: a penalised deformed grid keeps the full "
                 "metric per point, which a tensor-product metric set before it did not");
+  std::vector<double> ph_pad, ip_pad;
+  if (c->padded) {
+    const double one = 1.0;
+    ph_pad = pad_host(c, phi, 1, &one);
+    ip_pad = pad_host(c, inv_phi, 1, &one);
+    phi = ph_pad.data();
+    inv_phi = ip_pad.data();
+  }
   const size_t N = c->N;
   for (size_t i = 0; i < N; ++i)
     if (!std::isfinite(phi[i]) || !std::isfinite(inv_phi[i]))
@@ -1384,22 +1512,25 @@ int cruna_hip_filter_weighted(cruna_hip_ctx *c, int dir, double *q, int nv, cons
   if (!c || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (dir < 0 || dir > 2) return fail(CRUNA_HIP_EINVAL, "dir must be 0, 1 or 2");
   if (nv < 1 || nv > 5) return fail(CRUNA_HIP_EINVAL, "nv must be 1..5");
+  if (c->np[dir] != c->cfg.n[dir])
+    return fail(CRUNA_HIP_EINVAL, "the weighted filter's core is periodic: its direction needs a multiple of 8 points");
   if (int rc = ensure_tmp(c)) return rc;
   const size_t N = c->N;
   if (!c->wbuf) HIP_TRY(hipMalloc(&c->wbuf, 2 * N * sizeof(double)));
   double *in = c->tmp, *out = c->tmp + 5 * N;
-  HIP_TRY(hipMemcpyAsync(in, q, nv * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const double one = 1.0;
+  if (int rc = put_fields(c, in, q, nv, nullptr)) return rc;
   const double *w = nullptr, *s = nullptr;
   if (weight) {
-    HIP_TRY(hipMemcpyAsync(c->wbuf, weight, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (int rc = put_fields(c, c->wbuf, weight, 1, &one)) return rc;
     w = c->wbuf;
   }
   if (strength) {
-    HIP_TRY(hipMemcpyAsync(c->wbuf + N, strength, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (int rc = put_fields(c, c->wbuf + N, strength, 1, &one)) return rc;
     s = c->wbuf + N;
   }
   if (int rc = filter_weighted_dir(c, in, out, dir, nv, w, s)) return rc;
-  HIP_TRY(hipMemcpyAsync(q, in, nv * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (int rc = get_fields(c, q, in, nv)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
@@ -1410,14 +1541,14 @@ int cruna_hip_diff(cruna_hip_ctx *c, int dir, double *q_x, const double *q, doub
   if (!(h != 0.0)) return fail(CRUNA_HIP_EINVAL, "h must be nonzero");
   if (int rc = ensure_tmp(c)) return rc;
   double *in = c->tmp, *out = c->tmp + c->N;
-  HIP_TRY(hipMemcpyAsync(in, q, c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (int rc = put_fields(c, in, q, 1, nullptr)) return rc;
   {
     ProfScope p(c, KID_DIFF);
     DISPATCH_DIR(c, dir, k_diff, double, (const double *)in, out, c->dop64[dir].op, c->geom[dir],
                  1.0 / h);
   }
   if (int rc = check_launch()) return rc;
-  HIP_TRY(hipMemcpyAsync(q_x, out, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (int rc = get_fields(c, q_x, out, 1)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
@@ -1427,11 +1558,11 @@ int cruna_hip_filter(cruna_hip_ctx *c, int dir, double *q, int nv) {
   if (dir < 0 || dir > 2) return fail(CRUNA_HIP_EINVAL, "dir must be 0, 1 or 2");
   if (nv < 1 || nv > 5) return fail(CRUNA_HIP_EINVAL, "nv must be 1..5");
   if (int rc = ensure_tmp(c)) return rc;
-  HIP_TRY(hipMemcpyAsync(c->tmp, q, nv * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (int rc = put_fields(c, c->tmp, q, nv, nullptr)) return rc;
   int rc = c->cfg.precision == CRUNA_HIP_FP32_MIXED ? filter_dir<float>(c, c->tmp, dir, nv)
                                                     : filter_dir<double>(c, c->tmp, dir, nv);
   if (rc) return rc;
-  HIP_TRY(hipMemcpyAsync(q, c->tmp, nv * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = get_fields(c, q, c->tmp, nv))) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
@@ -1441,7 +1572,7 @@ int cruna_hip_rhs_direct(cruna_hip_ctx *c, double *rhs, const double *q, double 
   if (!c || !rhs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
   c->qs_fresh = false;
   if (int rc = ensure_tmp(c)) return rc;
-  HIP_TRY(hipMemcpyAsync(c->tmp, q, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (int rc = put_fields(c, c->tmp, q, 5, kRestState)) return rc;
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
   int rc = mixed ? run_stage<float>(c, c->tmp, 4) : run_stage<double>(c, c->tmp, 4);
   if (rc) return rc;
@@ -1451,6 +1582,7 @@ int cruna_hip_rhs_direct(cruna_hip_ctx *c, double *rhs, const double *q, double 
 int cruna_hip_box_filter_state(cruna_hip_ctx *c, int f, double *q_box, double *rhs_box) {
   if (!c || !q_box) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (f < 1) return fail(CRUNA_HIP_EINVAL, "box factor must be >= 1");
+  if (c->padded) return fail(CRUNA_HIP_EINVAL, "box filtering needs every grid size a multiple of 8");
   for (int d = 0; d < 3; ++d)
     if (c->cfg.n[d] % f) return fail(CRUNA_HIP_EINVAL, "box factor must divide every grid size");
   if (int rc = ensure_tmp(c)) return rc;
@@ -1482,12 +1614,11 @@ int cruna_hip_box_filter_state(cruna_hip_ctx *c, int f, double *q_box, double *r
 int stage_adjoint_inputs(cruna_hip_ctx *c, const double *qs, const double *q, const double *g) {
   if (int rc = ensure_tmp(c)) return rc;
   if (int rc = ensure_adjoint(c)) return rc;
-  HIP_TRY(hipMemcpyAsync(c->qa, qs, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->tmp + 5 * c->N, q, 5 * c->N * sizeof(double), hipMemcpyHostToDevice,
-                         c->stream));
+  if (int rc = put_fields(c, c->qa, qs, 5, nullptr)) return rc;
+  if (int rc = put_fields(c, c->tmp + 5 * c->N, q, 5, kRestState)) return rc;
   if (g) {
     if (!c->gobj) HIP_TRY(hipMalloc(&c->gobj, 5 * c->N * sizeof(double)));
-    HIP_TRY(hipMemcpyAsync(c->gobj, g, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (int rc = put_fields(c, c->gobj, g, 5, nullptr)) return rc;
   }
   return CRUNA_HIP_OK;
 }
@@ -1510,11 +1641,11 @@ int cruna_hip_time_stepper_direct(cruna_hip_ctx *c, double *q1, const double *q0
   (void)t;
   if (!c || !q1 || !q0) return fail(CRUNA_HIP_EINVAL, "null argument");
   c->qs_fresh = false;
-  HIP_TRY(hipMemcpyAsync(c->qn, q0, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (int rc = put_fields(c, c->qn, q0, 5, kRestState)) return rc;
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
   int rc = mixed ? rk4_step<float>(c) : rk4_step<double>(c);
   if (rc) return rc;
-  HIP_TRY(hipMemcpyAsync(q1, c->qn, 5 * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = get_fields(c, q1, c->qn, 5))) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
@@ -1530,7 +1661,7 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *
   const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
   int rc = mixed ? ark4_step<float>(c, gd) : ark4_step<double>(c, gd);
   if (rc) return rc;
-  HIP_TRY(hipMemcpyAsync(qs1, c->qa, 5 * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (int rc = get_fields(c, qs1, c->qa, 5)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
@@ -1538,7 +1669,7 @@ int cruna_hip_time_stepper_adjoint(cruna_hip_ctx *c, double *qs1, const double *
 int cruna_hip_set_adjoint_state(cruna_hip_ctx *c, const double *qs) {
   if (!c || !qs) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (int rc = ensure_adjoint(c)) return rc;
-  HIP_TRY(hipMemcpyAsync(c->qa, qs, 5 * c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (int rc = put_fields(c, c->qa, qs, 5, nullptr)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
@@ -1546,7 +1677,7 @@ int cruna_hip_set_adjoint_state(cruna_hip_ctx *c, const double *qs) {
 int cruna_hip_get_adjoint_state(cruna_hip_ctx *c, double *qs) {
   if (!c || !qs) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (!c->qa) return fail(CRUNA_HIP_EINVAL, "no adjoint state (cruna_hip_set_adjoint_state)");
-  HIP_TRY(hipMemcpyAsync(qs, c->qa, 5 * c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (int rc = get_fields(c, qs, c->qa, 5)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
@@ -1573,8 +1704,7 @@ int cruna_hip_trajectory_alloc(cruna_hip_ctx *c, int nslots) {
 int cruna_hip_trajectory_set(cruna_hip_ctx *c, int nt, const double *q) {
   if (!c || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (nt < 1 || nt > c->traj_slots) return fail(CRUNA_HIP_EINVAL, "nt outside the trajectory");
-  HIP_TRY(hipMemcpyAsync(traj_slot(c, nt), q, 5 * c->N * sizeof(double), hipMemcpyHostToDevice,
-                         c->stream));
+  if (int rc = put_fields(c, traj_slot(c, nt), q, 5, kRestState)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
@@ -1582,8 +1712,7 @@ int cruna_hip_trajectory_set(cruna_hip_ctx *c, int nt, const double *q) {
 int cruna_hip_trajectory_get(cruna_hip_ctx *c, int nt, double *q) {
   if (!c || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
   if (nt < 1 || nt > c->traj_slots) return fail(CRUNA_HIP_EINVAL, "nt outside the trajectory");
-  HIP_TRY(hipMemcpyAsync(q, traj_slot(c, nt), 5 * c->N * sizeof(double), hipMemcpyDeviceToHost,
-                         c->stream));
+  if (int rc = get_fields(c, q, traj_slot(c, nt), 5)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
@@ -1764,8 +1893,8 @@ int cruna_hip_autotune(cruna_hip_ctx *c, int steps, int tile_lines[3]) {
   };
   // coordinate search: one direction at a time over the valid line counts
   for (int d = 0; d < 3 && !rc; ++d) {
-    const long lines = d == 0 ? (long)c->cfg.n[1] * c->cfg.n[2] : c->cfg.n[0];
-    const int NB = c->cfg.n[d] / PPT;
+    const long lines = d == 0 ? (long)c->np[1] * c->np[2] : c->np[0];
+    const int NB = c->np[d] / PPT;
     int arg = best[d];
     float tbest = 0.f;
     bool first = true;

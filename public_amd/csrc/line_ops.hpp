@@ -25,6 +25,16 @@
 // In exact arithmetic this is the same solution as the LU solve; the reduced
 // system is truncated once its couplings fall below 1e-18 (periodic) or
 // solved exactly (open).  The tables are built here in FP64.
+// Open lines of any length n >= 9 are carried as lines of n_pad (a multiple
+// of 8) points: pf padding points, the n real points, then padding to the
+// end.  The padding rows are decoupled: B is the identity there and A is
+// zero (derivative) or the identity (filter).  The open closures never reach
+// past the line's ends and B's first and last real rows have no coupling
+// beyond them, so the real values of the solution are exactly those of the
+// n-point line; the padding points keep a derivative of 0 and an unfiltered
+// value.  pf (0..4) puts the last real point at position 3..7 of its block,
+// so that the one-sided closure rows (which reach up to 6 points inward) find
+// their points within one block and its 3-point halo.
 #pragma once
 
 #include <cmath>
@@ -41,8 +51,10 @@ constexpr int MAXK = 12;         // max reduced-system stages
 
 enum OpKind { OP_DIFF = 0, OP_FILTER = 1, OP_DIFF_O4E = 2, OP_DIFF_O4P = 3, OP_DIFF_O2E = 4 };
 
-// M, L, R are stored for three block types: 0 = first block, 1 = interior
-// blocks, 2 = last block (all equal on a periodic line).  W is stored per
+// M, L, R are stored for four block types: 0 = first block, 1 = interior
+// blocks, 2 = last block, 3 = the block before the last when rows of the
+// end closure reach into it (a padded open line); all equal on a periodic
+// line.  W is stored per
 // block for open lines ([K][NB][12]) and once per stage for periodic lines
 // ([K][1][12]): W0, Wm, Wp as row-major 2x2 matrices.
 // I holds the interior-block form of M (an 8x14 product) as the explicit
@@ -52,12 +64,16 @@ enum OpKind { OP_DIFF = 0, OP_FILTER = 1, OP_DIFF_O4E = 2, OP_DIFF_O4P = 3, OP_D
 //   I[12..19] = 1/d_0..1/d_7, I[20] = u:  x_i = (y_i - u x_{i+1}) / d_i
 constexpr int NI = 21;
 
+constexpr int NTYPES = 4;
+
 struct HostLineOp {
   int n = 0, NB = 0, K = 0, per = 1, nw = 1;
+  int n_real = 0, pf = 0;  // real points and the padding in front of them
+  int spec2 = 0;          // block NB-2 has its own dense operator (type 3)
   std::vector<double> I;  // [NI] interior blocks
-  std::vector<double> M;  // [3][PPT][FE]
-  std::vector<double> L;  // [3][PPT]
-  std::vector<double> R;  // [3][PPT]
+  std::vector<double> M;  // [NTYPES][PPT][FE]
+  std::vector<double> L;  // [NTYPES][PPT]
+  std::vector<double> R;  // [NTYPES][PPT]
   std::vector<double> W;  // [K][nw][12]
 };
 
@@ -277,27 +293,75 @@ inline void implicit_rows(OpKind kind, bool per, int n, double alpha, std::vecto
 
 }  // namespace detail
 
-// Build the block-partitioned operator for a line of n points.
-inline HostLineOp build_line_op(OpKind kind, bool per, int n, double alpha, double tol = 1e-18) {
+// Padding in front of an open line of n points (periodic lines: none), and
+// the padded length.
+inline int front_padding(bool per, int n) {
+  if (per || n % PPT == 0) return 0;
+  for (int pf = 0; pf <= 4; ++pf)
+    if ((pf + n - 1) % PPT >= 3) return pf;
+  return 0;  // unreachable: 5 consecutive residues always meet {3..7}
+}
+inline int padded_length(bool per, int n) {
+  return per ? n : (front_padding(per, n) + n + PPT - 1) / PPT * PPT;
+}
+
+// Build the block-partitioned operator for a line of n_real points: periodic
+// lines need n_real % 8 == 0 and n_real >= 16, open lines n_real >= 9 (padded
+// to padded_length(false, n_real) points as described above).
+inline HostLineOp build_line_op(OpKind kind, bool per, int n_real, double alpha, double tol = 1e-18) {
   using namespace detail;
-  if (n % PPT != 0 || n < 2 * PPT)
-    throw std::runtime_error("line length " + std::to_string(n) +
+  if (per && (n_real % PPT != 0 || n_real < 2 * PPT))
+    throw std::runtime_error("periodic line length " + std::to_string(n_real) +
                              " must be a multiple of 8 and at least 16");
+  if (!per && n_real < PPT + 1)
+    throw std::runtime_error("open line length " + std::to_string(n_real) + " must be at least 9");
+  const int n = padded_length(per, n_real);
+  const int pf = front_padding(per, n_real);
   HostLineOp op;
   op.n = n;
+  op.n_real = n_real;
+  op.pf = pf;
   op.NB = n / PPT;
   op.per = per ? 1 : 0;
   const int NB = op.NB;
   op.nw = per ? 1 : NB;
-  std::vector<std::vector<std::pair<int, double>>> A;
-  explicit_rows(kind, per, n, alpha, A);
-  std::vector<double> lo, di, up;
-  implicit_rows(kind, per, n, alpha, lo, di, up);
+  std::vector<std::vector<std::pair<int, double>>> Ar, A(n);
+  explicit_rows(kind, per, n_real, alpha, Ar);
+  std::vector<double> lr, dr, ur, lo(n, 0.0), di(n, 1.0), up(n, 0.0);
+  implicit_rows(kind, per, n_real, alpha, lr, dr, ur);
+  // real rows shifted by pf; decoupled padding rows: B = I, A = 0 (derivative)
+  // or I (filter)
+  for (int i = 0; i < n_real; ++i) {
+    for (auto &jc : Ar[i]) A[i + pf].push_back({jc.first + pf, jc.second});
+    lo[i + pf] = lr[i];
+    di[i + pf] = dr[i];
+    up[i + pf] = ur[i];
+  }
+  for (int i = 0; i < n; ++i)
+    if ((i < pf || i >= pf + n_real) && kind == OP_FILTER) A[i].push_back({i, 1.0});
+  // rows that differ from the interior form (the open closures, the padding)
+  std::vector<bool> special(n, false);
+  if (!per) {
+    const int mid = pf + n_real / 2;
+    for (int i = 0; i < n; ++i) {
+      bool sp = i < pf || i >= pf + n_real || lo[i] != lo[mid] || di[i] != di[mid] || up[i] != up[mid] ||
+                A[i].size() != A[mid].size();
+      for (size_t k = 0; !sp && k < A[i].size(); ++k)
+        sp = A[i][k].first - i != A[mid][k].first - mid || A[i][k].second != A[mid][k].second;
+      special[i] = sp;
+    }
+  }
+  for (int b = 1; b + 2 < NB; ++b)
+    for (int r = 0; r < PPT; ++r)
+      if (special[b * PPT + r]) throw std::runtime_error("closure rows beyond the two end blocks");
+  op.spec2 = 0;
+  if (!per && NB >= 3)
+    for (int r = 0; r < PPT; ++r) op.spec2 |= special[(NB - 2) * PPT + r] ? 1 : 0;
 
   op.I.assign(NI, 0.0);
-  op.M.assign(3 * PPT * FE, 0.0);
-  op.L.assign(3 * PPT, 0.0);
-  op.R.assign(3 * PPT, 0.0);
+  op.M.assign(NTYPES * PPT * FE, 0.0);
+  op.L.assign(NTYPES * PPT, 0.0);
+  op.R.assign(NTYPES * PPT, 0.0);
   std::vector<double> Mint(PPT * FE), Lint(PPT), Rint(PPT);
   std::vector<M2> Pm(NB), Qm(NB);
   for (int b = 0; b < NB; ++b) {
@@ -328,8 +392,9 @@ inline HostLineOp build_line_op(OpKind kind, bool per, int n, double alpha, doub
         if (jj < 0 || jj >= FE) throw std::runtime_error("explicit stencil wider than the halo");
         Aloc[r][jj] += jc.second;
       }
-    // block type: first, interior or last (a periodic line has one type)
-    const int ty = per ? -1 : (b == 0 ? 0 : (b == NB - 1 ? 2 : 1));
+    // block type: first, interior, last or the special block before the
+    // last (a periodic line has one type)
+    const int ty = per ? -1 : (b == 0 ? 0 : (b == NB - 1 ? 2 : (b == NB - 2 && op.spec2 ? 3 : 1)));
     double Mb[PPT * FE];
     for (int r = 0; r < PPT; ++r)
       for (int j = 0; j < FE; ++j) {
@@ -337,7 +402,7 @@ inline HostLineOp build_line_op(OpKind kind, bool per, int n, double alpha, doub
         for (int k = 0; k < PPT; ++k) s += T[r * PPT + k] * Aloc[k][j];
         Mb[r * FE + j] = s;
       }
-    for (int t = 0; t < 3; ++t) {
+    for (int t = 0; t < NTYPES; ++t) {
       const bool take = per ? (b == 0) : (t == ty && (t != 1 || b == 1));
       if (!take) continue;
       for (int r = 0; r < PPT; ++r) {

@@ -22,11 +22,12 @@ constexpr int NT_FILTER32 = 1024;
 template <typename T>
 struct LineOp {
   const T *I;  // [NI] interior blocks: stencil + local Thomas factors
-  const T *M;  // [3][PPT][FE]  dense local operators: first, interior, last
-  const T *L;  // [3][PPT]
-  const T *R;  // [3][PPT]
+  const T *M;  // [NTYPES][PPT][FE]  dense local operators: first, interior, last, before last
+  const T *L;  // [NTYPES][PPT]
+  const T *R;  // [NTYPES][PPT]
   const T *W;  // [K][nw][12]
   int n, NB, K, per;
+  int spec2;   // open line: block NB-2 uses its own dense operator (type 3)
 };
 
 // Sweep geometry: which lines a work-group owns and where they live in HBM.
@@ -189,8 +190,8 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T, NTM> 
 #pragma unroll
     for (int i = PPT - 2; i >= 0; --i) y[i] = (y[i] - up * y[i + 1]) * I[12 + i];
   }
-  if (!PER && (b == 0 || b == NB - 1)) {
-    ty = b == 0 ? 0 : 2;
+  if (!PER && (b == 0 || b == NB - 1 || (b == NB - 2 && op.spec2))) {
+    ty = b == 0 ? 0 : (b == NB - 1 ? 2 : 3);
     const T *Mb = op.M + ty * PPT * FE;
 #pragma unroll 1
     for (int r = 0; r < PPT; ++r) {

@@ -1,6 +1,7 @@
 // CPU emulation of the device line engine (public_amd/csrc/line_solve.hpp)
 // driven by the host tables of public_amd/csrc/line_ops.hpp, compared with the
 // oracle's LU-based cores (oracle/cruna_oracle.c) on random lines.
+// Open lines of any length run as padded lines (decoupled rows beyond n).
 // Prints one line per case: kind scheme n K max_rel_err.
 #include <cmath>
 #include <cstdio>
@@ -12,8 +13,18 @@
 
 using namespace cruna;
 
-static std::vector<double> emulate(const HostLineOp &op, const std::vector<double> &f) {
+static int block_type(const HostLineOp &op, int b) {
+  if (op.per) return 1;
+  const int NB = op.NB;
+  return b == 0 ? 0 : (b == NB - 1 ? 2 : (b == NB - 2 && op.spec2 ? 3 : 1));
+}
+
+// f holds the real points; the padding points of an open line are zero
+// (op.pf of them in front); the result is in the padded layout
+static std::vector<double> emulate(const HostLineOp &op, const std::vector<double> &f_real) {
   const int n = op.n, NB = op.NB;
+  std::vector<double> f(n, 0.0);
+  for (size_t i = 0; i < f_real.size(); ++i) f[i + op.pf] = f_real[i];
   std::vector<double> y(n), x(n);
   for (int b = 0; b < NB; ++b) {
     double fe[FE];
@@ -22,7 +33,7 @@ static std::vector<double> emulate(const HostLineOp &op, const std::vector<doubl
       if (op.per) p = ((p % n) + n) % n;
       fe[j] = (p >= 0 && p < n) ? f[p] : 0.0;
     }
-    const bool bnd = !op.per && (b == 0 || b == NB - 1);
+    const bool bnd = block_type(op, b) != 1;
     if (!bnd) {
       const double *I = op.I.data();
       double r[PPT];
@@ -37,7 +48,7 @@ static std::vector<double> emulate(const HostLineOp &op, const std::vector<doubl
       for (int i = PPT - 2; i >= 0; --i) r[i] = (r[i] - I[20] * r[i + 1]) * I[12 + i];
       for (int i = 0; i < PPT; ++i) y[b * PPT + i] = r[i];
     } else {
-      const int ty = b == 0 ? 0 : 2;
+      const int ty = block_type(op, b);
       for (int r = 0; r < PPT; ++r) {
         double acc = 0.0;
         for (int j = 0; j < FE; ++j) acc += op.M[(ty * PPT + r) * FE + j] * fe[j];
@@ -67,7 +78,7 @@ static std::vector<double> emulate(const HostLineOp &op, const std::vector<doubl
   for (int b = 0; b < NB; ++b) {
     const bool hl = op.per || b > 0, hr = op.per || b < NB - 1;
     const double xl = hl ? z1[(b - 1 + NB) % NB] : 0, xf = hr ? z0[(b + 1) % NB] : 0;
-    const int ty = op.per ? 1 : (b == 0 ? 0 : (b == NB - 1 ? 2 : 1));
+    const int ty = block_type(op, b);
     for (int r = 0; r < PPT; ++r)
       x[b * PPT + r] = y[b * PPT + r] + op.L[ty * PPT + r] * xl + op.R[ty * PPT + r] * xf;
   }
@@ -78,11 +89,18 @@ int main() {
   std::mt19937_64 rng(7);
   std::normal_distribution<double> nd;
   const int ns[] = {16, 24, 48, 64, 96, 128, 256, 512};
+  // open lines of other lengths (padded): 2^k + 1 and the reference's
+  // regression geometry 60 x 65 x 70
+  const int ns_open[] = {9, 13, 17, 20, 33, 60, 65, 70, 129, 257};
   const OpKind kinds[] = {OP_DIFF, OP_FILTER, OP_DIFF_O4E, OP_DIFF_O4P, OP_DIFF_O2E};
   const char *names[] = {"diff", "filter", "diffo4", "diffpade", "diffo2"};
+  std::vector<std::pair<int, int>> cases;  // (periodic, n)
+  for (int per = 1; per >= 0; --per)
+    for (int n : ns) cases.push_back({per, n});
+  for (int n : ns_open) cases.push_back({0, n});
   for (int kind = 0; kind < 5; ++kind)
-    for (int per = 1; per >= 0; --per)
-      for (int n : ns) {
+    for (auto pc : cases) {
+        const int per = pc.first, n = pc.second;
         if (kinds[kind] == OP_DIFF_O2E && per) continue;  // open lines only
         const double alpha = 0.35;
         HostLineOp op = build_line_op(kinds[kind], per, n, alpha);
@@ -105,9 +123,12 @@ int main() {
           std::vector<double> x = emulate(op, f);
           double num = 0, den = 0;
           for (int i = 0; i < n; ++i) {
-            num = std::fmax(num, std::fabs(x[i] - ref[i]));
+            num = std::fmax(num, std::fabs(x[i + op.pf] - ref[i]));
             den = std::fmax(den, std::fabs(ref[i]));
           }
+          // the padding rows stay decoupled: zero derivative, unfiltered zero
+          for (int i = 0; i < op.n; ++i)
+            if (i < op.pf || i >= op.pf + n) num = std::fmax(num, std::fabs(x[i]));
           worst = std::fmax(worst, num / den);
         }
         std::printf("%s %s %d %d %.3e\n", names[kind], per ? "periodic" : "open", n,

@@ -19,6 +19,6 @@ def test_block_engine_matches_lu_cores(tmp_path):
     subprocess.run(["g++", "-O2", "-std=c++17", src, str(obj), "-o", str(exe), "-lm"], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
     rows = [line.split() for line in out.strip().splitlines()]
-    assert len(rows) == 72
+    assert len(rows) == 72 + 5 * 10
     for kind, scheme, n, K, err in rows:
         assert float(err) < 1e-14, (kind, scheme, n, K, err)
