@@ -593,10 +593,21 @@ double orc_lambda(const orc_cfg *c, double mu) {
 static int bc_type(const orc_cfg *c, int r) { return (int)lround(c->bc[r][0]); }
 
 /* Visit the points of the box of BC row r (1-based inclusive indices). */
+/* Box bounds clamped to the grid as modify_boundary_conditions_array does on
+ * one rank (src/boundaries/boundary_conditions.f90:586-596: max(1, start),
+ * min(n, end); a box outside the grid becomes empty), 0-based. */
+static int box_lo(const orc_cfg *c, int r, int col) {
+  const long v = lround(c->bc[r][col]);
+  return (int)(v < 1 ? 0 : v - 1);
+}
+static int box_hi(const orc_cfg *c, int r, int col, int d) {
+  const long v = lround(c->bc[r][col]);
+  return (int)(v > c->n[d] ? c->n[d] - 1 : v - 1);
+}
 #define FOR_BOX(c, r, idx)                                                                      \
-  for (int kk = (int)lround((c)->bc[r][5]) - 1; kk <= (int)lround((c)->bc[r][6]) - 1; ++kk)    \
-    for (int jj = (int)lround((c)->bc[r][3]) - 1; jj <= (int)lround((c)->bc[r][4]) - 1; ++jj)  \
-      for (int ii = (int)lround((c)->bc[r][1]) - 1; ii <= (int)lround((c)->bc[r][2]) - 1; ++ii) \
+  for (int kk = box_lo(c, r, 5); kk <= box_hi(c, r, 6, 2); ++kk)                                \
+    for (int jj = box_lo(c, r, 3); jj <= box_hi(c, r, 4, 1); ++jj)                              \
+      for (int ii = box_lo(c, r, 1); ii <= box_hi(c, r, 2, 0); ++ii)                            \
         for (size_t idx = (size_t)ii + (size_t)(c)->n[0] * (jj + (size_t)(c)->n[1] * kk), _o = 1; _o; _o = 0)
 
 /* src/boundaries/nonreflecting.f90:46-137 on one point (stride_v = distance

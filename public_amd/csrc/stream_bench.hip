@@ -5,6 +5,8 @@
 //   scale  b = s a          (1 read, 1 write)
 //   triad  c = a + s b      (2 reads, 1 write)
 //   read   sum of a         (1 read; one partial per work-group)
+// plus copy and triad with four nontemporal 16-byte accesses per thread and
+// iteration, and the copy with only two work-groups per CU.
 // 16-byte accesses, grid-stride over enough work-groups to fill every CU,
 // timed with HIP events over `reps` launches after one warm-up.  Prints one
 // JSON line: bytes moved per launch / average time for each kernel.
@@ -53,6 +55,41 @@ __global__ __launch_bounds__(TPB) void k_read(const double2 *__restrict__ a, dou
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = w[0];
+}
+
+// four independent 16-byte accesses per thread and iteration, nontemporal
+// (streaming) loads and stores: more bytes in flight per wave
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(TPB) void k_copy4(const dv2 *__restrict__ a, dv2 *__restrict__ b, long n2) {
+  const long stride = (long)gridDim.x * TPB;
+  long i = (long)blockIdx.x * TPB + threadIdx.x;
+  for (; i + 3 * stride < n2; i += 4 * stride) {
+    dv2 x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x[u] = __builtin_nontemporal_load(a + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) __builtin_nontemporal_store(x[u], b + i + u * stride);
+  }
+  for (; i < n2; i += stride) b[i] = a[i];
+}
+
+__global__ __launch_bounds__(TPB) void k_triad4(const dv2 *__restrict__ a, const dv2 *__restrict__ b,
+                                                dv2 *__restrict__ c, double s, long n2) {
+  const long stride = (long)gridDim.x * TPB;
+  long i = (long)blockIdx.x * TPB + threadIdx.x;
+  for (; i + 3 * stride < n2; i += 4 * stride) {
+    dv2 x[4], y[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      x[u] = __builtin_nontemporal_load(a + i + u * stride);
+      y[u] = __builtin_nontemporal_load(b + i + u * stride);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      __builtin_nontemporal_store(x[u] + s * y[u], c + i + u * stride);
+  }
+  for (; i < n2; i += stride) c[i] = a[i] + s * b[i];
 }
 
 __global__ void k_init(double *a, long n, double v) {
@@ -119,6 +156,15 @@ int main(int argc, char **argv) {
   time_it("triad", 3 * bytes,
           [&] { hipLaunchKernelGGL(k_triad, dim3(blocks), dim3(TPB), 0, 0, A, B, (double2 *)c, 0.5, n2); });
   time_it("read", bytes, [&] { hipLaunchKernelGGL(k_read, dim3(blocks), dim3(TPB), 0, 0, A, part, n2); });
+  time_it("copy4_nt", 2 * bytes,
+          [&] { hipLaunchKernelGGL(k_copy4, dim3(blocks), dim3(TPB), 0, 0, (const dv2 *)a, (dv2 *)c, n2); });
+  time_it("triad4_nt", 3 * bytes,
+          [&] { hipLaunchKernelGGL(k_triad4, dim3(blocks), dim3(TPB), 0, 0, (const dv2 *)a, (const dv2 *)b, (dv2 *)c, 0.5,
+                                   n2); });
+  // the copy with fewer work-groups (2 per CU): the sweeps' occupancy
+  const int few = 2 * cus;
+  time_it("copy_2wg_per_cu", 2 * bytes,
+          [&] { hipLaunchKernelGGL(k_copy, dim3(few), dim3(TPB), 0, 0, A, (double2 *)c, n2); });
   std::string out = "{\"array_bytes\": " + std::to_string((long)bytes) + ", \"reps\": " + std::to_string(reps) +
                     ", \"blocks\": " + std::to_string(blocks) + ", \"cus\": " + std::to_string(cus);
   for (auto &r : res) {
