@@ -17,8 +17,9 @@ its own rate.
 `roofline` describes the dominant kernel (largest share of the timed step):
 its algorithmic bytes per launch (compulsory reads and writes of the sweep)
 over its HIP-event-timed average duration, against 8 TB/s.  `cpu_baseline`
-times the C oracle on a bounded sample of the same workload (one RHS
-evaluation and the three filter passes of the 256^3 state, one host core).
+times one full step of a 128^3 box of the same workload on the host with the
+reference's own compiled o6 cores (all host threads, and one thread), and
+the restated oracle beside it.
 `closure_64_vs_256` runs the committed learned closure at 64^3 against the
 plain 256^3 FP64 solve over the same physical time from a held-out seed:
 wall-clock speedup, velocity-spectrum error against the box-filtered fine
@@ -430,28 +431,83 @@ def rel_errors(q, ref, gamma):
     return out
 
 
-def cpu_baseline(q0, n, h, dt, kw):
-    """Time the C oracle (one host core) on one RHS evaluation and the three
-    filter passes of the same state; an RK4 step is four RHS evaluations plus
-    the filters (the stage combinations are < 1% and are left out)."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_step_child(n, mode, threads, seed, mach):
+    """Child process: one full step (RK4 + filters i1, i2, i3) of the periodic
+    box on the host, timed end to end; prints one JSON line."""
     from oracle import oracle as O
-    cfg = O.Cfg((n,) * 3, (h,) * 3, dt=dt, filter_freq=(1, 1, 1), **kw)
+    from public_amd import ic
+    h = ic.periodic_dx(n)
+    q = ic.periodic_box_state(n, seed, mach)
+    kw = dict(mu0=1.0 / 1600.0, T0=1.0 / (1.4 * mach * mach), visc_law=O.POWER, visc_exp=0.76,
+              dt=ic.cfl_dt(q, h), filter_freq=(1, 1, 1))
+    O.set_threads(threads)
+    if mode == "reference" and not O.use_reference_cores(threads=threads if threads > 1 else 0):
+        print(json.dumps({"error": "oracle/_ref is not built"}))
+        return
+    cfg = O.Cfg((n,) * 3, (h,) * 3, **kw)
     t0 = time.perf_counter()
-    O.rhs_direct(cfg, q0)
-    t1 = time.perf_counter()
-    q = q0
-    for d in range(3):
-        q = O.filter4d(cfg, d, q)
-    t2 = time.perf_counter()
-    step_s = 4.0 * (t1 - t0) + (t2 - t1)
+    O.run_direct(cfg, q, 1)
+    el = time.perf_counter() - t0
+    print(json.dumps({"seconds": el, "mpts": n ** 3 / el / 1e6}))
+
+
+def cpu_baseline(n_sample=128, seed=0, mach=0.3):
+    """The CPU path on the host cores, one full step (RK4 with its four RHS
+    evaluations, then the three filter passes) of a periodic box of
+    n_sample^3 points of the same workload, timed end to end:
+      * reference cores, 1 thread: the reference's own compiled o6 derivative
+        and filter cores (oracle/_ref, the reference's Fortran built in
+        place) for every line solve, with the oracle's point-wise work;
+      * reference cores, all threads: the reference's OpenMP o6 derivative
+        core (diff_pen_core_o6_i_c_p_omp, tnr_omp = all host threads; the
+        reference's filter has no threaded core) and the oracle's threaded
+        point-wise loops;
+      * port, all threads: the oracle's restated cores, threaded per line.
+    Each runs in a child process (OMP_WAIT_POLICY=passive: the two OpenMP
+    runtimes of a mixed step do not spin against each other)."""
+    import subprocess
+    threads = len(os.sched_getaffinity(0))
+    env = dict(os.environ, OMP_WAIT_POLICY="passive")
+
+    def run(mode, th):
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-step-child", mode, str(th),
+                            str(n_sample), str(seed), str(mach)], capture_output=True, text=True, env=env,
+                           timeout=600)
+        try:
+            return json.loads(r.stdout.strip().splitlines()[-1])
+        except (ValueError, IndexError):
+            return {"error": (r.stderr or r.stdout)[-300:]}
+
+    one = run("reference", 1)
+    allr = run("reference", threads)
+    port = run("port", threads)
+    model = _cpu_model()
+    if "mpts" not in allr:
+        return {"value": port.get("mpts"), "unit": "Mpts/s", "cores": threads, "kind": "port",
+                "cpu": model, "sample": f"one full step of the {n_sample}^3 box (oracle/_ref unavailable)"}
     return {
-        "value": round(n ** 3 / step_s / 1e6, 4),
-        "unit": "Mpts/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": (f"C oracle (oracle/cruna_oracle.c, gcc -O2) on the {n}^3 state: one "
-                   f"right_hand_side_direct_3d ({t1 - t0:.1f} s) and filter i1,i2,i3 "
-                   f"({t2 - t1:.1f} s); step time = 4 RHS + filters = {step_s:.1f} s"),
+        "value": round(allr["mpts"], 4), "unit": "Mpts/s", "cores": threads, "kind": "reference",
+        "cpu": model,
+        "sample": (f"one full step (RK4 + filters i1, i2, i3) of the {n_sample}^3 periodic box of the "
+                   f"same workload on {threads} host threads: the reference's compiled OpenMP o6 "
+                   f"derivative core and its serial o6 filter core for every line solve, the oracle's "
+                   f"threaded point-wise work; {allr['seconds']:.2f} s"),
+        "single_core": {"value": round(one["mpts"], 4) if "mpts" in one else None, "unit": "Mpts/s",
+                        "cores": 1, "kind": "reference",
+                        "seconds": round(one["seconds"], 2) if "seconds" in one else None},
+        "port_all_cores": {"value": round(port["mpts"], 4) if "mpts" in port else None, "unit": "Mpts/s",
+                           "cores": threads, "kind": "port",
+                           "seconds": round(port["seconds"], 2) if "seconds" in port else None},
     }
 
 
@@ -582,7 +638,7 @@ def main():
             hip.hipFree(a)
             hip.hipFree(b)
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(q0, n, h, dt, kw)
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
 
     out = {
         "metric": "grid-pt RK4 updates/s (Mpts/s) FP64 vs FP32-mixed; % HBM roofline; err vs FP64",
@@ -615,4 +671,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-step-child":
+        _cpu_step_child(int(sys.argv[4]), sys.argv[2], int(sys.argv[3]), int(sys.argv[5]), float(sys.argv[6]))
+    else:
+        main()

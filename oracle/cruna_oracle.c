@@ -507,6 +507,31 @@ static void scatter_pencils(const orc_cfg *c, int dir, const double *pen, double
     }
 }
 
+/* Optional hooks that run the reference's own compiled cores (the o6
+ * derivative core with the signature of ref_diff in oracle/ref_harness.f90,
+ * the o6 filter core with that of ref_filter) in place of the restated ones:
+ * the CPU baseline times the reference's cores this way.  They are called on
+ * all lines of a pencil at once, from one thread (the reference's cores keep
+ * module state).  NULL (the default): the restatement. */
+typedef void (*orc_diff_hook)(int scheme, double *q, int n, int nlines, double dx);
+typedef void (*orc_filter_hook)(int scheme, double *q, int n, int nlines);
+static orc_diff_hook g_diff_hook = NULL;
+static orc_filter_hook g_filter_hook = NULL;
+
+void orc_set_core_hooks(void *diff, void *filter) {
+  g_diff_hook = (orc_diff_hook)diff;
+  g_filter_hook = (orc_filter_hook)filter;
+}
+
+/* threads of the oracle's own parallel loops (OpenMP) */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
 /* src/diff/diff_i1_mpi_cartesian.f90:83-103 (and i2/i3). */
 void orc_diff3d(const orc_cfg *c, int dir, double *out, const double *in, double h) {
   const size_t N = npts(c);
@@ -514,6 +539,12 @@ void orc_diff3d(const orc_cfg *c, int dir, double *out, const double *in, double
   gather_pencils(c, dir, in, pen);
   const int nd = c->n[dir];
   const long nl = (long)(N / nd);
+  if (g_diff_hook && c->diff_kind[dir] == ORC_DIFF_O6_COMPACT) {
+    g_diff_hook(c->scheme[dir], pen, nd, (int)nl, h);
+    scatter_pencils(c, dir, pen, out);
+    free(pen);
+    return;
+  }
   /* the lines are independent: each thread runs the core on its own share */
 #pragma omp parallel
   {
@@ -548,6 +579,11 @@ void orc_filter4d(const orc_cfg *c, int dir, double *q, int nv) {
     gather_pencils(c, dir, q + v * N, pen);
     const int nd = c->n[dir];
     const long nl = (long)(N / nd);
+    if (g_filter_hook) {
+      g_filter_hook(c->scheme[dir], pen, nd, (int)nl);
+      scatter_pencils(c, dir, pen, q + v * N);
+      continue;
+    }
 #pragma omp parallel
     {
       long l0 = 0, l1 = nl;

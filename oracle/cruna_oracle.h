@@ -157,6 +157,11 @@ void orc_time_stepper_adjoint(const orc_cfg *c, double *qs1, const double *qs0,
 void orc_run_direct(const orc_cfg *c, double *q, int nt0, int nsteps, const double *qref,
                     const double *shape, int heat_unit_spacing);
 /* Q holds nsteps direct states (state after step nt at Q + (nt-1)*N*5). */
+/* run the reference's compiled o6 derivative / filter cores in place of the
+ * restated ones (function pointers with the signatures of ref_diff and
+ * ref_filter; NULL restores the restatement), and the oracle's thread count */
+void orc_set_core_hooks(void *diff, void *filter);
+void orc_set_threads(int n);
 /* cruna_hpc's loop with the residual check of run.cfreq / run.ctol */
 void orc_run_direct_conv(const orc_cfg *c, double *q, int nt0, int nsteps, int cfreq, double ctol,
                          const double *qref, const double *shape, int heat_unit_spacing,

@@ -102,6 +102,10 @@ def lib():
         L.orc_time_stepper_adjoint.argtypes = [P, _dp, _dp, _dp, _vp, _vp]
         L.orc_run_direct.argtypes = [P, _dp, C.c_int, C.c_int, _vp, _vp, C.c_int]
         L.orc_run_adjoint.argtypes = [P, _dp, _dp, C.c_int, _vp]
+        L.orc_set_core_hooks.argtypes = [_vp, _vp]
+        L.orc_set_core_hooks.restype = None
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = None
         L.orc_run_direct_conv.argtypes = [P, _dp, C.c_int, C.c_int, C.c_int, C.c_double, _vp, _vp, C.c_int,
                                           C.POINTER(C.c_double), C.POINTER(C.c_int)]
         _lib = L
@@ -126,8 +130,35 @@ def ref_lib():
         L.ref_characteristic_q.argtypes = [_dp, _dp, C.c_int, _dp]
         L.ref_characteristic_qs.argtypes = [_dp, _dp, C.c_int, _dp]
         L.ref_sponge.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int]
+        L.ref_diff_omp.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double]
+        L.ref_set_threads.argtypes = [C.c_int]
         _ref = L
     return _ref
+
+
+def use_reference_cores(threads=0, alpha=0.35):
+    """Run the reference's own compiled o6 cores (oracle/_ref) inside the
+    oracle's derivatives and filters: the serial ones, or with threads > 0
+    the reference's OpenMP derivative core with that many threads (periodic
+    lines; the filter has no threaded core in the reference).  Returns False
+    when oracle/_ref is not built.  use_restated_cores() undoes it."""
+    R = ref_lib()
+    if R is None:
+        return False
+    R.ref_init(alpha, 1.4, 1.0, 1.0, 0.71, 0.0, 1.0, 1)
+    if threads > 0:
+        R.ref_set_threads(int(threads))
+    diff = C.cast(R.ref_diff_omp if threads > 0 else R.ref_diff, C.c_void_p)
+    lib().orc_set_core_hooks(diff, C.cast(R.ref_filter, C.c_void_p))
+    return True
+
+
+def use_restated_cores():
+    lib().orc_set_core_hooks(None, None)
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
 
 
 def _ptr(a):

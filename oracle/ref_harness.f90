@@ -16,6 +16,7 @@ module ref_harness
   use diff_pen_core_o2_e_c_o, only: d2o_i1 => diff_i1
   use diff_pen_core_o4_i_c_p_pade, only: dpp_i1 => diff_i1
   use diff_pen_core_o4_i_c_o_pade, only: dpo_i1 => diff_i1
+  use diff_pen_core_o6_i_c_p_omp, only: dpomp_i1 => diff_i1
   use filter_pen_core_o6_i_c_p, only: fp_i1 => filter_i1
   use filter_pen_core_o6_i_c_o, only: fo_i1 => filter_i1
   use ns_rho_rhou_p_trafos, only: give_T, give_mu, give_lambda
@@ -51,6 +52,22 @@ contains
       call do_i1(q, dx)
     end if
   end subroutine ref_diff
+
+  ! the reference's OpenMP variant of the periodic o6 core, with
+  ! parallelism.tnr_omp threads (read once, on its first call)
+  subroutine ref_diff_omp(scheme, q, n, nlines, dx) bind(C, name="ref_diff_omp")
+    integer(c_int), value :: scheme, n, nlines  ! periodic lines only (scheme 0)
+    real(c_double), intent(inout) :: q(n, nlines)
+    real(c_double), value :: dx
+    call dpomp_i1(q, dx)
+  end subroutine ref_diff_omp
+
+  subroutine ref_set_threads(nthreads) bind(C, name="ref_set_threads")
+    integer(c_int), value :: nthreads
+    integer :: tnr
+    tnr = nthreads
+    call set_parameter(tnr, 'parallelism.tnr_omp')
+  end subroutine ref_set_threads
 
   ! the 4th-order explicit cores (no saved state)
   subroutine ref_diff_o4(scheme, q, n, nlines, dx) bind(C, name="ref_diff_o4")

@@ -94,3 +94,14 @@ def sponge(kind, y, shape, q, qref, gamma):
     y = np.asfortranarray(y, dtype=np.float64).copy(order="F")
     L.ref_sponge(kind, y, np.asfortranarray(shape), np.asfortranarray(q), np.asfortranarray(qref), n)
     return y
+
+
+def step_with_reference_cores(q, h, kw, threads):
+    """One step of the oracle's loop with the reference's compiled o6 cores
+    (serial, or its OpenMP derivative core with `threads` threads)."""
+    from oracle import oracle as O
+    n = q.shape[0]
+    if not O.use_reference_cores(threads=threads):
+        raise RuntimeError("oracle/_ref is not built")
+    cfg = O.Cfg((n,) * 3, (h,) * 3, **kw)
+    return O.run_direct(cfg, q, 1)

@@ -324,3 +324,19 @@ def test_run_direct_conv_is_the_loop_with_its_residual():
     # tolerance below it: no stop, the residual of the last check is reported
     got, res, ntc = O.run_direct_conv(cfg, q, 6, 5, 0.5 * r5)
     assert ntc == 0 and res == r5 and np.array_equal(got, q6)
+
+
+# --------------------------------------------------------------------------- reference cores in the loop
+@need_ref
+@pytest.mark.parametrize("threads", [0, 3])
+def test_step_with_reference_cores_equals_restatement(threads):
+    """The CPU baseline's step with the reference's compiled cores (its
+    serial o6 core or its OpenMP variant) is the oracle's step to rounding."""
+    n = 16
+    h = 2 * np.pi / n
+    q = ic.periodic_box_state(n, 2, 0.3)
+    kw = dict(mu0=1 / 1600, T0=1 / (1.4 * 0.09), dt=ic.cfl_dt(q, h), visc_law=O.POWER, filter_freq=(1, 1, 1))
+    ref = refproc.run("step_with_reference_cores", q, h, kw, threads)
+    mine = O.run_direct(O.Cfg((n,) * 3, (h,) * 3, **kw), q, 1)
+    assert np.abs(ref - mine).max() < 1e-13 * np.abs(mine).max()
+    assert np.abs(ref - q).max() > 1e-6
