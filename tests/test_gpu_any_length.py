@@ -87,14 +87,20 @@ def test_reference_pulse_case_60_65_70():
         g.synchronize()
         got = g.get_state()
     assert rel(got, ref) < 1e-12, rel(got, ref)
-    # the pulse is 1e-6 of p0: the fluctuation and the momenta agree to the
-    # rounding of the mean pressure, measured against their largest values
+    # The pulse is 1e-6 of p0, so the fluctuation p - p0 and the momenta it
+    # drives carry the rounding of the 1e5-sized pressure: every RK stage and
+    # each of the five applications of the characteristic rows per step
+    # rebuilds p from terms of that size.  The oracle itself, compiled with
+    # fused multiply-adds instead of separate roundings (gcc -ffp-contract=fast
+    # -mfma), moves by 31 ulp(p0) in the fluctuation (at a z face) and by
+    # 2.4e-8 of the momentum peak after these 10 steps; the bars are twice
+    # that rounding floor.
     dp_got, dp_ref = got[..., 4] - p0, ref[..., 4] - p0
     assert np.abs(dp_ref).max() > 0.01
-    assert np.abs(dp_got - dp_ref).max() < 1e-9 * np.abs(dp_ref).max()
+    assert np.abs(dp_got - dp_ref).max() < 64 * np.spacing(p0), np.abs(dp_got - dp_ref).max() / np.spacing(p0)
     for v in (1, 2, 3):
         m = np.abs(ref[..., v]).max()
-        assert m > 0 and np.abs(got[..., v] - ref[..., v]).max() < 1e-9 * m, v
+        assert m > 0 and np.abs(got[..., v] - ref[..., v]).max() < 5e-8 * m, v
 
 
 def _open_state(shape, seed):
