@@ -107,62 +107,81 @@ __device__ __forceinline__ T lambda_of(const Phys &ph, T mu) {
 
 // A thread's block is PPT consecutive points of its line.  Along x those are
 // contiguous and 64-byte aligned (FP64), so they move as 16-byte vectors;
-// along y/z they are PS elements apart.  Addresses are formed as a uniform
-// field base (SGPRs) plus a 32-bit byte offset, so every field of a sweep
-// shares the same PPT offset registers (the host checks that a field fits
-// in 4 GiB).
-// The field base goes through an opaque SGPR copy so that the compiler keeps
-// the (scalar base + 32-bit vector offset) form instead of folding the bases of
-// several fields into per-point 64-bit vector addresses.
-template <typename Q>
-using gptr = __attribute__((address_space(1))) Q *;  // global memory
-template <typename Q>
-__device__ __forceinline__ gptr<const Q> at(const Q *p, unsigned byte_off) {
-  asm volatile("" : "+s"(p));
-  // the opaque copy is a generic pointer again; naming the global address
-  // space keeps global (not flat) loads, whose completion LDS waits ignore
-  return (gptr<const Q>)(reinterpret_cast<const char *>(p) + byte_off);
-}
-template <typename Q>
-__device__ __forceinline__ gptr<Q> at(Q *p, unsigned byte_off) {
-  asm volatile("" : "+s"(p));
-  return (gptr<Q>)(reinterpret_cast<char *>(p) + byte_off);
+// along y/z they are PS elements apart.  Field accesses are raw buffer loads
+// and stores: the field base in a buffer resource (SGPRs), one 32-bit byte
+// offset per thread (a single VGPR) and, along y/z, the uniform distance
+// m*PS between a block's points as the scalar offset, so a block of PPT
+// points costs one address register instead of PPT 64-bit addresses (the
+// host checks that a field fits in 4 GiB).  Buffer accesses count against
+// vmcnt only, so the line engine's LDS waits never wait for them.
+typedef unsigned int bu2 __attribute__((ext_vector_type(2)));
+typedef unsigned int bu4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p) {
+  // raw buffer (stride 0) over the whole 32-bit offset range; word 3 is the
+  // gfx9 format word of a plain dword buffer
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, -1, 0x00020000);
 }
 
 template <int DIR, typename T, typename Q>
 __device__ __forceinline__ void ld(const Q *__restrict__ p, long base, long PS, T (&v)[PPT]) {
-  const unsigned b0 = unsigned(base), ps = unsigned(PS);
+  const __amdgpu_buffer_rsrc_t r = rsrc(p);
+  const unsigned vo = unsigned(base) * unsigned(sizeof(Q));
   if (DIR == 0) {
-    typedef Q V __attribute__((ext_vector_type(16 / sizeof(Q))));
-    constexpr int W = 16 / sizeof(Q);
 #pragma unroll
-    for (int k = 0; k < PPT / W; ++k) {
-      const V t = *(gptr<const V>)(at(p, (b0 + k * W) * unsigned(sizeof(Q))));
+    for (int k = 0; k < PPT * int(sizeof(Q)) / 16; ++k) {
+      const bu4 t = __builtin_amdgcn_raw_buffer_load_b128(r, vo, k * 16, 0);
+      if constexpr (sizeof(Q) == 8) {
+        v[2 * k] = T(__builtin_bit_cast(double, bu2{t[0], t[1]}));
+        v[2 * k + 1] = T(__builtin_bit_cast(double, bu2{t[2], t[3]}));
+      } else {
+        // each element through a scalar: a bit cast of a vector element
+        // lvalue reads the vector's first element
 #pragma unroll
-      for (int j = 0; j < W; ++j) v[k * W + j] = T(t[j]);
+        for (int j = 0; j < 4; ++j) {
+          const unsigned e = t[j];
+          v[4 * k + j] = T(__builtin_bit_cast(float, e));
+        }
+      }
     }
   } else {
+    const unsigned ps = unsigned(PS) * unsigned(sizeof(Q));
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) v[m] = T(*at(p, (b0 + m * ps) * unsigned(sizeof(Q))));
+    for (int m = 0; m < PPT; ++m) {
+      if constexpr (sizeof(Q) == 8)
+        v[m] = T(__builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, m * ps, 0)));
+      else
+        v[m] = T(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, m * ps, 0)));
+    }
   }
 }
 
 template <int DIR, typename T, typename Q>
 __device__ __forceinline__ void st(Q *__restrict__ p, long base, long PS, const T (&v)[PPT]) {
-  const unsigned b0 = unsigned(base), ps = unsigned(PS);
+  const __amdgpu_buffer_rsrc_t r = rsrc(p);
+  const unsigned vo = unsigned(base) * unsigned(sizeof(Q));
   if (DIR == 0) {
-    typedef Q V __attribute__((ext_vector_type(16 / sizeof(Q))));
-    constexpr int W = 16 / sizeof(Q);
 #pragma unroll
-    for (int k = 0; k < PPT / W; ++k) {
-      V t;
+    for (int k = 0; k < PPT * int(sizeof(Q)) / 16; ++k) {
+      bu4 t;
+      if constexpr (sizeof(Q) == 8) {
+        const bu2 a = __builtin_bit_cast(bu2, double(v[2 * k])), b = __builtin_bit_cast(bu2, double(v[2 * k + 1]));
+        t = bu4{a[0], a[1], b[0], b[1]};
+      } else {
 #pragma unroll
-      for (int j = 0; j < W; ++j) t[j] = Q(v[k * W + j]);
-      *(gptr<V>)(at(p, (b0 + k * W) * unsigned(sizeof(Q)))) = t;
+        for (int j = 0; j < 4; ++j) t[j] = __builtin_bit_cast(unsigned, float(v[4 * k + j]));
+      }
+      __builtin_amdgcn_raw_buffer_store_b128(t, r, vo, k * 16, 0);
     }
   } else {
+    const unsigned ps = unsigned(PS) * unsigned(sizeof(Q));
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) *at(p, (b0 + m * ps) * unsigned(sizeof(Q))) = Q(v[m]);
+    for (int m = 0; m < PPT; ++m) {
+      if constexpr (sizeof(Q) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(bu2, double(v[m])), r, vo, m * ps, 0);
+      else
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, float(v[m])), r, vo, m * ps, 0);
+    }
   }
 }
 
@@ -428,6 +447,11 @@ __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__r
         t3[m] = t33;            // tau_33
       }
     }
+    // the stresses now, so that mu and the gradients are not kept across
+    // the solves below
+    materialize(t1);
+    materialize(t2);
+    materialize(t3);
     if (LAST) {
       ld<DIR>(G + 5 * N, B, PS, sa);
 #pragma unroll
