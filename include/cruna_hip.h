@@ -171,6 +171,17 @@ int cruna_hip_filter(cruna_hip_ctx *ctx, int dir, double *q, int nv);
 /* right_hand_side_direct_3d(rhs,q_in,t) (src/equations/ns_rho_rhou_p.f90:31-253). */
 int cruna_hip_rhs_direct(cruna_hip_ctx *ctx, double *rhs, const double *q, double t);
 
+/* cruna_hip_rhs_direct without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, code between graph captures, harnesses that forbid
+ * hipMalloc).  Same arguments and results as cruna_hip_rhs_direct, which
+ * allocates its staging buffer on first use; this variant stages through a
+ * static device array of 2^24 doubles (128 MiB per GPU, held until the
+ * process ends) and returns CRUNA_HIP_EINVAL when 5 (N_device + N) words do
+ * not fit it (N = n1 n2 n3, N_device the padded point count).  Calls that
+ * overlap on one GPU share that array and race. */
+int cruna_hip_rhs_direct_noalloc(cruna_hip_ctx *ctx, double *rhs, const double *q, double t);
+
 /* Box averages over f x f x f blocks of the device state q^n and (rhs_box
  * not NULL) of its direct RHS, into host arrays (n1/f, n2/f, n3/f, 5): the
  * filtered fine fields from which a closure's training data are made.  No

@@ -93,6 +93,7 @@ def lib():
         L.cruna_hip_diff.argtypes = [vp, ip, dp, dp, C.c_double]
         L.cruna_hip_filter.argtypes = [vp, ip, dp, ip]
         L.cruna_hip_rhs_direct.argtypes = [vp, dp, dp, C.c_double]
+        L.cruna_hip_rhs_direct_noalloc.argtypes = [vp, dp, dp, C.c_double]
         L.cruna_hip_rhs_adjoint.argtypes = [vp, dp, dp, dp, dp, C.c_double]
         L.cruna_hip_time_stepper_direct.argtypes = [vp, dp, dp, C.c_double]
         L.cruna_hip_time_stepper_adjoint.argtypes = [vp, dp, dp, dp, dp, C.c_double]
@@ -131,7 +132,7 @@ def lib():
         L.cruna_hip_set_convergence.argtypes = [vp, ip, C.c_double]
         L.cruna_hip_get_residual.argtypes = [vp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         for f in ("create", "destroy", "set_state", "get_state", "set_qref", "set_sponge_shape", "set_kvector", "diff",
-                  "filter", "rhs_direct", "rhs_adjoint", "time_stepper_direct", "time_stepper_adjoint",
+                  "filter", "rhs_direct", "rhs_direct_noalloc", "rhs_adjoint", "time_stepper_direct", "time_stepper_adjoint",
                   "run_direct", "synchronize", "profile_enable", "profile_read", "profile_count",
                   "set_adjoint_state", "get_adjoint_state", "trajectory_alloc", "trajectory_set",
                   "trajectory_get", "trajectory_record", "run_adjoint", "set_metric", "set_penalization", "filter_weighted", "autotune",
@@ -278,6 +279,14 @@ class Cruna:
         q = _f64(q, self.shape4)
         rhs = np.empty(self.shape4, order="F")
         _check(lib().cruna_hip_rhs_direct(self._h, _dp(rhs), _dp(q), float(t)))
+        return rhs
+
+    def right_hand_side_direct_3d_noalloc(self, q, t=0.0):
+        """The direct RHS through the allocation-free entry point (static
+        device staging; see cruna_hip_rhs_direct_noalloc)."""
+        q = _f64(q, self.shape4)
+        rhs = np.empty(self.shape4, order="F")
+        _check(lib().cruna_hip_rhs_direct_noalloc(self._h, _dp(rhs), _dp(q), float(t)))
         return rhs
 
     def time_stepper_direct(self, q0, t=0.0):

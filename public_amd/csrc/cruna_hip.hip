@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2721,6 +2722,117 @@ int cruna_hip_set_closure(cruna_hip_ctx *c, cruna_hip_cnn *n) {
     if (n) cnn_user(n, +1);
   }
   c->cnn = n;
+  return CRUNA_HIP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free direct RHS (cruna_hip_rhs_direct_noalloc).
+//
+// Its staging memory is one file-scope device array, `scratch`, of
+// NOALLOC_WORDS doubles (128 MiB per GPU, held until the process ends): the
+// input state in the context's device layout (5 N words) and a contiguous
+// copy of the real points for the host transfers (5 N_real words).  Host code
+// never uses the array's name as an address: the device address comes from
+// hipGetSymbolAddress, asked once per device with that device current, and
+// is kept per device.  Every call on a GPU shares that GPU's array, so calls
+// that overlap on one GPU race; each call writes every word it reads.
+constexpr long NOALLOC_WORDS = 1L << 24;
+static __device__ __align__(256) double scratch[NOALLOC_WORDS];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+// device address of `scratch` on device dev (looked up once, with dev current)
+int scratch_on(int dev, double **out) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(CRUNA_HIP_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    int cur = -1;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != dev) HIP_TRY(hipSetDevice(dev));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIP_TRY(hipSetDevice(cur));
+    if (e != hipSuccess)
+      return fail(CRUNA_HIP_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return CRUNA_HIP_OK;
+}
+
+// Real points (n1, n2, n3; contiguous, Fortran order) of nv fields into the
+// padded device layout (np1, np2, np3, front padding pf), padding points set
+// to fill[v].
+__global__ void k_scatter_real(const double *__restrict__ src, double *__restrict__ dst, int nv, int n1, int n2,
+                               int n3, int p1, int p2, int p3, int f1, int f2, int f3, double fill0,
+                               double fill4) {
+  const long N = (long)p1 * p2 * p3, Nr = (long)n1 * n2 * n3;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nv * N; t += (long)gridDim.x * blockDim.x) {
+    const int v = (int)(t / N);
+    const long p = t - v * N;
+    const int i = (int)(p % p1) - f1, j = (int)((p / p1) % p2) - f2, k = (int)(p / ((long)p1 * p2)) - f3;
+    const bool real = i >= 0 && i < n1 && j >= 0 && j < n2 && k >= 0 && k < n3;
+    dst[t] = real ? src[v * Nr + i + (long)n1 * (j + (long)n2 * k)] : (v == 0 ? fill0 : (v == 4 ? fill4 : 0.0));
+  }
+}
+
+// The real points of nv padded device-layout fields (type T) into a
+// contiguous FP64 array.
+template <typename T>
+__global__ void k_gather_real(const T *__restrict__ src, double *__restrict__ dst, int nv, int n1, int n2, int n3,
+                              int p1, int p2, int p3, int f1, int f2, int f3) {
+  const long N = (long)p1 * p2 * p3, Nr = (long)n1 * n2 * n3;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nv * Nr; t += (long)gridDim.x * blockDim.x) {
+    const int v = (int)(t / Nr);
+    const long r = t - v * Nr;
+    const int i = (int)(r % n1), j = (int)((r / n1) % n2), k = (int)(r / ((long)n1 * n2));
+    dst[t] = double(src[v * N + (i + f1) + (long)p1 * ((j + f2) + (long)p2 * (k + f3))]);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cruna_hip_rhs_direct_noalloc(cruna_hip_ctx *c, double *rhs, const double *q, double t) {
+  (void)t;  // the direct RHS of the reference does not depend on t
+  if (!c || !rhs || !q) return fail(CRUNA_HIP_EINVAL, "null argument");
+  const int *n = c->cfg.n;
+  const long Nr = (long)n[0] * n[1] * n[2];
+  if (5 * (c->N + Nr) > NOALLOC_WORDS)
+    return fail(CRUNA_HIP_EINVAL, "cruna_hip_rhs_direct_noalloc: 5 (N_device + N) = " +
+                                      std::to_string(5 * (c->N + Nr)) + " words exceed its static scratch of " +
+                                      std::to_string(NOALLOC_WORDS) + " doubles");
+  double *s = nullptr;
+  if (int rc = scratch_on(c->device, &s)) return rc;
+  double *stage = s, *real = s + 5 * c->N;
+  c->qs_fresh = false;
+  const int *np = c->np, *pf = c->pf;
+  const unsigned blocks = (unsigned)std::min<long>(8192, (5 * c->N + 255) / 256);
+  // the input state: contiguous host copy, then the device layout
+  HIP_TRY(hipMemcpyAsync(real, q, 5 * Nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_scatter_real, dim3(blocks), dim3(256), 0, c->stream, (const double *)real, stage, 5, n[0],
+                     n[1], n[2], np[0], np[1], np[2], pf[0], pf[1], pf[2], kRestState[0], kRestState[4]);
+  if (int rc = check_launch()) return rc;
+  const bool mixed = c->cfg.precision == CRUNA_HIP_FP32_MIXED;
+  int rc = mixed ? run_stage<float>(c, stage, 4) : run_stage<double>(c, stage, 4);
+  if (rc) return rc;
+  // the RHS: real points in FP64, then to the host
+  if (mixed)
+    hipLaunchKernelGGL((k_gather_real<float>), dim3(blocks), dim3(256), 0, c->stream, (const float *)c->rhs, real,
+                       5, n[0], n[1], n[2], np[0], np[1], np[2], pf[0], pf[1], pf[2]);
+  else
+    hipLaunchKernelGGL((k_gather_real<double>), dim3(blocks), dim3(256), 0, c->stream, (const double *)c->rhs,
+                       real, 5, n[0], n[1], n[2], np[0], np[1], np[2], pf[0], pf[1], pf[2]);
+  if (int rc2 = check_launch()) return rc2;
+  HIP_TRY(hipMemcpyAsync(rhs, real, 5 * Nr * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return CRUNA_HIP_OK;
 }
 
