@@ -276,9 +276,12 @@ def run_ensemble(steps, nmembers=32, seed=0, autotune=True, subset_steps=10):
     checkpoints are written by an untimed forward run; the adjoint recomputes
     each subset from its checkpoint inside the timed region).  Timed: all
     members advanced by `steps` steps, (a) one member after the other,
-    eagerly, and (b) together through cruna_hip_ensemble_run's replayed
-    member graphs.  Mpts/s counts N^3 per member and step (an adjoint step's
-    recompute is not counted as work)."""
+    eagerly, (b) together through cruna_hip_ensemble_run's replayed member
+    graphs, (c) through its batched launches with one launch per sweep kind
+    over all members, and (d) through the default batched launches: four
+    bins of members, each bin one launch per sweep kind over its members,
+    the bins side by side.  Mpts/s counts N^3 per member and step (an
+    adjoint step's recompute is not counted as work)."""
     from public_amd import ADJOINT, FORWARD, Cruna, Ensemble, workloads
     specs, states, adj = workloads.ensemble_members(nmembers, seed)
     sizes = [s["n"][0] for s in specs]
@@ -312,27 +315,40 @@ def run_ensemble(steps, nmembers=32, seed=0, autotune=True, subset_steps=10):
     ref = [solo[i].get_adjoint_state() if i in adj else solo[i].get_state() for i in range(nmembers)]
     for g in solo:
         g.close()
-    # (b) the ensemble
-    with Ensemble(specs) as ens:
-        prepare(lambda i: ens[i])
-        tuned = ens.autotune(1) if autotune else None
-        for i in range(nmembers):
-            ens.set_role(i, ADJOINT if i in adj else FORWARD, steps if i in adj else 0)
-        t0 = time.perf_counter()
-        ens.run(steps)
-        t_graph = time.perf_counter() - t0
-        same = all(np.array_equal(ens[i].get_adjoint_state() if i in adj else ens[i].get_state(), ref[i])
-                   for i in range(nmembers))
+    # (b) the ensemble: per-member graphs on member streams, then batched
+    # launches: one per sweep kind over all members (one bin), and the
+    # default, four bins of about equal points side by side
+    res = {}
+    tuned = None
+    for batching in (0, 1, -1):
+        with Ensemble(specs) as ens:
+            ens.set_batching(batching)
+            prepare(lambda i: ens[i])
+            if autotune:
+                tuned = ens.autotune(1)
+            for i in range(nmembers):
+                ens.set_role(i, ADJOINT if i in adj else FORWARD, steps if i in adj else 0)
+            t0 = time.perf_counter()
+            ens.run(steps)
+            secs = time.perf_counter() - t0
+            same = all(np.array_equal(ens[i].get_adjoint_state() if i in adj else ens[i].get_state(), ref[i])
+                       for i in range(nmembers))
+            res[batching] = {"value": round(pts * steps / secs / 1e6, 2), "unit": "Mpts/s",
+                             "seconds": round(secs, 4), "speedup_vs_serial": round(t_serial / secs, 3),
+                             "batched_steps": ens.batched_steps(), "identical_to_serial": bool(same)}
+    res[0]["note"] = "one graph per member and period, replayed on the member streams; includes recording"
+    res[1]["note"] = ("one launch per sweep kind over all members (work list of member tiles), one graph per "
+                      "subset period; includes recording")
+    res[-1]["note"] = ("the members in 4 bins of about equal points, each bin one launch per sweep kind over its "
+                       "members, the bins' graphs replayed side by side on 4 streams; includes recording")
     return {"members": nmembers, "sizes": [int(s) for s in sizes], "forward": nmembers - len(adj),
             "adjoint": len(adj), "steps": steps, "checkpoint_every": subset_steps,
             "points_per_step": pts,
             "serial_eager": {"value": round(pts * steps / t_serial / 1e6, 2), "unit": "Mpts/s",
                              "seconds": round(t_serial, 4)},
-            "graph": {"value": round(pts * steps / t_graph / 1e6, 2), "unit": "Mpts/s",
-                      "seconds": round(t_graph, 4),
-                      "note": "includes the first (eager) period and recording the graphs"},
-            "speedup": round(t_serial / t_graph, 3), "tile_lines": tuned,
-            "identical_to_serial": bool(same)}
+            "member_graphs": res[0], "batched_one_launch": res[1], "batched": res[-1],
+            "speedup": res[-1]["speedup_vs_serial"], "tile_lines": tuned,
+            "identical_to_serial": all(r["identical_to_serial"] for r in res.values())}
 
 
 def run_closure(coarse_steps, seed=2001, mach=0.3):

@@ -294,6 +294,24 @@ int cruna_hip_ensemble_member(cruna_hip_ensemble *ens, int i, cruna_hip_ctx **ct
  * and trajectory set). */
 int cruna_hip_ensemble_set_role(cruna_hip_ensemble *ens, int i, int role, int nt);
 int cruna_hip_ensemble_run(cruna_hip_ensemble *ens, int nsteps);
+/* Batched runs (on by default): when every member is FP64 on a Cartesian grid
+ * without boundary rows, closure or weighted filter, the members share their
+ * schemes and filter switches (frequency 0 or 1), and the adjoint members run
+ * either all over whole trajectories or all over subset checkpoints of one
+ * length from a subset boundary, cruna_hip_ensemble_run advances whole
+ * periods (a step, or a subset) in batched launches: the members split into
+ * `bins` of about equal points, and each bin runs one launch per sweep kind
+ * over its members (a work list of member tiles), its period recorded once
+ * into a HIP graph replayed on the bin's own stream; the bins run side by
+ * side.  The subset members' recomputed trajectories step with the forward
+ * members.  Results are those of the members' own runs, bit for bit.
+ * No counterpart in the reference (one box per process,
+ * src/diff/diff_pen_core_o6_i_c_p.f90:27-32).  set_batching(ens, bins):
+ * -1 automatic (the default: 4 bins, at most one per member), 1 one launch
+ * per sweep kind over all members, 0 the per-member graphs; batched_steps
+ * reports the steps of the last run that went through batched launches. */
+int cruna_hip_ensemble_set_batching(cruna_hip_ensemble *ens, int bins);
+int cruna_hip_ensemble_batched_steps(const cruna_hip_ensemble *ens, int *steps);
 /* cruna_hip_autotune on every member; tile_lines (may be NULL) receives 3
  * values per member. */
 int cruna_hip_ensemble_autotune(cruna_hip_ensemble *ens, int steps, int *tile_lines);

@@ -90,6 +90,8 @@ def lib():
         L.cruna_hip_ensemble_set_role.argtypes = [vp, ip, ip, ip]
         L.cruna_hip_ensemble_run.argtypes = [vp, ip]
         L.cruna_hip_ensemble_autotune.argtypes = [vp, ip, C.POINTER(C.c_int)]
+        L.cruna_hip_ensemble_set_batching.argtypes = [vp, ip]
+        L.cruna_hip_ensemble_batched_steps.argtypes = [vp, C.POINTER(C.c_int)]
         L.cruna_hip_diff.argtypes = [vp, ip, dp, dp, C.c_double]
         L.cruna_hip_filter.argtypes = [vp, ip, dp, ip]
         L.cruna_hip_rhs_direct.argtypes = [vp, dp, dp, C.c_double]
@@ -137,7 +139,7 @@ def lib():
                   "set_adjoint_state", "get_adjoint_state", "trajectory_alloc", "trajectory_set",
                   "trajectory_get", "trajectory_record", "run_adjoint", "set_metric", "set_penalization", "filter_weighted", "autotune",
                   "ensemble_create", "ensemble_destroy", "ensemble_member", "ensemble_set_role",
-                  "ensemble_run", "ensemble_autotune", "cnn_create", "cnn_destroy", "cnn_init_params",
+                  "ensemble_run", "ensemble_autotune", "ensemble_set_batching", "ensemble_batched_steps", "cnn_create", "cnn_destroy", "cnn_init_params",
                   "cnn_set_params", "cnn_get_params", "cnn_set_norm", "cnn_forward", "cnn_gradients",
                   "cnn_add_sample", "cnn_train", "set_closure", "box_filter_state", "cnn_set_gain", "subsets_alloc", "run_direct_subsets",
                   "run_adjoint_subsets", "set_convergence", "get_residual"):
@@ -485,6 +487,20 @@ class Ensemble:
 
     def run(self, nsteps):
         _check(lib().cruna_hip_ensemble_run(self._h, int(nsteps)))
+
+    def set_batching(self, bins=-1):
+        """Batched launches when the members allow them: `bins` concurrent
+        batches (-1 automatic, the default; 1: one launch per sweep kind over
+        all members); 0 or False keeps one replayed graph per member."""
+        if bins is True:
+            bins = -1
+        _check(lib().cruna_hip_ensemble_set_batching(self._h, int(bins)))
+
+    def batched_steps(self):
+        """Steps of the last run() that went through batched launches."""
+        n = C.c_int()
+        _check(lib().cruna_hip_ensemble_batched_steps(self._h, C.byref(n)))
+        return n.value
 
     def autotune(self, steps=2):
         """Tile autotuner on every member; returns the lines per work-group per member."""

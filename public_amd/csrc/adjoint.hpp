@@ -59,18 +59,17 @@ __device__ __forceinline__ void finish(int v, T (&r)[PPT], long B, long PS, long
 // The x sweep (LAST) completes the stage: mode 0..3 RK4 update with step dt
 // (negative for the backward integration), mode 4 writes the RHS.
 template <typename T, typename Q, int DIR, bool PER>
-__global__ __launch_bounds__(NT) void k_asweep(const Q *qs, const double *const *__restrict__ q0p,
-                                               T *__restrict__ rhs, const double *__restrict__ gobj,
-                                               LineOp<T> op, SweepGeom g, double gamma,
-                                               double inv_h, long N, int mode,
-                                               double *__restrict__ qn, double *__restrict__ acc,
-                                               T *qs_out, double dt) {
+__device__ __forceinline__ void asweep_body(const ThreadPos &tp, const Q *qs,
+                                            const double *const *__restrict__ q0p, T *__restrict__ rhs,
+                                            const double *__restrict__ gobj, const LineOp<T> &op,
+                                            const SweepGeom &g, double gamma, double inv_h, long N, int mode,
+                                            double *__restrict__ qn, double *__restrict__ acc, T *qs_out,
+                                            double dt) {
   constexpr bool FIRST = DIR == 2, LAST = DIR == 0;
   const double *__restrict__ q = *q0p;
   __shared__ LineLds<T> lds;
-  const ThreadPos tp = thread_pos<DIR>(g);
   const long B = tp.base, PS = g.PS;
-  const T ih = T(inv_h), gam = T(gamma), gm1 = T(gamma - 1.0);
+  const T ih = T(inv_h), gam = T(gamma), gm1 = T(gamma - 1.0), ggm = T(gamma / (gamma - 1.0));
   T ud[PPT], irho[PPT], S[PPT], R5[PPT], f[PPT], x[PPT], r[PPT];
   {
     T gdp[PPT], qs5[PPT];
@@ -83,9 +82,9 @@ __global__ __launch_bounds__(NT) void k_asweep(const Q *qs, const double *const 
       issue_loads();
 #pragma unroll
       for (int m = 0; m < PPT; ++m) {
-        ud[m] = md[m] / rho[m];
         irho[m] = T(1) / rho[m];
-        gdp[m] = gam * f[m] / (rho[m] * gm1);
+        ud[m] = md[m] * irho[m];
+        gdp[m] = ggm * f[m] * irho[m];
       }
     }
     // P = D p
@@ -107,14 +106,16 @@ __global__ __launch_bounds__(NT) void k_asweep(const Q *qs, const double *const 
 #pragma unroll
   for (int m = 0; m < PPT; ++m) R5[m] = R5[m] + gm1 * (x[m] * ih);
   // B_d = D qs_{d+2} completes r5 (the fences keep each group of loads next
-  // to the solve it feeds)
+  // to the solve it feeds, and the accumulators and forward momenta that a
+  // solve's result meets are loaded after it, so that they take no registers
+  // across it)
   T Ta[PPT];
   {
-    T um[PPT];
     asm volatile("" ::: "memory");
     ld<DIR>(qs + (1 + DIR) * N, B, PS, f);
-    if (!FIRST) ld<DIR>(rhs + 4 * N, B, PS, r);
     line_solve<T, PER>(op, lds, tp, f, x);
+    asm volatile("" ::: "memory");
+    if (!FIRST) ld<DIR>(rhs + 4 * N, B, PS, r);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
       const T Bd = x[m] * ih;
@@ -125,12 +126,14 @@ __global__ __launch_bounds__(NT) void k_asweep(const Q *qs, const double *const 
     }
     finish<DIR, T, LAST>(4, r, B, PS, N, rhs, gobj, mode, qn, acc, qs_out, dt);
   }
-  // A = D qs1
+  // A = D qs1; A + u_d B_d waits in LDS (the thread's own slots) until the
+  // end, which keeps the x sweep within 256 registers without spilling
+  __shared__ T stash[PPT][NT];
   asm volatile("" ::: "memory");
   ld<DIR>(qs, B, PS, f);
   line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) Ta[m] = Ta[m] + x[m] * ih;
+  for (int m = 0; m < PPT; ++m) stash[m][threadIdx.x] = Ta[m] + x[m] * ih;
   // B_m = D qs_{m+2}, m != d
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -138,9 +141,10 @@ __global__ __launch_bounds__(NT) void k_asweep(const Q *qs, const double *const 
     T um[PPT];
     asm volatile("" ::: "memory");
     ld<DIR>(qs + (1 + i) * N, B, PS, f);
+    line_solve<T, PER>(op, lds, tp, f, x);
+    asm volatile("" ::: "memory");
     ld<DIR>(q + (1 + i) * N, B, PS, um);
     if (!FIRST) ld<DIR>(rhs + (1 + i) * N, B, PS, r);
-    line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
       const T Bm = x[m] * ih;
@@ -157,11 +161,21 @@ __global__ __launch_bounds__(NT) void k_asweep(const Q *qs, const double *const 
     if (!FIRST) ld<DIR>(rhs + var * N, B, PS, r);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
-      const T c = v == 0 ? ud[m] * S[m] : -(Ta[m] + S[m]);
+      const T c = v == 0 ? ud[m] * S[m] : -(stash[m][threadIdx.x] + S[m]);
       r[m] = FIRST ? c : r[m] + c;
     }
     finish<DIR, T, LAST>(var, r, B, PS, N, rhs, gobj, mode, qn, acc, qs_out, dt);
   }
+}
+
+template <typename T, typename Q, int DIR, bool PER>
+__global__ __launch_bounds__(NT) void k_asweep(const Q *qs, const double *const *__restrict__ q0p,
+                                               T *__restrict__ rhs, const double *__restrict__ gobj,
+                                               LineOp<T> op, SweepGeom g, double gamma, double inv_h, long N,
+                                               int mode, double *__restrict__ qn, double *__restrict__ acc,
+                                               T *qs_out, double dt) {
+  asweep_body<T, Q, DIR, PER>(thread_pos<DIR>(g), qs, q0p, rhs, gobj, op, g, gamma, inv_h, N, mode, qn, acc, qs_out,
+                              dt);
 }
 
 // characteristic_qs_k1d (src/boundaries/nonreflecting.f90:453-554) on the

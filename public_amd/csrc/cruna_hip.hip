@@ -21,6 +21,7 @@
 #include "adjoint.hpp"
 #include "closure.hpp"
 #include "residual.hpp"
+#include "batch.hpp"
 
 using namespace cruna;
 
@@ -104,6 +105,8 @@ struct cruna_hip_ctx {
   int wg[3] = {0, 0, 0};
   SweepGeom fgeom[3]{};  // filter passes (FP32: up to 32 y/z lines per work-group)
   int fwg[3] = {0, 0, 0};
+  SweepGeom sgeom[3]{};  // Cartesian viscous sweeps (FP32: up to 32 y/z lines per work-group)
+  int swg[3] = {0, 0, 0};
   DevOp<double> dop64[3], fop64[3];
   DevOp<float> dop32[3], fop32[3];
   Phys phys{};
@@ -243,6 +246,15 @@ int build_geom(cruna_hip_ctx *c, int d) {
   if (d != 0 && cfg.precision == CRUNA_HIP_FP32_MIXED) FTL = auto_tl(np, d, NT_FILTER32, 32);
   c->fgeom[d] = make_geom(np, d, FTL);
   c->fwg[d] = NB * FTL;
+  // Cartesian viscous sweeps: in FP32, y/z tiles of up to 32 lines as well,
+  // unless the tile size is given (the autotuner's choice holds for all);
+  // measured on the 256^3 step, the first-derivative y sweep is faster with
+  // the 16-line tiles and the viscous y and z sweeps with 32
+  int STL = TL;
+  if (d != 0 && cfg.precision == CRUNA_HIP_FP32_MIXED && cfg.tile_lines[d] == 0)
+    STL = auto_tl(np, d, NT_SWEEP32, 32);
+  c->sgeom[d] = make_geom(np, d, STL);
+  c->swg[d] = NB * STL;
   return CRUNA_HIP_OK;
 }
 
@@ -551,7 +563,7 @@ int launch_sweep1(cruna_hip_ctx *c, const Q *in) {
 
 template <typename T, int DIR, int FORM>
 void launch_sweep2_form(cruna_hip_ctx *c, int mode, bool fuse_filter) {
-  const dim3 grid(c->geom[DIR].ntiles), block(c->wg[DIR]);
+  const dim3 grid(c->sgeom[DIR].ntiles), block(c->swg[DIR]);
   T *rhs = (T *)c->rhs, *qs = (T *)c->qs;
   const T *G = (const T *)c->G;
   const LineOp<T> &op = dops<T>(c)[DIR].op, &fop = fops<T>(c)[DIR].op;
@@ -562,22 +574,22 @@ void launch_sweep2_form(cruna_hip_ctx *c, int mode, bool fuse_filter) {
     if (fuse_filter && mode == 3) {
       if (per)
         hipLaunchKernelGGL((k_sweep2<T, DIR, true, FORM, true>), grid, block, 0, c->stream, rhs, G, op, fop,
-                           c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
+                           c->sgeom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
                            penal<T>(c));
       else
         hipLaunchKernelGGL((k_sweep2<T, DIR, false, FORM, true>), grid, block, 0, c->stream, rhs, G, op, fop,
-                           c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
+                           c->sgeom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
                            penal<T>(c));
       return;
     }
   }
   if (per) {
     hipLaunchKernelGGL((k_sweep2<T, DIR, true, FORM, false>), grid, block, 0, c->stream, rhs, G, op, fop,
-                       c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
+                       c->sgeom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
                            penal<T>(c));
   } else {
     hipLaunchKernelGGL((k_sweep2<T, DIR, false, FORM, false>), grid, block, 0, c->stream, rhs, G, op, fop,
-                       c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
+                       c->sgeom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
                            penal<T>(c));
   }
 }
@@ -1165,8 +1177,8 @@ int cruna_hip_create(const cruna_hip_cfg *cfg, cruna_hip_ctx **out) {
   {
     double pts = 1.0;
     for (int d = 0; d < 3; ++d) pts *= padded_length(cfg->scheme[d] == CRUNA_HIP_PERIODIC, cfg->n[d]);
-    if (pts * sizeof(double) >= 4294967296.0)
-      return fail(CRUNA_HIP_EINVAL, "one FP64 field must fit in 4 GiB (32-bit device offsets)");
+    if (pts * sizeof(double) >= 2147483648.0)
+      return fail(CRUNA_HIP_EINVAL, "one FP64 field must fit in 2 GiB (the sweeps' buffer extent)");
   }
   if (cfg->precision != CRUNA_HIP_FP64 && cfg->precision != CRUNA_HIP_FP32_MIXED)
     return fail(CRUNA_HIP_EINVAL, "unknown precision");
@@ -1971,6 +1983,15 @@ int cruna_hip_profile_count(void) { return KID_COUNT; }
 
 }  // extern "C"
 
+// One bin of a batched ensemble (host side): work-groups and work-group size
+// of each launch, its adjoint members and their filter switches, its graph.
+struct BatchBin {
+  int blocks[2][2][3] = {}, wg[2][2][3] = {};  // [group][kind][dir]
+  int nadj = 0;
+  int afreq[3] = {0, 0, 0};
+  hipGraphExec_t exec = nullptr;
+};
+
 struct cruna_hip_ensemble {
   std::vector<cruna_hip_ctx *> m;
   std::vector<int> role;  // CRUNA_HIP_FORWARD / CRUNA_HIP_ADJOINT
@@ -1985,6 +2006,20 @@ struct cruna_hip_ensemble {
   std::vector<unsigned> gen;         // each member's setter generation when recorded
   int period = 0;
   bool dirty = true;  // the recorded graph no longer matches the members
+  // batched launches (batch.hpp): the members split into bins; a bin runs one
+  // launch per sweep kind over its members, recorded into a graph that is
+  // replayed on the bin's stream, and the bins run side by side
+  int batching = -1;            // bins; 0: per-member graphs; -1: automatic
+  int batched_steps = 0;        // steps of the last run done by batched launches
+  BatchMember *bmem = nullptr;  // device: one descriptor per member
+  BatchList *blist = nullptr;   // device: per bin, lists [group: stepping forward, adjoint][kind: sweep, filter][dir]
+  int blist_cap = 0;            // lists allocated
+  std::vector<BatchBin> bins;
+  std::vector<hipStream_t> bstreams;  // one per bin (kept while the ensemble lives)
+  bool bper[3] = {true, true, true};
+  int bperiod = 0;              // steps of one recorded batched period
+  std::vector<unsigned> bgen;   // members' setter generations when recorded
+  std::vector<int> brole;       // members' roles (2: adjoint over subsets) when recorded
 };
 
 namespace {
@@ -2059,6 +2094,336 @@ void drop_graphs(cruna_hip_ensemble *e) {
   e->exec.clear();
 }
 
+// ---------------------------------------------------------------------------
+// Batched ensemble steps (batch.hpp)
+
+void drop_batch(cruna_hip_ensemble *e) {
+  for (auto &b : e->bins)
+    if (b.exec) (void)hipGraphExecDestroy(b.exec);
+  e->bins.clear();
+  e->bperiod = 0;
+}
+
+// Whether the members can run as one batch, and the period of the batched
+// launch sequence: every member FP64 on a Cartesian grid without boundary
+// rows, closure or weighted filter, all with the same schemes per direction
+// and the same filter switches (frequency 0 or 1, so that every step runs
+// the same passes); the adjoint members all over subset checkpoints of the
+// same length, each at a subset boundary (period: one subset), or all over
+// whole trajectories (period: one step).
+int batch_period(const cruna_hip_ensemble *e, int nsteps) {
+  const size_t M = e->m.size();
+  if (M > (size_t)BATCH_MAX) return 0;
+  const cruna_hip_ctx *c0 = e->m[0];
+  int sub = -1;  // -1: no adjoint member yet; 0: whole trajectories; S: subsets of S steps
+  for (size_t i = 0; i < M; ++i) {
+    const cruna_hip_ctx *c = e->m[i];
+    if (c->cfg.precision != CRUNA_HIP_FP64 || c->gt || c->curv || c->ev || c->padded || c->cnn ||
+        c->has_bcq || c->has_bcrhs || c->has_bclate || !c->bc.empty())
+      return 0;
+    for (int d = 0; d < 3; ++d) {
+      if (c->cfg.scheme[d] != c0->cfg.scheme[d] || c->cfg.wfilter_freq[d] > 0) return 0;
+      const int f = c->cfg.filter_freq[d], f0 = c0->cfg.filter_freq[d];
+      if ((f > 1) || (f0 > 1) || ((f > 0) != (f0 > 0))) return 0;
+    }
+    if (e->role[i] == CRUNA_HIP_ADJOINT) {
+      const int si = e->subm[i] ? c->sub_steps : 0;
+      if (sub >= 0 && si != sub) return 0;
+      sub = si;
+      if (e->subm[i] && (e->pos[i] != 0 || e->nt[i] % c->sub_steps != 0)) return 0;
+      if (!e->subm[i] && c->traj_slots > 1 && e->nt[i] - nsteps < 0) return 0;
+      for (int d = 0; d < 3; ++d) {
+        const int f = c->cfg.afilter_freq[d];
+        if (f > 1) return 0;
+        for (size_t j = 0; j < i; ++j)
+          if (e->role[j] == CRUNA_HIP_ADJOINT && (e->m[j]->cfg.afilter_freq[d] > 0) != (f > 0)) return 0;
+      }
+    }
+  }
+  return sub > 0 ? sub : 1;
+}
+
+// Descriptors and work lists of the batch on the device: the members split
+// into nb bins of about equal points (largest member first, each to the bin
+// with the fewest points so far).  Group 0 of a bin steps forward (forward
+// members and, with subsets, the adjoint members' recomputed trajectories),
+// group 1 runs the bin's adjoint members backward.
+int batch_upload(cruna_hip_ensemble *e, bool subsets, int nb) {
+  const size_t M = e->m.size();
+  std::vector<BatchMember> hm(M);
+  for (size_t i = 0; i < M; ++i) {
+    cruna_hip_ctx *c = e->m[i];
+    BatchMember &b = hm[i];
+    b.qn = c->qn;
+    b.acc = c->acc;
+    b.qs = (double *)c->qs;
+    b.rhs = (double *)c->rhs;
+    b.G = (double *)c->G;
+    b.qa = c->qa;
+    b.q0p = c->q0p;
+    b.ntp = c->ntp;
+    b.sckp = c->sckp;
+    b.traj = c->traj;
+    b.ckpt = c->ckpt;
+    b.traj_slots = c->traj_slots;
+    for (int d = 0; d < 3; ++d) {
+      b.op[d] = c->dop64[d].op;
+      b.fop[d] = c->fop64[d].op;
+    }
+    b.ph = c->phys;
+    b.N = c->N;
+    b.dt = c->cfg.dt;
+    b.gamma = c->cfg.gamma;
+  }
+  // tiles of as many lines as fill a work-group of 256 threads along x (the
+  // contiguous lines, as the contexts' own x tiles) and 512 along y and z,
+  // where a tile's lines are whole 64-byte segments of x-rows (a multiple of
+  // 8 lines) when that leaves it at least 8 lines
+  for (size_t i = 0; i < M; ++i)
+    for (int d = 0; d < 3; ++d) {
+      const int *np = e->m[i]->np;
+      const int NB = np[d] / PPT;
+      int TL = std::max(1, (d == 0 ? 256 : NT) / NB);
+      if (d != 0 && TL >= 8) TL -= TL % 8;
+      SweepGeom g = make_geom(np, d, 1);
+      g.TL = TL;
+      g.lines = d == 0 ? (long)np[1] * np[2] : (long)np[0] * np[3 - d];
+      g.ntiles = (int)((g.lines + TL - 1) / TL);
+      g.ntiles_in = 0;
+      hm[i].geom[d] = g;
+      hm[i].fgeom[d] = g;
+    }
+  std::vector<int> order(M);
+  for (size_t i = 0; i < M; ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hm[a].N > hm[b].N; });
+  std::vector<std::vector<int>> bm(nb);
+  std::vector<long> load(nb, 0);
+  for (int i : order) {
+    const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    bm[k].push_back(i);
+    load[k] += hm[i].N;
+  }
+  e->bins.assign(nb, BatchBin{});
+  std::vector<BatchList> hl((size_t)nb * 12);
+  for (int k = 0; k < nb; ++k) {
+    std::sort(bm[k].begin(), bm[k].end());
+    BatchBin &B = e->bins[k];
+    for (int i : bm[k])
+      if (e->role[i] == CRUNA_HIP_ADJOINT) {
+        ++B.nadj;
+        for (int d = 0; d < 3; ++d) B.afreq[d] = e->m[i]->cfg.afilter_freq[d];
+      }
+    for (int grp = 0; grp < 2; ++grp) {
+      std::vector<int> idx;
+      for (int i : bm[k])
+        if (grp == 0 ? (e->role[i] == CRUNA_HIP_FORWARD || subsets) : e->role[i] == CRUNA_HIP_ADJOINT)
+          idx.push_back(i);
+      for (int kind = 0; kind < 2; ++kind)
+        for (int d = 0; d < 3; ++d) {
+          BatchList &L = hl[(((size_t)k * 2 + grp) * 2 + kind) * 3 + d];
+          L.nm = (int)idx.size();
+          int blocks = 0, wg = 0;
+          for (size_t j = 0; j < idx.size(); ++j) {
+            const SweepGeom &g = kind == 0 ? hm[idx[j]].geom[d] : hm[idx[j]].fgeom[d];
+            L.mem[j] = idx[j];
+            L.start[j] = blocks;
+            blocks += g.ntiles;
+            wg = std::max(wg, g.TL * g.NB);
+          }
+          L.start[idx.size()] = blocks;
+          B.blocks[grp][kind][d] = blocks;
+          B.wg[grp][kind][d] = wg;
+        }
+    }
+  }
+  if (!e->bmem) HIP_TRY(hipMalloc(&e->bmem, BATCH_MAX * sizeof(BatchMember)));
+  if (e->blist_cap < nb * 12) {
+    if (e->blist) HIP_TRY(hipFree(e->blist));
+    e->blist = nullptr;
+    e->blist_cap = 0;
+    HIP_TRY(hipMalloc(&e->blist, hl.size() * sizeof(BatchList)));
+    e->blist_cap = nb * 12;
+  }
+  HIP_TRY(hipMemcpy(e->bmem, hm.data(), M * sizeof(BatchMember), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(e->blist, hl.data(), hl.size() * sizeof(BatchList), hipMemcpyHostToDevice));
+  for (int d = 0; d < 3; ++d) e->bper[d] = e->m[0]->cfg.scheme[d] == CRUNA_HIP_PERIODIC;
+  return CRUNA_HIP_OK;
+}
+
+// The launches of one bin go to stream s.
+struct BatchSlot {
+  const cruna_hip_ensemble *e;
+  int bin;
+  hipStream_t s;
+  const BatchBin &bb() const { return e->bins[bin]; }
+  const BatchList *list(int grp, int kind, int d) const {
+    return e->blist + (((size_t)bin * 2 + grp) * 2 + kind) * 3 + d;
+  }
+  const BatchMember *mem() const { return e->bmem; }
+};
+
+// launch of a batched kernel over list (grp, kind, d) of the bin; nothing for
+// an empty list
+#define BATCH_LAUNCH(S, grp, kind, d, KER, ...)                                                          \
+  do {                                                                                                   \
+    if ((S).bb().blocks[grp][kind][d] > 0)                                                               \
+      hipLaunchKernelGGL(KER, dim3((S).bb().blocks[grp][kind][d]), dim3((S).bb().wg[grp][kind][d]), 0,   \
+                         (S).s, (S).mem(), (S).list(grp, kind, d), __VA_ARGS__);                         \
+  } while (0)
+
+template <int DIR>
+void batch_sweep1(const BatchSlot &S, int from_qs) {
+  if (S.e->bper[DIR]) BATCH_LAUNCH(S, 0, 0, DIR, (kb_sweep1<DIR, true>), from_qs);
+  else BATCH_LAUNCH(S, 0, 0, DIR, (kb_sweep1<DIR, false>), from_qs);
+}
+
+template <int DIR, bool FF>
+void batch_sweep2(const BatchSlot &S, int mode) {
+  if (S.e->bper[DIR]) BATCH_LAUNCH(S, 0, 0, DIR, (kb_sweep2<DIR, true, FF>), mode);
+  else BATCH_LAUNCH(S, 0, 0, DIR, (kb_sweep2<DIR, false, FF>), mode);
+}
+
+template <int DIR>
+void batch_filter(const BatchSlot &S, int grp) {
+  if (S.e->bper[DIR]) BATCH_LAUNCH(S, grp, 1, DIR, (kb_filter<DIR, true>), grp);
+  else BATCH_LAUNCH(S, grp, 1, DIR, (kb_filter<DIR, false>), grp);
+}
+
+template <int DIR>
+void batch_asweep(const BatchSlot &S, int mode, int from_qs) {
+  if (S.e->bper[DIR]) BATCH_LAUNCH(S, 1, 0, DIR, (kb_asweep<DIR, true>), mode, from_qs);
+  else BATCH_LAUNCH(S, 1, 0, DIR, (kb_asweep<DIR, false>), mode, from_qs);
+}
+
+// One forward step of the bin's group 0 (step_once of each member: RK4 with
+// the x filter fused into the last stage, then the y and z filters); with
+// `record` the bin's subset members store Q(:,:,:,:,slot).
+int batch_forward_step(const BatchSlot &S, int record, int slot) {
+  const cruna_hip_cfg &cf = S.e->m[0]->cfg;
+  const bool fuse = cf.filter_freq[0] > 0;
+  for (int st = 0; st < 4; ++st) {
+    batch_sweep1<0>(S, st > 0);
+    batch_sweep1<1>(S, st > 0);
+    batch_sweep1<2>(S, st > 0);
+    batch_sweep2<2, false>(S, st);
+    batch_sweep2<1, false>(S, st);
+    if (st == 3 && fuse) batch_sweep2<0, true>(S, st);
+    else batch_sweep2<0, false>(S, st);
+  }
+  if (cf.filter_freq[1] > 0) batch_filter<1>(S, 0);
+  if (cf.filter_freq[2] > 0) batch_filter<2>(S, 0);
+  if (record && S.bb().nadj > 0)
+    hipLaunchKernelGGL(kb_record, dim3(256, S.bb().nadj), dim3(256), 0, S.s, S.mem(), S.list(1, 0, 0), slot);
+  return check_launch();
+}
+
+// One backward step of the bin's adjoint members (astep_once of each).
+int batch_adjoint_step(const BatchSlot &S) {
+  const BatchBin &B = S.bb();
+  hipLaunchKernelGGL(kb_adjoint_slot, dim3(1, B.nadj), dim3(64), 0, S.s, S.mem(), S.list(1, 0, 0));
+  for (int st = 0; st < 4; ++st) {
+    batch_asweep<2>(S, st, st > 0);
+    batch_asweep<1>(S, st, st > 0);
+    batch_asweep<0>(S, st, st > 0);
+  }
+  if (B.afreq[0] > 0) batch_filter<0>(S, 1);
+  if (B.afreq[1] > 0) batch_filter<1>(S, 1);
+  if (B.afreq[2] > 0) batch_filter<2>(S, 1);
+  return check_launch();
+}
+
+// The launches of one batched period of a bin: with subsets, the adjoint
+// members restore their subset's checkpoint, step it forward with the forward
+// members recording its trajectory, then run backward over it; without, one
+// forward step of the forward members and one backward step of the adjoint
+// members.
+int batch_record_period(const BatchSlot &S, int P, bool subsets) {
+  const int nadj = S.bb().nadj;
+  if (subsets) {
+    if (nadj > 0) {
+      hipLaunchKernelGGL(kb_restore, dim3(256, nadj), dim3(256), 0, S.s, S.mem(), S.list(1, 0, 0));
+      hipLaunchKernelGGL(kb_count_down, dim3(1, nadj), dim3(64), 0, S.s, S.mem(), S.list(1, 0, 0));
+      if (int rc = check_launch()) return rc;
+    }
+    for (int k = 1; k <= P; ++k)
+      if (int rc = batch_forward_step(S, 1, k)) return rc;
+    if (nadj > 0) {
+      hipLaunchKernelGGL(kb_set_counter, dim3(1, nadj), dim3(64), 0, S.s, S.mem(), S.list(1, 0, 0), P);
+      if (int rc = check_launch()) return rc;
+      for (int k = 0; k < P; ++k)
+        if (int rc = batch_adjoint_step(S)) return rc;
+    }
+    return CRUNA_HIP_OK;
+  }
+  if (int rc = batch_forward_step(S, 0, 0)) return rc;
+  if (nadj > 0)
+    if (int rc = batch_adjoint_step(S)) return rc;
+  return CRUNA_HIP_OK;
+}
+
+// Bins of a batched run: automatic (nb < 0) takes one bin per hardware queue
+// of the process (4), at most one per member.
+int batch_bins(const cruna_hip_ensemble *e) {
+  const int M = (int)e->m.size();
+  const int nb = e->batching < 0 ? 4 : e->batching;
+  return std::max(1, std::min(nb, M));
+}
+
+// Run `periods` batched periods of P steps (each bin's period recorded into
+// one graph once).
+int batch_run(cruna_hip_ensemble *e, int P, int periods) {
+  const size_t M = e->m.size();
+  const int nb = batch_bins(e);
+  bool subsets = false;
+  for (size_t i = 0; i < M; ++i) subsets = subsets || e->subm[i];
+  std::vector<int> roles(M);
+  for (size_t i = 0; i < M; ++i) roles[i] = e->subm[i] ? 2 : e->role[i];
+  bool stale = e->bins.size() != (size_t)nb || e->bperiod != P || e->bgen.size() != M || roles != e->brole;
+  for (size_t i = 0; i < M && !stale; ++i) stale = e->m[i]->gen != e->bgen[i];
+  while (e->bstreams.size() < (size_t)nb) {
+    hipStream_t st;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    e->bstreams.push_back(st);
+  }
+  for (auto *c : e->m) HIP_TRY(hipStreamSynchronize(c->stream));
+  if (stale) {
+    drop_batch(e);
+    if (int rc = batch_upload(e, subsets, nb)) return rc;
+    for (int k = 0; k < nb; ++k) {
+      const BatchSlot S{e, k, e->bstreams[k]};
+      HIP_TRY(hipStreamBeginCapture(S.s, hipStreamCaptureModeThreadLocal));
+      const int rc = batch_record_period(S, P, subsets);
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(S.s, &graph);
+      if (rc || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        drop_batch(e);
+        return rc ? rc : fail(CRUNA_HIP_EDEVICE, std::string("capture: ") + hipGetErrorString(ec));
+      }
+      const hipError_t ei = hipGraphInstantiate(&e->bins[k].exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        e->bins[k].exec = nullptr;
+        drop_batch(e);
+        return fail(CRUNA_HIP_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
+      }
+    }
+    e->bperiod = P;
+    e->bgen.resize(M);
+    for (size_t i = 0; i < M; ++i) e->bgen[i] = e->m[i]->gen;
+    e->brole = roles;
+  }
+  for (int r = 0; r < periods; ++r)
+    for (int k = 0; k < nb; ++k) HIP_TRY(hipGraphLaunch(e->bins[k].exec, e->bstreams[k]));
+  for (int k = 0; k < nb; ++k) HIP_TRY(hipStreamSynchronize(e->bstreams[k]));
+  for (size_t i = 0; i < M; ++i) {
+    cruna_hip_ctx *c = e->m[i];
+    c->qs_fresh = false;
+    e->nt[i] += e->role[i] == CRUNA_HIP_FORWARD ? P * periods : -P * periods;
+  }
+  return CRUNA_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2087,7 +2452,12 @@ int cruna_hip_ensemble_create(const cruna_hip_cfg *cfgs, int nmembers, cruna_hip
 int cruna_hip_ensemble_destroy(cruna_hip_ensemble *e) {
   if (!e) return CRUNA_HIP_OK;
   for (auto *c : e->m) (void)hipStreamSynchronize(c->stream);
+  for (auto st : e->bstreams) (void)hipStreamSynchronize(st);
   drop_graphs(e);
+  drop_batch(e);
+  if (e->bmem) (void)hipFree(e->bmem);
+  if (e->blist) (void)hipFree(e->blist);
+  for (auto st : e->bstreams) (void)hipStreamDestroy(st);
   for (auto *c : e->m) cruna_hip_destroy(c);
   delete e;
   return CRUNA_HIP_OK;
@@ -2153,6 +2523,24 @@ int cruna_hip_ensemble_run(cruna_hip_ensemble *e, int nsteps) {
     c->prof = false;
   }
   int left = nsteps;
+  // batched launches over all members for whole periods, when the members
+  // allow them (the rest of the steps as below)
+  e->batched_steps = 0;
+  if (e->batching != 0) {
+    const int BP = batch_period(e, nsteps);
+    if (BP > 0 && left >= BP) {
+      const int periods = left / BP;
+      if (int rc = batch_run(e, BP, periods)) return rc;
+      e->batched_steps = BP * periods;
+      left -= BP * periods;
+      if (left == 0) return CRUNA_HIP_OK;
+      for (size_t i = 0; i < M; ++i) {
+        cruna_hip_ctx *c = e->m[i];
+        if (e->role[i] == CRUNA_HIP_ADJOINT && !e->subm[i])
+          HIP_TRY(hipMemcpy(c->ntp, &e->nt[i], sizeof(int), hipMemcpyHostToDevice));
+      }
+    }
+  }
   // eager steps: members not yet in their repeating state, and the remainder
   // that does not fill a period
   bool all_warm = true;
@@ -2212,6 +2600,19 @@ int cruna_hip_ensemble_run(cruna_hip_ensemble *e, int nsteps) {
     for (size_t i = 0; i < M; ++i) HIP_TRY(hipGraphLaunch(e->exec[i], e->m[i]->stream));
   for (auto *c : e->m) HIP_TRY(hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < M; ++i) e->nt[i] += e->role[i] == CRUNA_HIP_FORWARD ? left : -left;
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_ensemble_set_batching(cruna_hip_ensemble *e, int bins) {
+  if (!e) return fail(CRUNA_HIP_EINVAL, "null argument");
+  if (bins < -1) return fail(CRUNA_HIP_EINVAL, "bins: -1 (automatic), 0 (per-member graphs) or a count");
+  e->batching = bins;
+  return CRUNA_HIP_OK;
+}
+
+int cruna_hip_ensemble_batched_steps(const cruna_hip_ensemble *e, int *steps) {
+  if (!e || !steps) return fail(CRUNA_HIP_EINVAL, "null argument");
+  *steps = e->batched_steps;
   return CRUNA_HIP_OK;
 }
 

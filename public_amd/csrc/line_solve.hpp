@@ -16,8 +16,10 @@
 namespace cruna {
 
 constexpr int NT = 512;  // max threads per work-group of the RHS sweeps
-// the FP32 filter's work-groups take up to 32 y/z lines (128-byte rows of floats)
+// the FP32 filter's and Cartesian sweeps' work-groups take up to 32 y/z lines
+// (128-byte rows of floats)
 constexpr int NT_FILTER32 = 1024;
+constexpr int NT_SWEEP32 = 1024;
 
 template <typename T>
 struct LineOp {
@@ -40,6 +42,7 @@ struct SweepGeom {
   long PS;        // distance between consecutive line points in memory
   int ntiles;     // work-groups in the grid
   int ntiles_in;  // (y/z) tiles along i
+  long lines;     // batched launches: lines of the direction, taken TL at a time
 };
 
 // Thread placement inside a tile.  The LDS slot of a (line, block) pair is the
@@ -59,11 +62,24 @@ __device__ inline int xcd_tile(int bid, int ntiles) {
   return (bid % 8) * (ntiles / 8) + bid / 8;
 }
 
-template <int DIR>
-__device__ inline ThreadPos thread_pos(const SweepGeom &g) {
+// Element offset of an idle thread's block: 2^28 FP64 elements is the 2 GiB
+// buffer extent of the sweeps' field accesses (the host keeps every field
+// below it), so an idle thread's loads return zero and its stores are dropped
+// by the buffer range check.  FP64 fields only (the batched launches).
+constexpr long IDLE_BASE = 1L << 28;
+
+// Placement in tile `tile` of a sweep of geometry g (the launch has exactly
+// TL x NB threads per work-group).
+// BATCH (a batched launch over members of several sizes): tile t holds lines
+// t*TL .. t*TL+TL-1 of the direction in line order (x: j + n2 k; y: i + n1 k;
+// z: i + n1 j), so that a tile of any TL fills its work-group; the work-group
+// may be larger than the tile, and the threads beyond it and the lines past
+// the member's last line are idle: their block lies at IDLE_BASE and their
+// exchange slots are those of block 0 of line 0, which they only read.
+template <int DIR, bool BATCH = false>
+__device__ inline ThreadPos thread_pos_tile(const SweepGeom &g, int tile) {
   ThreadPos t;
   const int tid = threadIdx.x;
-  const int tile = xcd_tile(blockIdx.x, g.ntiles);
   if (DIR == 0) {
     t.b = tid % g.NB;
     t.l = tid / g.NB;
@@ -72,22 +88,43 @@ __device__ inline ThreadPos thread_pos(const SweepGeom &g) {
     // x-lines take NB consecutive threads: with NB dividing the wavefront
     // size no line straddles two wavefronts
     t.wave_line = (64 % g.NB) == 0;
-    const long line = (long)tile * g.TL + t.l;
-    t.base = line * g.n1 + (long)t.b * PPT;
-    t.lidx = line;
   } else {
     t.l = tid % g.TL;
     t.b = tid / g.TL;
     t.BS = g.TL;
     t.LS = 1;
     t.wave_line = false;
-    const int i = (tile % g.ntiles_in) * g.TL + t.l;
-    const long o = tile / g.ntiles_in;  // k for y-lines, j for z-lines
-    const long off = (DIR == 1) ? (long)i + (long)g.n1 * g.n2 * o : (long)i + (long)g.n1 * o;
+  }
+  const long line = (long)tile * g.TL + t.l;
+  if (BATCH && (tid >= g.TL * g.NB || line >= g.lines)) {
+    t.l = 0;
+    t.b = 0;
+    t.base = IDLE_BASE;
+    t.lidx = 0;
+    return t;
+  }
+  if (DIR == 0) {
+    t.base = line * g.n1 + (long)t.b * PPT;
+    t.lidx = line;
+  } else {
+    long i, o;  // o: k for y-lines, j for z-lines
+    if (BATCH) {
+      i = line % g.n1;
+      o = line / g.n1;
+    } else {
+      i = (long)(tile % g.ntiles_in) * g.TL + t.l;
+      o = tile / g.ntiles_in;
+    }
+    const long off = (DIR == 1) ? i + (long)g.n1 * g.n2 * o : i + (long)g.n1 * o;
     t.base = off + (long)t.b * PPT * g.PS;
-    t.lidx = (long)i + (long)g.n1 * o;
+    t.lidx = i + (long)g.n1 * o;
   }
   return t;
+}
+
+template <int DIR>
+__device__ inline ThreadPos thread_pos(const SweepGeom &g) {
+  return thread_pos_tile<DIR>(g, xcd_tile(blockIdx.x, g.ntiles));
 }
 
 // Work-group barrier that orders LDS traffic only.  __syncthreads() also waits
@@ -190,17 +227,25 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T, NTM> 
 #pragma unroll
     for (int i = PPT - 2; i >= 0; --i) y[i] = (y[i] - up * y[i + 1]) * I[12 + i];
   }
-  if (!PER && (b == 0 || b == NB - 1 || (b == NB - 2 && op.spec2))) {
-    ty = b == 0 ? 0 : (b == NB - 1 ? 2 : 3);
-    const T *Mb = op.M + ty * PPT * FE;
+  if (!PER) {
+    // the end blocks of an open line: one pass per block type present in the
+    // wavefront, with that type's operator read through the scalar cache
+    // (uniform), so that its 14 coefficients per row take no vector registers
+    ty = b == 0 ? 0 : (b == NB - 1 ? 2 : ((b == NB - 2 && op.spec2) ? 3 : 1));
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (t == 1) continue;
+      if (__builtin_amdgcn_ballot_w64(ty == t) == 0) continue;
+      const cptr<T> Mb = launder_uniform(op.M + t * PPT * FE);
 #pragma unroll 1
-    for (int r = 0; r < PPT; ++r) {
-      T acc = T(0);
+      for (int r = 0; r < PPT; ++r) {
+        T acc = T(0);
 #pragma unroll
-      for (int j = 0; j < FE; ++j) acc += Mb[r * FE + j] * fe[j];
+        for (int j = 0; j < FE; ++j) acc += Mb[r * FE + j] * fe[j];
 #pragma unroll
-      for (int q = 0; q < PPT; ++q)
-        if (q == r) y[q] = acc;
+        for (int q = 0; q < PPT; ++q)
+          if (q == r && ty == t) y[q] = acc;
+      }
     }
   }
   // 3. reduced 2x2 block system by cyclic reduction over blocks
@@ -254,10 +299,19 @@ __device__ __forceinline__ void line_solve(const LineOp<T> &op, LineLds<T, NTM> 
 #pragma unroll
     for (int r = 0; r < PPT; ++r) x[r] = y[r] + opL[PPT + r] * xl + opR[PPT + r] * xf;
   } else {
-    const T *L = op.L + ty * PPT;
-    const T *R = op.R + ty * PPT;
 #pragma unroll
-    for (int r = 0; r < PPT; ++r) x[r] = y[r] + L[r] * xl + R[r] * xf;
+    for (int r = 0; r < PPT; ++r) x[r] = y[r];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (__builtin_amdgcn_ballot_w64(ty == t) == 0) continue;
+      const cptr<T> L = launder_uniform(op.L + t * PPT);
+      const cptr<T> R = launder_uniform(op.R + t * PPT);
+#pragma unroll
+      for (int r = 0; r < PPT; ++r) {
+        const T v = y[r] + L[r] * xl + R[r] * xf;
+        if (ty == t) x[r] = v;
+      }
+    }
   }
 }
 

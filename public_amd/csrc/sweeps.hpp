@@ -108,19 +108,30 @@ __device__ __forceinline__ T lambda_of(const Phys &ph, T mu) {
 // A thread's block is PPT consecutive points of its line.  Along x those are
 // contiguous and 64-byte aligned (FP64), so they move as 16-byte vectors;
 // along y/z they are PS elements apart.  Field accesses are raw buffer loads
-// and stores: the field base in a buffer resource (SGPRs), one 32-bit byte
-// offset per thread (a single VGPR) and, along y/z, the uniform distance
-// m*PS between a block's points as the scalar offset, so a block of PPT
-// points costs one address register instead of PPT 64-bit addresses (the
-// host checks that a field fits in 4 GiB).  Buffer accesses count against
-// vmcnt only, so the line engine's LDS waits never wait for them.
+// and stores: the field base in a buffer resource (SGPRs) and 32-bit byte
+// offsets (the host checks that a field fits in 2 GiB).  Along x and y a
+// block costs one offset VGPR: the distance m*PS between its points goes in
+// the scalar offset.  Along z that distance is a multiple of the plane size
+// (512 KiB at 256^2 FP64), and measured on the 256^3 step the z sweeps run
+// 5-40 % faster with the point offsets in VGPRs and a zero scalar offset.
+// Buffer accesses count against vmcnt only, so the line engine's LDS waits
+// never wait for them.
 typedef unsigned int bu2 __attribute__((ext_vector_type(2)));
 typedef unsigned int bu4 __attribute__((ext_vector_type(4)));
 
+// Buffer extent: 2 GiB from the field base (the host keeps one field below
+// that).  An idle thread of a batched launch (IDLE_BASE) has the extent
+// itself as its FP64 offset, so that its loads return zero and its stores are
+// dropped by the buffer range check (which tests the per-thread offset); along
+// z its point offsets stay below 4 GiB, as a field's plane distance times 7
+// stays below 2 GiB.
+constexpr unsigned FIELD_EXTENT = 0x80000000u;
+static_assert(IDLE_BASE * 8 == FIELD_EXTENT, "idle FP64 offset is the buffer extent");
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p) {
-  // raw buffer (stride 0) over the whole 32-bit offset range; word 3 is the
-  // gfx9 format word of a plain dword buffer
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, -1, 0x00020000);
+  // raw buffer (stride 0); word 3 is the gfx9 format word of a plain dword
+  // buffer
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, int(FIELD_EXTENT), 0x00020000);
 }
 
 template <int DIR, typename T, typename Q>
@@ -148,10 +159,11 @@ __device__ __forceinline__ void ld(const Q *__restrict__ p, long base, long PS, 
     const unsigned ps = unsigned(PS) * unsigned(sizeof(Q));
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
+      const unsigned vm = DIR == 2 ? vo + m * ps : vo, sm = DIR == 2 ? 0u : m * ps;
       if constexpr (sizeof(Q) == 8)
-        v[m] = T(__builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, m * ps, 0)));
+        v[m] = T(__builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vm, sm, 0)));
       else
-        v[m] = T(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, m * ps, 0)));
+        v[m] = T(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vm, sm, 0)));
     }
   }
 }
@@ -177,13 +189,19 @@ __device__ __forceinline__ void st(Q *__restrict__ p, long base, long PS, const 
     const unsigned ps = unsigned(PS) * unsigned(sizeof(Q));
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
+      const unsigned vm = DIR == 2 ? vo + m * ps : vo, sm = DIR == 2 ? 0u : m * ps;
       if constexpr (sizeof(Q) == 8)
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(bu2, double(v[m])), r, vo, m * ps, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(bu2, double(v[m])), r, vm, sm, 0);
       else
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, float(v[m])), r, vo, m * ps, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, float(v[m])), r, vm, sm, 0);
     }
   }
 }
+
+// Work-group size limit of the Cartesian viscous sweeps: FP32 tiles take up
+// to 32 y/z lines so that rows of floats are 128 bytes, as FP64 rows of 16.
+template <typename T>
+constexpr int sweep_nt() { return sizeof(T) == 4 ? NT_SWEEP32 : NT; }
 
 // ----------------------------------------------------------------------------
 // S1 / S2 / S3: first-derivative sweeps along DIR.
@@ -207,12 +225,11 @@ __device__ __forceinline__ void st(Q *__restrict__ p, long base, long PS, const 
 // the stored 1/phi, distributed over the directions (the pressure
 // gradient and u.grad(p) are not); both heat-flux derivatives take no h.
 template <typename T, typename Q, int DIR, bool PER, int FORM>
-__global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__restrict__ rhs,
-                                               T *__restrict__ G, LineOp<T> op, SweepGeom g,
-                                               Phys ph, long N, Metric<T> mt, Penal<T> pe) {
+__device__ __forceinline__ void sweep1_body(const ThreadPos &tp, const Q *__restrict__ qs, T *__restrict__ rhs,
+                                            T *__restrict__ G, const LineOp<T> &op, const SweepGeom &g,
+                                            const Phys &ph, long N, const Metric<T> &mt, const Penal<T> &pe) {
   constexpr bool GT = FORM == RHS_GT, EV = FORM == RHS_EV;
   __shared__ LineLds<T> lds;
-  const ThreadPos tp = thread_pos<DIR>(g);
   const long B = tp.base, PS = g.PS;
   const T mline = GT ? mt.m[DIR][tp.lidx] : T(1);
   const T ih = GT ? T(ph.inv_h[DIR]) * mline : T(ph.inv_h[DIR]);
@@ -348,6 +365,13 @@ __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__re
   st<DIR>(rhs + 4 * N, B, PS, e);
 }
 
+template <typename T, typename Q, int DIR, bool PER, int FORM>
+__global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__restrict__ rhs,
+                                                          T *__restrict__ G, LineOp<T> op, SweepGeom g,
+                                                          Phys ph, long N, Metric<T> mt, Penal<T> pe) {
+  sweep1_body<T, Q, DIR, PER, FORM>(thread_pos<DIR>(g), qs, rhs, G, op, g, ph, N, mt, pe);
+}
+
 // RK4 stage update of variable v from its complete RHS k (rk_o4_e.f90:45-50,
 // and :90-95 for the adjoint with dt -> -dt): stage inputs q0 + dt/2 k1,
 // q0 + dt/2 k2, q0 + dt k3, and q1 = q0 + dt/6 (k1 + 2 k2 + 2 k3 + k4).  The
@@ -404,16 +428,15 @@ __device__ __forceinline__ void rk_update(int mode, int v, const T (&k)[PPT], lo
 // more (ns_rho_rhou_p_grid_transform.f90:190-276).
 // EV: d/dx_d (phi tau_id), times the stored 1/phi (ns_rho_rhou_p_ev_phi.f90:161-201).
 template <typename T, int DIR, bool PER, int FORM, bool FF>
-__global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__restrict__ G,
-                                               LineOp<T> op, LineOp<T> fop, SweepGeom g, Phys ph,
-                                               long N, int mode,
-                                               double *__restrict__ qn, double *__restrict__ acc,
-                                               T *__restrict__ qs_out, double dt, Metric<T> mt,
-                                               const T *__restrict__ force, Penal<T> pe) {
+__device__ __forceinline__ void sweep2_body(const ThreadPos &tp, T *__restrict__ rhs, const T *__restrict__ G,
+                                            const LineOp<T> &op, const LineOp<T> &fop, const SweepGeom &g,
+                                            const Phys &ph, long N, int mode, double *__restrict__ qn,
+                                            double *__restrict__ acc, T *__restrict__ qs_out, double dt,
+                                            const Metric<T> &mt, const T *__restrict__ force,
+                                            const Penal<T> &pe) {
   constexpr bool GT = FORM == RHS_GT, EV = FORM == RHS_EV;
   constexpr bool LAST = DIR == 0;
-  __shared__ LineLds<T> lds;
-  const ThreadPos tp = thread_pos<DIR>(g);
+  __shared__ LineLds<T, sweep_nt<T>()> lds;
   const long B = tp.base, PS = g.PS;
   const T ih = GT ? T(ph.inv_h[DIR]) * mt.m[DIR][tp.lidx] : T(ph.inv_h[DIR]);
   T t1[PPT], t2[PPT], t3[PPT], diss[PPT];
@@ -548,6 +571,17 @@ __global__ __launch_bounds__(NT) void k_sweep2(T *__restrict__ rhs, const T *__r
   }
 }
 
+template <typename T, int DIR, bool PER, int FORM, bool FF>
+__global__ __launch_bounds__(sweep_nt<T>()) void k_sweep2(T *__restrict__ rhs, const T *__restrict__ G,
+                                                          LineOp<T> op, LineOp<T> fop, SweepGeom g, Phys ph,
+                                                          long N, int mode, double *__restrict__ qn,
+                                                          double *__restrict__ acc, T *__restrict__ qs_out,
+                                                          double dt, Metric<T> mt, const T *__restrict__ force,
+                                                          Penal<T> pe) {
+  sweep2_body<T, DIR, PER, FORM, FF>(thread_pos<DIR>(g), rhs, G, op, fop, g, ph, N, mode, qn, acc, qs_out, dt, mt,
+                                     force, pe);
+}
+
 // ----------------------------------------------------------------------------
 // Single derivative (Dx1/2/3) of one field: out = (B^-1 A in) / h.
 template <typename T, int DIR, bool PER>
@@ -568,11 +602,9 @@ template <typename T>
 constexpr int filter_nt() { return sizeof(T) == 4 ? NT_FILTER32 : NT; }
 
 template <typename T, int DIR, bool PER>
-__global__ __launch_bounds__(filter_nt<T>()) void k_filter(double *__restrict__ q, LineOp<T> op,
-                                                           SweepGeom g, long N, int nv,
-                                                           T *__restrict__ copy) {
+__device__ __forceinline__ void filter_body(const ThreadPos &tp, double *__restrict__ q, const LineOp<T> &op,
+                                            const SweepGeom &g, long N, int nv, T *__restrict__ copy) {
   __shared__ LineLds<T, filter_nt<T>()> lds;
-  const ThreadPos tp = thread_pos<DIR>(g);
   double a[PPT];
   ld<DIR>(q, tp.base, g.PS, a);
   for (int v = 0; v < nv; ++v) {
@@ -587,6 +619,12 @@ __global__ __launch_bounds__(filter_nt<T>()) void k_filter(double *__restrict__ 
     // reads it (it equals the stored FP64 value converted back)
     if (copy) st<DIR>(copy + v * N, tp.base, g.PS, x);
   }
+}
+
+template <typename T, int DIR, bool PER>
+__global__ __launch_bounds__(filter_nt<T>()) void k_filter(double *__restrict__ q, LineOp<T> op, SweepGeom g,
+                                                           long N, int nv, T *__restrict__ copy) {
+  filter_body<T, DIR, PER>(thread_pos<DIR>(g), q, op, g, N, nv, copy);
 }
 
 }  // namespace cruna

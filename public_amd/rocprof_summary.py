@@ -1,15 +1,16 @@
 """Reduce rocprofv3 result databases of a libcruna_hip run to per-kernel tables.
 
     python3 -m public_amd.rocprof_summary trace  <kernel-trace results.db>
-    python3 -m public_amd.rocprof_summary traffic <FETCH_SIZE results.db> <WRITE_SIZE results.db> [copied_bytes]
+    python3 -m public_amd.rocprof_summary traffic <FETCH_SIZE results.db> <WRITE_SIZE results.db> [copy_bytes]
 
 `trace` lists every kernel with its dispatch count and average duration.
 `traffic` lists FETCH_SIZE and WRITE_SIZE per dispatch (KiB, as the counters
 report them) and the HBM bytes per dispatch after scaling each counter by what
-it reads for the device-to-device copies in the same run, which move
-`copied_bytes` in total (default 3 GiB: bench.py --copy-calibration issues three
-1 GiB hipMemcpy copies, whatever number of dispatches the runtime splits them
-into): the scale makes the copies' fetched and written bytes equal that total.
+it reads for the calibration copies in the same run: bench.py
+--copy-calibration issues three device-to-device copies of 1 GiB each, which
+are the run's largest copy dispatches (other copies of the run, e.g. state
+copies of a 256^3 box, are smaller); the scale makes each of those copies read
+and write `copy_bytes` (default 1 GiB).
 
 Library kernels are named as the library's profiler names them (sweep1_x,
 sweep2_y_rk, filter_z, asweep_x_rk, ...) with a /fp64 or /fp32 suffix for the
@@ -76,6 +77,17 @@ def counters(db, counter):
     return agg
 
 
+def calibration_copies(db, counter):
+    """Counter values (KiB) of the calibration copies: the copy dispatches
+    within 1 % of the largest copy dispatch."""
+    vals = [v for name, cn, v in _rows(db, "select kernel_name, counter_name, value from counters_collection")
+            if cn == counter and kernel_name(name).startswith("copy:")]
+    if not vals:
+        return []
+    top = max(vals)
+    return [v for v in vals if v >= 0.99 * top]
+
+
 def print_trace(db):
     agg = trace(db)
     print(f"{'kernel':34s} {'dispatches':>10s} {'avg_us':>10s} {'grid':>8s} {'wg':>5s}")
@@ -83,16 +95,15 @@ def print_trace(db):
         print(f"{name:34s} {n:10d} {tot / n / 1e3:10.2f} {g:8d} {w:5d}")
 
 
-def print_traffic(fetch_db, write_db, copied_bytes=3.0 * (1 << 30)):
+def print_traffic(fetch_db, write_db, copy_bytes=float(1 << 30)):
     fe, wr = counters(fetch_db, "FETCH_SIZE"), counters(write_db, "WRITE_SIZE")
-    kf = sum(v for k, (n, v) in fe.items() if k.startswith("copy:"))
-    kw = sum(v for k, (n, v) in wr.items() if k.startswith("copy:"))
+    cf, cw = calibration_copies(fetch_db, "FETCH_SIZE"), calibration_copies(write_db, "WRITE_SIZE")
     fscale = wscale = None
-    if kf > 0 and kw > 0:
-        fscale = copied_bytes / (kf * 1024.0)
-        wscale = copied_bytes / (kw * 1024.0)
-        print(f"calibration: copies of {copied_bytes:.0f} B in total read FETCH_SIZE {kf:.0f} KiB and "
-              f"WRITE_SIZE {kw:.0f} KiB -> scale fetch x{fscale:.3f}, write x{wscale:.3f}")
+    if cf and cw:
+        fscale = len(cf) * copy_bytes / (sum(cf) * 1024.0)
+        wscale = len(cw) * copy_bytes / (sum(cw) * 1024.0)
+        print(f"calibration: {len(cf)} / {len(cw)} copies of {copy_bytes:.0f} B read FETCH_SIZE {sum(cf):.0f} KiB and "
+              f"WRITE_SIZE {sum(cw):.0f} KiB -> scale fetch x{fscale:.3f}, write x{wscale:.3f}")
     print(f"{'kernel':34s} {'FETCH_KiB':>12s} {'WRITE_KiB':>12s} {'hbm_bytes':>14s}")
     for name, (n, v) in fe.items():
         f = v / n
@@ -106,7 +117,7 @@ if __name__ == "__main__":
     if len(sys.argv) >= 3 and sys.argv[1] == "trace":
         print_trace(sys.argv[2])
     elif len(sys.argv) >= 4 and sys.argv[1] == "traffic":
-        print_traffic(sys.argv[2], sys.argv[3], float(sys.argv[4]) if len(sys.argv) > 4 else 3.0 * (1 << 30))
+        print_traffic(sys.argv[2], sys.argv[3], float(sys.argv[4]) if len(sys.argv) > 4 else float(1 << 30))
     else:
         print(__doc__)
         sys.exit(2)

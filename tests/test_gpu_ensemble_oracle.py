@@ -8,9 +8,13 @@
   (1e-12 relative L2 per variable), through eager steps and replayed graphs.
 * The batched workload as stated: 32 boxes of N in {48, 64, 96, 128}^3 and
   Ma in [0.05, 0.3] drawn from seed 0, half forward, half adjoint over 100
-  steps checkpointed every 10: every member bit-identical to its own run
+  steps checkpointed every 10, all of it through batched launches (one per
+  sweep kind over every member): every member bit-identical to its own run
   outside the ensemble, and one forward and one adjoint 48^3 member against
   the oracle.
+* Batched launches with an open direction and adjoint members over whole
+  trajectories, against the members' own runs, the per-member graphs and the
+  oracle.
 """
 import numpy as np
 import pytest
@@ -148,6 +152,9 @@ def test_batched_workload_as_stated():
             else:
                 ens.set_role(i, FORWARD, 0)
         ens.run(steps)
+        # FP64 periodic boxes with every filter each step and one subset
+        # length: the whole run goes through batched launches
+        assert ens.batched_steps() == steps
         out = [ens[i].get_adjoint_state() if i in adj else ens[i].get_state() for i in range(len(specs))]
     print("ensemble done", flush=True)
     for i, spec in enumerate(specs):
@@ -177,3 +184,84 @@ def test_batched_workload_as_stated():
     for v in range(5):
         assert rel(out[ai][..., v], ref[..., v]) < 1e-12, (v, rel(out[ai][..., v], ref[..., v]))
     print(f"adjoint member {ai} ({specs[ai]['n'][0]}^3) matches the oracle", flush=True)
+
+
+def test_batched_open_direction_and_whole_trajectories():
+    """Members of three sizes on grids periodic in x and z and open in y (no
+    boundary rows), two forward and two adjoint over whole trajectories
+    recorded on the device: the batched runs (all members in one bin, and in
+    two bins side by side) equal each member's own run and the per-member
+    graphs bit for bit, and match the oracle."""
+    from public_amd import ADJOINT, FORWARD, OPEN, PERIODIC, Cruna, Ensemble, ic
+    scheme = (PERIODIC, OPEN, PERIODIC)
+    sizes = [(32, 24, 16), (16, 32, 24), (24, 16, 32), (32, 32, 16)]
+    steps = 5
+    specs, states = [], []
+    for k, n in enumerate(sizes):
+        # smooth fields on the unit cube, periodic in x and z
+        dx = (1.0 / n[0], 1.0 / (n[1] - 1), 1.0 / n[2])
+        X, Y, Z = np.meshgrid(*(np.arange(m) * h for m, h in zip(n, dx)), indexing="ij")
+        noise = np.random.default_rng(20 + k).standard_normal(n)
+        q = np.zeros(n + (5,), order="F")
+        q[..., 0] = 1 + 0.1 * np.sin(2 * np.pi * X) * np.cos(np.pi * Y + 2 * np.pi * Z)
+        q[..., 1] = q[..., 0] * (0.3 * np.cos(2 * np.pi * Z) + 0.01 * noise)
+        q[..., 2] = q[..., 0] * 0.1 * np.sin(2 * np.pi * (X + Z)) * np.sin(np.pi * Y)
+        q[..., 3] = q[..., 0] * 0.1 * np.cos(2 * np.pi * X) * np.cos(np.pi * Y)
+        q[..., 4] = 1 / 1.4 + 0.05 * np.cos(2 * np.pi * X + np.pi * Y) * np.sin(2 * np.pi * Z)
+        specs.append(dict(n=n, dx=dx, scheme=scheme, mu0=1e-3, T0=1.0, dt=1e-3, afilter_freq=(1, 1, 1)))
+        states.append(q)
+    adj = {2, 3}
+    rng = np.random.default_rng(11)
+    qsT = {i: np.asfortranarray(rng.standard_normal(states[i].shape)) for i in adj}
+
+    def prepare(g, i):
+        g.set_state(states[i])
+        if i in adj:
+            g.trajectory_alloc(steps)
+            g.trajectory_record(True)
+            g.run_direct(steps)
+            g.trajectory_record(False)
+            g.set_adjoint_state(qsT[i])
+            g.synchronize()
+
+    outs = {}
+    for bins in (1, 2, 0):
+        with Ensemble(specs) as ens:
+            ens.set_batching(bins)
+            for i in range(len(specs)):
+                prepare(ens[i], i)
+                ens.set_role(i, ADJOINT if i in adj else FORWARD, steps if i in adj else 0)
+            ens.run(steps)
+            assert ens.batched_steps() == (steps if bins else 0)
+            outs[bins] = [ens[i].get_adjoint_state() if i in adj else ens[i].get_state()
+                          for i in range(len(specs))]
+    for i, spec in enumerate(specs):
+        kw = dict(spec)
+        with Cruna(kw.pop("n"), kw.pop("dx"), **kw) as g:
+            prepare(g, i)
+            if i in adj:
+                g.run_adjoint(steps)
+                g.synchronize()
+                ref = g.get_adjoint_state()
+            else:
+                g.run_direct(steps)
+                g.synchronize()
+                ref = g.get_state()
+        assert np.all(np.isfinite(ref)), i
+        for bins in (1, 2):
+            assert np.array_equal(outs[bins][i], ref), (bins, i, float(np.abs(outs[bins][i] - ref).max()))
+        assert np.array_equal(outs[0][i], ref), i
+        cfg = O.Cfg(spec["n"], spec["dx"], scheme=scheme, mu0=spec["mu0"], T0=spec["T0"], dt=spec["dt"],
+                    afilter_freq=(1, 1, 1))
+        if i in adj:
+            Q = np.zeros(states[i].shape + (steps,), order="F")
+            q = states[i]
+            for nt in range(1, steps + 1):
+                q = O.run_direct(cfg, q, 1, nt0=nt - 1)
+                Q[..., nt - 1] = q
+            oref = O.run_adjoint(cfg, qsT[i], Q)
+        else:
+            oref = O.run_direct(cfg, states[i], steps)
+        for v in range(5):
+            e = rel(outs[1][i][..., v], oref[..., v])
+            assert e < 1e-12, (i, v, e)
