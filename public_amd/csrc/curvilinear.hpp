@@ -182,7 +182,7 @@ __global__ __launch_bounds__(NT) void k_curv_sweep2(T *__restrict__ rhs, const T
                                                     const T *__restrict__ TG, LineOp<T> op,
                                                     LineOp<T> fop, SweepGeom g, Phys ph, long N,
                                                     int mode, double *__restrict__ qn,
-                                                    double *__restrict__ acc,
+                                                    T *__restrict__ acc,
                                                     T *__restrict__ qs_out, double dt,
                                                     CurvMetric<T> mt, const T *__restrict__ force,
                                                     Penal<T> pe) {
@@ -280,10 +280,10 @@ __global__ __launch_bounds__(NT) void k_curv_sweep2(T *__restrict__ rhs, const T
     }
     if constexpr (FF) {
       double a[PPT];
-      ld<DIR>(acc + v * N, B, PS, a);
+      rk_new_state<DIR, T>(v, k, B, PS, N, qn, acc, dt, a);
       T f[PPT], x[PPT];
 #pragma unroll
-      for (int m = 0; m < PPT; ++m) f[m] = T(a[m] + dt / 6.0 * double(k[m]));
+      for (int m = 0; m < PPT; ++m) f[m] = T(a[m]);
       line_solve<T, PER>(fop, lds, tp, f, x);
       st<DIR>(qn + v * N, B, PS, x);
     } else {

@@ -109,8 +109,9 @@ __device__ inline ThreadPos thread_pos_tile(const SweepGeom &g, int tile) {
   } else {
     long i, o;  // o: k for y-lines, j for z-lines
     if (BATCH) {
-      i = line % g.n1;
-      o = line / g.n1;
+      // a member's lines are fewer than 2^31
+      i = int(line) % g.n1;
+      o = int(line) / g.n1;
     } else {
       i = (long)(tile % g.ntiles_in) * g.TL + t.l;
       o = tile / g.ntiles_in;

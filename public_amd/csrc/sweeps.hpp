@@ -374,32 +374,54 @@ __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__re
 
 // RK4 stage update of variable v from its complete RHS k (rk_o4_e.f90:45-50,
 // and :90-95 for the adjoint with dt -> -dt): stage inputs q0 + dt/2 k1,
-// q0 + dt/2 k2, q0 + dt k3, and q1 = q0 + dt/6 (k1 + 2 k2 + 2 k3 + k4).  The
-// FP64 accumulator carries the partial new state q0 + dt/6 k1 + dt/3 k2 +
-// dt/3 k3, so the last stage reads it alone (not q0 as well); the sum is
-// associated differently from the reference's, a rounding-level change.
+// q0 + dt/2 k2, q0 + dt k3, and q1 = q0 + dt/6 (k1 + 2 k2 + 2 k3 + k4).
+// The accumulator has the arithmetic type T.  FP64: it carries the partial
+// new state q0 + dt/6 k1 + dt/3 k2 + dt/3 k3, so the last stage reads it
+// alone (not q0 as well).  FP32-mixed: it carries the increment
+// dt/6 k1 + dt/3 k2 + dt/3 k3 alone, in FP32 (half the bytes of an FP64
+// accumulator in three stages out of four), and the last stage adds q0 in
+// FP64; the increment is a step's change of the state, so its FP32 rounding
+// is far below that of the FP32 stage inputs.  Either sum is associated
+// differently from the reference's, a rounding-level change.
 // mode 4 writes k into rhs instead (a single RHS evaluation).
+template <int DIR, typename T>
+__device__ __forceinline__ void rk_new_state(int v, const T (&k)[PPT], long B, long PS, long N,
+                                             const double *__restrict__ qn, const T *__restrict__ acc,
+                                             double dt, double (&q1)[PPT]) {
+  if constexpr (sizeof(T) == 8) {
+    ld<DIR>(acc + v * N, B, PS, q1);
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) q1[m] = q1[m] + dt / 6.0 * double(k[m]);
+  } else {
+    T a[PPT];
+    ld<DIR>(qn + v * N, B, PS, q1);
+    ld<DIR>(acc + v * N, B, PS, a);
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) q1[m] = q1[m] + (double(a[m]) + dt / 6.0 * double(k[m]));
+  }
+}
+
 template <int DIR, typename T>
 __device__ __forceinline__ void rk_update(int mode, int v, const T (&k)[PPT], long B, long PS,
                                           long N, T *rhs, double *__restrict__ qn,
-                                          double *__restrict__ acc, T *qs_out, double dt) {
+                                          T *__restrict__ acc, T *qs_out, double dt) {
   if (mode == 4) {
     st<DIR>(rhs + v * N, B, PS, k);
     return;
   }
   double a[PPT];
   if (mode == 3) {
-    ld<DIR>(acc + v * N, B, PS, a);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) a[m] = a[m] + dt / 6.0 * double(k[m]);
+    rk_new_state<DIR, T>(v, k, B, PS, N, qn, acc, dt, a);
     st<DIR>(qn + v * N, B, PS, a);
     return;
   }
   double q0[PPT];
   ld<DIR>(qn + v * N, B, PS, q0);
+  // the accumulator's value after this stage
+  const double base = sizeof(T) == 8 ? 1.0 : 0.0;  // FP32: the increment alone
   if (mode == 0) {
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) a[m] = q0[m] + dt / 6.0 * double(k[m]);
+    for (int m = 0; m < PPT; ++m) a[m] = base * q0[m] + dt / 6.0 * double(k[m]);
   } else {
     ld<DIR>(acc + v * N, B, PS, a);
 #pragma unroll
@@ -431,7 +453,7 @@ template <typename T, int DIR, bool PER, int FORM, bool FF>
 __device__ __forceinline__ void sweep2_body(const ThreadPos &tp, T *__restrict__ rhs, const T *__restrict__ G,
                                             const LineOp<T> &op, const LineOp<T> &fop, const SweepGeom &g,
                                             const Phys &ph, long N, int mode, double *__restrict__ qn,
-                                            double *__restrict__ acc, T *__restrict__ qs_out, double dt,
+                                            T *__restrict__ acc, T *__restrict__ qs_out, double dt,
                                             const Metric<T> &mt, const T *__restrict__ force,
                                             const Penal<T> &pe) {
   constexpr bool GT = FORM == RHS_GT, EV = FORM == RHS_EV;
@@ -489,10 +511,10 @@ __device__ __forceinline__ void sweep2_body(const ThreadPos &tp, T *__restrict__
   auto finish = [&](int v, T(&k)[PPT]) {
     if constexpr (FF) {
       double a[PPT];
-      ld<DIR>(acc + v * N, B, PS, a);
+      rk_new_state<DIR, T>(v, k, B, PS, N, qn, acc, dt, a);
       T f[PPT], x[PPT];
 #pragma unroll
-      for (int m = 0; m < PPT; ++m) f[m] = T(a[m] + dt / 6.0 * double(k[m]));
+      for (int m = 0; m < PPT; ++m) f[m] = T(a[m]);
       line_solve<T, PER>(fop, lds, tp, f, x);
       st<DIR>(qn + v * N, B, PS, x);
     } else {
@@ -575,7 +597,7 @@ template <typename T, int DIR, bool PER, int FORM, bool FF>
 __global__ __launch_bounds__(sweep_nt<T>()) void k_sweep2(T *__restrict__ rhs, const T *__restrict__ G,
                                                           LineOp<T> op, LineOp<T> fop, SweepGeom g, Phys ph,
                                                           long N, int mode, double *__restrict__ qn,
-                                                          double *__restrict__ acc, T *__restrict__ qs_out,
+                                                          T *__restrict__ acc, T *__restrict__ qs_out,
                                                           double dt, Metric<T> mt, const T *__restrict__ force,
                                                           Penal<T> pe) {
   sweep2_body<T, DIR, PER, FORM, FF>(thread_pos<DIR>(g), rhs, G, op, fop, g, ph, N, mode, qn, acc, qs_out, dt, mt,
