@@ -444,6 +444,10 @@ def rel_errors(q, ref, gamma):
         out[name] = float(np.linalg.norm(q[..., v] - ref[..., v]) / np.linalg.norm(ref[..., v]))
     rE, rEr = ic.rho_E(q, gamma), ic.rho_E(ref, gamma)
     out["rho_E"] = float(np.linalg.norm(rE - rEr) / np.linalg.norm(rEr))
+    # the momentum as one vector field: a transverse component that is small
+    # (a shear layer's rho v, rho w) has a large error relative to itself
+    # but not relative to the momentum
+    out["momentum"] = float(np.linalg.norm(q[..., 1:4] - ref[..., 1:4]) / np.linalg.norm(ref[..., 1:4]))
     return out
 
 
