@@ -417,6 +417,7 @@ def run_closure(coarse_steps, seed=2001, mach=0.3):
         cl.close()
     out["validated_error_at_horizon"] = model.validated_error(coarse_steps)
     out["closure_used_by_fallback_rule"] = model.usable(coarse_steps)
+    out["validated_horizon_coarse_steps"] = model.horizon()
     out["gain"] = [float(x) for x in model.gain]
     return out
 

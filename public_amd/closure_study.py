@@ -271,12 +271,16 @@ def evaluate(closure, seed, mach, n_fine=256, factor=4, steps=300, every=50, log
 
 class ClosureModel:
     """A trained closure for one coarse grid with its inference gain and its
-    validation: the spectrum error of a validation rollout at coarse steps
-    0, every, 2 every, ...  `solve` runs the coarse grid with the closure
-    when the validated error at the requested horizon is within
-    `threshold`, else the plain fine solver (the fallback)."""
+    validation: the largest spectrum error of the validation rollouts (one
+    or several seeds) at coarse steps 0, every, 2 every, ...  `solve` runs
+    the coarse grid with the closure when the validated error at the
+    requested horizon is within margin x threshold, else the plain fine
+    solver (the fallback).  The margin (default 1/2) allows for flows that
+    drift faster than the validation seeds: a held-out Ma 0.3 seed reaches
+    10.7 % at coarse step 125, where the worst of four validation seeds
+    reads 8.6 %."""
 
-    def __init__(self, path, threshold=0.10):
+    def __init__(self, path, threshold=0.10, margin=0.5):
         d = np.load(path)
         self.S = int(d["S"])
         self.factor = int(d["factor"])
@@ -286,6 +290,7 @@ class ClosureModel:
         self.val_steps = np.asarray(d["val_steps"], dtype=int)
         self.val_errors = np.asarray(d["val_errors"], dtype=float)
         self.threshold = threshold
+        self.margin = margin
 
     def closure(self):
         c = Closure(self.S)
@@ -300,9 +305,16 @@ class ClosureModel:
         k = np.searchsorted(self.val_steps, coarse_steps)
         return float(self.val_errors[k]) if k < len(self.val_steps) else np.inf
 
+    def horizon(self):
+        """Coarse steps from the start over which the validated error stays
+        within margin x threshold (the last such checkpoint)."""
+        ok = self.val_errors <= self.margin * self.threshold
+        k = len(ok) if ok.all() else int(np.argmin(ok))
+        return int(self.val_steps[k - 1]) if k > 0 else 0
+
     def usable(self, coarse_steps):
         e = self.validated_error(coarse_steps)
-        return bool(np.isfinite(e) and e <= self.threshold)
+        return bool(np.isfinite(e) and e <= self.margin * self.threshold)
 
     def solve(self, q_fine0, mach, fine_steps):
         """(box-filtered state after fine_steps fine steps' time, "closure" or
@@ -331,6 +343,29 @@ def save(path, closure, stats, factor, gain, val_steps, val_errors):
              val_steps=np.asarray(val_steps), val_errors=np.asarray(val_errors))
 
 
+def revalidate(path, out_path, seeds, machs, n_fine=256, factor=4, steps=300, every=25, log=print):
+    """Validation curve of a trained closure over several seeds: at each
+    checkpoint the largest spectrum error among them, so that the closure's
+    horizon holds for each of them.  Writes the closure file with that curve
+    and returns each seed's curve."""
+    model = ClosureModel(path)
+    net = model.closure()
+    curves = []
+    try:
+        for s, m in zip(seeds, machs):
+            r = evaluate(net, s, m, n_fine, factor, steps, every=every, log=log, gains=[model.gain])
+            curves.append(r[gain_label(model.gain)]["spectrum_error"])
+    finally:
+        net.close()
+    d = dict(np.load(path))
+    d["val_steps"] = np.arange(0, steps + 1, every)
+    d["val_errors"] = np.max(np.array(curves), axis=0)
+    d["val_seeds"] = np.asarray(seeds)
+    d["val_machs"] = np.asarray(machs)
+    np.savez(out_path, **d)
+    return curves
+
+
 def main():
     ap = argparse.ArgumentParser(description="generate closure data, train, validate")
     ap.add_argument("--out", required=True)
@@ -354,11 +389,24 @@ def main():
     ap.add_argument("--target-steps", type=int, default=1, help="coarse steps of a 'step' target")
     ap.add_argument("--gains", nargs="*", default=["1", "1,1.1,1.1,1.1,1", "1,1.2,1.2,1.2,1", "1,1.3,1.3,1.3,1"],
                     help="inference gains to validate: a value or 5 comma-separated values each")
+    ap.add_argument("--revalidate", default=None,
+                    help="a trained closure file: only recompute its validation curve over --val-seeds")
+    ap.add_argument("--val-seeds", type=int, nargs="*", default=[2000, 2004, 2005, 2006])
+    ap.add_argument("--val-machs", type=float, nargs="*", default=[0.25, 0.3, 0.2, 0.1])
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
 
     def log(msg):
         print(msg, flush=True)
+
+    if args.revalidate:
+        curves = revalidate(args.revalidate, os.path.join(args.out, "closure_params.npz"), args.val_seeds,
+                            args.val_machs, args.n_fine, args.factor, args.rollout_steps, args.check_every, log)
+        with open(os.path.join(args.out, "closure_revalidation.json"), "w") as f:
+            json.dump({"seeds": args.val_seeds, "machs": args.val_machs, "every": args.check_every,
+                       "spectrum_error": curves}, f, indent=1)
+        log(json.dumps({"seeds": args.val_seeds, "final": [c[-1] for c in curves]}))
+        return
 
     rng = np.random.default_rng(args.seed)
     seeds = list(range(1000, 1000 + args.runs))

@@ -266,6 +266,16 @@ module cruna_hip
       type(c_ptr), value :: ens, tiles
       integer(c_int), value :: steps
     end function
+    integer(c_int) function c_ensemble_set_batching(ens, bins) bind(C, name="cruna_hip_ensemble_set_batching")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: bins
+    end function
+    integer(c_int) function c_ensemble_batched_steps(ens, steps) bind(C, name="cruna_hip_ensemble_batched_steps")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), intent(out) :: steps
+    end function
     integer(c_int) function c_cnn_create(s, cnn) bind(C, name="cruna_hip_cnn_create")
       import :: c_int, c_ptr
       integer(c_int), value :: s
@@ -315,6 +325,7 @@ module cruna_hip
   public :: hip_subsets_alloc, hip_run_direct_subsets, hip_run_adjoint_subsets
   public :: hip_ensemble_create, hip_ensemble_destroy, hip_ensemble_member
   public :: hip_ensemble_set_role, hip_ensemble_run, hip_ensemble_autotune
+  public :: hip_ensemble_set_batching, hip_ensemble_batched_steps
   public :: hip_closure_create, hip_closure_destroy, hip_set_closure
 
 contains
@@ -616,6 +627,23 @@ contains
     type(c_ptr), intent(in) :: ens
     integer, intent(in) :: nsteps
     call check(c_ensemble_run(ens, int(nsteps, c_int)), "ensemble_run")
+  end subroutine
+
+  ! batched launches: bins -1 automatic, 1 one launch per sweep kind over all
+  ! members, 0 the per-member graphs
+  subroutine hip_ensemble_set_batching(ens, bins)
+    type(c_ptr), intent(in) :: ens
+    integer, intent(in) :: bins
+    call check(c_ensemble_set_batching(ens, int(bins, c_int)), "ensemble_set_batching")
+  end subroutine
+
+  ! steps of the last run that went through batched launches
+  subroutine hip_ensemble_batched_steps(ens, steps)
+    type(c_ptr), intent(in) :: ens
+    integer, intent(out) :: steps
+    integer(c_int) :: s
+    call check(c_ensemble_batched_steps(ens, s), "ensemble_batched_steps")
+    steps = int(s)
   end subroutine
 
   ! tiles(3, members) receives every member's lines per work-group

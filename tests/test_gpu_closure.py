@@ -195,19 +195,30 @@ def test_closure_rejects_other_grids():
 def test_trained_closure_gains_over_plain_coarse_grid():
     """The committed closure (trained on the GPU by the library's own
     kernels on box-filtered 256^3 solves) against the same 64^3 solver
-    without it, from a held-out seed over 300 coarse steps (1200 fine
-    steps, the plain FP64 256^3 solve as the reference): the velocity-spectrum
-    error must fall well below the plain coarse grid's (measured at seed
-    2001, Ma 0.3: 0.25 against 0.58)."""
+    without it, from the three held-out seeds (Ma 0.3, 0.2, 0.15) over 300
+    coarse steps (1200 fine steps, the plain FP64 256^3 solve as the
+    reference).  The 10 % bar on the filtered velocity-spectrum error holds
+    up to the horizon the fallback rule validated (worst of four validation
+    seeds within half the bar: 100 coarse steps) on every held-out seed, and
+    at 300 steps, where the bar is out of reach (measured 25 / 21 / 22 %),
+    the closure still cuts the plain coarse grid's error (58 / 39 / 27 %);
+    the rule sends a 300-step request to the fine solver."""
     import os
     from public_amd import closure_study as CS
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     model = CS.ClosureModel(os.path.join(root, "public_amd", "data", "closure_64.npz"))
+    h = model.horizon()
+    assert h == 100, h
+    assert model.usable(h) and not model.usable(300)
     net = model.closure()
-    res = CS.evaluate(net, 2001, 0.3, 256, 4, 300, every=150, gains=[model.gain], log=print)
-    net.close()
-    with_c = res[CS.gain_label(model.gain)]["spectrum_error"][-1]
-    without = res["no_closure"]["spectrum_error"][-1]
-    print("spectrum error after 300 coarse steps: closure", with_c, "plain", without)
-    assert res[CS.gain_label(model.gain)]["finite"]
-    assert with_c < 0.6 * without, (with_c, without)
+    try:
+        for seed, mach in ((2001, 0.3), (2002, 0.2), (2003, 0.15)):
+            res = CS.evaluate(net, seed, mach, 256, 4, 300, every=h // 4, gains=[model.gain], log=print)
+            errs = res[CS.gain_label(model.gain)]["spectrum_error"]
+            plain = res["no_closure"]["spectrum_error"]
+            print(f"seed {seed}: closure {errs[4]:.4f} at step {h}, {errs[-1]:.4f} at 300; plain {plain[-1]:.4f}")
+            assert res[CS.gain_label(model.gain)]["finite"]
+            assert errs[4] <= model.threshold, (seed, errs[4])
+            assert errs[-1] < 0.85 * plain[-1], (seed, errs[-1], plain[-1])
+    finally:
+        net.close()
