@@ -126,11 +126,8 @@ __device__ __forceinline__ AxisOffsets axis_offsets(int i, int X, int stride, bo
 // a[d] for a runtime d in 0..2 without indexing a register array
 __device__ __forceinline__ int pick3(const int (&a)[3], int d) { return d == 0 ? a[0] : (d == 1 ? a[1] : a[2]); }
 
-// One convolution as an implicit GEMM.  A work-group stages the layer's
-// weights in LDS once (rows padded by 8 elements against bank conflicts) and
-// then walks blocks of 256 consecutive points (grid-stride); each wavefront
-// takes 64 points as 4 tiles of 16, reuses every weight fragment for the 4
-// tiles, and finds the 27 neighbours of a point from per-axis offset tables.
+// One convolution as an implicit GEMM on BF16 MFMA: out[p][co] over the 27
+// taps and CK input channels of each point's neighbours.
 // With 32 input channels a lane loads 16 bytes (channels 8(l/16)..+7) of a
 // neighbour row per tap, the K = 32 operand of one v_mfma_f32_16x16x32_bf16;
 // with 16 it loads 8 bytes for v_mfma_f32_16x16x16_bf16.
@@ -141,90 +138,137 @@ __device__ __forceinline__ int pick3(const int (&a)[3], int d) { return d == 0 ?
 //                   operator; out16 <- bf16(conv * gelu'(zin)) (zin: the
 //                   pre-activations of this layer's input, CN channels)
 // npts is a multiple of 64 and npts * 32 < 2^31 (checked by the host).
-constexpr int CONV_TILES = 4;      // 16-point tiles per wavefront
-constexpr int CONV_BLOCK = 256;    // points per work-group block
+constexpr int CONV_TILES = 4;      // 16-point MFMA tiles per wavefront
 constexpr int CONV_XCDS = 8;       // XCDs of the MI355X, each with its own L2
-template <int CK, int CN>
-constexpr int conv_lds_row() { return CK + 8; }  // 16-byte aligned rows, staggered banks
+// The convolution with weights and input served from LDS: a work-group
+// stages the layer's weights in LDS once and then walks tiles of 16 x 4 x 8 output points of one
+// box (x, y, z; partial tiles at the box's far faces), loads each tile's
+// input with a one-point halo, 18 x 6 x 10 points of CK channels, into LDS
+// once (wrapped on periodic boxes, zero outside a crop), and takes all 27
+// taps from there: each input row is read from L2 once per tile instead of
+// once per tap; the next tile's rows are in flight (in registers) while a
+// tile computes.  Wavefront w owns z-plane w of the tile and its 4 MFMA
+// tiles are the plane's 4 x-rows of 16 points; it reuses every weight
+// fragment for the 4 tiles, and each accumulator takes the taps in the
+// order t = 9 dz + 3 dy + dx.  The work-groups are persistent: work-group g
+// takes a contiguous range of tiles, the ranges of the work-groups on one
+// XCD (g % 8) adjacent, so that neighbouring tiles share their halo rows in
+// that XCD's L2.
+constexpr int CT_X = 16, CT_Y = 4, CT_Z = 8;
+constexpr int CT_THREADS = 64 * CT_Z;  // one wavefront per z-plane of the tile
+constexpr int CH_X = CT_X + 2, CH_Y = CT_Y + 2, CH_Z = CT_Z + 2;
+constexpr int CT_HALO = CH_X * CH_Y * CH_Z;  // input points per tile
 
 template <int CK, int CN, int MODE>
-__global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in, const bf16 *__restrict__ W,
-                                              const float *__restrict__ bias, ConvGeom geo, long npts,
-                                              float *__restrict__ out32, bf16 *__restrict__ out16,
-                                              bf16 *__restrict__ zsave, const bf16 *__restrict__ zin) {
-  constexpr int ROW = conv_lds_row<CK, CN>();
-  __shared__ __align__(16) bf16 wl[27 * CN * ROW];
-  // stage W [t][CN][CK] -> wl [t][CN][ROW] in 16-byte pieces, every load
-  // issued before the first store
+__global__ __launch_bounds__(CT_THREADS) void k_conv_tile(const bf16 *__restrict__ in, const bf16 *__restrict__ W,
+                                                   const float *__restrict__ bias, ConvGeom geo, long npts,
+                                                   float *__restrict__ out32, bf16 *__restrict__ out16,
+                                                   bf16 *__restrict__ zsave, const bf16 *__restrict__ zin) {
+  // weight rows of CK channels: a fragment read (16 rows, 4 lane groups)
+  // covers one contiguous run of LDS, free of bank conflicts
+  __shared__ __align__(16) bf16 wl[27 * CN * CK];
+  __shared__ __align__(16) bf16 xt[CT_HALO * CK];
   {
     constexpr int PIECES = 27 * CN * CK / 8, BATCH = 4;
-    for (int base = 0; base < PIECES; base += 256 * BATCH) {
+    for (int base = 0; base < PIECES; base += CT_THREADS * BATCH) {
       uint4 v[BATCH];
 #pragma unroll
       for (int u = 0; u < BATCH; ++u) {
-        const int e = base + threadIdx.x + 256 * u;
+        const int e = base + threadIdx.x + CT_THREADS * u;
         v[u] = e < PIECES ? *reinterpret_cast<const uint4 *>(W + (long)e * 8) : uint4{0u, 0u, 0u, 0u};
       }
 #pragma unroll
       for (int u = 0; u < BATCH; ++u) {
-        const int e = base + threadIdx.x + 256 * u;
-        if (e < PIECES) *reinterpret_cast<uint4 *>(wl + (e / (CK / 8)) * ROW + (e % (CK / 8)) * 8) = v[u];
+        const int e = base + threadIdx.x + CT_THREADS * u;
+        if (e < PIECES) *reinterpret_cast<uint4 *>(wl + (long)e * 8) = v[u];
       }
     }
   }
-  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 15, kg = lane >> 4;
-  const int sign = MODE == CONV_BACKWARD ? -1 : 1;
   const bool per = geo.periodic != 0;
-  // work-groups are dealt to the 8 XCDs in turn (id % 8): give XCD x the
-  // contiguous range of blocks [x span, (x + 1) span), so that the
-  // neighbour planes a block gathers from are in that XCD's own L2
-  const int nblk = (int)((npts + CONV_BLOCK - 1) / CONV_BLOCK);
-  const int span = (nblk + CONV_XCDS - 1) / CONV_XCDS;
-  for (int v = blockIdx.x; v < span * CONV_XCDS; v += gridDim.x) {
-    const int blk = (v % CONV_XCDS) * span + v / CONV_XCDS;
-    if (blk >= nblk) continue;
-    const int p0 = blk * CONV_BLOCK + wave * 16 * CONV_TILES;
-    if (p0 >= npts) continue;  // npts is a multiple of 64: a wave's 64 points are all in or all out
-    int pt[CONV_TILES];
-    AxisOffsets ax[CONV_TILES], ay[CONV_TILES], az[CONV_TILES];
+  const int X = geo.X, Y = geo.Y, Z = geo.Z;
+  const long XY = (long)X * Y, XYZ = XY * Z;
+  const int nbx = (X + CT_X - 1) / CT_X, nby = (Y + CT_Y - 1) / CT_Y, nbz = (Z + CT_Z - 1) / CT_Z;
+  const int per_box = nbx * nby * nbz;
+  const int ntiles = (int)(npts / XYZ) * per_box;
+  // contiguous range of tiles of this work-group (see above)
+  const int G = gridDim.x;
+  const int slot = (blockIdx.x % CONV_XCDS) * ((G + CONV_XCDS - 1) / CONV_XCDS) + blockIdx.x / CONV_XCDS;
+  const int chunk = (ntiles + G - 1) / G;
+  const int t0 = slot * chunk, t1 = t0 + chunk < ntiles ? t0 + chunk : ntiles;
+  constexpr int C8 = CK / 8;  // 16-byte pieces per input row
+  constexpr int PIECES = CT_HALO * C8, PER_THREAD = (PIECES + CT_THREADS - 1) / CT_THREADS;
+  // a tile's input rows: every load of the tile issued into registers
+  // (the next tile's while the current one computes), then stored to LDS
+  uint4 pre[PER_THREAD];
+  auto fetch = [&](int tile) {
+    const int b = tile / per_box, rem = tile % per_box;
+    const int i0 = (rem % nbx) * CT_X, j0 = ((rem / nbx) % nby) * CT_Y, k0 = (rem / (nbx * nby)) * CT_Z;
+    const long base = (long)b * XYZ;
 #pragma unroll
-    for (int j = 0; j < CONV_TILES; ++j) {
-      pt[j] = p0 + 16 * j + r;
-      const Coord cc = coord_of(pt[j], geo);
-      ax[j] = axis_offsets(cc.i, geo.X, 1, per, sign);
-      ay[j] = axis_offsets(cc.j, geo.Y, geo.X, per, sign);
-      az[j] = axis_offsets(cc.k, geo.Z, geo.X * geo.Y, per, sign);
+    for (int u = 0; u < PER_THREAD; ++u) {
+      const int e = threadIdx.x + CT_THREADS * u;
+      const int hp = e / C8, c8 = e % C8;
+      int gi = i0 + hp % CH_X - 1, gj = j0 + (hp / CH_X) % CH_Y - 1, gk = k0 + hp / (CH_X * CH_Y) - 1;
+      bool ok = e < PIECES;
+      if (per) {
+        gi = gi < 0 ? gi + X : (gi >= X ? gi - X : gi);
+        gj = gj < 0 ? gj + Y : (gj >= Y ? gj - Y : gj);
+        gk = gk < 0 ? gk + Z : (gk >= Z ? gk - Z : gk);
+      } else {
+        ok = ok && gi >= 0 && gi < X && gj >= 0 && gj < Y && gk >= 0 && gk < Z;
+      }
+      // points of a partial tile beyond the far face read a wrapped or zero
+      // row: their outputs are not stored
+      gi = gi < X ? gi : X - 1;
+      gj = gj < Y ? gj : Y - 1;
+      gk = gk < Z ? gk : Z - 1;
+      const long gp = base + gi + X * ((long)gj + (long)Y * gk);
+      pre[u] = ok ? *reinterpret_cast<const uint4 *>(in + gp * CK + c8 * 8) : uint4{0u, 0u, 0u, 0u};
     }
+  };
+  auto put = [&](bf16 *buf) {
+#pragma unroll
+    for (int u = 0; u < PER_THREAD; ++u) {
+      const int e = threadIdx.x + CT_THREADS * u;
+      if (e < PIECES) *reinterpret_cast<uint4 *>(buf + (long)e * 8) = pre[u];
+    }
+  };
+  if (t0 < t1) fetch(t0);
+  for (int tile = t0; tile < t1; ++tile) {
+    __syncthreads();  // the previous tile's reads of xt are done
+    put(xt);
+    __syncthreads();
+    if (tile + 1 < t1) fetch(tile + 1);
+    const bf16 *x = xt;
     f32x4 acc[CONV_TILES][CN / 16];
 #pragma unroll
     for (int j = 0; j < CONV_TILES; ++j)
 #pragma unroll
       for (int n = 0; n < CN / 16; ++n) acc[j][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // the 9 taps of each dz unrolled: the LDS reads of later taps issue
+    // while the matrix cores work on earlier ones
 #pragma unroll 1
     for (int dz = 0; dz < 3; ++dz)
-#pragma unroll 1
+#pragma unroll
       for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
         for (int dx = 0; dx < 3; ++dx) {
           const int t = 9 * dz + 3 * dy + dx;
-          int nb[CONV_TILES];
-          bool ok[CONV_TILES];
-#pragma unroll
-          for (int j = 0; j < CONV_TILES; ++j) {
-            nb[j] = pt[j] + ax[j].off[dx] + pick3(ay[j].off, dy) + pick3(az[j].off, dz);
-            ok[j] = per || ((ax[j].valid >> dx) & (ay[j].valid >> dy) & (az[j].valid >> dz) & 1u);
-          }
+          // neighbour (dx - 1, dy - 1, dz - 1), mirrored for the transposed operator
+          const int ex = MODE == CONV_BACKWARD ? 2 - dx : dx, ey = MODE == CONV_BACKWARD ? 2 - dy : dy,
+                    ez = MODE == CONV_BACKWARD ? 2 - dz : dz;
+          const int row0 = ((wave + ez) * CH_Y + ey) * CH_X + r + ex;  // x-row j adds j * CH_X
           if constexpr (CK == 32) {
             uint4 a[CONV_TILES];
 #pragma unroll
             for (int j = 0; j < CONV_TILES; ++j)
-              a[j] = ok[j] ? *reinterpret_cast<const uint4 *>(in + nb[j] * CK + kg * 8) : uint4{0u, 0u, 0u, 0u};
+              a[j] = *reinterpret_cast<const uint4 *>(x + (row0 + j * CH_X) * CK + kg * 8);
 #pragma unroll
             for (int n = 0; n < CN / 16; ++n) {
               const bf16x8 w = __builtin_bit_cast(
-                  bf16x8, *reinterpret_cast<const uint4 *>(wl + (t * CN + n * 16 + r) * ROW + kg * 8));
+                  bf16x8, *reinterpret_cast<const uint4 *>(wl + (t * CN + n * 16 + r) * CK + kg * 8));
 #pragma unroll
               for (int j = 0; j < CONV_TILES; ++j)
                 acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[j]), w, acc[j][n],
@@ -235,24 +279,32 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in, const
             bf16x4 a[CONV_TILES];
 #pragma unroll
             for (int j = 0; j < CONV_TILES; ++j)
-              a[j] = ok[j] ? *reinterpret_cast<const bf16x4 *>(in + nb[j] * CK + kg * 4) : bf16x4{0, 0, 0, 0};
+              a[j] = *reinterpret_cast<const bf16x4 *>(x + (row0 + j * CH_X) * CK + kg * 4);
 #pragma unroll
             for (int n = 0; n < CN / 16; ++n) {
-              const bf16x4 w = *reinterpret_cast<const bf16x4 *>(wl + (t * CN + n * 16 + r) * ROW + kg * 4);
+              const bf16x4 w = *reinterpret_cast<const bf16x4 *>(wl + (t * CN + n * 16 + r) * CK + kg * 4);
 #pragma unroll
               for (int j = 0; j < CONV_TILES; ++j)
                 acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[j], w, acc[j][n], 0, 0, 0);
             }
           }
         }
+    const int b = tile / per_box, rem = tile % per_box;
+    const int i0 = (rem % nbx) * CT_X, j0 = ((rem / nbx) % nby) * CT_Y, k0 = (rem / (nbx * nby)) * CT_Z;
+    const long base = (long)b * XYZ;
+    const int kk = k0 + wave;
 #pragma unroll
-    for (int j = 0; j < CONV_TILES; ++j)
+    for (int j = 0; j < CONV_TILES; ++j) {
+      const int jj = j0 + j;
+      if (jj >= Y || kk >= Z) continue;
 #pragma unroll
       for (int n = 0; n < CN / 16; ++n) {
         const int ch = n * 16 + r;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-          const int o = (p0 + 16 * j + 4 * kg + rr) * CN + ch;
+          const int ii = i0 + 4 * kg + rr;
+          if (ii >= X) continue;
+          const long o = (base + ii + X * ((long)jj + (long)Y * kk)) * CN + ch;
           if (MODE == CONV_GELU) {
             const float z = acc[j][n][rr] + bias[ch];
             if (zsave) zsave[o] = to_bf16(z);
@@ -264,6 +316,7 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in, const
           }
         }
       }
+    }
   }
 }
 

@@ -2669,12 +2669,21 @@ namespace {
 
 constexpr int WGRAD_CHUNK = 8192;
 
-// work-groups of k_conv: one per 256-point block up to two per CU (each
-// stages the weights once), a multiple of the XCD count so that a
-// work-group stays on its XCD's block range
-dim3 conv_grid(long npts) {
-  const long nblk = (npts + CONV_BLOCK - 1) / CONV_BLOCK;
-  return dim3((unsigned)std::min<long>((nblk + CONV_XCDS - 1) / CONV_XCDS * CONV_XCDS, 512));
+// launch of a convolution (k_conv_tile): persistent work-groups, as many as
+// the LDS lets reside on the MI355X's 256 CUs (160 KB each), a multiple of
+// the XCD count
+constexpr long CONV_CUS = 256, CONV_LDS_PER_CU = 163840;
+template <int CK, int CN, int MODE>
+void launch_conv(hipStream_t s, const ConvGeom &geo, long npts, const bf16 *in, const bf16 *W, const float *bias,
+                 float *out32, bf16 *out16, bf16 *zsave, const bf16 *zin) {
+  constexpr long lds = 27L * CN * CK * 2 + (long)CT_HALO * CK * 2;
+  const long box = (long)geo.X * geo.Y * geo.Z;
+  const long ntiles = npts / box * ((geo.X + CT_X - 1) / CT_X) * ((geo.Y + CT_Y - 1) / CT_Y) *
+                      ((geo.Z + CT_Z - 1) / CT_Z);
+  long G = std::min(ntiles, CONV_CUS * std::max(1L, CONV_LDS_PER_CU / lds));
+  G = (G + CONV_XCDS - 1) / CONV_XCDS * CONV_XCDS;
+  hipLaunchKernelGGL((k_conv_tile<CK, CN, MODE>), dim3((unsigned)G), dim3(CT_THREADS), 0, s, in, W, bias, geo, npts, out32,
+                     out16, zsave, zin);
 }
 
 ConvGeom conv_geom(int X, int Y, int Z, bool periodic) {
@@ -2744,20 +2753,15 @@ int cnn_pack(cruna_hip_cnn *n, hipStream_t s) {
 // `train` the pre-activations are kept for the backward pass
 int cnn_forward(cruna_hip_cnn *n, hipStream_t s, const ConvGeom &geo, long npts, bool train) {
   if (npts % 64 != 0 || npts > n->cap) return fail(CRUNA_HIP_EINVAL, "closure batch size");
-  const dim3 grid = conv_grid(npts), block(256);
   const float *P = n->params;
-  hipLaunchKernelGGL((k_conv<16, 32, CONV_GELU>), grid, block, 0, s, n->act[0], n->wf + n->wpk[0],
-                     P + n->boff[0], geo, npts, (float *)nullptr, n->act[1], train ? n->z[0] : nullptr,
-                     (const bf16 *)nullptr);
-  hipLaunchKernelGGL((k_conv<32, 32, CONV_GELU>), grid, block, 0, s, n->act[1], n->wf + n->wpk[1],
-                     P + n->boff[1], geo, npts, (float *)nullptr, n->act[2], train ? n->z[1] : nullptr,
-                     (const bf16 *)nullptr);
-  hipLaunchKernelGGL((k_conv<32, 32, CONV_GELU>), grid, block, 0, s, n->act[2], n->wf + n->wpk[2],
-                     P + n->boff[2], geo, npts, (float *)nullptr, n->act[3], train ? n->z[2] : nullptr,
-                     (const bf16 *)nullptr);
-  hipLaunchKernelGGL((k_conv<32, 16, CONV_LINEAR>), grid, block, 0, s, n->act[3], n->wf + n->wpk[3],
-                     P + n->boff[3], geo, npts, n->out32, (bf16 *)nullptr, (bf16 *)nullptr,
-                     (const bf16 *)nullptr);
+  launch_conv<16, 32, CONV_GELU>(s, geo, npts, n->act[0], n->wf + n->wpk[0], P + n->boff[0], nullptr, n->act[1],
+                                 train ? n->z[0] : nullptr, nullptr);
+  launch_conv<32, 32, CONV_GELU>(s, geo, npts, n->act[1], n->wf + n->wpk[1], P + n->boff[1], nullptr, n->act[2],
+                                 train ? n->z[1] : nullptr, nullptr);
+  launch_conv<32, 32, CONV_GELU>(s, geo, npts, n->act[2], n->wf + n->wpk[2], P + n->boff[2], nullptr, n->act[3],
+                                 train ? n->z[2] : nullptr, nullptr);
+  launch_conv<32, 16, CONV_LINEAR>(s, geo, npts, n->act[3], n->wf + n->wpk[3], P + n->boff[3], n->out32, nullptr,
+                                   nullptr, nullptr);
   return check_launch();
 }
 
@@ -2779,23 +2783,22 @@ void cnn_wgrad(cruna_hip_cnn *n, hipStream_t s, int l, const ConvGeom &geo, long
 // with `train`; *loss (if set) receives the mean-square loss
 int cnn_backward(cruna_hip_cnn *n, hipStream_t s, const ConvGeom &geo, long npts, int halo, long counted,
                  float *loss) {
-  const dim3 grid = conv_grid(npts), block(256);
   const int nb = (int)((npts + 255) / 256);
   const float scale = (float)(2.0 / ((double)counted * 5));
   hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, s, (const float *)n->out32, (const float *)n->tgt, npts,
                      5, geo, halo, scale, n->dz_a, n->lpart);
   // layer 3 (32 -> 16, linear)
   cnn_wgrad<32, 16>(n, s, 3, geo, npts, n->act[3], n->dz_a);
-  hipLaunchKernelGGL((k_conv<16, 32, CONV_BACKWARD>), grid, block, 0, s, n->dz_a, n->wt + n->wpk[3],
-                     (const float *)nullptr, geo, npts, (float *)nullptr, n->dz_b, (bf16 *)nullptr, n->z[2]);
+  launch_conv<16, 32, CONV_BACKWARD>(s, geo, npts, n->dz_a, n->wt + n->wpk[3], nullptr, nullptr, n->dz_b, nullptr,
+                                     n->z[2]);
   // layer 2
   cnn_wgrad<32, 32>(n, s, 2, geo, npts, n->act[2], n->dz_b);
-  hipLaunchKernelGGL((k_conv<32, 32, CONV_BACKWARD>), grid, block, 0, s, n->dz_b, n->wt + n->wpk[2],
-                     (const float *)nullptr, geo, npts, (float *)nullptr, n->dz_a, (bf16 *)nullptr, n->z[1]);
+  launch_conv<32, 32, CONV_BACKWARD>(s, geo, npts, n->dz_b, n->wt + n->wpk[2], nullptr, nullptr, n->dz_a, nullptr,
+                                     n->z[1]);
   // layer 1
   cnn_wgrad<32, 32>(n, s, 1, geo, npts, n->act[1], n->dz_a);
-  hipLaunchKernelGGL((k_conv<32, 32, CONV_BACKWARD>), grid, block, 0, s, n->dz_a, n->wt + n->wpk[1],
-                     (const float *)nullptr, geo, npts, (float *)nullptr, n->dz_b, (bf16 *)nullptr, n->z[0]);
+  launch_conv<32, 32, CONV_BACKWARD>(s, geo, npts, n->dz_a, n->wt + n->wpk[1], nullptr, nullptr, n->dz_b, nullptr,
+                                     n->z[0]);
   // layer 0 (16 -> 32)
   cnn_wgrad<16, 32>(n, s, 0, geo, npts, n->act[0], n->dz_b);
   if (int rc = check_launch()) return rc;
