@@ -6,7 +6,8 @@
 //   triad  c = a + s b      (2 reads, 1 write)
 //   read   sum of a         (1 read; one partial per work-group)
 // plus copy and triad with four nontemporal 16-byte accesses per thread and
-// iteration, and the copy with only two work-groups per CU.
+// iteration, the copy with only two work-groups per CU, and the sweeps'
+// mixes of fields read and written per point (fields_rR_wW).
 // 16-byte accesses, grid-stride over enough work-groups to fill every CU,
 // timed with HIP events over `reps` launches after one warm-up.  Prints one
 // JSON line: bytes moved per launch / average time for each kernel.
@@ -92,6 +93,31 @@ __global__ __launch_bounds__(TPB) void k_triad4(const dv2 *__restrict__ a, const
   for (; i < n2; i += stride) c[i] = a[i] + s * b[i];
 }
 
+// The sweeps' access mix: R fields read and W fields written per point
+// (sweep1_x reads 5 and writes 9 FP64 fields, sweep1_y 12 and 8,
+// sweep2_x_rk 17 and 11 in its middle stages), fields of F doubles each, the
+// point's 16-byte pair of every field per thread and iteration
+constexpr int MAXF = 28;
+struct Fields {
+  const double2 *in[MAXF];
+  double2 *out[MAXF];
+};
+
+template <int R, int W>
+__global__ __launch_bounds__(TPB) void k_fields(Fields f, long n2) {
+  for (long i = (long)blockIdx.x * TPB + threadIdx.x; i < n2; i += (long)gridDim.x * TPB) {
+    double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double2 x = f.in[r][i];
+      s.x += x.x;
+      s.y += x.y;
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) f.out[w][i] = make_double2(s.x * (w + 1), s.y * (w + 1));
+  }
+}
+
 __global__ void k_init(double *a, long n, double v) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     a[i] = v + 1e-9 * double(i & 1023);
@@ -165,6 +191,30 @@ int main(int argc, char **argv) {
   const int few = 2 * cus;
   time_it("copy_2wg_per_cu", 2 * bytes,
           [&] { hipLaunchKernelGGL(k_copy, dim3(few), dim3(TPB), 0, 0, A, (double2 *)c, n2); });
+  // the sweeps' field mixes on fields of 2^24 doubles (a 256^3 field)
+  {
+    const long nf = 1L << 24, nf2 = nf / 2;
+    std::vector<double *> fl(MAXF);
+    for (auto &p : fl) {
+      CHECK(hipMalloc(&p, nf * sizeof(double)));
+      hipLaunchKernelGGL(k_init, dim3(4096), dim3(256), 0, 0, p, nf, 1.0);
+    }
+    CHECK(hipDeviceSynchronize());
+    Fields f{};
+    const int fb = (int)std::min<long>(16L * cus, (nf2 + TPB - 1) / TPB);
+    auto mix = [&](const char *name, int R, int W, auto kern) {
+      for (int r = 0; r < R; ++r) f.in[r] = (const double2 *)fl[r];
+      for (int w = 0; w < W; ++w) f.out[w] = (double2 *)fl[R + w];
+      time_it(name, double(R + W) * nf * sizeof(double),
+              [&] { hipLaunchKernelGGL(kern, dim3(fb), dim3(TPB), 0, 0, f, nf2); });
+    };
+    mix("fields_r5_w9", 5, 9, k_fields<5, 9>);
+    mix("fields_r12_w8", 12, 8, k_fields<12, 8>);
+    mix("fields_r13_w8", 13, 8, k_fields<13, 8>);
+    mix("fields_r17_w11", 17, 11, k_fields<17, 11>);
+    mix("fields_r5_w5", 5, 5, k_fields<5, 5>);
+    for (auto p : fl) CHECK(hipFree(p));
+  }
   std::string out = "{\"array_bytes\": " + std::to_string((long)bytes) + ", \"reps\": " + std::to_string(reps) +
                     ", \"blocks\": " + std::to_string(blocks) + ", \"cus\": " + std::to_string(cus);
   for (auto &r : res) {
