@@ -341,11 +341,24 @@ def run_ensemble(steps, nmembers=32, seed=0, autotune=True, subset_steps=10):
                       "subset period; includes recording")
     res[-1]["note"] = ("the members in 4 bins of about equal points, each bin one launch per sweep kind over its "
                        "members, the bins' graphs replayed side by side on 4 streams; includes recording")
+    # HBM bytes the run moves by the sweeps' word counts: a forward step of
+    # every member (the adjoint members' recomputed trajectories included),
+    # a backward step of each adjoint member, the recorded trajectory slot
+    # (5 words written) and the subset's checkpoint restore (10 words per
+    # subset)
+    fwd_words = 4 * sum(SWEEP_WORDS[k] for k in ("sweep1_x", "sweep1_y", "sweep1_z", "sweep2_z", "sweep2_y",
+                                                   "sweep2_x_rk")) + SWEEP_WORDS["filter_y"] + SWEEP_WORDS["filter_z"]
+    adj_words = 4 * sum(SWEEP_WORDS[k] for k in ("asweep_z", "asweep_y", "asweep_x_rk")) + 3 * SWEEP_WORDS["filter_x"]
+    pts_adj = float(sum(int(sizes[i]) ** 3 for i in adj))
+    step_bytes = 8.0 * (pts * fwd_words + pts_adj * (adj_words + 5.0 + 10.0 / subset_steps))
+    for r in res.values():
+        r["hbm_TBps_by_word_counts"] = round(step_bytes * steps / r["seconds"] / 1e12, 3)
     return {"members": nmembers, "sizes": [int(s) for s in sizes], "forward": nmembers - len(adj),
             "adjoint": len(adj), "steps": steps, "checkpoint_every": subset_steps,
-            "points_per_step": pts,
+            "points_per_step": pts, "bytes_per_step_by_word_counts": step_bytes,
             "serial_eager": {"value": round(pts * steps / t_serial / 1e6, 2), "unit": "Mpts/s",
-                             "seconds": round(t_serial, 4)},
+                             "seconds": round(t_serial, 4),
+                             "hbm_TBps_by_word_counts": round(step_bytes * steps / t_serial / 1e12, 3)},
             "member_graphs": res[0], "batched_one_launch": res[1], "batched": res[-1],
             "speedup": res[-1]["speedup_vs_serial"], "tile_lines": tuned,
             "identical_to_serial": all(r["identical_to_serial"] for r in res.values())}
