@@ -51,6 +51,10 @@ SWEEP_WORDS = {
     "sweep2_z": 12.0,
     "sweep2_y": 12.0,
     "sweep2_x_rk": 28.25,  # mean over the four stages: 27, 32, 32, 22
+    # sweep1_z and sweep2_z fused (k_sweep12_z): sweep1_z's 21 words and the
+    # two velocity gradients (u_x, v_y) that only the viscous part reads; the
+    # rest of what the viscous part reads the kernel wrote itself
+    "sweep12_z": 23.0,
     "filter_x": 10.0,
     "filter_y": 10.0,
     "filter_z": 10.0,
@@ -64,7 +68,7 @@ SWEEP_WORDS = {
 # compute precision (the rest are FP64 state or accumulator words)
 SWEEP_COMPUTE_WORDS = {
     "sweep1_x": 14.0, "sweep1_y": 20.0, "sweep1_z": 21.0, "sweep2_z": 12.0, "sweep2_y": 12.0,
-    "sweep2_x_rk": 15.75, "filter_x": 0.0, "filter_y": 0.0, "filter_z": 0.0,
+    "sweep12_z": 23.0, "sweep2_x_rk": 15.75, "filter_x": 0.0, "filter_y": 0.0, "filter_z": 0.0,
     "asweep_z": 5.0, "asweep_y": 10.0, "asweep_x_rk": 12.5,
 }
 # The step-level byte model of the reference's dependency graph (4624 B per
@@ -346,7 +350,7 @@ def run_ensemble(steps, nmembers=32, seed=0, autotune=True, subset_steps=10):
     # a backward step of each adjoint member, the recorded trajectory slot
     # (5 words written) and the subset's checkpoint restore (10 words per
     # subset)
-    fwd_words = 4 * sum(SWEEP_WORDS[k] for k in ("sweep1_x", "sweep1_y", "sweep1_z", "sweep2_z", "sweep2_y",
+    fwd_words = 4 * sum(SWEEP_WORDS[k] for k in ("sweep1_x", "sweep1_y", "sweep12_z", "sweep2_y",
                                                    "sweep2_x_rk")) + SWEEP_WORDS["filter_y"] + SWEEP_WORDS["filter_z"]
     adj_words = 4 * sum(SWEEP_WORDS[k] for k in ("asweep_z", "asweep_y", "asweep_x_rk")) + 3 * SWEEP_WORDS["filter_x"]
     pts_adj = float(sum(int(sizes[i]) ** 3 for i in adj))

@@ -80,6 +80,20 @@ __global__ __launch_bounds__(NT) void kb_sweep2(const BatchMember *__restrict__ 
                                                Penal<double>{});
 }
 
+// the z first-derivative and viscous sweeps as one kernel (k_sweep12_z)
+template <bool PER>
+__global__ __launch_bounds__(NT) void kb_sweep12_z(const BatchMember *__restrict__ M,
+                                                   const BatchList *__restrict__ L, int from_qs) {
+  const int e = batch_entry(L);
+  const BatchMember &b = M[L->mem[e]];
+  const ThreadPos tp = thread_pos_tile<2, true>(b.geom[2], blockIdx.x - L->start[e]);
+  sweep1_body<double, double, 2, PER, RHS_CART>(tp, from_qs ? b.qs : b.qn, b.rhs, b.G, b.op[2], b.geom[2], b.ph,
+                                                b.N, Metric<double>{}, Penal<double>{});
+  asm volatile("" ::: "memory");
+  sweep2_body<double, 2, PER, RHS_CART, false>(tp, b.rhs, b.G, b.op[2], b.fop[2], b.geom[2], b.ph, b.N, 0, b.qn,
+                                               b.acc, b.qs, b.dt, Metric<double>{}, nullptr, Penal<double>{});
+}
+
 // filter of the direct (adj = 0) or adjoint state in place
 template <int DIR, bool PER>
 __global__ __launch_bounds__(NT) void kb_filter(const BatchMember *__restrict__ M, const BatchList *__restrict__ L,

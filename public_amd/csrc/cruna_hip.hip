@@ -43,14 +43,14 @@ int fail(int code, const std::string &msg) {
 
 enum KernelId {
   KID_S1X = 0, KID_S2Y, KID_S3Z, KID_S4Z, KID_S5Y, KID_S6X, KID_FX, KID_FY, KID_FZ, KID_DIFF,
-  KID_AX, KID_AY, KID_AZ, KID_CLOSURE, KID_COUNT
+  KID_AX, KID_AY, KID_AZ, KID_CLOSURE, KID_S12Z, KID_COUNT
 };
 // KID_CLOSURE times one whole closure evaluation (field means, features,
 // the four convolutions, the force and its mean projection)
 const char *kKernelNames[KID_COUNT] = {"sweep1_x", "sweep1_y",    "sweep1_z", "sweep2_z",
                                        "sweep2_y", "sweep2_x_rk", "filter_x", "filter_y",
                                        "filter_z", "diff",        "asweep_x_rk", "asweep_y",
-                                       "asweep_z", "closure"};
+                                       "asweep_z", "closure",     "sweep12_z"};
 
 template <typename T>
 struct DevOp {
@@ -549,6 +549,33 @@ void launch_curv_sweep1(cruna_hip_ctx *c, const Q *in) {
                        c->geom[DIR], c->phys, c->N, curv_metric<T>(c), penal<T>(c));
 }
 
+// the last first-derivative sweep and the first viscous sweep (both along z)
+// as one kernel over the same z-line tiles (k_sweep12_z)
+template <typename T, typename Q, int FORM>
+void launch_sweep12z_form(cruna_hip_ctx *c, const Q *in) {
+  const dim3 grid(c->geom[2].ntiles), block(c->wg[2]);
+  T *rhs = (T *)c->rhs, *G = (T *)c->G;
+  const LineOp<T> &op = dops<T>(c)[2].op;
+  if (c->cfg.scheme[2] == CRUNA_HIP_PERIODIC)
+    hipLaunchKernelGGL((k_sweep12_z<T, Q, true, FORM>), grid, block, 0, c->stream, in, rhs, G, op, c->geom[2],
+                       c->phys, c->N, metric<T>(c), penal<T>(c));
+  else
+    hipLaunchKernelGGL((k_sweep12_z<T, Q, false, FORM>), grid, block, 0, c->stream, in, rhs, G, op, c->geom[2],
+                       c->phys, c->N, metric<T>(c), penal<T>(c));
+}
+
+template <typename T, typename Q>
+int launch_sweep12z(cruna_hip_ctx *c, const Q *in) {
+  ProfScope p(c, KID_S12Z);
+  if (c->gt)
+    launch_sweep12z_form<T, Q, RHS_GT>(c, in);
+  else if (c->ev)
+    launch_sweep12z_form<T, Q, RHS_EV>(c, in);
+  else
+    launch_sweep12z_form<T, Q, RHS_CART>(c, in);
+  return check_launch();
+}
+
 template <typename T, typename Q, int DIR>
 int launch_sweep1(cruna_hip_ctx *c, const Q *in) {
   ProfScope p(c, DIR == 0 ? KID_S1X : (DIR == 1 ? KID_S2Y : KID_S3Z));
@@ -654,8 +681,12 @@ int rhs_stage(cruna_hip_ctx *c, const Q *in, int mode, bool fuse_filter = false)
     return fail(CRUNA_HIP_EINVAL, kEulerCartesian);
   if (int rc = launch_sweep1<T, Q, 0>(c, in)) return rc;
   if (int rc = launch_sweep1<T, Q, 1>(c, in)) return rc;
-  if (int rc = launch_sweep1<T, Q, 2>(c, in)) return rc;
-  if (int rc = launch_sweep2<T, 2>(c, mode, false)) return rc;
+  if (c->curv) {
+    if (int rc = launch_sweep1<T, Q, 2>(c, in)) return rc;
+    if (int rc = launch_sweep2<T, 2>(c, mode, false)) return rc;
+  } else {
+    if (int rc = launch_sweep12z<T, Q>(c, in)) return rc;
+  }
   if (int rc = launch_sweep2<T, 1>(c, mode, false)) return rc;
   if (c->has_bcrhs && !c->has_bclate)
     if (int rc = bc_rhs<T, Q>(c, (T *)c->rhs, in)) return rc;
@@ -2279,6 +2310,11 @@ void batch_sweep1(const BatchSlot &S, int from_qs) {
   else BATCH_LAUNCH(S, 0, 0, DIR, (kb_sweep1<DIR, false>), from_qs);
 }
 
+void batch_sweep12z(const BatchSlot &S, int from_qs) {
+  if (S.e->bper[2]) BATCH_LAUNCH(S, 0, 0, 2, (kb_sweep12_z<true>), from_qs);
+  else BATCH_LAUNCH(S, 0, 0, 2, (kb_sweep12_z<false>), from_qs);
+}
+
 template <int DIR, bool FF>
 void batch_sweep2(const BatchSlot &S, int mode) {
   if (S.e->bper[DIR]) BATCH_LAUNCH(S, 0, 0, DIR, (kb_sweep2<DIR, true, FF>), mode);
@@ -2306,8 +2342,7 @@ int batch_forward_step(const BatchSlot &S, int record, int slot) {
   for (int st = 0; st < 4; ++st) {
     batch_sweep1<0>(S, st > 0);
     batch_sweep1<1>(S, st > 0);
-    batch_sweep1<2>(S, st > 0);
-    batch_sweep2<2, false>(S, st);
+    batch_sweep12z(S, st > 0);
     batch_sweep2<1, false>(S, st);
     if (st == 3 && fuse) batch_sweep2<0, true>(S, st);
     else batch_sweep2<0, false>(S, st);

@@ -604,6 +604,26 @@ __global__ __launch_bounds__(sweep_nt<T>()) void k_sweep2(T *__restrict__ rhs, c
                                      force, pe);
 }
 
+// S3 + S4: the z first-derivative sweep and the z viscous sweep as one
+// kernel over the same z-line tiles.  The z sweep is the last
+// first-derivative sweep of a stage, so once a thread has completed the
+// velocity gradients of its block (w_z, u_z + w_x, v_z + w_y) the stresses
+// of the block are known and the viscous z derivatives can follow at once:
+// the gradients, viscosity and momentum RHS that the viscous part reads are
+// what the same thread has just written (still in the caches), instead of a
+// second pass over HBM.  Each part keeps its own LDS exchange buffers.
+template <typename T, typename Q, bool PER, int FORM>
+__global__ __launch_bounds__(NT) void k_sweep12_z(const Q *__restrict__ qs, T *__restrict__ rhs,
+                                                   T *__restrict__ G, LineOp<T> op, SweepGeom g, Phys ph,
+                                                   long N, Metric<T> mt, Penal<T> pe) {
+  const ThreadPos tp = thread_pos<2>(g);
+  sweep1_body<T, Q, 2, PER, FORM>(tp, qs, rhs, G, op, g, ph, N, mt, pe);
+  asm volatile("" ::: "memory");
+  sweep2_body<T, 2, PER, FORM, false>(tp, rhs, G, op, op, g, ph, N, 0, nullptr, nullptr, nullptr, 0.0, mt,
+                                      nullptr, pe);
+}
+
+
 // ----------------------------------------------------------------------------
 // Single derivative (Dx1/2/3) of one field: out = (B^-1 A in) / h.
 template <typename T, int DIR, bool PER>

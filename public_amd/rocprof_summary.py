@@ -38,6 +38,8 @@ def kernel_name(symbol):
         kind = "fp64" if args[0] == "double" else "fp32"
         d = next((a for a in args if a in ("0", "1", "2")), None)
         base = _SWEEP_NAMES.get((m.group(1), d), m.group(1) + "<" + ",".join(args) + ">")
+        if m.group(1) in ("k_sweep12_z", "kb_sweep12_z"):
+            base = "sweep12_z"
         return f"{base}/{kind}"
     if "copyBuffer" in symbol or "copy" in symbol.lower():
         return "copy:" + symbol[:40]
