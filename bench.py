@@ -51,11 +51,11 @@ SWEEP_WORDS = {
     "sweep2_z": 12.0,
     "sweep2_y": 12.0,
     "sweep2_x_rk": 28.25,  # mean over the four stages: 27, 32, 32, 22
-    # sweep1_z and sweep2_z fused (k_sweep12_z): the two sweeps' words; the
-    # viscous part re-reads what the same thread has just written, and those
-    # re-reads miss L2 (rocprofv3 FETCH_SIZE of the fused kernel is the two
-    # sweeps' sum within 2 %), so the compulsory 23 words are not what moves
-    "sweep12_z": 33.0,
+    # sweep1_z and sweep2_z fused (k_sweep12_z): sweep1_z's 21 words and the
+    # two velocity gradients (u_x, v_y) that only the viscous part reads; the
+    # other gradients, the viscosity and the momentum RHS pass between the
+    # parts in registers and LDS
+    "sweep12_z": 23.0,
     "filter_x": 10.0,
     "filter_y": 10.0,
     "filter_z": 10.0,
@@ -69,7 +69,7 @@ SWEEP_WORDS = {
 # compute precision (the rest are FP64 state or accumulator words)
 SWEEP_COMPUTE_WORDS = {
     "sweep1_x": 14.0, "sweep1_y": 20.0, "sweep1_z": 21.0, "sweep2_z": 12.0, "sweep2_y": 12.0,
-    "sweep12_z": 33.0, "sweep2_x_rk": 15.75, "filter_x": 0.0, "filter_y": 0.0, "filter_z": 0.0,
+    "sweep12_z": 23.0, "sweep2_x_rk": 15.75, "filter_x": 0.0, "filter_y": 0.0, "filter_z": 0.0,
     "asweep_z": 5.0, "asweep_y": 10.0, "asweep_x_rk": 12.5,
 }
 # The step-level byte model of the reference's dependency graph (4624 B per

@@ -63,8 +63,9 @@ __global__ __launch_bounds__(NT) void kb_sweep1(const BatchMember *__restrict__ 
   const int e = batch_entry(L);
   const BatchMember &b = M[L->mem[e]];
   const ThreadPos tp = thread_pos_tile<DIR, true>(b.geom[DIR], blockIdx.x - L->start[e]);
+  __shared__ LineLds<double> lds;
   sweep1_body<double, double, DIR, PER, RHS_CART>(tp, from_qs ? b.qs : b.qn, b.rhs, b.G, b.op[DIR], b.geom[DIR],
-                                                  b.ph, b.N, Metric<double>{}, Penal<double>{});
+                                                  b.ph, b.N, Metric<double>{}, Penal<double>{}, lds);
 }
 
 // viscous sweep; the x sweep finishes RK4 stage `mode` (FF: the last stage
@@ -87,11 +88,13 @@ __global__ __launch_bounds__(NT) void kb_sweep12_z(const BatchMember *__restrict
   const int e = batch_entry(L);
   const BatchMember &b = M[L->mem[e]];
   const ThreadPos tp = thread_pos_tile<2, true>(b.geom[2], blockIdx.x - L->start[e]);
-  sweep1_body<double, double, 2, PER, RHS_CART>(tp, from_qs ? b.qs : b.qn, b.rhs, b.G, b.op[2], b.geom[2], b.ph,
-                                                b.N, Metric<double>{}, Penal<double>{});
-  asm volatile("" ::: "memory");
-  sweep2_body<double, 2, PER, RHS_CART, false>(tp, b.rhs, b.G, b.op[2], b.fop[2], b.geom[2], b.ph, b.N, 0, b.qn,
-                                               b.acc, b.qs, b.dt, Metric<double>{}, nullptr, Penal<double>{});
+  __shared__ LineLds<double> lds;
+  __shared__ double mom[3][PPT][NT];
+  ZCarry<double> zc;
+  zc.mom = mom;
+  sweep1_body<double, double, 2, PER, RHS_CART, true>(tp, from_qs ? b.qs : b.qn, b.rhs, b.G, b.op[2], b.geom[2],
+                                                      b.ph, b.N, Metric<double>{}, Penal<double>{}, lds, &zc);
+  viscous_z_carry<double, PER>(tp, b.rhs, b.G, b.op[2], b.geom[2], b.ph, b.N, lds, zc);
 }
 
 // filter of the direct (adj = 0) or adjoint state in place

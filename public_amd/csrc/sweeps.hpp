@@ -224,12 +224,24 @@ constexpr int sweep_nt() { return sizeof(T) == 4 ? NT_SWEEP32 : NT; }
 // carries phi inside its derivative and its divergence is multiplied by
 // the stored 1/phi, distributed over the directions (the pressure
 // gradient and u.grad(p) are not); both heat-flux derivatives take no h.
-template <typename T, typename Q, int DIR, bool PER, int FORM>
+// What the fused z kernel's viscous part takes from its first-derivative
+// part (k_sweep12_z, Cartesian form): the block's velocity gradients that the
+// z sweep completes (s13 = u_z + w_x, s23 = v_z + w_y, w_z) and the
+// viscosity in registers; the momentum RHS of the block in LDS (the thread's
+// own slots) instead of a store and a reload through HBM.
+template <typename T>
+struct ZCarry {
+  T s13[PPT], s23[PPT], wz[PPT], mu[PPT];
+  T (*mom)[PPT][NT];  // LDS: [3][PPT][NT] momentum RHS (x, y, z)
+};
+
+template <typename T, typename Q, int DIR, bool PER, int FORM, bool FUSE = false>
 __device__ __forceinline__ void sweep1_body(const ThreadPos &tp, const Q *__restrict__ qs, T *__restrict__ rhs,
                                             T *__restrict__ G, const LineOp<T> &op, const SweepGeom &g,
-                                            const Phys &ph, long N, const Metric<T> &mt, const Penal<T> &pe) {
+                                            const Phys &ph, long N, const Metric<T> &mt, const Penal<T> &pe,
+                                            LineLds<T> &lds, ZCarry<T> *zc = nullptr) {
   constexpr bool GT = FORM == RHS_GT, EV = FORM == RHS_EV;
-  __shared__ LineLds<T> lds;
+  static_assert(!FUSE || (DIR == 2 && FORM == RHS_CART), "the fused z sweep is Cartesian");
   const long B = tp.base, PS = g.PS;
   const T mline = GT ? mt.m[DIR][tp.lidx] : T(1);
   const T ih = GT ? T(ph.inv_h[DIR]) * mline : T(ph.inv_h[DIR]);
@@ -276,6 +288,10 @@ __device__ __forceinline__ void sweep1_body(const ThreadPos &tp, const Q *__rest
       if (gsum(i)) x[m] = x[m] + gs[m];
     }
     st<DIR>(G + gslot(i) * N, B, PS, x);
+    if constexpr (FUSE) {
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) (k == 0 ? zc->s13 : zc->s23)[m] = x[m];
+    }
     if (DIR != 0) ld<DIR>(rhs + (1 + i) * N, B, PS, r);
     if (k == 1) ld<DIR>(qs + 4 * N, B, PS, p);  // in flight for the pressure terms
 #pragma unroll
@@ -283,7 +299,12 @@ __device__ __forceinline__ void sweep1_body(const ThreadPos &tp, const Q *__rest
     line_solve<T, PER>(op, lds, tp, f, x);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) r[m] = DIR == 0 ? div(m, -(x[m] * ih)) : r[m] + div(m, -(x[m] * ih));
-    st<DIR>(rhs + (1 + i) * N, B, PS, r);
+    if constexpr (FUSE) {
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) zc->mom[i][m][threadIdx.x] = r[m];
+    } else {
+      st<DIR>(rhs + (1 + i) * N, B, PS, r);
+    }
   }
 
   // d(u_DIR)/dx_DIR
@@ -294,6 +315,10 @@ __device__ __forceinline__ void sweep1_body(const ThreadPos &tp, const Q *__rest
 #pragma unroll
   for (int m = 0; m < PPT; ++m) x[m] = x[m] * ih;
   st<DIR>(G + DIR * N, B, PS, x);
+  if constexpr (FUSE) {
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) zc->wz[m] = x[m];
+  }
 
   // mass: -d/dx_DIR (rho u_DIR) (:131-135)
   if (DIR != 0) ld<DIR>(rhs, B, PS, r);
@@ -324,7 +349,12 @@ __device__ __forceinline__ void sweep1_body(const ThreadPos &tp, const Q *__rest
     r[m] = DIR == 0 ? div(m, -(x[m] * ih)) : r[m] + div(m, -(x[m] * ih));
     r[m] = r[m] - pd[m];
   }
-  st<DIR>(rhs + (1 + DIR) * N, B, PS, r);
+  if constexpr (FUSE) {
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) zc->mom[DIR][m][threadIdx.x] = r[m];
+  } else {
+    st<DIR>(rhs + (1 + DIR) * N, B, PS, r);
+  }
 
   // pressure-equation bracket: gamma/(gamma-1) (-d(p u_DIR)) + u_DIR p_DIR (:180-187)
   T e[PPT];
@@ -340,6 +370,10 @@ __device__ __forceinline__ void sweep1_body(const ThreadPos &tp, const Q *__rest
   if (DIR != 0) {
     ld<DIR>(rhs + 4 * N, B, PS, r);
     ld<DIR>(G + 6 * N, B, PS, mu);
+  }
+  if constexpr (FUSE) {
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) zc->mu[m] = mu[m];
   }
   if (GT) ld<DIR>(mt.J, B, PS, Jv);
 #pragma unroll
@@ -369,7 +403,8 @@ template <typename T, typename Q, int DIR, bool PER, int FORM>
 __global__ __launch_bounds__(NT) void k_sweep1(const Q *__restrict__ qs, T *__restrict__ rhs,
                                                           T *__restrict__ G, LineOp<T> op, SweepGeom g,
                                                           Phys ph, long N, Metric<T> mt, Penal<T> pe) {
-  sweep1_body<T, Q, DIR, PER, FORM>(thread_pos<DIR>(g), qs, rhs, G, op, g, ph, N, mt, pe);
+  __shared__ LineLds<T> lds;
+  sweep1_body<T, Q, DIR, PER, FORM>(thread_pos<DIR>(g), qs, rhs, G, op, g, ph, N, mt, pe, lds);
 }
 
 // RK4 stage update of variable v from its complete RHS k (rk_o4_e.f90:45-50,
@@ -608,19 +643,60 @@ __global__ __launch_bounds__(sweep_nt<T>()) void k_sweep2(T *__restrict__ rhs, c
 // kernel over the same z-line tiles.  The z sweep is the last
 // first-derivative sweep of a stage, so once a thread has completed the
 // velocity gradients of its block (w_z, u_z + w_x, v_z + w_y) the stresses
-// of the block are known and the viscous z derivatives can follow at once:
-// the gradients, viscosity and momentum RHS that the viscous part reads are
-// what the same thread has just written (still in the caches), instead of a
-// second pass over HBM.  Each part keeps its own LDS exchange buffers.
+// of the block are known and the viscous z derivatives can follow at once.
+// Cartesian form: the gradients and the viscosity stay in registers and the
+// momentum RHS in LDS between the two parts (ZCarry), so the viscous part
+// reads only u_x and v_y and writes the momentum RHS once; the deformed-grid
+// and penalised forms run the two sweep bodies one after the other.
+template <typename T, bool PER>
+__device__ __forceinline__ void viscous_z_carry(const ThreadPos &tp, T *__restrict__ rhs, const T *__restrict__ G,
+                                                const LineOp<T> &op, const SweepGeom &g, const Phys &ph, long N,
+                                                LineLds<T> &lds, const ZCarry<T> &zc) {
+  const long B = tp.base, PS = g.PS;
+  const T ih = T(ph.inv_h[2]);
+  T t1[PPT], t2[PPT], t3[PPT];
+  {
+    T ux[PPT], vy[PPT];
+    ld<2>(G + 0 * N, B, PS, ux);
+    ld<2>(G + 1 * N, B, PS, vy);
+    issue_loads();
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+      const T mu = zc.mu[m], wz = zc.wz[m];
+      const T md = -2.0 / 3.0 * mu * (ux[m] + vy[m] + wz);
+      t1[m] = mu * zc.s13[m];  // tau_13
+      t2[m] = mu * zc.s23[m];  // tau_23
+      t3[m] = md + 2 * mu * wz;  // tau_33
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    T x[PPT], k[PPT];
+    line_solve<T, PER>(op, lds, tp, i == 0 ? t1 : (i == 1 ? t2 : t3), x);
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) k[m] = zc.mom[i][m][threadIdx.x] + x[m] * ih;
+    st<2>(rhs + (1 + i) * N, B, PS, k);
+  }
+}
+
 template <typename T, typename Q, bool PER, int FORM>
 __global__ __launch_bounds__(NT) void k_sweep12_z(const Q *__restrict__ qs, T *__restrict__ rhs,
                                                    T *__restrict__ G, LineOp<T> op, SweepGeom g, Phys ph,
                                                    long N, Metric<T> mt, Penal<T> pe) {
   const ThreadPos tp = thread_pos<2>(g);
-  sweep1_body<T, Q, 2, PER, FORM>(tp, qs, rhs, G, op, g, ph, N, mt, pe);
-  asm volatile("" ::: "memory");
-  sweep2_body<T, 2, PER, FORM, false>(tp, rhs, G, op, op, g, ph, N, 0, nullptr, nullptr, nullptr, 0.0, mt,
-                                      nullptr, pe);
+  __shared__ LineLds<T> lds;
+  if constexpr (FORM == RHS_CART) {
+    __shared__ T mom[3][PPT][NT];
+    ZCarry<T> zc;
+    zc.mom = mom;
+    sweep1_body<T, Q, 2, PER, FORM, true>(tp, qs, rhs, G, op, g, ph, N, mt, pe, lds, &zc);
+    viscous_z_carry<T, PER>(tp, rhs, G, op, g, ph, N, lds, zc);
+  } else {
+    sweep1_body<T, Q, 2, PER, FORM>(tp, qs, rhs, G, op, g, ph, N, mt, pe, lds);
+    asm volatile("" ::: "memory");
+    sweep2_body<T, 2, PER, FORM, false>(tp, rhs, G, op, op, g, ph, N, 0, nullptr, nullptr, nullptr, 0.0, mt,
+                                        nullptr, pe);
+  }
 }
 
 
