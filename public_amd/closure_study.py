@@ -313,8 +313,13 @@ class ClosureModel:
         return int(self.val_steps[k - 1]) if k > 0 else 0
 
     def usable(self, coarse_steps):
-        e = self.validated_error(coarse_steps)
-        return bool(np.isfinite(e) and e <= self.margin * self.threshold)
+        """Whether the validated error stays within margin x threshold at
+        every checkpoint up to the first one at or beyond coarse_steps."""
+        k = int(np.searchsorted(self.val_steps, coarse_steps))
+        if k >= len(self.val_steps):
+            return False
+        e = self.val_errors[:k + 1]
+        return bool(np.all(np.isfinite(e)) and np.all(e <= self.margin * self.threshold))
 
     def solve(self, q_fine0, mach, fine_steps):
         """(box-filtered state after fine_steps fine steps' time, "closure" or
