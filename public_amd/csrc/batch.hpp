@@ -104,7 +104,8 @@ __global__ __launch_bounds__(NT) void kb_filter(const BatchMember *__restrict__ 
   const int e = batch_entry(L);
   const BatchMember &b = M[L->mem[e]];
   const ThreadPos tp = thread_pos_tile<DIR, true>(b.fgeom[DIR], blockIdx.x - L->start[e]);
-  filter_body<double, DIR, PER>(tp, adj ? b.qa : b.qn, b.fop[DIR], b.fgeom[DIR], b.N, 5, nullptr);
+  double *q = adj ? b.qa : b.qn;
+  filter_body<double, DIR, PER, double>(tp, q, q, b.fop[DIR], b.fgeom[DIR], b.N, 5, (double *)nullptr);
 }
 
 // adjoint sweep (backward in time: dt -> -dt); stage input qs^n (from_qs = 0) or qs

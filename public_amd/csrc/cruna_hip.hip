@@ -732,25 +732,33 @@ int rk4_step(cruna_hip_ctx *c, bool fuse_filter = false) {
 }
 
 // copy: also store the filtered state in the compute precision there (or null)
-template <typename T, int DIR>
-void launch_filter(cruna_hip_ctx *c, double *q, int nv, T *copy) {
+template <typename T, int DIR, typename S>
+void launch_filter(cruna_hip_ctx *c, const S *src, double *q64, int nv, T *copy) {
   const dim3 grid(c->fgeom[DIR].ntiles), block(c->fwg[DIR]);
   const LineOp<T> &op = fops<T>(c)[DIR].op;
   if (c->cfg.scheme[DIR] == CRUNA_HIP_PERIODIC)
-    hipLaunchKernelGGL((k_filter<T, DIR, true>), grid, block, 0, c->stream, q, op, c->fgeom[DIR],
+    hipLaunchKernelGGL((k_filter<T, DIR, true, S>), grid, block, 0, c->stream, src, q64, op, c->fgeom[DIR],
                        c->N, nv, copy);
   else
-    hipLaunchKernelGGL((k_filter<T, DIR, false>), grid, block, 0, c->stream, q, op, c->fgeom[DIR],
+    hipLaunchKernelGGL((k_filter<T, DIR, false, S>), grid, block, 0, c->stream, src, q64, op, c->fgeom[DIR],
                        c->N, nv, copy);
 }
 
+// one filter pass along d: from src (FP64 or the compute precision) into the
+// FP64 fields q64 and/or the compute-precision fields copy
+template <typename T, typename S>
+int filter_pass(cruna_hip_ctx *c, const S *src, double *q64, int d, int nv, T *copy) {
+  ProfScope p(c, KID_FX + d);
+  if (d == 0) launch_filter<T, 0, S>(c, src, q64, nv, copy);
+  if (d == 1) launch_filter<T, 1, S>(c, src, q64, nv, copy);
+  if (d == 2) launch_filter<T, 2, S>(c, src, q64, nv, copy);
+  return check_launch();
+}
+
+// the FP64 fields q filtered in place (and copied to `copy`)
 template <typename T>
 int filter_dir(cruna_hip_ctx *c, double *q, int d, int nv, T *copy = nullptr) {
-  ProfScope p(c, KID_FX + d);
-  if (d == 0) launch_filter<T, 0>(c, q, nv, copy);
-  if (d == 1) launch_filter<T, 1>(c, q, nv, copy);
-  if (d == 2) launch_filter<T, 2>(c, q, nv, copy);
-  return check_launch();
+  return filter_pass<T, double>(c, q, q, d, nv, copy);
 }
 
 // filter_weighted_iN on nv fields of the FP64 state q in place (through out)
@@ -820,13 +828,23 @@ int step_once(cruna_hip_ctx *c, int nt, bool weighted = true, bool *converged = 
     const int fq = c->cfg.filter_freq[d];
     if (fq > 0 && nt % fq == 0) last = d;
   }
+  // FP32-mixed: a pass's output is FP32 by construction (FP32 arithmetic), so
+  // a pass that another pass of the step follows leaves it in FP32 only (in
+  // qs) and the next pass reads it from there; the step's last pass writes
+  // the FP64 state and, for the next step's first stage, its FP32 copy
+  bool in32 = false;
   for (int d = fuse_fx ? 1 : 0; d < 3; ++d) {
     const int fq = c->cfg.filter_freq[d];
     if (fq > 0 && nt % fq == 0) {
-      // in FP32-mixed mode the step's last filter pass also leaves the state
-      // in FP32 for the next step's first stage
-      rc = mixed ? filter_dir<float>(c, c->qn, d, 5, d == last ? (float *)c->qs : nullptr)
-                 : filter_dir<double>(c, c->qn, d, 5);
+      if (mixed) {
+        float *q32 = (float *)c->qs;
+        double *q64 = d == last ? c->qn : nullptr;
+        rc = in32 ? filter_pass<float, float>(c, q32, q64, d, 5, q32)
+                  : filter_pass<float, double>(c, c->qn, q64, d, 5, q32);
+        in32 = true;
+      } else {
+        rc = filter_dir<double>(c, c->qn, d, 5);
+      }
       if (rc) return rc;
     }
   }

@@ -719,30 +719,34 @@ __global__ __launch_bounds__(NT) void k_diff(const T *__restrict__ in, T *__rest
 template <typename T>
 constexpr int filter_nt() { return sizeof(T) == 4 ? NT_FILTER32 : NT; }
 
-template <typename T, int DIR, bool PER>
-__device__ __forceinline__ void filter_body(const ThreadPos &tp, double *__restrict__ q, const LineOp<T> &op,
-                                            const SweepGeom &g, long N, int nv, T *__restrict__ copy) {
+// src: the fields to filter (FP64, or the compute precision); q64: where
+// the filtered FP64 fields go (null: nowhere); copy: the filtered fields in
+// the compute precision (null: nowhere).  src may be q64 or copy (in place:
+// every thread loads its block before the line solve that the stores follow).
+template <typename T, int DIR, bool PER, typename S>
+__device__ __forceinline__ void filter_body(const ThreadPos &tp, const S *src, double *q64, const LineOp<T> &op,
+                                            const SweepGeom &g, long N, int nv, T *copy) {
   __shared__ LineLds<T, filter_nt<T>()> lds;
   double a[PPT];
-  ld<DIR>(q, tp.base, g.PS, a);
+  ld<DIR>(src, tp.base, g.PS, a);
   for (int v = 0; v < nv; ++v) {
     T f[PPT], x[PPT];
 #pragma unroll
     for (int m = 0; m < PPT; ++m) f[m] = T(a[m]);
     // the next field's loads fly while this one is filtered
-    if (v + 1 < nv) ld<DIR>(q + (v + 1) * N, tp.base, g.PS, a);
+    if (v + 1 < nv) ld<DIR>(src + (v + 1) * N, tp.base, g.PS, a);
     line_solve<T, PER>(op, lds, tp, f, x);
-    st<DIR>(q + v * N, tp.base, g.PS, x);
+    if (q64) st<DIR>(q64 + v * N, tp.base, g.PS, x);
     // the filtered state in the compute precision, when the next RK stage
-    // reads it (it equals the stored FP64 value converted back)
+    // or filter pass reads it (it equals the FP64 value converted back)
     if (copy) st<DIR>(copy + v * N, tp.base, g.PS, x);
   }
 }
 
-template <typename T, int DIR, bool PER>
-__global__ __launch_bounds__(filter_nt<T>()) void k_filter(double *__restrict__ q, LineOp<T> op, SweepGeom g,
-                                                           long N, int nv, T *__restrict__ copy) {
-  filter_body<T, DIR, PER>(thread_pos<DIR>(g), q, op, g, N, nv, copy);
+template <typename T, int DIR, bool PER, typename S>
+__global__ __launch_bounds__(filter_nt<T>()) void k_filter(const S *src, double *q64, LineOp<T> op, SweepGeom g,
+                                                           long N, int nv, T *copy) {
+  filter_body<T, DIR, PER, S>(thread_pos<DIR>(g), src, q64, op, g, N, nv, copy);
 }
 
 }  // namespace cruna
