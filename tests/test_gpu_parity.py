@@ -152,7 +152,12 @@ def test_run_direct_matches_oracle(mach, nsteps):
         assert rel(got[..., v], ref[..., v]) < 1e-12, (v, rel(got[..., v], ref[..., v]))
 
 
-def test_fp32_mixed_close_to_fp64():
+# FP32-mixed filter passes hand their FP32 output to the next pass of the
+# step; the frequencies vary which passes run, which one is last and whether
+# the x pass runs inside the last stage.  A pass that read a stale FP32
+# state (the last stage input) would be off by O(dt |k|), far above 1e-5.
+@pytest.mark.parametrize("ffreq", [(1, 1, 1), (1, 2, 3), (2, 1, 2), (3, 2, 0)])
+def test_fp32_mixed_close_to_fp64(ffreq):
     from public_amd import FP32_MIXED
     Cruna = _gpu()
     n = 32
@@ -160,13 +165,13 @@ def test_fp32_mixed_close_to_fp64():
     q = _box(n, 0, 0.3)
     out = []
     for prec in (0, FP32_MIXED):
-        with Cruna((n,) * 3, (h,) * 3, precision=prec, **kw) as g:
+        with Cruna((n,) * 3, (h,) * 3, precision=prec, filter_freq=ffreq, **kw) as g:
             g.set_state(q)
             g.run_direct(10)
             g.synchronize()
             out.append(g.get_state())
     err = rel(out[1], out[0])
-    print(f"fp32-mixed vs fp64 after 10 steps: {err:.3e}")
+    print(f"fp32-mixed vs fp64 after 10 steps, filter frequencies {ffreq}: {err:.3e}")
     assert err < 1e-5
 
 
