@@ -191,35 +191,46 @@ __global__ __launch_bounds__(NT) void k_curv_sweep2(T *__restrict__ rhs, const T
   const ThreadPos tp = thread_pos<DIR>(g);
   const long B = tp.base, PS = g.PS;
   const T ih = T(ph.inv_h[DIR]);
-  T M0[PPT], M1[PPT], M2[PPT], Jv[PPT], mu[PPT];
+  T M0[PPT], M1[PPT], M2[PPT], mu[PPT];
   ld<DIR>(mt.M + (DIR + 0) * N, B, PS, M0);
   ld<DIR>(mt.M + (DIR + 3) * N, B, PS, M1);
   ld<DIR>(mt.M + (DIR + 6) * N, B, PS, M2);
-  ld<DIR>(mt.J, B, PS, Jv);
   ld<DIR>(G + 6 * N, B, PS, mu);
-  T PH[PPT], IP[PPT];
-  if (EV) {
-    ld<DIR>(pe.phi, B, PS, PH);
-    ld<DIR>(pe.inv_phi, B, PS, IP);
-  }
   // heat flux along DIR first (the metric row and mu are live for it and the
-  // stresses only): sum_j M(DIR,j) D_DIR(lambda J T_j), no h (:252-268)
+  // stresses only): sum_j M(DIR,j) D_DIR(lambda J T_j), no h (:252-268).
+  // J, phi and 1/phi are read where they are used (again later in the
+  // sweep, from cache) rather than held across it: holding them spilled the
+  // open and penalised forms.
   T hs[PPT];
+  {
+    T lj[PPT];  // (phi) lambda J
+    ld<DIR>(mt.J, B, PS, lj);
+    if (EV) {
+      T PH[PPT];
+      ld<DIR>(pe.phi, B, PS, PH);
 #pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    T f[PPT], x[PPT];
-    ld<DIR>(TG + j * N, B, PS, f);
+      for (int m = 0; m < PPT; ++m) lj[m] = (PH[m] * lambda_of(ph, mu[m])) * lj[m];
+    } else {
 #pragma unroll
-    for (int m = 0; m < PPT; ++m)
-      f[m] = (EV ? (PH[m] * lambda_of(ph, mu[m])) * Jv[m] : lambda_of(ph, mu[m]) * Jv[m]) * f[m];
-    line_solve<T, PER>(op, lds, tp, f, x);
+      for (int m = 0; m < PPT; ++m) lj[m] = lambda_of(ph, mu[m]) * lj[m];
+    }
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-      const T c = j == 0 ? M0[m] : (j == 1 ? M1[m] : M2[m]);
-      hs[m] = j == 0 ? c * x[m] : hs[m] + c * x[m];
+    for (int j = 0; j < 3; ++j) {
+      T f[PPT], x[PPT];
+      ld<DIR>(TG + j * N, B, PS, f);
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) f[m] = lj[m] * f[m];
+      line_solve<T, PER>(op, lds, tp, f, x);
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) {
+        const T c = j == 0 ? M0[m] : (j == 1 ? M1[m] : M2[m]);
+        hs[m] = j == 0 ? c * x[m] : hs[m] + c * x[m];
+      }
     }
   }
   if (EV) {
+    T IP[PPT];
+    ld<DIR>(pe.inv_phi, B, PS, IP);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) hs[m] = IP[m] * hs[m];
   }
@@ -291,6 +302,8 @@ __global__ __launch_bounds__(NT) void k_curv_sweep2(T *__restrict__ rhs, const T
     }
   };
   if (EV) {  // phi tau_proj inside the derivative
+    T PH[PPT];
+    ld<DIR>(pe.phi, B, PS, PH);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
       t1[m] = PH[m] * t1[m];
@@ -298,6 +311,8 @@ __global__ __launch_bounds__(NT) void k_curv_sweep2(T *__restrict__ rhs, const T
       t3[m] = PH[m] * t3[m];
     }
   }
+  T Jv[PPT];
+  if (LAST) ld<DIR>(mt.J, B, PS, Jv);
   if (LAST) {  // pressure: J (gamma - 1) (bracket + heat + J tau:grad(u)) (:270-276)
     T k[PPT];
     ld<DIR>(rhs + 4 * N, B, PS, k);
@@ -322,9 +337,15 @@ __global__ __launch_bounds__(NT) void k_curv_sweep2(T *__restrict__ rhs, const T
     asm volatile("" ::: "memory");
     T k[PPT];
     ld<DIR>(rhs + (1 + i) * N, B, PS, k);
+    if (EV) {
+      T IP[PPT];
+      ld<DIR>(pe.inv_phi, B, PS, IP);
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) x[m] = IP[m] * x[m];
+    }
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
-      k[m] = k[m] + (EV ? IP[m] * (x[m] * ih) : x[m] * ih);
+      k[m] = k[m] + x[m] * ih;
       if (LAST) k[m] = Jv[m] * k[m];
     }
     if (LAST)
