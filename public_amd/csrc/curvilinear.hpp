@@ -44,13 +44,20 @@ __global__ __launch_bounds__(NT) void k_curv_sweep1(const Q *__restrict__ qs, T 
   const ThreadPos tp = thread_pos<DIR>(g);
   const long B = tp.base, PS = g.PS;
   const T ih = T(ph.inv_h[DIR]);
-  T M0[PPT], M1[PPT], M2[PPT], irho[PPT], uh[PPT], x[PPT], f[PPT];
+  // In FP64, rho, and in the periodic x sweep the momenta, stay in
+  // registers from their first read to the convective fluxes: those sweeps
+  // have the registers for them at two waves per SIMD (the open x sweep
+  // would spill, FP32 sweeps would lose waves); re-reading them there
+  // missed the caches
+  constexpr bool KEEPR = sizeof(T) == 8, KEEPM = KEEPR && DIR == 0 && PER;
+  T M0[PPT], M1[PPT], M2[PPT], rho[PPT], irho[PPT], uh[PPT], x[PPT], f[PPT];
+  T mk[KEEPM ? 3 : 1][PPT];
   ld<DIR>(mt.M + (DIR + 0) * N, B, PS, M0);
   ld<DIR>(mt.M + (DIR + 3) * N, B, PS, M1);
   ld<DIR>(mt.M + (DIR + 6) * N, B, PS, M2);
-  ld<DIR>(qs, B, PS, f);
+  ld<DIR>(qs, B, PS, rho);
 #pragma unroll
-  for (int m = 0; m < PPT; ++m) irho[m] = T(1) / f[m];
+  for (int m = 0; m < PPT; ++m) irho[m] = T(1) / rho[m];
   // accumulate a value into field slot `fld` (the first sweep writes it)
   auto acc = [&](T *fld, T(&v)[PPT]) {
     if (!FIRST) {
@@ -69,6 +76,10 @@ __global__ __launch_bounds__(NT) void k_curv_sweep1(const Q *__restrict__ qs, T 
   for (int i = 0; i < 3; ++i) {
     T ui[PPT];
     ld<DIR>(qs + (1 + i) * N, B, PS, ui);
+    if constexpr (KEEPM) {
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) mk[i][m] = ui[m];
+    }
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
       ui[m] = ui[m] * irho[m];
@@ -145,8 +156,7 @@ __global__ __launch_bounds__(NT) void k_curv_sweep1(const Q *__restrict__ qs, T 
   }
   auto div = [&](int m, T v) -> T { return EV ? IP[m] * v : v; };
   {  // mass
-    T rho[PPT];
-    ld<DIR>(qs, B, PS, rho);
+    if constexpr (!KEEPR) ld<DIR>(qs, B, PS, rho);
 #pragma unroll
     for (int m = 0; m < PPT; ++m) f[m] = EV ? (PH[m] * rho[m]) * uh[m] : rho[m] * uh[m];
     line_solve<T, PER>(op, lds, tp, f, x);
@@ -157,7 +167,12 @@ __global__ __launch_bounds__(NT) void k_curv_sweep1(const Q *__restrict__ qs, T 
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     T mom[PPT];
-    ld<DIR>(qs + (1 + i) * N, B, PS, mom);
+    if constexpr (KEEPM) {
+#pragma unroll
+      for (int m = 0; m < PPT; ++m) mom[m] = mk[i][m];
+    } else {
+      ld<DIR>(qs + (1 + i) * N, B, PS, mom);
+    }
 #pragma unroll
     for (int m = 0; m < PPT; ++m) f[m] = EV ? (PH[m] * mom[m]) * uh[m] : mom[m] * uh[m];
     line_solve<T, PER>(op, lds, tp, f, x);
